@@ -1,0 +1,334 @@
+"""CPU oracle for the render hot path -- test infrastructure only.
+
+The C++ sources in this directory restate the reference algorithms (file:line
+citations in each header). This module loads ``oracle/liborc.so`` with ctypes
+and wraps it for numpy. Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import it; the product package
+``mutracer_amd`` never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _declare(L):
+    u32, f32, vp = C.c_uint32, C.c_float, C.c_void_p
+    for name in ("orc_sample_hash",):
+        getattr(L, name).restype = u32
+        getattr(L, name).argtypes = [u32]
+    for name in ("orc_sample_seed", "orc_owen", "orc_morton2"):
+        getattr(L, name).restype = u32
+        getattr(L, name).argtypes = [u32, u32]
+    L.orc_zsobol_index.restype = u32
+    L.orc_zsobol_index.argtypes = [u32, u32, u32]
+    L.orc_sobol_d1.restype = u32
+    L.orc_sobol_d1.argtypes = [u32]
+    L.orc_sobol2d_batch.argtypes = [vp, u32, u32, vp]
+    L.orc_structured_sample.argtypes = [u32] * 8 + [vp]
+    L.orc_bvh_build.restype = vp
+    L.orc_bvh_build.argtypes = [vp, u32, u32]
+    L.orc_bvh_info.argtypes = [vp, vp]
+    L.orc_bvh_get.argtypes = [vp] * 7
+    L.orc_bvh_free.argtypes = [vp]
+    L.orc_blas_build.restype = vp
+    L.orc_blas_build.argtypes = [vp, u32, vp, u32, u32]
+    L.orc_tlas_build.restype = vp
+    L.orc_tlas_build.argtypes = [vp, vp, vp, vp, vp, u32, vp, vp, u32, u32]
+    L.orc_tlas_refit.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32]
+    L.orc_flat_info.argtypes = [vp, vp]
+    L.orc_flat_get.argtypes = [vp, vp, vp, vp]
+    L.orc_flat_free.argtypes = [vp]
+    L.orc_instance_world_bounds.argtypes = [vp, vp, vp, vp]
+    L.orc_scene_create.restype = vp
+    L.orc_scene_create.argtypes = [vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64, vp, u32, u32]
+    L.orc_scene_free.argtypes = [vp]
+    L.orc_trace_closest_batch.argtypes = [vp, vp, u32, u32, vp, vp]
+    L.orc_trace_any_batch.argtypes = [vp, vp, u32, vp]
+    L.orc_trace_brute_batch.argtypes = [vp, vp, u32, u32, vp]
+    if hasattr(L, "orc_render"):
+        _declare_render(L)
+
+
+def _declare_render(L):
+    vp, u32 = C.c_void_p, C.c_uint32
+    L.orc_render.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp]
+    L.orc_render.restype = C.c_int
+
+
+# ---------------------------------------------------------------- sampler
+def sample_hash(x):
+    return lib().orc_sample_hash(x & 0xFFFFFFFF)
+
+
+def sample_seed(a, b):
+    return lib().orc_sample_seed(a & 0xFFFFFFFF, b & 0xFFFFFFFF)
+
+
+def owen(x, s):
+    return lib().orc_owen(x & 0xFFFFFFFF, s & 0xFFFFFFFF)
+
+
+def morton2(x, y):
+    return lib().orc_morton2(x, y)
+
+
+def zsobol_index(m, dim, log_spp):
+    return lib().orc_zsobol_index(m & 0xFFFFFFFF, dim & 0xFFFFFFFF, log_spp)
+
+
+def sobol_d1(i):
+    return lib().orc_sobol_d1(i & 0xFFFFFFFF)
+
+
+def sobol2d_bits(indices, seed):
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    out = np.zeros((idx.size, 2), dtype=np.uint32)
+    lib().orc_sobol2d_batch(_p(idx), idx.size, seed & 0xFFFFFFFF, _p(out))
+    return out
+
+
+def sobol2d(indices, seed):
+    return sobol2d_bits(indices, seed).astype(np.float64) / 16777216.0
+
+
+def structured_sample(pattern, px, py, width, index, dim, log_spp, seed):
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_structured_sample(pattern, px, py, width, index & 0xFFFFFFFF, dim & 0xFFFFFFFF,
+                                log_spp, seed & 0xFFFFFFFF, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- BVH
+def build_binned_sah(aabbs, bins=8):
+    a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    L = lib()
+    h = L.orc_bvh_build(_p(a), a.shape[0], bins)
+    info = np.zeros(4, dtype=np.uint32)
+    L.orc_bvh_info(h, _p(info))
+    n = int(info[0])
+    bounds = np.zeros((n, 6), np.float32)
+    lf = np.zeros(n, np.uint32)
+    pc = np.zeros(n, np.uint32)
+    pi = np.zeros(a.shape[0], np.uint32)
+    order = np.zeros(n, np.uint32)
+    offsets = np.zeros(n, np.uint32)
+    L.orc_bvh_get(h, _p(bounds), _p(lf), _p(pc), _p(pi), _p(order), _p(offsets))
+    L.orc_bvh_free(h)
+    return dict(node_count=n, bounds=bounds, left_first=lf, prim_count=pc, prim_indices=pi,
+                order=order, offsets=offsets, vec4_count=int(info[1]), max_depth=int(info[2]),
+                max_leaf=int(info[3]))
+
+
+def _flat(h, with_source=False):
+    L = lib()
+    info = np.zeros(5, dtype=np.uint64)
+    L.orc_flat_info(h, _p(info))
+    words = np.zeros(int(info[0]), np.uint32)
+    inst = np.zeros(int(info[1]), np.uint32)
+    source = np.zeros(int(info[0]) // 4, np.uint32) if with_source else None
+    L.orc_flat_get(h, _p(words), _p(inst), _p(source))
+    area = info[4:5].view(np.float64)[0]
+    return dict(words=words, instance_words=inst, max_depth=int(info[2]), max_leaf=int(info[3]),
+                build_surface_area=float(area), source_of_record=source)
+
+
+def build_blas(positions, indices, bins=8):
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    L = lib()
+    h = L.orc_blas_build(_p(p), p.shape[0], _p(i), i.size // 3, bins)
+    out = _flat(h, with_source=True)
+    L.orc_flat_free(h)
+    return out
+
+
+def _tlas_args(inst):
+    bi = np.ascontiguousarray(inst["blas_index"], np.uint32)
+    bounds = np.ascontiguousarray(inst["bounds"], np.float32).reshape(-1, 6)
+    tr = np.ascontiguousarray(inst["transform"], np.float64).reshape(-1, 16)
+    mat = np.ascontiguousarray(inst["material_word"], np.uint32)
+    col = np.ascontiguousarray(inst.get("color", np.ones((bi.size, 3))), np.float32).reshape(-1, 3)
+    return bi, bounds, tr, mat, col
+
+
+def build_tlas(inst, blas_offsets, vertex_offsets, bins=64, per_instance_bounds=True):
+    bi, bounds, tr, mat, col = _tlas_args(inst)
+    bo = np.ascontiguousarray(blas_offsets, np.uint32)
+    vo = np.ascontiguousarray(vertex_offsets, np.uint32)
+    L = lib()
+    h = L.orc_tlas_build(_p(bi), _p(bounds), _p(tr), _p(mat), _p(col), bi.size, _p(bo), _p(vo),
+                         bins, int(per_instance_bounds))
+    return _TlasHandle(h, bo, vo)
+
+
+class _TlasHandle:
+    def __init__(self, h, bo, vo):
+        self.h, self.bo, self.vo = h, bo, vo
+
+    def get(self):
+        return _flat(self.h)
+
+    def refit(self, inst, dirty=None):
+        bi, bounds, tr, mat, col = _tlas_args(inst)
+        d = None if dirty is None else np.ascontiguousarray(dirty, np.uint32)
+        lib().orc_tlas_refit(self.h, _p(bi), _p(bounds), _p(tr), _p(mat), _p(col), bi.size,
+                             _p(self.bo), _p(self.vo), _p(d), 0 if d is None else d.size)
+
+    def __del__(self):
+        try:
+            lib().orc_flat_free(self.h)
+        except Exception:
+            pass
+
+
+def instance_world_bounds(transform, lo, hi):
+    m = np.ascontiguousarray(transform, np.float64).reshape(16)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    out = np.zeros(6, np.float32)
+    lib().orc_instance_world_bounds(_p(m), _p(lo), _p(hi), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- traversal
+class TraceScene:
+    """Reference-layout scene (TLAS + concatenated BLAS + instance records)."""
+
+    def __init__(self, tlas_words, blas_words, instance_words, positions, mat_double_sided,
+                 per_instance_bounds=True):
+        self.keep = [np.ascontiguousarray(tlas_words, np.uint32),
+                     np.ascontiguousarray(blas_words, np.uint32),
+                     np.ascontiguousarray(instance_words, np.uint32),
+                     np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+                     np.ascontiguousarray(mat_double_sided, np.uint8)]
+        t, b, i, p, m = self.keep
+        self.h = lib().orc_scene_create(_p(t), t.size, _p(b), b.size, _p(i), i.size, _p(p),
+                                        p.shape[0], _p(m), m.size, int(per_instance_bounds))
+
+    def closest(self, rays, double_sided=False, stats=False):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((r.shape[0], 8), np.uint32)
+        st = np.zeros((r.shape[0], 3), np.uint32) if stats else None
+        lib().orc_trace_closest_batch(self.h, _p(r), r.shape[0], int(double_sided), _p(hits), _p(st))
+        return (hits, st) if stats else hits
+
+    def any(self, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        occ = np.zeros(r.shape[0], np.uint32)
+        lib().orc_trace_any_batch(self.h, _p(r), r.shape[0], _p(occ))
+        return occ
+
+    def brute(self, rays, double_sided=False):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((r.shape[0], 8), np.uint32)
+        lib().orc_trace_brute_batch(self.h, _p(r), r.shape[0], int(double_sided), _p(hits))
+        return hits
+
+    def __del__(self):
+        try:
+            lib().orc_scene_free(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- scene assembly
+def assemble_scene(meshes, instances, blas_bins=8, tlas_bins=64, per_instance_bounds=True,
+                   material_double_sided=None):
+    """Reference-layout scene from meshes and instances (oracle side).
+
+    meshes: list of dict(positions (V,3) f32, indices (T*3,) u32).
+    instances: dict(mesh (N,), transform (N,16) f64 column-major, material (N,)).
+    Follows WGPURayTracePass._uploadBVH (WGPURayTracePass.ts:2627-2694): BLAS
+    buffers concatenated (offsets in vec4 units, bvhAccel.ts:288-304), one
+    global vertex array, TLAS built over the instances' world boxes.
+    """
+    blas_words, blas_offsets, vertex_offsets, positions, roots = [], [], [], [], []
+    source_tri = []
+    off = voff = 0
+    max_depth = 1
+    for m in meshes:
+        flat = build_blas(m["positions"], m["indices"], blas_bins)
+        w = flat["words"]
+        blas_offsets.append(off)
+        vertex_offsets.append(voff)
+        blas_words.append(w)
+        src = flat["source_of_record"]
+        source_tri.append(src)
+        lo = w[0:3].view(np.float32) if w.size else np.zeros(3, np.float32)
+        hi = w[4:7].view(np.float32) if w.size else np.zeros(3, np.float32)
+        roots.append((lo.copy(), hi.copy()))
+        max_depth = max(max_depth, flat["max_depth"])
+        off += w.size // 4
+        voff += np.asarray(m["positions"]).reshape(-1, 3).shape[0]
+        positions.append(np.asarray(m["positions"], np.float32).reshape(-1, 3))
+    mesh_idx = np.asarray(instances["mesh"], np.int64)
+    tr = np.asarray(instances["transform"], np.float64).reshape(-1, 16)
+    bounds = np.array([instance_world_bounds(tr[i], *roots[mesh_idx[i]]) for i in range(mesh_idx.size)],
+                      np.float32).reshape(-1, 6)
+    mats = np.asarray(instances.get("material", np.zeros(mesh_idx.size)), np.int64)
+    inst = dict(blas_index=mesh_idx, bounds=bounds, transform=tr, material_word=(mats << 8).astype(np.uint32),
+                color=instances.get("color", np.ones((mesh_idx.size, 3))))
+    h = build_tlas(inst, blas_offsets, vertex_offsets, tlas_bins, per_instance_bounds)
+    t = h.get()
+    all_blas = np.concatenate(blas_words) if blas_words else np.zeros(0, np.uint32)
+    all_pos = np.concatenate(positions) if positions else np.zeros((0, 3), np.float32)
+    nmat = int(mats.max()) + 1 if mats.size else 1
+    mds = np.zeros(nmat, np.uint8) if material_double_sided is None else np.asarray(material_double_sided, np.uint8)
+    scene = TraceScene(t["words"], all_blas, t["instance_words"], all_pos, mds, per_instance_bounds)
+    # identities for parity: record -> (mesh, source triangle); target -> source instance
+    rec_tri = np.full(all_blas.size // 4, 0xFFFFFFFF, np.uint32)
+    for k, src in enumerate(source_tri):
+        rec_tri[blas_offsets[k]:blas_offsets[k] + src.size] = src
+    ints = t["words"].view(np.int32)
+    target_src = np.zeros(mesh_idx.size, np.int64)
+    stack = [0] if t["words"].size else []
+    while stack:
+        node = stack.pop()
+        c = ints[node * 4 + 3]
+        if c >= 0:
+            stack += [c, node + 2]
+            continue
+        for k in range(-c):
+            target_src[ints[node * 4 + 8 + k * 4]] = ints[node * 4 + 8 + k * 4 + 2]
+    scene.record_triangle = rec_tri
+    scene.target_source = target_src
+    scene.tlas_handle = h
+    scene.tlas_max_depth = t["max_depth"]
+    scene.blas_max_depth = max_depth
+    scene.instance_bounds = bounds
+    return scene
+
+
+def hits_to_ids(scene, hits):
+    """(hit, source instance, source triangle, t bits) per ray from oracle hit words."""
+    h = np.asarray(hits, np.uint32).reshape(-1, 8)
+    hit = h[:, 0].astype(bool)
+    inst = np.where(hit, scene.target_source[np.where(hit, h[:, 1], 0)], -1)
+    tri = np.where(hit, scene.record_triangle[np.where(hit, h[:, 2], 0)].astype(np.int64), -1)
+    return hit, inst, tri, h[:, 3]

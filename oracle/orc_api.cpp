@@ -1,0 +1,180 @@
+// Oracle C entry points for the Python test harness (ctypes). Test
+// infrastructure only: nothing in the product links this library.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include "orc_bvh.h"
+#include "orc_sobol.h"
+#include "orc_trace.h"
+
+using namespace orc;
+
+extern "C" {
+
+// ---------------------------------------------------------------- sampler
+uint32_t orc_sample_hash(uint32_t x) { return sample_hash(x); }
+uint32_t orc_sample_seed(uint32_t a, uint32_t b) { return sample_seed(a, b); }
+uint32_t orc_owen(uint32_t x, uint32_t s) { return owen(x, s); }
+uint32_t orc_morton2(uint32_t x, uint32_t y) { return morton2(x, y); }
+uint32_t orc_zsobol_index(uint32_t m, uint32_t dim, uint32_t logSPP) { return zsobol_index(m, dim, logSPP); }
+uint32_t orc_sobol_d1(uint32_t i) { return sobol_d1(i); }
+
+void orc_sobol2d_batch(const uint32_t *index, uint32_t n, uint32_t seed, uint32_t *out) {
+  for (uint32_t i = 0; i < n; i++) sobol2d_bits(index[i], seed, out + 2 * i);
+}
+
+// Structured sample for (pixel, sample index, dimension) with the EPS remap,
+// as sobolReference.ts:52-68 referenceSample (hashed = structured dims off).
+void orc_structured_sample(uint32_t pattern, uint32_t px, uint32_t py, uint32_t width,
+                           uint32_t index, uint32_t dim, uint32_t logSPP, uint32_t seed,
+                           float *out) {
+  Sampler s{};
+  s.px = px; s.py = py; s.width = width; s.pixelSampleIndex = index;
+  s.sequenceSeed = seed; s.zSobolLogSPP = logSPP; s.pattern = pattern;
+  s.sample2d(dim, out);
+}
+
+// ---------------------------------------------------------------- BVH
+struct OrcBvh { BinnedSahBvh bvh; Bvh2Layout layout; };
+
+void *orc_bvh_build(const float *aabbs, uint32_t count, uint32_t bins) {
+  OrcBvh *h = new OrcBvh;
+  h->bvh = build_binned_sah(aabbs, count, bins);
+  h->layout = layout_bvh2(h->bvh);
+  return h;
+}
+void orc_bvh_info(void *hp, uint32_t *info) {
+  OrcBvh *h = (OrcBvh *)hp;
+  info[0] = h->bvh.nodeCount; info[1] = h->layout.vec4Count;
+  info[2] = h->layout.maxDepth; info[3] = h->layout.maxLeafPrimitives;
+}
+void orc_bvh_get(void *hp, float *bounds, uint32_t *leftFirst, uint32_t *primCount,
+                 uint32_t *primIndices, uint32_t *order, uint32_t *offsets) {
+  OrcBvh *h = (OrcBvh *)hp;
+  uint32_t n = h->bvh.nodeCount;
+  if (bounds) std::memcpy(bounds, h->bvh.bounds.data(), (size_t)n * 24);
+  if (leftFirst) std::memcpy(leftFirst, h->bvh.leftFirst.data(), (size_t)n * 4);
+  if (primCount) std::memcpy(primCount, h->bvh.primCount.data(), (size_t)n * 4);
+  if (primIndices) std::memcpy(primIndices, h->bvh.primIndices.data(), h->bvh.primIndices.size() * 4);
+  if (order) std::memcpy(order, h->layout.order.data(), (size_t)n * 4);
+  if (offsets) std::memcpy(offsets, h->layout.offsets.data(), (size_t)n * 4);
+}
+void orc_bvh_free(void *hp) { delete (OrcBvh *)hp; }
+
+struct OrcFlat { FlatBvh f; std::vector<uint32_t> source; };
+
+void *orc_blas_build(const float *positions, uint32_t vertexCount, const uint32_t *indices,
+                     uint32_t triCount, uint32_t bins) {
+  OrcFlat *h = new OrcFlat;
+  h->f = build_blas(positions, vertexCount, indices, triCount, bins, &h->source);
+  return h;
+}
+
+void *orc_tlas_build(const uint32_t *blasIndex, const float *bounds, const double *transforms,
+                     const uint32_t *materials, const float *colors, uint32_t count,
+                     const uint32_t *blasOffsets, const uint32_t *vertexOffsets, uint32_t bins,
+                     uint32_t perInstanceBounds) {
+  std::vector<TlasInstance> inst(count);
+  for (uint32_t i = 0; i < count; i++) {
+    inst[i].blasIndex = blasIndex[i];
+    std::memcpy(inst[i].bounds, bounds + (size_t)i * 6, 24);
+    std::memcpy(inst[i].transform, transforms + (size_t)i * 16, 128);
+    inst[i].material = materials[i];
+    for (int k = 0; k < 3; k++) inst[i].color[k] = colors ? colors[i * 3 + k] : 1.0f;
+  }
+  OrcFlat *h = new OrcFlat;
+  h->f = build_tlas(inst.data(), count, blasOffsets, vertexOffsets, bins, perInstanceBounds != 0);
+  return h;
+}
+
+void orc_tlas_refit(void *hp, const uint32_t *blasIndex, const float *bounds,
+                    const double *transforms, const uint32_t *materials, const float *colors,
+                    uint32_t count, const uint32_t *blasOffsets, const uint32_t *vertexOffsets,
+                    const uint32_t *dirty, uint32_t dirtyCount) {
+  std::vector<TlasInstance> inst(count);
+  for (uint32_t i = 0; i < count; i++) {
+    inst[i].blasIndex = blasIndex[i];
+    std::memcpy(inst[i].bounds, bounds + (size_t)i * 6, 24);
+    std::memcpy(inst[i].transform, transforms + (size_t)i * 16, 128);
+    inst[i].material = materials[i];
+    for (int k = 0; k < 3; k++) inst[i].color[k] = colors ? colors[i * 3 + k] : 1.0f;
+  }
+  OrcFlat *h = (OrcFlat *)hp;
+  refit_tlas(h->f, inst.data(), count, blasOffsets, vertexOffsets, dirty, dirtyCount);
+}
+
+void orc_flat_info(void *hp, uint64_t *info) {
+  OrcFlat *h = (OrcFlat *)hp;
+  info[0] = h->f.words.size(); info[1] = h->f.instanceWords.size();
+  info[2] = h->f.maxDepth; info[3] = h->f.maxLeafPrimitives;
+  double a = h->f.buildSurfaceArea; std::memcpy(&info[4], &a, 8);
+}
+void orc_flat_get(void *hp, uint32_t *words, uint32_t *instanceWords, uint32_t *source) {
+  OrcFlat *h = (OrcFlat *)hp;
+  if (words) std::memcpy(words, h->f.words.data(), h->f.words.size() * 4);
+  if (instanceWords) std::memcpy(instanceWords, h->f.instanceWords.data(), h->f.instanceWords.size() * 4);
+  if (source && !h->source.empty()) std::memcpy(source, h->source.data(), h->source.size() * 4);
+}
+void orc_flat_free(void *hp) { delete (OrcFlat *)hp; }
+
+void orc_instance_world_bounds(const double *m, const float *lo, const float *hi, float *out) {
+  instance_world_bounds(m, lo, hi, out);
+}
+
+// ---------------------------------------------------------------- traversal
+void *orc_scene_create(const uint32_t *tlas, uint64_t tlasWords, const uint32_t *blas,
+                       uint64_t blasWords, const uint32_t *instances, uint64_t instanceWords,
+                       const float *positions, uint64_t vertexCount, const uint8_t *matDouble,
+                       uint32_t materialCount, uint32_t perInstanceBounds) {
+  TraceScene *s = new TraceScene;
+  s->tlas.assign(tlas, tlas + tlasWords);
+  s->blas.assign(blas, blas + blasWords);
+  s->instances.assign(instances, instances + instanceWords);
+  s->positions.assign(positions, positions + vertexCount * 3);
+  if (matDouble) s->materialDoubleSided.assign(matDouble, matDouble + materialCount);
+  s->perInstanceBounds = perInstanceBounds != 0;
+  return s;
+}
+void orc_scene_free(void *s) { delete (TraceScene *)s; }
+
+// rays: n x 8 floats (o.xyz, tmax, d.xyz, unused). hits: n x 8 words
+// (hit, instance target, record, t bits, bary xyz bits, 0). stats: n x 3 (optional).
+void orc_trace_closest_batch(void *sp, const float *rays, uint32_t n, uint32_t doubleSided,
+                             uint32_t *hits, uint32_t *stats) {
+  TraceScene *s = (TraceScene *)sp;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const float *r = rays + i * 8;
+    TraceStats st;
+    ClosestHit h = trace_closest(*s, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3],
+                                 doubleSided != 0, 0, -1, -1, stats ? &st : nullptr);
+    uint32_t *o = hits + i * 8;
+    o[0] = h.hit; o[1] = (uint32_t)h.instance; o[2] = (uint32_t)h.record; o[3] = f2u(h.t);
+    o[4] = f2u(h.bary.x); o[5] = f2u(h.bary.y); o[6] = f2u(h.bary.z); o[7] = 0;
+    if (stats) { stats[i * 3] = (uint32_t)st.boxTests; stats[i * 3 + 1] = (uint32_t)st.triTests; stats[i * 3 + 2] = (uint32_t)st.instanceVisits; }
+  }
+}
+
+void orc_trace_any_batch(void *sp, const float *rays, uint32_t n, uint32_t *occluded) {
+  TraceScene *s = (TraceScene *)sp;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const float *r = rays + i * 8;
+    occluded[i] = trace_any(*s, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3], -1, -1, nullptr);
+  }
+}
+
+void orc_trace_brute_batch(void *sp, const float *rays, uint32_t n, uint32_t doubleSided,
+                           uint32_t *hits) {
+  TraceScene *s = (TraceScene *)sp;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const float *r = rays + i * 8;
+    ClosestHit h = trace_brute(*s, V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3], doubleSided != 0);
+    uint32_t *o = hits + i * 8;
+    o[0] = h.hit; o[1] = (uint32_t)h.instance; o[2] = (uint32_t)h.record; o[3] = f2u(h.t);
+    o[4] = f2u(h.bary.x); o[5] = f2u(h.bary.y); o[6] = f2u(h.bary.z); o[7] = 0;
+  }
+}
+
+}  // extern "C"
