@@ -1,0 +1,311 @@
+// C ABI of libmtx: handle registry, BLAS / scene acceleration objects and the
+// ray-query kernels.
+#include <atomic>
+#include <map>
+#include <mutex>
+
+#include "mtx_registry.h"
+
+namespace mtx {
+
+// ------------------------------------------------------------------ registry
+Registry &registry() {
+  static Registry r;
+  return r;
+}
+
+uint64_t Registry::next() { return nextHandle.fetch_add(1); }
+
+// ------------------------------------------------------------- trace kernels
+constexpr int kTraceBlock = 128;
+
+__device__ __forceinline__ bool xcd_next(uint32_t &rb, uint32_t rayBlocks, uint32_t &end, bool first) {
+  // Blocks of one XCD (blockIdx % 8) walk one contiguous eighth of the ray
+  // blocks, so spatially coherent rays share that XCD's L2.
+  const uint32_t xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3, perXcd = (gridDim.x + 7u) >> 3;
+  const uint32_t span = (rayBlocks + 7u) >> 3;
+  if (first) {
+    rb = xcd * span + rank;
+    end = min(rayBlocks, (xcd + 1u) * span);
+  } else {
+    rb += perXcd;
+  }
+  return rb < end;
+}
+
+template <int kMode>  // 0 closest, 1 any, 2 counters
+__global__ void __launch_bounds__(kTraceBlock)
+k_trace(const AccelView *__restrict__ viewp, const mtx_ray *__restrict__ rays, uint32_t n,
+        mtx_hit *__restrict__ hits, uint32_t *__restrict__ out, uint32_t flags, int32_t *__restrict__ spill) {
+  __shared__ int32_t lds[kLdsStack * kTraceBlock];
+  const AccelView acc = *viewp;
+  LaneStack st{lds + threadIdx.x, kTraceBlock,
+               spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
+  const uint32_t rayBlocks = (n + kTraceBlock - 1) / kTraceBlock;
+  const bool doubleSided = (flags & MTX_TRACE_DOUBLE_SIDED) != 0;
+  uint32_t rb, end;
+  for (bool ok = xcd_next(rb, rayBlocks, end, true); ok; ok = xcd_next(rb, rayBlocks, end, false)) {
+    uint32_t i = rb * kTraceBlock + threadIdx.x;
+    if (i >= n) continue;
+    const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
+    float4 a = r4[0], b = r4[1];
+    f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+    TraceHit h;
+    TraceCounters c{0, 0, 0};
+    if (kMode == 1) {
+      float tMax = fminf(fmaxf(a.w, 0.0f), kRayMaxDistance);
+      out[i] = traverse<true, false>(acc, o, d, tMax, true, 1, -1, -1, st, h, nullptr) ? 1u : 0u;
+      continue;
+    }
+    const bool counting = kMode == 2 || (flags & MTX_TRACE_COUNT);
+    if (counting) traverse<false, true>(acc, o, d, kRayMaxDistance, doubleSided, 0, -1, -1, st, h, &c);
+    else traverse<false, false>(acc, o, d, kRayMaxDistance, doubleSided, 0, -1, -1, st, h, nullptr);
+    if (kMode == 2) {
+      out[(size_t)i * 3] = c.boxes;
+      out[(size_t)i * 3 + 1] = c.tris;
+      out[(size_t)i * 3 + 2] = c.instances;
+      continue;
+    }
+    mtx_hit o4;
+    o4.hit = h.hit ? 1u : 0u;
+    if (h.hit) {
+      const InstSlot &s = acc.slots[h.slot];
+      o4.instance = fint(s.c2.w);
+      o4.primitive = fint(s.tris[h.tri].v0.w);
+      o4.t = h.t;
+      o4.barycentric[0] = h.bary.x;
+      o4.barycentric[1] = h.bary.y;
+      o4.barycentric[2] = h.bary.z;
+    } else {
+      o4.instance = -1;
+      o4.primitive = -1;
+      o4.t = kRayMaxDistance;
+      o4.barycentric[0] = o4.barycentric[1] = o4.barycentric[2] = 0.0f;
+    }
+    o4.visits = counting ? c.boxes + c.tris : 0u;
+    hits[i] = o4;
+  }
+}
+
+// Spill slices for the traversal stacks, one set per stream.
+struct SpillPool {
+  std::mutex m;
+  std::map<std::pair<int, hipStream_t>, std::shared_ptr<DevBuf<int32_t>>> bufs;
+};
+static SpillPool gSpill;
+
+uint32_t trace_grid() {
+  static uint32_t grid = 0;
+  if (!grid) {
+    int dev = 0, cus = 0;
+    MTX_HIP(hipGetDevice(&dev));
+    MTX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = (uint32_t)std::max(8, cus) * 8u;
+    grid = (grid + 7u) & ~7u;
+  }
+  return grid;
+}
+
+int32_t *spill_for(hipStream_t s, uint32_t blocksNeeded, uint32_t blockThreads) {
+  int dev = 0;
+  MTX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(gSpill.m);
+  auto &b = gSpill.bufs[{dev, s}];
+  size_t need = (size_t)blocksNeeded * blockThreads * (kMaxStack - kLdsStack);
+  if (!b || b->count < need) b = std::make_shared<DevBuf<int32_t>>(need);
+  return b->ptr;
+}
+
+static void launch_trace(int mode, const Accel &A, const mtx_ray *rays, uint32_t n, mtx_hit *hits, uint32_t *out,
+                         uint32_t flags, hipStream_t s) {
+  MTX_REQUIRE(A.max_stack() <= (uint32_t)kMaxStack, MTX_INVALID_STATE, "scene deeper than the traversal stack");
+  if (!n) return;
+  uint32_t grid = trace_grid();
+  uint32_t rayBlocks = (n + kTraceBlock - 1) / kTraceBlock;
+  grid = std::min<uint32_t>(grid, ((rayBlocks + 7u) / 8u) * 8u);
+  int32_t *spill = spill_for(s, grid, kTraceBlock);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+  else
+    hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+  MTX_HIP(hipGetLastError());
+}
+
+}  // namespace mtx
+
+using namespace mtx;
+
+#define MTX_API_BEGIN try {
+#define MTX_API_END(err, cap)                                     \
+  }                                                               \
+  catch (const mtx::Error &e) {                                   \
+    set_error(err, cap, e.what());                                \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::bad_alloc &) {                                \
+    set_error(err, cap, "out of host memory");                    \
+    return MTX_ERROR;                                             \
+  }                                                               \
+  catch (const std::exception &e) {                               \
+    set_error(err, cap, e.what());                                \
+    return MTX_ERROR;                                             \
+  }
+
+extern "C" {
+
+uint32_t mtx_abi_version(void) { return MTX_ABI_VERSION; }
+
+int32_t mtx_device_info(char *name, size_t cap, int32_t *cus) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MTX_UNAVAILABLE;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return MTX_UNAVAILABLE;
+  set_error(name, cap, std::string(p.name) + " " + p.gcnArchName);
+  if (cus) *cus = p.multiProcessorCount;
+  return MTX_OK;
+}
+
+int32_t mtx_blas_build(const float *positions, uint32_t vertex_count, const uint32_t *indices,
+                       uint32_t triangle_count, uint32_t bins, uint32_t flags, mtx_handle *out, char *err,
+                       size_t cap) {
+  MTX_API_BEGIN
+  MTX_REQUIRE(out, MTX_INVALID_ARGUMENT, "out_blas is NULL");
+  MTX_REQUIRE(triangle_count == 0 || (positions && indices), MTX_INVALID_ARGUMENT, "positions/indices are NULL");
+  MTX_REQUIRE(bins >= 2 && bins <= 64, MTX_INVALID_ARGUMENT, "bins must be in [2, 64]");
+  auto B = build_blas(positions, vertex_count, indices, triangle_count, bins, (flags & MTX_INPUT_DEVICE) != 0, 0);
+  auto &R = registry();
+  std::lock_guard<std::mutex> g(R.m);
+  uint64_t h = R.next();
+  R.blas[h] = B;
+  *out = h;
+  return MTX_OK;
+  MTX_API_END(err, cap)
+}
+
+int32_t mtx_blas_info(mtx_handle blas, uint32_t info[8]) {
+  MTX_API_BEGIN
+  auto B = registry().get_blas(blas);
+  MTX_REQUIRE(info, MTX_INVALID_ARGUMENT, "info is NULL");
+  info[0] = B->topo.nodeCount;
+  info[1] = B->topo.nodeCount * 2 + B->topo.primCount;
+  info[2] = B->topo.maxDepth;
+  info[3] = B->maxLeaf;
+  info[4] = B->topo.primCount;
+  uint64_t bytes = B->resident_bytes();
+  info[5] = (uint32_t)bytes;
+  info[6] = (uint32_t)(bytes >> 32);
+  info[7] = B->layout.interiorCount;
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_blas_export(mtx_handle blas, const uint32_t *indices, uint32_t *words, size_t capacity,
+                        size_t *written, uint32_t *source_of_record) {
+  MTX_API_BEGIN
+  auto B = registry().get_blas(blas);
+  size_t need = export_blas(*B, indices, nullptr, 0, nullptr, 0);
+  if (written) *written = need;
+  if (!words) return MTX_OK;
+  if (capacity < need) return MTX_BUFFER_TOO_SMALL;
+  export_blas(*B, indices, words, capacity, source_of_record, 0);
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_blas_destroy(mtx_handle blas) {
+  auto &R = registry();
+  std::lock_guard<std::mutex> g(R.m);
+  return R.blas.erase(blas) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+int32_t mtx_accel_create(const mtx_handle *blases, uint32_t blas_count, const mtx_instance_desc *instances,
+                         uint32_t instance_count, uint32_t tlas_bins, uint32_t flags, mtx_handle *out, char *err,
+                         size_t cap) {
+  MTX_API_BEGIN
+  MTX_REQUIRE(out, MTX_INVALID_ARGUMENT, "out_accel is NULL");
+  MTX_REQUIRE(instance_count == 0 || instances, MTX_INVALID_ARGUMENT, "instances is NULL");
+  MTX_REQUIRE(tlas_bins >= 2 && tlas_bins <= 64, MTX_INVALID_ARGUMENT, "tlas_bins must be in [2, 64]");
+  for (uint32_t i = 0; i < instance_count; i++)
+    MTX_REQUIRE(instances[i].struct_size >= sizeof(mtx_instance_desc), MTX_INVALID_ARGUMENT,
+                "mtx_instance_desc.struct_size too small");
+  std::vector<std::shared_ptr<Blas>> bl;
+  for (uint32_t i = 0; i < blas_count; i++) bl.push_back(registry().get_blas(blases[i]));
+  auto A = build_accel(bl, instances, instance_count, tlas_bins, flags, 0);
+  auto &R = registry();
+  std::lock_guard<std::mutex> g(R.m);
+  uint64_t h = R.next();
+  R.accel[h] = A;
+  *out = h;
+  return MTX_OK;
+  MTX_API_END(err, cap)
+}
+
+int32_t mtx_accel_refit(mtx_handle accel, const mtx_instance_desc *instances, uint32_t count, const uint32_t *dirty,
+                        uint32_t dirty_count, char *err, size_t cap) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  MTX_REQUIRE(count == 0 || instances, MTX_INVALID_ARGUMENT, "instances is NULL");
+  refit_accel(*A, instances, count, dirty, dirty_count, 0);
+  return MTX_OK;
+  MTX_API_END(err, cap)
+}
+
+int32_t mtx_accel_export_tlas(mtx_handle accel, uint32_t *tlas_words, size_t tlas_capacity, size_t *tlas_written,
+                              uint32_t *instance_words, size_t instance_capacity, size_t *instance_written) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  std::vector<uint32_t> tw, iw;
+  export_tlas(*A, tw, iw, 0);
+  if (tlas_written) *tlas_written = tw.size();
+  if (instance_written) *instance_written = iw.size();
+  if (tlas_words) {
+    if (tlas_capacity < tw.size()) return MTX_BUFFER_TOO_SMALL;
+    std::memcpy(tlas_words, tw.data(), tw.size() * 4);
+  }
+  if (instance_words) {
+    if (instance_capacity < iw.size()) return MTX_BUFFER_TOO_SMALL;
+    std::memcpy(instance_words, iw.data(), iw.size() * 4);
+  }
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_accel_destroy(mtx_handle accel) {
+  auto &R = registry();
+  std::lock_guard<std::mutex> g(R.m);
+  return R.accel.erase(accel) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+int32_t mtx_trace_closest(mtx_handle accel, const mtx_ray *rays, uint32_t count, mtx_hit *hits, uint32_t flags,
+                          void *stream) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  MTX_REQUIRE(count == 0 || (rays && hits), MTX_INVALID_ARGUMENT, "rays/hits are NULL");
+  launch_trace(0, *A, rays, count, hits, nullptr, flags, (hipStream_t)stream);
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_trace_any(mtx_handle accel, const mtx_ray *rays, uint32_t count, uint32_t *occluded, void *stream) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  MTX_REQUIRE(count == 0 || (rays && occluded), MTX_INVALID_ARGUMENT, "rays/occluded are NULL");
+  launch_trace(1, *A, rays, count, nullptr, occluded, 0, (hipStream_t)stream);
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, uint32_t *counters, uint32_t flags,
+                        void *stream) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  MTX_REQUIRE(count == 0 || (rays && counters), MTX_INVALID_ARGUMENT, "rays/counters are NULL");
+  launch_trace(2, *A, rays, count, nullptr, counters, flags, (hipStream_t)stream);
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+}  // extern "C"
