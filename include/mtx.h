@@ -153,6 +153,138 @@ int32_t mtx_trace_any(mtx_handle accel, const mtx_ray *rays, uint32_t count, uin
 int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, uint32_t *counters,
                         uint32_t flags, void *stream);
 
+
+/* ------------------------------------------------------------- rendering */
+/* Packed OpenPBR material: 32 floats at these offsets (reference parameter
+ * names and defaults: src/scene/openPbrSchema.ts:32-69). */
+#define MTX_MAT_BASE_WEIGHT 0
+#define MTX_MAT_BASE_COLOR 1          /* 3 floats */
+#define MTX_MAT_BASE_DIFFUSE_ROUGHNESS 4
+#define MTX_MAT_BASE_METALNESS 5
+#define MTX_MAT_SPECULAR_WEIGHT 6
+#define MTX_MAT_SPECULAR_COLOR 7      /* 3 floats (F82 tint for metals) */
+#define MTX_MAT_SPECULAR_ROUGHNESS 10
+#define MTX_MAT_SPECULAR_IOR 11
+#define MTX_MAT_TRANSMISSION_WEIGHT 12
+#define MTX_MAT_TRANSMISSION_COLOR 13 /* 3 floats */
+#define MTX_MAT_COAT_WEIGHT 16
+#define MTX_MAT_COAT_COLOR 17         /* 3 floats */
+#define MTX_MAT_COAT_ROUGHNESS 20
+#define MTX_MAT_COAT_IOR 21
+#define MTX_MAT_COAT_DARKENING 22
+#define MTX_MAT_EMISSION 23           /* 3 floats */
+#define MTX_MAT_DOUBLE_SIDED 26
+#define MTX_MAT_SPECULAR_ANISOTROPY 27
+#define MTX_MATERIAL_FLOATS 32
+
+/* Packed rect light (RectLight.ts:5-22): 16 floats. */
+#define MTX_LIGHT_TYPE 0              /* 0 = rect */
+#define MTX_LIGHT_POSITION 1          /* center, 3 floats */
+#define MTX_LIGHT_U 4                 /* edge vector, 3 floats */
+#define MTX_LIGHT_V 7                 /* edge vector, 3 floats; normal = normalize(u x v) */
+#define MTX_LIGHT_EMISSION 10         /* radiance = color * intensity, 3 floats */
+#define MTX_LIGHT_SPREAD 13
+#define MTX_LIGHT_AREA 14
+#define MTX_LIGHT_FLOATS 16
+
+typedef struct mtx_mesh_desc {
+  uint32_t struct_size;
+  uint32_t vertex_count;
+  uint32_t triangle_count;
+  uint32_t reserved;
+  const float *positions;   /* vertex_count * 3 */
+  const float *normals;     /* vertex_count * 3 */
+  const uint32_t *indices;  /* triangle_count * 3 */
+} mtx_mesh_desc;
+
+typedef struct mtx_scene_desc {
+  uint32_t struct_size;
+  uint32_t mesh_count;
+  const mtx_mesh_desc *meshes;
+  uint32_t instance_count;  /* instances[i].blas_index is a mesh index */
+  const mtx_instance_desc *instances;
+  uint32_t material_count;
+  const float *materials;   /* MTX_MATERIAL_FLOATS each */
+  uint32_t light_count;
+  const float *lights;      /* MTX_LIGHT_FLOATS each */
+  uint32_t env_width, env_height;
+  const float *env_rgba;    /* env_height rows of env_width RGBA f32, or NULL */
+  uint32_t blas_bins;       /* 8 = reference default */
+  uint32_t tlas_bins;       /* 64 = web TLAS */
+  uint32_t flags;           /* MTX_ACCEL_PER_INSTANCE_BOUNDS */
+} mtx_scene_desc;
+
+/* Scene commit (the GPU side of Engine.update / WGPURayTracePass.setMeshes,
+ * WGPURayTracePass.ts:2246, setLights :3180): builds every BLAS on the GPU,
+ * the TLAS, the light alias table (lightSampling.ts:172-212) and the
+ * environment CDF (envMapDistribution.ts:57-134), and keeps them resident. */
+int32_t mtx_scene_create(const mtx_scene_desc *desc, mtx_handle *out_scene, char *error, size_t error_capacity);
+
+/* Transform-only edit of the listed source instances (refitTLAS). */
+int32_t mtx_scene_set_transforms(mtx_handle scene, const double *transforms, uint32_t instance_count,
+                                 const uint32_t *dirty, uint32_t dirty_count, char *error, size_t error_capacity);
+
+/* info[0] triangles, [1] instances, [2] BLAS count, [3] resident MiB. */
+int32_t mtx_scene_info(mtx_handle scene, uint64_t info[4]);
+int32_t mtx_scene_destroy(mtx_handle scene);
+
+typedef struct mtx_render_settings {
+  uint32_t struct_size;
+  uint32_t width, height;
+  uint32_t bounces;                 /* RayTraceUniform.bounces */
+  uint32_t direct_light_candidates; /* NEE RIS candidates (render.ts:21) */
+  uint32_t sampling_pattern;        /* 0 sobol, 1 zsobol (sampling.ts:3-8) */
+  uint32_t sequence_seed;
+  uint32_t max_samples;             /* ZSobol sample budget */
+  uint32_t sample_sequence_offset;  /* Render.ts:151 */
+  float clamping;                   /* 0 = off */
+  uint32_t reserved[2];
+} mtx_render_settings;
+
+/* Camera block (cameraUniform.ts:2-17, browser projection-inverse form). */
+typedef struct mtx_camera_desc {
+  uint32_t struct_size;
+  float projection_inverse[16];
+  float view_inverse[16];
+  float aperture_radius;
+  float focus_distance;
+} mtx_camera_desc;
+
+/* Render target: resident FP64 per-pixel running sums (color RGBA, albedo,
+ * normal, luminance^2) and the per-pixel sample counter that gives each
+ * pixel's next Sobol index (rayTrace.compute.ts:1188). Replaces
+ * RenderTarget.trace (RenderingPipeline.ts:218) / WGPURayTracePass.createPass. */
+int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings, const mtx_camera_desc *camera,
+                          mtx_handle *out_target, char *error, size_t error_capacity);
+
+/* Adds `samples` path samples to every pixel on `stream` (NULL = default);
+ * asynchronous. One megakernel launch per call. */
+int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream);
+
+/* Blocks until the target's submitted work is done. */
+int32_t mtx_target_synchronize(mtx_handle target);
+
+#define MTX_PIXELS_COLOR 0   /* HDR color RGBA: sum / samples */
+#define MTX_PIXELS_ALBEDO 1  /* albedo RGB, coverage in A */
+#define MTX_PIXELS_NORMAL 2  /* normal RGB, A = 1 */
+#define MTX_PIXELS_VARIANCE 3 /* luminance variance of the mean in R */
+
+/* Resolved RGBA f32, top row first (Render.readPixels, Render.ts:782-801).
+ * pixels == NULL queries the float count through *written. */
+int32_t mtx_target_read_pixels(mtx_handle target, uint32_t kind, float *pixels, size_t capacity_floats,
+                               size_t *written);
+
+/* Raw FP64 sums, 13 per pixel in bottom-row-first order (color rgba, albedo
+ * rgba, normal rgba, luminance^2), plus samples per pixel. */
+int32_t mtx_target_read_accumulators(mtx_handle target, double *sums, size_t capacity_doubles, uint32_t *counts,
+                                     size_t capacity_counts);
+
+/* stats[0] rays traced, [1] samples per pixel completed. */
+int32_t mtx_target_stats(mtx_handle target, uint64_t stats[2]);
+int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera);
+int32_t mtx_target_reset(mtx_handle target);
+int32_t mtx_target_destroy(mtx_handle target);
+
 #ifdef __cplusplus
 }
 #endif

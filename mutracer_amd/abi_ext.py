@@ -1,11 +1,51 @@
-"""ctypes signatures of the render, job, serving and denoise entry points."""
+"""ctypes structures and signatures of the scene/render entry points of include/mtx.h."""
 import ctypes as C
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
+                ("reserved", C.c_uint32), ("positions", C.c_void_p), ("normals", C.c_void_p),
+                ("indices", C.c_void_p)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("mesh_count", C.c_uint32), ("meshes", C.c_void_p),
+                ("instance_count", C.c_uint32), ("instances", C.c_void_p), ("material_count", C.c_uint32),
+                ("materials", C.c_void_p), ("light_count", C.c_uint32), ("lights", C.c_void_p),
+                ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_rgba", C.c_void_p),
+                ("blas_bins", C.c_uint32), ("tlas_bins", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("bounces", C.c_uint32), ("direct_light_candidates", C.c_uint32),
+                ("sampling_pattern", C.c_uint32), ("sequence_seed", C.c_uint32), ("max_samples", C.c_uint32),
+                ("sample_sequence_offset", C.c_uint32), ("clamping", C.c_float), ("reserved", C.c_uint32 * 2)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("projection_inverse", C.c_float * 16),
+                ("view_inverse", C.c_float * 16), ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]
 
 
 def declare(L):
     u32, i32, u64, vp, sz = C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p, C.c_size_t
     P = C.POINTER
-    sig = {}
+    sig = {
+        "mtx_scene_create": (i32, [vp, P(u64), C.c_char_p, sz]),
+        "mtx_scene_set_transforms": (i32, [u64, vp, u32, vp, u32, C.c_char_p, sz]),
+        "mtx_scene_info": (i32, [u64, vp]),
+        "mtx_scene_destroy": (i32, [u64]),
+        "mtx_target_create": (i32, [u64, vp, vp, P(u64), C.c_char_p, sz]),
+        "mtx_target_render": (i32, [u64, u32, vp]),
+        "mtx_target_synchronize": (i32, [u64]),
+        "mtx_target_read_pixels": (i32, [u64, u32, vp, sz, P(sz)]),
+        "mtx_target_read_accumulators": (i32, [u64, vp, sz, vp, sz]),
+        "mtx_target_stats": (i32, [u64, vp]),
+        "mtx_target_set_camera": (i32, [u64, vp]),
+        "mtx_target_reset": (i32, [u64]),
+        "mtx_target_destroy": (i32, [u64]),
+    }
     for name, (res, args) in sig.items():
         if hasattr(L, name):
             f = getattr(L, name)

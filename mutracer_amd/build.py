@@ -27,13 +27,33 @@ FLAGS = [
 ]
 
 
+LUT = os.path.join(HERE, "data", "openpbr_energy_lut.f32")
+
+
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+
+
+def _lut_source():
+    """Writes the OpenPBR energy tables (2177 rows x 32 f32) as a C++ array."""
+    import numpy as np
+    bits = np.fromfile(LUT, dtype="<u4")
+    if bits.size != 32 * 2177:
+        raise RuntimeError(f"{LUT}: expected {32 * 2177} floats, found {bits.size}")
+    path = os.path.join(BUILD, "mtx_energy_lut.cpp")
+    rows = ",\n".join(",".join(f"0x{int(b):08x}u" for b in bits[i:i + 16]) for i in range(0, bits.size, 16))
+    text = ("#include <stdint.h>\nnamespace mtx {\nextern const uint32_t kEnergyLutBits[69664];\n"
+            "alignas(16) const uint32_t kEnergyLutBits[69664] = {\n" + rows + "};\n}\n")
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -42,7 +62,7 @@ def _compile(src):
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sources()
+    srcs = sources() + [_lut_source()]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

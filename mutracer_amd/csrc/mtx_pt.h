@@ -1,0 +1,66 @@
+// Path tracer: resident render scene, render targets and kernel entry points.
+#pragma once
+#include "mtx_scene.h"
+
+namespace mtx {
+
+// Shading data per instance slot (leaf order, parallel to InstSlot).
+struct ShadeSlot {
+  float4 l2w0, l2w1, l2w2, l2w3;  // local-to-world columns (xyz)
+  const float *normals;           // source mesh vertex normals (xyz)
+  const uint32_t *indices;        // source mesh triangle indices
+  uint32_t material;
+  float color[3];
+};
+static_assert(sizeof(ShadeSlot) == 96, "ShadeSlot");
+
+// Alias table entry (lightSampling.ts:172-212): probability, alias, type, index.
+struct LightChoice {
+  float prob;
+  int32_t alias, type, index;
+};
+
+// Packed OpenPBR material (32 floats) and rect light (16 floats); the field
+// offsets are documented with mtx_material_desc / mtx_light_desc in mtx.h.
+struct GpuScene {
+  const AccelView *accel;
+  const ShadeSlot *shade;
+  const float *materials;
+  const float *lights;
+  const float *lightPmf;       // per light selection pmf
+  const LightChoice *choices;
+  int32_t choiceCount;
+  uint32_t lightCount;
+  float envPmf;
+  float lightWeightTotal;
+  const float4 *env;           // RGBA f32, envH rows of envW
+  const float2 *envCdf;        // (envW + 2) x (envH + 1)
+  uint32_t envW, envH;
+  const float *lut;            // 32 x 2177 energy tables
+};
+
+struct RenderParams {
+  float projInv[16], viewInv[16];
+  float aperture, focus;
+  uint32_t width, height;
+  uint32_t bounces, candidates, pattern, seed, logSPP, sampleOffset;
+  float clamping;
+  uint32_t samples;            // samples per pixel this launch
+  uint32_t pixelBase;          // first pixel of this launch (row-major)
+  uint32_t pixelCount;
+};
+
+// FP64 accumulators, one plane per channel (13 planes) + u32 sample counts.
+struct Accumulators {
+  double *planes;  // 13 * pixels
+  uint32_t *count;
+  uint64_t *rays;  // traced rays (camera + continuation + shadow)
+  uint32_t pixels;
+};
+
+struct RenderSceneObj;
+struct TargetObj;
+
+void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s);
+
+}  // namespace mtx

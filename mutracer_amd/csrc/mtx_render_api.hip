@@ -1,0 +1,499 @@
+// C ABI for resident render scenes and render targets (progressive FP64
+// accumulation), plus the host-side light tables: the Walker alias table over
+// {environment, rect lights} (lightSampling.ts:72-107,172-212,415-431) and the
+// environment CDF with its 3x3 max-dilated luminance (envMapDistribution.ts:57-134).
+#include <cmath>
+#include <mutex>
+
+#include "mtx_pt.h"
+#include "mtx_registry.h"
+
+namespace mtx {
+
+extern const uint32_t kEnergyLutBits[32 * 2177];
+
+struct RenderSceneObj {
+  std::vector<std::shared_ptr<Blas>> blases;
+  std::shared_ptr<Accel> accel;
+  std::vector<DevBuf<float>> normals;
+  std::vector<DevBuf<uint32_t>> indices;
+  DevBuf<ShadeSlot> shade;
+  DevBuf<float> materials, lights, lightPmf, lut;
+  DevBuf<LightChoice> choices;
+  DevBuf<float4> env;
+  DevBuf<float2> envCdf;
+  std::vector<mtx_instance_desc> inst;
+  GpuScene view{};
+  uint64_t triangles = 0;
+  size_t bytes() const {
+    size_t b = shade.bytes() + materials.bytes() + lights.bytes() + lut.bytes() + env.bytes() + envCdf.bytes();
+    for (auto &x : normals) b += x.bytes();
+    for (auto &x : indices) b += x.bytes();
+    for (auto &x : blases) b += x->resident_bytes();
+    return b;
+  }
+};
+
+struct TargetObj {
+  std::shared_ptr<RenderSceneObj> scene;
+  RenderParams params{};
+  DevBuf<double> planes;
+  DevBuf<uint32_t> count;
+  DevBuf<uint64_t> rays;
+  uint32_t samplesDone = 0;
+  hipStream_t stream = nullptr;
+  std::mutex busy;
+};
+
+static std::mutex gRenderMutex;
+static std::unordered_map<uint64_t, std::shared_ptr<RenderSceneObj>> gScenes;
+static std::unordered_map<uint64_t, std::shared_ptr<TargetObj>> gTargets;
+
+std::shared_ptr<RenderSceneObj> get_scene(uint64_t h) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  auto it = gScenes.find(h);
+  if (it == gScenes.end()) throw Error(MTX_INVALID_HANDLE, "unknown scene handle");
+  return it->second;
+}
+std::shared_ptr<TargetObj> get_target(uint64_t h) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  auto it = gTargets.find(h);
+  if (it == gTargets.end()) throw Error(MTX_INVALID_HANDLE, "unknown render target handle");
+  return it->second;
+}
+
+// ---------------------------------------------------------------- light tables
+struct LightTables {
+  std::vector<LightChoice> choices;
+  std::vector<float> lightPmf;
+  float envPmf = 0.0f;
+  double total = 0.0;
+};
+
+static LightTables light_tables(const float *lights, uint32_t nLights, const float *env, uint32_t W, uint32_t H) {
+  struct In { int32_t type, index; double w; };
+  std::vector<In> in;
+  auto push = [&](int32_t t, int32_t i, double w) {
+    if (w > 0 && std::isfinite(w)) in.push_back({t, i, w});
+  };
+  double envPower = 0.0;
+  if (env && W && H) {
+    const double kPiD = 3.141592653589793;
+    for (uint32_t y = 0; y < H; y++) {
+      double sinT = std::sin(kPiD * (y + 0.5) / H);
+      for (uint32_t x = 0; x < W; x++) {
+        const float *p = env + ((size_t)y * W + x) * 4;
+        double l = (double)p[0] * 0.2126 + (double)p[1] * 0.7152 + (double)p[2] * 0.0722;
+        envPower += (l > 0 ? l : 0) * sinT;
+      }
+    }
+    envPower = envPower * (2 * kPiD * kPiD) / ((double)W * H);
+  }
+  push(0, 0, envPower);
+  for (uint32_t i = 0; i < nLights; i++) {
+    const float *L = lights + (size_t)i * MTX_LIGHT_FLOATS;
+    double lum = (double)L[MTX_LIGHT_EMISSION] * 0.2126 + (double)L[MTX_LIGHT_EMISSION + 1] * 0.7152 +
+                 (double)L[MTX_LIGHT_EMISSION + 2] * 0.0722;
+    if (lum > 0) push(1, (int32_t)i, lum * 3.141592653589793 * std::max((double)L[MTX_LIGHT_AREA], 0.0));
+  }
+  LightTables T;
+  T.lightPmf.assign(nLights, 0.0f);
+  const size_t n = in.size();
+  double total = 0.0;
+  for (auto &e : in) total += e.w;
+  if (!n || !(total > 0)) return T;
+  T.total = total;
+  T.choices.resize(n);
+  std::vector<float> pmf(n);
+  std::vector<double> scaled(n);
+  std::vector<uint32_t> small, large;
+  for (size_t i = 0; i < n; i++) {
+    pmf[i] = (float)(in[i].w / total);
+    scaled[i] = (double)pmf[i] * (double)n;
+    (scaled[i] < 1 ? small : large).push_back((uint32_t)i);
+  }
+  while (!small.empty() && !large.empty()) {
+    uint32_t lo = small.back(), hi = large.back();
+    small.pop_back();
+    large.pop_back();
+    T.choices[lo].prob = (float)scaled[lo];
+    T.choices[lo].alias = (int32_t)hi;
+    scaled[hi] = scaled[hi] + scaled[lo] - 1;
+    (scaled[hi] < 1 ? small : large).push_back(hi);
+  }
+  for (uint32_t i : small) { T.choices[i].prob = 1.0f; T.choices[i].alias = (int32_t)i; }
+  for (uint32_t i : large) { T.choices[i].prob = 1.0f; T.choices[i].alias = (int32_t)i; }
+  for (size_t i = 0; i < n; i++) {
+    T.choices[i].type = in[i].type;
+    T.choices[i].index = in[i].index;
+    if (in[i].type == 0) T.envPmf = pmf[i];
+    else T.lightPmf[in[i].index] = pmf[i];
+  }
+  return T;
+}
+
+static std::vector<float2> env_cdf(const float *env, uint32_t W, uint32_t H) {
+  const uint32_t CW = W + 2, CH = H + 1;
+  std::vector<float2> cdf((size_t)CW * CH, make_float2(0.0f, 0.0f));
+  auto X = [&](uint32_t x, uint32_t y) -> float & { return cdf[(size_t)y * CW + x].x; };
+  auto Y = [&](uint32_t x, uint32_t y) -> float & { return cdf[(size_t)y * CW + x].y; };
+  std::vector<float> lum((size_t)W * H), horiz((size_t)W * H);
+  for (size_t i = 0; i < (size_t)W * H; i++) {
+    const float *p = env + i * 4;
+    lum[i] = (float)(0.2126 * p[0] + 0.7152 * p[1] + 0.0722 * p[2]);
+  }
+  for (uint32_t y = 0; y < H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      size_t r = (size_t)y * W;
+      double a = lum[r + (x ? x - 1 : 0)], b = lum[r + x], c = lum[r + std::min(W - 1, x + 1)];
+      horiz[r + x] = (float)std::max(std::max(0.0, a), std::max(b, c));
+    }
+  for (uint32_t y = 0; y < H; y++) {
+    double sinT = std::sin(3.141592653589793 * (y + 0.5) / H);
+    size_t above = (size_t)(y ? y - 1 : 0) * W, row = (size_t)y * W, below = (size_t)std::min(H - 1, y + 1) * W;
+    for (uint32_t x = 0; x < W; x++) {
+      double w = std::max(std::max((double)horiz[above + x], (double)horiz[row + x]),
+                          std::max((double)horiz[below + x], 1e-12)) * sinT;
+      X(x + 2, y) = (float)((double)X(x + 1, y) + w / W);
+      Y(x + 1, y) = (float)w;
+    }
+    double rowIntegral = X(CW - 1, y);
+    for (uint32_t x = 1; x < CW; x++) {
+      X(x, y) = (float)((double)X(x, y) / rowIntegral);
+      Y(x, y) = (float)((double)Y(x, y) / rowIntegral);
+    }
+    X(0, y + 1) = (float)((double)X(0, y) + rowIntegral / H);
+    Y(0, y) = (float)rowIntegral;
+  }
+  double integral = X(0, CH - 1);
+  for (uint32_t y = 0; y < CH; y++) {
+    X(0, y) = (float)((double)X(0, y) / integral);
+    Y(0, y) = (float)((double)Y(0, y) / integral);
+  }
+  return cdf;
+}
+
+static void fill_shade(RenderSceneObj &R) {
+  const Accel &A = *R.accel;
+  std::vector<ShadeSlot> sh(std::max<uint32_t>(1, A.instanceCount));
+  for (uint32_t q = 0; q < A.instanceCount; q++) {
+    uint32_t src = A.slotSource[q];
+    const mtx_instance_desc &d = R.inst[src];
+    const double *m = d.transform;
+    ShadeSlot s{};
+    s.l2w0 = make_float4((float)m[0], (float)m[1], (float)m[2], 0.0f);
+    s.l2w1 = make_float4((float)m[4], (float)m[5], (float)m[6], 0.0f);
+    s.l2w2 = make_float4((float)m[8], (float)m[9], (float)m[10], 0.0f);
+    s.l2w3 = make_float4((float)m[12], (float)m[13], (float)m[14], 0.0f);
+    s.normals = R.normals[d.blas_index].ptr;
+    s.indices = R.indices[d.blas_index].ptr;
+    s.material = d.material_word >> 8;
+    for (int k = 0; k < 3; k++) s.color[k] = d.color[k];
+    sh[q] = s;
+  }
+  R.shade.upload_min1(sh.data(), A.instanceCount, 0);
+}
+
+static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
+  MTX_REQUIRE(D.struct_size >= sizeof(mtx_scene_desc), MTX_INVALID_ARGUMENT, "mtx_scene_desc.struct_size too small");
+  MTX_REQUIRE(D.mesh_count == 0 || D.meshes, MTX_INVALID_ARGUMENT, "meshes is NULL");
+  MTX_REQUIRE(D.instance_count == 0 || D.instances, MTX_INVALID_ARGUMENT, "instances is NULL");
+  MTX_REQUIRE(D.material_count > 0 && D.materials, MTX_INVALID_ARGUMENT, "at least one material is required");
+  MTX_REQUIRE(D.light_count == 0 || D.lights, MTX_INVALID_ARGUMENT, "lights is NULL");
+  auto R = std::make_shared<RenderSceneObj>();
+  const uint32_t bins = D.blas_bins ? D.blas_bins : 8u, tbins = D.tlas_bins ? D.tlas_bins : 64u;
+  for (uint32_t i = 0; i < D.mesh_count; i++) {
+    const mtx_mesh_desc &m = D.meshes[i];
+    MTX_REQUIRE(m.struct_size >= sizeof(mtx_mesh_desc), MTX_INVALID_ARGUMENT, "mtx_mesh_desc.struct_size too small");
+    MTX_REQUIRE(m.positions && m.normals && m.indices, MTX_INVALID_ARGUMENT, "mesh arrays are NULL");
+    for (size_t k = 0; k < (size_t)m.triangle_count * 3; k++)
+      MTX_REQUIRE(m.indices[k] < m.vertex_count, MTX_INVALID_ARGUMENT, "mesh index out of range");
+    R->blases.push_back(build_blas(m.positions, m.vertex_count, m.indices, m.triangle_count, bins, false, 0));
+    R->normals.emplace_back();
+    R->normals.back().upload_min1(m.normals, (size_t)m.vertex_count * 3, 0);
+    R->indices.emplace_back();
+    R->indices.back().upload_min1(m.indices, (size_t)m.triangle_count * 3, 0);
+  }
+  R->inst.assign(D.instances, D.instances + D.instance_count);
+  for (auto &d : R->inst) {
+    MTX_REQUIRE(d.blas_index < D.mesh_count, MTX_INVALID_ARGUMENT, "instance mesh index out of range");
+    uint32_t mat = d.material_word >> 8;
+    MTX_REQUIRE(mat < D.material_count, MTX_INVALID_ARGUMENT, "instance material out of range");
+    if (D.materials[(size_t)mat * MTX_MATERIAL_FLOATS + MTX_MAT_DOUBLE_SIDED] > 0.5f) d.double_sided = 1;
+    R->triangles += D.meshes[d.blas_index].triangle_count;
+  }
+  R->accel = build_accel(R->blases, R->inst.data(), D.instance_count, tbins, D.flags, 0);
+  MTX_REQUIRE(R->accel->max_stack() <= (uint32_t)kMaxStack, MTX_INVALID_ARGUMENT, "scene deeper than the traversal stack");
+  fill_shade(*R);
+  R->materials.upload(D.materials, (size_t)D.material_count * MTX_MATERIAL_FLOATS, 0);
+  R->lights.upload_min1(D.lights, (size_t)D.light_count * MTX_LIGHT_FLOATS, 0);
+  const bool hasEnv = D.env_rgba && D.env_width && D.env_height;
+  LightTables T = light_tables(D.lights, D.light_count, hasEnv ? D.env_rgba : nullptr, D.env_width, D.env_height);
+  R->lightPmf.upload_min1(T.lightPmf.data(), T.lightPmf.size(), 0);
+  R->choices.upload_min1(T.choices.data(), T.choices.size(), 0);
+  if (hasEnv) {
+    R->env.upload(reinterpret_cast<const float4 *>(D.env_rgba), (size_t)D.env_width * D.env_height, 0);
+    std::vector<float2> cdf = env_cdf(D.env_rgba, D.env_width, D.env_height);
+    R->envCdf.upload(cdf.data(), cdf.size(), 0);
+  }
+  R->lut.upload(reinterpret_cast<const float *>(kEnergyLutBits), 32 * 2177, 0);
+  GpuScene &v = R->view;
+  v.accel = R->accel->view.ptr;
+  v.shade = R->shade.ptr;
+  v.materials = R->materials.ptr;
+  v.lights = R->lights.ptr;
+  v.lightPmf = R->lightPmf.ptr;
+  v.choices = R->choices.ptr;
+  v.choiceCount = (int32_t)T.choices.size();
+  v.lightCount = D.light_count;
+  v.envPmf = T.envPmf;
+  v.lightWeightTotal = (float)T.total;
+  v.env = hasEnv ? R->env.ptr : nullptr;
+  v.envCdf = hasEnv ? R->envCdf.ptr : nullptr;
+  v.envW = hasEnv ? D.env_width : 0;
+  v.envH = hasEnv ? D.env_height : 0;
+  v.lut = R->lut.ptr;
+  MTX_HIP(hipStreamSynchronize(0));
+  return R;
+}
+
+static uint32_t log2spp(uint32_t pattern, uint32_t maxSamples) {
+  double m = pattern == 1 ? (double)maxSamples - 1 : (double)maxSamples;
+  if (m < 1) m = 1;
+  double l = std::ceil(std::log2(m));
+  return (uint32_t)std::min(16.0, std::max(0.0, l));
+}
+
+static void set_camera(RenderParams &p, const mtx_camera_desc &c) {
+  MTX_REQUIRE(c.struct_size >= sizeof(mtx_camera_desc), MTX_INVALID_ARGUMENT, "mtx_camera_desc.struct_size too small");
+  std::memcpy(p.projInv, c.projection_inverse, 64);
+  std::memcpy(p.viewInv, c.view_inverse, 64);
+  p.aperture = c.aperture_radius;
+  p.focus = c.focus_distance;
+}
+
+void target_reset(TargetObj &T) {
+  MTX_HIP(hipMemsetAsync(T.planes.ptr, 0, T.planes.bytes(), T.stream));
+  MTX_HIP(hipMemsetAsync(T.count.ptr, 0, T.count.bytes(), T.stream));
+  MTX_HIP(hipMemsetAsync(T.rays.ptr, 0, T.rays.bytes(), T.stream));
+  T.samplesDone = 0;
+}
+
+std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, const mtx_render_settings &s,
+                                         const mtx_camera_desc &cam) {
+  MTX_REQUIRE(s.struct_size >= sizeof(mtx_render_settings), MTX_INVALID_ARGUMENT,
+              "mtx_render_settings.struct_size too small");
+  MTX_REQUIRE(s.width > 0 && s.height > 0 && (uint64_t)s.width * s.height <= (1ull << 28), MTX_INVALID_ARGUMENT,
+              "invalid target size");
+  MTX_REQUIRE(s.sampling_pattern <= 1, MTX_INVALID_ARGUMENT, "sampling_pattern must be 0 (sobol) or 1 (zsobol)");
+  MTX_REQUIRE(s.bounces <= 50, MTX_INVALID_ARGUMENT, "bounces above 50");
+  auto T = std::make_shared<TargetObj>();
+  T->scene = scene;
+  RenderParams &p = T->params;
+  set_camera(p, cam);
+  p.width = s.width;
+  p.height = s.height;
+  p.bounces = s.bounces;
+  p.candidates = std::max(1u, s.direct_light_candidates);
+  p.pattern = s.sampling_pattern;
+  p.seed = s.sequence_seed;
+  p.logSPP = log2spp(s.sampling_pattern, s.max_samples);
+  p.sampleOffset = s.sample_sequence_offset;
+  p.clamping = s.clamping;
+  p.pixelBase = 0;
+  p.pixelCount = s.width * s.height;
+  const size_t N = (size_t)s.width * s.height;
+  T->planes.alloc(N * 13);
+  T->count.alloc(N);
+  T->rays.alloc(1);
+  target_reset(*T);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  return T;
+}
+
+void target_render(TargetObj &T, uint32_t samples, hipStream_t s) {
+  RenderParams p = T.params;
+  p.samples = samples;
+  Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height};
+  launch_render(T.scene->view, p, acc, s);
+  T.stream = s;
+  T.samplesDone += samples;
+}
+
+}  // namespace mtx
+
+using namespace mtx;
+
+#define API_TRY try {
+#define API_CATCH(err, cap)                       \
+  }                                               \
+  catch (const mtx::Error &e) {                   \
+    set_error(err, cap, e.what());                \
+    return e.code;                                \
+  }                                               \
+  catch (const std::exception &e) {               \
+    set_error(err, cap, e.what());                \
+    return MTX_ERROR;                             \
+  }
+
+extern "C" {
+
+int32_t mtx_scene_create(const mtx_scene_desc *desc, mtx_handle *out, char *err, size_t cap) {
+  API_TRY
+  MTX_REQUIRE(desc && out, MTX_INVALID_ARGUMENT, "desc/out is NULL");
+  auto R = create_scene(*desc);
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  uint64_t h = registry().next();
+  gScenes[h] = R;
+  *out = h;
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
+int32_t mtx_scene_set_transforms(mtx_handle scene, const double *transforms, uint32_t count, const uint32_t *dirty,
+                                 uint32_t dirtyCount, char *err, size_t cap) {
+  API_TRY
+  auto R = get_scene(scene);
+  MTX_REQUIRE(transforms, MTX_INVALID_ARGUMENT, "transforms is NULL");
+  MTX_REQUIRE(count == R->inst.size(), MTX_INVALID_ARGUMENT, "instance count mismatch");
+  for (uint32_t i = 0; i < count; i++) std::memcpy(R->inst[i].transform, transforms + (size_t)i * 16, 128);
+  refit_accel(*R->accel, R->inst.data(), count, dirty, dirtyCount, 0);
+  fill_shade(*R);
+  MTX_HIP(hipStreamSynchronize(0));
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
+int32_t mtx_scene_info(mtx_handle scene, uint64_t info[4]) {
+  API_TRY
+  auto R = get_scene(scene);
+  MTX_REQUIRE(info, MTX_INVALID_ARGUMENT, "info is NULL");
+  info[0] = R->triangles;
+  info[1] = R->inst.size();
+  info[2] = R->blases.size();
+  info[3] = (R->bytes() + (1u << 20) - 1) >> 20;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_scene_destroy(mtx_handle scene) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  return gScenes.erase(scene) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings, const mtx_camera_desc *camera,
+                          mtx_handle *out, char *err, size_t cap) {
+  API_TRY
+  MTX_REQUIRE(settings && camera && out, MTX_INVALID_ARGUMENT, "settings/camera/out is NULL");
+  auto T = create_target(get_scene(scene), *settings, *camera);
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  uint64_t h = registry().next();
+  gTargets[h] = T;
+  *out = h;
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
+int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream) {
+  API_TRY
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  target_render(*T, samples, (hipStream_t)stream);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_synchronize(mtx_handle target) {
+  API_TRY
+  auto T = get_target(target);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_read_accumulators(mtx_handle target, double *sums, size_t capSums, uint32_t *counts,
+                                     size_t capCounts) {
+  API_TRY
+  auto T = get_target(target);
+  const size_t N = (size_t)T->params.width * T->params.height;
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  if (sums) {
+    if (capSums < N * 13) return MTX_BUFFER_TOO_SMALL;
+    std::vector<double> planes = T->planes.to_host(N * 13, T->stream);
+    for (size_t i = 0; i < N; i++)
+      for (int k = 0; k < 13; k++) sums[i * 13 + k] = planes[(size_t)k * N + i];
+  }
+  if (counts) {
+    if (capCounts < N) return MTX_BUFFER_TOO_SMALL;
+    T->count.download(counts, N, T->stream);
+    MTX_HIP(hipStreamSynchronize(T->stream));
+  }
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_read_pixels(mtx_handle target, uint32_t kind, float *pixels, size_t cap, size_t *written) {
+  API_TRY
+  auto T = get_target(target);
+  const uint32_t W = T->params.width, H = T->params.height;
+  const size_t N = (size_t)W * H;
+  if (written) *written = N * 4;
+  if (!pixels) return MTX_OK;
+  if (cap < N * 4) return MTX_BUFFER_TOO_SMALL;
+  MTX_REQUIRE(kind <= MTX_PIXELS_VARIANCE, MTX_INVALID_ARGUMENT, "unknown pixel kind");
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  std::vector<double> pl = T->planes.to_host(N * 13, T->stream);
+  std::vector<uint32_t> cnt = T->count.to_host(N, T->stream);
+  for (uint32_t y = 0; y < H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      size_t i = (size_t)y * W + x, o = ((size_t)(H - 1 - y) * W + x) * 4;
+      double n = cnt[i] ? (double)cnt[i] : 1.0;
+      if (kind == MTX_PIXELS_VARIANCE) {
+        double mean = (0.2126 * pl[i] + 0.7152 * pl[N + i] + 0.0722 * pl[2 * N + i]) / n;
+        double var = std::max(0.0, pl[12 * N + i] / n - mean * mean) / n;
+        pixels[o] = (float)var; pixels[o + 1] = pixels[o + 2] = 0.0f; pixels[o + 3] = 1.0f;
+        continue;
+      }
+      int base = kind == MTX_PIXELS_COLOR ? 0 : (kind == MTX_PIXELS_ALBEDO ? 4 : 8);
+      for (int k = 0; k < 4; k++) pixels[o + k] = (float)(pl[(size_t)(base + k) * N + i] / n);
+    }
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_stats(mtx_handle target, uint64_t stats[2]) {
+  API_TRY
+  auto T = get_target(target);
+  MTX_REQUIRE(stats, MTX_INVALID_ARGUMENT, "stats is NULL");
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  T->rays.download(&stats[0], 1, T->stream);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  stats[1] = T->samplesDone;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera) {
+  API_TRY
+  auto T = get_target(target);
+  MTX_REQUIRE(camera, MTX_INVALID_ARGUMENT, "camera is NULL");
+  set_camera(T->params, *camera);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_reset(mtx_handle target) {
+  API_TRY
+  auto T = get_target(target);
+  target_reset(*T);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_destroy(mtx_handle target) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  return gTargets.erase(target) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""Path-tracer host: resident render scene, render targets and converge().
+
+Mirrors the reference's operator surface for the render hot path:
+  - RenderScene  ~ WGPURayTracePass.setMeshes / setLights (WGPURayTracePass.ts
+    :2246, :3180) and the TLAS refit of Engine.update (bvhAccel.ts:579);
+  - RenderTarget ~ RenderTarget.trace(camera, maxSamples) (RenderingPipeline.ts
+    :218) with setSamplingPattern (:2073), setSampleSequenceOffset (:2085),
+    setBounce (:2040), restartSamples and readPixels (Render.ts:782-801);
+  - Render.converge(samples) ~ Render.converge (Render.ts:827-900): batches of
+    at most `samples_per_batch` samples until the budget is reached.
+Every call goes to libmtx (HIP); nothing here computes pixels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi_ext import CameraDesc, MeshDesc, RenderSettings, SceneDesc
+from .sampling import sampling_pattern_code
+
+_P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+
+PIXELS = dict(color=0, albedo=1, normal=2, variance=3)
+
+
+class RenderScene:
+    """A committed scene: per-mesh BLASes, the TLAS, lights and environment,
+    resident on the current HIP device."""
+
+    def __init__(self, scene, blas_bins=8, tlas_bins=64, per_instance_bounds=True):
+        L = abi.lib()
+        self.scene = scene
+        self._keep = []
+        meshes = (MeshDesc * max(1, len(scene.meshes)))()
+        for i, m in enumerate(scene.meshes):
+            pos = np.ascontiguousarray(m["positions"], np.float32)
+            nrm = np.ascontiguousarray(m["normals"], np.float32)
+            idx = np.ascontiguousarray(m["indices"], np.uint32)
+            self._keep += [pos, nrm, idx]
+            meshes[i] = MeshDesc(C.sizeof(MeshDesc), pos.size // 3, idx.size // 3, 0, _P(pos).value,
+                                 _P(nrm).value, _P(idx).value)
+        n = len(scene.mesh_index)
+        inst = (abi.InstanceDesc * max(1, n))()
+        for i in range(n):
+            d = inst[i]
+            d.struct_size = C.sizeof(abi.InstanceDesc)
+            d.blas_index = int(scene.mesh_index[i])
+            d.material_word = int(scene.instance_material[i]) << 8
+            d.double_sided = 0
+            d.transform[:] = [float(v) for v in scene.transforms[i]]
+            d.color[:] = [1.0, 1.0, 1.0]
+        mats = np.ascontiguousarray(scene.packed_materials(), np.float32)
+        lights = np.ascontiguousarray(scene.packed_lights(), np.float32)
+        env = None if scene.env is None else np.ascontiguousarray(scene.env, np.float32)
+        self._keep += [mats, lights, env, meshes, inst]
+        desc = SceneDesc(C.sizeof(SceneDesc), len(scene.meshes), C.cast(meshes, C.c_void_p).value, n,
+                         C.cast(inst, C.c_void_p).value, mats.shape[0], _P(mats).value, lights.shape[0],
+                         _P(lights).value if lights.size else None,
+                         0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
+                         None if env is None else _P(env).value, blas_bins, tlas_bins,
+                         abi.MTX_ACCEL_PER_INSTANCE_BOUNDS if per_instance_bounds else 0)
+        self.handle = C.c_uint64(0)
+        err = abi.errbuf()
+        abi.check(L.mtx_scene_create(C.byref(desc), C.byref(self.handle), err, 512), err)
+        self._keep = None
+
+    def info(self):
+        out = np.zeros(4, np.uint64)
+        abi.check(abi.lib().mtx_scene_info(self.handle.value, _P(out)))
+        return dict(triangles=int(out[0]), instances=int(out[1]), blas=int(out[2]), resident_mib=int(out[3]))
+
+    def set_transforms(self, transforms, dirty=None):
+        """Transform-only edit: refits the TLAS over the dirty instances."""
+        t = np.ascontiguousarray(np.asarray(transforms, np.float64).reshape(-1, 16))
+        d = None if dirty is None else np.ascontiguousarray(dirty, np.uint32)
+        err = abi.errbuf()
+        abi.check(abi.lib().mtx_scene_set_transforms(self.handle.value, _P(t), t.shape[0],
+                                                     None if d is None else _P(d), 0 if d is None else d.size,
+                                                     err, 512), err)
+        self.scene.transforms = [row.copy() for row in t]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and abi._lib is not None:
+            abi.lib().mtx_scene_destroy(h.value)
+
+
+def camera_desc(camera, width, height):
+    pi, vi = camera.matrices(width, height)
+    d = CameraDesc()
+    d.struct_size = C.sizeof(CameraDesc)
+    d.projection_inverse[:] = [float(v) for v in pi]
+    d.view_inverse[:] = [float(v) for v in vi]
+    d.aperture_radius = camera.aperture
+    d.focus_distance = camera.focus_distance
+    return d
+
+
+class RenderTarget:
+    """FP64 progressive accumulation of one view of a RenderScene."""
+
+    def __init__(self, render_scene, width, height, bounces=8, direct_light_candidates=1, pattern="sobol",
+                 sequence_seed=0, max_samples=1, sample_sequence_offset=0, clamping=0.0, camera=None):
+        self.scene = render_scene
+        self.width, self.height = width, height
+        s = RenderSettings()
+        s.struct_size = C.sizeof(RenderSettings)
+        s.width, s.height, s.bounces = width, height, bounces
+        s.direct_light_candidates = direct_light_candidates
+        s.sampling_pattern = sampling_pattern_code(pattern) if isinstance(pattern, str) else int(pattern)
+        s.sequence_seed = sequence_seed & 0xFFFFFFFF
+        s.max_samples = max_samples
+        s.sample_sequence_offset = sample_sequence_offset
+        s.clamping = clamping
+        self.settings = s
+        cam = camera_desc(camera or render_scene.scene.camera, width, height)
+        self.handle = C.c_uint64(0)
+        err = abi.errbuf()
+        abi.check(abi.lib().mtx_target_create(render_scene.handle.value, C.byref(s), C.byref(cam),
+                                              C.byref(self.handle), err, 512), err)
+
+    def render(self, samples, stream=None):
+        """Adds `samples` samples to every pixel (asynchronous on `stream`)."""
+        sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+        abi.check(abi.lib().mtx_target_render(self.handle.value, samples, sp))
+
+    def trace(self, camera=None, max_samples=1):
+        """RenderTarget.trace: one more sample unless `max_samples` are done.
+        Returns True while samples remain."""
+        if camera is not None:
+            self.set_camera(camera)
+        done = self.stats()["samples"]
+        if done >= max_samples:
+            return False
+        self.render(1)
+        return done + 1 < max_samples
+
+    def synchronize(self):
+        abi.check(abi.lib().mtx_target_synchronize(self.handle.value))
+
+    def set_camera(self, camera):
+        d = camera_desc(camera, self.width, self.height)
+        abi.check(abi.lib().mtx_target_set_camera(self.handle.value, C.byref(d)))
+
+    def restart_samples(self):
+        abi.check(abi.lib().mtx_target_reset(self.handle.value))
+
+    def stats(self):
+        out = np.zeros(2, np.uint64)
+        abi.check(abi.lib().mtx_target_stats(self.handle.value, _P(out)))
+        return dict(rays=int(out[0]), samples=int(out[1]))
+
+    def read_pixels(self, kind="color"):
+        """(H, W, 4) f32, top row first."""
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        written = C.c_size_t(0)
+        abi.check(abi.lib().mtx_target_read_pixels(self.handle.value, PIXELS[kind], _P(out), out.size,
+                                                   C.byref(written)))
+        return out
+
+    def read_accumulators(self):
+        """(sums (H, W, 13) f64 bottom row first, counts (H, W) u32)."""
+        sums = np.zeros((self.height, self.width, 13), np.float64)
+        counts = np.zeros((self.height, self.width), np.uint32)
+        abi.check(abi.lib().mtx_target_read_accumulators(self.handle.value, _P(sums), sums.size, _P(counts),
+                                                         counts.size))
+        return sums, counts
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and abi._lib is not None:
+            abi.lib().mtx_target_destroy(h.value)
+
+
+class Render:
+    """Offline converge loop over one RenderTarget (Render.converge, Render.ts:827-900)."""
+
+    def __init__(self, target, samples_per_batch=8):
+        self.target = target
+        self.samples_per_batch = samples_per_batch
+
+    def converge(self, samples):
+        self.target.restart_samples()
+        left = samples
+        while left > 0:
+            n = min(self.samples_per_batch, left)
+            self.target.render(n)
+            left -= n
+        self.target.synchronize()
+        return self.target.read_pixels("color")

@@ -1,0 +1,383 @@
+"""Host-side scene description and the procedural benchmark scenes.
+
+A Scene holds plain numpy data: meshes (positions, normals, indices),
+instances (mesh, column-major f64 transform, material), OpenPBR materials,
+rect lights, an optional equirectangular RGBA f32 environment and a camera.
+Its fields mirror the reference's scene inputs (OpenPBR parameters of
+src/scene/openPbrSchema.ts, RectLight of src/scene/RectLight.ts, Camera of
+src/scene/Camera.ts); the packing helpers produce the arrays the C ABI
+(include/mtx.h) consumes.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Packed material layout (32 floats), see include/mtx.h mtx_material_desc.
+MAT_FIELDS = dict(base_weight=0, base_color=1, base_diffuse_roughness=4, base_metalness=5, specular_weight=6,
+                  specular_color=7, specular_roughness=10, specular_ior=11, transmission_weight=12,
+                  transmission_color=13, coat_weight=16, coat_color=17, coat_roughness=20, coat_ior=21,
+                  coat_darkening=22, emission=23, double_sided=26, specular_anisotropy=27)
+
+
+@dataclass
+class Material:
+    """OpenPBR parameters with the reference defaults (openPbrSchema.ts:32-69)."""
+    base_weight: float = 1.0
+    base_color: tuple = (0.8, 0.8, 0.8)
+    base_diffuse_roughness: float = 0.0
+    base_metalness: float = 0.0
+    specular_weight: float = 1.0
+    specular_color: tuple = (1.0, 1.0, 1.0)
+    specular_roughness: float = 0.3
+    specular_ior: float = 1.5
+    specular_anisotropy: float = 0.0
+    transmission_weight: float = 0.0
+    transmission_color: tuple = (1.0, 1.0, 1.0)
+    coat_weight: float = 0.0
+    coat_color: tuple = (1.0, 1.0, 1.0)
+    coat_roughness: float = 0.0
+    coat_ior: float = 1.6
+    coat_darkening: float = 1.0
+    emission: tuple = (0.0, 0.0, 0.0)
+    double_sided: bool = False
+
+    def pack(self):
+        p = np.zeros(32, np.float32)
+        for k, off in MAT_FIELDS.items():
+            v = getattr(self, k)
+            if isinstance(v, tuple):
+                p[off:off + 3] = v
+            else:
+                p[off] = float(v)
+        return p
+
+
+@dataclass
+class RectLight:
+    """Rect area light: center, edge vectors u, v (normal = normalize(u x v)),
+    radiance = color * intensity (RectLight.ts:5-22), spread 1 = Lambertian."""
+    position: tuple
+    u: tuple
+    v: tuple
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+    spread: float = 1.0
+
+    def pack(self):
+        p = np.zeros(16, np.float32)
+        p[0] = 0.0
+        p[1:4] = self.position
+        p[4:7] = self.u
+        p[7:10] = self.v
+        p[10:13] = np.asarray(self.color, np.float64) * self.intensity
+        p[13] = self.spread
+        p[14] = np.linalg.norm(np.cross(self.u, self.v))
+        return p
+
+
+@dataclass
+class Camera:
+    eye: tuple
+    target: tuple
+    up: tuple = (0.0, 1.0, 0.0)
+    fov: float = 45.0          # vertical, degrees (Camera.ts:43-44)
+    near: float = 0.1
+    far: float = 2000.0
+    aperture: float = 0.0
+    focus_distance: float = 1.0
+
+    def matrices(self, width, height):
+        """(projection inverse, view inverse) as column-major f32, computed in f64."""
+        aspect = width / height
+        f = 1.0 / math.tan(math.radians(self.fov) / 2)
+        nf = 1.0 / (self.near - self.far)
+        P = np.zeros((4, 4))
+        P[0, 0] = f / aspect
+        P[1, 1] = f
+        P[2, 2] = (self.far + self.near) * nf
+        P[2, 3] = 2 * self.far * self.near * nf
+        P[3, 2] = -1.0
+        eye, target, up = (np.asarray(a, np.float64) for a in (self.eye, self.target, self.up))
+        z = eye - target
+        z /= np.linalg.norm(z)
+        x = np.cross(up, z)
+        x /= np.linalg.norm(x)
+        y = np.cross(z, x)
+        world = np.eye(4)
+        world[:3, 0], world[:3, 1], world[:3, 2], world[:3, 3] = x, y, z, eye
+        proj_inv = np.linalg.inv(P)
+        return (proj_inv.T.reshape(16).astype(np.float32), world.T.reshape(16).astype(np.float32))
+
+    def pack(self, width, height):
+        pi, vi = self.matrices(width, height)
+        out = np.zeros(36, np.float32)
+        out[0:16] = pi
+        out[16:32] = vi
+        out[32] = self.aperture
+        out[33] = self.focus_distance
+        return out
+
+
+@dataclass
+class Scene:
+    meshes: list = field(default_factory=list)        # dict(positions, normals, indices)
+    mesh_index: list = field(default_factory=list)
+    transforms: list = field(default_factory=list)    # 16 f64 column-major
+    instance_material: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+    lights: list = field(default_factory=list)
+    env: np.ndarray | None = None                      # (H, W, 4) f32
+    camera: Camera | None = None
+
+    def add_mesh(self, positions, indices, normals=None):
+        positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        indices = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        if normals is None:
+            normals = vertex_normals(positions, indices)
+        self.meshes.append(dict(positions=positions, indices=indices,
+                                normals=np.ascontiguousarray(normals, np.float32).reshape(-1, 3)))
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh, transform, material):
+        self.mesh_index.append(mesh)
+        self.transforms.append(np.asarray(transform, np.float64).reshape(16))
+        self.instance_material.append(material)
+
+    @property
+    def triangle_count(self):
+        return sum(self.meshes[m]["indices"].size // 3 for m in self.mesh_index)
+
+    def packed_materials(self):
+        return np.stack([m.pack() for m in self.materials]).astype(np.float32)
+
+    def packed_lights(self):
+        return (np.stack([l.pack() for l in self.lights]).astype(np.float32) if self.lights
+                else np.zeros((0, 16), np.float32))
+
+    def double_sided(self):
+        return [1 if self.materials[m].double_sided else 0 for m in self.instance_material]
+
+
+def vertex_normals(positions, indices):
+    p = positions.astype(np.float64)
+    tri = indices.reshape(-1, 3)
+    fn = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, tri[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    ln[ln == 0] = 1
+    return (n / ln).astype(np.float32)
+
+
+def translate(t):
+    m = np.eye(4)
+    m[:3, 3] = t
+    return m.T.reshape(16).copy()
+
+
+def rot_y(deg):
+    c, s = math.cos(math.radians(deg)), math.sin(math.radians(deg))
+    m = np.eye(4)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = c, s, -s, c
+    return m
+
+
+# ---------------------------------------------------------------- config 1
+def quad(a, b, c, d):
+    """Two triangles a-b-c, a-c-d with the flat normal of a-b-c."""
+    p = np.array([a, b, c, d], np.float32)
+    n = np.cross(p[1] - p[0], p[2] - p[0])
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    return p, np.array([0, 1, 2, 0, 2, 3], np.uint32), np.tile(n, (4, 1))
+
+
+def box_mesh(sx, sy, sz):
+    """Axis-aligned box [−sx/2,sx/2]×[0,sy]×[−sz/2,sz/2], outward flat faces (12 tris)."""
+    x, y, z = sx / 2, sy, sz / 2
+    faces = [
+        [(-x, 0, z), (x, 0, z), (x, y, z), (-x, y, z)],      # +z
+        [(x, 0, -z), (-x, 0, -z), (-x, y, -z), (x, y, -z)],  # -z
+        [(x, 0, z), (x, 0, -z), (x, y, -z), (x, y, z)],      # +x
+        [(-x, 0, -z), (-x, 0, z), (-x, y, z), (-x, y, -z)],  # -x
+        [(-x, y, z), (x, y, z), (x, y, -z), (-x, y, -z)],    # +y
+        [(-x, 0, -z), (x, 0, -z), (x, 0, z), (-x, 0, z)],    # -y
+    ]
+    P, I, N = [], [], []
+    for f in faces:
+        p, i, n = quad(*f)
+        I.append(i + 4 * len(P))
+        P.append(p)
+        N.append(n)
+    return np.concatenate(P), np.concatenate(I), np.concatenate(N)
+
+
+def cornell_box(width=256, height=256):
+    """Config 1: box [-1,1]x[0,2]x[-1,1] (5 quads), two rotated boxes, one
+    0.25x0.25 rect light of radiance 15 just under the ceiling. 34 triangles."""
+    s = Scene()
+    white = Material(base_color=(0.73, 0.73, 0.73))
+    red = Material(base_color=(0.65, 0.05, 0.05))
+    green = Material(base_color=(0.12, 0.45, 0.15))
+    s.materials = [white, red, green]
+    walls = [
+        ([(-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1)], 0),     # floor (+y)
+        ([(-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)], 0),     # ceiling (-y)
+        ([(-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)], 0),   # back (+z)
+        ([(-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1)], 1),   # left red (+x)
+        ([(1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)], 2),       # right green (-x)
+    ]
+    for pts, mat in walls:
+        p, i, n = quad(*pts)
+        s.add_instance(s.add_mesh(p, i, n), np.eye(4).reshape(16), mat)
+    short = box_mesh(0.6, 0.6, 0.6)
+    tall = box_mesh(0.6, 1.2, 0.6)
+    m_short = rot_y(-18)
+    m_short[:3, 3] = (0.33, 0.0, 0.3)
+    m_tall = rot_y(20)
+    m_tall[:3, 3] = (-0.33, 0.0, -0.3)
+    s.add_instance(s.add_mesh(*short), m_short.T.reshape(16), 0)
+    s.add_instance(s.add_mesh(*tall), m_tall.T.reshape(16), 0)
+    s.lights = [RectLight(position=(0.0, 1.999, 0.0), u=(0.25, 0.0, 0.0), v=(0.0, 0.0, 0.25), intensity=15.0)]
+    s.camera = Camera(eye=(0.0, 1.0, 3.94), target=(0.0, 1.0, 0.0), fov=39.3)
+    return s
+
+
+# ---------------------------------------------------------------- config 2
+def icosphere(freq):
+    """Geodesic (class I) icosphere: 20 * freq^2 triangles, shared vertices."""
+    t = (1 + 5 ** 0.5) / 2
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    keys, pts = {}, []
+    tris = []
+
+    def vid(p):
+        k = tuple(np.round(p, 9))
+        if k not in keys:
+            keys[k] = len(pts)
+            pts.append(p)
+        return keys[k]
+
+    for a, b, c in F:
+        A, B, C = V[a], V[b], V[c]
+        grid = {}
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                p = A + (B - A) * (i / freq) + (C - A) * (j / freq)
+                grid[i, j] = vid(p / np.linalg.norm(p))
+        for i in range(freq):
+            for j in range(freq - i):
+                tris.append([grid[i, j], grid[i + 1, j], grid[i, j + 1]])
+                if i + j + 1 < freq:
+                    tris.append([grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]])
+    return np.array(pts, np.float64), np.array(tris, np.uint32).reshape(-1)
+
+
+def _hash3(ix, iy, iz, seed):
+    h = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+    h = h & 0xFFFFFFFF
+    h ^= h >> 13
+    h = (h * 0x5BD1E995) & 0xFFFFFFFF
+    h ^= h >> 15
+    return h / 4294967295.0
+
+
+def value_noise(p, seed):
+    q = np.floor(p).astype(np.int64)
+    f = p - q
+    u = f * f * (3 - 2 * f)
+    out = np.zeros(p.shape[0])
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (np.where(dx, u[:, 0], 1 - u[:, 0]) * np.where(dy, u[:, 1], 1 - u[:, 1]) *
+                     np.where(dz, u[:, 2], 1 - u[:, 2]))
+                out += w * _hash3(q[:, 0] + dx, q[:, 1] + dy, q[:, 2] + dz, seed)
+    return out * 2 - 1
+
+
+def fbm_displaced_sphere(base_pts, indices, seed, amplitude=0.1, octaves=4, lacunarity=2.0, gain=0.5, freq0=2.0):
+    disp = np.zeros(base_pts.shape[0])
+    amp, fr = 1.0, freq0
+    for _ in range(octaves):
+        disp += amp * value_noise(base_pts * fr, seed)
+        amp *= gain
+        fr *= lacunarity
+    pos = base_pts * (1.0 + amplitude * disp)[:, None]
+    pos = pos.astype(np.float32)
+    return pos, indices, vertex_normals(pos, indices)
+
+
+def sky_env(width=1024, height=512, sun_dir=(0.45, 0.57, -0.69), sun_radius_deg=0.5, sun_scale=1e4):
+    """Sky gradient plus a sun lobe (radiance 1e4 x sky) of 0.5 degree radius."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * math.pi
+    phi = u * 2 * math.pi
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    d = np.stack([-st * np.cos(phi)[None], np.broadcast_to(ct, (height, width)), -st * np.sin(phi)[None]], -1)
+    up = np.clip(d[..., 1], 0, 1)
+    horizon = np.array([0.9, 0.85, 0.8])
+    zenith = np.array([0.25, 0.45, 0.85])
+    ground = np.array([0.3, 0.28, 0.25])
+    sky = np.where(d[..., 1:2] >= 0, horizon + (zenith - horizon) * up[..., None] ** 0.5, ground)
+    sd = np.asarray(sun_dir, np.float64)
+    sd /= np.linalg.norm(sd)
+    cosang = (d * sd).sum(-1)
+    sun = cosang > math.cos(math.radians(sun_radius_deg))
+    rgb = sky.copy()
+    rgb[sun] = sun_scale * np.array([1.0, 0.95, 0.85])
+    env = np.ones((height, width, 4), np.float32)
+    env[..., :3] = rgb
+    return env
+
+
+def instanced_field(n_instances=64, freq=28, width=1920, height=1080, seed=1, env_size=(1024, 512)):
+    """Config 2: n displaced geodesic icospheres (20*freq^2 tris each, one
+    BLAS per instance), random TRS over a 20x20 ground quad, OpenPBR
+    metal/glass/coated-diffuse mix, sky+sun environment."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = Scene()
+    base_pts, idx = icosphere(freq)
+    s.materials = [Material(base_color=(0.5, 0.5, 0.5))]  # ground
+    for i in range(n_instances):
+        kind = i % 3
+        if kind == 0:
+            m = Material(base_color=(0.9, 0.6, 0.3), base_metalness=1.0, specular_color=(0.95, 0.8, 0.6),
+                         specular_roughness=float(rng.uniform(0.05, 0.5)))
+        elif kind == 1:
+            m = Material(base_color=(1.0, 1.0, 1.0), transmission_weight=1.0, specular_ior=1.5,
+                         specular_roughness=float(rng.uniform(0.05, 0.5)))
+        else:
+            m = Material(base_color=tuple(rng.uniform(0.2, 0.8, 3)), coat_weight=0.3, coat_roughness=0.1,
+                         specular_roughness=0.5)
+        s.materials.append(m)
+    ground = np.array([[-10, 0, 10], [10, 0, 10], [10, 0, -10], [-10, 0, -10]], np.float32)
+    gp, gi, gn = quad(*ground)
+    s.add_instance(s.add_mesh(gp, gi, gn), np.eye(4).reshape(16), 0)
+    for i in range(n_instances):
+        pos, ind, nrm = fbm_displaced_sphere(base_pts, idx, seed=i)
+        mesh = s.add_mesh(pos, ind, nrm)
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        scale = float(rng.lognormal(0.0, 0.3))
+        t = np.array([rng.uniform(-9, 9), scale * 1.05, rng.uniform(-9, 9)])
+        x, y, z, w = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        M = np.eye(4)
+        M[:3, :3] = R * scale
+        M[:3, 3] = t
+        s.add_instance(mesh, M.T.reshape(16), 1 + i)
+    s.env = sky_env(*env_size)
+    s.camera = Camera(eye=(0.0, 7.0, 17.0), target=(0.0, 0.8, 0.0), fov=40.0)
+    return s

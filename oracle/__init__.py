@@ -332,3 +332,63 @@ def hits_to_ids(scene, hits):
     inst = np.where(hit, scene.target_source[np.where(hit, h[:, 1], 0)], -1)
     tri = np.where(hit, scene.record_triangle[np.where(hit, h[:, 2], 0)].astype(np.int64), -1)
     return hit, inst, tri, h[:, 3]
+
+
+# ---------------------------------------------------------------- integrator
+_LUT = None
+
+
+def _energy_lut():
+    global _LUT
+    if _LUT is None:
+        path = os.path.join(_HERE, "..", "mutracer_amd", "data", "openpbr_energy_lut.f32")
+        _LUT = np.fromfile(path, "<f4")
+        assert _LUT.size == 32 * 2177
+        lib().orc_set_energy_lut(_p(_LUT))
+    return _LUT
+
+
+def _declare_render2(L):
+    vp, u32 = C.c_void_p, C.c_uint32
+    L.orc_set_energy_lut.argtypes = [vp]
+    L.orc_render.restype = C.c_uint64
+    L.orc_render.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, u32,
+                             u32, u32, vp]
+
+
+class OracleRenderer:
+    """CPU restatement of the megakernel over a mutracer_amd.scene.Scene."""
+
+    def __init__(self, scene, width, height, bounces=8, candidates=1, pattern=0, seed=0, max_samples=None,
+                 clamping=0.0, sample_offset=0):
+        L = lib()
+        _declare_render2(L)
+        _energy_lut()
+        self.scene = scene
+        self.trace = assemble_scene(scene.meshes, dict(mesh=scene.mesh_index, transform=np.array(scene.transforms),
+                                                       material=scene.instance_material),
+                                    material_double_sided=[1 if m.double_sided else 0 for m in scene.materials])
+        self.normals = np.ascontiguousarray(np.concatenate([m["normals"] for m in scene.meshes]), np.float32)
+        self.materials = scene.packed_materials()
+        self.lights = scene.packed_lights()
+        self.env = None if scene.env is None else np.ascontiguousarray(scene.env, np.float32)
+        self.camera = scene.camera.pack(width, height)
+        from mutracer_amd.sampling import zsobol_log2spp
+        log = zsobol_log2spp((max_samples or 1) - 1 if pattern == 1 else (max_samples or 1))
+        self.settings = np.array([width, height, bounces, candidates, pattern, seed & 0xFFFFFFFF, log, sample_offset],
+                                 np.uint32)
+        self.clamping = clamping
+        self.width, self.height = width, height
+        self.rays = 0
+
+    def render(self, sample_start, count, x0=0, y0=0, w=None, h=None):
+        w = self.width if w is None else w
+        h = self.height if h is None else h
+        out = np.zeros((h, w, 13), np.float64)
+        env = self.env
+        ew, eh = (0, 0) if env is None else (env.shape[1], env.shape[0])
+        self.rays += lib().orc_render(self.trace.h, _p(self.normals), _p(self.materials), self.materials.shape[0],
+                                      _p(self.lights), self.lights.shape[0], None if env is None else _p(env),
+                                      ew, eh, _p(self.camera), _p(self.settings), self.clamping, x0, y0, w, h,
+                                      sample_start, count, _p(out))
+        return out

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <vector>
 #include "orc_bvh.h"
+#include "orc_render.h"
 #include "orc_sobol.h"
 #include "orc_trace.h"
 
@@ -175,6 +176,29 @@ void orc_trace_brute_batch(void *sp, const float *rays, uint32_t n, uint32_t dou
     o[0] = h.hit; o[1] = (uint32_t)h.instance; o[2] = (uint32_t)h.record; o[3] = f2u(h.t);
     o[4] = f2u(h.bary.x); o[5] = f2u(h.bary.y); o[6] = f2u(h.bary.z); o[7] = 0;
   }
+}
+
+// ---------------------------------------------------------------- integrator
+void orc_set_energy_lut(const float *lut) { set_energy_lut(lut); }
+
+// settings: width, height, bounces, candidates, pattern, seed, logSPP, sampleOffset.
+// out: w*h*13 doubles per pixel (color rgba, albedo rgba, normal rgba, lum^2).
+uint64_t orc_render(void *trace, const float *normals, const float *materials, uint32_t materialCount,
+                    const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
+                    const float *camera, const uint32_t *settings, float clamping, uint32_t x0, uint32_t y0,
+                    uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out) {
+  RenderScene s{};
+  s.trace = (const TraceScene *)trace;
+  s.normals = normals;
+  s.materials = materials; s.materialCount = materialCount;
+  s.lights = lights; s.lightCount = lightCount;
+  s.env = env; s.envW = envW; s.envH = envH;
+  s.camera = camera;
+  s.width = settings[0]; s.height = settings[1]; s.bounces = settings[2]; s.candidates = settings[3];
+  s.pattern = settings[4]; s.seed = settings[5]; s.logSPP = settings[6]; s.sampleOffset = settings[7];
+  s.clamping = clamping;
+  static_assert(sizeof(Accum) == 13 * sizeof(double), "Accum layout");
+  return render(s, x0, y0, w, h, sampleStart, count, reinterpret_cast<Accum *>(out));
 }
 
 }  // extern "C"

@@ -1,0 +1,1281 @@
+// Oracle: RGB megakernel integrator, OpenPBR lobes, lights (test infrastructure only).
+// See orc_render.h for the reference files restated here.
+#include "orc_render.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "orc_sobol.h"
+
+namespace orc {
+
+// ------------------------------------------------------------------ constants
+// WGSL folds constant expressions in abstract (f64) precision and rounds once.
+static const float K_PI = (float)3.14159265359;
+static const float K_QUARTER_PI = (float)(0.25 * 3.14159265359);
+static const float K_HALF_PI = (float)(0.5 * 3.14159265359);
+static const float K_ALPHA_MIN = (float)(0.001 * 0.001);
+static const float K_FON_A = (float)(0.5 - 2.0 / (3.0 * 3.14159265359));
+static const float K_FON_B = (float)(2.0 / 3.0 - 28.0 / (15.0 * 3.14159265359));
+static const float K_EDGE_COS = (float)(1.0 / 7.0);
+static const V3 K_LUM(0.2126f, 0.7152f, 0.0722f);  // rayTraceCore.wgsl.ts:45
+
+static const float *gLut = nullptr;
+void set_energy_lut(const float *lut) { gLut = lut; }
+static inline float lutf(int col, int row) { return gLut[row * 32 + col]; }
+enum { LUT_METAL_ROW = 1056, LUT_METAL_AVG_ROW = 1088, LUT_DIEL_ROW = 1089, LUT_DIEL_AVG_ROW = 2113,
+       LUT_RATIO_ROW = 2145 };
+
+static inline float fractf(float x) { return x - floorf(x); }
+static inline V3 vpow(V3 a, V3 b) { return V3(powf(a.x, b.x), powf(a.y, b.y), powf(a.z, b.z)); }
+static inline V3 vmaxs(V3 a, float s) { return vmax(a, V3(s)); }
+static inline V3 clamp3(V3 v, float lo, float hi) { return V3(clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)); }
+static inline V3 reflect(V3 e1, V3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }
+static inline V3 refract(V3 e1, V3 e2, float e3) {
+  float c = dot(e2, e1);
+  float k = 1.0f - e3 * e3 * (1.0f - c * c);
+  if (k < 0.0f) return V3(0.0f);
+  return e3 * e1 - (e3 * c + std::sqrt(k)) * e2;
+}
+
+// ------------------------------------------------------------------ sampling.wgsl.ts
+static V3 ggx_visible_normal(V3 view, float ax, float ay, float sx, float sy) {
+  V3 stretch(ax, ay, 1.0f);
+  V3 sv = normalize(stretch * view);
+  float vz = mnf(sv.z, 1.0f);
+  float capSpan = 1.0f + vz;
+  float nz = (1.0f - sy) * capSpan;
+  float omz = sy * capSpan;
+  float opz = nz + (1.0f - vz);
+  float radial = std::sqrt(mxf(0.0f, opz * omz));
+  float az = TWOPI * sx;
+  V3 sn(sv.x + radial * std::cos(az), sv.y + radial * std::sin(az), nz);
+  return normalize(stretch * sn);
+}
+static float ggx_ndf(V3 h, float ax, float ay) {
+  V3 e(h.x / ax, h.y / ay, h.z);
+  float den = dot(e, e);
+  return 1.0f / (PI * ax * ay * den * den);
+}
+static float smith_g1(V3 d, float ax, float ay) {
+  float nc = fabsf(d.z);
+  if (nc <= 1e-8f) return 0.0f;
+  float rx = ax * d.x, ry = ay * d.y;
+  float tan2 = (rx * rx + ry * ry) / (nc * nc);
+  return 2.0f / (1.0f + std::sqrt(1.0f + tan2));
+}
+static float schlick5(float c) { return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
+static float dielectric_fresnel_s(float incidentCos, float eta) {
+  float cosI = clampf(fabsf(incidentCos), 0.0f, 1.0f);
+  float sinT2 = eta * eta * mxf(0.0f, 1.0f - cosI * cosI);
+  if (sinT2 >= 1.0f) return 1.0f;
+  float cosT = std::sqrt(mxf(0.0f, 1.0f - sinT2));
+  float perp = (cosI - eta * cosT) / mxf(cosI + eta * cosT, 1e-8f);
+  float par = (eta * cosI - cosT) / mxf(eta * cosI + cosT, 1e-8f);
+  return 0.5f * (perp * perp + par * par);
+}
+static void concentric_disk(float sx, float sy, float &ox, float &oy) {
+  float x = 2.0f * sx - 1.0f, y = 2.0f * sy - 1.0f;
+  if (x == 0.0f && y == 0.0f) { ox = oy = 0.0f; return; }
+  float r, a;
+  if (fabsf(x) > fabsf(y)) { r = x; a = K_QUARTER_PI * (y / x); }
+  else { r = y; a = K_HALF_PI - K_QUARTER_PI * (x / y); }
+  ox = r * std::cos(a);
+  oy = r * std::sin(a);
+}
+static V3 cosine_hemisphere(float sx, float sy) {
+  float dx, dy;
+  concentric_disk(sx, sy, dx, dy);
+  return V3(dx, dy, std::sqrt(mxf(0.0f, 1.0f - (dx * dx + dy * dy))));
+}
+static float power_heuristic(float a, float b) {
+  float scale = mxf(a, b);
+  if (scale <= 0.0f) return 0.0f;
+  float na = a / scale, nb = b / scale;
+  float sa = na * na, sb = nb * nb;
+  return sa / (sa + sb);
+}
+static void orthonormal_basis(V3 n, V3 &t, V3 &b) {
+  float signZ = n.z >= 0.0f ? 1.0f : -1.0f;
+  float factor = -1.0f / (signZ + n.z);
+  float xy = n.x * n.y * factor;
+  t = V3(1.0f + signZ * n.x * n.x * factor, signZ * xy, -signZ * n.x);
+  b = V3(xy, signZ + n.y * n.y * factor, -n.y);
+}
+
+// ------------------------------------------------------------------ materials
+struct Mat {
+  V3 baseColor;
+  float baseWeight, diffuseRoughness, baseMetalness;
+  float specularWeight;
+  V3 specularColor;
+  float specularRoughness, specularIor, specularAnisotropy;
+  float transmissionWeight;
+  V3 transmissionColor;
+  float coatWeight;
+  V3 coatColor;
+  float coatRoughness, coatIor, coatDarkening;
+  V3 emission;
+  float alpha;
+  bool inside;
+  float relativeIor, alphaX, alphaY;
+};
+
+static void roughness_to_alpha(float r, float aniso, float &ax, float &ay) {
+  float a = r * r;
+  if (aniso == 0.0f) { ax = ay = mxf(K_ALPHA_MIN, a); return; }
+  float c = 1.0f - aniso;
+  float at = a * std::sqrt(2.0f / (1.0f + c * c));
+  ax = mxf(K_ALPHA_MIN, at);
+  ay = mxf(K_ALPHA_MIN, c * at);
+}
+
+// getMaterial (materialBuffer.wgsl.ts:170-245) without textures.
+static Mat get_material(const float *m, V3 instanceColor) {
+  Mat ma{};
+  ma.baseColor = V3(m[M_BASE_COLOR], m[M_BASE_COLOR + 1], m[M_BASE_COLOR + 2]);
+  ma.specularRoughness = m[M_SPEC_ROUGHNESS];
+  ma.baseMetalness = m[M_METALNESS];
+  ma.specularAnisotropy = clampf(m[M_SPEC_ANISOTROPY], 0.0f, 1.0f);
+  ma.emission = V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
+  ma.inside = false;
+  ma.specularIor = m[M_SPEC_IOR];
+  ma.transmissionWeight = m[M_TRANS_WEIGHT];
+  ma.alpha = 1.0f;
+  ma.baseWeight = clampf(m[M_BASE_WEIGHT], 0.0f, 1.0f);
+  ma.diffuseRoughness = clampf(m[M_DIFFUSE_ROUGHNESS], 0.0f, 1.0f);
+  ma.specularWeight = mxf(m[M_SPEC_WEIGHT], 0.0f);
+  ma.specularColor = vmaxs(V3(m[M_SPEC_COLOR], m[M_SPEC_COLOR + 1], m[M_SPEC_COLOR + 2]), 0.0f);
+  ma.transmissionColor = vmaxs(V3(m[M_TRANS_COLOR], m[M_TRANS_COLOR + 1], m[M_TRANS_COLOR + 2]), 0.0f);
+  ma.coatWeight = clampf(m[M_COAT_WEIGHT], 0.0f, 1.0f);
+  ma.coatColor = vmaxs(V3(m[M_COAT_COLOR], m[M_COAT_COLOR + 1], m[M_COAT_COLOR + 2]), 0.0f);
+  ma.coatRoughness = clampf(m[M_COAT_ROUGHNESS], ROUGHNESS_MIN, 1.0f);
+  ma.coatIor = mxf(m[M_COAT_IOR], 1.0f);
+  ma.coatDarkening = clampf(m[M_COAT_DARKENING], 0.0f, 1.0f);
+  if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
+    float b2 = ma.specularRoughness * ma.specularRoughness;
+    float c2 = ma.coatRoughness * ma.coatRoughness;
+    ma.specularRoughness = mixf(ma.specularRoughness, powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
+  }
+  ma.baseColor = ma.baseColor * instanceColor;
+  ma.specularRoughness = clampf(ma.specularRoughness, ROUGHNESS_MIN, 1.0f);
+  ma.baseMetalness = clampf(ma.baseMetalness, 0.0f, 1.0f);
+  roughness_to_alpha(ma.specularRoughness, ma.specularAnisotropy, ma.alphaX, ma.alphaY);
+  return ma;
+}
+
+// ------------------------------------------------------------------ OpenPBR
+enum { L_FUZZ = 0, L_COAT = 1, L_METAL = 2, L_SPEC = 3, L_TRANS = 4, L_DIFF = 5, L_SSS = 6, L_COUNT = 7 };
+
+struct Basis { V3 t, b, n; };
+struct Prepared {
+  Mat material;
+  Basis basis;
+  V3 viewLocal, diffuseColor;
+  float specularEta, refractionEta, transmissionReflectionIor, specularAlpha;
+  float diffuseSpecularEnergyCompensation;
+  V3 coatBaseScale, metalMsScale;
+  float metalMsFactor, metalMsProbability;
+  V3 dielMsReflScale, dielMsTransScale;
+  float dielMsViewLoss, dielMsRatio, dielMsProbability;
+  float w[L_COUNT], p[L_COUNT];
+};
+
+static float lum_pbr(V3 c) { return dot(c, V3(0.212671f, 0.715160f, 0.072169f)); }
+static float f0_from_ior(float e) { float v = (e - 1.0f) / (e + 1.0f); return v * v; }
+static float apply_specular_weight_to_ior(float eta, float w) {
+  float f0 = mnf(f0_from_ior(eta) * w, 0.9999f);
+  float s = std::sqrt(f0);
+  float ext = (1.0f + s) / (1.0f - s);
+  return eta < 1.0f ? 1.0f / ext : ext;
+}
+static float pbr_dielectric_fresnel(float c, float rel) {
+  if (fabsf(rel - 1.0f) < 1e-6f) return 0.0f;
+  return dielectric_fresnel_s(c, rel);
+}
+static float energy_ior_index(float ior) {
+  if (ior < 1.0f) return 15.0f - ((1.0f / ior - 1.0f) / 1.5f) * 15.0f;
+  return 16.0f + ((ior - 1.0f) / 1.5f) * 15.0f;
+}
+static float trilinear(int rowBase, float eI, float eA, float eC) {
+  float iI = clampf(eI, 0.0f, 31.0f), aI = clampf(eA, 0.0f, 31.0f), cI = clampf(eC, 0.0f, 31.0f);
+  int i0 = (int)floorf(iI), a0 = (int)floorf(aI), c0 = (int)floorf(cI);
+  int i1 = std::min(i0 + 1, 31), a1 = std::min(a0 + 1, 31), c1 = std::min(c0 + 1, 31);
+  float im = iI - (float)i0, am = aI - (float)a0, cm = cI - (float)c0;
+  float v000 = lutf(c0, rowBase + i0 * 32 + a0), v001 = lutf(c1, rowBase + i0 * 32 + a0);
+  float v010 = lutf(c0, rowBase + i0 * 32 + a1), v011 = lutf(c1, rowBase + i0 * 32 + a1);
+  float v100 = lutf(c0, rowBase + i1 * 32 + a0), v101 = lutf(c1, rowBase + i1 * 32 + a0);
+  float v110 = lutf(c0, rowBase + i1 * 32 + a1), v111 = lutf(c1, rowBase + i1 * 32 + a1);
+  float v00 = mixf(v000, v100, im), v01 = mixf(v001, v101, im);
+  float v10 = mixf(v010, v110, im), v11 = mixf(v011, v111, im);
+  return mixf(mixf(v00, v10, am), mixf(v01, v11, am), cm);
+}
+static float bilinear(int rowBase, float eI, float eA) {
+  float iI = clampf(eI, 0.0f, 31.0f), aI = clampf(eA, 0.0f, 31.0f);
+  int i0 = (int)floorf(iI), a0 = (int)floorf(aI);
+  int i1 = std::min(i0 + 1, 31), a1 = std::min(a0 + 1, 31);
+  float im = iI - (float)i0, am = aI - (float)a0;
+  float v00 = lutf(a0, rowBase + i0), v01 = lutf(a1, rowBase + i0);
+  float v10 = lutf(a0, rowBase + i1), v11 = lutf(a1, rowBase + i1);
+  return mixf(mixf(v00, v10, im), mixf(v01, v11, im), am);
+}
+static float ideal_loss(float ior, float a, float c) { return trilinear(LUT_DIEL_ROW, energy_ior_index(ior), std::sqrt(a) * 31.0f, c * 31.0f); }
+static float ideal_avg_loss(float ior, float a) { return bilinear(LUT_DIEL_AVG_ROW, energy_ior_index(ior), std::sqrt(a) * 31.0f); }
+static float ideal_ratio(float ior, float a) { return bilinear(LUT_RATIO_ROW, energy_ior_index(ior), std::sqrt(a) * 31.0f); }
+static float metal_energy_complement(float a, float c) {
+  float rI = clampf(std::sqrt(a) * 31.0f, 0.0f, 31.0f), cI = clampf(c * 31.0f, 0.0f, 31.0f);
+  int r0 = (int)floorf(rI), c0 = (int)floorf(cI);
+  int r1 = std::min(r0 + 1, 31), c1 = std::min(c0 + 1, 31);
+  float fc = fractf(cI), fr = fractf(rI);
+  return mixf(mixf(lutf(c0, LUT_METAL_ROW + r0), lutf(c1, LUT_METAL_ROW + r0), fc),
+              mixf(lutf(c0, LUT_METAL_ROW + r1), lutf(c1, LUT_METAL_ROW + r1), fc), fr);
+}
+static float metal_average_energy_complement(float a) {
+  float idx = clampf(std::sqrt(a) * 31.0f, 0.0f, 31.0f);
+  int i0 = (int)floorf(idx);
+  return mixf(lutf(i0, LUT_METAL_AVG_ROW), lutf(std::min(i0 + 1, 31), LUT_METAL_AVG_ROW), fractf(idx));
+}
+static float extrapolate(float v, float ior) {
+  if (ior > 2.5f || ior < 0.4f) {
+    float maxF0 = f0_from_ior(2.5f);
+    float progress = (f0_from_ior(ior) - maxF0) / (1.0f - maxF0);
+    return (1.0f - progress) * v;
+  }
+  return v;
+}
+static float opaque_energy_complement(float ior, float a, float c) {
+  return extrapolate(trilinear(0, energy_ior_index(ior), std::sqrt(a) * 31.0f, c * 31.0f), ior);
+}
+static float opaque_average_energy_complement(float ior, float a) {
+  return extrapolate(bilinear(1024, energy_ior_index(ior), std::sqrt(a) * 31.0f), ior);
+}
+static V3 to_world(const Basis &B, V3 d) { return d.x * B.t + d.y * B.b + d.z * B.n; }
+static V3 to_local(const Basis &B, V3 d) { return V3(dot(d, B.t), dot(d, B.b), dot(d, B.n)); }
+
+static V3 metal_fresnel(float c, V3 f0, V3 tint) {
+  V3 schlick = mix3(f0, V3(1.0f), schlick5(c));
+  if (tint.x == 1.0f && tint.y == 1.0f && tint.z == 1.0f) return schlick;
+  float edgeComp = 1.0f - K_EDGE_COS;
+  V3 edgeF = mix3(f0, V3(1.0f), schlick5(K_EDGE_COS));
+  float comp = 1.0f - c;
+  float correction = c * schlick5(c) * comp / (K_EDGE_COS * powf(edgeComp, 6.0f));
+  return clamp3(schlick - correction * (edgeF - tint * edgeF), 0.0f, 1.0f);
+}
+static V3 eval_diffuse(const Mat &m, V3 color, V3 v, V3 l, float comp, float &pdf) {
+  pdf = 0.0f;
+  if (l.z <= 0.0f) return V3(0.0f);
+  float r = m.diffuseRoughness;
+  float muI = l.z, muO = v.z;
+  float directional = dot(l, v) - muI * muO;
+  float dot_ = directional > 0.0f ? directional / mxf(mxf(muI, muO), 1e-7f) : directional;
+  float ssA = 1.0f / (1.0f + K_FON_A * r);
+  V3 single = color * INVPI * ssA * (1.0f + r * dot_);
+  float ic = 1.0f - muI, oc = 1.0f - muO;
+  float iG = ic * (0.0571085289f + ic * (0.491881867f + ic * (-0.332181442f + ic * 0.0714429953f)));
+  float oG = oc * (0.0571085289f + oc * (0.491881867f + oc * (-0.332181442f + oc * 0.0714429953f)));
+  float iAlb = (1.0f + r * iG) / (1.0f + K_FON_A * r);
+  float oAlb = (1.0f + r * oG) / (1.0f + K_FON_A * r);
+  float avg = ssA * (1.0f + K_FON_B * r);
+  V3 msAlb = color * color * avg / vmax(V3(1e-7f), V3(1.0f) - color * (1.0f - avg));
+  V3 multiple = msAlb * INVPI * mxf(1e-7f, 1.0f - oAlb) * mxf(1e-7f, 1.0f - iAlb) / mxf(1e-7f, 1.0f - avg);
+  pdf = l.z * INVPI;
+  return (single + multiple) * comp;
+}
+static V3 eval_refl(float ax, float ay, V3 v, V3 l, V3 h, V3 F, float &pdf) {
+  pdf = 0.0f;
+  if (l.z <= 0.0f) return V3(0.0f);
+  float D = ggx_ndf(h, ax, ay);
+  float G1 = smith_g1(v, ax, ay);
+  float G2 = G1 * smith_g1(l, ax, ay);
+  pdf = G1 * D / (4.0f * mxf(v.z, 1e-8f));
+  return F * D * G2 / (4.0f * mxf(l.z * v.z, 1e-8f));
+}
+static V3 eval_refr(float ax, float ay, float rel, V3 tint, V3 v, V3 l, V3 h, V3 F, float &pdf) {
+  pdf = 0.0f;
+  if (l.z >= 0.0f) return V3(0.0f);
+  float lh = dot(l, h), vh = dot(v, h);
+  if (vh <= 0.0f || lh >= 0.0f) return V3(0.0f);
+  float D = ggx_ndf(h, ax, ay);
+  float G1 = smith_g1(v, ax, ay);
+  float G2 = G1 * smith_g1(l, ax, ay);
+  float den = lh + vh * rel;
+  den *= den;
+  float jac = fabsf(lh) / mxf(den, 1e-8f);
+  float rel2 = rel * rel;
+  pdf = G1 * mxf(0.0f, vh) * D * jac / mxf(v.z, 1e-8f);
+  return tint * (V3(1.0f) - F) * D * G2 * fabsf(vh) * jac * rel2 / mxf(fabsf(l.z * v.z), 1e-8f);
+}
+static float average_fresnel(float eta) {
+  if (fabsf(eta - 1.0f) < 1e-6f) return 0.0f;
+  if (eta > 1.0f) return (eta - 1.0f) / (4.08567f + 1.00071f * eta);
+  return 0.997118f + 0.1014f * eta - 0.965241f * eta * eta - 0.130607f * eta * eta * eta;
+}
+static V3 metal_average_fresnel(V3 f0, V3 tint) {
+  V3 edgeF = mix3(f0, V3(1.0f), schlick5(K_EDGE_COS));
+  V3 b = edgeF * (V3(1.0f) - tint) / (K_EDGE_COS * powf(1.0f - K_EDGE_COS, 6.0f));
+  return clamp3(f0 + (V3(1.0f) - f0) / 21.0f - b / 126.0f, 0.0f, 1.0f);
+}
+static V3 coat_passage(const Mat &m, float c) {
+  if (c <= 0.0f) return V3(1.0f);
+  float rc = std::sqrt(mxf(0.0f, 1.0f - (1.0f - c * c) / (m.coatIor * m.coatIor)));
+  V3 passage = vpow(vmaxs(m.coatColor, 0.0f), V3(0.5f / mxf(rc, 1e-5f)));
+  return mix3(V3(1.0f), passage, m.coatWeight);
+}
+static float coat_energy(const Mat &m, float c) {
+  if (m.coatRoughness <= ROUGHNESS_MIN) return pbr_dielectric_fresnel(c, 1.0f / m.coatIor);
+  return clampf(1.0f - opaque_energy_complement(m.coatIor, m.coatRoughness * m.coatRoughness, c), 0.0f, 1.0f);
+}
+static V3 coat_darkening(const Prepared &P) {
+  const Mat &m = P.material;
+  float diel = 1.0f - m.baseMetalness;
+  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - 0.0f);
+  float sss = diel * (1.0f - m.transmissionWeight) * 0.0f;
+  float trans = diel * m.transmissionWeight;
+  float oSpec = average_fresnel(P.specularEta);
+  float tSpec = average_fresnel(1.0f / P.transmissionReflectionIor);
+  float specularity = clampf(opaque * oSpec + (sss + trans) * tSpec + m.baseMetalness, 0.0f, 1.0f);
+  float effR = mixf(1.0f, m.specularRoughness, specularity);
+  float ks = average_fresnel(m.coatIor);
+  float kr = 1.0f - (1.0f - ks) / (m.coatIor * m.coatIor);
+  float k = mixf(ks, kr, effR);
+  V3 eb = m.baseMetalness * m.specularWeight * metal_average_fresnel(m.baseWeight * m.baseColor, m.specularColor) +
+          opaque * mix3(m.baseWeight * m.baseColor, V3(1.0f), oSpec) + sss * V3(0.8f) + V3(trans);
+  V3 delta = V3(1.0f - k) / vmax(V3(1.0f) - eb * k, V3(1e-6f));
+  return mix3(V3(1.0f), delta, m.coatWeight * m.coatDarkening);
+}
+static V3 coat_base_scale(const Prepared &P, float lightCos) {
+  if (P.material.coatWeight <= 0.0f) return V3(1.0f);
+  if (P.material.inside) return coat_passage(P.material, -lightCos);
+  return P.coatBaseScale * coat_passage(P.material, lightCos);
+}
+
+// openpbr_prepare (openPbrBsdf.wgsl.ts:853-1078) without fuzz, subsurface,
+// thin walls, thin film and anisotropy frames.
+static Prepared prepare(const Mat &m, V3 view, V3 normal) {
+  Prepared P{};
+  P.material = m;
+  orthonormal_basis(normal, P.basis.t, P.basis.b);
+  P.basis.n = normal;
+  P.viewLocal = to_local(P.basis, view);
+  P.diffuseColor = m.baseColor;
+  float dielBase = 1.0f - m.baseMetalness;
+  float opaqueDiel = dielBase * (1.0f - m.transmissionWeight);
+  P.w[L_FUZZ] = 0.0f;
+  P.w[L_COAT] = m.inside ? 0.0f : m.coatWeight;
+  P.w[L_METAL] = m.baseMetalness;
+  P.w[L_SPEC] = dielBase;
+  P.w[L_TRANS] = dielBase * m.transmissionWeight;
+  P.w[L_DIFF] = opaqueDiel * (1.0f - 0.0f) * m.baseWeight;
+  P.w[L_SSS] = opaqueDiel * 0.0f;
+  P.specularEta = apply_specular_weight_to_ior(1.0f / m.relativeIor, m.specularWeight);
+  P.material.relativeIor = 1.0f / P.specularEta;
+  P.refractionEta = P.specularEta;
+  P.transmissionReflectionIor = P.material.relativeIor;
+  if (m.coatWeight > 0.0f) {
+    float wCoat = 1.0f;
+    if (!m.inside) wCoat = mixf(1.0f, m.coatIor, m.coatWeight);
+    float wBase = wCoat * apply_specular_weight_to_ior(m.specularIor / wCoat, m.specularWeight);
+    float reflEta = wBase / wCoat;
+    if ((wCoat > wBase) && (wBase >= 1.0f)) reflEta = 1.0f / reflEta;
+    P.specularEta = reflEta;
+    P.transmissionReflectionIor = 1.0f / reflEta;
+    P.refractionEta = wBase;
+    if (m.inside) {
+      P.refractionEta = 1.0f / wBase;
+      P.transmissionReflectionIor = wBase;
+    }
+    P.material.relativeIor = 1.0f / P.refractionEta;
+  }
+  P.specularAlpha = m.specularRoughness * m.specularRoughness;
+  float vc = fabsf(P.viewLocal.z);
+  float sif = pbr_dielectric_fresnel(vc, 1.0f / P.specularEta);
+  V3 dielF = m.specularColor * sif;
+  V3 metalF = m.specularWeight * metal_fresnel(vc, m.baseWeight * m.baseColor, m.specularColor);
+  float transF = pbr_dielectric_fresnel(vc, P.transmissionReflectionIor);
+  V3 transTint = m.transmissionColor;
+  float viewEC = opaque_energy_complement(P.specularEta, P.specularAlpha, vc);
+  float avgEC = opaque_average_energy_complement(P.specularEta, P.specularAlpha);
+  P.diffuseSpecularEnergyCompensation = viewEC / mxf(avgEC, 1e-12f);
+  P.p[L_FUZZ] = 0.0f;
+  P.p[L_COAT] = 0.0f;
+  P.coatBaseScale = V3(1.0f);
+  if ((m.coatWeight > 0.0f) && !m.inside) {
+    float ce = coat_energy(m, vc);
+    P.p[L_COAT] = P.w[L_COAT] * ce;
+    P.coatBaseScale = coat_passage(m, vc) * (1.0f - P.w[L_COAT] * ce) * coat_darkening(P);
+  }
+  P.p[L_METAL] = P.w[L_METAL] * lum_pbr(metalF);
+  P.metalMsScale = V3(0.0f);
+  P.metalMsFactor = 0.0f;
+  P.metalMsProbability = 0.0f;
+  if ((m.specularRoughness >= 0.04f) && (P.w[L_METAL] > 0.0f)) {
+    V3 avgF = metal_average_fresnel(m.baseWeight * m.baseColor, m.specularColor);
+    P.metalMsScale = m.specularWeight * avgF * avgF;
+    float viewLoss = metal_energy_complement(P.specularAlpha, vc);
+    P.metalMsFactor = viewLoss / mxf(metal_average_energy_complement(P.specularAlpha), 1e-12f);
+    float est = P.w[L_METAL] * lum_pbr(P.metalMsScale) * viewLoss;
+    P.p[L_METAL] += est;
+    P.metalMsProbability = est / mxf(P.p[L_METAL], 1e-12f);
+  }
+  P.p[L_SPEC] = P.w[L_SPEC] * lum_pbr(dielF);
+  P.p[L_TRANS] = P.w[L_TRANS] * (1.0f - transF) * lum_pbr(transTint);
+  P.dielMsReflScale = V3(0.0f);
+  P.dielMsTransScale = V3(0.0f);
+  P.dielMsViewLoss = 0.0f;
+  P.dielMsRatio = 0.0f;
+  P.dielMsProbability = 0.0f;
+  if ((m.transmissionWeight > 0.0f) & (P.specularAlpha >= 0.0016f) & (P.w[L_TRANS] > 0.0f) &
+      (fabsf(P.refractionEta - 1.0f) > 1e-6f)) {
+    float nonOpaque = P.w[L_TRANS] + P.w[L_SSS];
+    P.dielMsReflScale = (m.inside ? V3(1.0f) : m.specularColor) * nonOpaque;
+    P.dielMsTransScale = transTint * P.w[L_TRANS] + V3(P.w[L_SSS]);
+    P.dielMsRatio = ideal_ratio(P.refractionEta, P.specularAlpha);
+    P.dielMsViewLoss = ideal_loss(P.refractionEta, P.specularAlpha, vc);
+    V3 avgScale = mix3(P.dielMsTransScale, P.dielMsReflScale, P.dielMsRatio);
+    float est = lum_pbr(avgScale) * P.dielMsViewLoss;
+    P.p[L_TRANS] += est;
+    P.dielMsProbability = est / mxf(P.p[L_TRANS], 1e-12f);
+  }
+  P.p[L_DIFF] = P.w[L_DIFF] * lum_pbr(P.diffuseColor) * P.diffuseSpecularEnergyCompensation;
+  P.p[L_SSS] = 0.0f;
+  if ((m.coatWeight > 0.0f) && !m.inside) {
+    float bs = maxc(P.coatBaseScale);
+    P.p[L_METAL] *= bs; P.p[L_SPEC] *= bs; P.p[L_TRANS] *= bs; P.p[L_DIFF] *= bs; P.p[L_SSS] *= bs;
+  }
+  float total = P.p[0];
+  for (int l = 1; l < L_COUNT; l++) total += P.p[l];
+  if (total <= 1e-8f) {
+    for (int l = 0; l < L_COUNT; l++) P.p[l] = 0.0f;
+  } else {
+    float inv = 1.0f / total;
+    for (int l = 0; l < L_COUNT; l++) P.p[l] *= inv;
+  }
+  return P;
+}
+
+static float substrate_fresnel(const Prepared &P, float c, bool reflection) {
+  if (P.material.coatWeight <= 0.0f) return pbr_dielectric_fresnel(c, P.material.relativeIor);
+  float tF = pbr_dielectric_fresnel(c, P.transmissionReflectionIor);
+  if (!reflection) return tF;
+  float oF = pbr_dielectric_fresnel(c, 1.0f / P.specularEta);
+  float nonOpaque = P.material.transmissionWeight + (1.0f - P.material.transmissionWeight) * 0.0f;
+  return mixf(oF, tF, nonOpaque);
+}
+
+// openpbr_eval_prepared_lobe (openPbrBsdf.wgsl.ts:1092-1421), allowedLobe = all.
+static V3 eval_prepared(const Prepared &P, V3 lightDir, float &pdf) {
+  const Mat &m = P.material;
+  V3 v = P.viewLocal, l = to_local(P.basis, lightDir);
+  bool reflection = l.z * v.z > 0.0f;
+  V3 h = reflection ? normalize(l + v) : normalize(l + v * m.relativeIor);
+  if (h.z < 0.0f) h = -h;
+  V3 response(0.0f);
+  pdf = 0.0f;
+  float lobePdf = 0.0f;
+  float viewHalf = fabsf(dot(v, h));
+  bool evalDiel = (reflection & (P.w[L_SPEC] > 0.0f)) | (!reflection & (P.w[L_TRANS] > 0.0f));
+  bool evalMetal = reflection & (P.w[L_METAL] > 0.0f);
+  V3 fr0(0.0f), fr1(0.0f);
+  if (evalDiel) fr0 = V3(substrate_fresnel(P, viewHalf, reflection));
+  if (evalMetal) fr1 = metal_fresnel(viewHalf, m.baseWeight * m.baseColor, m.specularColor);
+  float comp = P.diffuseSpecularEnergyCompensation * opaque_energy_complement(P.specularEta, P.specularAlpha, fabsf(l.z));
+  if (reflection) {
+    if (P.w[L_DIFF] > 0.0f) {
+      response = response + P.w[L_DIFF] * eval_diffuse(m, P.diffuseColor, v, l, comp, lobePdf);
+      pdf += P.p[L_DIFF] * lobePdf;
+    }
+    if (P.w[L_SPEC] > 0.0f) {
+      V3 F = m.specularColor * fr0;
+      response = response + P.w[L_SPEC] * eval_refl(m.alphaX, m.alphaY, v, l, h, F, lobePdf);
+      pdf += P.p[L_SPEC] * lobePdf;
+    }
+    if (P.w[L_METAL] > 0.0f) {
+      V3 F = m.specularWeight * fr1;
+      response = response + P.w[L_METAL] * eval_refl(m.alphaX, m.alphaY, v, l, h, F, lobePdf);
+      if (P.metalMsFactor > 0.0f) {
+        float factor = mnf(P.metalMsFactor * metal_energy_complement(P.specularAlpha, l.z), 1.0f / mxf(l.z, 1e-8f));
+        response = response + P.w[L_METAL] * P.metalMsScale * factor * INVPI;
+        lobePdf = mixf(lobePdf, 0.5f * INVPI, P.metalMsProbability);
+      }
+      pdf += P.p[L_METAL] * lobePdf;
+    }
+  } else {
+    if (P.w[L_TRANS] > 0.0f) {
+      response = response + P.w[L_TRANS] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l, h,
+                                                     fr0, lobePdf);
+      if (P.dielMsProbability > 0.0f) lobePdf *= 1.0f - P.dielMsProbability;
+      pdf += P.p[L_TRANS] * lobePdf;
+    }
+  }
+  if (P.dielMsProbability > 0.0f) {
+    float c = fabsf(l.z);
+    if (c > 0.0f) {
+      float ior = reflection ? P.refractionEta : 1.0f / P.refractionEta;
+      float ratio = reflection ? P.dielMsRatio : 1.0f - P.dielMsRatio;
+      float factors = P.dielMsViewLoss * ideal_loss(ior, P.specularAlpha, c) / mxf(ideal_avg_loss(ior, P.specularAlpha), 1e-12f);
+      float factor = mnf(factors * ratio, 1.0f / c);
+      V3 scale = reflection ? P.dielMsReflScale : P.dielMsTransScale;
+      response = response + scale * factor * INVPI;
+      pdf += P.p[L_TRANS] * P.dielMsProbability * ratio * 0.5f * INVPI;
+    }
+  }
+  if (m.coatWeight > 0.0f) response = response * coat_base_scale(P, l.z);
+  if (reflection & (P.w[L_COAT] > 0.0f)) {
+    V3 coatF(pbr_dielectric_fresnel(viewHalf, 1.0f / m.coatIor));
+    float cax, cay;
+    roughness_to_alpha(m.coatRoughness, 0.0f, cax, cay);
+    response = response + P.w[L_COAT] * eval_refl(cax, cay, v, l, h, coatF, lobePdf);
+    pdf += P.p[L_COAT] * lobePdf;
+  }
+  return response * fabsf(l.z);
+}
+
+// openpbr_sample_direction (openPbrBsdf.wgsl.ts:1456-1656).
+static bool sample_direction(const Prepared &P, float rx, float ry, float lobe, V3 &lightDir, bool &transmission) {
+  const Mat &m = P.material;
+  V3 v = P.viewLocal;
+  transmission = false;
+  lightDir = P.basis.n;
+  float cdf[L_COUNT];
+  cdf[0] = P.p[0];
+  for (int i = 1; i < L_COUNT; i++) cdf[i] = cdf[i - 1] + P.p[i];
+  if (cdf[L_SSS] <= 0.0f) return false;
+  V3 l;
+  if ((lobe < cdf[L_FUZZ]) | ((lobe >= cdf[L_TRANS]) & (lobe < cdf[L_DIFF]))) {
+    l = cosine_hemisphere(rx, ry);
+  } else if (lobe < cdf[L_COAT]) {
+    float cax, cay;
+    roughness_to_alpha(m.coatRoughness, 0.0f, cax, cay);
+    V3 h = ggx_visible_normal(v, cax, cay, rx, ry);
+    if (h.z < 0.0f) h = -h;
+    l = normalize(reflect(-v, h));
+  } else if (lobe < cdf[L_SPEC]) {
+    bool isMetal = lobe < cdf[L_METAL];
+    float metalDraw = (lobe - cdf[L_COAT]) / mxf(P.p[L_METAL], 1e-12f);
+    if (isMetal & (metalDraw < P.metalMsProbability)) {
+      float z = rx;
+      float radial = std::sqrt(mxf(0.0f, 1.0f - z * z));
+      float az = TWOPI * ry;
+      l = V3(radial * std::cos(az), radial * std::sin(az), z);
+    } else {
+      V3 h = ggx_visible_normal(v, m.alphaX, m.alphaY, rx, ry);
+      if (h.z < 0.0f) h = -h;
+      l = normalize(reflect(-v, h));
+    }
+  } else if (lobe < cdf[L_TRANS]) {
+    transmission = true;
+    float msDraw = (lobe - cdf[L_SPEC]) / mxf(P.p[L_TRANS], 1e-12f);
+    if ((P.dielMsProbability > 0.0f) & (msDraw < P.dielMsProbability)) {
+      float ratio = P.dielMsRatio;
+      bool reflectMs = rx < ratio;
+      float z = reflectMs ? mnf(rx / ratio, 0.99999994f) : -mnf((rx - ratio) / (1.0f - ratio), 0.99999994f);
+      float radial = std::sqrt(mxf(0.0f, 1.0f - z * z));
+      float az = TWOPI * ry;
+      l = V3(radial * std::cos(az), radial * std::sin(az), z);
+      transmission = !reflectMs;
+    } else {
+      V3 h = ggx_visible_normal(v, m.alphaX, m.alphaY, rx, ry);
+      if (h.z < 0.0f) h = -h;
+      l = refract(-v, h, m.relativeIor);
+      if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
+      l = normalize(l);
+    }
+  } else {
+    transmission = true;
+    l = cosine_hemisphere(rx, ry);
+    l.z = -l.z;
+  }
+  lightDir = to_world(P.basis, l);
+  if ((l.z == 0.0f) | ((l.z < 0.0f) != transmission)) return false;
+  return true;
+}
+
+// ------------------------------------------------------------------ lights
+struct LightSample {
+  V3 normal, emission, direction;
+  float dist, pdf, selectionPmf;
+  bool isDelta;
+  uint32_t sourceType;
+};
+
+struct HostTables {
+  // alias table entries: probability, alias, type (0 env, 1 analytic), index
+  std::vector<float> prob;
+  std::vector<int32_t> alias, type, index;
+  std::vector<float> lightPmf;  // per analytic light
+  float envPmf = 0.0f;
+  double totalWeight = 0.0;
+  std::vector<float> cdf;       // (W+2) x (H+1) x 2
+  uint32_t cdfW = 0, cdfH = 0;
+};
+
+// lightSampling.ts:172-212 (Walker alias table) over {env, analytic lights}.
+static void build_tables(const RenderScene &s, HostTables &T) {
+  std::vector<int32_t> types, idxs;
+  std::vector<double> weights;
+  auto push = [&](int32_t ty, int32_t ix, double w) {
+    if (!(w > 0) || !std::isfinite(w)) return;
+    types.push_back(ty); idxs.push_back(ix); weights.push_back(w);
+  };
+  double envPower = 0.0;
+  if (s.env) {  // lightSampling.ts:415-431
+    double acc = 0.0;
+    for (uint32_t y = 0; y < s.envH; y++) {
+      double sinT = std::sin(3.141592653589793 * (y + 0.5) / s.envH);
+      for (uint32_t x = 0; x < s.envW; x++) {
+        const float *p = s.env + ((size_t)y * s.envW + x) * 4;
+        double l = (double)p[0] * 0.2126 + (double)p[1] * 0.7152 + (double)p[2] * 0.0722;
+        acc += (l > 0 ? l : 0) * sinT;
+      }
+    }
+    envPower = acc * (2 * 3.141592653589793 * 3.141592653589793) / ((double)s.envW * s.envH);
+  }
+  push(0, 0, envPower);
+  for (uint32_t i = 0; i < s.lightCount; i++) {  // lightSampling.ts:72-77 (RectLight)
+    const float *L = s.lights + (size_t)i * 16;
+    double lum = (double)L[L_EMISSION] * 0.2126 + (double)L[L_EMISSION + 1] * 0.7152 + (double)L[L_EMISSION + 2] * 0.0722;
+    lum = lum > 0 ? lum : 0;
+    push(1, (int32_t)i, lum > 0 ? lum * 3.141592653589793 * std::max((double)L[L_AREA], 0.0) : 0.0);
+  }
+  size_t n = weights.size();
+  T.lightPmf.assign(s.lightCount, 0.0f);
+  double total = 0.0;
+  for (double w : weights) total += w;
+  T.totalWeight = n && total > 0 ? total : 0.0;
+  T.prob.assign(std::max<size_t>(n, 1), 0.0f);
+  T.alias.assign(std::max<size_t>(n, 1), 0);
+  T.type.assign(std::max<size_t>(n, 1), 0);
+  T.index.assign(std::max<size_t>(n, 1), 0);
+  if (!n || !(total > 0)) { T.prob.clear(); return; }
+  std::vector<float> pmf(n);
+  std::vector<double> scaled(n);
+  std::vector<uint32_t> small, large;
+  for (size_t i = 0; i < n; i++) {
+    pmf[i] = (float)(weights[i] / total);
+    scaled[i] = (double)pmf[i] * (double)n;
+    if (scaled[i] < 1) small.push_back((uint32_t)i); else large.push_back((uint32_t)i);
+  }
+  while (!small.empty() && !large.empty()) {
+    uint32_t low = small.back(); small.pop_back();
+    uint32_t high = large.back(); large.pop_back();
+    T.prob[low] = (float)scaled[low];
+    T.alias[low] = (int32_t)high;
+    scaled[high] = scaled[high] + scaled[low] - 1;
+    if (scaled[high] < 1) small.push_back(high); else large.push_back(high);
+  }
+  for (uint32_t i : small) { T.prob[i] = 1.0f; T.alias[i] = (int32_t)i; }
+  for (uint32_t i : large) { T.prob[i] = 1.0f; T.alias[i] = (int32_t)i; }
+  for (size_t i = 0; i < n; i++) {
+    T.type[i] = types[i];
+    T.index[i] = idxs[i];
+    if (types[i] == 0) T.envPmf = pmf[i];
+    else T.lightPmf[idxs[i]] = pmf[i];
+  }
+  // Small/large lists pop from the back in the reference (small[--smallCount]).
+}
+
+// envMapDistribution (envMapDistribution.ts:57-134).
+static void build_env_cdf(const RenderScene &s, HostTables &T) {
+  if (!s.env) return;
+  const uint32_t W = s.envW, H = s.envH, CW = W + 2, CH = H + 1;
+  T.cdfW = CW; T.cdfH = CH;
+  T.cdf.assign((size_t)CW * CH * 2, 0.0f);
+  auto at = [&](uint32_t x, uint32_t y, int c) -> float & { return T.cdf[((size_t)y * CW + x) * 2 + c]; };
+  std::vector<float> lum((size_t)W * H), horiz((size_t)W * H);
+  for (size_t i = 0; i < (size_t)W * H; i++) {
+    const float *p = s.env + i * 4;
+    lum[i] = (float)(1.0 * (0.2126 * p[0] + 0.7152 * p[1] + 0.0722 * p[2]));
+  }
+  for (uint32_t y = 0; y < H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      size_t row = (size_t)y * W;
+      double a = lum[row + (x > 0 ? x - 1 : 0)], b = lum[row + x], c = lum[row + std::min(W - 1, x + 1)];
+      horiz[row + x] = (float)std::max(std::max(0.0, a), std::max(b, c));
+    }
+  for (uint32_t y = 0; y < H; y++) {
+    double sinT = std::sin(3.141592653589793 * (y + 0.5) / H);
+    size_t above = (size_t)(y > 0 ? y - 1 : 0) * W, row = (size_t)y * W, below = (size_t)std::min(H - 1, y + 1) * W;
+    for (uint32_t x = 0; x < W; x++) {
+      double w = std::max(std::max((double)horiz[above + x], (double)horiz[row + x]),
+                          std::max((double)horiz[below + x], 1e-12)) * sinT;
+      at(x + 2, y, 0) = (float)((double)at(x + 1, y, 0) + w / W);
+      at(x + 1, y, 1) = (float)w;
+    }
+    double rowIntegral = at(CW - 1, y, 0);
+    for (uint32_t x = 1; x < CW; x++) {
+      at(x, y, 0) = (float)((double)at(x, y, 0) / rowIntegral);
+      at(x, y, 1) = (float)((double)at(x, y, 1) / rowIntegral);
+    }
+    at(0, y + 1, 0) = (float)((double)at(0, y, 0) + rowIntegral / H);
+    at(0, y, 1) = (float)rowIntegral;
+  }
+  double integral = at(0, CH - 1, 0);
+  for (uint32_t y = 0; y < CH; y++) {
+    at(0, y, 0) = (float)((double)at(0, y, 0) / integral);
+    at(0, y, 1) = (float)((double)at(0, y, 1) / integral);
+  }
+}
+
+struct Ctx {
+  const RenderScene &s;
+  const HostTables &T;
+  uint64_t rays = 0;
+};
+
+static V3 env_texel(const RenderScene &s, int x, int y) {
+  const float *p = s.env + ((size_t)y * s.envW + x) * 4;
+  return V3(p[0], p[1], p[2]);
+}
+// textureLinear (envMap.wgsl.ts:10-30), rgb only.
+static V3 texture_linear(const RenderScene &s, float u, float v) {
+  float sx = (float)s.envW, sy = (float)s.envH;
+  float fx = u * sx - 0.5f, fy = v * sy - 0.5f;
+  float flx = floorf(fx), fly = floorf(fy);
+  float tx = fx - flx, ty = fy - fly;
+  int ix = (int)flx, iy = (int)fly;
+  int mx = (int)s.envW - 1, my = (int)s.envH - 1;
+  auto cl = [](int a, int hi) { return a < 0 ? 0 : (a > hi ? hi : a); };
+  V3 s1 = env_texel(s, cl(ix, mx), cl(iy, my)), s2 = env_texel(s, cl(ix + 1, mx), cl(iy, my));
+  V3 s3 = env_texel(s, cl(ix, mx), cl(iy + 1, my)), s4 = env_texel(s, cl(ix + 1, mx), cl(iy + 1, my));
+  return mix3(mix3(s1, s2, tx), mix3(s3, s4, tx), ty);
+}
+static void environment_uv(V3 d, float &u, float &v) {  // cartesianToEquirect, identity rotation
+  float phi = std::atan2(-d.z, -d.x);
+  if (phi < 0.0f) phi += TWOPI;
+  float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+  u = phi * 0.5f * INVPI;
+  v = theta * INVPI;
+}
+static float cdfx(const HostTables &T, int x, int y, int c) {
+  x = std::max(0, std::min(x, (int)T.cdfW - 1));
+  y = std::max(0, std::min(y, (int)T.cdfH - 1));
+  return T.cdf[((size_t)y * T.cdfW + x) * 2 + c];
+}
+static float env_solid_angle_pdf(float px, float py, float v) {
+  if (!(v > 0.0f && v < 1.0f)) return 0.0f;
+  float sinT = std::sin(mnf(v, 1.0f - v) * PI);
+  return px * py * INVPI2 / (2.0f * sinT);
+}
+static float env_map_pdf(const Ctx &c, float u, float v) {
+  float sx = (float)c.s.envW, sy = (float)c.s.envH;
+  int x = (int)(u * sx), y = (int)(v * sy);
+  x = std::max(0, std::min(x, (int)c.s.envW - 1));
+  y = std::max(0, std::min(y, (int)c.s.envH - 1));
+  return env_solid_angle_pdf(cdfx(c.T, 1 + x, y, 1), cdfx(c.T, 0, y, 1), v);
+}
+static V3 sample_envmap(const Ctx &c, float r0, float r1, float &u, float &v, float &pdf) {
+  int H = (int)c.s.envH, W = (int)c.s.envW;
+  int left = 0, right = H + 1;
+  while (left < right) {
+    int mid = (left + right) >> 1;
+    if (cdfx(c.T, 0, mid, 0) <= r0) left = mid + 1; else right = mid;
+  }
+  int vOff = left - 1;
+  float a0 = cdfx(c.T, 0, vOff, 0), a1 = cdfx(c.T, 0, vOff + 1, 0);
+  float py = cdfx(c.T, 0, vOff, 1);
+  v = ((float)vOff + (r0 - a0) / (a1 - a0)) / (float)H;
+  left = 0; right = W + 1;
+  while (left < right) {
+    int mid = (left + right) >> 1;
+    if (cdfx(c.T, 1 + mid, vOff, 0) <= r1) left = mid + 1; else right = mid;
+  }
+  int uOff = left - 1;
+  float b0 = cdfx(c.T, 1 + uOff, vOff, 0), b1 = cdfx(c.T, 1 + uOff + 1, vOff, 0);
+  float px = cdfx(c.T, 1 + uOff, vOff, 1);
+  u = ((float)uOff + (r1 - b0) / (b1 - b0)) / (float)W;
+  float phi = u * TWOPI, theta = v * PI;
+  float ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+  pdf = env_solid_angle_pdf(px, py, v);
+  return V3(-st * cp, ct, -st * sp);
+}
+
+static void light_sample_rect(const float *L, V3 p, float sx, float sy, LightSample &r) {
+  V3 U(L[L_U], L[L_U + 1], L[L_U + 2]), Vv(L[L_V], L[L_V + 1], L[L_V + 2]);
+  V3 pos(L[L_POS], L[L_POS + 1], L[L_POS + 2]);
+  V3 n = normalize(cross(U, Vv));
+  V3 lowerLeft = pos - 0.5f * (U + Vv);
+  V3 surf = lowerLeft + U * sx + Vv * sy;
+  V3 off = surf - p;
+  float d2 = dot(off, off);
+  r.dist = std::sqrt(d2);
+  r.direction = off / r.dist;
+  r.normal = n;
+  float ec = dot(-r.direction, n);
+  float spread = clampf(L[L_SPREAD], 0.0f, 1.0f);
+  float sc = std::cos(0.5f * PI * spread);
+  if (ec <= 0.0f || ec < sc) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
+  r.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+  r.pdf = d2 / mxf(L[L_AREA] * ec, 1e-8f);
+}
+
+// sampleLightChoiceTable (sampleLight.wgsl.ts:365-420) with the candidate's
+// stratified selection number (:298-308).
+static void sample_light_candidate(const Ctx &c, V3 p, float rx, float ry, float rz, uint32_t cand,
+                                   uint32_t count, LightSample &ls) {
+  uint32_t cnt = std::max(count, 1u);
+  rz = (rz + (float)std::min(cand, cnt - 1u)) / (float)cnt;
+  ls.emission = V3(0.0f); ls.pdf = 0.0f; ls.selectionPmf = 0.0f; ls.dist = INF; ls.isDelta = false; ls.sourceType = 0;
+  int choiceCount = (int)c.T.prob.size();
+  if (choiceCount <= 0) return;
+  float scaled = rz * (float)choiceCount;
+  int column = std::min((int)scaled, choiceCount - 1);
+  int sel = fractf(scaled) < c.T.prob[column] ? column : c.T.alias[column];
+  if (c.T.type[sel] == 0 && c.s.env) {
+    ls.sourceType = 0;
+    float u, v, pdf;
+    ls.direction = sample_envmap(c, rx, ry, u, v, pdf);
+    ls.pdf = pdf;
+    ls.emission = texture_linear(c.s, u, v);
+    ls.dist = INF;
+    ls.selectionPmf = c.T.envPmf;
+    ls.pdf *= c.T.envPmf;
+  } else if (c.T.type[sel] == 1) {
+    ls.sourceType = 2;
+    light_sample_rect(c.s.lights + (size_t)c.T.index[sel] * 16, p, rx, ry, ls);
+    ls.selectionPmf = c.T.lightPmf[c.T.index[sel]];
+    ls.pdf *= ls.selectionPmf;
+  }
+}
+
+// intersectLight for a rect (intersect.wgsl.ts:976-1021).
+static float intersect_rect_light(const float *L, V3 ro, V3 rd, LightSample &ls) {
+  ls.direction = rd; ls.normal = -rd; ls.dist = INF; ls.pdf = 0.0f; ls.emission = V3(0.0f); ls.isDelta = false;
+  V3 U(L[L_U], L[L_U + 1], L[L_U + 2]), Vv(L[L_V], L[L_V + 1], L[L_V + 2]);
+  V3 pos = V3(L[L_POS], L[L_POS + 1], L[L_POS + 2]) - U * 0.5f - Vv * 0.5f;
+  V3 n = normalize(cross(U, Vv));
+  if (dot(n, rd) > 0.0f) return INF;
+  float dt = dot(rd, n);
+  float t = (dot(n, pos) - dot(n, ro)) / dt;
+  float d = INF;
+  if (t > EPS) {
+    V3 p = ro + rd * t, vi = p - pos;
+    float a1 = dot(U / dot(U, U), vi);
+    if (a1 >= 0.0f && a1 <= 1.0f) {
+      float a2 = dot(Vv / dot(Vv, Vv), vi);
+      if (a2 >= 0.0f && a2 <= 1.0f) d = t;
+    }
+  }
+  if (d < 0.0f) return INF;
+  if (d == INF) return INF;
+  float cosT = dot(-rd, n);
+  if (cosT >= std::cos(L[L_SPREAD] * PI / 2.0f)) {
+    ls.direction = rd; ls.normal = n; ls.dist = d;
+    ls.pdf = (d * d) / (L[L_AREA] * cosT);
+    ls.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+    return d;
+  }
+  return INF;
+}
+
+// offsetSurfaceRay (common.wgsl.ts:14-19).
+static V3 offset_surface_ray(V3 p, V3 gn, V3 dir) {
+  V3 n = dot(gn, dir) >= 0.0f ? gn : -gn;
+  V3 out;
+  for (int k = 0; k < 3; k++) {
+    int32_t io = (int32_t)(256.0f * n[k]);
+    int32_t bits = f2i(p[k]);
+    float shifted = i2f(bits + (p[k] < 0.0f ? -io : io));
+    out[k] = fabsf(p[k]) < (float)(1.0 / 32.0) ? p[k] + n[k] / 65536.0f : shifted;
+  }
+  return out;
+}
+
+// ------------------------------------------------------------------ integrator
+struct Surface {
+  bool hit;
+  V3 position, normal, faceNormal, shadowOffset;
+  int32_t triangleIndex, instanceIndex, materialIndex;
+  bool albedoSampled;
+};
+struct PathState {
+  V3 ro, rd, li, beta;
+  float alpha, scatteringPdf;
+  bool fromEye, abort, scatteringIsDelta, isNextRefraction;
+};
+struct Aux { V3 albedo, normal; bool isBlank; };
+
+static inline V3 instance_color(const TraceScene &t, int32_t ins) {
+  const uint32_t *r = &t.instances[(size_t)ins * 40];
+  return V3(u2f(r[36]), u2f(r[37]), u2f(r[38]));
+}
+
+// intersectScene + completeSurfaceInteraction/fetchAttributes (intersect.wgsl.ts:1233-1601).
+static void intersect_scene(Ctx &c, const PathState &path, Surface &si, int bounces, bool doubleSided) {
+  c.rays++;
+  ClosestHit h = trace_closest(*c.s.trace, path.ro, path.rd, RAY_MAX_DISTANCE, doubleSided, bounces, si.triangleIndex,
+                               si.instanceIndex, nullptr);
+  si.hit = false;
+  if (!h.hit) return;
+  const TraceScene &t = *c.s.trace;
+  const uint32_t *ins = &t.instances[(size_t)h.instance * 40];
+  int32_t vo = (int32_t)ins[1];
+  size_t r3 = (size_t)h.record * 4;
+  int32_t i0 = (int32_t)t.blas[r3] + vo, i1 = (int32_t)t.blas[r3 + 1] + vo, i2 = (int32_t)t.blas[r3 + 2] + vo;
+  auto P = [&](int32_t i) { return V3(t.positions[(size_t)i * 3], t.positions[(size_t)i * 3 + 1], t.positions[(size_t)i * 3 + 2]); };
+  auto N = [&](int32_t i) { return V3(c.s.normals[(size_t)i * 3], c.s.normals[(size_t)i * 3 + 1], c.s.normals[(size_t)i * 3 + 2]); };
+  V3 p0 = P(i0), p1 = P(i1), p2 = P(i2), n0 = N(i0), n1 = N(i1), n2 = N(i2);
+  V3 b = h.bary;
+  si.hit = true;
+  si.triangleIndex = h.record;
+  si.instanceIndex = h.instance;
+  si.materialIndex = ((int32_t)ins[2]) >> 8;
+  V3 position = h.position;
+  V3 normal = normalize(b.x * n0 + b.y * n1 + b.z * n2);
+  // Shadow-terminator offset (intersect.wgsl.ts:1376-1407), object space.
+  V3 geometric = normalize(cross(p1 - p0, p2 - p0));
+  V3 smooth = b.x * n0 + b.y * n1 + b.z * n2;
+  float A = dot(n2 - n0, p0 - p2), Bv = dot(n2 - n1, p1 - p2), Cv = dot(n1 - n0, p1 - p0);
+  float hh = A * b.x * (b.x - 1.0f) + (A + Bv + Cv) * b.x * b.y + Bv * b.y * (b.y - 1.0f);
+  if (dot(smooth, geometric) > 0.0f) {
+    float h0 = mxf(mxf(dot(p1 - p0, n0), dot(p2 - p0, n0)), 0.0f);
+    float h1 = mxf(mxf(dot(p0 - p1, n1), dot(p2 - p1, n1)), 0.0f);
+    float h2 = mxf(mxf(dot(p0 - p2, n2), dot(p1 - p2, n2)), 0.0f);
+    h0 = mxf(dot(p0 - position, n0) + h0, 0.0f);
+    h1 = mxf(dot(p1 - position, n1) + h1, 0.0f);
+    h2 = mxf(dot(p2 - position, n2) + h2, 0.0f);
+    hh = mxf(mnf(mnf(h0, h1), h2), hh * 0.5f);
+  } else {
+    float h0 = mxf(mxf(dot(p0 - p1, n0), dot(p0 - p2, n0)), 0.0f);
+    float h1 = mxf(mxf(dot(p1 - p0, n1), dot(p1 - p2, n1)), 0.0f);
+    float h2 = mxf(mxf(dot(p2 - p0, n2), dot(p2 - p1, n2)), 0.0f);
+    h0 = mxf(dot(position - p0, n0) + h0, 0.0f);
+    h1 = mxf(dot(position - p1, n1) + h1, 0.0f);
+    h2 = mxf(dot(position - p2, n2) + h2, 0.0f);
+    hh = mnf(-mnf(mnf(h0, h1), h2), hh * 0.5f);
+  }
+  V3 shadowOffset = smooth * hh;
+  float l2w[16], w2l[16];
+  for (int k = 0; k < 16; k++) { w2l[k] = u2f(ins[4 + k]); l2w[k] = u2f(ins[20 + k]); }
+  auto nt = [&](V3 n) {  // transpose(mat3(w2l)) * n
+    return V3(dot(V3(w2l[0], w2l[1], w2l[2]), n), dot(V3(w2l[4], w2l[5], w2l[6]), n), dot(V3(w2l[8], w2l[9], w2l[10]), n));
+  };
+  si.faceNormal = normalize(nt(h.normal));
+  si.normal = normalize(nt(normal));
+  si.shadowOffset = mat_mul_point(l2w, shadowOffset, 0.0f);
+  si.position = mat_mul_point(l2w, position, 1.0f);
+}
+
+static float nee_mis(uint32_t count, float lightPdf, float scatPdf) {
+  if (count <= 1) return power_heuristic(lightPdf, scatPdf);
+  float w = (float)count * lightPdf;
+  return w / mxf(w + scatPdf, 1e-20f);
+}
+static float bsdf_mis(uint32_t count, float scatPdf, float lightPdf) {
+  if (count <= 1) return power_heuristic(scatPdf, lightPdf);
+  float w = (float)count * lightPdf;
+  return scatPdf / mxf(scatPdf + w, 1e-20f);
+}
+
+struct ShadowWork {
+  V3 ro, rd, contribution;
+  float maxDistance, risTarget, lightSelectionPmf;
+  int32_t triangleIndex;
+  bool valid;
+};
+
+// finishDirectLight (sampleMaterial.wgsl.ts:32-151), surface vertex.
+static bool finish_direct_light(const Ctx &c, const Surface &si, const PathState &path, V3 ff, const LightSample &ls,
+                                V3 f, float scatPdf, ShadowWork &w) {
+  w.valid = false; w.lightSelectionPmf = 0.0f; w.risTarget = 0.0f;
+  V3 Li = ls.emission;
+  V3 lightDir = ls.direction;
+  float lightPdf = ls.pdf, selPmf = ls.selectionPmf;
+  float cosThetaL = dot(ff, lightDir);
+  if ((lightPdf <= 0.0f) | (selPmf <= 0.0f) | (maxc(Li) <= 0.0f)) return false;
+  if ((scatPdf <= 0.0f) | (maxc(f) <= 0.0f)) return false;
+  V3 origin = offset_surface_ray(si.position, si.faceNormal, lightDir);
+  const float cutoff = 0.1f;
+  float nl = dot(si.normal, lightDir);
+  V3 gn = nl < 0.0f ? -si.faceNormal : si.faceNormal;
+  nl = fabsf(nl);
+  float gl = dot(gn, lightDir);
+  float amount = nl < cutoff ? clampf(2.0f - (gl + nl) / cutoff, 0.0f, 1.0f) : clampf(1.0f - gl / cutoff, 0.0f, 1.0f);
+  origin = origin + si.shadowOffset * amount;
+  float visDist = ls.dist - EPS;
+  if ((cosThetaL < 0.0f) & (ls.sourceType == 0u)) {
+    float u, v;
+    environment_uv(lightDir, u, v);
+    Li = c.s.env ? texture_linear(c.s, u, v) : V3(0.0f);
+  }
+  float mis = ls.isDelta ? 1.0f : nee_mis(c.s.candidates, lightPdf, scatPdf);
+  if (mis > 0.0f) {
+    V3 un = Li * f / lightPdf * path.beta;
+    w.risTarget = mxf(dot(vmax(un * selPmf, V3(0.0f)), K_LUM), 0.0f);
+    w.ro = origin; w.rd = lightDir; w.maxDistance = visDist; w.triangleIndex = si.triangleIndex;
+    w.contribution = mis * un;
+    w.lightSelectionPmf = selPmf;
+    w.valid = true;
+    return true;
+  }
+  return false;
+}
+
+struct Scatter { V3 direction, response; float pdf; bool transmission; };
+
+// sampleScattering (sampleMaterial.wgsl.ts:160-277), surface vertex.
+static Scatter sample_scattering(Ctx &c, Sampler &smp, const Surface &si, const Mat &ma, const Prepared &P,
+                                 PathState &path, V3 ff, bool &discreteOut) {
+  uint32_t count = std::max(c.s.candidates, 1u);
+  ShadowWork sel{};
+  sel.valid = false;
+  float selTarget = 0.0f, wsum = 0.0f;
+  Scatter res{};
+  res.pdf = 0.0f;
+  discreteOut = false;
+  for (uint32_t ev = 0; ev <= count; ev++) {
+    bool cont = ev == count;
+    LightSample ls{};
+    V3 lightDir;
+    float scatPdf = 0.0f;
+    V3 resp(0.0f);
+    bool discrete = false;
+    if (cont) {
+      float ucw = 1.0f;
+      bool valid = sel.valid;
+      if (count > 1u) {
+        valid = valid & (selTarget > 0.0f);
+        if (valid) ucw = sel.lightSelectionPmf * wsum / ((float)count * selTarget);
+      }
+      if (valid) {
+        c.rays++;
+        if (!trace_any(*c.s.trace, sel.ro, sel.rd, sel.maxDistance, sel.triangleIndex, si.instanceIndex, nullptr))
+          path.li = path.li + sel.contribution * ucw;
+      }
+      float d2[2];
+      smp.sample2d(smp.bounce_dim(BSDF_DIRECTION), d2);
+      float lobe = smp.sample1d(smp.bounce_dim(BSDF_LOBE));
+      discrete = (fabsf(P.material.relativeIor - 1.0f) <= 1e-6f) & (P.p[L_TRANS] >= 1.0f - 1e-6f);
+      if (discrete) {
+        lightDir = to_world(P.basis, -P.viewLocal);
+        resp = P.material.transmissionColor * coat_base_scale(P, -P.viewLocal.z);
+        scatPdf = 1.0f;
+        res.transmission = true;
+      } else {
+        bool tr;
+        if (!sample_direction(P, d2[0], d2[1], lobe, lightDir, tr)) return res;
+        res.transmission = tr;
+      }
+      discreteOut = discrete;
+    } else {
+      float l2[2];
+      smp.sample2d(smp.light_dim(ev, 1u), l2);
+      float l1 = smp.sample1d(smp.light_dim(ev, 0u));
+      sample_light_candidate(c, si.position, l2[0], l2[1], l1, ev, count, ls);
+      lightDir = ls.direction;
+      if ((ls.pdf <= 0.0f) | (ls.selectionPmf <= 0.0f) | (maxc(ls.emission) <= 0.0f)) continue;
+      float cosThetaL = dot(ff, lightDir);
+      if ((cosThetaL <= 0.0f) & (ma.transmissionWeight <= 0.0f)) continue;
+    }
+    if (!discrete) resp = eval_prepared(P, lightDir, scatPdf);
+    if (cont) {
+      res.direction = lightDir;
+      res.response = resp;
+      res.pdf = scatPdf;
+      return res;
+    }
+    ShadowWork cand{};
+    if (!finish_direct_light(c, si, path, ff, ls, resp, scatPdf, cand)) continue;
+    if (count == 1u) { sel = cand; continue; }
+    float sw = cand.risTarget / mxf(cand.lightSelectionPmf, 1e-20f);
+    if (!(sw > 0.0f) | (sw != sw)) continue;
+    wsum += sw;
+    if (!sel.valid || smp.light_candidate_random(ev) * wsum <= sw) {
+      sel = cand;
+      selTarget = cand.risTarget;
+    }
+  }
+  return res;
+}
+
+// SampleMaterial (sampleMaterial.wgsl.ts:316-508) without media.
+static void sample_material(Ctx &c, Sampler &smp, Surface &si, Mat &ma, PathState &path) {
+  float ndv1 = -dot(si.faceNormal, path.rd);
+  float ndv2 = -dot(si.normal, path.rd);
+  bool useFaceN = (ndv1 >= 0.0f) & (ndv2 < 0.0f);
+  float ndv = useFaceN ? ndv1 : ndv2;
+  V3 N = useFaceN ? si.faceNormal : si.normal;
+  ma.inside = ndv <= 0.0f;
+  ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
+  V3 ff = ndv >= 0.0f ? N : -N;
+  path.scatteringIsDelta = false;
+  // Surface emission: the configurations have no emissive materials.
+  Prepared P = prepare(ma, -path.rd, ff);
+  bool discrete = false;
+  Scatter sc = sample_scattering(c, smp, si, ma, P, path, ff, discrete);
+  V3 f = sc.response;
+  bool finite = std::isfinite(f.x) && std::isfinite(f.y) && std::isfinite(f.z) && fabsf(f.x) < 3.402823e38f &&
+                fabsf(f.y) < 3.402823e38f && fabsf(f.z) < 3.402823e38f;
+  if (!(sc.pdf > 0.0f) | !finite | (dot(sc.direction, sc.direction) < 0.5f)) {
+    path.scatteringPdf = 0.0f;
+    path.abort = true;
+    return;
+  }
+  path.scatteringIsDelta = discrete;
+  path.beta = path.beta * (f / sc.pdf);
+  if (path.beta.x == 0.0f && path.beta.y == 0.0f && path.beta.z == 0.0f) { path.abort = true; return; }
+  path.isNextRefraction = sc.transmission;
+  path.ro = si.position + EPS * sc.direction;
+  path.scatteringPdf = sc.pdf;
+  path.rd = sc.direction;
+  path.fromEye = false;
+}
+
+// camera (rayTrace.compute.ts:591-611, browser projectionInv variant).
+static void camera_ray(const float *cam, float cx, float cy, V3 &o, V3 &d) {
+  const float *projInv = cam, *viewInv = cam + 16;
+  float ndcX = cx * 2.0f - 1.0f, ndcY = cy * 2.0f - 1.0f;
+  float a[4] = {ndcX, ndcY, -1.0f, 1.0f}, b[4] = {ndcX, ndcY, 1.0f, 1.0f}, og[4], ce[4];
+  mat_mul4(projInv, a, og);
+  mat_mul4(projInv, b, ce);
+  V3 dir(ce[0] * og[3] - og[0] * ce[3], ce[1] * og[3] - og[1] * ce[3], ce[2] * og[3] - og[2] * ce[3]);
+  V3 op(og[0] / og[3], og[1] / og[3], og[2] / og[3]);
+  o = mat_mul_point(viewInv, op, 1.0f);
+  d = normalize(mat_mul_point(viewInv, dir, 0.0f));
+}
+
+static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleIndex, Accum &acc) {
+  const RenderScene &s = c.s;
+  Sampler smp{};
+  smp.px = px; smp.py = py; smp.width = s.width;
+  smp.pixelSampleIndex = sampleIndex + s.sampleOffset;
+  smp.sequenceSeed = s.seed; smp.zSobolLogSPP = s.logSPP; smp.pattern = s.pattern; smp.bevelSamples = 0;
+  smp.samplingBounce = 0; smp.hashedCounter = 0;
+  float W = (float)s.width, H = (float)s.height;
+  float uvx = ((float)px + 0.5f) / W, uvy = ((float)py + 0.5f) / H;
+  float j[2];
+  smp.sample2d(CAMERA_PIXEL, j);
+  float jx = (j[0] - 0.5f) / W, jy = (j[1] - 0.5f) / H;
+  float cx = uvx + jx, cy = uvy + jy;
+  PathState path{};
+  camera_ray(s.camera, cx, cy, path.ro, path.rd);
+  Aux aux{V3(0.0f), V3(0.0f), false};
+  // integrator (rayTrace.compute.ts:891-946)
+  Surface si{};
+  si.triangleIndex = 0; si.instanceIndex = 0; si.materialIndex = 0; si.albedoSampled = false;
+  Mat ma{};
+  path.alpha = 1.0f;
+  path.beta = V3(1.0f);
+  path.fromEye = true;
+  int bounceLimit = (int)s.bounces;
+  int eventLimit = std::min(bounceLimit * 3, 50);
+  bool isNotTransparent = false;
+  int depth = 0;
+  for (int event = 0; event <= eventLimit; event++) {
+    intersect_scene(c, path, si, depth, event != 0);
+    if (si.hit) {
+      int32_t mi = si.materialIndex;
+      ma = get_material(s.materials + (size_t)mi * 32, instance_color(*s.trace, si.instanceIndex));
+    }
+    if (event == 0) {
+      if (si.hit) {
+        aux.albedo = ma.transmissionWeight > 0.5f ? V3(1.0f) : ma.baseColor;
+        V3 camPos(s.camera[16 + 12], s.camera[16 + 13], s.camera[16 + 14]);
+        aux.normal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
+        si.albedoSampled = ma.transmissionWeight > 0.5f;
+      }
+      isNotTransparent = ma.transmissionWeight < 0.001f && ma.alpha > 0.99f;
+    } else {
+      // completeVertex (rayTrace.compute.ts:514-589), opaque alpha.
+      for (uint32_t i = 0; i < s.lightCount; i++) {
+        LightSample ls{};
+        const float *L = s.lights + (size_t)i * 16;
+        float d = intersect_rect_light(L, path.ro, path.rd, ls);
+        ls.pdf *= c.T.lightPmf[i];
+        V3 dd = si.position - path.ro;
+        bool occluded = si.hit && d > length(dd);
+        if (d != INF && !occluded) {
+          float mis = bsdf_mis(s.candidates, path.scatteringPdf, ls.pdf);
+          path.li = path.li + mis * path.beta * ls.emission;
+          path.alpha = 1.0f;
+          path.abort = true;
+        }
+      }
+      if (s.env && !si.hit) {
+        float u, v;
+        environment_uv(path.rd, u, v);
+        V3 em = texture_linear(s, u, v);
+        if (path.isNextRefraction) {  // displayed background = environment here
+          float mis = path.scatteringIsDelta ? 1.0f
+                                             : bsdf_mis(s.candidates, path.scatteringPdf, env_map_pdf(c, u, v) * c.T.envPmf);
+          path.li = path.li + mis * path.beta * em * 1.0f;
+        } else {
+          float mis = bsdf_mis(s.candidates, path.scatteringPdf, env_map_pdf(c, u, v) * c.T.envPmf);
+          path.li = path.li + path.beta * em * mis;
+        }
+        path.alpha = 1.0f;
+        path.abort = true;
+      }
+      depth++;
+    }
+    if (isNotTransparent && event > bounceLimit) path.abort = true;
+    if (path.abort || event >= eventLimit) break;
+    // shadeVertex (rayTrace.compute.ts:343-512)
+    smp.samplingBounce = (uint32_t)std::max(depth, 0);
+    if (!si.hit) {
+      if (path.fromEye) {
+        V3 bg(0.0f);
+        float ba = 0.0f;
+        if (s.env) {
+          float u, v;
+          environment_uv(path.rd, u, v);
+          bg = texture_linear(s, u, v);
+          ba = 1.0f;
+        }
+        path.li = path.beta * bg * ba;
+        path.alpha = ba;
+        aux.isBlank = ba < 0.001f;
+      }
+      path.abort = true;
+      break;
+    }
+    sample_material(c, smp, si, ma, path);
+    if (!si.albedoSampled) {
+      if (!(ma.transmissionWeight > 0.5f || (ma.specularRoughness <= 0.02f && ma.baseMetalness > 0.98f))) {
+        aux.albedo = ma.baseColor;
+        V3 camPos(s.camera[16 + 12], s.camera[16 + 13], s.camera[16 + 14]);
+        aux.normal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
+        si.albedoSampled = true;
+      }
+      if (depth > 4) si.albedoSampled = true;
+    }
+    if (depth >= 2) {
+      float q = mxf(0.0f, 1.0f - maxc(path.beta));
+      if (smp.sample1d(smp.bounce_dim(RUSSIAN_ROULETTE)) < q) {
+        path.alpha = 1.0f;
+        path.abort = true;
+      } else {
+        path.beta = path.beta / (1.0f - q);
+      }
+    }
+    if (path.abort) break;
+  }
+  V3 li = path.li;
+  float alpha = path.alpha;
+  if (s.clamping > 0.0f) li = clamp3(li, -s.clamping, s.clamping);
+  if (!(li.x < INF && li.x > -EPS)) { li = V3(0.0f); alpha = 1.0f; }
+  acc.color[0] += li.x; acc.color[1] += li.y; acc.color[2] += li.z; acc.color[3] += alpha;
+  acc.albedo[0] += aux.albedo.x; acc.albedo[1] += aux.albedo.y; acc.albedo[2] += aux.albedo.z;
+  acc.albedo[3] += aux.isBlank ? 0.0 : 1.0;
+  acc.normal[0] += aux.normal.x; acc.normal[1] += aux.normal.y; acc.normal[2] += aux.normal.z; acc.normal[3] += 1.0;
+  double lum = (double)dot(li, K_LUM);
+  acc.lum2 += lum * lum;
+}
+
+uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sampleStart,
+                uint32_t count, Accum *out) {
+  HostTables T;
+  build_tables(s, T);
+  build_env_cdf(s, T);
+  uint64_t rays = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : rays)
+  for (int64_t i = 0; i < (int64_t)w * h; i++) {
+    uint32_t px = x0 + (uint32_t)(i % w), py = y0 + (uint32_t)(i / w);
+    Ctx c{s, T, 0};
+    for (uint32_t k = 0; k < count; k++) integrate_pixel(c, px, py, sampleStart + k, out[i]);
+    rays += c.rays;
+  }
+  return rays;
+}
+
+}  // namespace orc

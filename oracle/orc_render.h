@@ -1,0 +1,54 @@
+// Oracle: RGB megakernel integrator (test infrastructure only).
+// Restates src/renderer/wgsl/rayTrace.compute.ts (integrator :891-946,
+// shadeVertex :343-512, completeVertex :514-589, main :1157-1263),
+// sampleMaterial.wgsl.ts, sampleLight.wgsl.ts, envMap.wgsl.ts,
+// openPbrBsdf.wgsl.ts (lobes used by the configurations: coat, metal with
+// F82 tint and multiple scattering, dielectric specular, solid transmission
+// with dielectric multiple scattering, EON diffuse) and the host tables of
+// lightSampling.ts / envMapDistribution.ts.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "orc_trace.h"
+
+namespace orc {
+
+// Packed OpenPBR material parameters (32 floats), shared input format.
+enum MatField {
+  M_BASE_WEIGHT = 0, M_BASE_COLOR = 1, M_DIFFUSE_ROUGHNESS = 4, M_METALNESS = 5,
+  M_SPEC_WEIGHT = 6, M_SPEC_COLOR = 7, M_SPEC_ROUGHNESS = 10, M_SPEC_IOR = 11,
+  M_TRANS_WEIGHT = 12, M_TRANS_COLOR = 13, M_COAT_WEIGHT = 16, M_COAT_COLOR = 17,
+  M_COAT_ROUGHNESS = 20, M_COAT_IOR = 21, M_COAT_DARKENING = 22, M_EMISSION = 23,
+  M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27
+};
+// Packed light (16 floats): type(0 rect), position, u, v, emission, spread, area.
+enum LightField { L_TYPE = 0, L_POS = 1, L_U = 4, L_V = 7, L_EMISSION = 10, L_SPREAD = 13, L_AREA = 14 };
+
+struct RenderScene {
+  const TraceScene *trace;
+  const float *normals;       // per global vertex xyz
+  const float *materials;     // 32 per material
+  uint32_t materialCount;
+  const float *lights;        // 16 per light
+  uint32_t lightCount;
+  const float *env;           // RGBA f32 env map (nullptr: none)
+  uint32_t envW, envH;
+  const float *camera;        // viewProjInv[16], viewInv[16], aperture, focus
+  uint32_t width, height;
+  uint32_t bounces, candidates, pattern, seed, logSPP, sampleOffset;
+  float clamping;
+};
+
+struct Accum {
+  double color[4], albedo[4], normal[4], lum2;
+};
+
+void set_energy_lut(const float *lut);  // 32 x 2177 floats
+// Renders samples [sampleStart, sampleStart+count) for the pixel rectangle
+// and adds them to out (one Accum per pixel of the rectangle, row-major).
+// Returns the number of rays traced (camera + continuation + shadow).
+uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                uint32_t sampleStart, uint32_t count, Accum *out);
+
+}  // namespace orc
