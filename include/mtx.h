@@ -261,6 +261,12 @@ int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings,
  * asynchronous. One megakernel launch per call. */
 int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream);
 
+/* mtx_target_render plus per-launch totals of the traversal work:
+ * work[0] box tests, [1] triangle tests, [2] instance visits, [3] closest
+ * hits, [4] shadow rays. Synchronous. No reference counterpart: it feeds the
+ * algorithmic-bytes figure of the roofline. */
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[5], void *stream);
+
 /* Blocks until the target's submitted work is done. */
 int32_t mtx_target_synchronize(mtx_handle target);
 

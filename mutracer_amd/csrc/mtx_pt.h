@@ -56,6 +56,7 @@ struct Accumulators {
   uint32_t *count;
   uint64_t *rays;  // traced rays (camera + continuation + shadow)
   uint32_t pixels;
+  uint64_t *work;  // optional: box tests, triangle tests, instance visits, hits, shadow rays
 };
 
 struct RenderSceneObj;

@@ -872,6 +872,9 @@ struct Shadow {
   bool valid;
 };
 
+// kCount adds per-launch totals of the traversal work (box tests, triangle
+// tests, instance visits, closest hits, shadow rays) to acc.work[0..4].
+template <bool kCount>
 __global__ void __launch_bounds__(kBlock)
 k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLdsStack * kBlock];
@@ -880,6 +883,8 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   const AccelView av = *S.accel;
   const Lut L{S.lut};
   uint64_t rays = 0;
+  TraceCounters tc{0u, 0u, 0u};
+  uint32_t nHits = 0, nShadow = 0;
   const uint32_t nBlocks = (P.pixelCount + kBlock - 1) / kBlock;
   // XCD-aware ordering: blocks resident on one XCD (blockIdx % 8) walk one
   // contiguous eighth of the pixel range so neighbouring pixels share an L2.
@@ -931,8 +936,9 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
       for (int event = 0; event <= eventLimit; event++) {
         TraceHit h;
         rays++;
-        traverse<false, false>(av, ro, rd, kRayMaxDistance, event != 0, depth, si.tri, si.slot, st, h, nullptr);
+        traverse<false, kCount>(av, ro, rd, kRayMaxDistance, event != 0, depth, si.tri, si.slot, st, h, &tc);
         si.hit = h.hit;
+        if (kCount) nHits += h.hit ? 1u : 0u;
         if (h.hit) {
           complete_surface(S, h, si);
           ma = load_material(S.materials + (size_t)si.material * 32, si.color);
@@ -1030,7 +1036,8 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
                 TraceHit sh;
                 rays++;
                 float tMax = fminf(fmaxf(sel.maxDistance, 0.0f), kRayMaxDistance);
-                if (!traverse<true, false>(av, sel.ro, sel.rd, tMax, true, 1, si.tri, si.slot, st, sh, nullptr))
+                if (kCount) nShadow++;
+                if (!traverse<true, kCount>(av, sel.ro, sel.rd, tMax, true, 1, si.tri, si.slot, st, sh, &tc))
                   li = li + sel.contribution * ucw;
               }
               float d0, d1;
@@ -1159,6 +1166,14 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   // One atomic per wave for the ray counter.
   for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
   if ((threadIdx.x & 63) == 0 && rays) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)rays);
+  if (kCount) {
+    uint64_t v[5] = {tc.boxes, tc.tris, tc.instances, nHits, nShadow};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+      if ((threadIdx.x & 63) == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + k, (unsigned long long)v[k]);
+    }
+  }
 }
 
 void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s) {
@@ -1167,7 +1182,10 @@ void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulat
   uint32_t need = ((p.pixelCount + kBlock - 1) / kBlock + 7u) / 8u * 8u;
   grid = std::min(grid, need);
   int32_t *spill = spill_for(s, grid, kBlock);
-  hipLaunchKernelGGL(k_render, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  if (acc.work)
+    hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  else
+    hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
   MTX_HIP(hipGetLastError());
 }
 

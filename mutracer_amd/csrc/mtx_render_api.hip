@@ -311,10 +311,10 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
   return T;
 }
 
-void target_render(TargetObj &T, uint32_t samples, hipStream_t s) {
+void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work = nullptr) {
   RenderParams p = T.params;
   p.samples = samples;
-  Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height};
+  Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height, work};
   launch_render(T.scene->view, p, acc, s);
   T.stream = s;
   T.samplesDone += samples;
@@ -400,6 +400,22 @@ int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream) {
   std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
   if (!lk.owns_lock()) return MTX_BUSY;
   target_render(*T, samples, (hipStream_t)stream);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[5], void *stream) {
+  API_TRY
+  MTX_REQUIRE(work, MTX_INVALID_ARGUMENT, "work is NULL");
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  hipStream_t s = (hipStream_t)stream;
+  DevBuf<uint64_t> w(5);
+  MTX_HIP(hipMemsetAsync(w.ptr, 0, w.bytes(), s));
+  target_render(*T, samples, s, w.ptr);
+  w.download(work, 5, s);
+  MTX_HIP(hipStreamSynchronize(s));
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

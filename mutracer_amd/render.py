@@ -126,6 +126,14 @@ class RenderTarget:
         sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
         abi.check(abi.lib().mtx_target_render(self.handle.value, samples, sp))
 
+    def render_counted(self, samples, stream=None):
+        """render() plus the launch's traversal work totals (synchronous)."""
+        sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+        out = np.zeros(5, np.uint64)
+        abi.check(abi.lib().mtx_target_render_counted(self.handle.value, samples, _P(out), sp))
+        return dict(box_tests=int(out[0]), tri_tests=int(out[1]), instance_visits=int(out[2]), hits=int(out[3]),
+                    shadow_rays=int(out[4]))
+
     def trace(self, camera=None, max_samples=1):
         """RenderTarget.trace: one more sample unless `max_samples` are done.
         Returns True while samples remain."""

@@ -33,12 +33,19 @@ def lib():
     return _lib
 
 
+def threads():
+    """OpenMP threads the oracle's batch entry points run on."""
+    return int(lib().orc_threads())
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
 def _declare(L):
     u32, f32, vp = C.c_uint32, C.c_float, C.c_void_p
+    L.orc_threads.restype = C.c_int32
+    L.orc_threads.argtypes = []
     for name in ("orc_sample_hash",):
         getattr(L, name).restype = u32
         getattr(L, name).argtypes = [u32]

@@ -3,6 +3,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "orc_bvh.h"
 #include "orc_render.h"
 #include "orc_sobol.h"
@@ -202,3 +205,12 @@ uint64_t orc_render(void *trace, const float *normals, const float *materials, u
 }
 
 }  // extern "C"
+
+// OpenMP threads the batch entry points use (for the CPU-baseline report).
+extern "C" int32_t orc_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -115,3 +115,21 @@ def test_read_pixels_resolve(env, orc):
     mean = (sums[..., 0:4] / counts[..., None])[::-1]
     assert np.allclose(px, mean.astype(np.float32), rtol=0, atol=0)
     assert t.stats()["rays"] > 32 * 24 * 4
+
+
+def test_counted_launch_matches_plain(env, orc):
+    """The instrumented launch renders the same samples as the plain one."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=6, freq=8, env_size=(64, 32))
+    rs = render.RenderScene(sc)
+    a = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4)
+    b = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4)
+    a.render(4)
+    work = b.render_counted(4)
+    sa, _ = a.read_accumulators()
+    sb, _ = b.read_accumulators()
+    assert np.array_equal(sa, sb)
+    rays = b.stats()["rays"]
+    assert 0 < work["hits"] <= rays - work["shadow_rays"]
+    assert work["box_tests"] > work["tri_tests"] > 0
+    assert work["instance_visits"] >= work["hits"]
