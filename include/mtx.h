@@ -291,6 +291,14 @@ int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera);
 int32_t mtx_target_reset(mtx_handle target);
 int32_t mtx_target_destroy(mtx_handle target);
 
+/* ------------------------------------------------------------ diagnostics */
+/* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on
+ * the device: kind 0 sin(x), 1 cos(x), 2 pow(x, y), 3 atan2(x, y) with x the
+ * ordinate, 4 acos(x). Device pointers, n elements, asynchronous on `stream`.
+ * No reference counterpart: it lets hosts check that device and host builds
+ * of these functions agree bit for bit. */
+int32_t mtx_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

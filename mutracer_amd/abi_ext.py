@@ -46,6 +46,7 @@ def declare(L):
         "mtx_target_set_camera": (i32, [u64, vp]),
         "mtx_target_reset": (i32, [u64]),
         "mtx_target_destroy": (i32, [u64]),
+        "mtx_math_eval": (i32, [u32, vp, vp, vp, u64, vp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

@@ -12,6 +12,7 @@
 // (running sums of color, alpha, albedo, normal and luminance squared), the
 // per-pixel sample counter giving each pixel's next Sobol sample index.
 #include "mtx_pt.h"
+#include "../../include/mtx_fmath.h"
 #include "mtx_registry.h"
 
 namespace mtx {
@@ -33,7 +34,7 @@ constexpr int kBlock = 128;
 enum { kLutMetal = 1056, kLutMetalAvg = 1088, kLutDiel = 1089, kLutDielAvg = 2113, kLutRatio = 2145 };
 
 __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
-__device__ __forceinline__ f3 vpow(f3 a, f3 b) { return f3{powf(a.x, b.x), powf(a.y, b.y), powf(a.z, b.z)}; }
+__device__ __forceinline__ f3 vpow(f3 a, f3 b) { return f3{mtxfm::powf(a.x, b.x), mtxfm::powf(a.y, b.y), mtxfm::powf(a.z, b.z)}; }
 __device__ __forceinline__ f3 vmaxs(f3 a, float s) { return vmax(a, splat(s)); }
 __device__ __forceinline__ f3 clamp3(f3 v, float lo, float hi) {
   return f3{clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)};
@@ -158,7 +159,7 @@ __device__ f3 ggx_vndf(f3 view, float ax, float ay, float sx, float sy) {
   float nz = (1.0f - sy) * span, omz = sy * span, opz = nz + (1.0f - vz);
   float radial = sqrtf(mxf(0.0f, opz * omz));
   float az = kTwoPi * sx;
-  f3 sn{sv.x + radial * cosf(az), sv.y + radial * sinf(az), nz};
+  f3 sn{sv.x + radial * mtxfm::cosf(az), sv.y + radial * mtxfm::sinf(az), nz};
   return normalize(st * sn);
 }
 __device__ __forceinline__ float ggx_ndf(f3 h, float ax, float ay) {
@@ -173,7 +174,7 @@ __device__ __forceinline__ float smith_g1(f3 d, float ax, float ay) {
   float tan2 = (rx * rx + ry * ry) / (nc * nc);
   return 2.0f / (1.0f + sqrtf(1.0f + tan2));
 }
-__device__ __forceinline__ float schlick5(float c) { return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
+__device__ __forceinline__ float schlick5(float c) { return mtxfm::powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
 __device__ float fresnel_dielectric(float incidentCos, float eta) {
   float cosI = clampf(fabsf(incidentCos), 0.0f, 1.0f);
   float sinT2 = eta * eta * mxf(0.0f, 1.0f - cosI * cosI);
@@ -189,8 +190,8 @@ __device__ f3 cosine_hemisphere(float sx, float sy) {
     float r, a;
     if (fabsf(x) > fabsf(y)) { r = x; a = kQuarterPi * (y / x); }
     else { r = y; a = kHalfPi - kQuarterPi * (x / y); }
-    dx = r * cosf(a);
-    dy = r * sinf(a);
+    dx = r * mtxfm::cosf(a);
+    dy = r * mtxfm::sinf(a);
   }
   return f3{dx, dy, sqrtf(mxf(0.0f, 1.0f - (dx * dx + dy * dy)))};
 }
@@ -255,7 +256,7 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.coatDarkening = clampf(m[22], 0.0f, 1.0f);
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
-    ma.specularRoughness = mixf(ma.specularRoughness, powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
+    ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
   }
   ma.baseColor = ma.baseColor * f3{col[0], col[1], col[2]};
   ma.specularRoughness = clampf(ma.specularRoughness, kRoughnessMin, 1.0f);
@@ -344,7 +345,7 @@ __device__ f3 metal_fresnel(float c, f3 f0, f3 tint) {
   float edgeComp = 1.0f - kEdgeCos;
   f3 edgeF = mix3(f0, splat(1.0f), schlick5(kEdgeCos));
   float comp = 1.0f - c;
-  float correction = c * schlick5(c) * comp / (kEdgeCos * powf(edgeComp, 6.0f));
+  float correction = c * schlick5(c) * comp / (kEdgeCos * mtxfm::powf(edgeComp, 6.0f));
   return clamp3(schlick - correction * (edgeF - tint * edgeF), 0.0f, 1.0f);
 }
 __device__ f3 eval_diffuse(const Mat &m, f3 color, f3 v, f3 l, float comp, float &pdf) {
@@ -393,7 +394,7 @@ __device__ __forceinline__ float avg_fresnel(float eta) {
 }
 __device__ f3 metal_avg_fresnel(f3 f0, f3 tint) {
   f3 edgeF = mix3(f0, splat(1.0f), schlick5(kEdgeCos));
-  f3 b = edgeF * (splat(1.0f) - tint) / (kEdgeCos * powf(1.0f - kEdgeCos, 6.0f));
+  f3 b = edgeF * (splat(1.0f) - tint) / (kEdgeCos * mtxfm::powf(1.0f - kEdgeCos, 6.0f));
   return clamp3(f0 + (splat(1.0f) - f0) / 21.0f - b / 126.0f, 0.0f, 1.0f);
 }
 __device__ f3 coat_passage(const Mat &m, float c) {
@@ -622,7 +623,7 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
     float metalDraw = (lobe - cdf[LC]) / mxf(P.p[LM], 1e-12f);
     if (isMetal & (metalDraw < P.metalMsProb)) {
       float z = rx, radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * cosf(az), radial * sinf(az), z};
+      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
     } else {
       f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
       if (h.z < 0.0f) h = -h;
@@ -636,7 +637,7 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
       bool reflectMs = rx < ratio;
       float z = reflectMs ? mnf(rx / ratio, 0.99999994f) : -mnf((rx - ratio) / (1.0f - ratio), 0.99999994f);
       float radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * cosf(az), radial * sinf(az), z};
+      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
       transmission = !reflectMs;
     } else {
       f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
@@ -675,9 +676,9 @@ __device__ f3 texture_linear(const GpuScene &S, float u, float v) {
   return mix3(mix3(s1, s2, tx), mix3(s3, s4, tx), ty);
 }
 __device__ __forceinline__ void env_uv(f3 d, float &u, float &v) {
-  float phi = atan2f(-d.z, -d.x);
+  float phi = mtxfm::atan2f(-d.z, -d.x);
   if (phi < 0.0f) phi += kTwoPi;
-  float theta = acosf(clampf(d.y, -1.0f, 1.0f));
+  float theta = mtxfm::acosf(clampf(d.y, -1.0f, 1.0f));
   u = phi * 0.5f * kInvPi;
   v = theta * kInvPi;
 }
@@ -688,7 +689,7 @@ __device__ __forceinline__ float2 cdf_at(const GpuScene &S, int x, int y) {
 }
 __device__ __forceinline__ float env_solid_pdf(float px, float py, float v) {
   if (!(v > 0.0f && v < 1.0f)) return 0.0f;
-  float sinT = sinf(mnf(v, 1.0f - v) * kPi);
+  float sinT = mtxfm::sinf(mnf(v, 1.0f - v) * kPi);
   return px * py * kInvPi2 / (2.0f * sinT);
 }
 __device__ float env_pdf(const GpuScene &S, float u, float v) {
@@ -715,7 +716,7 @@ __device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float 
   float2 b0 = cdf_at(S, 1 + uo, vo), b1 = cdf_at(S, 1 + uo + 1, vo);
   u = ((float)uo + (r1 - b0.x) / (b1.x - b0.x)) / (float)W;
   float phi = u * kTwoPi, theta = v * kPi;
-  float ct = cosf(theta), st = sinf(theta), cp = cosf(phi), sp = sinf(phi);
+  float ct = mtxfm::cosf(theta), st = mtxfm::sinf(theta), cp = mtxfm::cosf(phi), sp = mtxfm::sinf(phi);
   pdf = env_solid_pdf(b0.y, a0.y, v);
   return f3{-st * cp, ct, -st * sp};
 }
@@ -730,7 +731,7 @@ __device__ void sample_rect(const float *__restrict__ L, f3 p, float sx, float s
   r.direction = off / r.dist;
   float ec = dot(-r.direction, n);
   float spread = clampf(L[13], 0.0f, 1.0f);
-  float sc = cosf(0.5f * kPi * spread);
+  float sc = mtxfm::cosf(0.5f * kPi * spread);
   if (ec <= 0.0f || ec < sc) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
   r.emission = f3{L[10], L[11], L[12]};
   r.pdf = d2 / mxf(L[14] * ec, 1e-8f);
@@ -780,7 +781,7 @@ __device__ float intersect_rect_light(const float *__restrict__ L, f3 ro, f3 rd,
   }
   if (d == kInf) return kInf;
   float cosT = dot(-rd, n);
-  if (cosT >= cosf(L[13] * kPi / 2.0f)) {
+  if (cosT >= mtxfm::cosf(L[13] * kPi / 2.0f)) {
     ls.pdf = (d * d) / (L[14] * cosT);
     ls.emission = f3{L[10], L[11], L[12]};
     return d;

@@ -33,6 +33,18 @@ def lib():
     return _lib
 
 
+def math_eval(kind, x, y=None):
+    """include/mtx_fmath.h on the host: kind in {"sin","cos","pow","atan2","acos"}."""
+    k = ["sin", "cos", "pow", "atan2", "acos"].index(kind)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
+    out = np.empty_like(x)
+    L = lib()
+    L.orc_math_eval.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+    L.orc_math_eval(k, _p(x), _p(y), _p(out), x.size)
+    return out
+
+
 def threads():
     """OpenMP threads the oracle's batch entry points run on."""
     return int(lib().orc_threads())

@@ -6,6 +6,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include "../include/mtx_fmath.h"
 #include "orc_bvh.h"
 #include "orc_render.h"
 #include "orc_sobol.h"
@@ -213,4 +214,18 @@ extern "C" int32_t orc_threads() {
 #else
   return 1;
 #endif
+}
+
+// Elementary functions of include/mtx_fmath.h on the host: kind 0 sin, 1 cos,
+// 2 pow(x, y), 3 atan2(x, y) (x is the ordinate), 4 acos.
+extern "C" void orc_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) {
+    switch (kind) {
+      case 0: out[i] = mtxfm::sinf(x[i]); break;
+      case 1: out[i] = mtxfm::cosf(x[i]); break;
+      case 2: out[i] = mtxfm::powf(x[i], y[i]); break;
+      case 3: out[i] = mtxfm::atan2f(x[i], y[i]); break;
+      default: out[i] = mtxfm::acosf(x[i]); break;
+    }
+  }
 }

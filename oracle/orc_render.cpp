@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "orc_sobol.h"
+#include "../include/mtx_fmath.h"
 
 namespace orc {
 
@@ -28,7 +29,7 @@ enum { LUT_METAL_ROW = 1056, LUT_METAL_AVG_ROW = 1088, LUT_DIEL_ROW = 1089, LUT_
        LUT_RATIO_ROW = 2145 };
 
 static inline float fractf(float x) { return x - floorf(x); }
-static inline V3 vpow(V3 a, V3 b) { return V3(powf(a.x, b.x), powf(a.y, b.y), powf(a.z, b.z)); }
+static inline V3 vpow(V3 a, V3 b) { return V3(mtxfm::powf(a.x, b.x), mtxfm::powf(a.y, b.y), mtxfm::powf(a.z, b.z)); }
 static inline V3 vmaxs(V3 a, float s) { return vmax(a, V3(s)); }
 static inline V3 clamp3(V3 v, float lo, float hi) { return V3(clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)); }
 static inline V3 reflect(V3 e1, V3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }
@@ -50,7 +51,7 @@ static V3 ggx_visible_normal(V3 view, float ax, float ay, float sx, float sy) {
   float opz = nz + (1.0f - vz);
   float radial = std::sqrt(mxf(0.0f, opz * omz));
   float az = TWOPI * sx;
-  V3 sn(sv.x + radial * std::cos(az), sv.y + radial * std::sin(az), nz);
+  V3 sn(sv.x + radial * mtxfm::cosf(az), sv.y + radial * mtxfm::sinf(az), nz);
   return normalize(stretch * sn);
 }
 static float ggx_ndf(V3 h, float ax, float ay) {
@@ -65,7 +66,7 @@ static float smith_g1(V3 d, float ax, float ay) {
   float tan2 = (rx * rx + ry * ry) / (nc * nc);
   return 2.0f / (1.0f + std::sqrt(1.0f + tan2));
 }
-static float schlick5(float c) { return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
+static float schlick5(float c) { return mtxfm::powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
 static float dielectric_fresnel_s(float incidentCos, float eta) {
   float cosI = clampf(fabsf(incidentCos), 0.0f, 1.0f);
   float sinT2 = eta * eta * mxf(0.0f, 1.0f - cosI * cosI);
@@ -81,8 +82,8 @@ static void concentric_disk(float sx, float sy, float &ox, float &oy) {
   float r, a;
   if (fabsf(x) > fabsf(y)) { r = x; a = K_QUARTER_PI * (y / x); }
   else { r = y; a = K_HALF_PI - K_QUARTER_PI * (x / y); }
-  ox = r * std::cos(a);
-  oy = r * std::sin(a);
+  ox = r * mtxfm::cosf(a);
+  oy = r * mtxfm::sinf(a);
 }
 static V3 cosine_hemisphere(float sx, float sy) {
   float dx, dy;
@@ -156,7 +157,7 @@ static Mat get_material(const float *m, V3 instanceColor) {
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness;
     float c2 = ma.coatRoughness * ma.coatRoughness;
-    ma.specularRoughness = mixf(ma.specularRoughness, powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
+    ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
   }
   ma.baseColor = ma.baseColor * instanceColor;
   ma.specularRoughness = clampf(ma.specularRoughness, ROUGHNESS_MIN, 1.0f);
@@ -259,7 +260,7 @@ static V3 metal_fresnel(float c, V3 f0, V3 tint) {
   float edgeComp = 1.0f - K_EDGE_COS;
   V3 edgeF = mix3(f0, V3(1.0f), schlick5(K_EDGE_COS));
   float comp = 1.0f - c;
-  float correction = c * schlick5(c) * comp / (K_EDGE_COS * powf(edgeComp, 6.0f));
+  float correction = c * schlick5(c) * comp / (K_EDGE_COS * mtxfm::powf(edgeComp, 6.0f));
   return clamp3(schlick - correction * (edgeF - tint * edgeF), 0.0f, 1.0f);
 }
 static V3 eval_diffuse(const Mat &m, V3 color, V3 v, V3 l, float comp, float &pdf) {
@@ -313,7 +314,7 @@ static float average_fresnel(float eta) {
 }
 static V3 metal_average_fresnel(V3 f0, V3 tint) {
   V3 edgeF = mix3(f0, V3(1.0f), schlick5(K_EDGE_COS));
-  V3 b = edgeF * (V3(1.0f) - tint) / (K_EDGE_COS * powf(1.0f - K_EDGE_COS, 6.0f));
+  V3 b = edgeF * (V3(1.0f) - tint) / (K_EDGE_COS * mtxfm::powf(1.0f - K_EDGE_COS, 6.0f));
   return clamp3(f0 + (V3(1.0f) - f0) / 21.0f - b / 126.0f, 0.0f, 1.0f);
 }
 static V3 coat_passage(const Mat &m, float c) {
@@ -557,7 +558,7 @@ static bool sample_direction(const Prepared &P, float rx, float ry, float lobe, 
       float z = rx;
       float radial = std::sqrt(mxf(0.0f, 1.0f - z * z));
       float az = TWOPI * ry;
-      l = V3(radial * std::cos(az), radial * std::sin(az), z);
+      l = V3(radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z);
     } else {
       V3 h = ggx_visible_normal(v, m.alphaX, m.alphaY, rx, ry);
       if (h.z < 0.0f) h = -h;
@@ -572,7 +573,7 @@ static bool sample_direction(const Prepared &P, float rx, float ry, float lobe, 
       float z = reflectMs ? mnf(rx / ratio, 0.99999994f) : -mnf((rx - ratio) / (1.0f - ratio), 0.99999994f);
       float radial = std::sqrt(mxf(0.0f, 1.0f - z * z));
       float az = TWOPI * ry;
-      l = V3(radial * std::cos(az), radial * std::sin(az), z);
+      l = V3(radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z);
       transmission = !reflectMs;
     } else {
       V3 h = ggx_visible_normal(v, m.alphaX, m.alphaY, rx, ry);
@@ -741,9 +742,9 @@ static V3 texture_linear(const RenderScene &s, float u, float v) {
   return mix3(mix3(s1, s2, tx), mix3(s3, s4, tx), ty);
 }
 static void environment_uv(V3 d, float &u, float &v) {  // cartesianToEquirect, identity rotation
-  float phi = std::atan2(-d.z, -d.x);
+  float phi = mtxfm::atan2f(-d.z, -d.x);
   if (phi < 0.0f) phi += TWOPI;
-  float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+  float theta = mtxfm::acosf(clampf(d.y, -1.0f, 1.0f));
   u = phi * 0.5f * INVPI;
   v = theta * INVPI;
 }
@@ -754,7 +755,7 @@ static float cdfx(const HostTables &T, int x, int y, int c) {
 }
 static float env_solid_angle_pdf(float px, float py, float v) {
   if (!(v > 0.0f && v < 1.0f)) return 0.0f;
-  float sinT = std::sin(mnf(v, 1.0f - v) * PI);
+  float sinT = mtxfm::sinf(mnf(v, 1.0f - v) * PI);
   return px * py * INVPI2 / (2.0f * sinT);
 }
 static float env_map_pdf(const Ctx &c, float u, float v) {
@@ -785,7 +786,7 @@ static V3 sample_envmap(const Ctx &c, float r0, float r1, float &u, float &v, fl
   float px = cdfx(c.T, 1 + uOff, vOff, 1);
   u = ((float)uOff + (r1 - b0) / (b1 - b0)) / (float)W;
   float phi = u * TWOPI, theta = v * PI;
-  float ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+  float ct = mtxfm::cosf(theta), st = mtxfm::sinf(theta), cp = mtxfm::cosf(phi), sp = mtxfm::sinf(phi);
   pdf = env_solid_angle_pdf(px, py, v);
   return V3(-st * cp, ct, -st * sp);
 }
@@ -803,7 +804,7 @@ static void light_sample_rect(const float *L, V3 p, float sx, float sy, LightSam
   r.normal = n;
   float ec = dot(-r.direction, n);
   float spread = clampf(L[L_SPREAD], 0.0f, 1.0f);
-  float sc = std::cos(0.5f * PI * spread);
+  float sc = mtxfm::cosf(0.5f * PI * spread);
   if (ec <= 0.0f || ec < sc) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
   r.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
   r.pdf = d2 / mxf(L[L_AREA] * ec, 1e-8f);
@@ -859,7 +860,7 @@ static float intersect_rect_light(const float *L, V3 ro, V3 rd, LightSample &ls)
   if (d < 0.0f) return INF;
   if (d == INF) return INF;
   float cosT = dot(-rd, n);
-  if (cosT >= std::cos(L[L_SPREAD] * PI / 2.0f)) {
+  if (cosT >= mtxfm::cosf(L[L_SPREAD] * PI / 2.0f)) {
     ls.direction = rd; ls.normal = n; ls.dist = d;
     ls.pdf = (d * d) / (L[L_AREA] * cosT);
     ls.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
