@@ -41,6 +41,10 @@ HIT_B = 128 + 48    # per closest hit: packed material (32 f32) + 3 normals and 
 SAMPLE_B = 13 * 16 + 8  # per sample: read+write of 13 FP64 accumulator planes + the u32 count
 
 
+KERNEL_NAMES = {"wavefront": "wavefront launch sequence (generate, trace, shade, shadow, accumulate)",
+                "megakernel": "k_render (megakernel)"}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -87,6 +91,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the path shards by "
@@ -108,7 +114,8 @@ def main():
     info = rs.info()
     log(f"scene committed on GPU (BVH build + upload): {build_ms:.0f} ms, {info}")
     tgt = render.RenderTarget(rs, WIDTH, HEIGHT, bounces=DEPTH, direct_light_candidates=CANDIDATES,
-                              pattern="sobol", sequence_seed=0, max_samples=SPP)
+                              pattern="sobol", sequence_seed=0, max_samples=SPP, kernel=args.kernel,
+                              max_paths_in_flight=args.max_paths)
     stream = torch.cuda.current_stream()
 
     tgt.restart_samples()
@@ -161,13 +168,13 @@ def main():
                                "1024x512 sky+sun env, OpenPBR metal/glass/coat mix",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "depth": DEPTH,
                    "direct_light_candidates": CANDIDATES, "sampling": "sobol seed 0",
-                   "triangles": info["triangles"], "accumulation": "fp64"},
+                   "triangles": info["triangles"], "accumulation": "fp64", "kernel": args.kernel},
         "mrays_per_s": round(rays_per_step / (ms_per_step / 1e3) / 1e6, 3),
         "rays_per_step": rays_per_step,
         "scene_commit_ms": round(build_ms, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_render (megakernel)", "kernel_ms": round(kernel_ms, 3),
+                     "traffic": traffic, "kernel": KERNEL_NAMES[args.kernel], "kernel_ms": round(kernel_ms, 3),
                      "algorithmic_bytes_per_launch": int(alg_bytes), "work_per_launch": work},
         "cpu_baseline": cpu,
         "parity_relmse_cpu_tiles": parity,

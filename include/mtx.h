@@ -238,8 +238,12 @@ typedef struct mtx_render_settings {
   uint32_t max_samples;             /* ZSobol sample budget */
   uint32_t sample_sequence_offset;  /* Render.ts:151 */
   float clamping;                   /* 0 = off */
-  uint32_t reserved[2];
+  uint32_t kernel;                  /* MTX_KERNEL_WAVEFRONT (0) or MTX_KERNEL_MEGAKERNEL (1) */
+  uint32_t max_paths_in_flight;     /* wavefront generation size; 0 = 8M paths */
 } mtx_render_settings;
+
+#define MTX_KERNEL_WAVEFRONT 0u
+#define MTX_KERNEL_MEGAKERNEL 1u
 
 /* Camera block (cameraUniform.ts:2-17, browser projection-inverse form). */
 typedef struct mtx_camera_desc {

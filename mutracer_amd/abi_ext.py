@@ -20,7 +20,8 @@ class RenderSettings(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("bounces", C.c_uint32), ("direct_light_candidates", C.c_uint32),
                 ("sampling_pattern", C.c_uint32), ("sequence_seed", C.c_uint32), ("max_samples", C.c_uint32),
-                ("sample_sequence_offset", C.c_uint32), ("clamping", C.c_float), ("reserved", C.c_uint32 * 2)]
+                ("sample_sequence_offset", C.c_uint32), ("clamping", C.c_float), ("kernel", C.c_uint32),
+                ("max_paths_in_flight", C.c_uint32)]
 
 
 class CameraDesc(C.Structure):

@@ -1,5 +1,6 @@
 // Path tracer: resident render scene, render targets and kernel entry points.
 #pragma once
+#include "mtx_internal.h"
 #include "mtx_scene.h"
 
 namespace mtx {
@@ -62,6 +63,47 @@ struct Accumulators {
 struct RenderSceneObj;
 struct TargetObj;
 
+// Wavefront path state (structure of arrays, one entry per path in flight):
+//   o    ray origin, w = scattering pdf        d    ray direction, w = alpha
+//   li   radiance, w = path flags (bits)       beta throughput, w = last hit triangle
+//   alb  albedo AOV, w = last hit slot         nrm  normal AOV
+//   hitA closest hit t, barycentrics           hitB (instance slot or -1, triangle)
+//   qA/qB path-id queues of even/odd events
+//   s0..s3 shadow queue: (origin, tMax), (direction, path id), (payload, tri), slot
+//   counts[e] = length of event e's queue, counts[kWfShadowBase + e] = its shadow queue
+constexpr int kWfMaxEvents = 51;
+constexpr int kWfShadowBase = kWfMaxEvents + 1;
+constexpr int kWfCounters = kWfShadowBase + kWfMaxEvents + 1;
+struct WfBuffers {
+  float4 *o, *d, *li, *beta, *alb, *nrm, *hitA;
+  int2 *hitB;
+  uint32_t *qA, *qB;
+  float4 *s0, *s1, *s2;
+  int32_t *s3;
+  uint32_t *counts;
+  uint32_t nPaths;  // paths in flight: gen * pixelCount
+  uint32_t gen;     // samples per pixel in flight
+  struct Owner {
+    DevBuf<float4> o, d, li, beta, alb, nrm, hitA, s0, s1, s2;
+    DevBuf<int2> hitB;
+    DevBuf<uint32_t> qA, qB, counts;
+    DevBuf<int32_t> s3;
+    uint32_t capacity = 0;
+    void ensure(uint32_t paths);
+    size_t bytes() const;
+    WfBuffers view(uint32_t nPaths, uint32_t gen) const;
+  };
+};
+
+// Launch shape of the wavefront kernels (the autotuner's search space).
+struct WfConfig {
+  uint32_t maxPaths = 8u << 20;  // paths in flight per generation
+  int traceOversub = 1;          // trace/shadow grid = CUs * resident blocks * this
+  int shadeOversub = 1;
+};
+
 void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s);
+void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const Accumulators &acc,
+                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s);
 
 }  // namespace mtx

@@ -11,867 +11,10 @@
 // the per-lane LDS stack. Results are accumulated in FP64 per pixel
 // (running sums of color, alpha, albedo, normal and luminance squared), the
 // per-pixel sample counter giving each pixel's next Sobol sample index.
-#include "mtx_pt.h"
-#include "../../include/mtx_fmath.h"
 #include "mtx_registry.h"
+#include "mtx_shade.h"
 
 namespace mtx {
-
-// ---------------------------------------------------------------- constants
-// Constant expressions are folded in f64 and rounded once, as WGSL folds
-// abstract-float constants.
-#define KF(x) ((float)(x))
-constexpr float kQuarterPi = KF(0.25 * 3.14159265359);
-constexpr float kHalfPi = KF(0.5 * 3.14159265359);
-constexpr float kAlphaMin = KF(0.001 * 0.001);
-constexpr float kFonA = KF(0.5 - 2.0 / (3.0 * 3.14159265359));
-constexpr float kFonB = KF(2.0 / 3.0 - 28.0 / (15.0 * 3.14159265359));
-constexpr float kEdgeCos = KF(1.0 / 7.0);
-constexpr float kEpsLo = 0.0005f;
-constexpr float kEpsScale = KF(1.0 - 2.0 * 0.0005);
-constexpr int kBlock = 128;
-
-enum { kLutMetal = 1056, kLutMetalAvg = 1088, kLutDiel = 1089, kLutDielAvg = 2113, kLutRatio = 2145 };
-
-__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
-__device__ __forceinline__ f3 vpow(f3 a, f3 b) { return f3{mtxfm::powf(a.x, b.x), mtxfm::powf(a.y, b.y), mtxfm::powf(a.z, b.z)}; }
-__device__ __forceinline__ f3 vmaxs(f3 a, float s) { return vmax(a, splat(s)); }
-__device__ __forceinline__ f3 clamp3(f3 v, float lo, float hi) {
-  return f3{clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)};
-}
-__device__ __forceinline__ f3 reflect3(f3 e1, f3 e2) { return e1 - (2.0f * dot(e2, e1)) * e2; }
-__device__ __forceinline__ f3 refract3(f3 e1, f3 e2, float e3) {
-  float c = dot(e2, e1);
-  float k = 1.0f - e3 * e3 * (1.0f - c * c);
-  if (k < 0.0f) return splat(0.0f);
-  return e3 * e1 - (e3 * c + sqrtf(k)) * e2;
-}
-
-// ---------------------------------------------------------------- sampler
-// Burley 2020 hash-based Owen scrambling of the 2D Sobol sequence
-// (sobol.wgsl.ts:23-52, 107-168); the second dimension uses the five-step
-// Pascal butterfly of sampling.ts:31-34.
-__device__ __forceinline__ uint32_t shash(uint32_t x) {
-  x ^= x >> 16; x *= 0x7feb352du;
-  x ^= x >> 15; x *= 0x846ca68bu;
-  return x ^ (x >> 16);
-}
-__device__ __forceinline__ uint32_t sseed(uint32_t a, uint32_t b) { return shash(a ^ shash(b + 0x9e3779b9u)); }
-__device__ __forceinline__ uint32_t lkperm(uint32_t x, uint32_t s) {
-  x += s;
-  x ^= x * 0x6c50b47cu; x ^= x * 0xb82f1e52u;
-  x ^= x * 0xc7afe638u; x ^= x * 0x8d22f6e6u;
-  return x;
-}
-__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint32_t s) {
-  return __builtin_bitreverse32(lkperm(__builtin_bitreverse32(x), s));
-}
-__device__ __forceinline__ uint32_t sobol_second(uint32_t v) {
-  uint32_t r = __builtin_bitreverse32(v);
-  r ^= (r << 1) & 0xaaaaaaaau;
-  r ^= (r << 2) & 0xccccccccu;
-  r ^= (r << 4) & 0xf0f0f0f0u;
-  r ^= (r << 8) & 0xff00ff00u;
-  r ^= (r << 16) & 0xffff0000u;
-  return r;
-}
-__device__ __forceinline__ uint32_t morton8(uint32_t x, uint32_t y) {
-  x &= 255u; y &= 255u;
-  x = (x | (x << 4)) & 0x0f0fu; y = (y | (y << 4)) & 0x0f0fu;
-  x = (x | (x << 2)) & 0x3333u; y = (y | (y << 2)) & 0x3333u;
-  x = (x | (x << 1)) & 0x5555u; y = (y | (y << 1)) & 0x5555u;
-  return x | (y << 1);
-}
-// The 24 base-4 digit permutations of pbrt-v4's ZSobolSampler, four 8-bit
-// codes (digit i in bits 2i..2i+1) per word (sampling.ts:20-33).
-__constant__ uint32_t kZPermPacked[6] = {0x78d8b4e4u, 0xb1e19c6cu, 0x8d2d39c9u, 0x72d236c6u, 0x87271e4eu, 0x93634b1bu};
-__device__ __forceinline__ uint32_t zperm(uint32_t p) { return (kZPermPacked[p >> 2] >> ((p & 3u) * 8u)) & 255u; }
-__device__ uint32_t zsobol_index(uint32_t morton, uint32_t dim, uint32_t logSPP) {
-  uint32_t odd = logSPP & 1u;
-  int digits = 8 + (int)((logSPP + 1u) >> 1);
-  uint32_t result = 0;
-  for (int digit = digits - 1; digit >= (int)odd; digit--) {
-    uint32_t shift = 2u * (uint32_t)digit - odd;
-    uint32_t higher = shift < 30u ? morton >> (shift + 2u) : 0u;
-    uint32_t perm = (shash(higher ^ (0x55555555u * dim)) >> 24) % 24u;
-    uint32_t d = (morton >> shift) & 3u;
-    result |= ((zperm(perm) >> (2u * d)) & 3u) << shift;
-  }
-  if (odd) result |= (morton & 1u) ^ (shash((morton >> 1) ^ (0x55555555u * dim)) & 1u);
-  return result;
-}
-
-struct Smp {
-  uint32_t px, py, width, index, seed, logSPP, pattern, bounce;
-  __device__ uint32_t bdim(uint32_t slot) const { return 3u + bounce * 64u + slot; }
-  __device__ uint32_t ldim(uint32_t cand, uint32_t slot) const {
-    if (cand == 0u) return slot == 2u ? bdim(8u) : bdim(slot);
-    return bdim(9u + (cand - 1u) * 3u + slot);
-  }
-  __device__ void raw(uint32_t dim, float &x, float &y, bool needY) const {
-    uint32_t pixelID = px + py * width;
-    uint32_t s = sseed(sseed(pixelID, dim), seed);
-    uint32_t idx = index;
-    if (pattern == 1u) {
-      uint32_t lg = logSPP;
-      if (idx == 0u) lg = 0u; else idx -= 1u;
-      uint32_t block = idx >> lg;
-      uint32_t mask = (1u << lg) - 1u;
-      uint32_t m = (morton8(px, py) << lg) | (idx & mask);
-      idx = zsobol_index(m, dim, lg);
-      s = sseed(sseed(dim, seed), sseed(px >> 8, py >> 8));
-      if (index != 0u) block += 1u;
-      s = sseed(s, block);
-    }
-    uint32_t i = owen_scramble(idx, sseed(s, 0u));
-    uint32_t xb = __builtin_bitreverse32(lkperm(i, sseed(s, 1u)));
-    x = (float)(xb >> 8) * (1.0f / 16777216.0f);
-    if (needY) {
-      uint32_t yb = owen_scramble(sobol_second(i), sseed(s, 2u));
-      y = (float)(yb >> 8) * (1.0f / 16777216.0f);
-    }
-  }
-  __device__ float s1(uint32_t dim) const {
-    float x, y;
-    raw(dim, x, y, false);
-    return kEpsLo + kEpsScale * x;
-  }
-  __device__ void s2(uint32_t dim, float &a, float &b) const {
-    float x, y;
-    raw(dim, x, y, true);
-    a = kEpsLo + kEpsScale * x;
-    b = kEpsLo + kEpsScale * y;
-  }
-  __device__ float candidate_random(uint32_t salt) const {
-    uint32_t pixelID = px + py * width;
-    uint32_t key = sseed(sseed(pixelID, index), seed);
-    uint32_t dimension = sseed(bounce, salt) ^ 0xa511e9b3u;
-    return (float)(shash(key ^ dimension) >> 8) * (1.0f / 16777216.0f);
-  }
-};
-
-// ---------------------------------------------------------------- BSDF math
-__device__ f3 ggx_vndf(f3 view, float ax, float ay, float sx, float sy) {
-  f3 st{ax, ay, 1.0f};
-  f3 sv = normalize(st * view);
-  float vz = mnf(sv.z, 1.0f);
-  float span = 1.0f + vz;
-  float nz = (1.0f - sy) * span, omz = sy * span, opz = nz + (1.0f - vz);
-  float radial = sqrtf(mxf(0.0f, opz * omz));
-  float az = kTwoPi * sx;
-  f3 sn{sv.x + radial * mtxfm::cosf(az), sv.y + radial * mtxfm::sinf(az), nz};
-  return normalize(st * sn);
-}
-__device__ __forceinline__ float ggx_ndf(f3 h, float ax, float ay) {
-  f3 e{h.x / ax, h.y / ay, h.z};
-  float den = dot(e, e);
-  return 1.0f / (kPi * ax * ay * den * den);
-}
-__device__ __forceinline__ float smith_g1(f3 d, float ax, float ay) {
-  float nc = fabsf(d.z);
-  if (nc <= 1e-8f) return 0.0f;
-  float rx = ax * d.x, ry = ay * d.y;
-  float tan2 = (rx * rx + ry * ry) / (nc * nc);
-  return 2.0f / (1.0f + sqrtf(1.0f + tan2));
-}
-__device__ __forceinline__ float schlick5(float c) { return mtxfm::powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
-__device__ float fresnel_dielectric(float incidentCos, float eta) {
-  float cosI = clampf(fabsf(incidentCos), 0.0f, 1.0f);
-  float sinT2 = eta * eta * mxf(0.0f, 1.0f - cosI * cosI);
-  if (sinT2 >= 1.0f) return 1.0f;
-  float cosT = sqrtf(mxf(0.0f, 1.0f - sinT2));
-  float perp = (cosI - eta * cosT) / mxf(cosI + eta * cosT, 1e-8f);
-  float par = (eta * cosI - cosT) / mxf(eta * cosI + cosT, 1e-8f);
-  return 0.5f * (perp * perp + par * par);
-}
-__device__ f3 cosine_hemisphere(float sx, float sy) {
-  float x = 2.0f * sx - 1.0f, y = 2.0f * sy - 1.0f, dx = 0.0f, dy = 0.0f;
-  if (!(x == 0.0f && y == 0.0f)) {
-    float r, a;
-    if (fabsf(x) > fabsf(y)) { r = x; a = kQuarterPi * (y / x); }
-    else { r = y; a = kHalfPi - kQuarterPi * (x / y); }
-    dx = r * mtxfm::cosf(a);
-    dy = r * mtxfm::sinf(a);
-  }
-  return f3{dx, dy, sqrtf(mxf(0.0f, 1.0f - (dx * dx + dy * dy)))};
-}
-__device__ __forceinline__ float power_heuristic(float a, float b) {
-  float scale = mxf(a, b);
-  if (scale <= 0.0f) return 0.0f;
-  float na = a / scale, nb = b / scale;
-  float sa = na * na, sb = nb * nb;
-  return sa / (sa + sb);
-}
-
-struct Basis { f3 t, b, n; };
-__device__ __forceinline__ Basis make_basis(f3 n) {
-  float signZ = n.z >= 0.0f ? 1.0f : -1.0f;
-  float factor = -1.0f / (signZ + n.z);
-  float xy = n.x * n.y * factor;
-  Basis B;
-  B.t = f3{1.0f + signZ * n.x * n.x * factor, signZ * xy, -signZ * n.x};
-  B.b = f3{xy, signZ + n.y * n.y * factor, -n.y};
-  B.n = n;
-  return B;
-}
-__device__ __forceinline__ f3 to_world(const Basis &B, f3 d) { return d.x * B.t + d.y * B.b + d.z * B.n; }
-__device__ __forceinline__ f3 to_local(const Basis &B, f3 d) { return f3{dot(d, B.t), dot(d, B.b), dot(d, B.n)}; }
-
-// ---------------------------------------------------------------- materials
-struct Mat {
-  f3 baseColor, specularColor, transmissionColor, coatColor;
-  float baseWeight, diffuseRoughness, baseMetalness, specularWeight, specularRoughness, specularIor;
-  float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening;
-  float relativeIor, alphaX, alphaY;
-  bool inside;
-};
-
-__device__ __forceinline__ void rough_alpha(float r, float aniso, float &ax, float &ay) {
-  float a = r * r;
-  if (aniso == 0.0f) { ax = ay = mxf(kAlphaMin, a); return; }
-  float c = 1.0f - aniso;
-  float at = a * sqrtf(2.0f / (1.0f + c * c));
-  ax = mxf(kAlphaMin, at);
-  ay = mxf(kAlphaMin, c * at);
-}
-
-__device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
-  Mat ma;
-  ma.baseColor = f3{m[1], m[2], m[3]};
-  ma.specularRoughness = m[10];
-  ma.baseMetalness = m[5];
-  float aniso = clampf(m[27], 0.0f, 1.0f);
-  ma.inside = false;
-  ma.specularIor = m[11];
-  ma.transmissionWeight = m[12];
-  ma.baseWeight = clampf(m[0], 0.0f, 1.0f);
-  ma.diffuseRoughness = clampf(m[4], 0.0f, 1.0f);
-  ma.specularWeight = mxf(m[6], 0.0f);
-  ma.specularColor = vmaxs(f3{m[7], m[8], m[9]}, 0.0f);
-  ma.transmissionColor = vmaxs(f3{m[13], m[14], m[15]}, 0.0f);
-  ma.coatWeight = clampf(m[16], 0.0f, 1.0f);
-  ma.coatColor = vmaxs(f3{m[17], m[18], m[19]}, 0.0f);
-  ma.coatRoughness = clampf(m[20], kRoughnessMin, 1.0f);
-  ma.coatIor = mxf(m[21], 1.0f);
-  ma.coatDarkening = clampf(m[22], 0.0f, 1.0f);
-  if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
-    float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
-    ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
-  }
-  ma.baseColor = ma.baseColor * f3{col[0], col[1], col[2]};
-  ma.specularRoughness = clampf(ma.specularRoughness, kRoughnessMin, 1.0f);
-  ma.baseMetalness = clampf(ma.baseMetalness, 0.0f, 1.0f);
-  rough_alpha(ma.specularRoughness, aniso, ma.alphaX, ma.alphaY);
-  ma.relativeIor = 1.0f;
-  return ma;
-}
-
-// ---------------------------------------------------------------- energy tables
-struct Lut {
-  const float *__restrict__ d;
-  __device__ __forceinline__ float at(int col, int row) const { return d[row * 32 + col]; }
-  __device__ float tri(int base, float eI, float eA, float eC) const {
-    float iI = clampf(eI, 0.0f, 31.0f), aI = clampf(eA, 0.0f, 31.0f), cI = clampf(eC, 0.0f, 31.0f);
-    int i0 = (int)floorf(iI), a0 = (int)floorf(aI), c0 = (int)floorf(cI);
-    int i1 = min(i0 + 1, 31), a1 = min(a0 + 1, 31), c1 = min(c0 + 1, 31);
-    float im = iI - (float)i0, am = aI - (float)a0, cm = cI - (float)c0;
-    float v000 = at(c0, base + i0 * 32 + a0), v001 = at(c1, base + i0 * 32 + a0);
-    float v010 = at(c0, base + i0 * 32 + a1), v011 = at(c1, base + i0 * 32 + a1);
-    float v100 = at(c0, base + i1 * 32 + a0), v101 = at(c1, base + i1 * 32 + a0);
-    float v110 = at(c0, base + i1 * 32 + a1), v111 = at(c1, base + i1 * 32 + a1);
-    float v00 = mixf(v000, v100, im), v01 = mixf(v001, v101, im);
-    float v10 = mixf(v010, v110, im), v11 = mixf(v011, v111, im);
-    return mixf(mixf(v00, v10, am), mixf(v01, v11, am), cm);
-  }
-  __device__ float bi(int base, float eI, float eA) const {
-    float iI = clampf(eI, 0.0f, 31.0f), aI = clampf(eA, 0.0f, 31.0f);
-    int i0 = (int)floorf(iI), a0 = (int)floorf(aI);
-    int i1 = min(i0 + 1, 31), a1 = min(a0 + 1, 31);
-    float im = iI - (float)i0, am = aI - (float)a0;
-    float v00 = at(a0, base + i0), v01 = at(a1, base + i0), v10 = at(a0, base + i1), v11 = at(a1, base + i1);
-    return mixf(mixf(v00, v10, im), mixf(v01, v11, im), am);
-  }
-};
-__device__ __forceinline__ float f0_ior(float e) { float v = (e - 1.0f) / (e + 1.0f); return v * v; }
-__device__ __forceinline__ float ior_index(float ior) {
-  return ior < 1.0f ? 15.0f - ((1.0f / ior - 1.0f) / 1.5f) * 15.0f : 16.0f + ((ior - 1.0f) / 1.5f) * 15.0f;
-}
-__device__ __forceinline__ float extrapolate(float v, float ior) {
-  if (ior > 2.5f || ior < 0.4f) {
-    float maxF0 = f0_ior(2.5f);
-    float progress = (f0_ior(ior) - maxF0) / (1.0f - maxF0);
-    return (1.0f - progress) * v;
-  }
-  return v;
-}
-__device__ __forceinline__ float opaque_ec(const Lut &L, float ior, float a, float c) {
-  return extrapolate(L.tri(0, ior_index(ior), sqrtf(a) * 31.0f, c * 31.0f), ior);
-}
-__device__ __forceinline__ float opaque_avg_ec(const Lut &L, float ior, float a) {
-  return extrapolate(L.bi(1024, ior_index(ior), sqrtf(a) * 31.0f), ior);
-}
-__device__ __forceinline__ float ideal_loss(const Lut &L, float ior, float a, float c) {
-  return L.tri(kLutDiel, ior_index(ior), sqrtf(a) * 31.0f, c * 31.0f);
-}
-__device__ __forceinline__ float ideal_avg(const Lut &L, float ior, float a) { return L.bi(kLutDielAvg, ior_index(ior), sqrtf(a) * 31.0f); }
-__device__ __forceinline__ float ideal_ratio(const Lut &L, float ior, float a) { return L.bi(kLutRatio, ior_index(ior), sqrtf(a) * 31.0f); }
-__device__ float metal_ec(const Lut &L, float a, float c) {
-  float rI = clampf(sqrtf(a) * 31.0f, 0.0f, 31.0f), cI = clampf(c * 31.0f, 0.0f, 31.0f);
-  int r0 = (int)floorf(rI), c0 = (int)floorf(cI);
-  int r1 = min(r0 + 1, 31), c1 = min(c0 + 1, 31);
-  float fc = fractf(cI), fr = fractf(rI);
-  return mixf(mixf(L.at(c0, kLutMetal + r0), L.at(c1, kLutMetal + r0), fc),
-              mixf(L.at(c0, kLutMetal + r1), L.at(c1, kLutMetal + r1), fc), fr);
-}
-__device__ float metal_avg_ec(const Lut &L, float a) {
-  float idx = clampf(sqrtf(a) * 31.0f, 0.0f, 31.0f);
-  int i0 = (int)floorf(idx);
-  return mixf(L.at(i0, kLutMetalAvg), L.at(min(i0 + 1, 31), kLutMetalAvg), fractf(idx));
-}
-
-__device__ __forceinline__ float lum_pbr(f3 c) { return dot(c, f3{0.212671f, 0.715160f, 0.072169f}); }
-__device__ __forceinline__ float spec_weighted_ior(float eta, float w) {
-  float f0 = mnf(f0_ior(eta) * w, 0.9999f);
-  float s = sqrtf(f0);
-  float ext = (1.0f + s) / (1.0f - s);
-  return eta < 1.0f ? 1.0f / ext : ext;
-}
-__device__ __forceinline__ float pbr_fresnel(float c, float rel) {
-  return fabsf(rel - 1.0f) < 1e-6f ? 0.0f : fresnel_dielectric(c, rel);
-}
-__device__ f3 metal_fresnel(float c, f3 f0, f3 tint) {
-  f3 schlick = mix3(f0, splat(1.0f), schlick5(c));
-  if (tint.x == 1.0f && tint.y == 1.0f && tint.z == 1.0f) return schlick;
-  float edgeComp = 1.0f - kEdgeCos;
-  f3 edgeF = mix3(f0, splat(1.0f), schlick5(kEdgeCos));
-  float comp = 1.0f - c;
-  float correction = c * schlick5(c) * comp / (kEdgeCos * mtxfm::powf(edgeComp, 6.0f));
-  return clamp3(schlick - correction * (edgeF - tint * edgeF), 0.0f, 1.0f);
-}
-__device__ f3 eval_diffuse(const Mat &m, f3 color, f3 v, f3 l, float comp, float &pdf) {
-  pdf = 0.0f;
-  if (l.z <= 0.0f) return splat(0.0f);
-  float r = m.diffuseRoughness, muI = l.z, muO = v.z;
-  float dir = dot(l, v) - muI * muO;
-  float dot_ = dir > 0.0f ? dir / mxf(mxf(muI, muO), 1e-7f) : dir;
-  float ssA = 1.0f / (1.0f + kFonA * r);
-  f3 single = color * kInvPi * ssA * (1.0f + r * dot_);
-  float ic = 1.0f - muI, oc = 1.0f - muO;
-  float iG = ic * (0.0571085289f + ic * (0.491881867f + ic * (-0.332181442f + ic * 0.0714429953f)));
-  float oG = oc * (0.0571085289f + oc * (0.491881867f + oc * (-0.332181442f + oc * 0.0714429953f)));
-  float iAlb = (1.0f + r * iG) / (1.0f + kFonA * r);
-  float oAlb = (1.0f + r * oG) / (1.0f + kFonA * r);
-  float avg = ssA * (1.0f + kFonB * r);
-  f3 msAlb = color * color * avg / vmax(splat(1e-7f), splat(1.0f) - color * (1.0f - avg));
-  f3 multiple = msAlb * kInvPi * mxf(1e-7f, 1.0f - oAlb) * mxf(1e-7f, 1.0f - iAlb) / mxf(1e-7f, 1.0f - avg);
-  pdf = l.z * kInvPi;
-  return (single + multiple) * comp;
-}
-__device__ f3 eval_refl(float ax, float ay, f3 v, f3 l, f3 h, f3 F, float &pdf) {
-  pdf = 0.0f;
-  if (l.z <= 0.0f) return splat(0.0f);
-  float D = ggx_ndf(h, ax, ay), G1 = smith_g1(v, ax, ay), G2 = G1 * smith_g1(l, ax, ay);
-  pdf = G1 * D / (4.0f * mxf(v.z, 1e-8f));
-  return F * D * G2 / (4.0f * mxf(l.z * v.z, 1e-8f));
-}
-__device__ f3 eval_refr(float ax, float ay, float rel, f3 tint, f3 v, f3 l, f3 h, f3 F, float &pdf) {
-  pdf = 0.0f;
-  if (l.z >= 0.0f) return splat(0.0f);
-  float lh = dot(l, h), vh = dot(v, h);
-  if (vh <= 0.0f || lh >= 0.0f) return splat(0.0f);
-  float D = ggx_ndf(h, ax, ay), G1 = smith_g1(v, ax, ay), G2 = G1 * smith_g1(l, ax, ay);
-  float den = lh + vh * rel;
-  den *= den;
-  float jac = fabsf(lh) / mxf(den, 1e-8f);
-  float rel2 = rel * rel;
-  pdf = G1 * mxf(0.0f, vh) * D * jac / mxf(v.z, 1e-8f);
-  return tint * (splat(1.0f) - F) * D * G2 * fabsf(vh) * jac * rel2 / mxf(fabsf(l.z * v.z), 1e-8f);
-}
-__device__ __forceinline__ float avg_fresnel(float eta) {
-  if (fabsf(eta - 1.0f) < 1e-6f) return 0.0f;
-  if (eta > 1.0f) return (eta - 1.0f) / (4.08567f + 1.00071f * eta);
-  return 0.997118f + 0.1014f * eta - 0.965241f * eta * eta - 0.130607f * eta * eta * eta;
-}
-__device__ f3 metal_avg_fresnel(f3 f0, f3 tint) {
-  f3 edgeF = mix3(f0, splat(1.0f), schlick5(kEdgeCos));
-  f3 b = edgeF * (splat(1.0f) - tint) / (kEdgeCos * mtxfm::powf(1.0f - kEdgeCos, 6.0f));
-  return clamp3(f0 + (splat(1.0f) - f0) / 21.0f - b / 126.0f, 0.0f, 1.0f);
-}
-__device__ f3 coat_passage(const Mat &m, float c) {
-  if (c <= 0.0f) return splat(1.0f);
-  float rc = sqrtf(mxf(0.0f, 1.0f - (1.0f - c * c) / (m.coatIor * m.coatIor)));
-  f3 passage = vpow(vmaxs(m.coatColor, 0.0f), splat(0.5f / mxf(rc, 1e-5f)));
-  return mix3(splat(1.0f), passage, m.coatWeight);
-}
-
-enum { LF = 0, LC = 1, LM = 2, LS = 3, LT = 4, LD = 5, LSS = 6, LN = 7 };
-
-struct Prep {
-  Mat m;
-  Basis basis;
-  f3 v, coatBaseScale, metalMsScale, msRefl, msTrans;
-  float specEta, refrEta, transReflIor, specAlpha, dsComp;
-  float metalMsFactor, metalMsProb, msViewLoss, msRatio, msProb;
-  float w[LN], p[LN];
-};
-
-__device__ f3 coat_darkening(const Prep &P) {
-  const Mat &m = P.m;
-  float diel = 1.0f - m.baseMetalness;
-  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - 0.0f);
-  float sss = diel * (1.0f - m.transmissionWeight) * 0.0f;
-  float trans = diel * m.transmissionWeight;
-  float oSpec = avg_fresnel(P.specEta), tSpec = avg_fresnel(1.0f / P.transReflIor);
-  float specularity = clampf(opaque * oSpec + (sss + trans) * tSpec + m.baseMetalness, 0.0f, 1.0f);
-  float effR = mixf(1.0f, m.specularRoughness, specularity);
-  float ks = avg_fresnel(m.coatIor);
-  float kr = 1.0f - (1.0f - ks) / (m.coatIor * m.coatIor);
-  float k = mixf(ks, kr, effR);
-  f3 eb = m.baseMetalness * m.specularWeight * metal_avg_fresnel(m.baseWeight * m.baseColor, m.specularColor) +
-          opaque * mix3(m.baseWeight * m.baseColor, splat(1.0f), oSpec) + sss * splat(0.8f) + splat(trans);
-  f3 delta = splat(1.0f - k) / vmax(splat(1.0f) - eb * k, splat(1e-6f));
-  return mix3(splat(1.0f), delta, m.coatWeight * m.coatDarkening);
-}
-__device__ __forceinline__ f3 coat_base_scale(const Prep &P, float lc) {
-  if (P.m.coatWeight <= 0.0f) return splat(1.0f);
-  if (P.m.inside) return coat_passage(P.m, -lc);
-  return P.coatBaseScale * coat_passage(P.m, lc);
-}
-
-__device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L) {
-  P.m = m;
-  P.basis = make_basis(normal);
-  P.v = to_local(P.basis, view);
-  float diel = 1.0f - m.baseMetalness, opaqueDiel = diel * (1.0f - m.transmissionWeight);
-  P.w[LF] = 0.0f;
-  P.w[LC] = m.inside ? 0.0f : m.coatWeight;
-  P.w[LM] = m.baseMetalness;
-  P.w[LS] = diel;
-  P.w[LT] = diel * m.transmissionWeight;
-  P.w[LD] = opaqueDiel * (1.0f - 0.0f) * m.baseWeight;
-  P.w[LSS] = opaqueDiel * 0.0f;
-  P.specEta = spec_weighted_ior(1.0f / m.relativeIor, m.specularWeight);
-  P.m.relativeIor = 1.0f / P.specEta;
-  P.refrEta = P.specEta;
-  P.transReflIor = P.m.relativeIor;
-  if (m.coatWeight > 0.0f) {
-    float wCoat = m.inside ? 1.0f : mixf(1.0f, m.coatIor, m.coatWeight);
-    float wBase = wCoat * spec_weighted_ior(m.specularIor / wCoat, m.specularWeight);
-    float reflEta = wBase / wCoat;
-    if ((wCoat > wBase) && (wBase >= 1.0f)) reflEta = 1.0f / reflEta;
-    P.specEta = reflEta;
-    P.transReflIor = 1.0f / reflEta;
-    P.refrEta = wBase;
-    if (m.inside) { P.refrEta = 1.0f / wBase; P.transReflIor = wBase; }
-    P.m.relativeIor = 1.0f / P.refrEta;
-  }
-  P.specAlpha = m.specularRoughness * m.specularRoughness;
-  float vc = fabsf(P.v.z);
-  f3 dielF = m.specularColor * pbr_fresnel(vc, 1.0f / P.specEta);
-  f3 metalF = m.specularWeight * metal_fresnel(vc, m.baseWeight * m.baseColor, m.specularColor);
-  float transF = pbr_fresnel(vc, P.transReflIor);
-  f3 tint = m.transmissionColor;
-  P.dsComp = opaque_ec(L, P.specEta, P.specAlpha, vc) / mxf(opaque_avg_ec(L, P.specEta, P.specAlpha), 1e-12f);
-  P.p[LF] = 0.0f;
-  P.p[LC] = 0.0f;
-  P.coatBaseScale = splat(1.0f);
-  if ((m.coatWeight > 0.0f) && !m.inside) {
-    float ce = m.coatRoughness <= kRoughnessMin
-                   ? pbr_fresnel(vc, 1.0f / m.coatIor)
-                   : clampf(1.0f - opaque_ec(L, m.coatIor, m.coatRoughness * m.coatRoughness, vc), 0.0f, 1.0f);
-    P.p[LC] = P.w[LC] * ce;
-    P.coatBaseScale = coat_passage(m, vc) * (1.0f - P.w[LC] * ce) * coat_darkening(P);
-  }
-  P.p[LM] = P.w[LM] * lum_pbr(metalF);
-  P.metalMsScale = splat(0.0f);
-  P.metalMsFactor = 0.0f;
-  P.metalMsProb = 0.0f;
-  if ((m.specularRoughness >= 0.04f) && (P.w[LM] > 0.0f)) {
-    f3 avgF = metal_avg_fresnel(m.baseWeight * m.baseColor, m.specularColor);
-    P.metalMsScale = m.specularWeight * avgF * avgF;
-    float viewLoss = metal_ec(L, P.specAlpha, vc);
-    P.metalMsFactor = viewLoss / mxf(metal_avg_ec(L, P.specAlpha), 1e-12f);
-    float est = P.w[LM] * lum_pbr(P.metalMsScale) * viewLoss;
-    P.p[LM] += est;
-    P.metalMsProb = est / mxf(P.p[LM], 1e-12f);
-  }
-  P.p[LS] = P.w[LS] * lum_pbr(dielF);
-  P.p[LT] = P.w[LT] * (1.0f - transF) * lum_pbr(tint);
-  P.msRefl = splat(0.0f);
-  P.msTrans = splat(0.0f);
-  P.msViewLoss = 0.0f;
-  P.msRatio = 0.0f;
-  P.msProb = 0.0f;
-  if ((m.transmissionWeight > 0.0f) & (P.specAlpha >= 0.0016f) & (P.w[LT] > 0.0f) & (fabsf(P.refrEta - 1.0f) > 1e-6f)) {
-    float nonOpaque = P.w[LT] + P.w[LSS];
-    P.msRefl = (m.inside ? splat(1.0f) : m.specularColor) * nonOpaque;
-    P.msTrans = tint * P.w[LT] + splat(P.w[LSS]);
-    P.msRatio = ideal_ratio(L, P.refrEta, P.specAlpha);
-    P.msViewLoss = ideal_loss(L, P.refrEta, P.specAlpha, vc);
-    f3 avgScale = mix3(P.msTrans, P.msRefl, P.msRatio);
-    float est = lum_pbr(avgScale) * P.msViewLoss;
-    P.p[LT] += est;
-    P.msProb = est / mxf(P.p[LT], 1e-12f);
-  }
-  P.p[LD] = P.w[LD] * lum_pbr(m.baseColor) * P.dsComp;
-  P.p[LSS] = 0.0f;
-  if ((m.coatWeight > 0.0f) && !m.inside) {
-    float bs = maxc(P.coatBaseScale);
-    P.p[LM] *= bs; P.p[LS] *= bs; P.p[LT] *= bs; P.p[LD] *= bs; P.p[LSS] *= bs;
-  }
-  float total = P.p[0];
-#pragma unroll
-  for (int l = 1; l < LN; l++) total += P.p[l];
-  if (total <= 1e-8f) {
-#pragma unroll
-    for (int l = 0; l < LN; l++) P.p[l] = 0.0f;
-  } else {
-    float inv = 1.0f / total;
-#pragma unroll
-    for (int l = 0; l < LN; l++) P.p[l] *= inv;
-  }
-}
-
-__device__ __forceinline__ float substrate_fresnel(const Prep &P, float c, bool reflection) {
-  if (P.m.coatWeight <= 0.0f) return pbr_fresnel(c, P.m.relativeIor);
-  float tF = pbr_fresnel(c, P.transReflIor);
-  if (!reflection) return tF;
-  float oF = pbr_fresnel(c, 1.0f / P.specEta);
-  float nonOpaque = P.m.transmissionWeight + (1.0f - P.m.transmissionWeight) * 0.0f;
-  return mixf(oF, tF, nonOpaque);
-}
-
-__device__ f3 eval_bsdf(const Prep &P, f3 lightDir, float &pdf, const Lut &L) {
-  const Mat &m = P.m;
-  f3 v = P.v, l = to_local(P.basis, lightDir);
-  bool reflection = l.z * v.z > 0.0f;
-  f3 h = reflection ? normalize(l + v) : normalize(l + v * m.relativeIor);
-  if (h.z < 0.0f) h = -h;
-  f3 resp = splat(0.0f);
-  pdf = 0.0f;
-  float lp = 0.0f;
-  float vh = fabsf(dot(v, h));
-  bool evalDiel = (reflection & (P.w[LS] > 0.0f)) | (!reflection & (P.w[LT] > 0.0f));
-  bool evalMetal = reflection & (P.w[LM] > 0.0f);
-  f3 fr0 = splat(0.0f), fr1 = splat(0.0f);
-  if (evalDiel) fr0 = splat(substrate_fresnel(P, vh, reflection));
-  if (evalMetal) fr1 = metal_fresnel(vh, m.baseWeight * m.baseColor, m.specularColor);
-  float comp = P.dsComp * opaque_ec(L, P.specEta, P.specAlpha, fabsf(l.z));
-  if (reflection) {
-    if (P.w[LD] > 0.0f) { resp = resp + P.w[LD] * eval_diffuse(m, m.baseColor, v, l, comp, lp); pdf += P.p[LD] * lp; }
-    if (P.w[LS] > 0.0f) {
-      resp = resp + P.w[LS] * eval_refl(m.alphaX, m.alphaY, v, l, h, m.specularColor * fr0, lp);
-      pdf += P.p[LS] * lp;
-    }
-    if (P.w[LM] > 0.0f) {
-      resp = resp + P.w[LM] * eval_refl(m.alphaX, m.alphaY, v, l, h, m.specularWeight * fr1, lp);
-      if (P.metalMsFactor > 0.0f) {
-        float factor = mnf(P.metalMsFactor * metal_ec(L, P.specAlpha, l.z), 1.0f / mxf(l.z, 1e-8f));
-        resp = resp + P.w[LM] * P.metalMsScale * factor * kInvPi;
-        lp = mixf(lp, 0.5f * kInvPi, P.metalMsProb);
-      }
-      pdf += P.p[LM] * lp;
-    }
-  } else if (P.w[LT] > 0.0f) {
-    resp = resp + P.w[LT] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l, h, fr0, lp);
-    if (P.msProb > 0.0f) lp *= 1.0f - P.msProb;
-    pdf += P.p[LT] * lp;
-  }
-  if (P.msProb > 0.0f) {
-    float c = fabsf(l.z);
-    if (c > 0.0f) {
-      float ior = reflection ? P.refrEta : 1.0f / P.refrEta;
-      float ratio = reflection ? P.msRatio : 1.0f - P.msRatio;
-      float factors = P.msViewLoss * ideal_loss(L, ior, P.specAlpha, c) / mxf(ideal_avg(L, ior, P.specAlpha), 1e-12f);
-      float factor = mnf(factors * ratio, 1.0f / c);
-      resp = resp + (reflection ? P.msRefl : P.msTrans) * factor * kInvPi;
-      pdf += P.p[LT] * P.msProb * ratio * 0.5f * kInvPi;
-    }
-  }
-  if (m.coatWeight > 0.0f) resp = resp * coat_base_scale(P, l.z);
-  if (reflection & (P.w[LC] > 0.0f)) {
-    f3 coatF = splat(pbr_fresnel(vh, 1.0f / m.coatIor));
-    float cax, cay;
-    rough_alpha(m.coatRoughness, 0.0f, cax, cay);
-    resp = resp + P.w[LC] * eval_refl(cax, cay, v, l, h, coatF, lp);
-    pdf += P.p[LC] * lp;
-  }
-  return resp * fabsf(l.z);
-}
-
-__device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &dir, bool &transmission) {
-  const Mat &m = P.m;
-  f3 v = P.v;
-  transmission = false;
-  dir = P.basis.n;
-  float cdf[LN];
-  cdf[0] = P.p[0];
-#pragma unroll
-  for (int i = 1; i < LN; i++) cdf[i] = cdf[i - 1] + P.p[i];
-  if (cdf[LSS] <= 0.0f) return false;
-  f3 l;
-  if ((lobe < cdf[LF]) | ((lobe >= cdf[LT]) & (lobe < cdf[LD]))) {
-    l = cosine_hemisphere(rx, ry);
-  } else if (lobe < cdf[LC]) {
-    float cax, cay;
-    rough_alpha(m.coatRoughness, 0.0f, cax, cay);
-    f3 h = ggx_vndf(v, cax, cay, rx, ry);
-    if (h.z < 0.0f) h = -h;
-    l = normalize(reflect3(-v, h));
-  } else if (lobe < cdf[LS]) {
-    bool isMetal = lobe < cdf[LM];
-    float metalDraw = (lobe - cdf[LC]) / mxf(P.p[LM], 1e-12f);
-    if (isMetal & (metalDraw < P.metalMsProb)) {
-      float z = rx, radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
-    } else {
-      f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
-      if (h.z < 0.0f) h = -h;
-      l = normalize(reflect3(-v, h));
-    }
-  } else if (lobe < cdf[LT]) {
-    transmission = true;
-    float msDraw = (lobe - cdf[LS]) / mxf(P.p[LT], 1e-12f);
-    if ((P.msProb > 0.0f) & (msDraw < P.msProb)) {
-      float ratio = P.msRatio;
-      bool reflectMs = rx < ratio;
-      float z = reflectMs ? mnf(rx / ratio, 0.99999994f) : -mnf((rx - ratio) / (1.0f - ratio), 0.99999994f);
-      float radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
-      transmission = !reflectMs;
-    } else {
-      f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
-      if (h.z < 0.0f) h = -h;
-      l = refract3(-v, h, m.relativeIor);
-      if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
-      l = normalize(l);
-    }
-  } else {
-    transmission = true;
-    l = cosine_hemisphere(rx, ry);
-    l.z = -l.z;
-  }
-  dir = to_world(P.basis, l);
-  if ((l.z == 0.0f) | ((l.z < 0.0f) != transmission)) return false;
-  return true;
-}
-
-// ---------------------------------------------------------------- lights
-struct LSample {
-  f3 emission, direction;
-  float dist, pdf, selPmf;
-  uint32_t sourceType;
-  bool isDelta;
-};
-
-__device__ __forceinline__ float4 env_texel(const GpuScene &S, int x, int y) { return S.env[(size_t)y * S.envW + x]; }
-__device__ f3 texture_linear(const GpuScene &S, float u, float v) {
-  float fx = u * (float)S.envW - 0.5f, fy = v * (float)S.envH - 0.5f;
-  float flx = floorf(fx), fly = floorf(fy);
-  float tx = fx - flx, ty = fy - fly;
-  int ix = (int)flx, iy = (int)fly, mx = (int)S.envW - 1, my = (int)S.envH - 1;
-  int x0 = min(max(ix, 0), mx), x1 = min(max(ix + 1, 0), mx), y0 = min(max(iy, 0), my), y1 = min(max(iy + 1, 0), my);
-  float4 a = env_texel(S, x0, y0), b = env_texel(S, x1, y0), c = env_texel(S, x0, y1), d = env_texel(S, x1, y1);
-  f3 s1 = f3of(a), s2 = f3of(b), s3 = f3of(c), s4 = f3of(d);
-  return mix3(mix3(s1, s2, tx), mix3(s3, s4, tx), ty);
-}
-__device__ __forceinline__ void env_uv(f3 d, float &u, float &v) {
-  float phi = mtxfm::atan2f(-d.z, -d.x);
-  if (phi < 0.0f) phi += kTwoPi;
-  float theta = mtxfm::acosf(clampf(d.y, -1.0f, 1.0f));
-  u = phi * 0.5f * kInvPi;
-  v = theta * kInvPi;
-}
-__device__ __forceinline__ float2 cdf_at(const GpuScene &S, int x, int y) {
-  x = min(max(x, 0), (int)S.envW + 1);
-  y = min(max(y, 0), (int)S.envH);
-  return S.envCdf[(size_t)y * (S.envW + 2) + x];
-}
-__device__ __forceinline__ float env_solid_pdf(float px, float py, float v) {
-  if (!(v > 0.0f && v < 1.0f)) return 0.0f;
-  float sinT = mtxfm::sinf(mnf(v, 1.0f - v) * kPi);
-  return px * py * kInvPi2 / (2.0f * sinT);
-}
-__device__ float env_pdf(const GpuScene &S, float u, float v) {
-  int x = min(max((int)(u * (float)S.envW), 0), (int)S.envW - 1);
-  int y = min(max((int)(v * (float)S.envH), 0), (int)S.envH - 1);
-  return env_solid_pdf(cdf_at(S, 1 + x, y).y, cdf_at(S, 0, y).y, v);
-}
-__device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float &v, float &pdf) {
-  int H = (int)S.envH, W = (int)S.envW;
-  int left = 0, right = H + 1;
-  while (left < right) {
-    int mid = (left + right) >> 1;
-    if (cdf_at(S, 0, mid).x <= r0) left = mid + 1; else right = mid;
-  }
-  int vo = left - 1;
-  float2 a0 = cdf_at(S, 0, vo), a1 = cdf_at(S, 0, vo + 1);
-  v = ((float)vo + (r0 - a0.x) / (a1.x - a0.x)) / (float)H;
-  left = 0; right = W + 1;
-  while (left < right) {
-    int mid = (left + right) >> 1;
-    if (cdf_at(S, 1 + mid, vo).x <= r1) left = mid + 1; else right = mid;
-  }
-  int uo = left - 1;
-  float2 b0 = cdf_at(S, 1 + uo, vo), b1 = cdf_at(S, 1 + uo + 1, vo);
-  u = ((float)uo + (r1 - b0.x) / (b1.x - b0.x)) / (float)W;
-  float phi = u * kTwoPi, theta = v * kPi;
-  float ct = mtxfm::cosf(theta), st = mtxfm::sinf(theta), cp = mtxfm::cosf(phi), sp = mtxfm::sinf(phi);
-  pdf = env_solid_pdf(b0.y, a0.y, v);
-  return f3{-st * cp, ct, -st * sp};
-}
-__device__ void sample_rect(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
-  f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
-  f3 n = normalize(cross(U, V));
-  f3 lowerLeft = pos - 0.5f * (U + V);
-  f3 surf = lowerLeft + U * sx + V * sy;
-  f3 off = surf - p;
-  float d2 = dot(off, off);
-  r.dist = sqrtf(d2);
-  r.direction = off / r.dist;
-  float ec = dot(-r.direction, n);
-  float spread = clampf(L[13], 0.0f, 1.0f);
-  float sc = mtxfm::cosf(0.5f * kPi * spread);
-  if (ec <= 0.0f || ec < sc) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
-  r.emission = f3{L[10], L[11], L[12]};
-  r.pdf = d2 / mxf(L[14] * ec, 1e-8f);
-}
-__device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float rz, uint32_t cand, uint32_t count,
-                             LSample &ls) {
-  uint32_t cnt = max(count, 1u);
-  rz = (rz + (float)min(cand, cnt - 1u)) / (float)cnt;
-  ls.emission = splat(0.0f); ls.pdf = 0.0f; ls.selPmf = 0.0f; ls.dist = kInf; ls.isDelta = false; ls.sourceType = 0;
-  int n = S.choiceCount;
-  if (n <= 0) return;
-  float scaled = rz * (float)n;
-  int column = min((int)scaled, n - 1);
-  LightChoice c = S.choices[column];
-  int sel = fractf(scaled) < c.prob ? column : c.alias;
-  LightChoice ch = S.choices[sel];
-  if (ch.type == 0 && S.env) {
-    float u, v, pdf;
-    ls.direction = sample_env(S, rx, ry, u, v, pdf);
-    ls.pdf = pdf;
-    ls.emission = texture_linear(S, u, v);
-    ls.dist = kInf;
-    ls.selPmf = S.envPmf;
-    ls.pdf *= S.envPmf;
-  } else if (ch.type == 1) {
-    ls.sourceType = 2;
-    sample_rect(S.lights + (size_t)ch.index * 16, p, rx, ry, ls);
-    ls.selPmf = S.lightPmf[ch.index];
-    ls.pdf *= ls.selPmf;
-  }
-}
-__device__ float intersect_rect_light(const float *__restrict__ L, f3 ro, f3 rd, LSample &ls) {
-  ls.pdf = 0.0f; ls.emission = splat(0.0f);
-  f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]};
-  f3 pos = f3{L[1], L[2], L[3]} - U * 0.5f - V * 0.5f;
-  f3 n = normalize(cross(U, V));
-  if (dot(n, rd) > 0.0f) return kInf;
-  float t = (dot(n, pos) - dot(n, ro)) / dot(rd, n);
-  float d = kInf;
-  if (t > kEps) {
-    f3 vi = (ro + rd * t) - pos;
-    float a1 = dot(U / dot(U, U), vi);
-    if (a1 >= 0.0f && a1 <= 1.0f) {
-      float a2 = dot(V / dot(V, V), vi);
-      if (a2 >= 0.0f && a2 <= 1.0f) d = t;
-    }
-  }
-  if (d == kInf) return kInf;
-  float cosT = dot(-rd, n);
-  if (cosT >= mtxfm::cosf(L[13] * kPi / 2.0f)) {
-    ls.pdf = (d * d) / (L[14] * cosT);
-    ls.emission = f3{L[10], L[11], L[12]};
-    return d;
-  }
-  return kInf;
-}
-__device__ __forceinline__ f3 offset_surface_ray(f3 p, f3 gn, f3 dir) {
-  f3 n = dot(gn, dir) >= 0.0f ? gn : -gn;
-  const float lim = KF(1.0 / 32.0);
-  float pc[3] = {p.x, p.y, p.z}, nc[3] = {n.x, n.y, n.z}, o[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    int32_t io = (int32_t)(256.0f * nc[k]);
-    float shifted = intf(fint(pc[k]) + (pc[k] < 0.0f ? -io : io));
-    o[k] = fabsf(pc[k]) < lim ? pc[k] + nc[k] / 65536.0f : shifted;
-  }
-  return f3{o[0], o[1], o[2]};
-}
-
-// ---------------------------------------------------------------- integrator
-struct Surf {
-  f3 position, normal, faceNormal, shadowOffset;
-  int32_t tri, slot;
-  uint32_t material;
-  float color[3];
-  bool hit, albedoSampled;
-};
-
-__device__ void complete_surface(const GpuScene &S, const TraceHit &h, Surf &si) {
-  const ShadeSlot &sh = S.shade[h.slot];
-  const InstSlot &is = S.accel->slots[h.slot];
-  const TriRec r = is.tris[h.tri];
-  uint32_t src = (uint32_t)fint(r.v0.w);
-  uint32_t i0 = sh.indices[src * 3], i1 = sh.indices[src * 3 + 1], i2 = sh.indices[src * 3 + 2];
-  f3 p0 = f3of(r.v0), p1 = f3of(r.v1), p2 = f3of(r.v2);
-  f3 n0{sh.normals[i0 * 3], sh.normals[i0 * 3 + 1], sh.normals[i0 * 3 + 2]};
-  f3 n1{sh.normals[i1 * 3], sh.normals[i1 * 3 + 1], sh.normals[i1 * 3 + 2]};
-  f3 n2{sh.normals[i2 * 3], sh.normals[i2 * 3 + 1], sh.normals[i2 * 3 + 2]};
-  f3 b = h.bary, position = h.position;
-  f3 smooth = b.x * n0 + b.y * n1 + b.z * n2;
-  f3 normal = normalize(smooth);
-  f3 geometric = normalize(cross(p1 - p0, p2 - p0));
-  float A = dot(n2 - n0, p0 - p2), B = dot(n2 - n1, p1 - p2), C = dot(n1 - n0, p1 - p0);
-  float hh = A * b.x * (b.x - 1.0f) + (A + B + C) * b.x * b.y + B * b.y * (b.y - 1.0f);
-  if (dot(smooth, geometric) > 0.0f) {
-    float h0 = mxf(mxf(dot(p1 - p0, n0), dot(p2 - p0, n0)), 0.0f);
-    float h1 = mxf(mxf(dot(p0 - p1, n1), dot(p2 - p1, n1)), 0.0f);
-    float h2 = mxf(mxf(dot(p0 - p2, n2), dot(p1 - p2, n2)), 0.0f);
-    h0 = mxf(dot(p0 - position, n0) + h0, 0.0f);
-    h1 = mxf(dot(p1 - position, n1) + h1, 0.0f);
-    h2 = mxf(dot(p2 - position, n2) + h2, 0.0f);
-    hh = mxf(mnf(mnf(h0, h1), h2), hh * 0.5f);
-  } else {
-    float h0 = mxf(mxf(dot(p0 - p1, n0), dot(p0 - p2, n0)), 0.0f);
-    float h1 = mxf(mxf(dot(p1 - p0, n1), dot(p1 - p2, n1)), 0.0f);
-    float h2 = mxf(mxf(dot(p2 - p0, n2), dot(p2 - p1, n2)), 0.0f);
-    h0 = mxf(dot(position - p0, n0) + h0, 0.0f);
-    h1 = mxf(dot(position - p1, n1) + h1, 0.0f);
-    h2 = mxf(dot(position - p2, n2) + h2, 0.0f);
-    hh = mnf(-mnf(mnf(h0, h1), h2), hh * 0.5f);
-  }
-  Affine w2l{f3of(is.c0), f3of(is.c1), f3of(is.c2), f3of(is.c3)};
-  Affine l2w{f3of(sh.l2w0), f3of(sh.l2w1), f3of(sh.l2w2), f3of(sh.l2w3)};
-  si.hit = true;
-  si.tri = h.tri;
-  si.slot = h.slot;
-  si.material = sh.material;
-  si.color[0] = sh.color[0]; si.color[1] = sh.color[1]; si.color[2] = sh.color[2];
-  si.faceNormal = normalize(xform_normal_transposed(w2l, h.normal));
-  si.normal = normalize(xform_normal_transposed(w2l, normal));
-  si.shadowOffset = xform(l2w, smooth * hh, 0.0f);
-  si.position = xform(l2w, position, 1.0f);
-}
-
-__device__ __forceinline__ float nee_mis(uint32_t count, float lp, float sp) {
-  if (count <= 1u) return power_heuristic(lp, sp);
-  float w = (float)count * lp;
-  return w / mxf(w + sp, 1e-20f);
-}
-__device__ __forceinline__ float bsdf_mis(uint32_t count, float sp, float lp) {
-  if (count <= 1u) return power_heuristic(sp, lp);
-  float w = (float)count * lp;
-  return sp / mxf(sp + w, 1e-20f);
-}
-
-struct Shadow {
-  f3 ro, rd, contribution;
-  float maxDistance, risTarget, selPmf;
-  bool valid;
-};
 
 // kCount adds per-launch totals of the traversal work (box tests, triangle
 // tests, instance visits, closest hits, shadow rays) to acc.work[0..4].
@@ -892,277 +35,42 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   const uint32_t xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3, perXcd = (gridDim.x + 7u) >> 3;
   const uint32_t span = (nBlocks + 7u) >> 3;
   const uint32_t bEnd = min(nBlocks, (xcd + 1u) * span);
+  const int eventLimit = min((int)P.bounces * 3, 50);
   for (uint32_t pb = xcd * span + rank; pb < bEnd; pb += perXcd) {
     const uint32_t local = pb * kBlock + threadIdx.x;
     if (local >= P.pixelCount) continue;
     const uint32_t pix = P.pixelBase + local;
     const uint32_t px = pix % P.width, py = pix / P.width;
-    for (uint32_t k = 0; k < P.samples; k++) {
-      const uint32_t count = acc.count[pix];
-      Smp smp{px, py, P.width, count + P.sampleOffset, P.seed, P.logSPP, P.pattern, 0u};
-      const float Wf = (float)P.width, Hf = (float)P.height;
-      float uvx = ((float)px + 0.5f) / Wf, uvy = ((float)py + 0.5f) / Hf;
-      float jx, jy;
-      smp.s2(0u, jx, jy);
-      float cx = uvx + (jx - 0.5f) / Wf, cy = uvy + (jy - 0.5f) / Hf;
-      // initCameraRay (projection-inverse form) + optional thin lens.
-      f3 ro, rd;
-      {
-        float nx = cx * 2.0f - 1.0f, ny = cy * 2.0f - 1.0f;
-        float o4[4], c4[4];
+    double sums[13];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          float a = P.projInv[r] * nx; a = a + P.projInv[4 + r] * ny; a = a + P.projInv[8 + r] * -1.0f; a = a + P.projInv[12 + r] * 1.0f;
-          float b = P.projInv[r] * nx; b = b + P.projInv[4 + r] * ny; b = b + P.projInv[8 + r] * 1.0f; b = b + P.projInv[12 + r] * 1.0f;
-          o4[r] = a; c4[r] = b;
-        }
-        f3 dir{c4[0] * o4[3] - o4[0] * c4[3], c4[1] * o4[3] - o4[1] * c4[3], c4[2] * o4[3] - o4[2] * c4[3]};
-        f3 op{o4[0] / o4[3], o4[1] / o4[3], o4[2] / o4[3]};
-        Affine vi{f3{P.viewInv[0], P.viewInv[1], P.viewInv[2]}, f3{P.viewInv[4], P.viewInv[5], P.viewInv[6]},
-                  f3{P.viewInv[8], P.viewInv[9], P.viewInv[10]}, f3{P.viewInv[12], P.viewInv[13], P.viewInv[14]}};
-        ro = xform(vi, op, 1.0f);
-        rd = normalize(xform(vi, dir, 0.0f));
-      }
-      const f3 camPos{P.viewInv[12], P.viewInv[13], P.viewInv[14]};
-      f3 li = splat(0.0f), beta = splat(1.0f), auxAlbedo = splat(0.0f), auxNormal = splat(0.0f);
-      float alpha = 1.0f, scatPdf = 0.0f;
-      bool fromEye = true, abort = false, isDelta = false, nextRefr = false, isBlank = false;
-      Surf si;
-      si.tri = 0; si.slot = 0; si.hit = false; si.albedoSampled = false;
-      Mat ma;
-      const int bounceLimit = (int)P.bounces;
-      const int eventLimit = min(bounceLimit * 3, 50);
-      bool isNotTransparent = false;
-      int depth = 0;
+    for (int c = 0; c < 13; c++) sums[c] = acc.planes[(size_t)c * acc.pixels + pix];
+    const uint32_t count0 = acc.count[pix];
+    for (uint32_t k = 0; k < P.samples; k++) {
+      Smp smp = path_sampler(P, px, py, count0 + k + P.sampleOffset);
+      PathRegs pr;
+      init_path(P, smp, pr);
       for (int event = 0; event <= eventLimit; event++) {
         TraceHit h;
         rays++;
-        traverse<false, kCount>(av, ro, rd, kRayMaxDistance, event != 0, depth, si.tri, si.slot, st, h, &tc);
-        si.hit = h.hit;
+        traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, event != 0, trace_depth(event), pr.curTri,
+                                pr.curSlot, st, h, &tc);
         if (kCount) nHits += h.hit ? 1u : 0u;
-        if (h.hit) {
-          complete_surface(S, h, si);
-          ma = load_material(S.materials + (size_t)si.material * 32, si.color);
+        ShadowReq sr;
+        const bool alive = shade_event(S, P, L, pr, h, event, smp, sr);
+        if (sr.valid) {
+          TraceHit sh;
+          rays++;
+          if (kCount) nShadow++;
+          if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &tc))
+            pr.li = pr.li + sr.payload;
         }
-        if (event == 0) {
-          if (si.hit) {
-            auxAlbedo = ma.transmissionWeight > 0.5f ? splat(1.0f) : ma.baseColor;
-            auxNormal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
-            si.albedoSampled = ma.transmissionWeight > 0.5f;
-            isNotTransparent = ma.transmissionWeight < 0.001f;
-          }
-        } else {
-          for (uint32_t i = 0; i < S.lightCount; i++) {
-            LSample ls;
-            float d = intersect_rect_light(S.lights + (size_t)i * 16, ro, rd, ls);
-            ls.pdf *= S.lightPmf[i];
-            bool occluded = si.hit && d > len(si.position - ro);
-            if (d != kInf && !occluded) {
-              float mis = bsdf_mis(P.candidates, scatPdf, ls.pdf);
-              li = li + mis * beta * ls.emission;
-              alpha = 1.0f;
-              abort = true;
-            }
-          }
-          if (S.env && !si.hit) {
-            float u, v;
-            env_uv(rd, u, v);
-            f3 em = texture_linear(S, u, v);
-            if (nextRefr) {
-              float mis = isDelta ? 1.0f : bsdf_mis(P.candidates, scatPdf, env_pdf(S, u, v) * S.envPmf);
-              li = li + mis * beta * em * 1.0f;
-            } else {
-              float mis = bsdf_mis(P.candidates, scatPdf, env_pdf(S, u, v) * S.envPmf);
-              li = li + beta * em * mis;
-            }
-            alpha = 1.0f;
-            abort = true;
-          }
-          depth++;
-        }
-        if (isNotTransparent && event > bounceLimit) abort = true;
-        if (abort || event >= eventLimit) break;
-        smp.bounce = (uint32_t)max(depth, 0);
-        if (!si.hit) {
-          if (fromEye) {
-            f3 bg = splat(0.0f);
-            float ba = 0.0f;
-            if (S.env) {
-              float u, v;
-              env_uv(rd, u, v);
-              bg = texture_linear(S, u, v);
-              ba = 1.0f;
-            }
-            li = beta * bg * ba;
-            alpha = ba;
-            isBlank = ba < 0.001f;
-          }
-          break;
-        }
-        // SampleMaterial (surface vertex, no media, no emission).
-        {
-          float ndv1 = -dot(si.faceNormal, rd), ndv2 = -dot(si.normal, rd);
-          bool useFaceN = (ndv1 >= 0.0f) & (ndv2 < 0.0f);
-          float ndv = useFaceN ? ndv1 : ndv2;
-          f3 N = useFaceN ? si.faceNormal : si.normal;
-          ma.inside = ndv <= 0.0f;
-          ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
-          f3 ff = ndv >= 0.0f ? N : -N;
-          isDelta = false;
-          Prep PR;
-          prepare(PR, ma, -rd, ff, L);
-          const uint32_t cc = max(P.candidates, 1u);
-          Shadow sel;
-          sel.valid = false;
-          float selTarget = 0.0f, wsum = 0.0f;
-          f3 sdir = splat(0.0f), sresp = splat(0.0f);
-          float spdf = 0.0f;
-          bool strans = false, discrete = false;
-          for (uint32_t ev = 0; ev <= cc; ev++) {
-            const bool cont = ev == cc;
-            LSample ls;
-            ls.pdf = 0.0f; ls.selPmf = 0.0f; ls.isDelta = false; ls.sourceType = 0; ls.dist = kInf;
-            f3 ldir;
-            float lpdf = 0.0f;
-            f3 resp = splat(0.0f);
-            bool disc = false;
-            if (cont) {
-              float ucw = 1.0f;
-              bool valid = sel.valid;
-              if (cc > 1u) {
-                valid = valid & (selTarget > 0.0f);
-                if (valid) ucw = sel.selPmf * wsum / ((float)cc * selTarget);
-              }
-              if (valid) {
-                TraceHit sh;
-                rays++;
-                float tMax = fminf(fmaxf(sel.maxDistance, 0.0f), kRayMaxDistance);
-                if (kCount) nShadow++;
-                if (!traverse<true, kCount>(av, sel.ro, sel.rd, tMax, true, 1, si.tri, si.slot, st, sh, &tc))
-                  li = li + sel.contribution * ucw;
-              }
-              float d0, d1;
-              smp.s2(smp.bdim(3u), d0, d1);
-              float lobe = smp.s1(smp.bdim(2u));
-              disc = (fabsf(PR.m.relativeIor - 1.0f) <= 1e-6f) & (PR.p[LT] >= 1.0f - 1e-6f);
-              if (disc) {
-                ldir = to_world(PR.basis, -PR.v);
-                resp = PR.m.transmissionColor * coat_base_scale(PR, -PR.v.z);
-                lpdf = 1.0f;
-                strans = true;
-              } else {
-                bool tr;
-                if (!sample_bsdf(PR, d0, d1, lobe, ldir, tr)) { spdf = 0.0f; break; }
-                strans = tr;
-              }
-              discrete = disc;
-            } else {
-              float a, b;
-              smp.s2(smp.ldim(ev, 1u), a, b);
-              float c = smp.s1(smp.ldim(ev, 0u));
-              sample_light(S, si.position, a, b, c, ev, cc, ls);
-              ldir = ls.direction;
-              if ((ls.pdf <= 0.0f) | (ls.selPmf <= 0.0f) | (maxc(ls.emission) <= 0.0f)) continue;
-              if ((dot(ff, ldir) <= 0.0f) & (ma.transmissionWeight <= 0.0f)) continue;
-            }
-            if (!disc) resp = eval_bsdf(PR, ldir, lpdf, L);
-            if (cont) {
-              sdir = ldir;
-              sresp = resp;
-              spdf = lpdf;
-              break;
-            }
-            // finishDirectLight
-            f3 Li = ls.emission;
-            if ((ls.pdf <= 0.0f) | (ls.selPmf <= 0.0f) | (maxc(Li) <= 0.0f)) continue;
-            if ((lpdf <= 0.0f) | (maxc(resp) <= 0.0f)) continue;
-            f3 origin = offset_surface_ray(si.position, si.faceNormal, ldir);
-            float nl = dot(si.normal, ldir);
-            f3 gn = nl < 0.0f ? -si.faceNormal : si.faceNormal;
-            nl = fabsf(nl);
-            float gl = dot(gn, ldir);
-            float amount = nl < 0.1f ? clampf(2.0f - (gl + nl) / 0.1f, 0.0f, 1.0f) : clampf(1.0f - gl / 0.1f, 0.0f, 1.0f);
-            origin = origin + si.shadowOffset * amount;
-            if ((dot(ff, ldir) < 0.0f) & (ls.sourceType == 0u)) {
-              if (S.env) {
-                float u, v;
-                env_uv(ldir, u, v);
-                Li = texture_linear(S, u, v);
-              } else {
-                Li = splat(0.0f);
-              }
-            }
-            float mis = ls.isDelta ? 1.0f : nee_mis(P.candidates, ls.pdf, lpdf);
-            if (!(mis > 0.0f)) continue;
-            f3 un = Li * resp / ls.pdf * beta;
-            Shadow cand;
-            cand.risTarget = mxf(dot(vmax(un * ls.selPmf, splat(0.0f)), f3{0.2126f, 0.7152f, 0.0722f}), 0.0f);
-            cand.ro = origin;
-            cand.rd = ldir;
-            cand.maxDistance = ls.dist - kEps;
-            cand.contribution = mis * un;
-            cand.selPmf = ls.selPmf;
-            cand.valid = true;
-            if (cc == 1u) { sel = cand; continue; }
-            float sw = cand.risTarget / mxf(cand.selPmf, 1e-20f);
-            if (!(sw > 0.0f) | (sw != sw)) continue;
-            wsum += sw;
-            if (!sel.valid || smp.candidate_random(ev) * wsum <= sw) {
-              sel = cand;
-              selTarget = cand.risTarget;
-            }
-          }
-          bool finite = isfinite(sresp.x) && isfinite(sresp.y) && isfinite(sresp.z) && fabsf(sresp.x) < 3.402823e38f &&
-                        fabsf(sresp.y) < 3.402823e38f && fabsf(sresp.z) < 3.402823e38f;
-          if (!(spdf > 0.0f) | !finite | (dot(sdir, sdir) < 0.5f)) {
-            scatPdf = 0.0f;
-            abort = true;
-          } else {
-            isDelta = discrete;
-            beta = beta * (sresp / spdf);
-            if (beta.x == 0.0f && beta.y == 0.0f && beta.z == 0.0f) {
-              abort = true;
-            } else {
-              nextRefr = strans;
-              ro = si.position + kEps * sdir;
-              scatPdf = spdf;
-              rd = sdir;
-              fromEye = false;
-            }
-          }
-        }
-        if (!si.albedoSampled) {
-          if (!(ma.transmissionWeight > 0.5f || (ma.specularRoughness <= 0.02f && ma.baseMetalness > 0.98f))) {
-            auxAlbedo = ma.baseColor;
-            auxNormal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
-            si.albedoSampled = true;
-          }
-          if (depth > 4) si.albedoSampled = true;
-        }
-        if (depth >= 2) {
-          float q = mxf(0.0f, 1.0f - maxc(beta));
-          if (smp.s1(smp.bdim(7u)) < q) {
-            alpha = 1.0f;
-            abort = true;
-          } else {
-            beta = beta / (1.0f - q);
-          }
-        }
-        if (abort) break;
+        if (!alive) break;
       }
-      if (P.clamping > 0.0f) li = clamp3(li, -P.clamping, P.clamping);
-      if (!(li.x < kInf && li.x > -kEps)) { li = splat(0.0f); alpha = 1.0f; }
-      const size_t N = acc.pixels;
-      double *pl = acc.planes;
-      pl[pix] += (double)li.x; pl[N + pix] += (double)li.y; pl[2 * N + pix] += (double)li.z; pl[3 * N + pix] += (double)alpha;
-      pl[4 * N + pix] += (double)auxAlbedo.x; pl[5 * N + pix] += (double)auxAlbedo.y; pl[6 * N + pix] += (double)auxAlbedo.z;
-      pl[7 * N + pix] += isBlank ? 0.0 : 1.0;
-      pl[8 * N + pix] += (double)auxNormal.x; pl[9 * N + pix] += (double)auxNormal.y; pl[10 * N + pix] += (double)auxNormal.z;
-      pl[11 * N + pix] += 1.0;
-      double lum = (double)dot(li, f3{0.2126f, 0.7152f, 0.0722f});
-      pl[12 * N + pix] += lum * lum;
-      acc.count[pix] = count + 1u;
+      add_sample(sums, finish_path(P, pr));
     }
+#pragma unroll
+    for (int c = 0; c < 13; c++) acc.planes[(size_t)c * acc.pixels + pix] = sums[c];
+    acc.count[pix] = count0 + P.samples;
   }
   // One atomic per wave for the ray counter.
   for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);

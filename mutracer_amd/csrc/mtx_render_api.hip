@@ -41,6 +41,9 @@ struct TargetObj {
   DevBuf<uint32_t> count;
   DevBuf<uint64_t> rays;
   uint32_t samplesDone = 0;
+  uint32_t kernel = MTX_KERNEL_WAVEFRONT;
+  WfBuffers::Owner wf;
+  WfConfig wfConfig;
   hipStream_t stream = nullptr;
   std::mutex busy;
 };
@@ -287,6 +290,7 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
               "invalid target size");
   MTX_REQUIRE(s.sampling_pattern <= 1, MTX_INVALID_ARGUMENT, "sampling_pattern must be 0 (sobol) or 1 (zsobol)");
   MTX_REQUIRE(s.bounces <= 50, MTX_INVALID_ARGUMENT, "bounces above 50");
+  MTX_REQUIRE(s.kernel <= MTX_KERNEL_MEGAKERNEL, MTX_INVALID_ARGUMENT, "unknown kernel mode");
   auto T = std::make_shared<TargetObj>();
   T->scene = scene;
   RenderParams &p = T->params;
@@ -302,6 +306,8 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
   p.clamping = s.clamping;
   p.pixelBase = 0;
   p.pixelCount = s.width * s.height;
+  T->kernel = s.kernel;
+  if (s.max_paths_in_flight) T->wfConfig.maxPaths = s.max_paths_in_flight;
   const size_t N = (size_t)s.width * s.height;
   T->planes.alloc(N * 13);
   T->count.alloc(N);
@@ -315,7 +321,10 @@ void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work
   RenderParams p = T.params;
   p.samples = samples;
   Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height, work};
-  launch_render(T.scene->view, p, acc, s);
+  if (T.kernel == MTX_KERNEL_MEGAKERNEL)
+    launch_render(T.scene->view, p, acc, s);
+  else
+    launch_render_wavefront(T.scene->view, p, acc, T.wf, T.wfConfig, s);
   T.stream = s;
   T.samplesDone += samples;
 }

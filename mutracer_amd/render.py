@@ -23,6 +23,7 @@ from .sampling import sampling_pattern_code
 _P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
 
 PIXELS = dict(color=0, albedo=1, normal=2, variance=3)
+KERNELS = dict(wavefront=0, megakernel=1)
 
 
 class RenderScene:
@@ -102,7 +103,8 @@ class RenderTarget:
     """FP64 progressive accumulation of one view of a RenderScene."""
 
     def __init__(self, render_scene, width, height, bounces=8, direct_light_candidates=1, pattern="sobol",
-                 sequence_seed=0, max_samples=1, sample_sequence_offset=0, clamping=0.0, camera=None):
+                 sequence_seed=0, max_samples=1, sample_sequence_offset=0, clamping=0.0, camera=None,
+                 kernel="wavefront", max_paths_in_flight=0):
         self.scene = render_scene
         self.width, self.height = width, height
         s = RenderSettings()
@@ -114,6 +116,8 @@ class RenderTarget:
         s.max_samples = max_samples
         s.sample_sequence_offset = sample_sequence_offset
         s.clamping = clamping
+        s.kernel = KERNELS[kernel]
+        s.max_paths_in_flight = max_paths_in_flight
         self.settings = s
         cam = camera_desc(camera or render_scene.scene.camera, width, height)
         self.handle = C.c_uint64(0)

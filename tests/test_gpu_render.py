@@ -2,12 +2,13 @@
 
 Both sides draw the same Owen-scrambled Sobol / ZSobol samples (pure
 functions of pixel, sample index, dimension and seed), traverse the same
-BVHs with bit-identical hit rules and run the same integrator, so at equal
-spp the images agree up to transcendental-function rounding (device libm vs
-glibc), which occasionally flips one path's branch. Bar (written here):
-  - per-pixel sample counts: exact;
-  - radiance: relMSE = mean((gpu - ref)^2 / (ref^2 + 1e-2)) <= 1e-4;
-  - first-hit albedo/normal AOV means: max abs difference <= 1e-4.
+BVHs with bit-identical hit rules, evaluate the same elementary functions
+(include/mtx_fmath.h) and run the same integrator with the same FP64
+accumulation order, so at equal spp the accumulators agree bit for bit.
+Bar (written here): FP64 sums and sample counts identical; the radiance
+relMSE = mean((gpu - ref)^2 / (ref^2 + 1e-2)) is also reported and must be
+<= 1e-4 (the tolerance the accuracy target states). Every case runs on both
+the wavefront kernels (default) and the megakernel.
 """
 import numpy as np
 import pytest
@@ -27,6 +28,9 @@ def env():
 
 def rel_mse(a, b):
     return float(np.mean((a - b) ** 2 / (b * b + 1e-2)))
+
+
+KERNELS = ["wavefront", "megakernel"]
 
 
 def _both(orc, render, sc, w, h, spp, batches=None, **kw):
@@ -53,32 +57,38 @@ def _check(sums, counts, ref, spp):
     assert err <= REL_MSE_MAX, f"radiance relMSE {err:.3e} > {REL_MSE_MAX}"
     assert np.abs(g[..., 4:11] - r[..., 4:11]).max() <= 1e-4, "albedo/normal AOVs differ"
     assert np.array_equal(g[..., 11], r[..., 11])
+    assert np.array_equal(sums, ref), "FP64 accumulators differ from the oracle's"
     return err
 
 
-def test_cornell_sobol(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_cornell_sobol(env, orc, kernel):
     torch, render, scene = env
     sc = scene.cornell_box()
-    sums, counts, ref, _ = _both(orc, render, sc, 96, 96, 32, bounces=8, max_samples=32)
+    sums, counts, ref, _ = _both(orc, render, sc, 96, 96, 32, bounces=8, max_samples=32, kernel=kernel)
     _check(sums, counts, ref, 32)
 
 
-def test_cornell_zsobol_candidates(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_cornell_zsobol_candidates(env, orc, kernel):
     torch, render, scene = env
     sc = scene.cornell_box()
     sums, counts, ref, _ = _both(orc, render, sc, 64, 64, 16, bounces=8, pattern="zsobol", max_samples=64,
-                                 direct_light_candidates=4, sequence_seed=7)
+                                 direct_light_candidates=4, sequence_seed=7, kernel=kernel)
     _check(sums, counts, ref, 16)
 
 
-def test_batches_equal_single_launch(env, orc):
-    """Samples split over launches continue the per-pixel Sobol index."""
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_batches_equal_single_launch(env, orc, kernel):
+    """Samples split over launches (and over wavefront generations) continue
+    the per-pixel Sobol index."""
     torch, render, scene = env
     sc = scene.cornell_box()
     rs = render.RenderScene(sc)
-    a = render.RenderTarget(rs, 48, 40, bounces=6, max_samples=12)
+    a = render.RenderTarget(rs, 48, 40, bounces=6, max_samples=12, kernel=kernel)
     a.render(12)
-    b = render.RenderTarget(rs, 48, 40, bounces=6, max_samples=12)
+    b = render.RenderTarget(rs, 48, 40, bounces=6, max_samples=12, kernel=kernel,
+                            max_paths_in_flight=48 * 40 * 2 + 7)
     for n in (5, 1, 6):
         b.render(n)
     sa, ca = a.read_accumulators()
@@ -88,19 +98,22 @@ def test_batches_equal_single_launch(env, orc):
     assert b.stats()["samples"] == 12
 
 
-def test_instanced_field_env(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_instanced_field_env(env, orc, kernel):
     """Reduced config-2 scene: metal / glass / coated instances, sky+sun env."""
     torch, render, scene = env
     sc = scene.instanced_field(n_instances=9, freq=10, env_size=(256, 128))
-    sums, counts, ref, _ = _both(orc, render, sc, 96, 54, 8, bounces=12, max_samples=8)
+    sums, counts, ref, _ = _both(orc, render, sc, 96, 54, 8, bounces=12, max_samples=8, kernel=kernel,
+                                 direct_light_candidates=4)
     _check(sums, counts, ref, 8)
 
 
-def test_clamping_and_offset(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_clamping_and_offset(env, orc, kernel):
     torch, render, scene = env
     sc = scene.instanced_field(n_instances=4, freq=6, env_size=(64, 32))
     sums, counts, ref, _ = _both(orc, render, sc, 40, 30, 6, bounces=5, max_samples=6, clamping=4.0,
-                                 sample_sequence_offset=17, direct_light_candidates=2)
+                                 sample_sequence_offset=17, direct_light_candidates=2, kernel=kernel)
     _check(sums, counts, ref, 6)
 
 
@@ -117,13 +130,14 @@ def test_read_pixels_resolve(env, orc):
     assert t.stats()["rays"] > 32 * 24 * 4
 
 
-def test_counted_launch_matches_plain(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_counted_launch_matches_plain(env, orc, kernel):
     """The instrumented launch renders the same samples as the plain one."""
     torch, render, scene = env
     sc = scene.instanced_field(n_instances=6, freq=8, env_size=(64, 32))
     rs = render.RenderScene(sc)
-    a = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4)
-    b = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4)
+    a = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4, kernel=kernel)
+    b = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4, kernel=kernel)
     a.render(4)
     work = b.render_counted(4)
     sa, _ = a.read_accumulators()
@@ -133,3 +147,17 @@ def test_counted_launch_matches_plain(env, orc):
     assert 0 < work["hits"] <= rays - work["shadow_rays"]
     assert work["box_tests"] > work["tri_tests"] > 0
     assert work["instance_visits"] >= work["hits"]
+
+
+def test_wavefront_equals_megakernel_large(env, orc):
+    """Both kernel forms on a 1M-triangle-class scene at 256x144, 4 spp."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=64, freq=28, env_size=(1024, 512))
+    rs = render.RenderScene(sc)
+    out = []
+    for kernel in KERNELS:
+        t = render.RenderTarget(rs, 256, 144, bounces=12, direct_light_candidates=4, max_samples=4, kernel=kernel)
+        t.render(4)
+        out.append(t.read_accumulators())
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
