@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
+    ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the path shards by "
@@ -116,6 +117,8 @@ def main():
     tgt = render.RenderTarget(rs, WIDTH, HEIGHT, bounces=DEPTH, direct_light_candidates=CANDIDATES,
                               pattern="sobol", sequence_seed=0, max_samples=SPP, kernel=args.kernel,
                               max_paths_in_flight=args.max_paths)
+    if args.tuning:
+        tgt.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
     stream = torch.cuda.current_stream()
 
     tgt.restart_samples()
@@ -168,7 +171,8 @@ def main():
                                "1024x512 sky+sun env, OpenPBR metal/glass/coat mix",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "depth": DEPTH,
                    "direct_light_candidates": CANDIDATES, "sampling": "sobol seed 0",
-                   "triangles": info["triangles"], "accumulation": "fp64", "kernel": args.kernel},
+                   "triangles": info["triangles"], "accumulation": "fp64", "kernel": args.kernel,
+                   "tuning": tgt.tuning() if args.kernel == "wavefront" else None},
         "mrays_per_s": round(rays_per_step / (ms_per_step / 1e3) / 1e6, 3),
         "rays_per_step": rays_per_step,
         "scene_commit_ms": round(build_ms, 1),

@@ -289,6 +289,20 @@ int32_t mtx_target_read_pixels(mtx_handle target, uint32_t kind, float *pixels, 
 int32_t mtx_target_read_accumulators(mtx_handle target, double *sums, size_t capacity_doubles, uint32_t *counts,
                                      size_t capacity_counts);
 
+/* Launch shape of the wavefront kernels (what the autotuner searches). No
+ * reference counterpart: the reference's equivalent knobs are its tile size
+ * and time budget (TileRender.ts:302-490). */
+typedef struct mtx_wavefront_tuning {
+  uint32_t struct_size;
+  uint32_t max_paths_in_flight;    /* paths per generation (0 = keep) */
+  uint32_t trace_kernel;           /* 0 one ray per thread, 1 persistent lane-refill walker */
+  uint32_t lds_stack;              /* walker LDS stack entries per lane: 16, 24 or 32 */
+  uint32_t trace_oversubscription; /* trace grid = CUs x resident blocks x this (>= 1) */
+  uint32_t shade_oversubscription; /* shade grid likewise (>= 1) */
+} mtx_wavefront_tuning;
+int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
+int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);
+
 /* stats[0] rays traced, [1] samples per pixel completed. */
 int32_t mtx_target_stats(mtx_handle target, uint64_t stats[2]);
 int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera);

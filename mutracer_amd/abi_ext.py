@@ -29,6 +29,12 @@ class CameraDesc(C.Structure):
                 ("view_inverse", C.c_float * 16), ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]
 
 
+class WavefrontTuning(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_paths_in_flight", C.c_uint32), ("trace_kernel", C.c_uint32),
+                ("lds_stack", C.c_uint32), ("trace_oversubscription", C.c_uint32),
+                ("shade_oversubscription", C.c_uint32)]
+
+
 def declare(L):
     u32, i32, u64, vp, sz = C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p, C.c_size_t
     P = C.POINTER
@@ -47,6 +53,8 @@ def declare(L):
         "mtx_target_set_camera": (i32, [u64, vp]),
         "mtx_target_reset": (i32, [u64]),
         "mtx_target_destroy": (i32, [u64]),
+        "mtx_target_set_tuning": (i32, [u64, vp]),
+        "mtx_target_get_tuning": (i32, [u64, vp]),
         "mtx_math_eval": (i32, [u32, vp, vp, vp, u64, vp]),
     }
     for name, (res, args) in sig.items():

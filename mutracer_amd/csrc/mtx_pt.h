@@ -70,10 +70,12 @@ struct TargetObj;
 //   hitA closest hit t, barycentrics           hitB (instance slot or -1, triangle)
 //   qA/qB path-id queues of even/odd events
 //   s0..s3 shadow queue: (origin, tMax), (direction, path id), (payload, tri), slot
-//   counts[e] = length of event e's queue, counts[kWfShadowBase + e] = its shadow queue
+//   counts[e] = length of event e's queue, counts[kWfShadowBase + e] = its shadow queue,
+//   counts[kWfFetchBase + 2e (+1)] = next entry the persistent trace (shadow) kernel takes
 constexpr int kWfMaxEvents = 51;
 constexpr int kWfShadowBase = kWfMaxEvents + 1;
-constexpr int kWfCounters = kWfShadowBase + kWfMaxEvents + 1;
+constexpr int kWfFetchBase = kWfShadowBase + kWfMaxEvents + 1;
+constexpr int kWfCounters = kWfFetchBase + 2 * (kWfMaxEvents + 1);
 struct WfBuffers {
   float4 *o, *d, *li, *beta, *alb, *nrm, *hitA;
   int2 *hitB;
@@ -97,9 +99,11 @@ struct WfBuffers {
 
 // Launch shape of the wavefront kernels (the autotuner's search space).
 struct WfConfig {
-  uint32_t maxPaths = 8u << 20;  // paths in flight per generation
+  uint32_t maxPaths = 1u << 25;  // paths in flight per generation
   int traceOversub = 1;          // trace/shadow grid = CUs * resident blocks * this
   int shadeOversub = 1;
+  bool walker = true;            // persistent lane-refill traversal (else one ray per thread)
+  int ldsStack = 16;             // LDS stack entries per lane for the walker (16, 24 or 32)
 };
 
 void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s);

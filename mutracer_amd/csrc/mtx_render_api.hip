@@ -516,6 +516,41 @@ int32_t mtx_target_reset(mtx_handle target) {
   API_CATCH(nullptr, 0)
 }
 
+int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) {
+  API_TRY
+  MTX_REQUIRE(t && t->struct_size >= sizeof(mtx_wavefront_tuning), MTX_INVALID_ARGUMENT, "bad tuning struct");
+  MTX_REQUIRE(t->trace_kernel <= 1, MTX_INVALID_ARGUMENT, "trace_kernel must be 0 or 1");
+  MTX_REQUIRE(t->lds_stack == 16 || t->lds_stack == 24 || t->lds_stack == 32, MTX_INVALID_ARGUMENT,
+              "lds_stack must be 16, 24 or 32");
+  MTX_REQUIRE(t->trace_oversubscription >= 1 && t->trace_oversubscription <= 16 && t->shade_oversubscription >= 1 &&
+                  t->shade_oversubscription <= 16,
+              MTX_INVALID_ARGUMENT, "oversubscription must be in 1..16");
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  if (t->max_paths_in_flight) T->wfConfig.maxPaths = t->max_paths_in_flight;
+  T->wfConfig.walker = t->trace_kernel == 1;
+  T->wfConfig.ldsStack = (int)t->lds_stack;
+  T->wfConfig.traceOversub = (int)t->trace_oversubscription;
+  T->wfConfig.shadeOversub = (int)t->shade_oversubscription;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
+  API_TRY
+  MTX_REQUIRE(t, MTX_INVALID_ARGUMENT, "tuning is NULL");
+  auto T = get_target(target);
+  t->struct_size = sizeof(mtx_wavefront_tuning);
+  t->max_paths_in_flight = T->wfConfig.maxPaths;
+  t->trace_kernel = T->wfConfig.walker ? 1u : 0u;
+  t->lds_stack = (uint32_t)T->wfConfig.ldsStack;
+  t->trace_oversubscription = (uint32_t)T->wfConfig.traceOversub;
+  t->shade_oversubscription = (uint32_t)T->wfConfig.shadeOversub;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
 int32_t mtx_target_destroy(mtx_handle target) {
   std::lock_guard<std::mutex> g(gRenderMutex);
   return gTargets.erase(target) ? MTX_OK : MTX_INVALID_HANDLE;

@@ -22,6 +22,7 @@
 // identical accumulators.
 #include "mtx_registry.h"
 #include "mtx_shade.h"
+#include "mtx_walk.h"
 
 namespace mtx {
 
@@ -215,6 +216,85 @@ k_wf_shadow(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t
   }
 }
 
+// Persistent closest-hit / any-hit kernels over Walker: each lane takes a
+// new queue entry (wave-aggregated atomic on counts[kWfFetchBase + 2e (+1)])
+// as soon as its previous ray has finished.
+template <bool kShadow, bool kCount, int kLds>
+__global__ void __launch_bounds__(kWfTraceBlock)
+k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *__restrict__ spill, Accumulators acc) {
+  __shared__ int32_t lds[kLds * kWfTraceBlock];
+  const uint32_t n = W.counts[kShadow ? kWfShadowBase + event : event];
+  if (n == 0u) return;
+  uint32_t *fetch = W.counts + kWfFetchBase + 2 * event + (kShadow ? 1 : 0);
+  WalkStack<kLds> st{lds + threadIdx.x, kWfTraceBlock,
+                     spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLds)};
+  const AccelView av = *accel;
+  const uint32_t *q = queue_of(W, event);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)n);
+  Walker<kShadow, kCount> w;
+  bool have = false, exhausted = false;
+  uint32_t entry = 0;
+  uint64_t totals[4] = {0, 0, 0, 0};
+  const bool doubleSided = kShadow ? true : event != 0;
+  const int bounces = kShadow ? 1 : trace_depth(event);
+  while (true) {
+    const bool need = !have && !exhausted;
+    const uint32_t idx = wave_append(fetch, need);
+    if (need) {
+      if (idx < n) {
+        entry = idx;
+        if (kShadow) {
+          const float4 a = W.s0[idx], b = W.s1[idx], c = W.s2[idx];
+          w.init(av, f3of(a), f3of(b), a.w, doubleSided, bounces, fint(c.w), W.s3[idx], st);
+        } else {
+          const uint32_t p = q[idx];
+          const float4 o = W.o[p], d = W.d[p];
+          w.init(av, f3of(o), f3of(d), kRayMaxDistance, doubleSided, bounces, fint(W.beta[p].w), fint(W.alb[p].w),
+                 st);
+        }
+        have = true;
+      } else {
+        exhausted = true;
+      }
+    }
+    if (!__any(have)) break;
+    if (have && (w.done || w.step(av, st))) {
+      have = false;
+      if (kShadow) {
+        if (!w.hit) {
+          const uint32_t p = (uint32_t)fint(W.s1[entry].w);
+          const float4 c = W.s2[entry];
+          float4 l = W.li[p];
+          l.x = l.x + c.x;
+          l.y = l.y + c.y;
+          l.z = l.z + c.z;
+          W.li[p] = l;
+        }
+      } else {
+        const uint32_t p = q[entry];
+        W.hitA[p] = make_float4(w.t, w.bary.x, w.bary.y, w.bary.z);
+        W.hitB[p] = make_int2(w.hit ? w.hitSlot : -1, w.hitTri);
+      }
+      if (kCount) {
+        totals[0] += w.cnt.boxes;
+        totals[1] += w.cnt.tris;
+        totals[2] += w.cnt.instances;
+        totals[3] += (!kShadow && w.hit) ? 1u : 0u;
+      }
+    }
+  }
+  if (kCount) {
+    if (kShadow && blockIdx.x == 0 && threadIdx.x == 0) totals[3] = n;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint64_t v = totals[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+      const int slot = k < 3 ? k : (kShadow ? 4 : 3);
+      if (lane_id() == 0 && v) atomicAdd((unsigned long long *)acc.work + slot, (unsigned long long)v);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_wf_accumulate(const RenderParams P, Accumulators acc, WfBuffers W) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t local = blockIdx.x * blockDim.x + threadIdx.x; local < P.pixelCount; local += stride) {
@@ -278,6 +358,27 @@ WfBuffers WfBuffers::Owner::view(uint32_t nPaths, uint32_t gen) const {
   return W;
 }
 
+template <bool kShadow, int kLds>
+static void launch_walk_lds(uint32_t grid, hipStream_t s, const AccelView *accel, const WfBuffers &W, int e,
+                            int32_t *spill, const Accumulators &acc) {
+  if (acc.work)
+    hipLaunchKernelGGL((k_wf_walk<kShadow, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, accel, W, e, spill, acc);
+  else
+    hipLaunchKernelGGL((k_wf_walk<kShadow, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, accel, W, e, spill, acc);
+}
+template <bool kShadow>
+static void launch_walk(const WfConfig &cfg, uint32_t grid, hipStream_t s, const AccelView *accel, const WfBuffers &W,
+                        int e, int32_t *spill, const Accumulators &acc) {
+  if (cfg.ldsStack <= 16) launch_walk_lds<kShadow, 16>(grid, s, accel, W, e, spill, acc);
+  else if (cfg.ldsStack <= 24) launch_walk_lds<kShadow, 24>(grid, s, accel, W, e, spill, acc);
+  else launch_walk_lds<kShadow, 32>(grid, s, accel, W, e, spill, acc);
+}
+static const void *walk_kernel(int lds) {
+  if (lds <= 16) return (const void *)k_wf_walk<false, false, 16>;
+  if (lds <= 24) return (const void *)k_wf_walk<false, false, 24>;
+  return (const void *)k_wf_walk<false, false, 32>;
+}
+
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, const Accumulators &acc,
                              WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s) {
   if (!p0.pixelCount || !p0.samples) return;
@@ -292,7 +393,11 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
   }
   const uint32_t traceGrid = (uint32_t)(cus * std::max(traceBpc, shadowBpc) * std::max(1, cfg.traceOversub));
   const uint32_t shadeGrid = (uint32_t)(cus * shadeBpc * std::max(1, cfg.shadeOversub));
-  int32_t *spill = spill_for(s, traceGrid, kWfTraceBlock);
+  static int walkBpc[3] = {0, 0, 0};
+  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
+  if (!walkBpc[li]) walkBpc[li] = blocks_per_cu(walk_kernel(cfg.ldsStack), kWfTraceBlock, 0);
+  const uint32_t walkGrid = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
+  int32_t *spill = spill_for(s, std::max(traceGrid, walkGrid), kWfTraceBlock);
   const int eventLimit = std::min((int)p0.bounces * 3, 50);
   uint32_t done = 0;
   while (done < p0.samples) {
@@ -305,15 +410,21 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
     const uint32_t genGrid = std::min<uint32_t>((nPaths + 255) / 256, (uint32_t)cus * 16u);
     hipLaunchKernelGGL(k_wf_generate, dim3(genGrid), dim3(256), 0, s, P, acc, W);
     for (int e = 0; e <= eventLimit; e++) {
-      if (acc.work)
+      if (cfg.walker) {
+        launch_walk<false>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
+      } else if (acc.work) {
         hipLaunchKernelGGL(k_wf_trace<true>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      else
+      } else {
         hipLaunchKernelGGL(k_wf_trace<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
+      }
       hipLaunchKernelGGL(k_wf_shade, dim3(shadeGrid), dim3(kWfShadeBlock), 0, s, scene, P, acc, W, e);
-      if (acc.work)
+      if (cfg.walker) {
+        launch_walk<true>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
+      } else if (acc.work) {
         hipLaunchKernelGGL(k_wf_shadow<true>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      else
+      } else {
         hipLaunchKernelGGL(k_wf_shadow<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
+      }
     }
     const uint32_t accGrid = std::min<uint32_t>((p0.pixelCount + 255) / 256, (uint32_t)cus * 16u);
     hipLaunchKernelGGL(k_wf_accumulate, dim3(accGrid), dim3(256), 0, s, P, acc, W);

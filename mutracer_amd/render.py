@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi_ext import CameraDesc, MeshDesc, RenderSettings, SceneDesc
+from .abi_ext import CameraDesc, MeshDesc, RenderSettings, SceneDesc, WavefrontTuning
 from .sampling import sampling_pattern_code
 
 _P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
@@ -148,6 +148,23 @@ class RenderTarget:
             return False
         self.render(1)
         return done + 1 < max_samples
+
+    def tuning(self):
+        t = WavefrontTuning()
+        abi.check(abi.lib().mtx_target_get_tuning(self.handle.value, C.byref(t)))
+        return dict(max_paths_in_flight=t.max_paths_in_flight, trace_kernel=t.trace_kernel, lds_stack=t.lds_stack,
+                    trace_oversubscription=t.trace_oversubscription, shade_oversubscription=t.shade_oversubscription)
+
+    def set_tuning(self, **kw):
+        """Changes the wavefront launch shape (keys as returned by tuning())."""
+        cur = self.tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown tuning keys {sorted(unknown)}")
+        cur.update(kw)
+        t = WavefrontTuning(C.sizeof(WavefrontTuning), *[int(cur[k]) for k in (
+            "max_paths_in_flight", "trace_kernel", "lds_stack", "trace_oversubscription", "shade_oversubscription")])
+        abi.check(abi.lib().mtx_target_set_tuning(self.handle.value, C.byref(t)))
 
     def synchronize(self):
         abi.check(abi.lib().mtx_target_synchronize(self.handle.value))

@@ -161,3 +161,25 @@ def test_wavefront_equals_megakernel_large(env, orc):
         out.append(t.read_accumulators())
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(trace_kernel=0, lds_stack=16),
+    dict(trace_kernel=1, lds_stack=16),
+    dict(trace_kernel=1, lds_stack=24, trace_oversubscription=2),
+    dict(trace_kernel=1, lds_stack=32, shade_oversubscription=3, max_paths_in_flight=3000),
+])
+def test_wavefront_tunings_match_oracle(env, orc, tuning):
+    """Every launch shape of the wavefront kernels yields the oracle's sums."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=12, freq=12, env_size=(128, 64))
+    rs = render.RenderScene(sc)
+    t = render.RenderTarget(rs, 80, 45, bounces=10, direct_light_candidates=3, max_samples=6)
+    t.set_tuning(**tuning)
+    got = t.tuning()
+    assert all(got[k] == v for k, v in tuning.items())
+    t.render(6)
+    sums, counts = t.read_accumulators()
+    ref = orc.OracleRenderer(sc, 80, 45, bounces=10, candidates=3, max_samples=6).render(0, 6)
+    assert np.all(counts == 6)
+    assert np.array_equal(sums, ref)
