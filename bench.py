@@ -6,12 +6,14 @@ a 2-triangle ground quad, F82 metal / dielectric / coated-diffuse OpenPBR
 materials, 1024x512 sky+sun environment — at 1920x1080, 16 samples per pixel,
 depth 12, 4 NEE candidates, Owen-scrambled Sobol seed 0. The scene (BVHs,
 materials, environment, FP64 accumulators) is resident in HBM before timing;
-each step clears the accumulators and renders the 16 spp in one megakernel
-launch.
+each step clears the accumulators and renders the 16 spp with the wavefront
+kernels (generate, then per event closest-hit trace, shade, shadow trace, then
+accumulate) or, with --kernel megakernel, in one megakernel launch.
 
-Prints one JSON line: Msamples/s (value), Mrays/s, the megakernel's roofline
-(algorithmic bytes per launch from the instrumented launch's work totals over
-the launch duration from HIP events on the launch stream) and the CPU
+Prints one JSON line: Msamples/s (value), Mrays/s, the roofline of the
+dominant kernel (the closest-hit trace; its algorithmic bytes from the
+instrumented launch's work totals, its GPU time from HIP events around each of
+its launches on the launch stream) and the CPU
 baseline: the oracle (C++ restatement of the same algorithm, OpenMP) timed on
 this host on 32x32-pixel tiles of the same image, whose results are also
 compared with the GPU's accumulators for those pixels.
@@ -29,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "Msamples/s at 1080p 1M-tri (plain megakernel)"
+METRIC = "Msamples/s at 1080p 1M-tri (plain: one job at a time)"
 HBM_PEAK_GBS = 8000.0
 WIDTH, HEIGHT, SPP, DEPTH, CANDIDATES = 1920, 1080, 16, 12, 4
 
@@ -38,10 +40,11 @@ BOX_B = 32          # one child AABB slab test (2 x vec3 f32 + padding)
 TRI_B = 48          # one triangle test (3 vertices, 12 B each, + 12 B index)
 INST_B = 80         # one instance entry (w2l 4x4 f32 + 16 B record)
 HIT_B = 128 + 48    # per closest hit: packed material (32 f32) + 3 normals and the index triple
+RAY_B = 32 + 32     # wavefront closest-hit query: ray in (origin, direction), hit record out
 SAMPLE_B = 13 * 16 + 8  # per sample: read+write of 13 FP64 accumulator planes + the u32 count
 
 
-KERNEL_NAMES = {"wavefront": "wavefront launch sequence (generate, trace, shade, shadow, accumulate)",
+KERNEL_NAMES = {"wavefront": "k_wf_trace (closest-hit traversal of the wavefront sequence)",
                 "megakernel": "k_render (megakernel)"}
 
 
@@ -58,6 +61,41 @@ def traffic_from_profile():
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def profile_summary(db_path):
+    """Per-kernel totals from a rocprofv3 results database (kernel trace and,
+    when collected, counters summed per kernel)."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    out = ["kernel | calls | total ms | avg us"]
+    for name, calls, total, avg in con.execute(
+            "select name, count(*), sum(duration), avg(duration) from kernels group by name "
+            "order by sum(duration) desc limit 25"):
+        out.append(f"{name.split('(')[0]} | {calls} | {total / 1e6:.3f} | {avg / 1e3:.1f}")
+    try:
+        rows = con.execute("select k.name, i.name, sum(e.value), count(*) from rocpd_pmc_event e "
+                           "join rocpd_info_pmc i on e.pmc_id = i.id join kernels k on k.event_id = e.event_id "
+                           "group by k.name, i.name order by k.name").fetchall()
+    except sqlite3.Error:
+        rows = []
+    if rows:
+        out.append("kernel | counter | sum over dispatches | dispatches")
+        for kname, cname, total, n in rows:
+            out.append(f"{kname.split('(')[0]} | {cname} | {total:.1f} | {n}")
+    return "\n".join(out)
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (reads and writes mib MiB each):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
 
 
 def cpu_baseline(sc, gpu_sums, seconds):
@@ -94,7 +132,15 @@ def main():
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
     ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
+    ap.add_argument("--profile-summary", default="", help="print a summary of a rocprofv3 results db and exit")
+    ap.add_argument("--copy-calibration", type=int, default=0, help="run one copy of N MiB and exit")
     args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary))
+        return
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the path shards by "
               f"independent sample ranges; multi-GPU runs are out of scope).", file=sys.stderr)
@@ -126,8 +172,16 @@ def main():
     rays = tgt.stats()["rays"]
     log(f"instrumented launch: {work}, rays {rays}")
     samples = WIDTH * HEIGHT * SPP
-    alg_bytes = (BOX_B * work["box_tests"] + TRI_B * work["tri_tests"] + INST_B * work["instance_visits"]
-                 + HIT_B * work["hits"] + SAMPLE_B * samples)
+    closest_rays = rays - work["shadow_rays"]
+    if args.kernel == "wavefront":
+        # Dominant kernel: the closest-hit trace (ray in + hit out + node/triangle/instance fetches).
+        alg_bytes = (RAY_B * closest_rays + BOX_B * work["box_tests"] + TRI_B * work["tri_tests"]
+                     + INST_B * work["instance_visits"])
+    else:
+        alg_bytes = (BOX_B * (work["box_tests"] + work["shadow_box_tests"])
+                     + TRI_B * (work["tri_tests"] + work["shadow_tri_tests"])
+                     + INST_B * (work["instance_visits"] + work["shadow_instance_visits"])
+                     + HIT_B * work["hits"] + SAMPLE_B * samples)
 
     for _ in range(args.warmup):
         tgt.restart_samples()
@@ -135,20 +189,20 @@ def main():
     torch.cuda.synchronize()
     log("warmup done")
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    # Timed region: every launch bracketed by HIP events on the launch stream
+    # (per kernel class), the steps themselves by the wall clock.
+    classes = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         tgt.restart_samples()
-        starts[k].record(stream)
-        tgt.render(SPP, stream)
-        ends[k].record(stream)
+        classes.append(tgt.render_timed(SPP, stream))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, launch {kernel_ms:.2f} ms")
+    class_ms = {k: float(np.mean([c[k] for c in classes])) for k in classes[0]}
+    kernel_ms = class_ms["trace"] if args.kernel == "wavefront" else class_ms["megakernel"]
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, per kernel class {class_ms}")
     st = tgt.stats()
     assert st["samples"] == SPP
     rays_per_step = st["rays"]
@@ -179,7 +233,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": KERNEL_NAMES[args.kernel], "kernel_ms": round(kernel_ms, 3),
-                     "algorithmic_bytes_per_launch": int(alg_bytes), "work_per_launch": work},
+                     "algorithmic_bytes_per_step": int(alg_bytes), "work_per_step": work,
+                     "kernel_class_ms_per_step": {k: round(v, 3) for k, v in class_ms.items()}},
         "cpu_baseline": cpu,
         "parity_relmse_cpu_tiles": parity,
     }

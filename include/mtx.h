@@ -266,10 +266,18 @@ int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings,
 int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream);
 
 /* mtx_target_render plus per-launch totals of the traversal work:
- * work[0] box tests, [1] triangle tests, [2] instance visits, [3] closest
- * hits, [4] shadow rays. Synchronous. No reference counterpart: it feeds the
- * algorithmic-bytes figure of the roofline. */
-int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[5], void *stream);
+ * work[0] closest-hit box tests, [1] closest-hit triangle tests, [2]
+ * closest-hit instance visits, [3] closest hits, [4] shadow rays, [5] shadow
+ * box tests, [6] shadow triangle tests, [7] shadow instance visits.
+ * Synchronous. No reference counterpart: it feeds the algorithmic-bytes
+ * figure of the roofline. */
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[8], void *stream);
+
+/* mtx_target_render with HIP events around every kernel launch on `stream`;
+ * returns the summed GPU time per kernel class: ms[0] generate + accumulate,
+ * [1] closest-hit trace, [2] shade, [3] shadow trace, [4] megakernel.
+ * Synchronous. No reference counterpart (measurement). */
+int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[5], void *stream);
 
 /* Blocks until the target's submitted work is done. */
 int32_t mtx_target_synchronize(mtx_handle target);

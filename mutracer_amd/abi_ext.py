@@ -46,6 +46,7 @@ def declare(L):
         "mtx_target_create": (i32, [u64, vp, vp, P(u64), C.c_char_p, sz]),
         "mtx_target_render": (i32, [u64, u32, vp]),
         "mtx_target_render_counted": (i32, [u64, u32, vp, vp]),
+        "mtx_target_render_timed": (i32, [u64, u32, vp, vp]),
         "mtx_target_synchronize": (i32, [u64]),
         "mtx_target_read_pixels": (i32, [u64, u32, vp, sz, P(sz)]),
         "mtx_target_read_accumulators": (i32, [u64, vp, sz, vp, sz]),

@@ -102,12 +102,27 @@ struct WfConfig {
   uint32_t maxPaths = 1u << 25;  // paths in flight per generation
   int traceOversub = 1;          // trace/shadow grid = CUs * resident blocks * this
   int shadeOversub = 1;
-  bool walker = true;            // persistent lane-refill traversal (else one ray per thread)
+  bool walker = false;           // persistent lane-refill traversal (else one ray per thread)
   int ldsStack = 16;             // LDS stack entries per lane for the walker (16, 24 or 32)
 };
 
-void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s);
+// Optional per-kernel-class timing with HIP events on the launch stream:
+// class 0 generate + accumulate, 1 closest-hit trace, 2 shade, 3 shadow
+// trace, 4 megakernel.
+constexpr int kTimerClasses = 5;
+struct LaunchTimer {
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> events;  // begin/end pairs
+  std::vector<int> classes;
+  void begin(int cls);
+  void end();
+  void collect(double ms[kTimerClasses]);  // synchronizes, sums, releases the events
+  ~LaunchTimer();
+};
+
+void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s,
+                   LaunchTimer *timer = nullptr);
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const Accumulators &acc,
-                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s);
+                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer = nullptr);
 
 }  // namespace mtx

@@ -16,8 +16,9 @@
 
 namespace mtx {
 
-// kCount adds per-launch totals of the traversal work (box tests, triangle
-// tests, instance visits, closest hits, shadow rays) to acc.work[0..4].
+// kCount adds per-launch totals of the traversal work to acc.work (layout of
+// mtx_target_render_counted: closest box/triangle/instance tests, hits,
+// shadow rays, shadow box/triangle/instance tests).
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock)
 k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__restrict__ spill) {
@@ -27,7 +28,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   const AccelView av = *S.accel;
   const Lut L{S.lut};
   uint64_t rays = 0;
-  TraceCounters tc{0u, 0u, 0u};
+  TraceCounters tc{0u, 0u, 0u}, ts{0u, 0u, 0u};
   uint32_t nHits = 0, nShadow = 0;
   const uint32_t nBlocks = (P.pixelCount + kBlock - 1) / kBlock;
   // XCD-aware ordering: blocks resident on one XCD (blockIdx % 8) walk one
@@ -61,7 +62,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
           TraceHit sh;
           rays++;
           if (kCount) nShadow++;
-          if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &tc))
+          if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts))
             pr.li = pr.li + sr.payload;
         }
         if (!alive) break;
@@ -76,25 +77,57 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
   if ((threadIdx.x & 63) == 0 && rays) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)rays);
   if (kCount) {
-    uint64_t v[5] = {tc.boxes, tc.tris, tc.instances, nHits, nShadow};
+    uint64_t v[8] = {tc.boxes, tc.tris, tc.instances, nHits, nShadow, ts.boxes, ts.tris, ts.instances};
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < 8; k++) {
       for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
       if ((threadIdx.x & 63) == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + k, (unsigned long long)v[k]);
     }
   }
 }
 
-void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s) {
+void LaunchTimer::begin(int cls) {
+  hipEvent_t e;
+  MTX_HIP(hipEventCreate(&e));
+  MTX_HIP(hipEventRecord(e, stream));
+  events.push_back(e);
+  classes.push_back(cls);
+}
+void LaunchTimer::end() {
+  hipEvent_t e;
+  MTX_HIP(hipEventCreate(&e));
+  MTX_HIP(hipEventRecord(e, stream));
+  events.push_back(e);
+}
+void LaunchTimer::collect(double ms[kTimerClasses]) {
+  for (int k = 0; k < kTimerClasses; k++) ms[k] = 0.0;
+  if (!events.empty()) MTX_HIP(hipEventSynchronize(events.back()));
+  for (size_t i = 0; i + 1 < events.size(); i += 2) {
+    float t = 0.0f;
+    MTX_HIP(hipEventElapsedTime(&t, events[i], events[i + 1]));
+    ms[classes[i / 2]] += (double)t;
+  }
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  events.clear();
+  classes.clear();
+}
+LaunchTimer::~LaunchTimer() {
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+}
+
+void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s,
+                   LaunchTimer *timer) {
   if (!p.pixelCount || !p.samples) return;
   uint32_t grid = trace_grid();
   uint32_t need = ((p.pixelCount + kBlock - 1) / kBlock + 7u) / 8u * 8u;
   grid = std::min(grid, need);
   int32_t *spill = spill_for(s, grid, kBlock);
+  if (timer) timer->begin(4);
   if (acc.work)
     hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
   else
     hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  if (timer) timer->end();
   MTX_HIP(hipGetLastError());
 }
 

@@ -317,14 +317,15 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
   return T;
 }
 
-void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work = nullptr) {
+void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work = nullptr,
+                   LaunchTimer *timer = nullptr) {
   RenderParams p = T.params;
   p.samples = samples;
   Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height, work};
   if (T.kernel == MTX_KERNEL_MEGAKERNEL)
-    launch_render(T.scene->view, p, acc, s);
+    launch_render(T.scene->view, p, acc, s, timer);
   else
-    launch_render_wavefront(T.scene->view, p, acc, T.wf, T.wfConfig, s);
+    launch_render_wavefront(T.scene->view, p, acc, T.wf, T.wfConfig, s, timer);
   T.stream = s;
   T.samplesDone += samples;
 }
@@ -413,18 +414,32 @@ int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream) {
   API_CATCH(nullptr, 0)
 }
 
-int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[5], void *stream) {
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[8], void *stream) {
   API_TRY
   MTX_REQUIRE(work, MTX_INVALID_ARGUMENT, "work is NULL");
   auto T = get_target(target);
   std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
   if (!lk.owns_lock()) return MTX_BUSY;
   hipStream_t s = (hipStream_t)stream;
-  DevBuf<uint64_t> w(5);
+  DevBuf<uint64_t> w(8);
   MTX_HIP(hipMemsetAsync(w.ptr, 0, w.bytes(), s));
   target_render(*T, samples, s, w.ptr);
-  w.download(work, 5, s);
+  w.download(work, 8, s);
   MTX_HIP(hipStreamSynchronize(s));
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[5], void *stream) {
+  API_TRY
+  MTX_REQUIRE(ms, MTX_INVALID_ARGUMENT, "ms is NULL");
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  LaunchTimer timer;
+  timer.stream = (hipStream_t)stream;
+  target_render(*T, samples, (hipStream_t)stream, nullptr, &timer);
+  timer.collect(ms);
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

@@ -211,7 +211,7 @@ k_wf_shadow(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
-      if (lane_id() == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + (k < 3 ? k : 4), (unsigned long long)v[k]);
+      if (lane_id() == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + (k < 3 ? 5 + k : 4), (unsigned long long)v[k]);
     }
   }
 }
@@ -289,7 +289,7 @@ k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *
     for (int k = 0; k < 4; k++) {
       uint64_t v = totals[k];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-      const int slot = k < 3 ? k : (kShadow ? 4 : 3);
+      const int slot = k < 3 ? (kShadow ? 5 + k : k) : (kShadow ? 4 : 3);
       if (lane_id() == 0 && v) atomicAdd((unsigned long long *)acc.work + slot, (unsigned long long)v);
     }
   }
@@ -380,7 +380,9 @@ static const void *walk_kernel(int lds) {
 }
 
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, const Accumulators &acc,
-                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s) {
+                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer) {
+  auto tb = [&](int c) { if (timer) timer->begin(c); };
+  auto te = [&]() { if (timer) timer->end(); };
   if (!p0.pixelCount || !p0.samples) return;
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
@@ -408,8 +410,11 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
     RenderParams P = p0;
     P.samples = gen;
     const uint32_t genGrid = std::min<uint32_t>((nPaths + 255) / 256, (uint32_t)cus * 16u);
+    tb(0);
     hipLaunchKernelGGL(k_wf_generate, dim3(genGrid), dim3(256), 0, s, P, acc, W);
+    te();
     for (int e = 0; e <= eventLimit; e++) {
+      tb(1);
       if (cfg.walker) {
         launch_walk<false>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
       } else if (acc.work) {
@@ -417,7 +422,11 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       } else {
         hipLaunchKernelGGL(k_wf_trace<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
       }
+      te();
+      tb(2);
       hipLaunchKernelGGL(k_wf_shade, dim3(shadeGrid), dim3(kWfShadeBlock), 0, s, scene, P, acc, W, e);
+      te();
+      tb(3);
       if (cfg.walker) {
         launch_walk<true>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
       } else if (acc.work) {
@@ -425,9 +434,12 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       } else {
         hipLaunchKernelGGL(k_wf_shadow<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
       }
+      te();
     }
     const uint32_t accGrid = std::min<uint32_t>((p0.pixelCount + 255) / 256, (uint32_t)cus * 16u);
+    tb(0);
     hipLaunchKernelGGL(k_wf_accumulate, dim3(accGrid), dim3(256), 0, s, P, acc, W);
+    te();
     MTX_HIP(hipGetLastError());
     done += gen;
   }

@@ -133,10 +133,18 @@ class RenderTarget:
     def render_counted(self, samples, stream=None):
         """render() plus the launch's traversal work totals (synchronous)."""
         sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
-        out = np.zeros(5, np.uint64)
+        out = np.zeros(8, np.uint64)
         abi.check(abi.lib().mtx_target_render_counted(self.handle.value, samples, _P(out), sp))
-        return dict(box_tests=int(out[0]), tri_tests=int(out[1]), instance_visits=int(out[2]), hits=int(out[3]),
-                    shadow_rays=int(out[4]))
+        keys = ("box_tests", "tri_tests", "instance_visits", "hits", "shadow_rays", "shadow_box_tests",
+                "shadow_tri_tests", "shadow_instance_visits")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def render_timed(self, samples, stream=None):
+        """render() with HIP events around every launch: GPU ms per kernel class (synchronous)."""
+        sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+        out = np.zeros(5, np.float64)
+        abi.check(abi.lib().mtx_target_render_timed(self.handle.value, samples, _P(out), sp))
+        return dict(zip(("generate_accumulate", "trace", "shade", "shadow", "megakernel"), map(float, out)))
 
     def trace(self, camera=None, max_samples=1):
         """RenderTarget.trace: one more sample unless `max_samples` are done.

@@ -147,6 +147,7 @@ def test_counted_launch_matches_plain(env, orc, kernel):
     assert 0 < work["hits"] <= rays - work["shadow_rays"]
     assert work["box_tests"] > work["tri_tests"] > 0
     assert work["instance_visits"] >= work["hits"]
+    assert work["shadow_box_tests"] > work["shadow_tri_tests"] > 0
 
 
 def test_wavefront_equals_megakernel_large(env, orc):
