@@ -54,11 +54,11 @@ def log(msg):
 
 def traffic_from_profile():
     """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", "megakernel_c2_1080p_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "wf_trace_c2_1080p_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith("traffic_bytes_per_step:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -75,8 +75,9 @@ def profile_summary(db_path):
         out.append(f"{name.split('(')[0]} | {calls} | {total / 1e6:.3f} | {avg / 1e3:.1f}")
     try:
         rows = con.execute("select k.name, i.name, sum(e.value), count(*) from rocpd_pmc_event e "
-                           "join rocpd_info_pmc i on e.pmc_id = i.id join kernels k on k.event_id = e.event_id "
-                           "group by k.name, i.name order by k.name").fetchall()
+                           "join rocpd_info_pmc i on e.pmc_id = i.id "
+                           "join rocpd_kernel_dispatch d on d.event_id = e.event_id join kernels k on k.id = d.id "
+                           "group by k.name, i.name order by sum(e.value) desc").fetchall()
     except sqlite3.Error:
         rows = []
     if rows:
