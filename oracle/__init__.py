@@ -45,6 +45,39 @@ def math_eval(kind, x, y=None):
     return out
 
 
+def light_tables(lights, env=None):
+    """Walker alias table and pmfs of the oracle (lights: (n,16) f32, env: (H,W,4) f32 or None)."""
+    L = lib()
+    lights = np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1, 16))
+    n = lights.shape[0]
+    envp = None if env is None else np.ascontiguousarray(env, np.float32)
+    cap = n + 1
+    prob = np.zeros(cap, np.float32)
+    alias, typ, idx = (np.zeros(cap, np.int32) for _ in range(3))
+    lpmf = np.zeros(max(n, 1), np.float32)
+    epmf = np.zeros(1, np.float32)
+    total = np.zeros(1, np.float64)
+    L.orc_light_tables.restype = C.c_uint32
+    L.orc_light_tables.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32] + [C.c_void_p] * 4 + [
+        C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    m = L.orc_light_tables(_p(lights), n, _p(envp), 0 if envp is None else envp.shape[1],
+                           0 if envp is None else envp.shape[0], _p(prob), _p(alias), _p(typ), _p(idx), cap,
+                           _p(lpmf), _p(epmf), _p(total))
+    return dict(prob=prob[:m], alias=alias[:m], type=typ[:m], index=idx[:m], light_pmf=lpmf[:n],
+                env_pmf=float(epmf[0]), total=float(total[0]))
+
+
+def env_cdf(env):
+    """Environment CDF of the oracle: (H+1, W+2, 2) f32."""
+    env = np.ascontiguousarray(env, np.float32)
+    h, w = env.shape[0], env.shape[1]
+    out = np.zeros((h + 1, w + 2, 2), np.float32)
+    L = lib()
+    L.orc_env_cdf.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.orc_env_cdf(_p(env), w, h, _p(out))
+    return out
+
+
 def threads():
     """OpenMP threads the oracle's batch entry points run on."""
     return int(lib().orc_threads())

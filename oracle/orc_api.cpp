@@ -229,3 +229,33 @@ extern "C" void orc_math_eval(uint32_t kind, const float *x, const float *y, flo
     }
   }
 }
+
+// Host light tables of a scene's lights and environment. Returns the alias
+// table length n; prob/alias/type/index get n entries (capacity cap),
+// lightPmf one per light, scalars[0] = env pmf, totalOut = total weight.
+extern "C" uint32_t orc_light_tables(const float *lights, uint32_t lightCount, const float *env, uint32_t envW,
+                                     uint32_t envH, float *prob, int32_t *alias, int32_t *type, int32_t *index,
+                                     uint32_t cap, float *lightPmf, float *envPmf, double *totalOut) {
+  RenderScene s{};
+  s.lights = lights; s.lightCount = lightCount;
+  s.env = env; s.envW = envW; s.envH = envH;
+  HostTables T;
+  build_tables(s, T);
+  uint32_t n = (uint32_t)T.prob.size();
+  for (uint32_t i = 0; i < n && i < cap; i++) {
+    prob[i] = T.prob[i]; alias[i] = T.alias[i]; type[i] = T.type[i]; index[i] = T.index[i];
+  }
+  for (uint32_t i = 0; i < lightCount; i++) lightPmf[i] = T.lightPmf[i];
+  *envPmf = T.envPmf;
+  *totalOut = T.totalWeight;
+  return n;
+}
+
+// Environment CDF: (W+2) x (H+1) x 2 floats into out.
+extern "C" void orc_env_cdf(const float *env, uint32_t envW, uint32_t envH, float *out) {
+  RenderScene s{};
+  s.env = env; s.envW = envW; s.envH = envH;
+  HostTables T;
+  build_env_cdf(s, T);
+  for (size_t i = 0; i < T.cdf.size(); i++) out[i] = T.cdf[i];
+}

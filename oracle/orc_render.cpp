@@ -600,19 +600,8 @@ struct LightSample {
   uint32_t sourceType;
 };
 
-struct HostTables {
-  // alias table entries: probability, alias, type (0 env, 1 analytic), index
-  std::vector<float> prob;
-  std::vector<int32_t> alias, type, index;
-  std::vector<float> lightPmf;  // per analytic light
-  float envPmf = 0.0f;
-  double totalWeight = 0.0;
-  std::vector<float> cdf;       // (W+2) x (H+1) x 2
-  uint32_t cdfW = 0, cdfH = 0;
-};
-
 // lightSampling.ts:172-212 (Walker alias table) over {env, analytic lights}.
-static void build_tables(const RenderScene &s, HostTables &T) {
+void build_tables(const RenderScene &s, HostTables &T) {
   std::vector<int32_t> types, idxs;
   std::vector<double> weights;
   auto push = [&](int32_t ty, int32_t ix, double w) {
@@ -677,7 +666,7 @@ static void build_tables(const RenderScene &s, HostTables &T) {
 }
 
 // envMapDistribution (envMapDistribution.ts:57-134).
-static void build_env_cdf(const RenderScene &s, HostTables &T) {
+void build_env_cdf(const RenderScene &s, HostTables &T) {
   if (!s.env) return;
   const uint32_t W = s.envW, H = s.envH, CW = W + 2, CH = H + 1;
   T.cdfW = CW; T.cdfH = CH;

@@ -44,6 +44,22 @@ struct Accum {
   double color[4], albedo[4], normal[4], lum2;
 };
 
+// Host light tables: Walker alias table over {environment, rect lights}
+// (lightSampling.ts:172-212, power estimates :72-107, :415-431) and the
+// environment CDF (envMapDistribution.ts:57-134).
+struct HostTables {
+  // alias table entries: probability, alias, type (0 env, 1 analytic), index
+  std::vector<float> prob;
+  std::vector<int32_t> alias, type, index;
+  std::vector<float> lightPmf;  // per analytic light
+  float envPmf = 0.0f;
+  double totalWeight = 0.0;
+  std::vector<float> cdf;       // (W+2) x (H+1) x 2
+  uint32_t cdfW = 0, cdfH = 0;
+};
+void build_tables(const RenderScene &s, HostTables &T);
+void build_env_cdf(const RenderScene &s, HostTables &T);
+
 void set_energy_lut(const float *lut);  // 32 x 2177 floats
 // Renders samples [sampleStart, sampleStart+count) for the pixel rectangle
 // and adds them to out (one Accum per pixel of the rectangle, row-major).
