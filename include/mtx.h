@@ -317,6 +317,61 @@ int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera);
 int32_t mtx_target_reset(mtx_handle target);
 int32_t mtx_target_destroy(mtx_handle target);
 
+/* -------------------------------------------------------------- serving */
+/* Batched render server: many render jobs (mixed scenes, resolutions, sample
+ * counts) share every wavefront generation. The reference's native job API
+ * runs one job per context and one sample batch per dispatch
+ * (mt_context_submit / mt_job_poll / mt_job_info / mt_job_cancel,
+ * mutracer-context.h:146-187); here a round gives every running job a slice
+ * of the round's path budget, so new jobs join while others are in flight
+ * (continuous batching). A job's result is identical to rendering it alone:
+ * its samples keep their Sobol indices and FP64 accumulation order. */
+typedef struct mtx_server_config {
+  uint32_t struct_size;
+  uint32_t max_paths_per_round;    /* path budget of one round (0 = 2^24) */
+  uint32_t max_jobs_per_round;     /* 1 = one job at a time (the plain form); 0 = 1024 */
+  uint32_t max_samples_per_job_round; /* per-job samples per round (0 = 64, as mt_job_poll) */
+  uint64_t hbm_budget_bytes;       /* accumulators of admitted jobs (0 = 200 GB) */
+} mtx_server_config;
+
+/* Job states: values of mt_job_state (mutracer-status.h:123-126). */
+#define MTX_JOB_QUEUED 0u
+#define MTX_JOB_RUNNING 1u
+#define MTX_JOB_COMPLETE 2u
+#define MTX_JOB_CANCELLED 3u
+#define MTX_JOB_FAILED 4u
+
+typedef struct mtx_job_info {
+  uint32_t struct_size;
+  uint32_t state;
+  uint32_t requested_samples, submitted_samples, completed_samples, base_samples;
+  double submit_ms, start_ms, complete_ms; /* host steady clock, milliseconds */
+} mtx_job_info;
+
+int32_t mtx_server_create(const mtx_server_config *config, mtx_handle *out_server, char *error, size_t error_capacity);
+/* Queues a job of `samples` samples per pixel on a committed scene (inputs copied). */
+int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_render_settings *settings,
+                          const mtx_camera_desc *camera, uint32_t samples, mtx_handle *out_job, char *error,
+                          size_t error_capacity);
+/* Runs one round on `stream` and waits for it: admits queued jobs in arrival
+ * order while their accumulators fit the HBM budget, gives each running job
+ * up to max_samples_per_job_round samples within the path budget, launches
+ * one wavefront generation over all of them. *completed = jobs finished by
+ * this round; *idle = 1 when there was nothing to run. */
+int32_t mtx_server_step(mtx_handle server, void *stream, uint32_t *completed, uint32_t *idle);
+int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_job_info *info);
+int32_t mtx_server_job_read_pixels(mtx_handle server, mtx_handle job, uint32_t kind, float *pixels,
+                                   size_t capacity_floats, size_t *written);
+int32_t mtx_server_job_read_accumulators(mtx_handle server, mtx_handle job, double *sums, size_t capacity_doubles,
+                                         uint32_t *counts, size_t capacity_counts);
+/* Queued jobs stop at once; running jobs stop after the current round. */
+int32_t mtx_server_job_cancel(mtx_handle server, mtx_handle job);
+/* Frees a finished job's accumulators (returns them to the HBM budget). */
+int32_t mtx_server_job_release(mtx_handle server, mtx_handle job);
+/* stats[0] rounds, [1] rays traced, [2] running jobs, [3] queued jobs, [4] admitted bytes. */
+int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]);
+int32_t mtx_server_destroy(mtx_handle server);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on
  * the device: kind 0 sin(x), 1 cos(x), 2 pow(x, y), 3 atan2(x, y) with x the

@@ -35,6 +35,17 @@ class WavefrontTuning(C.Structure):
                 ("shade_oversubscription", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_paths_per_round", C.c_uint32), ("max_jobs_per_round", C.c_uint32),
+                ("max_samples_per_job_round", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("state", C.c_uint32), ("requested_samples", C.c_uint32),
+                ("submitted_samples", C.c_uint32), ("completed_samples", C.c_uint32), ("base_samples", C.c_uint32),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double), ("complete_ms", C.c_double)]
+
+
 def declare(L):
     u32, i32, u64, vp, sz = C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p, C.c_size_t
     P = C.POINTER
@@ -57,6 +68,16 @@ def declare(L):
         "mtx_target_set_tuning": (i32, [u64, vp]),
         "mtx_target_get_tuning": (i32, [u64, vp]),
         "mtx_math_eval": (i32, [u32, vp, vp, vp, u64, vp]),
+        "mtx_server_create": (i32, [vp, P(u64), C.c_char_p, sz]),
+        "mtx_server_submit": (i32, [u64, u64, vp, vp, u32, P(u64), C.c_char_p, sz]),
+        "mtx_server_step": (i32, [u64, vp, P(u32), P(u32)]),
+        "mtx_server_job_info": (i32, [u64, u64, vp]),
+        "mtx_server_job_read_pixels": (i32, [u64, u64, u32, vp, sz, P(sz)]),
+        "mtx_server_job_read_accumulators": (i32, [u64, u64, vp, sz, vp, sz]),
+        "mtx_server_job_cancel": (i32, [u64, u64]),
+        "mtx_server_job_release": (i32, [u64, u64]),
+        "mtx_server_stats": (i32, [u64, vp]),
+        "mtx_server_destroy": (i32, [u64]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

@@ -82,19 +82,43 @@ struct WfBuffers {
   uint32_t *qA, *qB;
   float4 *s0, *s1, *s2;
   int32_t *s3;
+  uint16_t *item;   // job slot of each path (rounds with several jobs)
   uint32_t *counts;
-  uint32_t nPaths;  // paths in flight: gen * pixelCount
-  uint32_t gen;     // samples per pixel in flight
+  uint32_t nPaths;  // paths in flight this generation
   struct Owner {
     DevBuf<float4> o, d, li, beta, alb, nrm, hitA, s0, s1, s2;
     DevBuf<int2> hitB;
     DevBuf<uint32_t> qA, qB, counts;
     DevBuf<int32_t> s3;
+    DevBuf<uint16_t> item;
     uint32_t capacity = 0;
     void ensure(uint32_t paths);
     size_t bytes() const;
-    WfBuffers view(uint32_t nPaths, uint32_t gen) const;
+    WfBuffers view(uint32_t nPaths) const;
   };
+};
+
+// One job's share of a wavefront generation: its scene, parameters (samples =
+// samples per pixel in this generation), accumulators, and where its paths
+// and pixels sit in the generation's concatenated path / pixel ranges. Paths
+// of a job are ordered sample-major: path = pathBase + s * pixelCount + local.
+struct JobSlot {
+  GpuScene scene;
+  RenderParams params;
+  Accumulators acc;
+  uint32_t pathBase, pixelPrefix;
+};
+constexpr uint32_t kMaxJobsPerRound = 1024;
+
+// What every wavefront kernel of one generation receives: a single job by
+// value, or a device table of jobs (item[] maps paths to it).
+struct WfLaunch {
+  JobSlot one;
+  const JobSlot *jobs;
+  uint32_t nJobs;
+  uint32_t totalPixels;
+  uint64_t *rays;  // traced-ray counter
+  uint64_t *work;  // optional traversal work totals (8 counters)
 };
 
 // Launch shape of the wavefront kernels (the autotuner's search space).
@@ -124,5 +148,11 @@ void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulat
                    LaunchTimer *timer = nullptr);
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const Accumulators &acc,
                              WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer = nullptr);
+// One generation of several jobs (slots on the device, totals given): paths
+// = sum of pathCount, pixels = sum of pixelCount, events = the largest job's
+// event limit.
+void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
+                             int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
+                             hipStream_t s, LaunchTimer *timer = nullptr);
 
 }  // namespace mtx

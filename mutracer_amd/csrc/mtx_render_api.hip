@@ -2,7 +2,10 @@
 // accumulation), plus the host-side light tables: the Walker alias table over
 // {environment, rect lights} (lightSampling.ts:72-107,172-212,415-431) and the
 // environment CDF with its 3x3 max-dilated luminance (envMapDistribution.ts:57-134).
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <deque>
 #include <mutex>
 
 #include "mtx_pt.h"
@@ -282,8 +285,8 @@ void target_reset(TargetObj &T) {
   T.samplesDone = 0;
 }
 
-std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, const mtx_render_settings &s,
-                                         const mtx_camera_desc &cam) {
+// Validated render parameters of a settings + camera pair (whole image).
+static RenderParams params_from(const mtx_render_settings &s, const mtx_camera_desc &cam) {
   MTX_REQUIRE(s.struct_size >= sizeof(mtx_render_settings), MTX_INVALID_ARGUMENT,
               "mtx_render_settings.struct_size too small");
   MTX_REQUIRE(s.width > 0 && s.height > 0 && (uint64_t)s.width * s.height <= (1ull << 28), MTX_INVALID_ARGUMENT,
@@ -291,9 +294,8 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
   MTX_REQUIRE(s.sampling_pattern <= 1, MTX_INVALID_ARGUMENT, "sampling_pattern must be 0 (sobol) or 1 (zsobol)");
   MTX_REQUIRE(s.bounces <= 50, MTX_INVALID_ARGUMENT, "bounces above 50");
   MTX_REQUIRE(s.kernel <= MTX_KERNEL_MEGAKERNEL, MTX_INVALID_ARGUMENT, "unknown kernel mode");
-  auto T = std::make_shared<TargetObj>();
-  T->scene = scene;
-  RenderParams &p = T->params;
+  MTX_REQUIRE(std::isfinite(s.clamping) && s.clamping >= 0.0f, MTX_INVALID_ARGUMENT, "clamping must be finite");
+  RenderParams p{};
   set_camera(p, cam);
   p.width = s.width;
   p.height = s.height;
@@ -306,6 +308,39 @@ std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, 
   p.clamping = s.clamping;
   p.pixelBase = 0;
   p.pixelCount = s.width * s.height;
+  return p;
+}
+
+// Resolved RGBA f32, top row first, from 13 planes + counts (bottom row first).
+static void resolve_pixels(const double *pl, const uint32_t *cnt, uint32_t W, uint32_t H, uint32_t kind,
+                           float *pixels) {
+  const size_t N = (size_t)W * H;
+  for (uint32_t y = 0; y < H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      size_t i = (size_t)y * W + x, o = ((size_t)(H - 1 - y) * W + x) * 4;
+      double n = cnt[i] ? (double)cnt[i] : 1.0;
+      if (kind == MTX_PIXELS_VARIANCE) {
+        double mean = (0.2126 * pl[i] + 0.7152 * pl[N + i] + 0.0722 * pl[2 * N + i]) / n;
+        double var = std::max(0.0, pl[12 * N + i] / n - mean * mean) / n;
+        pixels[o] = (float)var; pixels[o + 1] = pixels[o + 2] = 0.0f; pixels[o + 3] = 1.0f;
+        continue;
+      }
+      int base = kind == MTX_PIXELS_COLOR ? 0 : (kind == MTX_PIXELS_ALBEDO ? 4 : 8);
+      for (int k = 0; k < 4; k++) pixels[o + k] = (float)(pl[(size_t)(base + k) * N + i] / n);
+    }
+}
+
+// Pixel-major copy of the 13 planes (13 doubles per pixel).
+static void interleave_planes(const std::vector<double> &planes, size_t N, double *sums) {
+  for (size_t i = 0; i < N; i++)
+    for (int k = 0; k < 13; k++) sums[i * 13 + k] = planes[(size_t)k * N + i];
+}
+
+std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, const mtx_render_settings &s,
+                                         const mtx_camera_desc &cam) {
+  auto T = std::make_shared<TargetObj>();
+  T->scene = scene;
+  T->params = params_from(s, cam);
   T->kernel = s.kernel;
   if (s.max_paths_in_flight) T->wfConfig.maxPaths = s.max_paths_in_flight;
   const size_t N = (size_t)s.width * s.height;
@@ -460,9 +495,7 @@ int32_t mtx_target_read_accumulators(mtx_handle target, double *sums, size_t cap
   MTX_HIP(hipStreamSynchronize(T->stream));
   if (sums) {
     if (capSums < N * 13) return MTX_BUFFER_TOO_SMALL;
-    std::vector<double> planes = T->planes.to_host(N * 13, T->stream);
-    for (size_t i = 0; i < N; i++)
-      for (int k = 0; k < 13; k++) sums[i * 13 + k] = planes[(size_t)k * N + i];
+    interleave_planes(T->planes.to_host(N * 13, T->stream), N, sums);
   }
   if (counts) {
     if (capCounts < N) return MTX_BUFFER_TOO_SMALL;
@@ -485,19 +518,7 @@ int32_t mtx_target_read_pixels(mtx_handle target, uint32_t kind, float *pixels, 
   MTX_HIP(hipStreamSynchronize(T->stream));
   std::vector<double> pl = T->planes.to_host(N * 13, T->stream);
   std::vector<uint32_t> cnt = T->count.to_host(N, T->stream);
-  for (uint32_t y = 0; y < H; y++)
-    for (uint32_t x = 0; x < W; x++) {
-      size_t i = (size_t)y * W + x, o = ((size_t)(H - 1 - y) * W + x) * 4;
-      double n = cnt[i] ? (double)cnt[i] : 1.0;
-      if (kind == MTX_PIXELS_VARIANCE) {
-        double mean = (0.2126 * pl[i] + 0.7152 * pl[N + i] + 0.0722 * pl[2 * N + i]) / n;
-        double var = std::max(0.0, pl[12 * N + i] / n - mean * mean) / n;
-        pixels[o] = (float)var; pixels[o + 1] = pixels[o + 2] = 0.0f; pixels[o + 3] = 1.0f;
-        continue;
-      }
-      int base = kind == MTX_PIXELS_COLOR ? 0 : (kind == MTX_PIXELS_ALBEDO ? 4 : 8);
-      for (int k = 0; k < 4; k++) pixels[o + k] = (float)(pl[(size_t)(base + k) * N + i] / n);
-    }
+  resolve_pixels(pl.data(), cnt.data(), W, H, kind, pixels);
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }
@@ -569,6 +590,297 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
 int32_t mtx_target_destroy(mtx_handle target) {
   std::lock_guard<std::mutex> g(gRenderMutex);
   return gTargets.erase(target) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+}  // extern "C"
+
+// ====================================================================== server
+namespace mtx {
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServerJob {
+  std::shared_ptr<RenderSceneObj> scene;
+  RenderParams params{};
+  DevBuf<double> planes;
+  DevBuf<uint32_t> count;
+  uint32_t requested = 0, submitted = 0, completed = 0;
+  uint32_t state = MTX_JOB_QUEUED;
+  bool cancel = false, admitted = false;
+  double submitMs = 0, startMs = 0, completeMs = 0;
+  size_t bytes() const { return (size_t)params.pixelCount * (13 * sizeof(double) + sizeof(uint32_t)); }
+};
+
+struct ServerObj {
+  std::mutex m;
+  uint32_t maxPaths = 1u << 24, maxJobs = kMaxJobsPerRound, maxSamplesPerRound = 64;
+  uint64_t budget = 200ull << 30, used = 0, rounds = 0;
+  std::deque<uint64_t> queued;   // submission order
+  std::vector<uint64_t> running; // admission order
+  std::unordered_map<uint64_t, std::shared_ptr<ServerJob>> jobs;
+  WfBuffers::Owner wf;
+  WfConfig cfg;
+  DevBuf<JobSlot> slots;
+  DevBuf<uint64_t> rays;
+};
+
+static std::unordered_map<uint64_t, std::shared_ptr<ServerObj>> gServers;
+
+static std::shared_ptr<ServerObj> get_server(uint64_t h) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  auto it = gServers.find(h);
+  if (it == gServers.end()) throw Error(MTX_INVALID_HANDLE, "unknown server handle");
+  return it->second;
+}
+static std::shared_ptr<ServerJob> get_job(ServerObj &S, uint64_t h) {
+  auto it = S.jobs.find(h);
+  if (it == S.jobs.end()) throw Error(MTX_INVALID_HANDLE, "unknown job handle");
+  return it->second;
+}
+
+// One round: admission, slicing, one multi-job generation.
+static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
+  // Admission in arrival order while the accumulators fit the budget.
+  while (!S.queued.empty()) {
+    auto J = S.jobs.at(S.queued.front());
+    if (J->cancel) { J->state = MTX_JOB_CANCELLED; J->completeMs = now_ms(); S.queued.pop_front(); continue; }
+    if (S.used + J->bytes() > S.budget) break;  // waits for running or unreleased jobs to free memory
+    J->planes.alloc((size_t)J->params.pixelCount * 13);
+    J->count.alloc(J->params.pixelCount);
+    MTX_HIP(hipMemsetAsync(J->planes.ptr, 0, J->planes.bytes(), s));
+    MTX_HIP(hipMemsetAsync(J->count.ptr, 0, J->count.bytes(), s));
+    J->admitted = true;
+    S.used += J->bytes();
+    S.running.push_back(S.queued.front());
+    S.queued.pop_front();
+  }
+  // Slices of the round's path budget, oldest job first.
+  std::vector<JobSlot> slots;
+  std::vector<std::pair<std::shared_ptr<ServerJob>, uint32_t>> parts;
+  uint32_t paths = 0, pixels = 0;
+  int eventLimit = 0;
+  for (uint64_t h : S.running) {
+    if (slots.size() >= S.maxJobs) break;
+    auto J = S.jobs.at(h);
+    if (J->state != MTX_JOB_QUEUED && J->state != MTX_JOB_RUNNING) continue;
+    if (J->cancel) continue;
+    const uint32_t left = J->requested - J->completed;
+    const uint32_t fit = (S.maxPaths - paths) / J->params.pixelCount;
+    const uint32_t g = std::min(std::min(left, fit), S.maxSamplesPerRound);
+    if (g == 0) {
+      if (fit == 0) break;  // round full: later jobs wait (arrival order is kept)
+      continue;
+    }
+    JobSlot js{};
+    js.scene = J->scene->view;
+    js.params = J->params;
+    js.params.samples = g;
+    js.acc = Accumulators{J->planes.ptr, J->count.ptr, S.rays.ptr, J->params.pixelCount, nullptr};
+    js.pathBase = paths;
+    js.pixelPrefix = pixels;
+    slots.push_back(js);
+    parts.emplace_back(J, g);
+    paths += g * J->params.pixelCount;
+    pixels += J->params.pixelCount;
+    eventLimit = std::max(eventLimit, std::min((int)J->params.bounces * 3, 50));
+    if (J->state == MTX_JOB_QUEUED) { J->state = MTX_JOB_RUNNING; J->startMs = now_ms(); }
+  }
+  idle = slots.empty();
+  uint32_t finished = 0;
+  if (!idle) {
+    S.slots.upload(slots.data(), slots.size(), s);
+    launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg, s);
+    MTX_HIP(hipStreamSynchronize(s));
+    const double t = now_ms();
+    for (auto &pj : parts) {
+      pj.first->submitted += pj.second;
+      pj.first->completed += pj.second;
+      if (pj.first->completed >= pj.first->requested) {
+        pj.first->state = MTX_JOB_COMPLETE;
+        pj.first->completeMs = t;
+        finished++;
+      }
+    }
+    S.rounds++;
+  }
+  // Cancelled running jobs end between rounds.
+  for (uint64_t h : S.running) {
+    auto J = S.jobs.at(h);
+    if (J->cancel && J->state == MTX_JOB_RUNNING) { J->state = MTX_JOB_CANCELLED; J->completeMs = now_ms(); }
+  }
+  std::vector<uint64_t> keep;
+  for (uint64_t h : S.running) {
+    auto st = S.jobs.at(h)->state;
+    if (st == MTX_JOB_QUEUED || st == MTX_JOB_RUNNING) keep.push_back(h);
+  }
+  S.running.swap(keep);
+  return finished;
+}
+
+}  // namespace mtx
+
+extern "C" {
+
+int32_t mtx_server_create(const mtx_server_config *cfg, mtx_handle *out, char *err, size_t cap) {
+  API_TRY
+  MTX_REQUIRE(cfg && out, MTX_INVALID_ARGUMENT, "config/out is NULL");
+  MTX_REQUIRE(cfg->struct_size >= sizeof(mtx_server_config), MTX_INVALID_ARGUMENT, "mtx_server_config.struct_size too small");
+  auto S = std::make_shared<ServerObj>();
+  if (cfg->max_paths_per_round) S->maxPaths = cfg->max_paths_per_round;
+  if (cfg->max_jobs_per_round) S->maxJobs = std::min<uint32_t>(cfg->max_jobs_per_round, kMaxJobsPerRound);
+  if (cfg->max_samples_per_job_round) S->maxSamplesPerRound = cfg->max_samples_per_job_round;
+  if (cfg->hbm_budget_bytes) S->budget = cfg->hbm_budget_bytes;
+  S->rays.alloc(1);
+  MTX_HIP(hipMemset(S->rays.ptr, 0, 8));
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  uint64_t h = registry().next();
+  gServers[h] = S;
+  *out = h;
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
+int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_render_settings *settings,
+                          const mtx_camera_desc *camera, uint32_t samples, mtx_handle *job, char *err, size_t cap) {
+  API_TRY
+  MTX_REQUIRE(settings && camera && job, MTX_INVALID_ARGUMENT, "settings/camera/job is NULL");
+  MTX_REQUIRE(samples >= 1 && samples <= 16384, MTX_INVALID_ARGUMENT, "samples must be in 1..16384");
+  auto S = get_server(server);
+  auto J = std::make_shared<ServerJob>();
+  J->scene = get_scene(scene);
+  J->params = params_from(*settings, *camera);
+  std::lock_guard<std::mutex> g(S->m);
+  MTX_REQUIRE(J->params.pixelCount <= S->maxPaths, MTX_INVALID_ARGUMENT, "image larger than the round's path budget");
+  MTX_REQUIRE(J->bytes() <= S->budget, MTX_INVALID_ARGUMENT, "job accumulators exceed the HBM budget");
+  J->requested = samples;
+  J->submitMs = now_ms();
+  uint64_t h = registry().next();
+  S->jobs[h] = J;
+  S->queued.push_back(h);
+  *job = h;
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
+int32_t mtx_server_step(mtx_handle server, void *stream, uint32_t *completed, uint32_t *idle) {
+  API_TRY
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  bool isIdle = true;
+  uint32_t n = server_round(*S, (hipStream_t)stream, isIdle);
+  if (completed) *completed = n;
+  if (idle) *idle = isIdle ? 1u : 0u;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_job_info *info) {
+  API_TRY
+  MTX_REQUIRE(info, MTX_INVALID_ARGUMENT, "info is NULL");
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  auto J = get_job(*S, job);
+  info->struct_size = sizeof(mtx_job_info);
+  info->state = J->state;
+  info->requested_samples = J->requested;
+  info->submitted_samples = J->submitted;
+  info->completed_samples = J->completed;
+  info->base_samples = 0;
+  info->submit_ms = J->submitMs;
+  info->start_ms = J->startMs;
+  info->complete_ms = J->completeMs;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_job_read_pixels(mtx_handle server, mtx_handle job, uint32_t kind, float *pixels, size_t cap,
+                                   size_t *written) {
+  API_TRY
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  auto J = get_job(*S, job);
+  const uint32_t W = J->params.width, H = J->params.height;
+  const size_t N = (size_t)W * H;
+  if (written) *written = N * 4;
+  if (!pixels) return MTX_OK;
+  if (cap < N * 4) return MTX_BUFFER_TOO_SMALL;
+  MTX_REQUIRE(kind <= MTX_PIXELS_VARIANCE, MTX_INVALID_ARGUMENT, "unknown pixel kind");
+  if (!J->admitted || !J->planes.ptr) return MTX_INVALID_STATE;
+  std::vector<double> pl = J->planes.to_host(N * 13, 0);
+  std::vector<uint32_t> cnt = J->count.to_host(N, 0);
+  resolve_pixels(pl.data(), cnt.data(), W, H, kind, pixels);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_job_read_accumulators(mtx_handle server, mtx_handle job, double *sums, size_t capSums,
+                                         uint32_t *counts, size_t capCounts) {
+  API_TRY
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  auto J = get_job(*S, job);
+  const size_t N = J->params.pixelCount;
+  if (!J->admitted || !J->planes.ptr) return MTX_INVALID_STATE;
+  if (sums) {
+    if (capSums < N * 13) return MTX_BUFFER_TOO_SMALL;
+    interleave_planes(J->planes.to_host(N * 13, 0), N, sums);
+  }
+  if (counts) {
+    if (capCounts < N) return MTX_BUFFER_TOO_SMALL;
+    J->count.download(counts, N, 0);
+    MTX_HIP(hipStreamSynchronize(0));
+  }
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_job_cancel(mtx_handle server, mtx_handle job) {
+  API_TRY
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  auto J = get_job(*S, job);
+  if (J->state == MTX_JOB_QUEUED || J->state == MTX_JOB_RUNNING) J->cancel = true;
+  if (J->state == MTX_JOB_QUEUED && !J->admitted) {
+    J->state = MTX_JOB_CANCELLED;
+    J->completeMs = now_ms();
+    S->queued.erase(std::remove(S->queued.begin(), S->queued.end(), job), S->queued.end());
+  }
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_job_release(mtx_handle server, mtx_handle job) {
+  API_TRY
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  auto J = get_job(*S, job);
+  if (J->state == MTX_JOB_QUEUED || J->state == MTX_JOB_RUNNING) return MTX_INVALID_STATE;
+  if (J->admitted) S->used -= J->bytes();
+  S->jobs.erase(job);
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]) {
+  API_TRY
+  MTX_REQUIRE(stats, MTX_INVALID_ARGUMENT, "stats is NULL");
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  stats[0] = S->rounds;
+  S->rays.download(&stats[1], 1, 0);
+  MTX_HIP(hipStreamSynchronize(0));
+  stats[2] = S->running.size();
+  stats[3] = S->queued.size();
+  stats[4] = S->used;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_destroy(mtx_handle server) {
+  std::lock_guard<std::mutex> g(gRenderMutex);
+  return gServers.erase(server) ? MTX_OK : MTX_INVALID_HANDLE;
 }
 
 }  // extern "C"

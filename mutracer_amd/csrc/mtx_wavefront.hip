@@ -4,7 +4,7 @@
 // split at its two ray queries, so that traversal runs in small,
 // high-occupancy kernels and shading in its own kernel:
 //
-//   generate  camera rays for G samples of every pixel (G * pixels paths)
+//   generate  camera rays for every path of the generation
 //   per event e = 0 .. min(3 * bounces, 50):
 //     trace   closest hit for the paths in queue e          (LDS short stack)
 //     shade   shade_event(): emission, NEE candidate choice, BSDF sample,
@@ -12,14 +12,17 @@
 //             continues and its shadow ray to the shadow queue of event e
 //     shadow  any-hit for the shadow queue; adds the payload of each
 //             unoccluded ray to its path's radiance
-//   accumulate the G samples of each pixel, in sample order, into the FP64
-//             running sums and advance the per-pixel sample counter
+//   accumulate each pixel's samples, in sample order, into the FP64 running
+//             sums and advance the per-pixel sample counter
 //
-// Queues are compacted with one atomic per wave (ballot + popcount). Every
-// kernel reads its queue length from device memory, so a launch sequence
-// needs no host round trip and can be captured into a graph. Each path's
-// arithmetic is exactly the megakernel's, in the same order, so both produce
-// identical accumulators.
+// A generation holds the paths of one job (a render target) or of several
+// jobs at once (the batched server): kMulti kernels look up each path's job
+// slot, so the paths of different scenes, resolutions and sample counts share
+// the same queues and launches. Queues are compacted with one atomic per wave
+// (ballot + popcount). Every kernel reads its queue length from device
+// memory, so a launch sequence needs no host round trip. Each path's
+// arithmetic is exactly the megakernel's, in the same order, so every form
+// produces identical accumulators.
 #include "mtx_registry.h"
 #include "mtx_shade.h"
 #include "mtx_walk.h"
@@ -44,14 +47,33 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
   return base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
 }
 
-__device__ __forceinline__ const uint32_t *queue_of(const WfBuffers &W, int event) {
-  return (event & 1) ? W.qB : W.qA;
-}
-__device__ __forceinline__ uint32_t *queue_of_mut(const WfBuffers &W, int event) {
-  return (event & 1) ? W.qB : W.qA;
+__device__ __forceinline__ uint32_t *queue_of(const WfBuffers &W, int event) { return (event & 1) ? W.qB : W.qA; }
+
+template <bool kMulti>
+__device__ __forceinline__ const JobSlot &job_of(const WfLaunch &L, const WfBuffers &W, uint32_t p) {
+  return kMulti ? L.jobs[W.item[p]] : L.one;
 }
 
-__global__ void __launch_bounds__(256) k_wf_generate(const RenderParams P, Accumulators acc, WfBuffers W) {
+// Sampler of path p: pixel and sample index within its job.
+__device__ __forceinline__ Smp path_smp(const JobSlot &J, uint32_t p) {
+  const RenderParams &P = J.params;
+  const uint32_t rel = p - J.pathBase;
+  const uint32_t local = rel % P.pixelCount, s = rel / P.pixelCount;
+  const uint32_t pix = P.pixelBase + local;
+  return path_sampler(P, pix % P.width, pix / P.width, J.acc.count[pix] + P.sampleOffset + s);
+}
+
+__device__ __forceinline__ void store_path(const WfBuffers &W, uint32_t p, const PathRegs &pr) {
+  W.o[p] = make_float4(pr.ro.x, pr.ro.y, pr.ro.z, pr.scatPdf);
+  W.d[p] = make_float4(pr.rd.x, pr.rd.y, pr.rd.z, pr.alpha);
+  W.li[p] = make_float4(pr.li.x, pr.li.y, pr.li.z, intf((int32_t)pr.flags));
+  W.beta[p] = make_float4(pr.beta.x, pr.beta.y, pr.beta.z, intf(pr.curTri));
+  W.alb[p] = make_float4(pr.auxAlbedo.x, pr.auxAlbedo.y, pr.auxAlbedo.z, intf(pr.curSlot));
+  W.nrm[p] = make_float4(pr.auxNormal.x, pr.auxNormal.y, pr.auxNormal.z, 0.0f);
+}
+
+template <bool kMulti>
+__global__ void __launch_bounds__(256) k_wf_generate(const WfLaunch L, WfBuffers W) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid == 0) {
@@ -59,65 +81,79 @@ __global__ void __launch_bounds__(256) k_wf_generate(const RenderParams P, Accum
     for (int k = 1; k < kWfCounters; k++) W.counts[k] = 0u;
   }
   for (uint32_t p = tid; p < W.nPaths; p += stride) {
-    const uint32_t local = p % P.pixelCount, s = p / P.pixelCount;
-    const uint32_t pix = P.pixelBase + local;
-    const uint32_t px = pix % P.width, py = pix / P.width;
-    Smp smp = path_sampler(P, px, py, acc.count[pix] + P.sampleOffset + s);
+    uint32_t j = 0;
+    if (kMulti) {  // last slot with pathBase <= p
+      uint32_t lo = 0, hi = L.nJobs;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.jobs[mid].pathBase <= p) lo = mid; else hi = mid;
+      }
+      j = lo;
+      W.item[p] = (uint16_t)j;
+    }
+    const JobSlot &J = kMulti ? L.jobs[j] : L.one;
+    Smp smp = path_smp(J, p);
     PathRegs pr;
-    init_path(P, smp, pr);
-    W.o[p] = make_float4(pr.ro.x, pr.ro.y, pr.ro.z, pr.scatPdf);
-    W.d[p] = make_float4(pr.rd.x, pr.rd.y, pr.rd.z, pr.alpha);
-    W.li[p] = make_float4(pr.li.x, pr.li.y, pr.li.z, intf((int32_t)pr.flags));
-    W.beta[p] = make_float4(pr.beta.x, pr.beta.y, pr.beta.z, intf(pr.curTri));
-    W.alb[p] = make_float4(pr.auxAlbedo.x, pr.auxAlbedo.y, pr.auxAlbedo.z, intf(pr.curSlot));
-    W.nrm[p] = make_float4(pr.auxNormal.x, pr.auxNormal.y, pr.auxNormal.z, 0.0f);
+    init_path(J.params, smp, pr);
+    store_path(W, p, pr);
     W.qA[p] = p;
   }
 }
 
-template <bool kCount>
+__device__ __forceinline__ void flush_work(uint64_t *work, const uint64_t v0[4], const int slots[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint64_t v = v0[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (lane_id() == 0 && v) atomicAdd((unsigned long long *)work + slots[k], (unsigned long long)v);
+  }
+}
+
+template <bool kMulti, bool kCount>
 __global__ void __launch_bounds__(kWfTraceBlock)
-k_wf_trace(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *__restrict__ spill,
-           Accumulators acc) {
+k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLdsStack * kWfTraceBlock];
   const uint32_t n = W.counts[event];
   if (n == 0u) return;
   LaneStack st{lds + threadIdx.x, kWfTraceBlock,
                spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
-  const AccelView av = *accel;
   const uint32_t *q = queue_of(W, event);
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  if (tid == 0) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)n);
+  if (tid == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
   TraceCounters tc{0u, 0u, 0u};
   uint32_t hits = 0;
+  AccelView one{};
+  if (!kMulti) one = *L.one.scene.accel;
   for (uint32_t i = tid; i < n; i += stride) {
     const uint32_t p = q[i];
     const float4 o = W.o[p], d = W.d[p];
     const int32_t curTri = fint(W.beta[p].w), curSlot = fint(W.alb[p].w);
     TraceHit h;
-    traverse<false, kCount>(av, f3of(o), f3of(d), kRayMaxDistance, event != 0, trace_depth(event), curTri, curSlot,
-                            st, h, &tc);
+    if (kMulti) {
+      const AccelView av = *job_of<true>(L, W, p).scene.accel;
+      traverse<false, kCount>(av, f3of(o), f3of(d), kRayMaxDistance, event != 0, trace_depth(event), curTri, curSlot,
+                              st, h, &tc);
+    } else {
+      traverse<false, kCount>(one, f3of(o), f3of(d), kRayMaxDistance, event != 0, trace_depth(event), curTri,
+                              curSlot, st, h, &tc);
+    }
     W.hitA[p] = make_float4(h.t, h.bary.x, h.bary.y, h.bary.z);
     W.hitB[p] = make_int2(h.hit ? h.slot : -1, h.tri);
     if (kCount) hits += h.hit ? 1u : 0u;
   }
   if (kCount) {
-    uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, hits};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
-      if (lane_id() == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + k, (unsigned long long)v[k]);
-    }
+    const uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, hits};
+    const int slots[4] = {0, 1, 2, 3};
+    flush_work(L.work, v, slots);
   }
 }
 
-__global__ void __launch_bounds__(kWfShadeBlock)
-k_wf_shade(const GpuScene S, const RenderParams P, Accumulators acc, WfBuffers W, int event) {
+template <bool kMulti>
+__global__ void __launch_bounds__(kWfShadeBlock) k_wf_shade(const WfLaunch L, WfBuffers W, int event) {
   const uint32_t n = W.counts[event];
   if (n == 0u) return;
-  const Lut L{S.lut};
   const uint32_t *q = queue_of(W, event);
-  uint32_t *qNext = queue_of_mut(W, event + 1);
+  uint32_t *qNext = queue_of(W, event + 1);
   uint32_t *nNext = W.counts + event + 1;
   uint32_t *nShadow = W.counts + kWfShadowBase + event;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -132,6 +168,10 @@ k_wf_shade(const GpuScene S, const RenderParams P, Accumulators acc, WfBuffers W
     PathRegs pr;
     if (active) {
       p = q[i];
+      const JobSlot &J = job_of<kMulti>(L, W, p);
+      Smp smp = path_smp(J, p);
+      const GpuScene &S = J.scene;
+      const RenderParams &P = J.params;
       const float4 o = W.o[p], d = W.d[p], l = W.li[p], b = W.beta[p], al = W.alb[p], nr = W.nrm[p];
       pr.ro = f3of(o); pr.scatPdf = o.w;
       pr.rd = f3of(d); pr.alpha = d.w;
@@ -157,16 +197,9 @@ k_wf_shade(const GpuScene S, const RenderParams P, Accumulators acc, WfBuffers W
         const f3 oo = xform(w2l, pr.ro, 1.0f), od = xform(w2l, pr.rd, 0.0f);
         h.position = oo + h.t * od;
       }
-      const uint32_t local = p % P.pixelCount, s = p / P.pixelCount;
-      const uint32_t pix = P.pixelBase + local;
-      Smp smp = path_sampler(P, pix % P.width, pix / P.width, acc.count[pix] + P.sampleOffset + s);
-      alive = shade_event(S, P, L, pr, h, event, smp, sr);
-      W.o[p] = make_float4(pr.ro.x, pr.ro.y, pr.ro.z, pr.scatPdf);
-      W.d[p] = make_float4(pr.rd.x, pr.rd.y, pr.rd.z, pr.alpha);
-      W.li[p] = make_float4(pr.li.x, pr.li.y, pr.li.z, intf((int32_t)pr.flags));
-      W.beta[p] = make_float4(pr.beta.x, pr.beta.y, pr.beta.z, intf(pr.curTri));
-      W.alb[p] = make_float4(pr.auxAlbedo.x, pr.auxAlbedo.y, pr.auxAlbedo.z, intf(pr.curSlot));
-      W.nrm[p] = make_float4(pr.auxNormal.x, pr.auxNormal.y, pr.auxNormal.z, 0.0f);
+      const Lut lut{S.lut};
+      alive = shade_event(S, P, lut, pr, h, event, smp, sr);
+      store_path(W, p, pr);
     }
     const uint32_t slot = wave_append(nNext, active && alive);
     if (active && alive) qNext[slot] = p;
@@ -180,60 +213,66 @@ k_wf_shade(const GpuScene S, const RenderParams P, Accumulators acc, WfBuffers W
   }
 }
 
-template <bool kCount>
+__device__ __forceinline__ void add_payload(const WfBuffers &W, uint32_t p, float4 c) {
+  float4 l = W.li[p];
+  l.x = l.x + c.x;
+  l.y = l.y + c.y;
+  l.z = l.z + c.z;
+  W.li[p] = l;
+}
+
+template <bool kMulti, bool kCount>
 __global__ void __launch_bounds__(kWfTraceBlock)
-k_wf_shadow(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *__restrict__ spill,
-            Accumulators acc) {
+k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLdsStack * kWfTraceBlock];
   const uint32_t n = W.counts[kWfShadowBase + event];
   if (n == 0u) return;
   LaneStack st{lds + threadIdx.x, kWfTraceBlock,
                spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
-  const AccelView av = *accel;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  if (tid == 0) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)n);
+  if (tid == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
   TraceCounters tc{0u, 0u, 0u};
+  AccelView one{};
+  if (!kMulti) one = *L.one.scene.accel;
   for (uint32_t i = tid; i < n; i += stride) {
     const float4 a = W.s0[i], b = W.s1[i], c = W.s2[i];
+    const uint32_t p = (uint32_t)fint(b.w);
     TraceHit h;
-    if (!traverse<true, kCount>(av, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc)) {
-      const uint32_t p = (uint32_t)fint(b.w);
-      float4 l = W.li[p];
-      l.x = l.x + c.x;
-      l.y = l.y + c.y;
-      l.z = l.z + c.z;
-      W.li[p] = l;
+    bool occluded;
+    if (kMulti) {
+      const AccelView av = *job_of<true>(L, W, p).scene.accel;
+      occluded = traverse<true, kCount>(av, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc);
+    } else {
+      occluded = traverse<true, kCount>(one, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc);
     }
+    if (!occluded) add_payload(W, p, c);
   }
   if (kCount) {
-    uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, 0};
-    v[3] = (blockIdx.x == 0 && threadIdx.x == 0) ? n : 0u;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
-      if (lane_id() == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + (k < 3 ? 5 + k : 4), (unsigned long long)v[k]);
-    }
+    const uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, (blockIdx.x == 0 && threadIdx.x == 0) ? (uint64_t)n : 0u};
+    const int slots[4] = {5, 6, 7, 4};
+    flush_work(L.work, v, slots);
   }
 }
 
 // Persistent closest-hit / any-hit kernels over Walker: each lane takes a
 // new queue entry (wave-aggregated atomic on counts[kWfFetchBase + 2e (+1)])
 // as soon as its previous ray has finished.
-template <bool kShadow, bool kCount, int kLds>
+template <bool kMulti, bool kShadow, bool kCount, int kLds>
 __global__ void __launch_bounds__(kWfTraceBlock)
-k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *__restrict__ spill, Accumulators acc) {
+k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLds * kWfTraceBlock];
   const uint32_t n = W.counts[kShadow ? kWfShadowBase + event : event];
   if (n == 0u) return;
   uint32_t *fetch = W.counts + kWfFetchBase + 2 * event + (kShadow ? 1 : 0);
   WalkStack<kLds> st{lds + threadIdx.x, kWfTraceBlock,
                      spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLds)};
-  const AccelView av = *accel;
   const uint32_t *q = queue_of(W, event);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
+  AccelView av{};
+  if (!kMulti) av = *L.one.scene.accel;
   Walker<kShadow, kCount> w;
   bool have = false, exhausted = false;
-  uint32_t entry = 0;
+  uint32_t entry = 0, path = 0;
   uint64_t totals[4] = {0, 0, 0, 0};
   const bool doubleSided = kShadow ? true : event != 0;
   const int bounces = kShadow ? 1 : trace_depth(event);
@@ -245,12 +284,15 @@ k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *
         entry = idx;
         if (kShadow) {
           const float4 a = W.s0[idx], b = W.s1[idx], c = W.s2[idx];
+          path = (uint32_t)fint(b.w);
+          if (kMulti) av = *job_of<true>(L, W, path).scene.accel;
           w.init(av, f3of(a), f3of(b), a.w, doubleSided, bounces, fint(c.w), W.s3[idx], st);
         } else {
-          const uint32_t p = q[idx];
-          const float4 o = W.o[p], d = W.d[p];
-          w.init(av, f3of(o), f3of(d), kRayMaxDistance, doubleSided, bounces, fint(W.beta[p].w), fint(W.alb[p].w),
-                 st);
+          path = q[idx];
+          if (kMulti) av = *job_of<true>(L, W, path).scene.accel;
+          const float4 o = W.o[path], d = W.d[path];
+          w.init(av, f3of(o), f3of(d), kRayMaxDistance, doubleSided, bounces, fint(W.beta[path].w),
+                 fint(W.alb[path].w), st);
         }
         have = true;
       } else {
@@ -261,19 +303,10 @@ k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *
     if (have && (w.done || w.step(av, st))) {
       have = false;
       if (kShadow) {
-        if (!w.hit) {
-          const uint32_t p = (uint32_t)fint(W.s1[entry].w);
-          const float4 c = W.s2[entry];
-          float4 l = W.li[p];
-          l.x = l.x + c.x;
-          l.y = l.y + c.y;
-          l.z = l.z + c.z;
-          W.li[p] = l;
-        }
+        if (!w.hit) add_payload(W, path, W.s2[entry]);
       } else {
-        const uint32_t p = q[entry];
-        W.hitA[p] = make_float4(w.t, w.bary.x, w.bary.y, w.bary.z);
-        W.hitB[p] = make_int2(w.hit ? w.hitSlot : -1, w.hitTri);
+        W.hitA[path] = make_float4(w.t, w.bary.x, w.bary.y, w.bary.z);
+        W.hitB[path] = make_int2(w.hit ? w.hitSlot : -1, w.hitTri);
       }
       if (kCount) {
         totals[0] += w.cnt.boxes;
@@ -285,25 +318,35 @@ k_wf_walk(const AccelView *__restrict__ accel, WfBuffers W, int event, int32_t *
   }
   if (kCount) {
     if (kShadow && blockIdx.x == 0 && threadIdx.x == 0) totals[3] = n;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint64_t v = totals[k];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-      const int slot = k < 3 ? (kShadow ? 5 + k : k) : (kShadow ? 4 : 3);
-      if (lane_id() == 0 && v) atomicAdd((unsigned long long *)acc.work + slot, (unsigned long long)v);
-    }
+    const int slots[4] = {kShadow ? 5 : 0, kShadow ? 6 : 1, kShadow ? 7 : 2, kShadow ? 4 : 3};
+    flush_work(L.work, totals, slots);
   }
 }
 
-__global__ void __launch_bounds__(256) k_wf_accumulate(const RenderParams P, Accumulators acc, WfBuffers W) {
+template <bool kMulti>
+__global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffers W) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t local = blockIdx.x * blockDim.x + threadIdx.x; local < P.pixelCount; local += stride) {
+  const uint32_t total = kMulti ? L.totalPixels : L.one.params.pixelCount;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    uint32_t j = 0;
+    if (kMulti) {  // last slot with pixelPrefix <= g
+      uint32_t lo = 0, hi = L.nJobs;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.jobs[mid].pixelPrefix <= g) lo = mid; else hi = mid;
+      }
+      j = lo;
+    }
+    const JobSlot &J = kMulti ? L.jobs[j] : L.one;
+    const RenderParams &P = J.params;
+    const Accumulators &acc = J.acc;
+    const uint32_t local = g - J.pixelPrefix;
     const uint32_t pix = P.pixelBase + local;
     double sums[13];
 #pragma unroll
     for (int c = 0; c < 13; c++) sums[c] = acc.planes[(size_t)c * acc.pixels + pix];
-    for (uint32_t s = 0; s < W.gen; s++) {
-      const uint32_t p = s * P.pixelCount + local;
+    for (uint32_t s = 0; s < P.samples; s++) {
+      const uint32_t p = J.pathBase + s * P.pixelCount + local;
       const float4 l = W.li[p], d = W.d[p], al = W.alb[p], nr = W.nrm[p];
       PathRegs pr;
       pr.li = f3of(l);
@@ -315,7 +358,7 @@ __global__ void __launch_bounds__(256) k_wf_accumulate(const RenderParams P, Acc
     }
 #pragma unroll
     for (int c = 0; c < 13; c++) acc.planes[(size_t)c * acc.pixels + pix] = sums[c];
-    acc.count[pix] += W.gen;
+    acc.count[pix] += P.samples;
   }
 }
 
@@ -340,109 +383,151 @@ void WfBuffers::Owner::ensure(uint32_t paths) {
   if (paths <= capacity) return;
   o.alloc(paths); d.alloc(paths); li.alloc(paths); beta.alloc(paths); alb.alloc(paths); nrm.alloc(paths);
   hitA.alloc(paths); hitB.alloc(paths); qA.alloc(paths); qB.alloc(paths);
-  s0.alloc(paths); s1.alloc(paths); s2.alloc(paths); s3.alloc(paths);
+  s0.alloc(paths); s1.alloc(paths); s2.alloc(paths); s3.alloc(paths); item.alloc(paths);
   if (!counts.ptr) counts.alloc(kWfCounters);
   capacity = paths;
 }
 size_t WfBuffers::Owner::bytes() const {
-  return o.bytes() * 6 + hitA.bytes() + hitB.bytes() + qA.bytes() * 2 + s0.bytes() * 3 + s3.bytes() + counts.bytes();
+  return o.bytes() * 6 + hitA.bytes() + hitB.bytes() + qA.bytes() * 2 + s0.bytes() * 3 + s3.bytes() + item.bytes() +
+         counts.bytes();
 }
-WfBuffers WfBuffers::Owner::view(uint32_t nPaths, uint32_t gen) const {
+WfBuffers WfBuffers::Owner::view(uint32_t nPaths) const {
   WfBuffers W;
   W.o = o.ptr; W.d = d.ptr; W.li = li.ptr; W.beta = beta.ptr; W.alb = alb.ptr; W.nrm = nrm.ptr;
   W.hitA = hitA.ptr; W.hitB = hitB.ptr; W.qA = qA.ptr; W.qB = qB.ptr;
-  W.s0 = s0.ptr; W.s1 = s1.ptr; W.s2 = s2.ptr; W.s3 = s3.ptr;
+  W.s0 = s0.ptr; W.s1 = s1.ptr; W.s2 = s2.ptr; W.s3 = s3.ptr; W.item = item.ptr;
   W.counts = counts.ptr;
   W.nPaths = nPaths;
-  W.gen = gen;
   return W;
 }
 
-template <bool kShadow, int kLds>
-static void launch_walk_lds(uint32_t grid, hipStream_t s, const AccelView *accel, const WfBuffers &W, int e,
-                            int32_t *spill, const Accumulators &acc) {
-  if (acc.work)
-    hipLaunchKernelGGL((k_wf_walk<kShadow, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, accel, W, e, spill, acc);
+template <bool kMulti, bool kShadow, int kLds>
+static void launch_walk_lds(uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W, int e,
+                            int32_t *spill) {
+  if (L.work)
+    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
   else
-    hipLaunchKernelGGL((k_wf_walk<kShadow, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, accel, W, e, spill, acc);
+    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
 }
-template <bool kShadow>
-static void launch_walk(const WfConfig &cfg, uint32_t grid, hipStream_t s, const AccelView *accel, const WfBuffers &W,
-                        int e, int32_t *spill, const Accumulators &acc) {
-  if (cfg.ldsStack <= 16) launch_walk_lds<kShadow, 16>(grid, s, accel, W, e, spill, acc);
-  else if (cfg.ldsStack <= 24) launch_walk_lds<kShadow, 24>(grid, s, accel, W, e, spill, acc);
-  else launch_walk_lds<kShadow, 32>(grid, s, accel, W, e, spill, acc);
+template <bool kMulti, bool kShadow>
+static void launch_walk(const WfConfig &cfg, uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
+                        int e, int32_t *spill) {
+  if (cfg.ldsStack <= 16) launch_walk_lds<kMulti, kShadow, 16>(grid, s, L, W, e, spill);
+  else if (cfg.ldsStack <= 24) launch_walk_lds<kMulti, kShadow, 24>(grid, s, L, W, e, spill);
+  else launch_walk_lds<kMulti, kShadow, 32>(grid, s, L, W, e, spill);
 }
-static const void *walk_kernel(int lds) {
-  if (lds <= 16) return (const void *)k_wf_walk<false, false, 16>;
-  if (lds <= 24) return (const void *)k_wf_walk<false, false, 24>;
-  return (const void *)k_wf_walk<false, false, 32>;
+
+struct Grids {
+  uint32_t trace, walk, shade, cus;
+};
+static Grids grids_for(const WfConfig &cfg) {
+  static int traceBpc = 0, shadeBpc = 0, walkBpc[3] = {0, 0, 0};
+  const int cus = cu_count();
+  if (!traceBpc) {
+    traceBpc = std::max(blocks_per_cu((const void *)k_wf_trace<false, false>, kWfTraceBlock, 0),
+                        blocks_per_cu((const void *)k_wf_shadow<false, false>, kWfTraceBlock, 0));
+    shadeBpc = blocks_per_cu((const void *)k_wf_shade<false>, kWfShadeBlock, 0);
+  }
+  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
+  if (!walkBpc[li]) {
+    const void *k = li == 0 ? (const void *)k_wf_walk<false, false, false, 16>
+                  : li == 1 ? (const void *)k_wf_walk<false, false, false, 24>
+                            : (const void *)k_wf_walk<false, false, false, 32>;
+    walkBpc[li] = blocks_per_cu(k, kWfTraceBlock, 0);
+  }
+  Grids g;
+  g.cus = (uint32_t)cus;
+  g.trace = (uint32_t)(cus * traceBpc * std::max(1, cfg.traceOversub));
+  g.walk = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
+  g.shade = (uint32_t)(cus * shadeBpc * std::max(1, cfg.shadeOversub));
+  return g;
+}
+
+// Launches one generation: generate, the event loop, accumulate.
+template <bool kMulti>
+static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit, uint32_t pixels, const WfConfig &cfg,
+                           const Grids &g, int32_t *spill, hipStream_t s, LaunchTimer *timer) {
+  auto tb = [&](int c) { if (timer) timer->begin(c); };
+  auto te = [&]() { if (timer) timer->end(); };
+  const uint32_t genGrid = std::min<uint32_t>((W.nPaths + 255) / 256, g.cus * 16u);
+  tb(0);
+  hipLaunchKernelGGL(k_wf_generate<kMulti>, dim3(genGrid), dim3(256), 0, s, L, W);
+  te();
+  for (int e = 0; e <= eventLimit; e++) {
+    tb(1);
+    if (cfg.walker)
+      launch_walk<kMulti, false>(cfg, g.walk, s, L, W, e, spill);
+    else if (L.work)
+      hipLaunchKernelGGL((k_wf_trace<kMulti, true>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    else
+      hipLaunchKernelGGL((k_wf_trace<kMulti, false>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    te();
+    tb(2);
+    hipLaunchKernelGGL(k_wf_shade<kMulti>, dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e);
+    te();
+    tb(3);
+    if (cfg.walker)
+      launch_walk<kMulti, true>(cfg, g.walk, s, L, W, e, spill);
+    else if (L.work)
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, true>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    else
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, false>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    te();
+  }
+  const uint32_t accGrid = std::min<uint32_t>((pixels + 255) / 256, g.cus * 16u);
+  tb(0);
+  hipLaunchKernelGGL(k_wf_accumulate<kMulti>, dim3(accGrid), dim3(256), 0, s, L, W);
+  te();
+  MTX_HIP(hipGetLastError());
 }
 
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, const Accumulators &acc,
                              WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer) {
-  auto tb = [&](int c) { if (timer) timer->begin(c); };
-  auto te = [&]() { if (timer) timer->end(); };
   if (!p0.pixelCount || !p0.samples) return;
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
-  const int cus = cu_count();
-  static int traceBpc = 0, shadowBpc = 0, shadeBpc = 0;
-  if (!traceBpc) {
-    traceBpc = blocks_per_cu((const void *)k_wf_trace<false>, kWfTraceBlock, 0);
-    shadowBpc = blocks_per_cu((const void *)k_wf_shadow<false>, kWfTraceBlock, 0);
-    shadeBpc = blocks_per_cu((const void *)k_wf_shade, kWfShadeBlock, 0);
-  }
-  const uint32_t traceGrid = (uint32_t)(cus * std::max(traceBpc, shadowBpc) * std::max(1, cfg.traceOversub));
-  const uint32_t shadeGrid = (uint32_t)(cus * shadeBpc * std::max(1, cfg.shadeOversub));
-  static int walkBpc[3] = {0, 0, 0};
-  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
-  if (!walkBpc[li]) walkBpc[li] = blocks_per_cu(walk_kernel(cfg.ldsStack), kWfTraceBlock, 0);
-  const uint32_t walkGrid = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
-  int32_t *spill = spill_for(s, std::max(traceGrid, walkGrid), kWfTraceBlock);
+  const Grids g = grids_for(cfg);
+  int32_t *spill = spill_for(s, std::max(g.trace, g.walk), kWfTraceBlock);
   const int eventLimit = std::min((int)p0.bounces * 3, 50);
   uint32_t done = 0;
   while (done < p0.samples) {
     const uint32_t gen = std::min(genMax, p0.samples - done);
     const uint32_t nPaths = gen * p0.pixelCount;
     buf.ensure(nPaths);
-    WfBuffers W = buf.view(nPaths, gen);
-    RenderParams P = p0;
-    P.samples = gen;
-    const uint32_t genGrid = std::min<uint32_t>((nPaths + 255) / 256, (uint32_t)cus * 16u);
-    tb(0);
-    hipLaunchKernelGGL(k_wf_generate, dim3(genGrid), dim3(256), 0, s, P, acc, W);
-    te();
-    for (int e = 0; e <= eventLimit; e++) {
-      tb(1);
-      if (cfg.walker) {
-        launch_walk<false>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
-      } else if (acc.work) {
-        hipLaunchKernelGGL(k_wf_trace<true>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      } else {
-        hipLaunchKernelGGL(k_wf_trace<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      }
-      te();
-      tb(2);
-      hipLaunchKernelGGL(k_wf_shade, dim3(shadeGrid), dim3(kWfShadeBlock), 0, s, scene, P, acc, W, e);
-      te();
-      tb(3);
-      if (cfg.walker) {
-        launch_walk<true>(cfg, walkGrid, s, scene.accel, W, e, spill, acc);
-      } else if (acc.work) {
-        hipLaunchKernelGGL(k_wf_shadow<true>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      } else {
-        hipLaunchKernelGGL(k_wf_shadow<false>, dim3(traceGrid), dim3(kWfTraceBlock), 0, s, scene.accel, W, e, spill, acc);
-      }
-      te();
-    }
-    const uint32_t accGrid = std::min<uint32_t>((p0.pixelCount + 255) / 256, (uint32_t)cus * 16u);
-    tb(0);
-    hipLaunchKernelGGL(k_wf_accumulate, dim3(accGrid), dim3(256), 0, s, P, acc, W);
-    te();
-    MTX_HIP(hipGetLastError());
+    const WfBuffers W = buf.view(nPaths);
+    WfLaunch L{};
+    L.one.scene = scene;
+    L.one.params = p0;
+    L.one.params.samples = gen;
+    L.one.acc = acc;
+    L.one.pathBase = 0;
+    L.one.pixelPrefix = 0;
+    L.jobs = nullptr;
+    L.nJobs = 1;
+    L.totalPixels = p0.pixelCount;
+    L.rays = acc.rays;
+    L.work = acc.work;
+    run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, s, timer);
     done += gen;
   }
+}
+
+void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
+                             int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
+                             hipStream_t s, LaunchTimer *timer) {
+  if (!nJobs || !nPaths) return;
+  MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
+  const Grids g = grids_for(cfg);
+  int32_t *spill = spill_for(s, std::max(g.trace, g.walk), kWfTraceBlock);
+  buf.ensure(nPaths);
+  const WfBuffers W = buf.view(nPaths);
+  WfLaunch L{};
+  L.jobs = devSlots;
+  L.nJobs = nJobs;
+  L.totalPixels = totalPixels;
+  L.rays = rays;
+  L.work = nullptr;
+  run_generation<true>(L, W, std::min(eventLimit, 50), totalPixels, cfg, g, spill, s, timer);
 }
 
 }  // namespace mtx

@@ -1,0 +1,123 @@
+"""Batched render server: every job's result is the same alone and in the mix.
+
+Jobs of different scenes, resolutions, sample counts, bounce limits, NEE
+candidate counts and sampling patterns share each round's wavefront
+generation. Each job keeps its own Sobol sample indices and FP64 accumulation
+order, so its accumulators must equal (bit for bit) those of the same job
+rendered alone on a RenderTarget, and the oracle's.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from mutracer_amd import render, scene, serve
+    scenes = [scene.cornell_box(), scene.instanced_field(n_instances=9, freq=10, env_size=(128, 64)),
+              scene.instanced_field(n_instances=5, freq=6, env_size=(64, 32), seed=3)]
+    rs = [render.RenderScene(s) for s in scenes]
+    return render, serve, scenes, rs
+
+
+JOBS = [  # (scene, width, height, samples, kwargs)
+    (0, 40, 30, 7, dict(bounces=8, direct_light_candidates=1)),
+    (1, 64, 36, 5, dict(bounces=12, direct_light_candidates=4)),
+    (2, 33, 17, 12, dict(bounces=6, direct_light_candidates=2, clamping=3.0)),
+    (0, 24, 24, 9, dict(bounces=5, direct_light_candidates=4, pattern="zsobol", max_samples=16, sequence_seed=9)),
+    (1, 50, 20, 3, dict(bounces=3, direct_light_candidates=1, sample_sequence_offset=11)),
+]
+
+
+def _alone(render, rs, si, w, h, spp, kw):
+    t = render.RenderTarget(rs[si], w, h, max_samples=kw.get("max_samples", spp),
+                            **{k: v for k, v in kw.items() if k != "max_samples"})
+    t.render(spp)
+    return t.read_accumulators()
+
+
+def _serve_all(server, rs, jobs):
+    handles = [server.submit(rs[si], w, h, spp, **kw) for si, w, h, spp, kw in jobs]
+    rounds = 0
+    while True:
+        _, idle = server.step()
+        if idle:
+            break
+        rounds += 1
+        assert rounds < 1000
+    return handles, rounds
+
+
+@pytest.mark.parametrize("max_jobs,slice_", [(0, 4), (0, 64), (1, 3)])
+def test_mixed_equals_alone(env, max_jobs, slice_):
+    render, serve, scenes, rs = env
+    server = serve.Server(max_jobs_per_round=max_jobs, max_samples_per_job_round=slice_)
+    handles, rounds = _serve_all(server, rs, JOBS)
+    assert rounds >= max(spp for *_, spp, _ in JOBS) // slice_
+    for h, (si, w, hh, spp, kw) in zip(handles, JOBS):
+        info = server.info(h)
+        assert info["state"] == "complete" and info["completed"] == spp
+        sums, counts = server.read_accumulators(h)
+        ref_sums, ref_counts = _alone(render, rs, si, w, hh, spp, kw)
+        assert np.array_equal(counts, ref_counts)
+        assert np.array_equal(sums, ref_sums), f"job {(si, w, hh, spp)} differs in the mix"
+    assert server.stats()["rays"] > 0
+
+
+def test_job_matches_oracle(env, orc):
+    render, serve, scenes, rs = env
+    server = serve.Server(max_samples_per_job_round=2)
+    other = server.submit(rs[0], 32, 32, 6, bounces=6)
+    job = server.submit(rs[1], 48, 27, 5, bounces=10, direct_light_candidates=3)
+    _serve_all(server, rs, [])
+    sums, _ = server.read_accumulators(job)
+    ref = orc.OracleRenderer(scenes[1], 48, 27, bounces=10, candidates=3, max_samples=5).render(0, 5)
+    assert np.array_equal(sums, ref)
+    assert server.info(other)["state"] == "complete"
+
+
+def test_admission_budget_and_release(env):
+    render, serve, scenes, rs = env
+    one_job = 40 * 30 * (13 * 8 + 4)
+    server = serve.Server(max_samples_per_job_round=2, hbm_budget_bytes=one_job + 1)
+    a = server.submit(rs[0], 40, 30, 4)
+    b = server.submit(rs[0], 40, 30, 4)
+    server.step()
+    assert server.info(a)["state"] == "running" and server.info(b)["state"] == "queued"
+    server.step()
+    assert server.info(a)["state"] == "complete" and server.info(b)["state"] == "queued"
+    server.step()  # a still holds its accumulators: b waits
+    assert server.info(b)["state"] == "queued"
+    server.release(a)
+    server.step()
+    assert server.info(b)["state"] == "running"
+    server.step()
+    assert server.info(b)["state"] == "complete"
+    assert server.stats()["admitted_bytes"] == one_job
+
+
+def test_cancel(env):
+    render, serve, scenes, rs = env
+    server = serve.Server(max_jobs_per_round=1, max_samples_per_job_round=1)
+    a = server.submit(rs[0], 16, 16, 3)
+    b = server.submit(rs[0], 16, 16, 3)
+    server.cancel(b)
+    assert server.info(b)["state"] == "cancelled"
+    server.step()
+    server.cancel(a)
+    server.step()
+    assert server.info(a)["state"] == "cancelled" and server.info(a)["completed"] == 1
+    _, idle = server.step()
+    assert idle
+
+
+def test_stream_latencies(env):
+    render, serve, scenes, rs = env
+    server = serve.Server(max_samples_per_job_round=8)
+    jobs = [dict(render_scene=rs[i % 3], width=32 + 8 * i, height=24, samples=2 + i % 5) for i in range(8)]
+    arrivals = np.cumsum(np.full(8, 0.005))
+    lat, makespan, _ = serve.run_stream(server, jobs, arrivals)
+    assert np.all(lat >= 0) and makespan > 0
