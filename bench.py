@@ -99,6 +99,50 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
 
 
+SERVE_RES = [(128, 128), (256, 192), (480, 270), (960, 540)]  # the stream's {512^2, 1024x768, 1080p, 4K} / 4 per axis
+
+
+def serving_benchmark(n_jobs=64, rate=40.0):
+    """Scaled serving stream: 8 resident instanced-field scenes with 10k-4M
+    triangles (log-uniform, seed 5), Poisson arrivals at `rate` jobs/s (seed 6),
+    resolutions SERVE_RES, 4-64 spp log-uniform, depth 8, 4 NEE candidates.
+    Runs the same stream through the batched server and through the plain
+    one-job-at-a-time form; returns aggregate throughput and latencies."""
+    import math
+    from mutracer_amd import render, scene, serve
+    rng = np.random.Generator(np.random.PCG64(5))
+    tris = np.exp(rng.uniform(math.log(1e4), math.log(4e6), 8))
+    scenes = []
+    for k, t in enumerate(tris):
+        f = max(2, int(round(math.sqrt(t / 1280))))
+        scenes.append(render.RenderScene(scene.instanced_field(n_instances=64, freq=f, env_size=(256, 128), seed=k)))
+    rng = np.random.Generator(np.random.PCG64(6))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    jobs, samples = [], 0
+    for i in range(n_jobs):
+        w, h = SERVE_RES[rng.integers(0, len(SERVE_RES))]
+        spp = int(round(math.exp(rng.uniform(math.log(4), math.log(64)))))
+        jobs.append(dict(render_scene=scenes[rng.integers(0, 8)], width=w, height=h, samples=spp, bounces=8,
+                         direct_light_candidates=4))
+        samples += w * h * spp
+    out = {}
+    for mode, kw in (("served", dict(max_jobs_per_round=0, max_samples_per_job_round=16)),
+                     ("plain", dict(max_jobs_per_round=1, max_samples_per_job_round=1 << 16))):
+        server = serve.Server(**kw)
+        server.submit(**dict(jobs[0], samples=1))  # warm the kernels and buffers
+        while not server.step()[1]:
+            pass
+        lat, makespan, _ = serve.run_stream(server, jobs, arrivals)
+        out[mode] = dict(msamples_per_s=round(samples / makespan / 1e6, 3), makespan_s=round(makespan, 3),
+                         p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+                         rounds=server.stats()["rounds"])
+        log(f"serving {mode}: {out[mode]}")
+    out["speedup"] = round(out["served"]["msamples_per_s"] / out["plain"]["msamples_per_s"], 3)
+    out["stream"] = (f"{n_jobs} jobs, Poisson {rate}/s (seed 6), 8 scenes {int(tris.min())}-{int(tris.max())} tris, "
+                     f"res {SERVE_RES}, 4-64 spp, depth 8, 4 candidates, {samples} samples")
+    return out
+
+
 def cpu_baseline(sc, gpu_sums, seconds):
     """Oracle on 32x32 tiles spread over the frame until `seconds` elapse."""
     import oracle
@@ -130,6 +174,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
     ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
@@ -217,6 +262,11 @@ def main():
         log("timing the CPU baseline (oracle)")
         cpu, parity = cpu_baseline(sc, sums, args.cpu_seconds)
 
+    serving = None
+    if not args.no_serving:
+        log("serving stream: batched vs one job at a time")
+        serving = serving_benchmark()
+
     traffic = traffic_from_profile()
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -237,6 +287,7 @@ def main():
                      "algorithmic_bytes_per_step": int(alg_bytes), "work_per_step": work,
                      "kernel_class_ms_per_step": {k: round(v, 3) for k, v in class_ms.items()}},
         "cpu_baseline": cpu,
+        "serving": serving,
         "parity_relmse_cpu_tiles": parity,
     }
     print(json.dumps(out), flush=True)
