@@ -99,10 +99,11 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
 
 
-SERVE_RES = [(128, 128), (256, 192), (480, 270), (960, 540)]  # the stream's {512^2, 1024x768, 1080p, 4K} / 4 per axis
+SERVE_RES_FULL = [(512, 512), (1024, 768), (1920, 1080), (3840, 2160)]
+SERVE_RES = [(w // 4, h // 4) for w, h in SERVE_RES_FULL]  # default: 1/4 per axis keeps the run short
 
 
-def serving_benchmark(n_jobs=64, rate=40.0):
+def serving_benchmark(n_jobs=64, rate=40.0, full=False):
     """Scaled serving stream: 8 resident instanced-field scenes with 10k-4M
     triangles (log-uniform, seed 5), Poisson arrivals at `rate` jobs/s (seed 6),
     resolutions SERVE_RES, 4-64 spp log-uniform, depth 8, 4 NEE candidates.
@@ -119,15 +120,22 @@ def serving_benchmark(n_jobs=64, rate=40.0):
     rng = np.random.Generator(np.random.PCG64(6))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
     jobs, samples = [], 0
+    res = SERVE_RES_FULL if full else SERVE_RES
+    spp_max = 256 if full else 64
     for i in range(n_jobs):
-        w, h = SERVE_RES[rng.integers(0, len(SERVE_RES))]
-        spp = int(round(math.exp(rng.uniform(math.log(4), math.log(64)))))
+        w, h = res[rng.integers(0, len(res))]
+        spp = int(round(math.exp(rng.uniform(math.log(4), math.log(spp_max)))))
         jobs.append(dict(render_scene=scenes[rng.integers(0, 8)], width=w, height=h, samples=spp, bounces=8,
                          direct_light_candidates=4))
         samples += w * h * spp
     out = {}
+    # plain: one job at a time, one sample of it per launch sequence (the
+    # reference's execution model: mt_job_poll dispatches one job's samples,
+    # one sample per dispatch); plain_whole_job: one job at a time with all of
+    # its samples in one wavefront generation (a stronger baseline).
     for mode, kw in (("served", dict(max_jobs_per_round=0, max_samples_per_job_round=16)),
-                     ("plain", dict(max_jobs_per_round=1, max_samples_per_job_round=1 << 16))):
+                     ("plain", dict(max_jobs_per_round=1, max_samples_per_job_round=1)),
+                     ("plain_whole_job", dict(max_jobs_per_round=1, max_samples_per_job_round=1 << 16))):
         server = serve.Server(**kw)
         server.submit(**dict(jobs[0], samples=1))  # warm the kernels and buffers
         while not server.step()[1]:
@@ -138,8 +146,9 @@ def serving_benchmark(n_jobs=64, rate=40.0):
                          rounds=server.stats()["rounds"])
         log(f"serving {mode}: {out[mode]}")
     out["speedup"] = round(out["served"]["msamples_per_s"] / out["plain"]["msamples_per_s"], 3)
+    out["speedup_vs_whole_job"] = round(out["served"]["msamples_per_s"] / out["plain_whole_job"]["msamples_per_s"], 3)
     out["stream"] = (f"{n_jobs} jobs, Poisson {rate}/s (seed 6), 8 scenes {int(tris.min())}-{int(tris.max())} tris, "
-                     f"res {SERVE_RES}, 4-64 spp, depth 8, 4 candidates, {samples} samples")
+                     f"res {res}, 4-{spp_max} spp, depth 8, 4 candidates, {samples} samples")
     return out
 
 
@@ -175,6 +184,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-full", action="store_true", help="full-size resolutions and 4-256 spp")
+    ap.add_argument("--serving-jobs", type=int, default=64)
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
     ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
@@ -265,7 +276,7 @@ def main():
     serving = None
     if not args.no_serving:
         log("serving stream: batched vs one job at a time")
-        serving = serving_benchmark()
+        serving = serving_benchmark(n_jobs=args.serving_jobs, full=args.serving_full)
 
     traffic = traffic_from_profile()
     out = {

@@ -128,6 +128,7 @@ struct WfConfig {
   int shadeOversub = 1;
   bool walker = false;           // persistent lane-refill traversal (else one ray per thread)
   int ldsStack = 16;             // LDS stack entries per lane for the walker (16, 24 or 32)
+  int shadeWaves = 1;            // occupancy target of the shade kernel (1..4 waves per SIMD)
 };
 
 // Optional per-kernel-class timing with HIP events on the launch stream:

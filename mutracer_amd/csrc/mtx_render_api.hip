@@ -561,6 +561,8 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   MTX_REQUIRE(t->trace_oversubscription >= 1 && t->trace_oversubscription <= 16 && t->shade_oversubscription >= 1 &&
                   t->shade_oversubscription <= 16,
               MTX_INVALID_ARGUMENT, "oversubscription must be in 1..16");
+  MTX_REQUIRE(t->shade_waves_per_simd >= 1 && t->shade_waves_per_simd <= 4, MTX_INVALID_ARGUMENT,
+              "shade_waves_per_simd must be in 1..4");
   auto T = get_target(target);
   std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
   if (!lk.owns_lock()) return MTX_BUSY;
@@ -569,6 +571,7 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   T->wfConfig.ldsStack = (int)t->lds_stack;
   T->wfConfig.traceOversub = (int)t->trace_oversubscription;
   T->wfConfig.shadeOversub = (int)t->shade_oversubscription;
+  T->wfConfig.shadeWaves = (int)t->shade_waves_per_simd;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }
@@ -583,6 +586,7 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->lds_stack = (uint32_t)T->wfConfig.ldsStack;
   t->trace_oversubscription = (uint32_t)T->wfConfig.traceOversub;
   t->shade_oversubscription = (uint32_t)T->wfConfig.shadeOversub;
+  t->shade_waves_per_simd = (uint32_t)T->wfConfig.shadeWaves;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

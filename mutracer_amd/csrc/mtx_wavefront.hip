@@ -148,8 +148,10 @@ k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill
   }
 }
 
-template <bool kMulti>
-__global__ void __launch_bounds__(kWfShadeBlock) k_wf_shade(const WfLaunch L, WfBuffers W, int event) {
+// kWaves: occupancy the register allocator must reach (waves per SIMD); the
+// allocator spills to reach it, so the best value is measured, not assumed.
+template <bool kMulti, int kWaves>
+__global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaunch L, WfBuffers W, int event) {
   const uint32_t n = W.counts[event];
   if (n == 0u) return;
   const uint32_t *q = queue_of(W, event);
@@ -421,12 +423,17 @@ struct Grids {
   uint32_t trace, walk, shade, cus;
 };
 static Grids grids_for(const WfConfig &cfg) {
-  static int traceBpc = 0, shadeBpc = 0, walkBpc[3] = {0, 0, 0};
+  static int traceBpc = 0, shadeBpc[5] = {0, 0, 0, 0, 0}, walkBpc[3] = {0, 0, 0};
   const int cus = cu_count();
   if (!traceBpc) {
     traceBpc = std::max(blocks_per_cu((const void *)k_wf_trace<false, false>, kWfTraceBlock, 0),
                         blocks_per_cu((const void *)k_wf_shadow<false, false>, kWfTraceBlock, 0));
-    shadeBpc = blocks_per_cu((const void *)k_wf_shade<false>, kWfShadeBlock, 0);
+  }
+  const int sw = std::min(4, std::max(1, cfg.shadeWaves));
+  if (!shadeBpc[sw]) {
+    const void *k = sw == 1 ? (const void *)k_wf_shade<false, 1> : sw == 2 ? (const void *)k_wf_shade<false, 2>
+                  : sw == 3 ? (const void *)k_wf_shade<false, 3> : (const void *)k_wf_shade<false, 4>;
+    shadeBpc[sw] = blocks_per_cu(k, kWfShadeBlock, 0);
   }
   const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
   if (!walkBpc[li]) {
@@ -439,7 +446,7 @@ static Grids grids_for(const WfConfig &cfg) {
   g.cus = (uint32_t)cus;
   g.trace = (uint32_t)(cus * traceBpc * std::max(1, cfg.traceOversub));
   g.walk = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
-  g.shade = (uint32_t)(cus * shadeBpc * std::max(1, cfg.shadeOversub));
+  g.shade = (uint32_t)(cus * shadeBpc[sw] * std::max(1, cfg.shadeOversub));
   return g;
 }
 
@@ -463,7 +470,12 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
       hipLaunchKernelGGL((k_wf_trace<kMulti, false>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
     te();
     tb(2);
-    hipLaunchKernelGGL(k_wf_shade<kMulti>, dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e);
+    switch (cfg.shadeWaves) {
+      case 2: hipLaunchKernelGGL((k_wf_shade<kMulti, 2>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+      case 3: hipLaunchKernelGGL((k_wf_shade<kMulti, 3>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+      case 4: hipLaunchKernelGGL((k_wf_shade<kMulti, 4>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+      default: hipLaunchKernelGGL((k_wf_shade<kMulti, 1>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+    }
     te();
     tb(3);
     if (cfg.walker)

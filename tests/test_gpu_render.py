@@ -169,6 +169,8 @@ def test_wavefront_equals_megakernel_large(env, orc):
     dict(trace_kernel=1, lds_stack=16),
     dict(trace_kernel=1, lds_stack=24, trace_oversubscription=2),
     dict(trace_kernel=1, lds_stack=32, shade_oversubscription=3, max_paths_in_flight=3000),
+    dict(trace_kernel=0, shade_waves_per_simd=2),
+    dict(trace_kernel=0, shade_waves_per_simd=4, shade_oversubscription=2),
 ])
 def test_wavefront_tunings_match_oracle(env, orc, tuning):
     """Every launch shape of the wavefront kernels yields the oracle's sums."""
