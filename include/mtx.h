@@ -308,6 +308,9 @@ typedef struct mtx_wavefront_tuning {
   uint32_t trace_oversubscription; /* trace grid = CUs x resident blocks x this (>= 1) */
   uint32_t shade_oversubscription; /* shade grid likewise (>= 1) */
   uint32_t shade_waves_per_simd;   /* register budget of the shade kernel: 1..4 resident waves per SIMD */
+  uint32_t use_graphs;             /* 1: capture each generation's launches into a hipGraph, replay while unchanged */
+  uint32_t graph_replays;          /* get only: generations launched from a graph */
+  uint32_t graph_captures;         /* get only: graphs captured */
 } mtx_wavefront_tuning;
 int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
 int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);

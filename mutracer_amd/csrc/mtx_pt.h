@@ -129,6 +129,7 @@ struct WfConfig {
   bool walker = false;           // persistent lane-refill traversal (else one ray per thread)
   int ldsStack = 16;             // LDS stack entries per lane for the walker (16, 24 or 32)
   int shadeWaves = 1;            // occupancy target of the shade kernel (1..4 waves per SIMD)
+  bool graphs = true;            // capture each generation's launch sequence into a hipGraph and replay it
 };
 
 // Optional per-kernel-class timing with HIP events on the launch stream:
@@ -145,10 +146,22 @@ struct LaunchTimer {
   ~LaunchTimer();
 };
 
+// A captured generation (hipGraph) and the launch arguments it was captured
+// with; replayed while they are unchanged.
+struct WfGraph {
+  hipGraphExec_t exec = nullptr;
+  uint64_t key = 0;
+  hipStream_t own = nullptr;  // capture/replay stream when the caller passes the null stream
+  uint64_t replays = 0, captures = 0;
+  void reset();
+  ~WfGraph();
+};
+
 void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulators &acc, hipStream_t s,
                    LaunchTimer *timer = nullptr);
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const Accumulators &acc,
-                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer = nullptr);
+                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer = nullptr,
+                             WfGraph *graph = nullptr);
 // One generation of several jobs (slots on the device, totals given): paths
 // = sum of pathCount, pixels = sum of pixelCount, events = the largest job's
 // event limit.

@@ -47,6 +47,7 @@ struct TargetObj {
   uint32_t kernel = MTX_KERNEL_WAVEFRONT;
   WfBuffers::Owner wf;
   WfConfig wfConfig;
+  WfGraph graph;
   hipStream_t stream = nullptr;
   std::mutex busy;
 };
@@ -360,7 +361,8 @@ void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work
   if (T.kernel == MTX_KERNEL_MEGAKERNEL)
     launch_render(T.scene->view, p, acc, s, timer);
   else
-    launch_render_wavefront(T.scene->view, p, acc, T.wf, T.wfConfig, s, timer);
+    launch_render_wavefront(T.scene->view, p, acc, T.wf, T.wfConfig, s, timer, work ? nullptr : &T.graph);
+  if (!s && T.graph.own && T.kernel != MTX_KERNEL_MEGAKERNEL) s = T.graph.own;
   T.stream = s;
   T.samplesDone += samples;
 }
@@ -572,6 +574,8 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   T->wfConfig.traceOversub = (int)t->trace_oversubscription;
   T->wfConfig.shadeOversub = (int)t->shade_oversubscription;
   T->wfConfig.shadeWaves = (int)t->shade_waves_per_simd;
+  T->wfConfig.graphs = t->use_graphs != 0;
+  T->graph.reset();
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }
@@ -587,6 +591,9 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->trace_oversubscription = (uint32_t)T->wfConfig.traceOversub;
   t->shade_oversubscription = (uint32_t)T->wfConfig.shadeOversub;
   t->shade_waves_per_simd = (uint32_t)T->wfConfig.shadeWaves;
+  t->use_graphs = T->wfConfig.graphs ? 1u : 0u;
+  t->graph_replays = (uint32_t)T->graph.replays;
+  t->graph_captures = (uint32_t)T->graph.captures;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

@@ -493,8 +493,25 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
   MTX_HIP(hipGetLastError());
 }
 
+void WfGraph::reset() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  exec = nullptr;
+  key = 0;
+}
+WfGraph::~WfGraph() {
+  reset();
+  if (own) (void)hipStreamDestroy(own);
+}
+
+static uint64_t fnv(uint64_t h, const void *data, size_t n) {
+  const unsigned char *b = (const unsigned char *)data;
+  for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
 void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, const Accumulators &acc,
-                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer) {
+                             WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer,
+                             WfGraph *graph) {
   if (!p0.pixelCount || !p0.samples) return;
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
@@ -507,7 +524,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
     const uint32_t nPaths = gen * p0.pixelCount;
     buf.ensure(nPaths);
     const WfBuffers W = buf.view(nPaths);
-    WfLaunch L{};
+    WfLaunch L;
+    std::memset((void *)&L, 0, sizeof L);  // padding too: the bytes key the graph cache
     L.one.scene = scene;
     L.one.params = p0;
     L.one.params.samples = gen;
@@ -519,7 +537,45 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
     L.totalPixels = p0.pixelCount;
     L.rays = acc.rays;
     L.work = acc.work;
-    run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, s, timer);
+    if (graph && cfg.graphs && !timer) {
+      // The sequence has no host-side dependence on device values, so one
+      // capture replays for as long as its arguments are unchanged.
+      hipStream_t gs = s;
+      if (!gs) {
+        if (!graph->own) MTX_HIP(hipStreamCreate(&graph->own));
+        gs = graph->own;  // blocking stream: ordered with the legacy null stream
+      }
+      uint64_t key = 1469598103934665603ull;
+      static_assert(sizeof(GpuScene) == 96 && sizeof(RenderParams) % 4 == 0, "key layouts");
+      key = fnv(key, &L.one.scene, sizeof L.one.scene);
+      key = fnv(key, &L.one.params, sizeof L.one.params);
+      const void *ptrs[6] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts};
+      key = fnv(key, ptrs, sizeof ptrs);
+      key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
+      key = fnv(key, &W.nPaths, sizeof W.nPaths);
+      const int64_t cf[8] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
+                             cfg.shadeWaves, cfg.graphs, 0};
+      key = fnv(key, cf, sizeof cf);
+      key = fnv(key, &spill, sizeof spill);
+      key = fnv(key, &eventLimit, sizeof eventLimit);
+      key = fnv(key, &gs, sizeof gs);
+      if (!graph->exec || graph->key != key) {
+        graph->reset();
+        hipGraph_t gr = nullptr;
+        MTX_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+        run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, gs, nullptr);
+        MTX_HIP(hipStreamEndCapture(gs, &gr));
+        hipError_t e = hipGraphInstantiate(&graph->exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        MTX_HIP(e);
+        graph->key = key;
+        graph->captures++;
+      }
+      MTX_HIP(hipGraphLaunch(graph->exec, gs));
+      graph->replays++;
+    } else {
+      run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, s, timer);
+    }
     done += gen;
   }
 }

@@ -162,18 +162,21 @@ class RenderTarget:
         abi.check(abi.lib().mtx_target_get_tuning(self.handle.value, C.byref(t)))
         return dict(max_paths_in_flight=t.max_paths_in_flight, trace_kernel=t.trace_kernel, lds_stack=t.lds_stack,
                     trace_oversubscription=t.trace_oversubscription, shade_oversubscription=t.shade_oversubscription,
-                    shade_waves_per_simd=t.shade_waves_per_simd)
+                    shade_waves_per_simd=t.shade_waves_per_simd, use_graphs=t.use_graphs,
+                    graph_replays=t.graph_replays, graph_captures=t.graph_captures)
 
     def set_tuning(self, **kw):
         """Changes the wavefront launch shape (keys as returned by tuning())."""
         cur = self.tuning()
+        cur.pop("graph_replays")
+        cur.pop("graph_captures")
         unknown = set(kw) - set(cur)
         if unknown:
             raise KeyError(f"unknown tuning keys {sorted(unknown)}")
         cur.update(kw)
         t = WavefrontTuning(C.sizeof(WavefrontTuning), *[int(cur[k]) for k in (
             "max_paths_in_flight", "trace_kernel", "lds_stack", "trace_oversubscription", "shade_oversubscription",
-            "shade_waves_per_simd")])
+            "shade_waves_per_simd", "use_graphs")])
         abi.check(abi.lib().mtx_target_set_tuning(self.handle.value, C.byref(t)))
 
     def synchronize(self):

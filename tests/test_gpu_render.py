@@ -186,3 +186,47 @@ def test_wavefront_tunings_match_oracle(env, orc, tuning):
     ref = orc.OracleRenderer(sc, 80, 45, bounces=10, candidates=3, max_samples=6).render(0, 6)
     assert np.all(counts == 6)
     assert np.array_equal(sums, ref)
+
+
+def test_graph_replays_match_eager(env, orc):
+    """Captured generations replayed between unrelated eager work give the
+    eager launches' bits (and the oracle's)."""
+    torch, render, scene = env
+    sc = scene.cornell_box()
+    rs = render.RenderScene(sc)
+    kw = dict(bounces=6, max_samples=8, max_paths_in_flight=48 * 40 * 2)
+    g = render.RenderTarget(rs, 48, 40, **kw)
+    e = render.RenderTarget(rs, 48, 40, **kw)
+    e.set_tuning(use_graphs=0)
+    for k in range(4):
+        g.render(2)
+        junk = torch.randn(1 << 16, device="cuda").cumsum(0)
+        e.render(2)
+        torch.cuda.synchronize()
+        assert float(junk[-1]) == float(junk[-1])
+        sg, cg = g.read_accumulators()
+        se, ce = e.read_accumulators()
+        assert np.array_equal(cg, ce) and np.array_equal(sg, se), f"replay {k} differs"
+    t = g.tuning()
+    assert t["graph_captures"] == 1 and t["graph_replays"] == 4
+    assert e.tuning()["graph_replays"] == 0
+    ref = orc.OracleRenderer(sc, 48, 40, bounces=6, max_samples=8).render(0, 8)
+    assert np.array_equal(g.read_accumulators()[0], ref)
+
+
+def test_autotune(env, orc, tmp_path):
+    torch, render, scene = env
+    from mutracer_amd import autotune
+    sc = scene.instanced_field(n_instances=6, freq=8, env_size=(64, 32))
+    rs = render.RenderScene(sc)
+    t = render.RenderTarget(rs, 96, 64, bounces=6, direct_light_candidates=2, max_samples=4)
+    best = autotune.tune(t, samples=1, cache_path=str(tmp_path / "tune.json"))
+    for k, v in best.items():
+        if k in autotune.SPACE:
+            assert v in autotune.SPACE[k] or k == "lds_stack"
+    again = autotune.tune(t, samples=1, cache_path=str(tmp_path / "tune.json"))
+    assert again == best
+    t.restart_samples()
+    t.render(4)
+    ref = orc.OracleRenderer(sc, 96, 64, bounces=6, candidates=2, max_samples=4).render(0, 4)
+    assert np.array_equal(t.read_accumulators()[0], ref)
