@@ -168,8 +168,9 @@ class RenderTarget:
     def set_tuning(self, **kw):
         """Changes the wavefront launch shape (keys as returned by tuning())."""
         cur = self.tuning()
-        cur.pop("graph_replays")
-        cur.pop("graph_captures")
+        for k in ("graph_replays", "graph_captures"):  # read-only counters
+            cur.pop(k)
+            kw.pop(k, None)
         unknown = set(kw) - set(cur)
         if unknown:
             raise KeyError(f"unknown tuning keys {sorted(unknown)}")
