@@ -376,6 +376,33 @@ int32_t mtx_server_job_release(mtx_handle server, mtx_handle job);
 int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]);
 int32_t mtx_server_destroy(mtx_handle server);
 
+/* ------------------------------------------------------------- denoiser */
+/* OIDN RT U-Net ("rt_hdr_alb_nrm", the TZA weights the reference ships) on
+ * MFMA with FP32 accumulation. Replaces mt_denoiser_create/run/destroy
+ * (mutracer-denoise.h:41-53) and the browser path's tileExecute call
+ * (WGPURenderingPipeline.ts:1445-1476). */
+#define MTX_DENOISE_FP16 1u
+#define MTX_DENOISE_BF16 2u
+/* weights: the bytes of a version-2 tensor archive (.tza). */
+int32_t mtx_denoiser_create(const void *weights, size_t size, uint32_t precision, mtx_handle *out_denoiser,
+                            char *error, size_t error_capacity);
+/* mt_denoiser_run: color, normal, albedo, output are RGBA f32 (host), count =
+ * width*height*4; HDR color through the 'pu' transfer; output alpha = color
+ * alpha; output may alias an input; nothing is written on failure. */
+int32_t mtx_denoiser_run(mtx_handle denoiser, uint32_t width, uint32_t height, const float *color,
+                         const float *normal, const float *albedo, float *output, size_t count, char *error,
+                         size_t error_capacity);
+/* The network alone: input H x W x 9 f32 (host, H and W multiples of 16) ->
+ * output H x W x 3 f32; layer_outputs (16 pointers or NULL) receive each
+ * convolution's output (H_l x W_l x C f32, NULL entries skipped). No
+ * reference counterpart: it exposes the per-layer precision. */
+int32_t mtx_denoiser_forward(mtx_handle denoiser, uint32_t width, uint32_t height, const float *input,
+                             float *output, float *const *layer_outputs, char *error, size_t error_capacity);
+/* Average GPU time of `iterations` network passes at width x height (HIP events). */
+int32_t mtx_denoiser_benchmark(mtx_handle denoiser, uint32_t width, uint32_t height, uint32_t iterations,
+                               double *ms_per_forward);
+int32_t mtx_denoiser_destroy(mtx_handle denoiser);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on
  * the device: kind 0 sin(x), 1 cos(x), 2 pow(x, y), 3 atan2(x, y) with x the

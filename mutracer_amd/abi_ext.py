@@ -79,6 +79,11 @@ def declare(L):
         "mtx_server_job_release": (i32, [u64, u64]),
         "mtx_server_stats": (i32, [u64, vp]),
         "mtx_server_destroy": (i32, [u64]),
+        "mtx_denoiser_create": (i32, [vp, sz, u32, P(u64), C.c_char_p, sz]),
+        "mtx_denoiser_run": (i32, [u64, u32, u32, vp, vp, vp, vp, sz, C.c_char_p, sz]),
+        "mtx_denoiser_forward": (i32, [u64, u32, u32, vp, vp, vp, C.c_char_p, sz]),
+        "mtx_denoiser_benchmark": (i32, [u64, u32, u32, u32, P(C.c_double)]),
+        "mtx_denoiser_destroy": (i32, [u64]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

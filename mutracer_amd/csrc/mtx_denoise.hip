@@ -1,0 +1,588 @@
+// OIDN RT U-Net denoiser on MFMA (gfx950), FP16 or BF16 storage with FP32
+// accumulation.
+//
+// Network (OIDN 2.3 "rt_hdr_alb_nrm", weights from the TZA the reference
+// ships, oidn/weights/rt_hdr_alb_nrm.tza; call site WGPURenderingPipeline.ts
+// :1445-1476): 16 3x3 convolutions, 4 max pools, 4 nearest upsamples with skip
+// concatenation (upsampled tensor first). Tensors are NHWC with channels padded
+// to a multiple of 16 (padding channels hold zeros), so a 16-channel slice of
+// one pixel is one 32-byte run.
+//
+// Convolution = implicit GEMM on v_mfma_f32_16x16x16 (f16 / bf16): one wave
+// computes 16 consecutive output pixels of a row (the M = 16 rows of the MFMA
+// tile) for all output channels (NT tiles of N = 16), stepping K over 9 taps x
+// 16-channel input slices. The A operand (16 pixels x 16 channels) is one
+// 8-byte load per lane straight from the NHWC tensor; the B operand (16
+// channels x 16 outputs) comes from weights pre-packed per (tap, slice,
+// output, channel). The decoder's upsample + concat is folded into the A load:
+// slices below the first source's channel count read the half-resolution
+// tensor at (y/2, x/2), the rest read the skip tensor. Bias, ReLU and the
+// conversion to the storage type are fused into the epilogue.
+#include <chrono>
+#include <mutex>
+
+#include "../../include/mtx_fmath.h"
+#include "mtx_registry.h"
+
+namespace mtx {
+
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------- conversions
+MTX_HD uint16_t f32_to_bf16(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+MTX_HD float bf16_to_f32(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+MTX_HD uint16_t f32_to_f16(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+MTX_HD float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+template <bool kBf16>
+MTX_HD uint16_t to_store(float f) { return kBf16 ? f32_to_bf16(f) : f32_to_f16(f); }
+template <bool kBf16>
+MTX_HD float from_store(uint16_t s) { return kBf16 ? bf16_to_f32(s) : f16_to_f32(s); }
+
+struct ConvArgs {
+  const uint16_t *src1;  // channels [0, c1): stride cs1; half resolution when up1
+  const uint16_t *src2;  // channels [c1, c1 + c2): stride cs2, full resolution
+  const uint16_t *wt;    // [9][kchunks][nt * 16][16]
+  const float *bias;
+  uint16_t *dst;         // [H][W][ds]
+  int c1, cs1, up1, c2, cs2;
+  int H, W, kchunks, kc1, cout, ds, relu;
+};
+
+template <bool kBf16, int NT>
+__global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int segs = (a.W + 15) >> 4;  // the last segment of a row may be partial
+  if (wave >= segs * a.H) return;   // wave-uniform
+  const int y = wave / segs, x0 = (wave - y * segs) << 4;
+  const int row = lane & 15, quad = lane >> 4;
+  float4v acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; j++) acc[j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+  const int W2 = a.W >> 1;
+  for (int t = 0; t < 9; t++) {
+    const int yy = y + t / 3 - 1, xx = x0 + row + t % 3 - 1;
+    const bool inside = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    for (int kc = 0; kc < a.kchunks; kc++) {
+      uint2 av = make_uint2(0u, 0u);
+      if (inside) {
+        const uint16_t *p;
+        if (kc < a.kc1) {
+          p = a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1
+                    : a.src1 + ((size_t)yy * a.W + xx) * a.cs1;
+          p += kc * 16 + quad * 4;
+        } else {
+          p = a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16 + quad * 4;
+        }
+        av = *reinterpret_cast<const uint2 *>(p);
+      }
+      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (NT * 16) + row) * 16 + quad * 4;
+#pragma unroll
+      for (int j = 0; j < NT; j++) {
+        const uint2 bv = *reinterpret_cast<const uint2 *>(wb + j * 256);
+        if (kBf16) {
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, av),
+                                                            __builtin_bit_cast(short4v, bv), acc[j], 0, 0, 0);
+        } else {
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, av),
+                                                         __builtin_bit_cast(half4v, bv), acc[j], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D layout: lane holds output channel (lane & 15) of pixels 4 * (lane >> 4) + e.
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    const int n = j * 16 + row;
+    if (n >= a.ds) continue;
+    const float b = n < a.cout ? a.bias[n] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int px = x0 + quad * 4 + e;
+      if (px >= a.W) continue;
+      float v = 0.0f;
+      if (n < a.cout) {
+        v = acc[j][e] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+      }
+      a.dst[((size_t)y * a.W + px) * a.ds + n] = to_store<kBf16>(v);
+    }
+  }
+}
+
+// 2x2 max pool over all (padded) channels; 8 channels (16 bytes) per thread.
+template <bool kBf16>
+__global__ void k_pool2(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst, int H, int W, int cs) {
+  const int Ho = H >> 1, Wo = W >> 1, groups = cs >> 3;
+  const size_t n = (size_t)Ho * Wo * groups;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int g = (int)(i % groups);
+    const size_t pix = i / groups;
+    const int x = (int)(pix % Wo), y = (int)(pix / Wo);
+    uint16_t out[8];
+    const uint16_t *s00 = src + ((size_t)(2 * y) * W + 2 * x) * cs + g * 8;
+    const uint16_t *s01 = s00 + cs, *s10 = s00 + (size_t)W * cs, *s11 = s10 + cs;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      float v = fmaxf(fmaxf(from_store<kBf16>(s00[k]), from_store<kBf16>(s01[k])),
+                      fmaxf(from_store<kBf16>(s10[k]), from_store<kBf16>(s11[k])));
+      out[k] = to_store<kBf16>(v);
+    }
+    *reinterpret_cast<uint4 *>(dst + pix * cs + g * 8) = *reinterpret_cast<const uint4 *>(out);
+  }
+}
+
+// ------------------------------------------------------ input / output transfer
+// HDR color goes through the perceptually uniform "pu" curve of OIDN
+// (hdrTransfer: 'pu', oidnNetwork.ts:107), normalized so that 65504 maps to 1.
+struct Pu {
+  static constexpr double A = 1.41283765e+03, B = 1.64593172e+00, C = 4.31384981e-01, D = -2.94139609e-03;
+  static constexpr double E = 1.92653254e-01, F = 6.26026094e-03, G = 9.98620152e-01;
+  static constexpr double Y0 = 1.57945760e-06, Y1 = 3.22087631e-02, X0 = 2.23151711e-03, X1 = 3.70974749e-01;
+};
+MTX_HD double pu_forward(double y) {
+  if (y <= Pu::Y0) return Pu::A * y;
+  if (y <= Pu::Y1) return Pu::B * mtxfm::exp_d(Pu::C * mtxfm::log_d(y)) + Pu::D;
+  return Pu::E * mtxfm::log_d(y + Pu::F) + Pu::G;
+}
+MTX_HD double pu_inverse(double x) {
+  if (x <= Pu::X0) return x / Pu::A;
+  if (x <= Pu::X1) return mtxfm::exp_d(mtxfm::log_d((x - Pu::D) / Pu::B) / Pu::C);
+  return mtxfm::exp_d((x - Pu::G) / Pu::E) - Pu::F;
+}
+
+// Input tensor (H x W padded, 16 channels): pu(color * scale) * norm, albedo
+// clamped to [0,1], normal clamped to [-1,1]; zero outside the image.
+template <bool kBf16>
+__global__ void k_denoise_in(const float4 *__restrict__ color, const float4 *__restrict__ albedo,
+                             const float4 *__restrict__ normal, int w, int h, int H, int W, float scale, float norm,
+                             uint16_t *__restrict__ dst) {
+  const size_t n = (size_t)H * W;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % W), y = (int)(i / W);
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = 0.0f;
+    if (x < w && y < h) {
+      const size_t s = (size_t)y * w + x;
+      const float4 c = color[s], al = albedo[s], nr = normal[s];
+      const float cc[3] = {c.x, c.y, c.z}, aa[3] = {al.x, al.y, al.z}, nn[3] = {nr.x, nr.y, nr.z};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double cv = fmax((double)cc[k] * scale, 0.0);
+        v[k] = (float)(pu_forward(cv) * norm);
+        v[3 + k] = fminf(fmaxf(aa[k], 0.0f), 1.0f);
+        v[6 + k] = fminf(fmaxf(nn[k], -1.0f), 1.0f);
+      }
+    }
+    uint16_t o[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) o[k] = to_store<kBf16>(v[k]);
+    uint4 *d = reinterpret_cast<uint4 *>(dst + i * 16);
+    d[0] = *reinterpret_cast<const uint4 *>(o);
+    d[1] = *reinterpret_cast<const uint4 *>(o + 8);
+  }
+}
+
+template <bool kBf16>
+__global__ void k_denoise_out(const uint16_t *__restrict__ net, const float4 *__restrict__ color, int w, int h, int W,
+                              float scale, float norm, float4 *__restrict__ out) {
+  const size_t n = (size_t)w * h;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % w), y = (int)(i / w);
+    const uint16_t *p = net + ((size_t)y * W + x) * 16;
+    float r[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double v = fmax((double)from_store<kBf16>(p[k]), 0.0) / norm;
+      r[k] = (float)(pu_inverse(v) / scale);
+    }
+    out[i] = make_float4(r[0], r[1], r[2], color[i].w);
+  }
+}
+
+// ------------------------------------------------------------------ host
+struct TzaTensor {
+  std::vector<uint32_t> shape;
+  std::vector<float> data;
+};
+
+static std::unordered_map<std::string, TzaTensor> parse_tza(const uint8_t *d, size_t n) {
+  auto need = [&](size_t off, size_t len) { MTX_REQUIRE(off + len <= n, MTX_INVALID_ARGUMENT, "truncated weights"); };
+  need(0, 12);
+  uint16_t magic;
+  std::memcpy(&magic, d, 2);
+  MTX_REQUIRE(magic == 0x41D7 && d[2] == 2, MTX_INVALID_ARGUMENT, "not a version-2 tensor archive");
+  uint64_t table;
+  std::memcpy(&table, d + 4, 8);
+  need(table, 4);
+  uint32_t count;
+  std::memcpy(&count, d + table, 4);
+  size_t p = table + 4;
+  std::unordered_map<std::string, TzaTensor> out;
+  for (uint32_t i = 0; i < count; i++) {
+    need(p, 2);
+    uint16_t nl;
+    std::memcpy(&nl, d + p, 2);
+    p += 2;
+    need(p, nl + 1);
+    std::string name((const char *)d + p, nl);
+    p += nl;
+    const uint8_t rank = d[p++];
+    need(p, 4 * rank + rank + 1 + 8);
+    TzaTensor t;
+    size_t elems = 1;
+    for (int k = 0; k < rank; k++) {
+      uint32_t s;
+      std::memcpy(&s, d + p, 4);
+      p += 4;
+      t.shape.push_back(s);
+      elems *= s;
+    }
+    p += rank;  // layout chars
+    const char dtype = (char)d[p++];
+    uint64_t off;
+    std::memcpy(&off, d + p, 8);
+    p += 8;
+    MTX_REQUIRE(dtype == 'h' || dtype == 'f', MTX_INVALID_ARGUMENT, "unsupported tensor type");
+    need(off, elems * (dtype == 'h' ? 2 : 4));
+    t.data.resize(elems);
+    for (size_t k = 0; k < elems; k++) {
+      if (dtype == 'h') {
+        uint16_t hb;
+        std::memcpy(&hb, d + off + 2 * k, 2);
+        t.data[k] = f16_to_f32(hb);
+      } else {
+        std::memcpy(&t.data[k], d + off + 4 * k, 4);
+      }
+    }
+    out[name] = std::move(t);
+  }
+  return out;
+}
+
+struct UNetLayer {
+  const char *name;
+  int level, c1, c2, cout, up1, relu;
+};
+// Inputs of each convolution: c1 channels from the previous tensor (upsampled
+// for the decoder's first conv of a level) and c2 skip channels.
+static const UNetLayer kLayers[16] = {
+    {"enc_conv0", 0, 9, 0, 32, 0, 1},     {"enc_conv1", 0, 32, 0, 32, 0, 1},   {"enc_conv2", 1, 32, 0, 48, 0, 1},
+    {"enc_conv3", 2, 48, 0, 64, 0, 1},    {"enc_conv4", 3, 64, 0, 80, 0, 1},   {"enc_conv5a", 4, 80, 0, 96, 0, 1},
+    {"enc_conv5b", 4, 96, 0, 96, 0, 1},   {"dec_conv4a", 3, 96, 64, 112, 1, 1}, {"dec_conv4b", 3, 112, 0, 112, 0, 1},
+    {"dec_conv3a", 2, 112, 48, 96, 1, 1}, {"dec_conv3b", 2, 96, 0, 96, 0, 1},  {"dec_conv2a", 1, 96, 32, 64, 1, 1},
+    {"dec_conv2b", 1, 64, 0, 64, 0, 1},   {"dec_conv1a", 0, 64, 9, 64, 1, 1},  {"dec_conv1b", 0, 64, 0, 32, 0, 1},
+    {"dec_conv0", 0, 32, 0, 3, 0, 0}};
+
+static int pad16(int c) { return (c + 15) & ~15; }
+
+struct DenoiserObj {
+  bool bf16 = false;
+  struct Layer {
+    DevBuf<uint16_t> wt;
+    DevBuf<float> bias;
+    int kchunks = 0, kc1 = 0, nt = 0;
+  } L[16];
+  // activations (sized for H x W)
+  int H = 0, W = 0;
+  DevBuf<uint16_t> in, act[16], pool[4];
+  DevBuf<float4> io[4];  // color, albedo, normal, output staging
+  size_t ioPixels = 0;
+  hipStream_t stream = nullptr;
+  std::mutex busy;
+  ~DenoiserObj() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static std::mutex gDenoiseMutex;
+static std::unordered_map<uint64_t, std::shared_ptr<DenoiserObj>> gDenoisers;
+
+static std::shared_ptr<DenoiserObj> get_denoiser(uint64_t h) {
+  std::lock_guard<std::mutex> g(gDenoiseMutex);
+  auto it = gDenoisers.find(h);
+  if (it == gDenoisers.end()) throw Error(MTX_INVALID_HANDLE, "unknown denoiser handle");
+  return it->second;
+}
+
+static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTensor &b) {
+  const UNetLayer &Ly = kLayers[li];
+  const int cin = Ly.c1 + Ly.c2;
+  MTX_REQUIRE(w.shape.size() == 4 && (int)w.shape[0] == Ly.cout && (int)w.shape[1] == cin && w.shape[2] == 3 &&
+                  w.shape[3] == 3 && (int)b.data.size() == Ly.cout,
+              MTX_INVALID_ARGUMENT, std::string("unexpected shape of ") + Ly.name);
+  auto &L = D.L[li];
+  L.kc1 = pad16(Ly.c1) / 16;
+  L.kchunks = L.kc1 + pad16(Ly.c2) / 16;
+  L.nt = pad16(Ly.cout) / 16;
+  const int N = L.nt * 16;
+  std::vector<uint16_t> packed((size_t)9 * L.kchunks * N * 16, 0);
+  for (int t = 0; t < 9; t++)
+    for (int kc = 0; kc < L.kchunks; kc++)
+      for (int n = 0; n < Ly.cout; n++)
+        for (int k = 0; k < 16; k++) {
+          int c;
+          if (kc < L.kc1) {
+            c = kc * 16 + k;
+            if (c >= Ly.c1) continue;
+          } else {
+            const int j = (kc - L.kc1) * 16 + k;
+            if (j >= Ly.c2) continue;
+            c = Ly.c1 + j;
+          }
+          const float v = w.data[(((size_t)n * cin + c) * 3 + t / 3) * 3 + t % 3];
+          packed[(((size_t)t * L.kchunks + kc) * N + n) * 16 + k] = D.bf16 ? f32_to_bf16(v) : f32_to_f16(v);
+        }
+  L.wt.upload(packed.data(), packed.size(), 0);
+  L.bias.upload(b.data.data(), b.data.size(), 0);
+}
+
+static void ensure_size(DenoiserObj &D, int H, int W) {
+  if (D.H == H && D.W == W) return;
+  D.in.alloc((size_t)H * W * 16);
+  for (int i = 0; i < 16; i++) {
+    const int lv = kLayers[i].level;
+    D.act[i].alloc((size_t)(H >> lv) * (W >> lv) * pad16(kLayers[i].cout));
+  }
+  const int poolC[4] = {32, 48, 64, 80};
+  for (int i = 0; i < 4; i++) D.pool[i].alloc((size_t)(H >> (i + 1)) * (W >> (i + 1)) * poolC[i]);
+  D.H = H;
+  D.W = W;
+}
+
+template <bool kBf16, int NT>
+static void launch_conv_nt(const ConvArgs &a, hipStream_t s) {
+  const int waves = ((a.W + 15) >> 4) * a.H;
+  hipLaunchKernelGGL((k_conv3x3<kBf16, NT>), dim3((waves + 3) / 4), dim3(256), 0, s, a);
+}
+template <bool kBf16>
+static void launch_conv(const ConvArgs &a, int nt, hipStream_t s) {
+  switch (nt) {
+    case 1: launch_conv_nt<kBf16, 1>(a, s); break;
+    case 2: launch_conv_nt<kBf16, 2>(a, s); break;
+    case 3: launch_conv_nt<kBf16, 3>(a, s); break;
+    case 4: launch_conv_nt<kBf16, 4>(a, s); break;
+    case 5: launch_conv_nt<kBf16, 5>(a, s); break;
+    case 6: launch_conv_nt<kBf16, 6>(a, s); break;
+    default: launch_conv_nt<kBf16, 7>(a, s); break;
+  }
+}
+
+// Network on D.in (H x W x 16) -> D.act[15]; layer i's output in D.act[i].
+template <bool kBf16>
+static void run_unet(DenoiserObj &D, hipStream_t s) {
+  const int H = D.H, W = D.W;
+  // inputs per layer: (src1, up1) and src2
+  const uint16_t *prev = D.in.ptr;
+  int prevC = 16;
+  for (int i = 0; i < 16; i++) {
+    const UNetLayer &Ly = kLayers[i];
+    const int lv = Ly.level;
+    ConvArgs a{};
+    a.H = H >> lv;
+    a.W = W >> lv;
+    a.src1 = prev;
+    a.cs1 = prevC;
+    a.c1 = Ly.c1;
+    a.up1 = Ly.up1;
+    a.c2 = Ly.c2;
+    if (Ly.c2) {
+      // skip source: pool3 (level 3), pool2 (2), pool1 (1), network input (0)
+      a.src2 = lv == 0 ? D.in.ptr : D.pool[lv - 1].ptr;
+      a.cs2 = lv == 0 ? 16 : pad16(Ly.c2);
+    }
+    a.wt = D.L[i].wt.ptr;
+    a.bias = D.L[i].bias.ptr;
+    a.kchunks = D.L[i].kchunks;
+    a.kc1 = D.L[i].kc1;
+    a.cout = Ly.cout;
+    a.ds = pad16(Ly.cout);
+    a.dst = D.act[i].ptr;
+    a.relu = Ly.relu;
+    launch_conv<kBf16>(a, D.L[i].nt, s);
+    prev = D.act[i].ptr;
+    prevC = a.ds;
+    if (i >= 1 && i <= 4) {  // enc_conv1..4 are pooled; the pooled tensor feeds the next level
+      const int ph = H >> lv, pw = W >> lv;
+      const size_t n = (size_t)(ph >> 1) * (pw >> 1) * (a.ds >> 3);
+      hipLaunchKernelGGL(k_pool2<kBf16>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0, s,
+                         D.act[i].ptr, D.pool[i - 1].ptr, ph, pw, a.ds);
+      prev = D.pool[i - 1].ptr;
+    }
+  }
+  MTX_HIP(hipGetLastError());
+}
+
+static void download_layer(DenoiserObj &D, int i, float *out, hipStream_t s) {
+  const int lv = kLayers[i].level, cs = pad16(kLayers[i].cout), co = kLayers[i].cout;
+  const size_t pix = (size_t)(D.H >> lv) * (D.W >> lv);
+  std::vector<uint16_t> h = D.act[i].to_host(pix * cs, s);
+  for (size_t p = 0; p < pix; p++)
+    for (int c = 0; c < co; c++) out[p * co + c] = D.bf16 ? bf16_to_f32(h[p * cs + c]) : f16_to_f32(h[p * cs + c]);
+}
+
+}  // namespace mtx
+
+using namespace mtx;
+
+extern "C" {
+
+int32_t mtx_denoiser_create(const void *weights, size_t size, uint32_t precision, mtx_handle *out, char *err,
+                            size_t cap) {
+  try {
+    MTX_REQUIRE(weights && size && out, MTX_INVALID_ARGUMENT, "weights/out is NULL");
+    MTX_REQUIRE(precision == MTX_DENOISE_FP16 || precision == MTX_DENOISE_BF16, MTX_INVALID_ARGUMENT,
+                "precision must be MTX_DENOISE_FP16 or MTX_DENOISE_BF16");
+    auto D = std::make_shared<DenoiserObj>();
+    D->bf16 = precision == MTX_DENOISE_BF16;
+    auto t = parse_tza((const uint8_t *)weights, size);
+    for (int i = 0; i < 16; i++) {
+      const std::string n = kLayers[i].name;
+      MTX_REQUIRE(t.count(n + ".weight") && t.count(n + ".bias"), MTX_INVALID_ARGUMENT, "missing tensor " + n);
+      pack_layer(*D, i, t[n + ".weight"], t[n + ".bias"]);
+    }
+    MTX_HIP(hipStreamCreate(&D->stream));
+    MTX_HIP(hipStreamSynchronize(0));
+    std::lock_guard<std::mutex> g(gDenoiseMutex);
+    uint64_t h = registry().next();
+    gDenoisers[h] = D;
+    *out = h;
+    return MTX_OK;
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(err, cap, e.what());
+    return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_forward(mtx_handle handle, uint32_t width, uint32_t height, const float *input, float *output,
+                             float *const *layer_outputs, char *err, size_t cap) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(input && output, MTX_INVALID_ARGUMENT, "input/output is NULL");
+    MTX_REQUIRE(width && height && width % 16 == 0 && height % 16 == 0 && (uint64_t)width * height <= (1u << 24),
+                MTX_INVALID_ARGUMENT, "width and height must be nonzero multiples of 16, at most 16M pixels");
+    ensure_size(*D, (int)height, (int)width);
+    const size_t pix = (size_t)width * height;
+    std::vector<uint16_t> in(pix * 16, 0);
+    for (size_t p = 0; p < pix; p++)
+      for (int c = 0; c < 9; c++) in[p * 16 + c] = D->bf16 ? f32_to_bf16(input[p * 9 + c]) : f32_to_f16(input[p * 9 + c]);
+    D->in.upload(in.data(), in.size(), D->stream);
+    if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+    MTX_HIP(hipStreamSynchronize(D->stream));
+    std::vector<float> o(pix * 3);
+    download_layer(*D, 15, o.data(), D->stream);
+    std::memcpy(output, o.data(), o.size() * sizeof(float));
+    if (layer_outputs)
+      for (int i = 0; i < 16; i++)
+        if (layer_outputs[i]) download_layer(*D, i, layer_outputs[i], D->stream);
+    return MTX_OK;
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(err, cap, e.what());
+    return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_benchmark(mtx_handle handle, uint32_t width, uint32_t height, uint32_t iterations,
+                               double *ms_per_forward) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(ms_per_forward && iterations, MTX_INVALID_ARGUMENT, "bad arguments");
+    MTX_REQUIRE(width && height && width % 16 == 0 && height % 16 == 0, MTX_INVALID_ARGUMENT,
+                "width and height must be multiples of 16");
+    ensure_size(*D, (int)height, (int)width);
+    MTX_HIP(hipMemsetAsync(D->in.ptr, 0, D->in.bytes(), D->stream));
+    if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+    hipEvent_t e0, e1;
+    MTX_HIP(hipEventCreate(&e0));
+    MTX_HIP(hipEventCreate(&e1));
+    MTX_HIP(hipEventRecord(e0, D->stream));
+    for (uint32_t k = 0; k < iterations; k++) {
+      if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+    }
+    MTX_HIP(hipEventRecord(e1, D->stream));
+    MTX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MTX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_forward = ms / iterations;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_run(mtx_handle handle, uint32_t width, uint32_t height, const float *color, const float *normal,
+                         const float *albedo, float *output, size_t count, char *err, size_t cap) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(color && normal && albedo && output, MTX_INVALID_ARGUMENT, "NULL image");
+    MTX_REQUIRE(width && height && (uint64_t)width * height <= 16777216ull, MTX_INVALID_ARGUMENT,
+                "dimensions must be nonzero, at most 16777216 pixels");
+    MTX_REQUIRE(count == (size_t)width * height * 4, MTX_INVALID_ARGUMENT, "count must be width*height*4");
+    const int H = pad16((int)height), W = pad16((int)width);
+    ensure_size(*D, H, W);
+    const size_t n = (size_t)width * height;
+    if (D->ioPixels < n) {
+      for (auto &b : D->io) b.alloc(n);
+      D->ioPixels = n;
+    }
+    hipStream_t s = D->stream;
+    D->io[0].upload(reinterpret_cast<const float4 *>(color), n, s);
+    D->io[1].upload(reinterpret_cast<const float4 *>(albedo), n, s);
+    D->io[2].upload(reinterpret_cast<const float4 *>(normal), n, s);
+    const float scale = 1.0f;
+    const float norm = (float)(1.0 / pu_forward(65504.0));
+    const unsigned grid = (unsigned)std::min<size_t>(((size_t)H * W + 255) / 256, 65536);
+    if (D->bf16) {
+      hipLaunchKernelGGL(k_denoise_in<true>, dim3(grid), dim3(256), 0, s, D->io[0].ptr, D->io[1].ptr, D->io[2].ptr,
+                         (int)width, (int)height, H, W, scale, norm, D->in.ptr);
+      run_unet<true>(*D, s);
+      hipLaunchKernelGGL(k_denoise_out<true>, dim3(grid), dim3(256), 0, s, D->act[15].ptr, D->io[0].ptr, (int)width,
+                         (int)height, W, scale, norm, D->io[3].ptr);
+    } else {
+      hipLaunchKernelGGL(k_denoise_in<false>, dim3(grid), dim3(256), 0, s, D->io[0].ptr, D->io[1].ptr, D->io[2].ptr,
+                         (int)width, (int)height, H, W, scale, norm, D->in.ptr);
+      run_unet<false>(*D, s);
+      hipLaunchKernelGGL(k_denoise_out<false>, dim3(grid), dim3(256), 0, s, D->act[15].ptr, D->io[0].ptr, (int)width,
+                         (int)height, W, scale, norm, D->io[3].ptr);
+    }
+    MTX_HIP(hipGetLastError());
+    std::vector<float4> res = D->io[3].to_host(n, s);  // no partial result on failure: copy after success
+    std::memcpy(output, res.data(), n * sizeof(float4));
+    return MTX_OK;
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(err, cap, e.what());
+    return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_destroy(mtx_handle handle) {
+  std::lock_guard<std::mutex> g(gDenoiseMutex);
+  return gDenoisers.erase(handle) ? MTX_OK : MTX_INVALID_HANDLE;
+}
+
+}  // extern "C"
