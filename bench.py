@@ -152,6 +152,82 @@ def serving_benchmark(n_jobs=64, rate=40.0, full=False):
     return out
 
 
+def microbench(verify_build=False, subset=1 << 20):
+    """Traversal/build microbench: 8M independent triangles (seed 3, lognormal
+    edges), binned SAH with 32 bins built on the GPU; 7,997,584 pinhole
+    primaries at 2828^2 plus 8M cosine-hemisphere bounces from their hits.
+    Closest-hit IDs are checked bit for bit against the oracle's
+    reference-order traversal on a random subset of each ray set (and, with
+    verify_build, the BVH2 words against the oracle's own build)."""
+    import torch
+    import oracle
+    from mutracer_amd import accel, workloads as W
+    t = time.perf_counter()
+    pos, idx = W.random_triangles(8_000_000, seed=3)
+    log(f"triangles generated in {time.perf_counter() - t:.1f} s")
+    dpos, didx = torch.from_numpy(pos).cuda(), torch.from_numpy(idx).cuda()
+    accel.Blas(dpos, didx, 32)  # warm-up build (kernels, allocator)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    blas = accel.Blas(dpos, didx, 32)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t) * 1e3
+    acc = accel.Accel([blas], [0], [W.identity_transform()], material=[0], double_sided=[1])
+    torch.cuda.synchronize()
+    out = {"triangles": 8_000_000, "bins": 32, "build_ms": round(build_ms, 2),
+           "build_mtris_per_s": round(8.0 / (build_ms / 1e3), 2)}
+
+    def trace(rays, label):
+        r = torch.from_numpy(rays).cuda()
+        h = torch.zeros((rays.shape[0], 8), dtype=torch.int32, device="cuda")
+        acc.trace_closest(r, h, double_sided=True)  # warm
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        acc.trace_closest(r, h, double_sided=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        out[label] = {"rays": rays.shape[0], "ms": round(ms, 3), "mrays_per_s": round(rays.shape[0] / ms / 1e3, 1)}
+        log(f"{label}: {out[label]}")
+        return h.cpu().numpy().view(np.uint32)
+
+    prim = W.pinhole_rays(2828)
+    hp = trace(prim, "primaries")
+    hit = hp[:, 0].astype(bool)
+    tt = hp[hit, 3].view(np.float32)
+    p = prim[hit, 0:3] + prim[hit, 4:7] * tt[:, None]
+    v = pos[idx.reshape(-1, 3)[hp[hit, 2].astype(np.int64)]]
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    n *= np.where((n * prim[hit, 4:7]).sum(1, keepdims=True) > 0, -1, 1)
+    bounce = W.hemisphere_bounces(p, n, 8_000_000, seed=0)
+    hb = trace(bounce, "bounces")
+    total_ms = out["primaries"]["ms"] + out["bounces"]["ms"]
+    out["mrays_per_s"] = round((prim.shape[0] + bounce.shape[0]) / total_ms / 1e3, 1)
+
+    # Oracle check on the GPU-built BVH (reference-order traversal).
+    words, src = blas.export()
+    tw, iw = acc.export_tlas()
+    scene = oracle.TraceScene(tw, words, iw, pos, np.ones(1, np.uint8))
+    rng = np.random.default_rng(11)
+    mism = 0
+    for rays, hits in ((prim, hp), (bounce, hb)):
+        sel = rng.choice(rays.shape[0], min(subset, rays.shape[0]), replace=False)
+        ref = scene.closest(rays[sel], double_sided=True)
+        g = hits[sel]
+        same = (g[:, 0] == ref[:, 0]) & (~ref[:, 0].astype(bool) | (
+            (src[ref[:, 2]] == g[:, 2]) & (g[:, 3] == ref[:, 3])))
+        mism += int((~same).sum())
+    out["oracle_subset_rays"] = 2 * subset
+    out["oracle_mismatches"] = mism
+    if verify_build:
+        t = time.perf_counter()
+        ref_words = oracle.build_blas(pos, idx, 32)["words"]
+        out["oracle_build_s"] = round(time.perf_counter() - t, 1)
+        out["build_words_identical"] = bool(np.array_equal(ref_words, words))
+    return out
+
+
 def cpu_baseline(sc, gpu_sums, seconds):
     """Oracle on 32x32 tiles spread over the frame until `seconds` elapse."""
     import oracle
@@ -191,12 +267,17 @@ def main():
     ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
     ap.add_argument("--profile-summary", default="", help="print a summary of a rocprofv3 results db and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, help="run one copy of N MiB and exit")
+    ap.add_argument("--microbench", action="store_true", help="8M-triangle build + 16M-ray traversal microbench")
+    ap.add_argument("--verify-build", action="store_true", help="with --microbench: compare with the oracle build")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary))
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.microbench:
+        print(json.dumps({"microbench": microbench(args.verify_build)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the path shards by "
