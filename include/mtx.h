@@ -315,9 +315,35 @@ typedef struct mtx_wavefront_tuning {
 int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
 int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);
 
+/* Adaptive sampling (two-stage converge, Render.ts:886-895): after a pilot
+ * render, plan per-pixel sample targets from the FP64 luminance moments
+ * (offlineAdaptive.compute.ts:29-106, options of OfflineAdaptiveSampling.ts
+ * :75-108): per tile of tile_size^2 pixels, the largest unbiased per-pixel
+ * variance v and the mean luminance m give
+ *   target = clamp(ceil(confidence^2 * v / (absolute_error + relative_error*|m|)^2),
+ *                  min_samples, max_samples).
+ * The plan is installed as the target's per-pixel limits: later renders skip
+ * pixels whose sample count has reached their target (rayTrace.compute.ts
+ * :1171-1180). Zero fields take the reference defaults (min 16 capped at max,
+ * tile 16, absolute 0.002, relative 0.03, confidence 1.96). */
+typedef struct mtx_adaptive_options {
+  uint32_t struct_size;
+  uint32_t max_samples, min_samples, tile_size;
+  double absolute_error, relative_error, confidence;
+} mtx_adaptive_options;
+/* plan[0] = largest target, plan[1] = average target per pixel. tile_variance
+ * (columns x rows doubles, optional) receives each tile's variance. */
+int32_t mtx_target_plan_adaptive(mtx_handle target, const mtx_adaptive_options *options, double plan[2],
+                                 double *tile_variance, size_t tile_capacity);
+/* Installs caller targets (width*height floats, bottom row first) or, with
+ * NULL, removes the limits. */
+int32_t mtx_target_set_sample_targets(mtx_handle target, const float *targets, size_t count);
+
 /* stats[0] rays traced, [1] samples per pixel completed. */
 int32_t mtx_target_stats(mtx_handle target, uint64_t stats[2]);
 int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera);
+/* setSampleSequenceOffset (WGPURayTracePass.ts:2085): first Sobol index of later samples. */
+int32_t mtx_target_set_sample_offset(mtx_handle target, uint32_t sample_sequence_offset);
 int32_t mtx_target_reset(mtx_handle target);
 int32_t mtx_target_destroy(mtx_handle target);
 

@@ -47,6 +47,12 @@ class JobInfo(C.Structure):
                 ("submit_ms", C.c_double), ("start_ms", C.c_double), ("complete_ms", C.c_double)]
 
 
+class AdaptiveOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_samples", C.c_uint32), ("min_samples", C.c_uint32),
+                ("tile_size", C.c_uint32), ("absolute_error", C.c_double), ("relative_error", C.c_double),
+                ("confidence", C.c_double)]
+
+
 def declare(L):
     u32, i32, u64, vp, sz = C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p, C.c_size_t
     P = C.POINTER
@@ -67,6 +73,9 @@ def declare(L):
         "mtx_target_reset": (i32, [u64]),
         "mtx_target_destroy": (i32, [u64]),
         "mtx_target_set_tuning": (i32, [u64, vp]),
+        "mtx_target_plan_adaptive": (i32, [u64, vp, vp, vp, sz]),
+        "mtx_target_set_sample_targets": (i32, [u64, vp, sz]),
+        "mtx_target_set_sample_offset": (i32, [u64, u32]),
         "mtx_target_get_tuning": (i32, [u64, vp]),
         "mtx_math_eval": (i32, [u32, vp, vp, vp, u64, vp]),
         "mtx_server_create": (i32, [vp, P(u64), C.c_char_p, sz]),

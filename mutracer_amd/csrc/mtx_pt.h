@@ -58,6 +58,7 @@ struct Accumulators {
   uint64_t *rays;  // traced rays (camera + continuation + shadow)
   uint32_t pixels;
   uint64_t *work;  // optional: box tests, triangle tests, instance visits, hits, shadow rays
+  const float *targets = nullptr;  // optional per-pixel sample targets (adaptive sampling)
 };
 
 struct RenderSceneObj;

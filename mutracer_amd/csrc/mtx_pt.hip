@@ -46,7 +46,10 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
 #pragma unroll
     for (int c = 0; c < 13; c++) sums[c] = acc.planes[(size_t)c * acc.pixels + pix];
     const uint32_t count0 = acc.count[pix];
+    uint32_t done = 0;
     for (uint32_t k = 0; k < P.samples; k++) {
+      if (acc.targets && (double)(count0 + k) >= (double)acc.targets[pix]) break;
+      done++;
       Smp smp = path_sampler(P, px, py, count0 + k + P.sampleOffset);
       PathRegs pr;
       init_path(P, smp, pr);
@@ -71,7 +74,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
     }
 #pragma unroll
     for (int c = 0; c < 13; c++) acc.planes[(size_t)c * acc.pixels + pix] = sums[c];
-    acc.count[pix] = count0 + P.samples;
+    acc.count[pix] = count0 + done;
   }
   // One atomic per wave for the ray counter.
   for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);

@@ -48,6 +48,8 @@ struct TargetObj {
   WfBuffers::Owner wf;
   WfConfig wfConfig;
   WfGraph graph;
+  DevBuf<float> targets;  // per-pixel sample limits (adaptive sampling)
+  bool useTargets = false;
   hipStream_t stream = nullptr;
   std::mutex busy;
 };
@@ -358,6 +360,7 @@ void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work
   RenderParams p = T.params;
   p.samples = samples;
   Accumulators acc{T.planes.ptr, T.count.ptr, T.rays.ptr, p.width * p.height, work};
+  acc.targets = T.useTargets ? T.targets.ptr : nullptr;
   if (T.kernel == MTX_KERNEL_MEGAKERNEL)
     launch_render(T.scene->view, p, acc, s, timer);
   else
@@ -365,6 +368,40 @@ void target_render(TargetObj &T, uint32_t samples, hipStream_t s, uint64_t *work
   if (!s && T.graph.own && T.kernel != MTX_KERNEL_MEGAKERNEL) s = T.graph.own;
   T.stream = s;
   T.samplesDone += samples;
+}
+
+// One thread per tile (offlineAdaptive.compute.ts:29-106), in FP64.
+__global__ void k_plan_adaptive(const double *__restrict__ planes, const uint32_t *__restrict__ count, uint32_t W,
+                                uint32_t H, uint32_t N, uint32_t tile, uint32_t cols, uint32_t rows, uint32_t minS,
+                                uint32_t maxS, double absErr, double relErr, double conf, float *__restrict__ targets,
+                                double *__restrict__ tileVar, uint32_t *__restrict__ tileTarget) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cols * rows) return;
+  const uint32_t tx = t % cols, ty = t / cols;
+  const uint32_t x0 = tx * tile, y0 = ty * tile, x1 = min(x0 + tile, W), y1 = min(y0 + tile, H);
+  double meanSum = 0.0, maxVar = 0.0;
+  uint32_t n = 0;
+  for (uint32_t y = y0; y < y1; y++)
+    for (uint32_t x = x0; x < x1; x++) {
+      const uint32_t i = y * W + x;
+      const double s = (double)count[i];
+      if (s <= 0.0) continue;
+      const double mean = (0.2126 * planes[i] + 0.7152 * planes[N + i] + 0.0722 * planes[2 * N + i]) / s;
+      const double second = planes[12 * (size_t)N + i] / s;
+      double var = fmax(second - mean * mean, 0.0);
+      if (s > 1.0) var = var * s / (s - 1.0);
+      meanSum += mean;
+      maxVar = fmax(maxVar, var);
+      n++;
+    }
+  const double mean = meanSum / (double)max(n, 1u);
+  const double tol = absErr + relErr * fabs(mean);
+  const double required = ceil(conf * conf * maxVar / fmax(tol * tol, 1e-16));
+  const double tgt = fmin((double)maxS, fmax((double)minS, required));
+  tileVar[t] = maxVar;
+  tileTarget[t] = (uint32_t)tgt;
+  for (uint32_t y = y0; y < y1; y++)
+    for (uint32_t x = x0; x < x1; x++) targets[y * W + x] = (float)tgt;
 }
 
 }  // namespace mtx
@@ -546,6 +583,16 @@ int32_t mtx_target_set_camera(mtx_handle target, const mtx_camera_desc *camera) 
   API_CATCH(nullptr, 0)
 }
 
+int32_t mtx_target_set_sample_offset(mtx_handle target, uint32_t offset) {
+  API_TRY
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  T->params.sampleOffset = offset;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
 int32_t mtx_target_reset(mtx_handle target) {
   API_TRY
   auto T = get_target(target);
@@ -594,6 +641,67 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->use_graphs = T->wfConfig.graphs ? 1u : 0u;
   t->graph_replays = (uint32_t)T->graph.replays;
   t->graph_captures = (uint32_t)T->graph.captures;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_set_sample_targets(mtx_handle target, const float *targets, size_t count) {
+  API_TRY
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  const size_t N = (size_t)T->params.width * T->params.height;
+  if (!targets) {
+    T->useTargets = false;
+    return MTX_OK;
+  }
+  MTX_REQUIRE(count == N, MTX_INVALID_ARGUMENT, "targets must hold width*height values");
+  T->targets.upload(targets, N, T->stream);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  T->useTargets = true;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_plan_adaptive(mtx_handle target, const mtx_adaptive_options *o, double plan[2],
+                                 double *tileVariance, size_t tileCap) {
+  API_TRY
+  MTX_REQUIRE(o && o->struct_size >= sizeof(mtx_adaptive_options) && plan, MTX_INVALID_ARGUMENT, "bad arguments");
+  auto T = get_target(target);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  // normalizeOfflineAdaptiveSamplingOptions (OfflineAdaptiveSampling.ts:75-108)
+  const uint32_t maxS = std::max(1u, o->max_samples);
+  const uint32_t minS = std::min(maxS, o->min_samples ? o->min_samples : std::min(maxS, 16u));
+  const uint32_t tile = o->tile_size ? o->tile_size : 16u;
+  const double absErr = std::max(o->absolute_error > 0 ? o->absolute_error : 0.002, 1e-8);
+  const double relErr = std::max(o->relative_error > 0 ? o->relative_error : 0.03, 0.0);
+  const double conf = std::max(o->confidence > 0 ? o->confidence : 1.96, 0.01);
+  const uint32_t W = T->params.width, H = T->params.height, N = W * H;
+  const uint32_t cols = (W + tile - 1) / tile, rows = (H + tile - 1) / tile;
+  if (T->targets.count < N) T->targets.alloc(N);
+  DevBuf<double> tv(cols * rows);
+  DevBuf<uint32_t> tt(cols * rows);
+  hipStream_t s = T->stream;
+  hipLaunchKernelGGL(k_plan_adaptive, dim3((cols * rows + 127) / 128), dim3(128), 0, s, T->planes.ptr, T->count.ptr, W,
+                     H, N, tile, cols, rows, minS, maxS, absErr, relErr, conf, T->targets.ptr, tv.ptr, tt.ptr);
+  MTX_HIP(hipGetLastError());
+  std::vector<uint32_t> tiles = tt.to_host(cols * rows, s);
+  std::vector<double> vars = tv.to_host(cols * rows, s);
+  double mx = 0, sum = 0;
+  for (uint32_t t = 0; t < cols * rows; t++) {
+    const uint32_t tx = t % cols, ty = t / cols;
+    const uint32_t w = std::min(tile, W - tx * tile), h = std::min(tile, H - ty * tile);
+    mx = std::max(mx, (double)tiles[t]);
+    sum += (double)tiles[t] * w * h;
+  }
+  plan[0] = mx;
+  plan[1] = sum / N;
+  if (tileVariance) {
+    if (tileCap < (size_t)cols * rows) return MTX_BUFFER_TOO_SMALL;
+    std::memcpy(tileVariance, vars.data(), vars.size() * sizeof(double));
+  }
+  T->useTargets = true;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

@@ -72,31 +72,55 @@ __device__ __forceinline__ void store_path(const WfBuffers &W, uint32_t p, const
   W.nrm[p] = make_float4(pr.auxNormal.x, pr.auxNormal.y, pr.auxNormal.z, 0.0f);
 }
 
+// Queue lengths of a generation start at zero; queue 0 holds every path
+// unless per-pixel sample targets select them.
+__global__ void k_wf_reset(WfBuffers W, uint32_t initial) {
+  for (int k = threadIdx.x; k < kWfCounters; k += blockDim.x) W.counts[k] = k == 0 ? initial : 0u;
+}
+
+// Whether sample s of path p's pixel is still below the pixel's target.
+__device__ __forceinline__ bool below_target(const JobSlot &J, uint32_t p) {
+  if (!J.acc.targets) return true;
+  const RenderParams &P = J.params;
+  const uint32_t rel = p - J.pathBase;
+  const uint32_t local = rel % P.pixelCount, s = rel / P.pixelCount;
+  const uint32_t pix = P.pixelBase + local;
+  return (double)(J.acc.count[pix] + s) < (double)J.acc.targets[pix];
+}
+
 template <bool kMulti>
-__global__ void __launch_bounds__(256) k_wf_generate(const WfLaunch L, WfBuffers W) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid == 0) {
-    W.counts[0] = W.nPaths;
-    for (int k = 1; k < kWfCounters; k++) W.counts[k] = 0u;
-  }
-  for (uint32_t p = tid; p < W.nPaths; p += stride) {
-    uint32_t j = 0;
-    if (kMulti) {  // last slot with pathBase <= p
-      uint32_t lo = 0, hi = L.nJobs;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (L.jobs[mid].pathBase <= p) lo = mid; else hi = mid;
+__global__ void __launch_bounds__(256) k_wf_generate(const WfLaunch L, WfBuffers W, int select) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nWaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t w = wave; w * 64u < W.nPaths; w += nWaves) {
+    const uint32_t p = w * 64u + lane_id();
+    bool alive = p < W.nPaths;
+    if (alive) {
+      uint32_t j = 0;
+      if (kMulti) {  // last slot with pathBase <= p
+        uint32_t lo = 0, hi = L.nJobs;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (L.jobs[mid].pathBase <= p) lo = mid; else hi = mid;
+        }
+        j = lo;
+        W.item[p] = (uint16_t)j;
       }
-      j = lo;
-      W.item[p] = (uint16_t)j;
+      const JobSlot &J = kMulti ? L.jobs[j] : L.one;
+      if (select) alive = below_target(J, p);
+      if (alive) {
+        Smp smp = path_smp(J, p);
+        PathRegs pr;
+        init_path(J.params, smp, pr);
+        store_path(W, p, pr);
+      }
     }
-    const JobSlot &J = kMulti ? L.jobs[j] : L.one;
-    Smp smp = path_smp(J, p);
-    PathRegs pr;
-    init_path(J.params, smp, pr);
-    store_path(W, p, pr);
-    W.qA[p] = p;
+    if (select) {
+      const uint32_t slot = wave_append(W.counts, alive);
+      if (alive) W.qA[slot] = p;
+    } else if (alive) {
+      W.qA[p] = p;
+    }
   }
 }
 
@@ -347,7 +371,14 @@ __global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffe
     double sums[13];
 #pragma unroll
     for (int c = 0; c < 13; c++) sums[c] = acc.planes[(size_t)c * acc.pixels + pix];
-    for (uint32_t s = 0; s < P.samples; s++) {
+    const uint32_t count0 = acc.count[pix];
+    uint32_t done = P.samples;
+    if (acc.targets) {
+      const double target = (double)acc.targets[pix];
+      done = 0;
+      while (done < P.samples && (double)(count0 + done) < target) done++;
+    }
+    for (uint32_t s = 0; s < done; s++) {
       const uint32_t p = J.pathBase + s * P.pixelCount + local;
       const float4 l = W.li[p], d = W.d[p], al = W.alb[p], nr = W.nrm[p];
       PathRegs pr;
@@ -360,7 +391,7 @@ __global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffe
     }
 #pragma unroll
     for (int c = 0; c < 13; c++) acc.planes[(size_t)c * acc.pixels + pix] = sums[c];
-    acc.count[pix] += P.samples;
+    acc.count[pix] = count0 + done;
   }
 }
 
@@ -457,8 +488,10 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
   auto tb = [&](int c) { if (timer) timer->begin(c); };
   auto te = [&]() { if (timer) timer->end(); };
   const uint32_t genGrid = std::min<uint32_t>((W.nPaths + 255) / 256, g.cus * 16u);
+  const bool select = !kMulti && L.one.acc.targets != nullptr;  // server jobs have no targets
   tb(0);
-  hipLaunchKernelGGL(k_wf_generate<kMulti>, dim3(genGrid), dim3(256), 0, s, L, W);
+  hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, s, W, select ? 0u : W.nPaths);
+  hipLaunchKernelGGL(k_wf_generate<kMulti>, dim3(genGrid), dim3(256), 0, s, L, W, select ? 1 : 0);
   te();
   for (int e = 0; e <= eventLimit; e++) {
     tb(1);
@@ -549,7 +582,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       static_assert(sizeof(GpuScene) == 96 && sizeof(RenderParams) % 4 == 0, "key layouts");
       key = fnv(key, &L.one.scene, sizeof L.one.scene);
       key = fnv(key, &L.one.params, sizeof L.one.params);
-      const void *ptrs[6] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts};
+      const void *ptrs[7] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts,
+                             L.one.acc.targets};
       key = fnv(key, ptrs, sizeof ptrs);
       key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
       key = fnv(key, &W.nPaths, sizeof W.nPaths);

@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi_ext import CameraDesc, MeshDesc, RenderSettings, SceneDesc, WavefrontTuning
+from .abi_ext import AdaptiveOptions, CameraDesc, MeshDesc, RenderSettings, SceneDesc, WavefrontTuning
 from .sampling import sampling_pattern_code
 
 _P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
@@ -187,6 +187,32 @@ class RenderTarget:
         d = camera_desc(camera, self.width, self.height)
         abi.check(abi.lib().mtx_target_set_camera(self.handle.value, C.byref(d)))
 
+    def set_sample_offset(self, offset):
+        """setSampleSequenceOffset: Sobol index of the next sample of each pixel = count + offset."""
+        abi.check(abi.lib().mtx_target_set_sample_offset(self.handle.value, offset))
+        self.settings.sample_sequence_offset = offset
+
+    def set_sample_targets(self, targets=None):
+        """Per-pixel sample limits ((H, W) floats, bottom row first) or None."""
+        if targets is None:
+            abi.check(abi.lib().mtx_target_set_sample_targets(self.handle.value, None, 0))
+            return
+        t = np.ascontiguousarray(targets, np.float32)
+        abi.check(abi.lib().mtx_target_set_sample_targets(self.handle.value, _P(t), t.size))
+
+    def plan_adaptive(self, max_samples, min_samples=0, tile_size=0, absolute_error=0.0, relative_error=0.0,
+                      confidence=0.0):
+        """Plans and installs per-pixel sample targets from the accumulated
+        (pilot) moments; returns (largest target, average target, tile variances)."""
+        o = AdaptiveOptions(C.sizeof(AdaptiveOptions), max_samples, min_samples, tile_size, absolute_error,
+                            relative_error, confidence)
+        ts = tile_size or 16
+        cols, rows = -(-self.width // ts), -(-self.height // ts)
+        plan = np.zeros(2, np.float64)
+        tv = np.zeros((rows, cols), np.float64)
+        abi.check(abi.lib().mtx_target_plan_adaptive(self.handle.value, C.byref(o), _P(plan), _P(tv), tv.size))
+        return int(plan[0]), float(plan[1]), tv
+
     def restart_samples(self):
         abi.check(abi.lib().mtx_target_reset(self.handle.value))
 
@@ -224,12 +250,36 @@ class Render:
         self.target = target
         self.samples_per_batch = samples_per_batch
 
-    def converge(self, samples):
-        self.target.restart_samples()
-        left = samples
+    def _stage(self, cap):
+        t = self.target
+        t.restart_samples()
+        left = cap
         while left > 0:
             n = min(self.samples_per_batch, left)
-            self.target.render(n)
+            t.render(n)
             left -= n
-        self.target.synchronize()
-        return self.target.read_pixels("color")
+        t.synchronize()
+
+    def converge(self, samples, adaptive=None):
+        """Offline convergence. With `adaptive` (dict of plan_adaptive options
+        plus pilot_samples, default 8) it runs the reference's two stages: a
+        pilot pass, a per-pixel plan from its FP64 moments, then a final pass
+        continuing the Sobol sequence after the pilot samples. Returns the
+        resolved color (top row first) and a result dict."""
+        t = self.target
+        base = t.settings.sample_sequence_offset
+        result = dict(samples=samples, average_samples_per_pixel=float(samples), pilot_samples=0, adaptive=False)
+        if adaptive:
+            opts = dict(adaptive)
+            pilot = min(samples, max(2, round(opts.pop("pilot_samples", 8))))
+            t.set_sample_targets(None)
+            self._stage(pilot)
+            mx, avg, _ = t.plan_adaptive(samples, **opts)
+            t.set_sample_offset(base + pilot)
+            self._stage(mx)
+            t.set_sample_offset(base)
+            result = dict(samples=mx, average_samples_per_pixel=avg, pilot_samples=pilot, adaptive=True)
+        else:
+            t.set_sample_targets(None)
+            self._stage(samples)
+        return t.read_pixels("color"), result

@@ -230,3 +230,44 @@ def test_autotune(env, orc, tmp_path):
     t.render(4)
     ref = orc.OracleRenderer(sc, 96, 64, bounces=6, candidates=2, max_samples=4).render(0, 4)
     assert np.array_equal(t.read_accumulators()[0], ref)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_adaptive_converge(env, orc, kernel):
+    """Pilot moments -> per-tile plan (the reference formula, recomputed here
+    in numpy) -> final pass whose pixels stop at their targets and continue
+    the Sobol sequence after the pilot samples (checked against the oracle)."""
+    torch, render, scene = env
+    sc = scene.cornell_box()
+    rs = render.RenderScene(sc)
+    W, H, S, pilot = 40, 32, 24, 4
+    t = render.RenderTarget(rs, W, H, bounces=5, max_samples=S, kernel=kernel)
+    t.render(pilot)
+    sums, counts = t.read_accumulators()
+    mx, avg, tv = t.plan_adaptive(S, tile_size=8, absolute_error=0.05, relative_error=0.1)
+    # reference formula (offlineAdaptive.compute.ts:29-106), per 8x8 tile
+    n = counts.astype(np.float64)
+    mean = (0.2126 * sums[..., 0] + 0.7152 * sums[..., 1] + 0.0722 * sums[..., 2]) / n
+    var = np.maximum(sums[..., 12] / n - mean * mean, 0) * n / (n - 1)
+    want = np.zeros((H, W))
+    for ty in range(H // 8):
+        for tx in range(W // 8):
+            sl = (slice(ty * 8, ty * 8 + 8), slice(tx * 8, tx * 8 + 8))
+            tol = 0.05 + 0.1 * abs(mean[sl].mean())
+            req = np.ceil(1.96 ** 2 * var[sl].max() / max(tol * tol, 1e-16))
+            want[sl] = min(S, max(16, req))
+            assert abs(tv[ty, tx] - var[sl].max()) <= 1e-12 * max(1.0, var[sl].max())
+    assert mx == int(want.max()) and abs(avg - want.mean()) < 1e-9
+    t.restart_samples()
+    t.set_sample_offset(pilot)
+    t.render(mx)
+    sums2, counts2 = t.read_accumulators()
+    assert np.array_equal(counts2, want.astype(np.uint32))
+    for v in np.unique(want):
+        ref = orc.OracleRenderer(sc, W, H, bounces=5, max_samples=S, sample_offset=pilot).render(0, int(v))
+        sel = want == v
+        assert np.array_equal(sums2[sel], ref[sel])
+    img, info = render.Render(t).converge(S, adaptive=dict(pilot_samples=pilot, tile_size=8, absolute_error=0.05,
+                                                            relative_error=0.1))
+    assert info["adaptive"] and info["samples"] == mx
+    assert np.array_equal(t.read_accumulators()[0], sums2)
