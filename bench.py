@@ -228,6 +228,58 @@ def microbench(verify_build=False, subset=1 << 20):
     return out
 
 
+def denoiser_benchmark(full_precision=False, iterations=5):
+    """U-Net at 1920x1080 (padded to 1088) in FP16 and BF16: GPU time per
+    pass (HIP events), TFLOP/s against the dense MFMA peak, and the error
+    against the FP64 oracle per layer on inputs rendered from the Cornell
+    box at 4 spp (a 256x256 centre crop by default, the full frame with
+    full_precision)."""
+    from oracle import unet
+    from mutracer_amd import denoise, render, scene
+    sc = scene.cornell_box()
+    rs = render.RenderScene(sc)
+    t = render.RenderTarget(rs, 1920, 1080, bounces=8, max_samples=4)
+    t.render(4)
+    color, albedo, normal = (t.read_pixels(k) for k in ("color", "albedo", "normal"))
+    norm = 1.0 / _pu_forward(65504.0)
+    x = np.zeros((1088, 1920, 9), np.float32)
+    x[:1080, :, 0:3] = _pu_forward(np.maximum(color[..., :3].astype(np.float64), 0)) * norm
+    x[:1080, :, 3:6] = np.clip(albedo[..., :3], 0, 1)
+    x[:1080, :, 6:9] = np.clip(normal[..., :3], -1, 1)
+    gflop = unet.flops(1088, 1920) / 1e9
+    weights = unet.read_tza(denoise.WEIGHTS)
+    crop = x if full_precision else x[416:672, 832:1088]
+    ref_out, ref = unet.forward(weights, crop.astype(np.float64))
+    out = {"input": "Cornell box 1920x1080 at 4 spp (color through pu, albedo, normal), padded to 1920x1088",
+           "gflop_per_pass": round(gflop, 2),
+           "precision_region": "full frame" if full_precision else "256x256 centre crop"}
+    for prec in ("fp16", "bf16"):
+        d = denoise.Denoiser(prec)
+        ms = d.benchmark(1920, 1088, iterations)
+        got, layers = d.forward(crop, layers=True)
+        errs = {name: round(float(np.abs(layers[name] - ref[name]).max() / max(np.abs(ref[name]).max(), 1e-9)), 6)
+                for name, _, _ in denoise.LAYERS}
+        diff = np.abs(got - ref_out)
+        out[prec] = {"ms_per_pass": round(ms, 3), "tflops": round(gflop / ms, 2),
+                     "mfma_frac": round(gflop / ms / 2500.0, 4),
+                     "layer_max_rel_error_vs_fp64": errs,
+                     "output_max_abs_error": float(diff.max()), "output_mean_abs_error": float(diff.mean()),
+                     "output_p99_abs_error": float(np.percentile(diff, 99))}
+        log(f"denoiser {prec}: {ms:.2f} ms, {gflop / ms:.1f} TFLOP/s, output max err {diff.max():.2e}")
+    return out
+
+
+def _pu_forward(y):
+    """OIDN's pu transfer (as the library's input stage)."""
+    y = np.asarray(y, np.float64)
+    A, B, C, D, E, F, G = 1.41283765e+03, 1.64593172e+00, 4.31384981e-01, -2.94139609e-03, 1.92653254e-01, \
+        6.26026094e-03, 9.98620152e-01
+    Y0, Y1 = 1.57945760e-06, 3.22087631e-02
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(y <= Y0, A * y, np.where(y <= Y1, B * np.power(np.maximum(y, 1e-30), C) + D,
+                                                 E * np.log(y + F) + G))
+
+
 def cpu_baseline(sc, gpu_sums, seconds):
     """Oracle on 32x32 tiles spread over the frame until `seconds` elapse."""
     import oracle
@@ -260,6 +312,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-denoiser", action="store_true")
+    ap.add_argument("--denoise-full-precision", action="store_true", help="FP64 oracle on the full 1080p frame")
     ap.add_argument("--serving-full", action="store_true", help="full-size resolutions and 4-256 spp")
     ap.add_argument("--serving-jobs", type=int, default=64)
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
@@ -354,6 +408,11 @@ def main():
         log("timing the CPU baseline (oracle)")
         cpu, parity = cpu_baseline(sc, sums, args.cpu_seconds)
 
+    denoiser = None
+    if not args.no_denoiser:
+        log("denoiser: U-Net at 1080p, FP16 and BF16")
+        denoiser = denoiser_benchmark(args.denoise_full_precision)
+
     serving = None
     if not args.no_serving:
         log("serving stream: batched vs one job at a time")
@@ -380,6 +439,7 @@ def main():
                      "kernel_class_ms_per_step": {k: round(v, 3) for k, v in class_ms.items()}},
         "cpu_baseline": cpu,
         "serving": serving,
+        "denoiser": denoiser,
         "parity_relmse_cpu_tiles": parity,
     }
     print(json.dumps(out), flush=True)

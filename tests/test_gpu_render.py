@@ -267,6 +267,7 @@ def test_adaptive_converge(env, orc, kernel):
         ref = orc.OracleRenderer(sc, W, H, bounces=5, max_samples=S, sample_offset=pilot).render(0, int(v))
         sel = want == v
         assert np.array_equal(sums2[sel], ref[sel])
+    t.set_sample_offset(0)
     img, info = render.Render(t).converge(S, adaptive=dict(pilot_samples=pilot, tile_size=8, absolute_error=0.05,
                                                             relative_error=0.1))
     assert info["adaptive"] and info["samples"] == mx
