@@ -247,7 +247,8 @@ def denoiser_benchmark(full_precision=False, iterations=5):
     x = np.zeros((1088, 1920, 9), np.float32)
     x[:1080, :, 0:3] = _pu_forward(np.maximum(color[..., :3].astype(np.float64), 0)) * norm
     x[:1080, :, 3:6] = np.clip(albedo[..., :3], 0, 1)
-    x[:1080, :, 6:9] = np.clip(normal[..., :3], -1, 1)
+    # readPixels('normal') is display-encoded (n * 0.5 + 0.5, A = 1 where valid)
+    x[:1080, :, 6:9] = np.clip(np.where(normal[..., 3:4] > 0, normal[..., :3] * 2.0 - 1.0, 0.0), -1, 1)
     gflop = unet.flops(1088, 1920) / 1e9
     weights = unet.read_tza(denoise.WEIGHTS)
     crop = x if full_precision else x[416:672, 832:1088]
@@ -366,11 +367,13 @@ def main():
     rays = tgt.stats()["rays"]
     log(f"instrumented launch: {work}, rays {rays}")
     samples = WIDTH * HEIGHT * SPP
-    closest_rays = rays - work["shadow_rays"]
     if args.kernel == "wavefront":
-        # Dominant kernel: the closest-hit trace (ray in + hit out + node/triangle/instance fetches).
-        alg_bytes = (RAY_B * closest_rays + BOX_B * work["box_tests"] + TRI_B * work["tri_tests"]
-                     + INST_B * work["instance_visits"])
+        # Dominant kernel: the closest-hit trace k_wf_trace (ray in + hit out +
+        # node/triangle/instance fetches), counted over its own launches only
+        # (the tail kernel's traversal work is the "tail" group).
+        wf = work["wavefront"]
+        alg_bytes = (RAY_B * wf["closest_rays"] + BOX_B * wf["box_tests"] + TRI_B * wf["tri_tests"]
+                     + INST_B * wf["instance_visits"])
     else:
         alg_bytes = (BOX_B * (work["box_tests"] + work["shadow_box_tests"])
                      + TRI_B * (work["tri_tests"] + work["shadow_tri_tests"])

@@ -265,30 +265,38 @@ int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings,
  * asynchronous. One megakernel launch per call. */
 int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream);
 
-/* mtx_target_render plus per-launch totals of the traversal work:
- * work[0] closest-hit box tests, [1] closest-hit triangle tests, [2]
- * closest-hit instance visits, [3] closest hits, [4] shadow rays, [5] shadow
- * box tests, [6] shadow triangle tests, [7] shadow instance visits.
- * Synchronous. No reference counterpart: it feeds the algorithmic-bytes
- * figure of the roofline. */
-int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[8], void *stream);
+/* mtx_target_render plus per-launch totals of the traversal work, in two
+ * groups of MTX_WORK_GROUP counters: work[0..8] for the wavefront trace and
+ * shadow kernels (or the megakernel), work[9..17] for the wavefront tail
+ * kernel. Within a group: [0] closest-hit rays, [1] closest-hit box tests,
+ * [2] closest-hit triangle tests, [3] closest-hit instance visits, [4]
+ * closest hits, [5] shadow rays, [6] shadow box tests, [7] shadow triangle
+ * tests, [8] shadow instance visits. Synchronous. No reference counterpart:
+ * it feeds the algorithmic-bytes figure of the roofline. */
+#define MTX_WORK_GROUP 9
+#define MTX_WORK_COUNTERS 18
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[MTX_WORK_COUNTERS],
+                                  void *stream);
 
 /* mtx_target_render with HIP events around every kernel launch on `stream`;
  * returns the summed GPU time per kernel class: ms[0] generate + accumulate,
- * [1] closest-hit trace, [2] shade, [3] shadow trace, [4] megakernel.
- * Synchronous. No reference counterpart (measurement). */
-int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[5], void *stream);
+ * [1] closest-hit trace, [2] shade, [3] shadow trace, [4] megakernel, [5]
+ * wavefront tail kernel. Synchronous. No reference counterpart (measurement). */
+#define MTX_TIMER_CLASSES 6
+int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[MTX_TIMER_CLASSES], void *stream);
 
 /* Blocks until the target's submitted work is done. */
 int32_t mtx_target_synchronize(mtx_handle target);
 
-#define MTX_PIXELS_COLOR 0   /* HDR color RGBA: sum / samples */
-#define MTX_PIXELS_ALBEDO 1  /* albedo RGB, coverage in A */
-#define MTX_PIXELS_NORMAL 2  /* normal RGB, A = 1 */
-#define MTX_PIXELS_VARIANCE 3 /* luminance variance of the mean in R */
+#define MTX_PIXELS_COLOR 0   /* readPixels('hdr'): RGB / hit count (or / samples without hits), A / samples */
+#define MTX_PIXELS_ALBEDO 1  /* readPixels('albedo'): RGB / samples, A = clamp(hits / samples, 0, 1) */
+#define MTX_PIXELS_NORMAL 2  /* readPixels('normal'): RGB / samples * 0.5 + 0.5, A = 1 (0 where |sum| is ~0) */
+#define MTX_PIXELS_VARIANCE 3 /* luminance variance of the mean in R (no reference counterpart) */
 
-/* Resolved RGBA f32, top row first (Render.readPixels, Render.ts:782-801).
- * pixels == NULL queries the float count through *written. */
+/* Resolved RGBA f32, top row first (Render.readPixels, Render.ts:782-801),
+ * normalized as the output pass does (WGPURayTraceOutput.ts:32-92,
+ * hdr_resolve_component in compose.frag.wgsl.ts:264-272); pixels without
+ * samples are 0. pixels == NULL queries the float count through *written. */
 int32_t mtx_target_read_pixels(mtx_handle target, uint32_t kind, float *pixels, size_t capacity_floats,
                                size_t *written);
 
@@ -304,13 +312,15 @@ typedef struct mtx_wavefront_tuning {
   uint32_t struct_size;
   uint32_t max_paths_in_flight;    /* paths per generation (0 = keep) */
   uint32_t trace_kernel;           /* 0 one ray per thread, 1 persistent lane-refill walker */
-  uint32_t lds_stack;              /* walker LDS stack entries per lane: 16, 24 or 32 */
+  uint32_t lds_stack;              /* traversal LDS stack entries per lane: 16, 24 or 32 */
   uint32_t trace_oversubscription; /* trace grid = CUs x resident blocks x this (>= 1) */
   uint32_t shade_oversubscription; /* shade grid likewise (>= 1) */
   uint32_t shade_waves_per_simd;   /* register budget of the shade kernel: 1..4 resident waves per SIMD */
   uint32_t use_graphs;             /* 1: capture each generation's launches into a hipGraph, replay while unchanged */
   uint32_t graph_replays;          /* get only: generations launched from a graph */
   uint32_t graph_captures;         /* get only: graphs captured */
+  uint32_t tail_paths;             /* once an event's queue holds at most this many paths, one kernel
+                                      finishes them all (0 = never) */
 } mtx_wavefront_tuning;
 int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
 int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);

@@ -30,8 +30,13 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define MTX_FM_HD __host__ __device__ inline
+/* The five public functions stay out of line: inlined at every call site of
+ * the shading code their f64 temporaries multiply the kernels' register
+ * pressure (and spills); as calls they cost a few instructions each. */
+#define MTX_FM_CALL __host__ __device__ inline __attribute__((noinline))
 #else
 #define MTX_FM_HD static inline
+#define MTX_FM_CALL static inline
 #endif
 
 namespace mtxfm {
@@ -94,7 +99,7 @@ MTX_FM_HD double reduce_pio2(double x, int &quadrant) {
   quadrant = (int)(ki & 3);
   return r;
 }
-MTX_FM_HD float sinf(float xf) {
+MTX_FM_CALL float sinf(float xf) {
   double x = xf;
   if (is_nan(x) || is_inf(x)) return ffrom(0x7fc00000u);
   if (x == 0.0) return xf;  /* keeps the sign of zero */
@@ -103,7 +108,7 @@ MTX_FM_HD float sinf(float xf) {
   double v = (q & 1) ? cos_poly(r) : sin_poly(r);
   return (float)((q & 2) ? -v : v);
 }
-MTX_FM_HD float cosf(float xf) {
+MTX_FM_CALL float cosf(float xf) {
   double x = xf;
   if (is_nan(x) || is_inf(x)) return ffrom(0x7fc00000u);
   int q;
@@ -164,7 +169,7 @@ MTX_FM_HD double log_d(double x) {
   return (de * kLn2Hi + lm) + de * kLn2Lo;
 }
 
-MTX_FM_HD float powf(float xf, float yf) {
+MTX_FM_CALL float powf(float xf, float yf) {
   double x = xf, y = yf;
   if (y == 0.0 || x == 1.0) return 1.0f;
   if (is_nan(x) || is_nan(y)) return ffrom(0x7fc00000u);
@@ -241,10 +246,10 @@ MTX_FM_HD double atan2_d(double y, double x, bool yNeg, bool xNeg) {
   }
   return yNeg ? -r : r;
 }
-MTX_FM_HD float atan2f(float y, float x) {
+MTX_FM_CALL float atan2f(float y, float x) {
   return (float)atan2_d((double)y, (double)x, (fbits(y) >> 31) != 0, (fbits(x) >> 31) != 0);
 }
-MTX_FM_HD float acosf(float xf) {
+MTX_FM_CALL float acosf(float xf) {
   double x = xf;
   if (is_nan(x) || x > 1.0 || x < -1.0) return ffrom(0x7fc00000u);
   double s = __builtin_sqrt((1.0 - x) * (1.0 + x));

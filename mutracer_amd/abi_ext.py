@@ -33,7 +33,8 @@ class WavefrontTuning(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_paths_in_flight", C.c_uint32), ("trace_kernel", C.c_uint32),
                 ("lds_stack", C.c_uint32), ("trace_oversubscription", C.c_uint32),
                 ("shade_oversubscription", C.c_uint32), ("shade_waves_per_simd", C.c_uint32),
-                ("use_graphs", C.c_uint32), ("graph_replays", C.c_uint32), ("graph_captures", C.c_uint32)]
+                ("use_graphs", C.c_uint32), ("graph_replays", C.c_uint32), ("graph_captures", C.c_uint32),
+                ("tail_paths", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

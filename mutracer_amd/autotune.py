@@ -4,7 +4,8 @@ The reference's only tuner is the tile-size predictor of TileRender.ts
 (:302-490), which fits tile area to a time budget from GPU timestamps. Here
 the knobs are the wavefront kernels' launch shape (mtx_wavefront_tuning):
 traversal form, LDS stack depth, grid oversubscription, the shade kernel's
-occupancy target and the number of paths in flight. The tuner measures each
+occupancy target, the queue length below which the tail kernel finishes a
+generation, and the number of paths in flight. The tuner measures each
 candidate with HIP events (RenderTarget.render_timed) on a short probe
 render of the real target and keeps the fastest, searching one dimension at
 a time (coordinate descent, two passes). Every candidate renders the same
@@ -26,8 +27,9 @@ SPACE = {
     "shade_waves_per_simd": [1, 2, 4],
     "shade_oversubscription": [1, 2],
     "max_paths_in_flight": [1 << 23, 1 << 24, 1 << 25],
+    "tail_paths": [0, 16384, 65536, 262144],
 }
-ORDER = ["max_paths_in_flight", "trace_kernel", "trace_oversubscription", "shade_waves_per_simd",
+ORDER = ["max_paths_in_flight", "tail_paths", "trace_kernel", "trace_oversubscription", "shade_waves_per_simd",
          "shade_oversubscription", "lds_stack"]
 
 
@@ -50,7 +52,7 @@ def probe(target, tuning, samples):
     target.render_timed(1)
     target.restart_samples()
     t = target.render_timed(samples)
-    return t["generate_accumulate"] + t["trace"] + t["shade"] + t["shadow"]
+    return t["generate_accumulate"] + t["trace"] + t["shade"] + t["shadow"] + t["tail"]
 
 
 def tune(target, samples=2, cache_path=None, log=None):
@@ -72,8 +74,6 @@ def tune(target, samples=2, cache_path=None, log=None):
     for _ in range(2):
         improved = False
         for dim in ORDER:
-            if dim == "lds_stack" and best["trace_kernel"] == 0:
-                continue
             for v in SPACE[dim]:
                 if v == best[dim]:
                     continue

@@ -112,32 +112,37 @@ struct TraceCounters {
   uint32_t boxes, tris, instances;
 };
 
-// Per-lane traversal stack: the first kLdsStack entries live in LDS
-// (lds[i * ldsStride]), deeper entries spill to a per-lane global slice.
+// Per-lane traversal stack: the first kLds entries live in LDS
+// (lds[i * ldsStride]), deeper entries spill to a per-lane global slice of
+// kMaxStack - kLds entries.
 constexpr int kLdsStack = 32;
+constexpr int kMinLdsStack = 16;  // smallest kLds of any kernel; sizes the spill slices
 constexpr int kMaxStack = 160;
 
-struct LaneStack {
+template <int kLds>
+struct LaneStackT {
+  static_assert(kLds >= kMinLdsStack && kLds <= kMaxStack, "LDS stack depth");
   int32_t *lds;
   int32_t ldsStride;
   int32_t *spill;
   __device__ __forceinline__ void put(int i, int32_t v) {
-    if (i < kLdsStack) lds[i * ldsStride] = v; else spill[i - kLdsStack] = v;
+    if (i < kLds) lds[i * ldsStride] = v; else spill[i - kLds] = v;
   }
   __device__ __forceinline__ int32_t get(int i) const {
-    return i < kLdsStack ? lds[i * ldsStride] : spill[i - kLdsStack];
+    return i < kLds ? lds[i * ldsStride] : spill[i - kLds];
   }
 };
+using LaneStack = LaneStackT<kLdsStack>;
 
 // Two-level traversal: TLAS near-first stack, leaf instances in reverse
 // order (optionally rejected by their world box), ray to object space with
 // the instance's w2l, BLAS near-first stack with the root pushed untested,
 // then the triangle loop with the self-hit skip and back-face culling of
 // getBVHTraverseCode. anyHit returns at the first accepted triangle.
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, class Stack>
 __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, float tMax,
                                          bool doubleSided, int bounces, int32_t curTri,
-                                         int32_t curSlot, LaneStack &st, TraceHit &h,
+                                         int32_t curSlot, Stack &st, TraceHit &h,
                                          TraceCounters *cnt) {
   h.hit = false;
   h.slot = -1;
@@ -231,7 +236,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
     int32_t ref = st.get(--sp);
     o = ro;
     d = rd;
-    invD = splat(1.0f) / d;
+    invD = worldInvD;
     if (ref >= 0) {
       const TNode n = acc.tlasNodes[ref];
       float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);

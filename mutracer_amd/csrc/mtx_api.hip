@@ -111,7 +111,9 @@ int32_t *spill_for(hipStream_t s, uint32_t blocksNeeded, uint32_t blockThreads) 
   MTX_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(gSpill.m);
   auto &b = gSpill.bufs[{dev, s}];
-  size_t need = (size_t)blocksNeeded * blockThreads * (kMaxStack - kLdsStack);
+  // Each lane owns kMaxStack - kLds entries at lane * (kMaxStack - kLds):
+  // sized for the smallest LDS depth any kernel uses.
+  size_t need = (size_t)blocksNeeded * blockThreads * (kMaxStack - kMinLdsStack);
   if (!b || b->count < need) b = std::make_shared<DevBuf<int32_t>>(need);
   return b->ptr;
 }

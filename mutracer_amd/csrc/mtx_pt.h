@@ -73,10 +73,17 @@ struct TargetObj;
 //   s0..s3 shadow queue: (origin, tMax), (direction, path id), (payload, tri), slot
 //   counts[e] = length of event e's queue, counts[kWfShadowBase + e] = its shadow queue,
 //   counts[kWfFetchBase + 2e (+1)] = next entry the persistent trace (shadow) kernel takes
+//   counts[kWfXcdBase + (2e (+1)) * 8 + x] = next 64-entry chunk of XCD x's segment of
+//                                           event e's (shadow) queue
+//   counts[kWfTailFlag] = 1 + the event at which the tail kernel took over the
+//                         generation's remaining paths (0 while it has not)
 constexpr int kWfMaxEvents = 51;
 constexpr int kWfShadowBase = kWfMaxEvents + 1;
 constexpr int kWfFetchBase = kWfShadowBase + kWfMaxEvents + 1;
-constexpr int kWfCounters = kWfFetchBase + 2 * (kWfMaxEvents + 1);
+constexpr int kWfXcdBase = kWfFetchBase + 2 * (kWfMaxEvents + 1);
+constexpr int kWfXcds = 8;
+constexpr int kWfTailFlag = kWfXcdBase + 2 * (kWfMaxEvents + 1) * kWfXcds;
+constexpr int kWfCounters = kWfTailFlag + 1;
 struct WfBuffers {
   float4 *o, *d, *li, *beta, *alb, *nrm, *hitA;
   int2 *hitB;
@@ -128,15 +135,24 @@ struct WfConfig {
   int traceOversub = 1;          // trace/shadow grid = CUs * resident blocks * this
   int shadeOversub = 1;
   bool walker = false;           // persistent lane-refill traversal (else one ray per thread)
-  int ldsStack = 16;             // LDS stack entries per lane for the walker (16, 24 or 32)
-  int shadeWaves = 1;            // occupancy target of the shade kernel (1..4 waves per SIMD)
+  int ldsStack = 16;             // LDS stack entries per lane of the traversal kernels (16, 24 or 32)
+  int shadeWaves = 2;            // occupancy target of the shade kernel (1..4 waves per SIMD)
   bool graphs = true;            // capture each generation's launch sequence into a hipGraph and replay it
+  uint32_t tailPaths = 65536;    // the tail kernel finishes the paths once a queue is this short (0 = never)
+};
+
+// Traversal work totals of an instrumented launch (acc.work): one group of
+// kWorkGroup counters for the wavefront (or megakernel) kernels, one for the
+// wavefront tail kernel at kWorkTailBase.
+enum : int {
+  kWorkClosestRays = 0, kWorkBoxes, kWorkTris, kWorkInstances, kWorkHits, kWorkShadowRays, kWorkShadowBoxes,
+  kWorkShadowTris, kWorkShadowInstances, kWorkGroup, kWorkTailBase = kWorkGroup, kWorkSlots = 2 * kWorkGroup
 };
 
 // Optional per-kernel-class timing with HIP events on the launch stream:
 // class 0 generate + accumulate, 1 closest-hit trace, 2 shade, 3 shadow
-// trace, 4 megakernel.
-constexpr int kTimerClasses = 5;
+// trace, 4 megakernel, 5 wavefront tail kernel.
+constexpr int kTimerClasses = 6;
 struct LaunchTimer {
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> events;  // begin/end pairs

@@ -16,9 +16,9 @@
 
 namespace mtx {
 
-// kCount adds per-launch totals of the traversal work to acc.work (layout of
-// mtx_target_render_counted: closest box/triangle/instance tests, hits,
-// shadow rays, shadow box/triangle/instance tests).
+// kCount adds per-launch totals of the traversal work to acc.work (first
+// group of mtx_target_render_counted: closest rays, closest box/triangle/
+// instance tests, hits, shadow rays, shadow box/triangle/instance tests).
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock)
 k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__restrict__ spill) {
@@ -29,7 +29,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   const Lut L{S.lut};
   uint64_t rays = 0;
   TraceCounters tc{0u, 0u, 0u}, ts{0u, 0u, 0u};
-  uint32_t nHits = 0, nShadow = 0;
+  uint64_t nClosest = 0, nHits = 0, nShadow = 0;
   const uint32_t nBlocks = (P.pixelCount + kBlock - 1) / kBlock;
   // XCD-aware ordering: blocks resident on one XCD (blockIdx % 8) walk one
   // contiguous eighth of the pixel range so neighbouring pixels share an L2.
@@ -56,6 +56,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
       for (int event = 0; event <= eventLimit; event++) {
         TraceHit h;
         rays++;
+        if (kCount) nClosest++;
         traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, event != 0, trace_depth(event), pr.curTri,
                                 pr.curSlot, st, h, &tc);
         if (kCount) nHits += h.hit ? 1u : 0u;
@@ -80,9 +81,9 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
   for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
   if ((threadIdx.x & 63) == 0 && rays) atomicAdd((unsigned long long *)acc.rays, (unsigned long long)rays);
   if (kCount) {
-    uint64_t v[8] = {tc.boxes, tc.tris, tc.instances, nHits, nShadow, ts.boxes, ts.tris, ts.instances};
+    uint64_t v[kWorkGroup] = {nClosest, tc.boxes, tc.tris, tc.instances, nHits, nShadow, ts.boxes, ts.tris, ts.instances};
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < kWorkGroup; k++) {
       for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
       if ((threadIdx.x & 63) == 0 && v[k]) atomicAdd((unsigned long long *)acc.work + k, (unsigned long long)v[k]);
     }

@@ -314,22 +314,47 @@ static RenderParams params_from(const mtx_render_settings &s, const mtx_camera_d
   return p;
 }
 
-// Resolved RGBA f32, top row first, from 13 planes + counts (bottom row first).
+// Resolved RGBA f32, top row first, from 13 planes + counts (bottom row
+// first): the normalize pass of WGPURayTraceOutput.ts:32-92 with
+// hdr_resolve_component (compose.frag.wgsl.ts:264-272), evaluated on the FP64
+// sums and rounded once. total = max(sample count, 1); hit = albedo plane A.
+//   color:  rgb / hit when hit > 0, else / total; a / total
+//   albedo: rgb / total, a = clamp(hit / total, 0, 1)
+//   normal: rgb / total * 0.5 + 0.5 and a = 1 when |sum|^2 > 0.1, else 0
+// Pixels without samples resolve to 0 in every kind.
 static void resolve_pixels(const double *pl, const uint32_t *cnt, uint32_t W, uint32_t H, uint32_t kind,
                            float *pixels) {
   const size_t N = (size_t)W * H;
   for (uint32_t y = 0; y < H; y++)
     for (uint32_t x = 0; x < W; x++) {
       size_t i = (size_t)y * W + x, o = ((size_t)(H - 1 - y) * W + x) * 4;
-      double n = cnt[i] ? (double)cnt[i] : 1.0;
+      double r[4] = {0.0, 0.0, 0.0, 0.0};
+      const double count = pl[11 * N + i];
       if (kind == MTX_PIXELS_VARIANCE) {
+        double n = cnt[i] ? (double)cnt[i] : 1.0;
         double mean = (0.2126 * pl[i] + 0.7152 * pl[N + i] + 0.0722 * pl[2 * N + i]) / n;
-        double var = std::max(0.0, pl[12 * N + i] / n - mean * mean) / n;
-        pixels[o] = (float)var; pixels[o + 1] = pixels[o + 2] = 0.0f; pixels[o + 3] = 1.0f;
-        continue;
+        r[0] = std::max(0.0, pl[12 * N + i] / n - mean * mean) / n;
+        r[3] = 1.0;
+      } else if (count > 0.0) {
+        const double total = std::max(count, 1.0), hit = pl[7 * N + i];
+        if (kind == MTX_PIXELS_COLOR) {
+          const double div = hit > 0.0 ? hit : total;
+          for (int k = 0; k < 3; k++) r[k] = pl[(size_t)k * N + i] / div;
+          r[3] = pl[3 * N + i] / total;
+        } else if (kind == MTX_PIXELS_ALBEDO) {
+          for (int k = 0; k < 3; k++) r[k] = pl[(size_t)(4 + k) * N + i] / total;
+          r[3] = std::min(std::max(hit / total, 0.0), 1.0);
+        } else {
+          const double nx = pl[8 * N + i], ny = pl[9 * N + i], nz = pl[10 * N + i];
+          if (nx * nx + ny * ny + nz * nz > 0.1) {
+            r[0] = nx / total * 0.5 + 0.5;
+            r[1] = ny / total * 0.5 + 0.5;
+            r[2] = nz / total * 0.5 + 0.5;
+            r[3] = 1.0;
+          }
+        }
       }
-      int base = kind == MTX_PIXELS_COLOR ? 0 : (kind == MTX_PIXELS_ALBEDO ? 4 : 8);
-      for (int k = 0; k < 4; k++) pixels[o + k] = (float)(pl[(size_t)(base + k) * N + i] / n);
+      for (int k = 0; k < 4; k++) pixels[o + k] = (float)r[k];
     }
 }
 
@@ -488,23 +513,25 @@ int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream) {
   API_CATCH(nullptr, 0)
 }
 
-int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[8], void *stream) {
+int32_t mtx_target_render_counted(mtx_handle target, uint32_t samples, uint64_t work[MTX_WORK_COUNTERS],
+                                  void *stream) {
   API_TRY
   MTX_REQUIRE(work, MTX_INVALID_ARGUMENT, "work is NULL");
   auto T = get_target(target);
   std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
   if (!lk.owns_lock()) return MTX_BUSY;
   hipStream_t s = (hipStream_t)stream;
-  DevBuf<uint64_t> w(8);
+  static_assert(kWorkSlots == MTX_WORK_COUNTERS && kWorkGroup == MTX_WORK_GROUP, "work counter layout");
+  DevBuf<uint64_t> w(kWorkSlots);
   MTX_HIP(hipMemsetAsync(w.ptr, 0, w.bytes(), s));
   target_render(*T, samples, s, w.ptr);
-  w.download(work, 8, s);
+  w.download(work, kWorkSlots, s);
   MTX_HIP(hipStreamSynchronize(s));
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }
 
-int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[5], void *stream) {
+int32_t mtx_target_render_timed(mtx_handle target, uint32_t samples, double ms[MTX_TIMER_CLASSES], void *stream) {
   API_TRY
   MTX_REQUIRE(ms, MTX_INVALID_ARGUMENT, "ms is NULL");
   auto T = get_target(target);
@@ -622,6 +649,7 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   T->wfConfig.shadeOversub = (int)t->shade_oversubscription;
   T->wfConfig.shadeWaves = (int)t->shade_waves_per_simd;
   T->wfConfig.graphs = t->use_graphs != 0;
+  T->wfConfig.tailPaths = t->tail_paths;
   T->graph.reset();
   return MTX_OK;
   API_CATCH(nullptr, 0)
@@ -641,6 +669,7 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->use_graphs = T->wfConfig.graphs ? 1u : 0u;
   t->graph_replays = (uint32_t)T->graph.replays;
   t->graph_captures = (uint32_t)T->graph.captures;
+  t->tail_paths = T->wfConfig.tailPaths;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

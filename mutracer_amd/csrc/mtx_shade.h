@@ -892,8 +892,21 @@ __device__ __forceinline__ Smp path_sampler(const RenderParams &P, uint32_t px, 
   return Smp{px, py, P.width, index, P.seed, P.logSPP, P.pattern, 0u};
 }
 
-// main (rayTrace.compute.ts:1188-1205): Sobol pixel jitter and initCameraRay
-// (:591-610, browser projection-inverse form).
+// sampleCircle (rayTrace.compute.ts:1148-1154): concentric square-to-disk
+// map of a 2D sample, with the reference's operation order.
+__device__ __forceinline__ void sample_circle(float ux, float uy, float &cx, float &cy) {
+  const float px = 2.0f * ux - 1.0f, py = 2.0f * uy - 1.0f;
+  const bool greater = fabsf(px) > fabsf(py);
+  const float r = greater ? px : py;
+  const float theta = greater ? kQuarterPi * py / px : kPi * (0.5f - 0.25f * px / py);
+  cx = r * mtxfm::cosf(theta);
+  cy = r * mtxfm::sinf(theta);
+}
+
+// main (rayTrace.compute.ts:1188-1205): Sobol pixel jitter, initCameraRay
+// (:591-610, browser projection-inverse form) and, with a positive aperture,
+// the thin lens: focus plane perpendicular to the view direction at the
+// focus distance, lens point = aperture * sampleCircle(CAMERA_LENS sample).
 __device__ void init_path(const RenderParams &P, const Smp &smp, PathRegs &pr) {
   const float Wf = (float)P.width, Hf = (float)P.height;
   float uvx = ((float)smp.px + 0.5f) / Wf, uvy = ((float)smp.py + 0.5f) / Hf;
@@ -914,6 +927,17 @@ __device__ void init_path(const RenderParams &P, const Smp &smp, PathRegs &pr) {
             f3{P.viewInv[8], P.viewInv[9], P.viewInv[10]}, f3{P.viewInv[12], P.viewInv[13], P.viewInv[14]}};
   pr.ro = xform(vi, op, 1.0f);
   pr.rd = normalize(xform(vi, dir, 0.0f));
+  if (P.aperture > 0.0f) {
+    const f3 eye{P.viewInv[12], P.viewInv[13], P.viewInv[14]};
+    const f3 forward = -normalize(f3{P.viewInv[8], P.viewInv[9], P.viewInv[10]});
+    const f3 focusPoint = eye + pr.rd * (P.focus / dot(pr.rd, forward));
+    float ux, uy, lx, ly;
+    smp.s2(1u, ux, uy);
+    sample_circle(ux, uy, lx, ly);
+    const f3 origin = xform(vi, f3{P.aperture * lx, P.aperture * ly, 0.0f}, 1.0f);
+    pr.ro = origin;
+    pr.rd = normalize(focusPoint - origin);
+  }
   pr.li = splat(0.0f);
   pr.beta = splat(1.0f);
   pr.auxAlbedo = splat(0.0f);

@@ -31,6 +31,13 @@ namespace mtx {
 
 constexpr int kWfTraceBlock = 128;
 constexpr int kWfShadeBlock = 128;
+// Occupancy the register allocator must reach for the traversal kernels
+// (waves per SIMD). Measured on the 1M-triangle 1080p workload: closest-hit
+// is fastest at 6 (80 VGPRs, a few spilled; 5 unconstrained, 8 spills too
+// much), any-hit at 8 (64 VGPRs). The 16-entry LDS stack keeps LDS from
+// capping these.
+constexpr int kTraceWaves = 6;
+constexpr int kShadowWaves = 8;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -54,11 +61,21 @@ __device__ __forceinline__ const JobSlot &job_of(const WfLaunch &L, const WfBuff
   return kMulti ? L.jobs[W.item[p]] : L.one;
 }
 
+// Paths of a job are numbered pixel-major: path pathBase + local * samples + s
+// is sample s of pixel local, so a wave holds a few pixels' samples and a
+// contiguous queue range covers a few image rows (coherent rays, a small
+// geometry working set per window of the queue).
+__device__ __forceinline__ void path_pixel(const JobSlot &J, uint32_t p, uint32_t &local, uint32_t &s) {
+  const uint32_t rel = p - J.pathBase;
+  local = rel / J.params.samples;
+  s = rel - local * J.params.samples;
+}
+
 // Sampler of path p: pixel and sample index within its job.
 __device__ __forceinline__ Smp path_smp(const JobSlot &J, uint32_t p) {
   const RenderParams &P = J.params;
-  const uint32_t rel = p - J.pathBase;
-  const uint32_t local = rel % P.pixelCount, s = rel / P.pixelCount;
+  uint32_t local, s;
+  path_pixel(J, p, local, s);
   const uint32_t pix = P.pixelBase + local;
   return path_sampler(P, pix % P.width, pix / P.width, J.acc.count[pix] + P.sampleOffset + s);
 }
@@ -81,10 +98,9 @@ __global__ void k_wf_reset(WfBuffers W, uint32_t initial) {
 // Whether sample s of path p's pixel is still below the pixel's target.
 __device__ __forceinline__ bool below_target(const JobSlot &J, uint32_t p) {
   if (!J.acc.targets) return true;
-  const RenderParams &P = J.params;
-  const uint32_t rel = p - J.pathBase;
-  const uint32_t local = rel % P.pixelCount, s = rel / P.pixelCount;
-  const uint32_t pix = P.pixelBase + local;
+  uint32_t local, s;
+  path_pixel(J, p, local, s);
+  const uint32_t pix = J.params.pixelBase + local;
   return (double)(J.acc.count[pix] + s) < (double)J.acc.targets[pix];
 }
 
@@ -124,31 +140,93 @@ __global__ void __launch_bounds__(256) k_wf_generate(const WfLaunch L, WfBuffers
   }
 }
 
-__device__ __forceinline__ void flush_work(uint64_t *work, const uint64_t v0[4], const int slots[4]) {
+// Work distribution of the traversal kernels. The queue of n entries is cut
+// into kWfXcds contiguous segments, one per XCD (workgroups are dispatched
+// to the XCDs round-robin, so block b runs on XCD b % 8). Neighbouring queue
+// entries are neighbouring pixels (pixel-major paths), so each XCD's L2
+// serves rays of a few image rows at a time.
+//   mode 0: grid-stride loop over the whole queue (no segments)
+//   mode 1: a wave takes the next kFetchChunk entries of its XCD's segment
+//           with one atomic and helps the other segments once its own is
+//           drained
+//   mode 2: static segments, each XCD's blocks stride over their own
+// Measured on the 1M-triangle 1080p workload: closest-hit is fastest with
+// mode 1 (its rays are long enough to amortize the atomic), any-hit with
+// mode 0; mode 2 loses to load imbalance, chunks above 64 serialize the
+// short queues of late events.
+constexpr int kTraceFetch = 1;
+constexpr int kShadowFetch = 0;
+constexpr uint32_t kFetchChunk = 64;
+__device__ __forceinline__ bool fetch_chunk(uint32_t *ctr, uint32_t n, uint32_t &seg, uint32_t &base,
+                                            uint32_t &limit) {
+  uint32_t b0 = 0, l0 = 0, s0 = seg;
+  if (lane_id() == 0) {
+    for (uint32_t k = 0; k < (uint32_t)kWfXcds; k++) {
+      const uint32_t x = (seg + k) % (uint32_t)kWfXcds;
+      const uint32_t b = (uint32_t)(((uint64_t)n * x) / kWfXcds);
+      const uint32_t e = (uint32_t)(((uint64_t)n * (x + 1)) / kWfXcds);
+      if (b >= e) continue;
+      const uint32_t c = atomicAdd(ctr + x, 1u);
+      if (c < (e - b + kFetchChunk - 1u) / kFetchChunk) {
+        b0 = b + c * kFetchChunk;
+        l0 = min(e, b0 + kFetchChunk);
+        s0 = x;
+        break;
+      }
+    }
+  }
+  base = __shfl(b0, 0);
+  limit = __shfl(l0, 0);
+  seg = __shfl(s0, 0);
+  return base < limit;
+}
+
+// Calls ray(i) once for every queue entry i < n, spread over the grid by kFetchMode.
+template <int kFetchMode, class F>
+__device__ __forceinline__ void for_each_ray(uint32_t n, uint32_t *ctr, F &&ray) {
+  if (kFetchMode == 1) {
+    // Waves beyond one per chunk could only probe drained segments: they
+    // leave at once (the first ceil(n / 64) waves keep fetching until every
+    // segment is drained, so no chunk is left behind).
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= (n + kFetchChunk - 1u) / kFetchChunk) return;
+    uint32_t seg = blockIdx.x % (uint32_t)kWfXcds, base, limit;
+    while (fetch_chunk(ctr, n, seg, base, limit))
+      for (uint32_t i = base + lane_id(); i < limit; i += 64u) ray(i);
+  } else if (kFetchMode == 2 && gridDim.x % (uint32_t)kWfXcds == 0u) {
+    const uint32_t x = blockIdx.x % (uint32_t)kWfXcds, nb = gridDim.x / (uint32_t)kWfXcds;
+    const uint32_t b = (uint32_t)(((uint64_t)n * x) / kWfXcds), e = (uint32_t)(((uint64_t)n * (x + 1)) / kWfXcds);
+    for (uint32_t i = b + (blockIdx.x / kWfXcds) * blockDim.x + threadIdx.x; i < e; i += nb * blockDim.x) ray(i);
+  } else {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ray(i);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void flush_work(uint64_t *work, const uint64_t (&v0)[N], const int (&slots)[N]) {
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < N; k++) {
     uint64_t v = v0[k];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if (lane_id() == 0 && v) atomicAdd((unsigned long long *)work + slots[k], (unsigned long long)v);
   }
 }
 
-template <bool kMulti, bool kCount>
-__global__ void __launch_bounds__(kWfTraceBlock)
+template <bool kMulti, bool kCount, int kLds>
+__global__ void __launch_bounds__(kWfTraceBlock, kTraceWaves)
 k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
-  __shared__ int32_t lds[kLdsStack * kWfTraceBlock];
+  __shared__ int32_t lds[kLds * kWfTraceBlock];
   const uint32_t n = W.counts[event];
-  if (n == 0u) return;
-  LaneStack st{lds + threadIdx.x, kWfTraceBlock,
-               spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
+  if (n == 0u || W.counts[kWfTailFlag]) return;
+  LaneStackT<kLds> st{lds + threadIdx.x, kWfTraceBlock,
+                      spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLds)};
   const uint32_t *q = queue_of(W, event);
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  if (tid == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
   TraceCounters tc{0u, 0u, 0u};
   uint32_t hits = 0;
   AccelView one{};
   if (!kMulti) one = *L.one.scene.accel;
-  for (uint32_t i = tid; i < n; i += stride) {
+  auto ray = [&](uint32_t i) {
     const uint32_t p = q[i];
     const float4 o = W.o[p], d = W.d[p];
     const int32_t curTri = fint(W.beta[p].w), curSlot = fint(W.alb[p].w);
@@ -164,10 +242,12 @@ k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill
     W.hitA[p] = make_float4(h.t, h.bary.x, h.bary.y, h.bary.z);
     W.hitB[p] = make_int2(h.hit ? h.slot : -1, h.tri);
     if (kCount) hits += h.hit ? 1u : 0u;
-  }
+  };
+  for_each_ray<kTraceFetch>(n, W.counts + kWfXcdBase + (2 * event) * kWfXcds, ray);
   if (kCount) {
-    const uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, hits};
-    const int slots[4] = {0, 1, 2, 3};
+    const uint64_t v[5] = {(blockIdx.x == 0 && threadIdx.x == 0) ? (uint64_t)n : 0u, tc.boxes, tc.tris, tc.instances,
+                           hits};
+    const int slots[5] = {kWorkClosestRays, kWorkBoxes, kWorkTris, kWorkInstances, kWorkHits};
     flush_work(L.work, v, slots);
   }
 }
@@ -177,7 +257,7 @@ k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill
 template <bool kMulti, int kWaves>
 __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaunch L, WfBuffers W, int event) {
   const uint32_t n = W.counts[event];
-  if (n == 0u) return;
+  if (n == 0u || W.counts[kWfTailFlag]) return;
   const uint32_t *q = queue_of(W, event);
   uint32_t *qNext = queue_of(W, event + 1);
   uint32_t *nNext = W.counts + event + 1;
@@ -247,20 +327,19 @@ __device__ __forceinline__ void add_payload(const WfBuffers &W, uint32_t p, floa
   W.li[p] = l;
 }
 
-template <bool kMulti, bool kCount>
-__global__ void __launch_bounds__(kWfTraceBlock)
+template <bool kMulti, bool kCount, int kLds>
+__global__ void __launch_bounds__(kWfTraceBlock, kShadowWaves)
 k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
-  __shared__ int32_t lds[kLdsStack * kWfTraceBlock];
+  __shared__ int32_t lds[kLds * kWfTraceBlock];
   const uint32_t n = W.counts[kWfShadowBase + event];
-  if (n == 0u) return;
-  LaneStack st{lds + threadIdx.x, kWfTraceBlock,
-               spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  if (tid == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
+  if (n == 0u || W.counts[kWfTailFlag]) return;
+  LaneStackT<kLds> st{lds + threadIdx.x, kWfTraceBlock,
+                      spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLds)};
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
   TraceCounters tc{0u, 0u, 0u};
   AccelView one{};
   if (!kMulti) one = *L.one.scene.accel;
-  for (uint32_t i = tid; i < n; i += stride) {
+  auto ray = [&](uint32_t i) {
     const float4 a = W.s0[i], b = W.s1[i], c = W.s2[i];
     const uint32_t p = (uint32_t)fint(b.w);
     TraceHit h;
@@ -272,10 +351,81 @@ k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spil
       occluded = traverse<true, kCount>(one, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc);
     }
     if (!occluded) add_payload(W, p, c);
-  }
+  };
+  for_each_ray<kShadowFetch>(n, W.counts + kWfXcdBase + (2 * event + 1) * kWfXcds, ray);
   if (kCount) {
     const uint64_t v[4] = {tc.boxes, tc.tris, tc.instances, (blockIdx.x == 0 && threadIdx.x == 0) ? (uint64_t)n : 0u};
-    const int slots[4] = {5, 6, 7, 4};
+    const int slots[4] = {kWorkShadowBoxes, kWorkShadowTris, kWorkShadowInstances, kWorkShadowRays};
+    flush_work(L.work, v, slots);
+  }
+}
+
+// Tail of a generation: once event `event`'s queue holds at most `tailPaths`
+// paths, one lane takes each of them through all its remaining events
+// (closest hit, shade_event, shadow any-hit, in the order the wavefront
+// kernels apply them), so the few long paths left (glass, deep bounces) no
+// longer pay three launches and a grid-wide pass per event. The queue length
+// was written by earlier kernels of the stream, so every block takes the same
+// decision; the flag it raises makes the remaining wavefront launches of the
+// generation return at once. Per-path arithmetic is that of the wavefront
+// kernels, so results do not depend on where the tail starts.
+template <bool kMulti, bool kCount>
+__global__ void __launch_bounds__(kWfShadeBlock)
+k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t *__restrict__ spill) {
+  __shared__ int32_t lds[kMinLdsStack * kWfShadeBlock];
+  const uint32_t n = W.counts[event];
+  const uint32_t flag = W.counts[kWfTailFlag];
+  if (n == 0u || n > tailPaths || (flag != 0u && flag != (uint32_t)event + 1u)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[kWfTailFlag] = (uint32_t)event + 1u;
+  LaneStackT<kMinLdsStack> st{lds + threadIdx.x, kWfShadeBlock,
+                              spill + ((size_t)blockIdx.x * kWfShadeBlock + threadIdx.x) * (kMaxStack - kMinLdsStack)};
+  const uint32_t *q = queue_of(W, event);
+  uint64_t rays = 0;
+  TraceCounters tc{0u, 0u, 0u}, ts{0u, 0u, 0u};
+  uint64_t nClosest = 0, nHits = 0, nShadow = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = q[i];
+    const JobSlot &J = job_of<kMulti>(L, W, p);
+    const GpuScene &S = J.scene;
+    const RenderParams &P = J.params;
+    const AccelView av = *S.accel;
+    const Lut lut{S.lut};
+    Smp smp = path_smp(J, p);
+    PathRegs pr;
+    const float4 o = W.o[p], d = W.d[p], l = W.li[p], b = W.beta[p], al = W.alb[p], nr = W.nrm[p];
+    pr.ro = f3of(o); pr.scatPdf = o.w;
+    pr.rd = f3of(d); pr.alpha = d.w;
+    pr.li = f3of(l); pr.flags = (uint32_t)fint(l.w);
+    pr.beta = f3of(b); pr.curTri = fint(b.w);
+    pr.auxAlbedo = f3of(al); pr.curSlot = fint(al.w);
+    pr.auxNormal = f3of(nr);
+    for (int e = event; e < kWfMaxEvents; e++) {
+      TraceHit h;
+      rays++;
+      if (kCount) nClosest++;
+      traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, e != 0, trace_depth(e), pr.curTri, pr.curSlot, st,
+                              h, &tc);
+      if (kCount) nHits += h.hit ? 1u : 0u;
+      ShadowReq sr;
+      const bool alive = shade_event(S, P, lut, pr, h, e, smp, sr);
+      if (sr.valid) {
+        TraceHit sh;
+        rays++;
+        if (kCount) nShadow++;
+        if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts))
+          pr.li = pr.li + sr.payload;
+      }
+      if (!alive) break;
+    }
+    store_path(W, p, pr);
+  }
+  for (int off = 32; off > 0; off >>= 1) rays += __shfl_down(rays, off);
+  if (lane_id() == 0 && rays) atomicAdd((unsigned long long *)L.rays, (unsigned long long)rays);
+  if (kCount) {
+    const uint64_t v[9] = {nClosest, tc.boxes, tc.tris, tc.instances, nHits, nShadow, ts.boxes, ts.tris, ts.instances};
+    int slots[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) slots[k] = kWorkTailBase + k;
     flush_work(L.work, v, slots);
   }
 }
@@ -288,7 +438,7 @@ __global__ void __launch_bounds__(kWfTraceBlock)
 k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLds * kWfTraceBlock];
   const uint32_t n = W.counts[kShadow ? kWfShadowBase + event : event];
-  if (n == 0u) return;
+  if (n == 0u || W.counts[kWfTailFlag]) return;
   uint32_t *fetch = W.counts + kWfFetchBase + 2 * event + (kShadow ? 1 : 0);
   WalkStack<kLds> st{lds + threadIdx.x, kWfTraceBlock,
                      spill + ((size_t)blockIdx.x * kWfTraceBlock + threadIdx.x) * (kMaxStack - kLds)};
@@ -343,9 +493,16 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
     }
   }
   if (kCount) {
-    if (kShadow && blockIdx.x == 0 && threadIdx.x == 0) totals[3] = n;
-    const int slots[4] = {kShadow ? 5 : 0, kShadow ? 6 : 1, kShadow ? 7 : 2, kShadow ? 4 : 3};
-    flush_work(L.work, totals, slots);
+    const uint64_t first = (blockIdx.x == 0 && threadIdx.x == 0) ? (uint64_t)n : 0u;
+    if (kShadow) {
+      const uint64_t v[4] = {totals[0], totals[1], totals[2], first};
+      const int slots[4] = {kWorkShadowBoxes, kWorkShadowTris, kWorkShadowInstances, kWorkShadowRays};
+      flush_work(L.work, v, slots);
+    } else {
+      const uint64_t v[5] = {first, totals[0], totals[1], totals[2], totals[3]};
+      const int slots[5] = {kWorkClosestRays, kWorkBoxes, kWorkTris, kWorkInstances, kWorkHits};
+      flush_work(L.work, v, slots);
+    }
   }
 }
 
@@ -379,7 +536,7 @@ __global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffe
       while (done < P.samples && (double)(count0 + done) < target) done++;
     }
     for (uint32_t s = 0; s < done; s++) {
-      const uint32_t p = J.pathBase + s * P.pixelCount + local;
+      const uint32_t p = J.pathBase + local * P.samples + s;
       const float4 l = W.li[p], d = W.d[p], al = W.alb[p], nr = W.nrm[p];
       PathRegs pr;
       pr.li = f3of(l);
@@ -450,15 +607,43 @@ static void launch_walk(const WfConfig &cfg, uint32_t grid, hipStream_t s, const
   else launch_walk_lds<kMulti, kShadow, 32>(grid, s, L, W, e, spill);
 }
 
+template <bool kMulti, bool kShadow, int kLds>
+static void launch_trace_lds(uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W, int e,
+                             int32_t *spill) {
+  if (kShadow) {
+    if (L.work)
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    else
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+  } else {
+    if (L.work)
+      hipLaunchKernelGGL((k_wf_trace<kMulti, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    else
+      hipLaunchKernelGGL((k_wf_trace<kMulti, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+  }
+}
+// Closest-hit (kShadow false) or any-hit launch with the configured LDS stack depth.
+template <bool kMulti, bool kShadow>
+static void launch_trace(const WfConfig &cfg, uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
+                         int e, int32_t *spill) {
+  if (cfg.ldsStack <= 16) launch_trace_lds<kMulti, kShadow, 16>(grid, s, L, W, e, spill);
+  else if (cfg.ldsStack <= 24) launch_trace_lds<kMulti, kShadow, 24>(grid, s, L, W, e, spill);
+  else launch_trace_lds<kMulti, kShadow, 32>(grid, s, L, W, e, spill);
+}
+
 struct Grids {
-  uint32_t trace, walk, shade, cus;
+  uint32_t trace, walk, shade, tail, cus;
 };
 static Grids grids_for(const WfConfig &cfg) {
-  static int traceBpc = 0, shadeBpc[5] = {0, 0, 0, 0, 0}, walkBpc[3] = {0, 0, 0};
+  static int traceBpc[3] = {0, 0, 0}, shadeBpc[5] = {0, 0, 0, 0, 0}, walkBpc[3] = {0, 0, 0}, tailBpc = 0;
   const int cus = cu_count();
-  if (!traceBpc) {
-    traceBpc = std::max(blocks_per_cu((const void *)k_wf_trace<false, false>, kWfTraceBlock, 0),
-                        blocks_per_cu((const void *)k_wf_shadow<false, false>, kWfTraceBlock, 0));
+  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
+  if (!traceBpc[li]) {
+    const void *t = li == 0 ? (const void *)k_wf_trace<false, false, 16>
+                  : li == 1 ? (const void *)k_wf_trace<false, false, 24> : (const void *)k_wf_trace<false, false, 32>;
+    const void *a = li == 0 ? (const void *)k_wf_shadow<false, false, 16>
+                  : li == 1 ? (const void *)k_wf_shadow<false, false, 24> : (const void *)k_wf_shadow<false, false, 32>;
+    traceBpc[li] = std::max(blocks_per_cu(t, kWfTraceBlock, 0), blocks_per_cu(a, kWfTraceBlock, 0));
   }
   const int sw = std::min(4, std::max(1, cfg.shadeWaves));
   if (!shadeBpc[sw]) {
@@ -466,16 +651,17 @@ static Grids grids_for(const WfConfig &cfg) {
                   : sw == 3 ? (const void *)k_wf_shade<false, 3> : (const void *)k_wf_shade<false, 4>;
     shadeBpc[sw] = blocks_per_cu(k, kWfShadeBlock, 0);
   }
-  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
   if (!walkBpc[li]) {
     const void *k = li == 0 ? (const void *)k_wf_walk<false, false, false, 16>
                   : li == 1 ? (const void *)k_wf_walk<false, false, false, 24>
                             : (const void *)k_wf_walk<false, false, false, 32>;
     walkBpc[li] = blocks_per_cu(k, kWfTraceBlock, 0);
   }
+  if (!tailBpc) tailBpc = blocks_per_cu((const void *)k_wf_tail<false, false>, kWfShadeBlock, 0);
   Grids g;
   g.cus = (uint32_t)cus;
-  g.trace = (uint32_t)(cus * traceBpc * std::max(1, cfg.traceOversub));
+  g.tail = (uint32_t)(cus * tailBpc);
+  g.trace = (uint32_t)(cus * traceBpc[li] * std::max(1, cfg.traceOversub));
   g.walk = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
   g.shade = (uint32_t)(cus * shadeBpc[sw] * std::max(1, cfg.shadeOversub));
   return g;
@@ -494,13 +680,21 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
   hipLaunchKernelGGL(k_wf_generate<kMulti>, dim3(genGrid), dim3(256), 0, s, L, W, select ? 1 : 0);
   te();
   for (int e = 0; e <= eventLimit; e++) {
+    if (cfg.tailPaths) {
+      tb(5);
+      if (L.work)
+        hipLaunchKernelGGL((k_wf_tail<kMulti, true>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e, cfg.tailPaths,
+                           spill);
+      else
+        hipLaunchKernelGGL((k_wf_tail<kMulti, false>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+                           cfg.tailPaths, spill);
+      te();
+    }
     tb(1);
     if (cfg.walker)
       launch_walk<kMulti, false>(cfg, g.walk, s, L, W, e, spill);
-    else if (L.work)
-      hipLaunchKernelGGL((k_wf_trace<kMulti, true>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
     else
-      hipLaunchKernelGGL((k_wf_trace<kMulti, false>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+      launch_trace<kMulti, false>(cfg, g.trace, s, L, W, e, spill);
     te();
     tb(2);
     switch (cfg.shadeWaves) {
@@ -513,10 +707,8 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
     tb(3);
     if (cfg.walker)
       launch_walk<kMulti, true>(cfg, g.walk, s, L, W, e, spill);
-    else if (L.work)
-      hipLaunchKernelGGL((k_wf_shadow<kMulti, true>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
     else
-      hipLaunchKernelGGL((k_wf_shadow<kMulti, false>), dim3(g.trace), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+      launch_trace<kMulti, true>(cfg, g.trace, s, L, W, e, spill);
     te();
   }
   const uint32_t accGrid = std::min<uint32_t>((pixels + 255) / 256, g.cus * 16u);
@@ -549,7 +741,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
   const Grids g = grids_for(cfg);
-  int32_t *spill = spill_for(s, std::max(g.trace, g.walk), kWfTraceBlock);
+  int32_t *spill = spill_for(s, std::max(std::max(g.trace, g.walk), g.tail), kWfTraceBlock);
   const int eventLimit = std::min((int)p0.bounces * 3, 50);
   uint32_t done = 0;
   while (done < p0.samples) {
@@ -588,7 +780,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
       key = fnv(key, &W.nPaths, sizeof W.nPaths);
       const int64_t cf[8] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
-                             cfg.shadeWaves, cfg.graphs, 0};
+                             cfg.shadeWaves, cfg.graphs, cfg.tailPaths};
       key = fnv(key, cf, sizeof cf);
       key = fnv(key, &spill, sizeof spill);
       key = fnv(key, &eventLimit, sizeof eventLimit);
@@ -620,7 +812,7 @@ void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t n
   if (!nJobs || !nPaths) return;
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
-  int32_t *spill = spill_for(s, std::max(g.trace, g.walk), kWfTraceBlock);
+  int32_t *spill = spill_for(s, std::max(std::max(g.trace, g.walk), g.tail), kWfTraceBlock);
   buf.ensure(nPaths);
   const WfBuffers W = buf.view(nPaths);
   WfLaunch L{};

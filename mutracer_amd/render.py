@@ -130,21 +130,30 @@ class RenderTarget:
         sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
         abi.check(abi.lib().mtx_target_render(self.handle.value, samples, sp))
 
+    WORK_KEYS = ("closest_rays", "box_tests", "tri_tests", "instance_visits", "hits", "shadow_rays",
+                 "shadow_box_tests", "shadow_tri_tests", "shadow_instance_visits")
+    TIMER_CLASSES = ("generate_accumulate", "trace", "shade", "shadow", "megakernel", "tail")
+
     def render_counted(self, samples, stream=None):
-        """render() plus the launch's traversal work totals (synchronous)."""
+        """render() plus the launch's traversal work (synchronous): totals under
+        WORK_KEYS, and the same counters per kernel group under "wavefront"
+        (trace/shadow kernels or the megakernel) and "tail" (tail kernel)."""
         sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
-        out = np.zeros(8, np.uint64)
+        n = len(self.WORK_KEYS)
+        out = np.zeros(2 * n, np.uint64)
         abi.check(abi.lib().mtx_target_render_counted(self.handle.value, samples, _P(out), sp))
-        keys = ("box_tests", "tri_tests", "instance_visits", "hits", "shadow_rays", "shadow_box_tests",
-                "shadow_tri_tests", "shadow_instance_visits")
-        return {k: int(v) for k, v in zip(keys, out)}
+        groups = {g: {k: int(v) for k, v in zip(self.WORK_KEYS, out[i * n:(i + 1) * n])}
+                  for i, g in enumerate(("wavefront", "tail"))}
+        res = {k: groups["wavefront"][k] + groups["tail"][k] for k in self.WORK_KEYS}
+        res.update(groups)
+        return res
 
     def render_timed(self, samples, stream=None):
         """render() with HIP events around every launch: GPU ms per kernel class (synchronous)."""
         sp = None if stream is None else C.c_void_p(int(stream.cuda_stream))
-        out = np.zeros(5, np.float64)
+        out = np.zeros(len(self.TIMER_CLASSES), np.float64)
         abi.check(abi.lib().mtx_target_render_timed(self.handle.value, samples, _P(out), sp))
-        return dict(zip(("generate_accumulate", "trace", "shade", "shadow", "megakernel"), map(float, out)))
+        return dict(zip(self.TIMER_CLASSES, map(float, out)))
 
     def trace(self, camera=None, max_samples=1):
         """RenderTarget.trace: one more sample unless `max_samples` are done.
@@ -163,7 +172,7 @@ class RenderTarget:
         return dict(max_paths_in_flight=t.max_paths_in_flight, trace_kernel=t.trace_kernel, lds_stack=t.lds_stack,
                     trace_oversubscription=t.trace_oversubscription, shade_oversubscription=t.shade_oversubscription,
                     shade_waves_per_simd=t.shade_waves_per_simd, use_graphs=t.use_graphs,
-                    graph_replays=t.graph_replays, graph_captures=t.graph_captures)
+                    graph_replays=t.graph_replays, graph_captures=t.graph_captures, tail_paths=t.tail_paths)
 
     def set_tuning(self, **kw):
         """Changes the wavefront launch shape (keys as returned by tuning())."""
@@ -178,6 +187,7 @@ class RenderTarget:
         t = WavefrontTuning(C.sizeof(WavefrontTuning), *[int(cur[k]) for k in (
             "max_paths_in_flight", "trace_kernel", "lds_stack", "trace_oversubscription", "shade_oversubscription",
             "shade_waves_per_simd", "use_graphs")])
+        t.tail_paths = int(cur["tail_paths"])
         abi.check(abi.lib().mtx_target_set_tuning(self.handle.value, C.byref(t)))
 
     def synchronize(self):

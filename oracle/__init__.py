@@ -78,6 +78,20 @@ def env_cdf(env):
     return out
 
 
+def camera_rays(camera, width, height, pixels, pattern=0, seed=0, log_spp=0):
+    """Primary rays of (px, py, sample index) rows: (n, 6) origin + direction,
+    pinhole or thin lens as the packed camera's aperture says."""
+    L = lib()
+    L.orc_camera_rays.restype = None
+    L.orc_camera_rays.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_void_p]
+    cam = np.ascontiguousarray(camera, np.float32)
+    pix = np.ascontiguousarray(pixels, np.uint32).reshape(-1, 3)
+    out = np.zeros((pix.shape[0], 6), np.float32)
+    L.orc_camera_rays(_p(cam), width, height, _p(pix), pix.shape[0], pattern, seed & 0xFFFFFFFF, log_spp, _p(out))
+    return out
+
+
 def threads():
     """OpenMP threads the oracle's batch entry points run on."""
     return int(lib().orc_threads())

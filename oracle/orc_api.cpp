@@ -259,3 +259,10 @@ extern "C" void orc_env_cdf(const float *env, uint32_t envW, uint32_t envH, floa
   build_env_cdf(s, T);
   for (size_t i = 0; i < T.cdf.size(); i++) out[i] = T.cdf[i];
 }
+
+extern "C" void orc_camera_rays(const float *camera, uint32_t width, uint32_t height, const uint32_t *pix,
+                                uint64_t n, uint32_t pattern, uint32_t seed, uint32_t logSPP, float *out) {
+  for (uint64_t i = 0; i < n; i++)
+    orc::camera_sample(camera, width, height, pix[3 * i], pix[3 * i + 1], pix[3 * i + 2], pattern, seed, logSPP,
+                       out + 6 * i);
+}

@@ -1124,21 +1124,68 @@ static void camera_ray(const float *cam, float cx, float cy, V3 &o, V3 &d) {
   d = normalize(mat_mul_point(viewInv, dir, 0.0f));
 }
 
-static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleIndex, Accum &acc) {
-  const RenderScene &s = c.s;
-  Sampler smp{};
-  smp.px = px; smp.py = py; smp.width = s.width;
-  smp.pixelSampleIndex = sampleIndex + s.sampleOffset;
-  smp.sequenceSeed = s.seed; smp.zSobolLogSPP = s.logSPP; smp.pattern = s.pattern; smp.bevelSamples = 0;
-  smp.samplingBounce = 0; smp.hashedCounter = 0;
-  float W = (float)s.width, H = (float)s.height;
+// sampleCircle (rayTrace.compute.ts:1148-1154).
+static void sample_circle(float ux, float uy, float &cx, float &cy) {
+  float px = 2.0f * ux - 1.0f, py = 2.0f * uy - 1.0f;
+  bool greater = fabsf(px) > fabsf(py);
+  float r = greater ? px : py;
+  float theta = greater ? K_QUARTER_PI * py / px : K_PI * (0.5f - 0.25f * px / py);
+  cx = r * mtxfm::cosf(theta);
+  cy = r * mtxfm::sinf(theta);
+}
+
+// Thin lens (rayTrace.compute.ts:1196-1205).
+static void thin_lens(const float *cam, Sampler &smp, V3 &o, V3 &d) {
+  const float *viewInv = cam + 16;
+  float aperture = cam[32], focus = cam[33];
+  V3 eye(viewInv[12], viewInv[13], viewInv[14]);
+  V3 forward = -normalize(V3(viewInv[8], viewInv[9], viewInv[10]));
+  V3 focusPoint = eye + d * (focus / dot(d, forward));
+  float u[2], lx, ly;
+  smp.sample2d(CAMERA_LENS, u);
+  sample_circle(u[0], u[1], lx, ly);
+  V3 origin = mat_mul_point(viewInv, V3(aperture * lx, aperture * ly, 0.0f), 1.0f);
+  o = origin;
+  d = normalize(focusPoint - origin);
+}
+
+// main (rayTrace.compute.ts:1188-1205): pixel-centre uv, Sobol jitter,
+// camera ray, thin lens.
+static void primary_ray(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py,
+                        Sampler &smp, V3 &o, V3 &d) {
+  float W = (float)width, H = (float)height;
   float uvx = ((float)px + 0.5f) / W, uvy = ((float)py + 0.5f) / H;
   float j[2];
   smp.sample2d(CAMERA_PIXEL, j);
   float jx = (j[0] - 0.5f) / W, jy = (j[1] - 0.5f) / H;
   float cx = uvx + jx, cy = uvy + jy;
+  camera_ray(camera, cx, cy, o, d);
+  if (camera[32] > 0.0f) thin_lens(camera, smp, o, d);
+}
+
+static Sampler make_sampler(uint32_t px, uint32_t py, uint32_t width, uint32_t index, uint32_t pattern,
+                            uint32_t seed, uint32_t logSPP) {
+  Sampler smp{};
+  smp.px = px; smp.py = py; smp.width = width;
+  smp.pixelSampleIndex = index;
+  smp.sequenceSeed = seed; smp.zSobolLogSPP = logSPP; smp.pattern = pattern; smp.bevelSamples = 0;
+  smp.samplingBounce = 0; smp.hashedCounter = 0;
+  return smp;
+}
+
+void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py, uint32_t index,
+                   uint32_t pattern, uint32_t seed, uint32_t logSPP, float out[6]) {
+  Sampler smp = make_sampler(px, py, width, index, pattern, seed, logSPP);
+  V3 o, d;
+  primary_ray(camera, width, height, px, py, smp, o, d);
+  out[0] = o.x; out[1] = o.y; out[2] = o.z; out[3] = d.x; out[4] = d.y; out[5] = d.z;
+}
+
+static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleIndex, Accum &acc) {
+  const RenderScene &s = c.s;
+  Sampler smp = make_sampler(px, py, s.width, sampleIndex + s.sampleOffset, s.pattern, s.seed, s.logSPP);
   PathState path{};
-  camera_ray(s.camera, cx, cy, path.ro, path.rd);
+  primary_ray(s.camera, s.width, s.height, px, py, smp, path.ro, path.rd);
   Aux aux{V3(0.0f), V3(0.0f), false};
   // integrator (rayTrace.compute.ts:891-946)
   Surface si{};

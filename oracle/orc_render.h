@@ -58,6 +58,9 @@ struct HostTables {
   uint32_t cdfW = 0, cdfH = 0;
 };
 void build_tables(const RenderScene &s, HostTables &T);
+// Primary ray of (pixel, sample index): origin xyz, direction xyz.
+void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py, uint32_t index,
+                   uint32_t pattern, uint32_t seed, uint32_t logSPP, float out[6]);
 void build_env_cdf(const RenderScene &s, HostTables &T);
 
 void set_energy_lut(const float *lut);  // 32 x 2177 floats

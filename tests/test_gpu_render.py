@@ -117,17 +117,57 @@ def test_clamping_and_offset(env, orc, kernel):
     _check(sums, counts, ref, 6)
 
 
-def test_read_pixels_resolve(env, orc):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_thin_lens(env, orc, kernel):
+    """Aperture > 0: lens samples from the CAMERA_LENS dimension, focus plane
+    perpendicular to the view direction (rayTrace.compute.ts:1196-1205)."""
     torch, render, scene = env
-    sc = scene.cornell_box()
+    sc = scene.instanced_field(n_instances=6, freq=6, env_size=(64, 32))
+    sc.camera.aperture = 0.4
+    sc.camera.focus_distance = 12.0
+    sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, bounces=6, max_samples=8, kernel=kernel,
+                                 direct_light_candidates=2)
+    _check(sums, counts, ref, 6)
+
+
+def _resolve_expected(sums):
+    """The output pass of WGPURayTraceOutput.ts:32-92 on the sums (bottom row
+    first), evaluated per pixel in Python floats."""
+    h, w, _ = sums.shape
+    out = {k: np.zeros((h, w, 4)) for k in ("color", "albedo", "normal")}
+    for y in range(h):
+        for x in range(w):
+            s = [float(v) for v in sums[y, x]]
+            n, hit = s[11], s[7]
+            if n <= 0:
+                continue
+            total = max(n, 1.0)
+            div = hit if hit > 0 else total
+            out["color"][y, x] = [s[0] / div, s[1] / div, s[2] / div, s[3] / total]
+            out["albedo"][y, x] = [s[4] / total, s[5] / total, s[6] / total, min(max(hit / total, 0.0), 1.0)]
+            if s[8] * s[8] + s[9] * s[9] + s[10] * s[10] > 0.1:
+                out["normal"][y, x] = [s[8] / total * 0.5 + 0.5, s[9] / total * 0.5 + 0.5,
+                                       s[10] / total * 0.5 + 0.5, 1.0]
+    return {k: v[::-1].astype(np.float32) for k, v in out.items()}
+
+
+def test_read_pixels_resolve(env, orc):
+    """Sky pixels without an environment are blank (hit count 0): their color
+    divides by the sample count, geometry pixels by the hit count."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=4, freq=4, env_size=(64, 32))
+    sc.env = None
     rs = render.RenderScene(sc)
-    t = render.RenderTarget(rs, 32, 24, bounces=4, max_samples=4)
-    t.render(4)
+    t = render.RenderTarget(rs, 48, 32, bounces=4, max_samples=4)
+    t.render(3)
     sums, counts = t.read_accumulators()
-    px = t.read_pixels("color")
-    mean = (sums[..., 0:4] / counts[..., None])[::-1]
-    assert np.allclose(px, mean.astype(np.float32), rtol=0, atol=0)
-    assert t.stats()["rays"] > 32 * 24 * 4
+    assert (sums[..., 7] == 0).any() and (sums[..., 7] > 0).any()
+    exp = _resolve_expected(sums)
+    for kind in ("color", "albedo", "normal"):
+        assert np.array_equal(t.read_pixels(kind), exp[kind]), kind
+    assert t.stats()["rays"] > 0
+    t.restart_samples()
+    assert not t.read_pixels("color").any()  # no samples: zeros
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -138,6 +178,8 @@ def test_counted_launch_matches_plain(env, orc, kernel):
     rs = render.RenderScene(sc)
     a = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4, kernel=kernel)
     b = render.RenderTarget(rs, 64, 48, bounces=6, direct_light_candidates=2, max_samples=4, kernel=kernel)
+    if kernel == "wavefront":
+        b.set_tuning(tail_paths=3000)  # the tail kernel takes over mid-generation
     a.render(4)
     work = b.render_counted(4)
     sa, _ = a.read_accumulators()
@@ -148,6 +190,10 @@ def test_counted_launch_matches_plain(env, orc, kernel):
     assert work["box_tests"] > work["tri_tests"] > 0
     assert work["instance_visits"] >= work["hits"]
     assert work["shadow_box_tests"] > work["shadow_tri_tests"] > 0
+    assert work["closest_rays"] + work["shadow_rays"] == rays
+    assert work["wavefront"]["box_tests"] + work["tail"]["box_tests"] == work["box_tests"]
+    if kernel == "wavefront":
+        assert work["wavefront"]["closest_rays"] > 0 and work["tail"]["closest_rays"] > 0
 
 
 def test_wavefront_equals_megakernel_large(env, orc):
@@ -171,9 +217,14 @@ def test_wavefront_equals_megakernel_large(env, orc):
     dict(trace_kernel=1, lds_stack=32, shade_oversubscription=3, max_paths_in_flight=3000),
     dict(trace_kernel=0, shade_waves_per_simd=2),
     dict(trace_kernel=0, shade_waves_per_simd=4, shade_oversubscription=2),
+    dict(tail_paths=0),
+    dict(tail_paths=1500),
+    dict(tail_paths=1 << 30),
+    dict(trace_kernel=1, tail_paths=3000, max_paths_in_flight=9000),
 ])
 def test_wavefront_tunings_match_oracle(env, orc, tuning):
-    """Every launch shape of the wavefront kernels yields the oracle's sums."""
+    """Every launch shape of the wavefront kernels yields the oracle's sums,
+    wherever the tail kernel takes over (never, mid-generation, at event 0)."""
     torch, render, scene = env
     sc = scene.instanced_field(n_instances=12, freq=12, env_size=(128, 64))
     rs = render.RenderScene(sc)
