@@ -504,10 +504,19 @@ __global__ void k_children(const BigState *__restrict__ big, uint32_t nBig, BNod
 // ------------------------------------------------------ small-subtree phase
 constexpr int kSmallItems = kSmall / kBuildThreads;  // 4
 
-__global__ void __launch_bounds__(kBuildThreads)
-k_small_build(const uint32_t *__restrict__ roots, BNode *__restrict__ nodes, uint32_t *__restrict__ prim,
-              const float4 *__restrict__ plo, const float4 *__restrict__ phi, const float4 *__restrict__ cen,
-              int32_t bins, double md0, double md1, double md2, uint32_t *__restrict__ counter) {
+// One subtree of at most kSmall primitives: its root node and the arrays of
+// the build it belongs to (the subtrees of every mesh of a scene go out in
+// one launch, so small meshes still fill the GPU).
+struct SmallTask {
+  BNode *nodes;
+  uint32_t *prim;
+  const float4 *plo, *phi, *cen;
+  uint32_t *counter;
+  double md[3];
+  uint32_t root, pad;
+};
+
+__global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *__restrict__ tasks, int32_t bins) {
   typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
   __shared__ typename Scan::TempStorage scanTmp;
   __shared__ uint32_t lp[kSmall], ltmp[kSmall], X[kSmall], Z[kSmall], rk[kSmall];
@@ -522,8 +531,15 @@ k_small_build(const uint32_t *__restrict__ roots, BNode *__restrict__ nodes, uin
   __shared__ int32_t doSplit;
   __shared__ uint32_t totalR;
 
-  const double minDim[3] = {md0, md1, md2};
-  const uint32_t root = roots[blockIdx.x];
+  const SmallTask &tk = tasks[blockIdx.x];
+  BNode *__restrict__ nodes = tk.nodes;
+  uint32_t *__restrict__ prim = tk.prim;
+  const float4 *__restrict__ plo = tk.plo;
+  const float4 *__restrict__ phi = tk.phi;
+  const float4 *__restrict__ cen = tk.cen;
+  uint32_t *__restrict__ counter = tk.counter;
+  const double minDim[3] = {tk.md[0], tk.md[1], tk.md[2]};
+  const uint32_t root = tk.root;
   const BNode rn = nodes[root];
   const uint32_t base = rn.leftFirst, total = rn.primCount;
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) lp[e] = prim[base + e];
@@ -781,14 +797,17 @@ __global__ void k_tlas_refit(TNode *__restrict__ nodes, const InstSlot *__restri
 static inline uint32_t blocks(size_t n, int b = 256) { return (uint32_t)((n + b - 1) / b); }
 
 // Builds the tree over T.primCount boxes in T.plo/T.phi with centers T.cen.
-static void build_topology(Topology &T, uint32_t bins, hipStream_t s) {
+// Splits every node above kSmall primitives level by level and appends the
+// remaining subtree roots to `small` (launched by run_small_builds).
+static void build_topology(Topology &T, uint32_t bins, hipStream_t s, std::vector<SmallTask> &small) {
   const uint32_t n = T.primCount;
   MTX_REQUIRE(bins >= 2 && bins <= (uint32_t)kMaxBins, MTX_INVALID_ARGUMENT, "bins must be in [2, 64]");
   T.nodes.alloc(std::max<uint32_t>(1, 2 * n));
   T.prim.alloc(std::max<uint32_t>(1, n));
   T.nodeCount = 0;
   if (n == 0) return;
-  DevBuf<uint32_t> counter(1);
+  T.counter.alloc(1);
+  DevBuf<uint32_t> &counter = T.counter;
   DevBuf<unsigned long long> keys(6);
   unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
   keys.upload(init, 6, s);
@@ -871,16 +890,27 @@ static void build_topology(Topology &T, uint32_t bins, hipStream_t s) {
     }
     MTX_HIP(hipStreamSynchronize(s));
   }
-  if (!smallIds.empty()) {
-    DevBuf<uint32_t> dRoots;
-    dRoots.upload(smallIds.data(), smallIds.size(), s);
-    hipLaunchKernelGGL(k_small_build, dim3((uint32_t)smallIds.size()), dim3(kBuildThreads), 0, s, dRoots.ptr,
-                       T.nodes.ptr, T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, (int32_t)bins, minDim[0],
-                       minDim[1], minDim[2], counter.ptr);
-    MTX_HIP(hipGetLastError());
-  }
+  for (uint32_t r : smallIds)
+    small.push_back(SmallTask{T.nodes.ptr, T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, counter.ptr,
+                              {minDim[0], minDim[1], minDim[2]}, r, 0u});
+}
+
+// Builds every pending small subtree (one workgroup each) in one launch.
+static void run_small_builds(const std::vector<SmallTask> &small, uint32_t bins, hipStream_t s) {
+  if (small.empty()) return;
+  DevBuf<SmallTask> dTasks;
+  dTasks.upload(small.data(), small.size(), s);
+  hipLaunchKernelGGL(k_small_build, dim3((uint32_t)small.size()), dim3(kBuildThreads), 0, s, dTasks.ptr,
+                     (int32_t)bins);
+  MTX_HIP(hipGetLastError());
+  MTX_HIP(hipStreamSynchronize(s));
+}
+
+// Node count once the small subtrees are built.
+static void finish_topology(Topology &T, hipStream_t s) {
+  if (T.primCount == 0) return;
   uint32_t nodeCount = 0;
-  counter.download(&nodeCount, 1, s);
+  T.counter.download(&nodeCount, 1, s);
   MTX_HIP(hipStreamSynchronize(s));
   T.nodeCount = nodeCount;
 }
@@ -922,62 +952,95 @@ static void layout_topology(Topology &T, Layout &L, hipStream_t s) {
   T.maxDepth = maxDepth;
 }
 
+// Builds the BLAS of every mesh: per mesh the primitive boxes and the wide
+// top levels, then all meshes' small subtrees in one launch, then per mesh
+// the depth-first layout and the leaf-ordered triangle records.
+std::vector<std::shared_ptr<Blas>> build_blases(const MeshInput *meshes, size_t count, uint32_t bins, hipStream_t s) {
+  struct Work {
+    std::shared_ptr<Blas> B;
+    DevBuf<float> hpos;
+    DevBuf<uint32_t> hidx;
+    const float *dpos = nullptr;
+    const uint32_t *didx = nullptr;
+  };
+  std::vector<Work> W(count);
+  std::vector<SmallTask> small;
+  for (size_t m = 0; m < count; m++) {
+    const MeshInput &in = meshes[m];
+    Work &w = W[m];
+    w.B = std::make_shared<Blas>();
+    Blas &B = *w.B;
+    B.triangleInput = in.tcount;
+    w.dpos = in.pos;
+    w.didx = in.idx;
+    if (!in.deviceInput) {
+      w.hpos.upload(in.pos, (size_t)in.vcount * 3, s);
+      w.hidx.upload(in.idx, (size_t)in.tcount * 3, s);
+      w.dpos = w.hpos.ptr;
+      w.didx = w.hidx.ptr;
+    }
+    Topology &T = B.topo;
+    uint32_t n = 0;
+    const uint32_t tcount = in.tcount;
+    if (tcount) {
+      DevBuf<float4> lo(tcount), hi(tcount);
+      DevBuf<uint32_t> valid(tcount), offs(tcount);
+      hipLaunchKernelGGL(k_tri_aabbs, dim3(blocks(tcount)), dim3(256), 0, s, w.dpos, w.didx, tcount, in.vcount,
+                         lo.ptr, hi.ptr, valid.ptr);
+      size_t tmpBytes = 0;
+      MTX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, valid.ptr, offs.ptr, tcount, s));
+      DevBuf<uint8_t> tmpScan(std::max<size_t>(1, tmpBytes));
+      MTX_HIP(hipcub::DeviceScan::ExclusiveSum(tmpScan.ptr, tmpBytes, valid.ptr, offs.ptr, tcount, s));
+      uint32_t lastOff = 0, lastValid = 0;
+      MTX_HIP(hipMemcpyAsync(&lastOff, offs.ptr + tcount - 1, 4, hipMemcpyDeviceToHost, s));
+      MTX_HIP(hipMemcpyAsync(&lastValid, valid.ptr + tcount - 1, 4, hipMemcpyDeviceToHost, s));
+      MTX_HIP(hipStreamSynchronize(s));
+      n = lastOff + lastValid;
+      T.primCount = n;
+      T.plo.alloc(std::max<uint32_t>(1, n));
+      T.phi.alloc(std::max<uint32_t>(1, n));
+      T.cen.alloc(std::max<uint32_t>(1, n));
+      T.source.alloc(std::max<uint32_t>(1, n));
+      hipLaunchKernelGGL(k_compact, dim3(blocks(tcount)), dim3(256), 0, s, lo.ptr, hi.ptr, valid.ptr, offs.ptr,
+                         tcount, T.plo.ptr, T.phi.ptr, T.cen.ptr, T.source.ptr);
+      MTX_HIP(hipGetLastError());
+    }
+    if (n) build_topology(T, bins, s, small);
+  }
+  run_small_builds(small, bins, s);
+  std::vector<std::shared_ptr<Blas>> out(count);
+  for (size_t m = 0; m < count; m++) {
+    Work &w = W[m];
+    Blas &B = *w.B;
+    Topology &T = B.topo;
+    out[m] = w.B;
+    const uint32_t n = T.primCount;
+    if (n == 0) continue;
+    finish_topology(T, s);
+    layout_topology(T, B.layout, s);
+    DevBuf<uint32_t> leafCount(n);
+    hipLaunchKernelGGL(k_leaf_counts, dim3(blocks(T.nodeCount)), dim3(256), 0, s, T.nodes.ptr, T.nodeCount,
+                       leafCount.ptr);
+    B.tris.alloc(n);
+    B.triSource.alloc(n);
+    hipLaunchKernelGGL(k_emit_tris, dim3(blocks(n)), dim3(256), 0, s, T.prim.ptr, T.source.ptr, leafCount.ptr, n,
+                       w.dpos, w.didx, B.tris.ptr, B.triSource.ptr);
+    MTX_HIP(hipGetLastError());
+    std::vector<BNode> hn = T.nodes.to_host(T.nodeCount, s);
+    for (auto &nd : hn) B.maxLeaf = std::max(B.maxLeaf, nd.primCount);
+    // The build scratch (centers, input boxes) is not needed on the hot path.
+    T.cen.release();
+    T.plo.release();
+    T.phi.release();
+    T.counter.release();
+  }
+  return out;
+}
+
 std::shared_ptr<Blas> build_blas(const float *pos, uint32_t vcount, const uint32_t *idx, uint32_t tcount,
                                  uint32_t bins, bool deviceInput, hipStream_t s) {
-  auto B = std::make_shared<Blas>();
-  B->triangleInput = tcount;
-  DevBuf<float> hpos;
-  DevBuf<uint32_t> hidx;
-  const float *dpos = pos;
-  const uint32_t *didx = idx;
-  if (!deviceInput) {
-    hpos.upload(pos, (size_t)vcount * 3, s);
-    hidx.upload(idx, (size_t)tcount * 3, s);
-    dpos = hpos.ptr;
-    didx = hidx.ptr;
-  }
-  Topology &T = B->topo;
-  uint32_t n = 0;
-  if (tcount) {
-    DevBuf<float4> lo(tcount), hi(tcount);
-    DevBuf<uint32_t> valid(tcount), offs(tcount);
-    hipLaunchKernelGGL(k_tri_aabbs, dim3(blocks(tcount)), dim3(256), 0, s, dpos, didx, tcount, vcount, lo.ptr,
-                       hi.ptr, valid.ptr);
-    size_t tmpBytes = 0;
-    MTX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, valid.ptr, offs.ptr, tcount, s));
-    DevBuf<uint8_t> tmpScan(std::max<size_t>(1, tmpBytes));
-    MTX_HIP(hipcub::DeviceScan::ExclusiveSum(tmpScan.ptr, tmpBytes, valid.ptr, offs.ptr, tcount, s));
-    uint32_t lastOff = 0, lastValid = 0;
-    MTX_HIP(hipMemcpyAsync(&lastOff, offs.ptr + tcount - 1, 4, hipMemcpyDeviceToHost, s));
-    MTX_HIP(hipMemcpyAsync(&lastValid, valid.ptr + tcount - 1, 4, hipMemcpyDeviceToHost, s));
-    MTX_HIP(hipStreamSynchronize(s));
-    n = lastOff + lastValid;
-    T.primCount = n;
-    T.plo.alloc(std::max<uint32_t>(1, n));
-    T.phi.alloc(std::max<uint32_t>(1, n));
-    T.cen.alloc(std::max<uint32_t>(1, n));
-    T.source.alloc(std::max<uint32_t>(1, n));
-    hipLaunchKernelGGL(k_compact, dim3(blocks(tcount)), dim3(256), 0, s, lo.ptr, hi.ptr, valid.ptr, offs.ptr, tcount,
-                       T.plo.ptr, T.phi.ptr, T.cen.ptr, T.source.ptr);
-    MTX_HIP(hipGetLastError());
-  }
-  if (n == 0) return B;
-  build_topology(T, bins, s);
-  layout_topology(T, B->layout, s);
-  DevBuf<uint32_t> leafCount(n);
-  hipLaunchKernelGGL(k_leaf_counts, dim3(blocks(T.nodeCount)), dim3(256), 0, s, T.nodes.ptr, T.nodeCount, leafCount.ptr);
-  B->tris.alloc(n);
-  B->triSource.alloc(n);
-  hipLaunchKernelGGL(k_emit_tris, dim3(blocks(n)), dim3(256), 0, s, T.prim.ptr, T.source.ptr, leafCount.ptr, n, dpos,
-                     didx, B->tris.ptr, B->triSource.ptr);
-  MTX_HIP(hipGetLastError());
-  std::vector<BNode> hn = T.nodes.to_host(T.nodeCount, s);
-  for (auto &nd : hn) B->maxLeaf = std::max(B->maxLeaf, nd.primCount);
-  // The build scratch (centers, input boxes) is not needed on the hot path.
-  T.cen.release();
-  T.plo.release();
-  T.phi.release();
-  return B;
+  const MeshInput in{pos, vcount, idx, tcount, deviceInput};
+  return build_blases(&in, 1, bins, s)[0];
 }
 
 size_t export_blas(const Blas &B, const uint32_t *indices, uint32_t *words, size_t capWords,
@@ -1067,7 +1130,10 @@ std::shared_ptr<Accel> build_accel(const std::vector<std::shared_ptr<Blas>> &bla
     T.source.alloc(count);
     hipLaunchKernelGGL(k_centers, dim3(blocks(count)), dim3(256), 0, s, T.plo.ptr, T.phi.ptr, count, T.cen.ptr,
                        T.source.ptr);
-    build_topology(T, bins, s);
+    std::vector<SmallTask> small;
+    build_topology(T, bins, s, small);
+    run_small_builds(small, bins, s);
+    finish_topology(T, s);
     layout_topology(T, A->layout, s);
   }
   A->maxDepth = T.maxDepth;

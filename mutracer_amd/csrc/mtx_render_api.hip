@@ -211,13 +211,18 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   MTX_REQUIRE(D.light_count == 0 || D.lights, MTX_INVALID_ARGUMENT, "lights is NULL");
   auto R = std::make_shared<RenderSceneObj>();
   const uint32_t bins = D.blas_bins ? D.blas_bins : 8u, tbins = D.tlas_bins ? D.tlas_bins : 64u;
+  std::vector<MeshInput> meshes;
   for (uint32_t i = 0; i < D.mesh_count; i++) {
     const mtx_mesh_desc &m = D.meshes[i];
     MTX_REQUIRE(m.struct_size >= sizeof(mtx_mesh_desc), MTX_INVALID_ARGUMENT, "mtx_mesh_desc.struct_size too small");
     MTX_REQUIRE(m.positions && m.normals && m.indices, MTX_INVALID_ARGUMENT, "mesh arrays are NULL");
     for (size_t k = 0; k < (size_t)m.triangle_count * 3; k++)
       MTX_REQUIRE(m.indices[k] < m.vertex_count, MTX_INVALID_ARGUMENT, "mesh index out of range");
-    R->blases.push_back(build_blas(m.positions, m.vertex_count, m.indices, m.triangle_count, bins, false, 0));
+    meshes.push_back(MeshInput{m.positions, m.vertex_count, m.indices, m.triangle_count, false});
+  }
+  R->blases = build_blases(meshes.data(), meshes.size(), bins, 0);
+  for (uint32_t i = 0; i < D.mesh_count; i++) {
+    const mtx_mesh_desc &m = D.meshes[i];
     R->normals.emplace_back();
     R->normals.back().upload_min1(m.normals, (size_t)m.vertex_count * 3, 0);
     R->indices.emplace_back();

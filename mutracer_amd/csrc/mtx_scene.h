@@ -89,6 +89,7 @@ MTX_HD void world_bounds(const double *m, const float lo[3], const float hi[3], 
 
 struct Topology {
   DevBuf<BNode> nodes;
+  DevBuf<uint32_t> counter;  // nodes allocated so far (device atomic during the build)
   DevBuf<uint32_t> prim;    // leaf-ordered compacted primitive ids
   DevBuf<uint32_t> source;  // compacted primitive -> input primitive (triangle / instance)
   DevBuf<float4> plo, phi, cen;
@@ -134,8 +135,16 @@ struct Accel {
   uint32_t max_stack() const { return maxDepth + blasMaxDepth + 2; }
 };
 
+struct MeshInput {
+  const float *pos;
+  uint32_t vcount;
+  const uint32_t *idx;
+  uint32_t tcount;
+  bool deviceInput;
+};
 std::shared_ptr<Blas> build_blas(const float *pos, uint32_t vcount, const uint32_t *idx, uint32_t tcount,
                                  uint32_t bins, bool deviceInput, hipStream_t s);
+std::vector<std::shared_ptr<Blas>> build_blases(const MeshInput *meshes, size_t count, uint32_t bins, hipStream_t s);
 size_t export_blas(const Blas &B, const uint32_t *indices, uint32_t *words, size_t capWords,
                    uint32_t *sourceOfRecord, hipStream_t s);
 std::shared_ptr<Accel> build_accel(const std::vector<std::shared_ptr<Blas>> &blases,
