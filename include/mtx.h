@@ -178,12 +178,30 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 #define MTX_MATERIAL_FLOATS 32
 
 /* Packed rect light (RectLight.ts:5-22): 16 floats. */
-#define MTX_LIGHT_TYPE 0              /* 0 = rect */
+/* Packed analytic light, the fields of the reference's Light record
+ * (rayTraceCore.wgsl.ts:228-248, packed as lightsData.ts:17-100). Types use
+ * the reference's constants (lightConstants.ts:10-15); 0 is read as a rect.
+ *   rect:        u, v edge vectors (normal = normalize(u x v)), spread in [0,1], area |u x v|
+ *   sphere:      radius (0 = point light, emission = intensity), area 4 pi r^2
+ *   spot:        u = unit direction, v = (cos outer, cos inner, 0), radius, area 4 pi r^2;
+ *                emission = color * intensity / (pi r^2) (r > 0) or color * intensity
+ *   directional: u = unit direction the light travels
+ *   disk:        u = x axis * radius, v = y axis * -radius (normal = normalize(u x v)), area pi r^2
+ *   cylinder:    u = axis * length, v = radial axis * radius, radius, area 2 pi r length
+ * Disk and cylinder emission is their radiance (intensity / area when normalized). */
+#define MTX_LIGHT_DIRECTIONAL 1
+#define MTX_LIGHT_SPHERE 2
+#define MTX_LIGHT_RECT 3
+#define MTX_LIGHT_SPOT 4
+#define MTX_LIGHT_DISK 5
+#define MTX_LIGHT_CYLINDER 6
+#define MTX_LIGHT_TYPE 0              /* one of MTX_LIGHT_* */
 #define MTX_LIGHT_POSITION 1          /* center, 3 floats */
-#define MTX_LIGHT_U 4                 /* edge vector, 3 floats */
-#define MTX_LIGHT_V 7                 /* edge vector, 3 floats; normal = normalize(u x v) */
-#define MTX_LIGHT_EMISSION 10         /* radiance = color * intensity, 3 floats */
-#define MTX_LIGHT_SPREAD 13
+#define MTX_LIGHT_U 4                 /* 3 floats, per type */
+#define MTX_LIGHT_V 7                 /* 3 floats, per type */
+#define MTX_LIGHT_EMISSION 10         /* 3 floats */
+#define MTX_LIGHT_SPREAD 13           /* rect spread */
+#define MTX_LIGHT_RADIUS 13           /* sphere / spot / cylinder radius */
 #define MTX_LIGHT_AREA 14
 #define MTX_LIGHT_FLOATS 16
 

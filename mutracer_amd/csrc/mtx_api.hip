@@ -4,6 +4,7 @@
 #include <map>
 #include <mutex>
 
+#include "mtx_fetch.h"
 #include "mtx_registry.h"
 
 namespace mtx {
@@ -19,34 +20,24 @@ uint64_t Registry::next() { return nextHandle.fetch_add(1); }
 // ------------------------------------------------------------- trace kernels
 constexpr int kTraceBlock = 128;
 
-__device__ __forceinline__ bool xcd_next(uint32_t &rb, uint32_t rayBlocks, uint32_t &end, bool first) {
-  // Blocks of one XCD (blockIdx % 8) walk one contiguous eighth of the ray
-  // blocks, so spatially coherent rays share that XCD's L2.
-  const uint32_t xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3, perXcd = (gridDim.x + 7u) >> 3;
-  const uint32_t span = (rayBlocks + 7u) >> 3;
-  if (first) {
-    rb = xcd * span + rank;
-    end = min(rayBlocks, (xcd + 1u) * span);
-  } else {
-    rb += perXcd;
-  }
-  return rb < end;
+__global__ void k_zero_counters(uint32_t *__restrict__ c) {
+  if (threadIdx.x < (uint32_t)kXcds) c[threadIdx.x] = 0u;
 }
 
+// Ray queries: the same traversal, occupancy targets and XCD-segmented work
+// distribution as the wavefront trace (closest hit) and shadow (any hit)
+// kernels; `ctr` holds the kXcds chunk counters, zeroed before the launch.
 template <int kMode>  // 0 closest, 1 any, 2 counters
-__global__ void __launch_bounds__(kTraceBlock)
+__global__ void __launch_bounds__(kTraceBlock, kMode == 1 ? kShadowWaves : kTraceWaves)
 k_trace(const AccelView *__restrict__ viewp, const mtx_ray *__restrict__ rays, uint32_t n,
-        mtx_hit *__restrict__ hits, uint32_t *__restrict__ out, uint32_t flags, int32_t *__restrict__ spill) {
-  __shared__ int32_t lds[kLdsStack * kTraceBlock];
+        mtx_hit *__restrict__ hits, uint32_t *__restrict__ out, uint32_t flags, int32_t *__restrict__ spill,
+        uint32_t *__restrict__ ctr) {
+  __shared__ int32_t lds[kMinLdsStack * kTraceBlock];
   const AccelView acc = *viewp;
-  LaneStack st{lds + threadIdx.x, kTraceBlock,
-               spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * (kMaxStack - kLdsStack)};
-  const uint32_t rayBlocks = (n + kTraceBlock - 1) / kTraceBlock;
+  LaneStackT<kMinLdsStack> st{lds + threadIdx.x, kTraceBlock,
+                              spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * (kMaxStack - kMinLdsStack)};
   const bool doubleSided = (flags & MTX_TRACE_DOUBLE_SIDED) != 0;
-  uint32_t rb, end;
-  for (bool ok = xcd_next(rb, rayBlocks, end, true); ok; ok = xcd_next(rb, rayBlocks, end, false)) {
-    uint32_t i = rb * kTraceBlock + threadIdx.x;
-    if (i >= n) continue;
+  auto ray = [&](uint32_t i) {
     const float4 *r4 = reinterpret_cast<const float4 *>(rays + i);
     float4 a = r4[0], b = r4[1];
     f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
@@ -55,7 +46,7 @@ k_trace(const AccelView *__restrict__ viewp, const mtx_ray *__restrict__ rays, u
     if (kMode == 1) {
       float tMax = fminf(fmaxf(a.w, 0.0f), kRayMaxDistance);
       out[i] = traverse<true, false>(acc, o, d, tMax, true, 1, -1, -1, st, h, nullptr) ? 1u : 0u;
-      continue;
+      return;
     }
     const bool counting = kMode == 2 || (flags & MTX_TRACE_COUNT);
     if (counting) traverse<false, true>(acc, o, d, kRayMaxDistance, doubleSided, 0, -1, -1, st, h, &c);
@@ -64,7 +55,7 @@ k_trace(const AccelView *__restrict__ viewp, const mtx_ray *__restrict__ rays, u
       out[(size_t)i * 3] = c.boxes;
       out[(size_t)i * 3 + 1] = c.tris;
       out[(size_t)i * 3 + 2] = c.instances;
-      continue;
+      return;
     }
     mtx_hit o4;
     o4.hit = h.hit ? 1u : 0u;
@@ -84,15 +75,27 @@ k_trace(const AccelView *__restrict__ viewp, const mtx_ray *__restrict__ rays, u
     }
     o4.visits = counting ? c.boxes + c.tris : 0u;
     hits[i] = o4;
-  }
+  };
+  for_each_ray<kMode == 1 ? kShadowFetch : kTraceFetch>(n, ctr, ray);
 }
 
-// Spill slices for the traversal stacks, one set per stream.
+// Spill slices for the traversal stacks and chunk counters of the ray
+// queries, one set per stream.
 struct SpillPool {
   std::mutex m;
   std::map<std::pair<int, hipStream_t>, std::shared_ptr<DevBuf<int32_t>>> bufs;
+  std::map<std::pair<int, hipStream_t>, std::shared_ptr<DevBuf<uint32_t>>> counters;
 };
 static SpillPool gSpill;
+
+static uint32_t *counters_for(hipStream_t s) {
+  int dev = 0;
+  MTX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(gSpill.m);
+  auto &b = gSpill.counters[{dev, s}];
+  if (!b) b = std::make_shared<DevBuf<uint32_t>>(kXcds);
+  return b->ptr;
+}
 
 uint32_t trace_grid() {
   static uint32_t grid = 0;
@@ -122,16 +125,29 @@ static void launch_trace(int mode, const Accel &A, const mtx_ray *rays, uint32_t
                          uint32_t flags, hipStream_t s) {
   MTX_REQUIRE(A.max_stack() <= (uint32_t)kMaxStack, MTX_INVALID_STATE, "scene deeper than the traversal stack");
   if (!n) return;
-  uint32_t grid = trace_grid();
+  // Persistent grid: the resident blocks of the kernel on every CU.
+  static uint32_t bpc[3] = {0, 0, 0};
+  const void *k = mode == 0 ? (const void *)k_trace<0> : mode == 1 ? (const void *)k_trace<1> : (const void *)k_trace<2>;
+  if (!bpc[mode]) {
+    int b = 0;
+    MTX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, kTraceBlock, 0));
+    bpc[mode] = (uint32_t)std::max(1, b);
+  }
+  uint32_t grid = trace_grid() / 8u * bpc[mode];
   uint32_t rayBlocks = (n + kTraceBlock - 1) / kTraceBlock;
   grid = std::min<uint32_t>(grid, ((rayBlocks + 7u) / 8u) * 8u);
   int32_t *spill = spill_for(s, grid, kTraceBlock);
+  uint32_t *ctr = counters_for(s);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, ctr);
   if (mode == 0)
-    hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+    hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill,
+                       ctr);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+    hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill,
+                       ctr);
   else
-    hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill);
+    hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, out, flags, spill,
+                       ctr);
   MTX_HIP(hipGetLastError());
 }
 

@@ -81,7 +81,7 @@ constexpr int kWfMaxEvents = 51;
 constexpr int kWfShadowBase = kWfMaxEvents + 1;
 constexpr int kWfFetchBase = kWfShadowBase + kWfMaxEvents + 1;
 constexpr int kWfXcdBase = kWfFetchBase + 2 * (kWfMaxEvents + 1);
-constexpr int kWfXcds = 8;
+constexpr int kWfXcds = 8;  // XCDs of an MI355X (counter segments per queue)
 constexpr int kWfTailFlag = kWfXcdBase + 2 * (kWfMaxEvents + 1) * kWfXcds;
 constexpr int kWfCounters = kWfTailFlag + 1;
 struct WfBuffers {

@@ -79,6 +79,31 @@ struct LightTables {
   double total = 0.0;
 };
 
+// estimateAnalyticLightPower (lightSampling.ts:72-107) from the packed light:
+// the emission holds color * intensity, or the radiance of disk, cylinder and
+// finite spot lights, so emitted luminance * pi * area covers the area lights;
+// point, spot (flux factor of its cone) and directional lights use 4 pi.
+static double analytic_power(const float *L) {
+  const double kPiD = 3.141592653589793;
+  const double lum = std::max((double)L[MTX_LIGHT_EMISSION] * 0.2126 + (double)L[MTX_LIGHT_EMISSION + 1] * 0.7152 +
+                                  (double)L[MTX_LIGHT_EMISSION + 2] * 0.0722, 0.0);
+  if (!(lum > 0)) return 0.0;
+  const int type = (int)L[MTX_LIGHT_TYPE];
+  const double area = (double)L[MTX_LIGHT_AREA], radius = (double)L[MTX_LIGHT_RADIUS];
+  switch (type) {
+    case 0:
+    case MTX_LIGHT_RECT:
+    case MTX_LIGHT_DISK:
+    case MTX_LIGHT_CYLINDER: return lum * kPiD * std::max(area, 0.0);
+    case MTX_LIGHT_SPHERE: return radius == 0.0 ? lum * 4 * kPiD : lum * kPiD * std::max(area, 0.0);
+    case MTX_LIGHT_SPOT: {
+      const double flux = 0.5 * (1 - 0.5 * ((double)L[MTX_LIGHT_V + 1] + (double)L[MTX_LIGHT_V]));
+      return radius == 0.0 ? lum * 4 * kPiD * flux : lum * kPiD * area * flux;
+    }
+    default: return lum * 4 * kPiD;  // directional
+  }
+}
+
 static LightTables light_tables(const float *lights, uint32_t nLights, const float *env, uint32_t W, uint32_t H) {
   struct In { int32_t type, index; double w; };
   std::vector<In> in;
@@ -99,12 +124,7 @@ static LightTables light_tables(const float *lights, uint32_t nLights, const flo
     envPower = envPower * (2 * kPiD * kPiD) / ((double)W * H);
   }
   push(0, 0, envPower);
-  for (uint32_t i = 0; i < nLights; i++) {
-    const float *L = lights + (size_t)i * MTX_LIGHT_FLOATS;
-    double lum = (double)L[MTX_LIGHT_EMISSION] * 0.2126 + (double)L[MTX_LIGHT_EMISSION + 1] * 0.7152 +
-                 (double)L[MTX_LIGHT_EMISSION + 2] * 0.0722;
-    if (lum > 0) push(1, (int32_t)i, lum * 3.141592653589793 * std::max((double)L[MTX_LIGHT_AREA], 0.0));
-  }
+  for (uint32_t i = 0; i < nLights; i++) push(1, (int32_t)i, analytic_power(lights + (size_t)i * MTX_LIGHT_FLOATS));
   LightTables T;
   T.lightPmf.assign(nLights, 0.0f);
   const size_t n = in.size();

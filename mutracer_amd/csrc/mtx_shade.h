@@ -716,6 +716,14 @@ __device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float 
   pdf = env_solid_pdf(b0.y, a0.y, v);
   return f3{-st * cp, ct, -st * sp};
 }
+// Analytic light types (lightConstants.ts:10-15); type 0 is read as a rect.
+enum : int { kLightDirectional = 1, kLightSphere = 2, kLightRect = 3, kLightSpot = 4, kLightDisk = 5, kLightCylinder = 6 };
+__device__ __forceinline__ int light_type(const float *__restrict__ L) {
+  const int t = (int)L[0];
+  return t == 0 ? kLightRect : t;
+}
+constexpr float kInv4Pi = KF(0.07957747154594766);
+
 __device__ void sample_rect(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
   f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
   f3 n = normalize(cross(U, V));
@@ -731,6 +739,119 @@ __device__ void sample_rect(const float *__restrict__ L, f3 p, float sx, float s
   if (ec <= 0.0f || ec < sc) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
   r.emission = f3{L[10], L[11], L[12]};
   r.pdf = d2 / mxf(L[14] * ec, 1e-8f);
+}
+// light_sample_sphere (sampleLight.wgsl.ts:10-89): a zero radius is a point
+// emitter (delta, emission = intensity / d^2); inside the sphere uniform
+// directions, outside cone sampling with the cancellation-free solid angle.
+__device__ void sample_sphere(const float *__restrict__ L, f3 p, float rx, float ry, LSample &r) {
+  const f3 pos{L[1], L[2], L[3]}, E{L[10], L[11], L[12]};
+  const f3 co = pos - p;
+  const float cd = len(co);
+  if (L[13] <= 0.0f) {
+    if (cd <= 1e-6f) {
+      r.direction = f3{0.0f, 0.0f, 1.0f};
+      r.dist = 0.0f;
+      r.pdf = 1.0f;
+      r.emission = splat(0.0f);
+      r.isDelta = true;
+      return;
+    }
+    r.direction = co / cd;
+    r.dist = cd;
+    r.pdf = 1.0f;
+    r.emission = E / mxf(cd * cd, 1e-8f);
+    r.isDelta = true;
+    return;
+  }
+  const float radius = mxf(L[13], 1e-6f);
+  r.isDelta = false;
+  f3 sd;
+  if (cd <= radius) {
+    const float z = 1.0f - 2.0f * rx;
+    const float radial = sqrtf(mxf(0.0f, 1.0f - z * z));
+    const float az = kTwoPi * ry;
+    sd = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
+    r.pdf = kInv4Pi;
+  } else {
+    const f3 axis = co / cd;
+    const float s2 = mnf(1.0f, radius * radius / (cd * cd));
+    const float cmax = sqrtf(mxf(0.0f, 1.0f - s2));
+    const float omc = s2 / (1.0f + cmax);
+    const float omct = rx * omc;
+    const float ct = 1.0f - omct;
+    const float st = sqrtf(mxf(0.0f, omct * (2.0f - omct)));
+    const float az = kTwoPi * ry;
+    const Basis B = make_basis(axis);
+    sd = axis * ct + B.t * (st * mtxfm::cosf(az)) + B.b * (st * mtxfm::sinf(az));
+    r.pdf = 1.0f / mxf(kTwoPi * omc, 1e-8f);
+  }
+  const float cp = dot(sd, co), c2 = dot(co, co);
+  const float disc = cp * cp - c2 + radius * radius;
+  const float nearD = cp - sqrtf(mxf(0.0f, disc)), farD = cp + sqrtf(mxf(0.0f, disc));
+  r.dist = nearD > 1e-6f ? nearD : farD;
+  r.direction = sd;
+  r.emission = E;
+}
+// evaluateSpotEmissionMultiplier (rayTraceCore.wgsl.ts:365-378), without IES
+// profiles: hard cone at cos(outer) = v.x, smoothstep to cos(inner) = v.y.
+__device__ float spot_multiplier(const float *__restrict__ L, f3 lightToSurface) {
+  const f3 axis = normalize(f3{L[4], L[5], L[6]});
+  const float cosT = dot(axis, lightToSurface);
+  float cone = cosT >= L[7] ? 1.0f : 0.0f;
+  if (L[8] > L[7] + 1e-6f) {
+    const float t = clampf((cosT - L[7]) / (L[8] - L[7]), 0.0f, 1.0f);
+    cone = t * t * (3.0f - 2.0f * t);
+  }
+  return cone;
+}
+// light_sample_disk / light_sample_cylinder (sampleLight.wgsl.ts:163-195):
+// area sampling, pdf = d^2 / (area cos).
+__device__ void sample_area_point(const float *__restrict__ L, f3 p, f3 surf, f3 normal, LSample &r) {
+  const f3 off = surf - p;
+  const float d2 = dot(off, off);
+  r.dist = sqrtf(d2);
+  r.direction = off / mxf(r.dist, 1e-8f);
+  const float ec = dot(-r.direction, normal);
+  if (ec <= 0.0f) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
+  r.emission = f3{L[10], L[11], L[12]};
+  r.pdf = d2 / mxf(L[14] * ec, 1e-8f);
+}
+__device__ void sample_disk(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
+  const f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
+  const float rr = sqrtf(sx), phi = kTwoPi * sy;
+  const f3 surf = pos + U * (rr * mtxfm::cosf(phi)) + V * (rr * mtxfm::sinf(phi));
+  sample_area_point(L, p, surf, normalize(cross(U, V)), r);
+}
+__device__ void sample_cylinder(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
+  const f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
+  const f3 axis = normalize(U), r0 = normalize(V), r1 = normalize(cross(axis, r0));
+  const float phi = kTwoPi * sy;
+  const f3 normal = r0 * mtxfm::cosf(phi) + r1 * mtxfm::sinf(phi);
+  const f3 surf = pos + U * (sx - 0.5f) + normal * L[13];
+  sample_area_point(L, p, surf, normal, r);
+}
+// sampleAnalyticLight (sampleLight.wgsl.ts:223-253).
+__device__ void sample_analytic(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
+  switch (light_type(L)) {
+    case kLightRect: sample_rect(L, p, sx, sy, r); break;
+    case kLightSphere: sample_sphere(L, p, sx, sy, r); break;
+    case kLightSpot: {
+      sample_sphere(L, p, sx, sy, r);
+      const f3 surf = p + r.direction * r.dist;
+      r.emission = r.emission * spot_multiplier(L, normalize(p - surf));
+      break;
+    }
+    case kLightDirectional:
+      r.direction = normalize(-f3{L[4], L[5], L[6]});
+      r.emission = f3{L[10], L[11], L[12]};
+      r.dist = kInf;
+      r.pdf = 1.0f;
+      r.isDelta = true;
+      break;
+    case kLightDisk: sample_disk(L, p, sx, sy, r); break;
+    case kLightCylinder: sample_cylinder(L, p, sx, sy, r); break;
+    default: break;
+  }
 }
 __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float rz, uint32_t cand, uint32_t count,
                              LSample &ls) {
@@ -754,35 +875,126 @@ __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float 
     ls.pdf *= S.envPmf;
   } else if (ch.type == 1) {
     ls.sourceType = 2;
-    sample_rect(S.lights + (size_t)ch.index * 16, p, rx, ry, ls);
+    sample_analytic(S.lights + (size_t)ch.index * 16, p, rx, ry, ls);
     ls.selPmf = S.lightPmf[ch.index];
     ls.pdf *= ls.selPmf;
   }
 }
-__device__ float intersect_rect_light(const float *__restrict__ L, f3 ro, f3 rd, LSample &ls) {
-  ls.pdf = 0.0f; ls.emission = splat(0.0f);
-  f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]};
-  f3 pos = f3{L[1], L[2], L[3]} - U * 0.5f - V * 0.5f;
-  f3 n = normalize(cross(U, V));
-  if (dot(n, rd) > 0.0f) return kInf;
-  float t = (dot(n, pos) - dot(n, ro)) / dot(rd, n);
-  float d = kInf;
-  if (t > kEps) {
-    f3 vi = (ro + rd * t) - pos;
-    float a1 = dot(U / dot(U, U), vi);
-    if (a1 >= 0.0f && a1 <= 1.0f) {
-      float a2 = dot(V / dot(V, V), vi);
-      if (a2 >= 0.0f && a2 <= 1.0f) d = t;
+// intersectSphere (intersect.wgsl.ts:924-945).
+__device__ float intersect_sphere(float rad, f3 pos, f3 ro, f3 rd) {
+  const f3 op = pos - ro;
+  const float b = dot(op, rd);
+  float det = b * b - dot(op, op) + rad * rad;
+  if (det < 0.0f) return kInf;
+  det = sqrtf(det);
+  const float t1 = b - det;
+  if (t1 > 0.001f) return t1;
+  const float t2 = b + det;
+  if (t2 > 0.001f) return t2;
+  return kInf;
+}
+// sphereSolidAnglePdf (intersect.wgsl.ts:966-974).
+__device__ float sphere_solid_angle_pdf(f3 origin, f3 center, float radius) {
+  const float cd = len(origin - center);
+  if (cd <= radius) return kInv4Pi;
+  const float s2 = mnf(1.0f, radius * radius / (cd * cd));
+  const float cmax = sqrtf(mxf(0.0f, 1.0f - s2));
+  return 1.0f / mxf(kTwoPi * (s2 / (1.0f + cmax)), 1e-8f);
+}
+// intersectLight (intersect.wgsl.ts:976-1122): distance to the analytic light
+// along a continuation ray (kInf when missed or not hittable) with the
+// light's emission and the conditional density NEE would have used.
+__device__ float intersect_light(const float *__restrict__ L, f3 ro, f3 rd, LSample &ls) {
+  ls.pdf = 0.0f;
+  ls.emission = splat(0.0f);
+  ls.isDelta = false;
+  const f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, P{L[1], L[2], L[3]}, E{L[10], L[11], L[12]};
+  const int type = light_type(L);
+  if (type == kLightRect) {
+    f3 pos = P - U * 0.5f - V * 0.5f;
+    f3 n = normalize(cross(U, V));
+    if (dot(n, rd) > 0.0f) return kInf;
+    float t = (dot(n, pos) - dot(n, ro)) / dot(rd, n);
+    float d = kInf;
+    if (t > kEps) {
+      f3 vi = (ro + rd * t) - pos;
+      float a1 = dot(U / dot(U, U), vi);
+      if (a1 >= 0.0f && a1 <= 1.0f) {
+        float a2 = dot(V / dot(V, V), vi);
+        if (a2 >= 0.0f && a2 <= 1.0f) d = t;
+      }
     }
+    if (d == kInf) return kInf;
+    float cosT = dot(-rd, n);
+    if (cosT >= mtxfm::cosf(L[13] * kPi / 2.0f)) {
+      ls.pdf = (d * d) / (L[14] * cosT);
+      ls.emission = E;
+      return d;
+    }
+    return kInf;
   }
-  if (d == kInf) return kInf;
-  float cosT = dot(-rd, n);
-  if (cosT >= mtxfm::cosf(L[13] * kPi / 2.0f)) {
-    ls.pdf = (d * d) / (L[14] * cosT);
-    ls.emission = f3{L[10], L[11], L[12]};
+  if (type == kLightDisk) {
+    const f3 n = normalize(cross(U, V));
+    if (dot(n, rd) >= 0.0f) return kInf;
+    const float d = dot(P - ro, n) / dot(n, rd);
+    const f3 hit = ro + rd * d - P;
+    const float radius = mxf(len(U), 1e-8f);
+    if (d <= kEps || dot(hit, hit) > radius * radius) return kInf;
+    const float cosine = dot(-rd, n);
+    ls.pdf = d * d / mxf(L[14] * cosine, 1e-8f);
+    ls.emission = E;
     return d;
   }
-  return kInf;
+  if (type == kLightCylinder) {
+    const f3 axis = normalize(U);
+    const float halfLength = 0.5f * len(U);
+    const f3 oc = ro - P;
+    const f3 rdp = rd - axis * dot(rd, axis), od = oc - axis * dot(oc, axis);
+    const float a = dot(rdp, rdp), b = dot(od, rdp), c = dot(od, od) - L[13] * L[13];
+    const float disc = b * b - a * c;
+    if (a <= 1e-12f || disc < 0.0f) return kInf;
+    const float root = sqrtf(disc);
+    const float nearD = (-b - root) / a, farD = (-b + root) / a;
+    const float d = nearD > kEps ? nearD : farD;
+    const float axial = dot(oc + rd * d, axis);
+    if (d <= kEps || fabsf(axial) > halfLength) return kInf;
+    const f3 hit = ro + rd * d;
+    const f3 n = normalize(hit - P - axis * axial);
+    const float cosine = dot(-rd, n);
+    if (cosine <= 0.0f) return kInf;
+    ls.pdf = d * d / mxf(L[14] * cosine, 1e-8f);
+    ls.emission = E;
+    return d;
+  }
+  if (type == kLightSphere || type == kLightSpot) {
+    float d;
+    if (L[13] <= 0.0f) {
+      const f3 toPoint = P - ro;
+      const float proj = dot(toPoint, rd);
+      const f3 perp = toPoint - proj * rd;
+      if (proj <= kEps || dot(perp, perp) > 1e-8f) return kInf;
+      d = proj;
+      ls.pdf = 1.0f;
+      ls.isDelta = true;
+    } else {
+      d = intersect_sphere(L[13], P, ro, rd);
+      if (type == kLightSphere && d > 0.0f && d != kInf) ls.pdf = sphere_solid_angle_pdf(ro, P, L[13]);
+    }
+    if (type == kLightSphere) {
+      if (d < 0.0f) return kInf;
+      ls.emission = L[13] <= 0.0f ? E / mxf(d * d, 1e-8f) : E;
+      return d;
+    }
+    if (d != kInf) {
+      const f3 hitPt = ro + d * rd;
+      const float m = spot_multiplier(L, normalize(ro - hitPt));
+      if (m <= 0.0f) return kInf;
+      if (L[13] > 0.0f) ls.pdf = sphere_solid_angle_pdf(ro, P, L[13]);
+      ls.emission = E * m * (L[13] <= 0.0f ? 1.0f / mxf(d * d, 1e-8f) : 1.0f);
+    }
+    return d;
+  }
+  return kInf;  // directional lights are not hit by rays
 }
 __device__ __forceinline__ f3 offset_surface_ray(f3 p, f3 gn, f3 dir) {
   f3 n = dot(gn, dir) >= 0.0f ? gn : -gn;
@@ -990,7 +1202,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
   } else {
     for (uint32_t i = 0; i < S.lightCount; i++) {
       LSample ls;
-      float d = intersect_rect_light(S.lights + (size_t)i * 16, pr.ro, pr.rd, ls);
+      float d = intersect_light(S.lights + (size_t)i * 16, pr.ro, pr.rd, ls);
       ls.pdf *= S.lightPmf[i];
       bool occluded = si.hit && d > len(si.position - pr.ro);
       if (d != kInf && !occluded) {

@@ -68,13 +68,147 @@ class RectLight:
 
     def pack(self):
         p = np.zeros(16, np.float32)
-        p[0] = 0.0
+        p[0] = LIGHT_RECT
         p[1:4] = self.position
         p[4:7] = self.u
         p[7:10] = self.v
         p[10:13] = np.asarray(self.color, np.float64) * self.intensity
         p[13] = self.spread
         p[14] = np.linalg.norm(np.cross(self.u, self.v))
+        return p
+
+
+# Analytic light types (lightConstants.ts:10-15).
+LIGHT_DIRECTIONAL, LIGHT_SPHERE, LIGHT_RECT, LIGHT_SPOT, LIGHT_DISK, LIGHT_CYLINDER = 1, 2, 3, 4, 5, 6
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def _frame(direction):
+    """Unit x, y axes perpendicular to `direction` with x cross y = -direction
+    (the reference's rigid light frame, lightsData.ts:103-116, with z = -direction)."""
+    z = -_unit(direction)
+    x = np.array([z[2], 0.0, -z[0]]) if abs(z[1]) < 0.99 else np.array([1.0, 0.0, 0.0])
+    x = _unit(x - z * np.dot(x, z))
+    return x, np.cross(z, x)
+
+
+@dataclass
+class SphereLight:
+    """Sphere light of `radius` (0: point light with intensity in candela);
+    emission = color * intensity (SphereLight, lightsData.ts:57-72)."""
+    position: tuple
+    radius: float = 0.0
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+    def pack(self):
+        p = np.zeros(16, np.float32)
+        p[0] = LIGHT_SPHERE
+        p[1:4] = self.position
+        p[10:13] = np.asarray(self.color, np.float64) * self.intensity
+        p[13] = self.radius
+        p[14] = 4 * math.pi * self.radius ** 2
+        return p
+
+
+@dataclass
+class SpotLight:
+    """Spot light toward `direction` with a smooth cone between the inner and
+    outer half-angles (degrees); radius 0 is a point emitter (lightsData.ts
+    :80-92, getSpotLightRadiance :145-149)."""
+    position: tuple
+    direction: tuple
+    radius: float = 0.0
+    inner_angle: float = 30.0
+    outer_angle: float = 45.0
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+    def pack(self):
+        p = np.zeros(16, np.float32)
+        p[0] = LIGHT_SPOT
+        p[1:4] = self.position
+        p[4:7] = _unit(self.direction)
+        p[7] = math.cos(math.radians(self.outer_angle))
+        p[8] = math.cos(math.radians(min(self.inner_angle, self.outer_angle)))
+        radiance = self.intensity if self.radius == 0 else self.intensity / (math.pi * self.radius ** 2)
+        p[10:13] = np.asarray(self.color, np.float64) * radiance
+        p[13] = self.radius
+        p[14] = 4 * math.pi * self.radius ** 2
+        return p
+
+
+@dataclass
+class DirectionalLight:
+    """Directional light travelling along `direction`, irradiance = intensity."""
+    direction: tuple
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+    def pack(self):
+        p = np.zeros(16, np.float32)
+        p[0] = LIGHT_DIRECTIONAL
+        p[4:7] = _unit(self.direction)
+        p[10:13] = np.asarray(self.color, np.float64) * self.intensity
+        return p
+
+
+@dataclass
+class DiskLight:
+    """One-sided disk light emitting toward `direction` (lightsData.ts:73-77);
+    with `normalize` the intensity is spread over the area."""
+    position: tuple
+    direction: tuple
+    radius: float = 0.5
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+    normalize: bool = False
+
+    def pack(self):
+        x, y = _frame(self.direction)
+        area = math.pi * self.radius ** 2
+        p = np.zeros(16, np.float32)
+        p[0] = LIGHT_DISK
+        p[1:4] = self.position
+        p[4:7] = x * self.radius
+        p[7:10] = y * -self.radius
+        p[10:13] = np.asarray(self.color, np.float64) * (self.intensity / max(area, 1e-8) if self.normalize
+                                                        else self.intensity)
+        p[13] = self.radius
+        p[14] = area
+        return p
+
+
+@dataclass
+class CylinderLight:
+    """Cylinder light along `axis` (length) with a `radial` reference
+    direction (lightsData.ts:78-82); emits outward from its side."""
+    position: tuple
+    axis: tuple
+    radial: tuple
+    radius: float = 0.5
+    length: float = 1.0
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+    normalize: bool = False
+
+    def pack(self):
+        a = _unit(self.axis)
+        r = _unit(np.asarray(self.radial, np.float64) - a * np.dot(self.radial, a))
+        area = 2 * math.pi * self.radius * self.length
+        p = np.zeros(16, np.float32)
+        p[0] = LIGHT_CYLINDER
+        p[1:4] = self.position
+        p[4:7] = a * self.length
+        p[7:10] = r * self.radius
+        p[10:13] = np.asarray(self.color, np.float64) * (self.intensity / max(area, 1e-8) if self.normalize
+                                                        else self.intensity)
+        p[13] = self.radius
+        p[14] = area
         return p
 
 

@@ -622,11 +622,28 @@ void build_tables(const RenderScene &s, HostTables &T) {
     envPower = acc * (2 * 3.141592653589793 * 3.141592653589793) / ((double)s.envW * s.envH);
   }
   push(0, 0, envPower);
-  for (uint32_t i = 0; i < s.lightCount; i++) {  // lightSampling.ts:72-77 (RectLight)
+  for (uint32_t i = 0; i < s.lightCount; i++) {  // estimateAnalyticLightPower, lightSampling.ts:72-107
     const float *L = s.lights + (size_t)i * 16;
     double lum = (double)L[L_EMISSION] * 0.2126 + (double)L[L_EMISSION + 1] * 0.7152 + (double)L[L_EMISSION + 2] * 0.0722;
     lum = lum > 0 ? lum : 0;
-    push(1, (int32_t)i, lum > 0 ? lum * 3.141592653589793 * std::max((double)L[L_AREA], 0.0) : 0.0);
+    const double pi = 3.141592653589793, area = (double)L[L_AREA], radius = (double)L[L_RADIUS];
+    int type = (int)L[L_TYPE];
+    if (type == 0) type = LT_RECT;
+    double w = 0.0;
+    if (lum > 0) {
+      if (type == LT_RECT || type == LT_DISK || type == LT_CYLINDER) {
+        w = lum * pi * std::max(area, 0.0);
+      } else if (type == LT_SPHERE) {
+        w = radius == 0.0 ? lum * 4 * pi : lum * pi * std::max(area, 0.0);
+      } else if (type == LT_SPOT) {
+        // getSpotLightShapingFluxFactor (lightsData.ts:150-154): v = (cos outer, cos inner)
+        double flux = 0.5 * (1 - 0.5 * ((double)L[L_V + 1] + (double)L[L_V]));
+        w = radius == 0.0 ? lum * 4 * pi * flux : lum * pi * area * flux;
+      } else {
+        w = lum * 4 * pi;  // directional
+      }
+    }
+    push(1, (int32_t)i, w);
   }
   size_t n = weights.size();
   T.lightPmf.assign(s.lightCount, 0.0f);
@@ -799,6 +816,125 @@ static void light_sample_rect(const float *L, V3 p, float sx, float sy, LightSam
   r.pdf = d2 / mxf(L[L_AREA] * ec, 1e-8f);
 }
 
+static int light_type_of(const float *L) {
+  int t = (int)L[L_TYPE];
+  return t == 0 ? LT_RECT : t;
+}
+static const float INV4PI_F = (float)0.07957747154594766;
+
+// light_sample_sphere (sampleLight.wgsl.ts:10-89).
+static void light_sample_sphere(const float *L, V3 p, float rx, float ry, LightSample &r) {
+  V3 pos(L[L_POS], L[L_POS + 1], L[L_POS + 2]), E(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+  V3 co = pos - p;
+  float cd = length(co);
+  if (L[L_RADIUS] <= 0.0f) {
+    if (cd <= 1e-6f) {
+      r.direction = V3(0.0f, 0.0f, 1.0f); r.normal = V3(0.0f, 0.0f, -1.0f); r.dist = 0.0f; r.pdf = 1.0f;
+      r.emission = V3(0.0f); r.isDelta = true;
+      return;
+    }
+    r.direction = co / cd; r.normal = -r.direction; r.dist = cd; r.pdf = 1.0f;
+    r.emission = E / mxf(cd * cd, 1e-8f);
+    r.isDelta = true;
+    return;
+  }
+  float radius = mxf(L[L_RADIUS], 1e-6f);
+  r.isDelta = false;
+  V3 sd;
+  if (cd <= radius) {
+    float z = 1.0f - 2.0f * rx;
+    float radial = std::sqrt(mxf(0.0f, 1.0f - z * z));
+    float az = TWOPI * ry;
+    sd = V3(radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z);
+    r.pdf = INV4PI_F;
+  } else {
+    V3 axis = co / cd;
+    float s2 = mnf(1.0f, radius * radius / (cd * cd));
+    float cmax = std::sqrt(mxf(0.0f, 1.0f - s2));
+    float omc = s2 / (1.0f + cmax);
+    float omct = rx * omc;
+    float ct = 1.0f - omct;
+    float st = std::sqrt(mxf(0.0f, omct * (2.0f - omct)));
+    float az = TWOPI * ry;
+    V3 t, b;
+    orthonormal_basis(axis, t, b);
+    sd = axis * ct + t * (st * mtxfm::cosf(az)) + b * (st * mtxfm::sinf(az));
+    r.pdf = 1.0f / mxf(TWOPI * omc, 1e-8f);
+  }
+  float cproj = dot(sd, co), c2 = dot(co, co);
+  float disc = cproj * cproj - c2 + radius * radius;
+  float nearD = cproj - std::sqrt(mxf(0.0f, disc)), farD = cproj + std::sqrt(mxf(0.0f, disc));
+  r.dist = nearD > 1e-6f ? nearD : farD;
+  r.direction = sd;
+  r.normal = normalize((p + sd * r.dist) - pos);
+  r.emission = E;
+}
+
+// evaluateSpotEmissionMultiplier (rayTraceCore.wgsl.ts:365-378), no IES profile.
+static float spot_emission_multiplier(const float *L, V3 lightToSurface) {
+  V3 axis = normalize(V3(L[L_U], L[L_U + 1], L[L_U + 2]));
+  float cosTheta = dot(axis, lightToSurface);
+  float lo = L[L_V], hi = L[L_V + 1];
+  float cone = cosTheta >= lo ? 1.0f : 0.0f;
+  if (hi > lo + 1e-6f) {  // WGSL smoothstep
+    float t = clampf((cosTheta - lo) / (hi - lo), 0.0f, 1.0f);
+    cone = t * t * (3.0f - 2.0f * t);
+  }
+  return cone;
+}
+
+// Area sample of a disk or cylinder point (sampleLight.wgsl.ts:163-195).
+static void area_light_point(const float *L, V3 p, V3 surfacePoint, V3 normal, LightSample &r) {
+  V3 offset = surfacePoint - p;
+  float d2 = dot(offset, offset);
+  r.dist = std::sqrt(d2);
+  r.direction = offset / mxf(r.dist, 1e-8f);
+  r.normal = normal;
+  float emitterCosine = dot(-r.direction, normal);
+  if (emitterCosine <= 0.0f) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
+  r.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+  r.pdf = d2 / mxf(L[L_AREA] * emitterCosine, 1e-8f);
+}
+
+// sampleAnalyticLight (sampleLight.wgsl.ts:223-253).
+static void light_sample_analytic(const float *L, V3 p, float sx, float sy, LightSample &r) {
+  V3 U(L[L_U], L[L_U + 1], L[L_U + 2]), Vv(L[L_V], L[L_V + 1], L[L_V + 2]);
+  V3 pos(L[L_POS], L[L_POS + 1], L[L_POS + 2]);
+  switch (light_type_of(L)) {
+    case LT_RECT: light_sample_rect(L, p, sx, sy, r); break;
+    case LT_SPHERE: light_sample_sphere(L, p, sx, sy, r); break;
+    case LT_SPOT: {
+      light_sample_sphere(L, p, sx, sy, r);
+      V3 surfacePoint = p + r.direction * r.dist;
+      r.emission = r.emission * spot_emission_multiplier(L, normalize(p - surfacePoint));
+      break;
+    }
+    case LT_DIRECTIONAL:
+      r.direction = normalize(-U);
+      r.normal = -r.direction;
+      r.emission = V3(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+      r.dist = INF;
+      r.pdf = 1.0f;
+      r.isDelta = true;
+      break;
+    case LT_DISK: {
+      float rr = std::sqrt(sx), phi = TWOPI * sy;
+      V3 surfacePoint = pos + U * (rr * mtxfm::cosf(phi)) + Vv * (rr * mtxfm::sinf(phi));
+      area_light_point(L, p, surfacePoint, normalize(cross(U, Vv)), r);
+      break;
+    }
+    case LT_CYLINDER: {
+      V3 axis = normalize(U), radial0 = normalize(Vv), radial1 = normalize(cross(axis, radial0));
+      float phi = TWOPI * sy;
+      V3 normal = radial0 * mtxfm::cosf(phi) + radial1 * mtxfm::sinf(phi);
+      V3 surfacePoint = pos + U * (sx - 0.5f) + normal * L[L_RADIUS];
+      area_light_point(L, p, surfacePoint, normal, r);
+      break;
+    }
+    default: break;
+  }
+}
+
 // sampleLightChoiceTable (sampleLight.wgsl.ts:365-420) with the candidate's
 // stratified selection number (:298-308).
 static void sample_light_candidate(const Ctx &c, V3 p, float rx, float ry, float rz, uint32_t cand,
@@ -822,7 +958,7 @@ static void sample_light_candidate(const Ctx &c, V3 p, float rx, float ry, float
     ls.pdf *= c.T.envPmf;
   } else if (c.T.type[sel] == 1) {
     ls.sourceType = 2;
-    light_sample_rect(c.s.lights + (size_t)c.T.index[sel] * 16, p, rx, ry, ls);
+    light_sample_analytic(c.s.lights + (size_t)c.T.index[sel] * 16, p, rx, ry, ls);
     ls.selectionPmf = c.T.lightPmf[c.T.index[sel]];
     ls.pdf *= ls.selectionPmf;
   }
@@ -856,6 +992,114 @@ static float intersect_rect_light(const float *L, V3 ro, V3 rd, LightSample &ls)
     return d;
   }
   return INF;
+}
+
+// intersectSphere (intersect.wgsl.ts:924-945).
+static float intersect_sphere(float rad, V3 pos, V3 ro, V3 rd) {
+  V3 op = pos - ro;
+  const float eps = 0.001f;
+  float b = dot(op, rd);
+  float det = b * b - dot(op, op) + rad * rad;
+  if (det < 0.0f) return INF;
+  det = std::sqrt(det);
+  float t1 = b - det;
+  if (t1 > eps) return t1;
+  float t2 = b + det;
+  if (t2 > eps) return t2;
+  return INF;
+}
+
+// sphereSolidAnglePdf (intersect.wgsl.ts:966-974).
+static float sphere_solid_angle_pdf(V3 origin, V3 center, float radius) {
+  float centerDistance = length(origin - center);
+  if (centerDistance <= radius) return INV4PI_F;
+  float s2 = mnf(1.0f, radius * radius / (centerDistance * centerDistance));
+  float cmax = std::sqrt(mxf(0.0f, 1.0f - s2));
+  float omc = s2 / (1.0f + cmax);
+  return 1.0f / mxf(TWOPI * omc, 1e-8f);
+}
+
+// intersectLight (intersect.wgsl.ts:976-1122) for every analytic type.
+static float intersect_light(const float *L, V3 ro, V3 rd, LightSample &ls) {
+  const int type = light_type_of(L);
+  if (type == LT_RECT) return intersect_rect_light(L, ro, rd, ls);
+  ls.direction = rd; ls.normal = -rd; ls.dist = INF; ls.pdf = 0.0f; ls.emission = V3(0.0f); ls.isDelta = false;
+  V3 U(L[L_U], L[L_U + 1], L[L_U + 2]), Vv(L[L_V], L[L_V + 1], L[L_V + 2]);
+  V3 pos(L[L_POS], L[L_POS + 1], L[L_POS + 2]), E(L[L_EMISSION], L[L_EMISSION + 1], L[L_EMISSION + 2]);
+  const float radiusOrSpread = L[L_RADIUS];
+  if (type == LT_DISK) {
+    V3 normal = normalize(cross(U, Vv));
+    if (dot(normal, rd) >= 0.0f) return INF;
+    float d = dot(pos - ro, normal) / dot(normal, rd);
+    V3 hit = ro + rd * d - pos;
+    float radius = mxf(length(U), 1e-8f);
+    if (d <= EPS || dot(hit, hit) > radius * radius) return INF;
+    float cosine = dot(-rd, normal);
+    ls.direction = rd; ls.normal = normal; ls.dist = d;
+    ls.pdf = d * d / mxf(L[L_AREA] * cosine, 1e-8f);
+    ls.emission = E;
+    return d;
+  }
+  if (type == LT_CYLINDER) {
+    V3 axis = normalize(U);
+    float halfLength = 0.5f * length(U);
+    V3 oc = ro - pos;
+    V3 rdp = rd - axis * dot(rd, axis), od = oc - axis * dot(oc, axis);
+    float a = dot(rdp, rdp), b = dot(od, rdp), cc = dot(od, od) - radiusOrSpread * radiusOrSpread;
+    float discriminant = b * b - a * cc;
+    if (a <= 1e-12f || discriminant < 0.0f) return INF;
+    float root = std::sqrt(discriminant);
+    float nearD = (-b - root) / a, farD = (-b + root) / a;
+    float d = nearD > EPS ? nearD : farD;
+    float axial = dot(oc + rd * d, axis);
+    if (d <= EPS || fabsf(axial) > halfLength) return INF;
+    V3 hit = ro + rd * d;
+    V3 normal = normalize(hit - pos - axis * axial);
+    float cosine = dot(-rd, normal);
+    if (cosine <= 0.0f) return INF;
+    ls.direction = rd; ls.normal = normal; ls.dist = d;
+    ls.pdf = d * d / mxf(L[L_AREA] * cosine, 1e-8f);
+    ls.emission = E;
+    return d;
+  }
+  if (type == LT_SPHERE || type == LT_SPOT) {
+    float d;
+    if (radiusOrSpread <= 0.0f) {
+      V3 toPoint = pos - ro;
+      float projection = dot(toPoint, rd);
+      V3 perpendicular = toPoint - projection * rd;
+      if (projection <= EPS || dot(perpendicular, perpendicular) > 1e-8f) return INF;
+      d = projection;
+      ls.direction = rd; ls.normal = -rd; ls.dist = d; ls.pdf = 1.0f; ls.isDelta = true;
+    } else {
+      d = intersect_sphere(radiusOrSpread, pos, ro, rd);
+      ls.isDelta = false;
+      if (type == LT_SPHERE && d > 0.0f && d != INF) {
+        ls.direction = rd; ls.dist = d;
+        ls.normal = normalize(ro + rd * d - pos);
+        ls.pdf = sphere_solid_angle_pdf(ro, pos, radiusOrSpread);
+      }
+    }
+    if (type == LT_SPHERE) {
+      if (d < 0.0f) return INF;
+      ls.direction = rd; ls.dist = d;
+      ls.emission = radiusOrSpread <= 0.0f ? E / mxf(d * d, 1e-8f) : E;
+      return d;
+    }
+    if (d != INF) {
+      V3 hitPt = ro + d * rd;
+      float multiplier = spot_emission_multiplier(L, normalize(ro - hitPt));
+      if (multiplier <= 0.0f) return INF;
+      if (radiusOrSpread > 0.0f) {
+        ls.direction = rd; ls.dist = d;
+        ls.normal = normalize(ro + rd * d - pos);
+        ls.pdf = sphere_solid_angle_pdf(ro, pos, radiusOrSpread);
+      }
+      ls.emission = E * multiplier * (radiusOrSpread <= 0.0f ? 1.0f / mxf(d * d, 1e-8f) : 1.0f);
+    }
+    return d;
+  }
+  return INF;  // directional lights are not intersected
 }
 
 // offsetSurfaceRay (common.wgsl.ts:14-19).
@@ -1217,7 +1461,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
       for (uint32_t i = 0; i < s.lightCount; i++) {
         LightSample ls{};
         const float *L = s.lights + (size_t)i * 16;
-        float d = intersect_rect_light(L, path.ro, path.rd, ls);
+        float d = intersect_light(L, path.ro, path.rd, ls);
         ls.pdf *= c.T.lightPmf[i];
         V3 dd = si.position - path.ro;
         bool occluded = si.hit && d > length(dd);

@@ -22,8 +22,11 @@ enum MatField {
   M_COAT_ROUGHNESS = 20, M_COAT_IOR = 21, M_COAT_DARKENING = 22, M_EMISSION = 23,
   M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27
 };
-// Packed light (16 floats): type(0 rect), position, u, v, emission, spread, area.
-enum LightField { L_TYPE = 0, L_POS = 1, L_U = 4, L_V = 7, L_EMISSION = 10, L_SPREAD = 13, L_AREA = 14 };
+// Packed light (16 floats): type, position, u, v, emission, spread or radius,
+// area (the reference's Light record, rayTraceCore.wgsl.ts:228-248). Types are
+// lightConstants.ts:10-15; 0 is read as a rect.
+enum LightField { L_TYPE = 0, L_POS = 1, L_U = 4, L_V = 7, L_EMISSION = 10, L_SPREAD = 13, L_RADIUS = 13, L_AREA = 14 };
+enum LightType { LT_DIRECTIONAL = 1, LT_SPHERE = 2, LT_RECT = 3, LT_SPOT = 4, LT_DISK = 5, LT_CYLINDER = 6 };
 
 struct RenderScene {
   const TraceScene *trace;

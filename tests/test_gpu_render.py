@@ -130,6 +130,31 @@ def test_thin_lens(env, orc, kernel):
     _check(sums, counts, ref, 6)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_all_light_types(env, orc, kernel):
+    """Rect, sphere, point, spot (point and finite), directional, disk and
+    cylinder lights together with the environment: NEE candidates pick
+    among all of them, BSDF continuations hit the hittable ones."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=6, freq=6, env_size=(64, 32))
+    sc.lights = [
+        scene.RectLight(position=(2.0, 4.0, 0.0), u=(1.0, 0.0, 0.0), v=(0.0, 0.0, 1.0), intensity=6.0),
+        scene.SphereLight(position=(-3.0, 3.0, 1.0), radius=0.5, color=(1.0, 0.8, 0.6), intensity=4.0),
+        scene.SphereLight(position=(0.0, 5.0, -2.0), radius=0.0, intensity=20.0),
+        scene.SpotLight(position=(4.0, 4.0, 4.0), direction=(-1.0, -1.0, -1.0), inner_angle=15, outer_angle=30,
+                        intensity=40.0),
+        scene.SpotLight(position=(-4.0, 3.0, -4.0), direction=(1.0, -1.0, 1.0), radius=0.3, inner_angle=25,
+                        outer_angle=40, intensity=30.0),
+        scene.DirectionalLight(direction=(0.3, -1.0, 0.2), color=(1.0, 0.9, 0.8), intensity=1.5),
+        scene.DiskLight(position=(0.0, 3.5, 3.0), direction=(0.0, -1.0, -0.3), radius=0.6, intensity=5.0),
+        scene.CylinderLight(position=(-2.0, 2.5, -3.0), axis=(1.0, 0.0, 0.2), radial=(0.0, 1.0, 0.0), radius=0.2,
+                            length=2.0, intensity=6.0, normalize=False),
+    ]
+    sums, counts, ref, _ = _both(orc, render, sc, 64, 40, 6, bounces=6, max_samples=8, kernel=kernel,
+                                 direct_light_candidates=4)
+    _check(sums, counts, ref, 6)
+
+
 def _resolve_expected(sums):
     """The output pass of WGPURayTraceOutput.ts:32-92 on the sums (bottom row
     first), evaluated per pixel in Python floats."""
