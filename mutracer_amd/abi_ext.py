@@ -93,6 +93,8 @@ def declare(L):
         "mtx_denoiser_run": (i32, [u64, u32, u32, vp, vp, vp, vp, sz, C.c_char_p, sz]),
         "mtx_denoiser_forward": (i32, [u64, u32, u32, vp, vp, vp, C.c_char_p, sz]),
         "mtx_denoiser_benchmark": (i32, [u64, u32, u32, u32, P(C.c_double)]),
+        "mtx_denoiser_set_tiling": (i32, [u64, u32, u32, u32]),
+        "mtx_denoiser_get_tiling": (i32, [u64, u32, u32, u32, P(u32), P(u32)]),
         "mtx_denoiser_destroy": (i32, [u64]),
     }
     for name, (res, args) in sig.items():

@@ -95,3 +95,32 @@ def tune(target, samples=2, cache_path=None, log=None):
     except OSError:
         pass
     return best
+
+
+# ---------------------------------------------------------------- denoiser
+DENOISER_SHAPES = [(mt, nt) for mt in (1, 2, 4, 8) for nt in (1, 2, 3, 4, 7)]
+
+
+def tune_denoiser(denoiser, width, height, iterations=3, log=None):
+    """Picks each U-Net layer's convolution launch shape (pixel tiles x
+    output-channel tiles per wave) by measuring whole forward passes with
+    one layer changed at a time (HIP events, mtx_denoiser_benchmark), and
+    applies the fastest. Returns {layer: (pixel_tiles, channel_tiles)} and the
+    pass time before and after."""
+    before = min(denoiser.benchmark(width, height, iterations) for _ in range(2))
+    best = {}
+    for layer in range(16):
+        cur = denoiser.tiling(layer, width, height)
+        trials = []
+        for mt, nt in DENOISER_SHAPES:
+            denoiser.set_tiling(layer, mt, nt)
+            trials.append((min(denoiser.benchmark(width, height, iterations) for _ in range(2)), mt, nt))
+        trials.sort()
+        ms, mt, nt = trials[0]
+        best[layer] = (mt, nt)
+        denoiser.set_tiling(layer, mt, nt)
+        if log:
+            log(f"denoiser layer {layer}: {cur} -> {(mt, nt)} ({ms:.3f} ms per pass)")
+    after = min(denoiser.benchmark(width, height, iterations) for _ in range(2))
+    return best, before, after
+

@@ -9,9 +9,9 @@
 // one pixel is one 32-byte run.
 //
 // Convolution = implicit GEMM on v_mfma_f32_16x16x16 (f16 / bf16): one wave
-// computes 16 consecutive output pixels of a row (the M = 16 rows of the MFMA
-// tile) for all output channels (NT tiles of N = 16), stepping K over 9 taps x
-// 16-channel input slices. The A operand (16 pixels x 16 channels) is one
+// computes kMT tiles of 16 consecutive output pixels of a row (the M = 16
+// rows of each MFMA tile) for all output channels (NT tiles of N = 16),
+// stepping K over 9 taps x 16-channel input slices. The A operand (16 pixels x 16 channels) is one
 // 8-byte load per lane straight from the NHWC tensor; the B operand (16
 // channels x 16 outputs) comes from weights pre-packed per (tap, slice,
 // output, channel). The decoder's upsample + concat is folded into the A load:
@@ -53,66 +53,86 @@ struct ConvArgs {
   uint16_t *dst;         // [H][W][ds]
   int c1, cs1, up1, c2, cs2;
   int H, W, kchunks, kc1, cout, ds, relu;
+  int ntot;     // output tiles of 16 channels (weights row stride / 16)
+  int ngroups;  // waves sharing one pixel segment, each on NT of the ntot tiles
 };
 
-template <bool kBf16, int NT>
+// kMT: 16-pixel tiles per wave along the row. Each K step loads the NT weight
+// fragments once and reuses them for all kMT tiles (kMT x NT MFMAs per
+// 1 + ... loads), which is what keeps the matrix cores fed: with kMT = 1 the
+// wave issues one weight load per MFMA.
+template <bool kBf16, int NT, int kMT>
 __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
   const int lane = threadIdx.x & 63;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int segs = (a.W + 15) >> 4;  // the last segment of a row may be partial
-  if (wave >= segs * a.H) return;   // wave-uniform
-  const int y = wave / segs, x0 = (wave - y * segs) << 4;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int segs = (a.W + 16 * kMT - 1) / (16 * kMT);  // the last segment of a row may be partial
+  if (gw >= segs * a.H * a.ngroups) return;             // wave-uniform
+  const int g = gw % a.ngroups, wave = gw / a.ngroups;  // the groups of a segment are neighbours
+  const int y = wave / segs, x0 = (wave - y * segs) * (16 * kMT);
+  const int j0 = g * NT;                                // first output tile of this wave
   const int row = lane & 15, quad = lane >> 4;
-  float4v acc[NT];
+  float4v acc[kMT][NT];
 #pragma unroll
-  for (int j = 0; j < NT; j++) acc[j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int m = 0; m < kMT; m++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[m][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
   const int W2 = a.W >> 1;
   for (int t = 0; t < 9; t++) {
-    const int yy = y + t / 3 - 1, xx = x0 + row + t % 3 - 1;
-    const bool inside = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    const int yy = y + t / 3 - 1, dx = t % 3 - 1;
+    const bool rowIn = yy >= 0 && yy < a.H;
     for (int kc = 0; kc < a.kchunks; kc++) {
-      uint2 av = make_uint2(0u, 0u);
-      if (inside) {
-        const uint16_t *p;
-        if (kc < a.kc1) {
-          p = a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1
-                    : a.src1 + ((size_t)yy * a.W + xx) * a.cs1;
-          p += kc * 16 + quad * 4;
-        } else {
-          p = a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16 + quad * 4;
-        }
-        av = *reinterpret_cast<const uint2 *>(p);
-      }
-      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (NT * 16) + row) * 16 + quad * 4;
+      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (a.ntot * 16) + j0 * 16 + row) * 16 + quad * 4;
+      uint2 bv[NT];
 #pragma unroll
-      for (int j = 0; j < NT; j++) {
-        const uint2 bv = *reinterpret_cast<const uint2 *>(wb + j * 256);
-        if (kBf16) {
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, av),
-                                                            __builtin_bit_cast(short4v, bv), acc[j], 0, 0, 0);
-        } else {
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, av),
-                                                         __builtin_bit_cast(half4v, bv), acc[j], 0, 0, 0);
+      for (int j = 0; j < NT; j++) bv[j] = j0 + j < a.ntot ? *reinterpret_cast<const uint2 *>(wb + j * 256)
+                                                            : make_uint2(0u, 0u);
+#pragma unroll
+      for (int m = 0; m < kMT; m++) {
+        const int xx = x0 + 16 * m + row + dx;
+        uint2 av = make_uint2(0u, 0u);
+        if (rowIn && xx >= 0 && xx < a.W) {
+          const uint16_t *p;
+          if (kc < a.kc1) {
+            p = a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1
+                      : a.src1 + ((size_t)yy * a.W + xx) * a.cs1;
+            p += kc * 16 + quad * 4;
+          } else {
+            p = a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16 + quad * 4;
+          }
+          av = *reinterpret_cast<const uint2 *>(p);
+        }
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+          if (kBf16) {
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, av),
+                                                                 __builtin_bit_cast(short4v, bv[j]), acc[m][j], 0, 0, 0);
+          } else {
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, av),
+                                                              __builtin_bit_cast(half4v, bv[j]), acc[m][j], 0, 0, 0);
+          }
         }
       }
     }
   }
   // D layout: lane holds output channel (lane & 15) of pixels 4 * (lane >> 4) + e.
 #pragma unroll
-  for (int j = 0; j < NT; j++) {
-    const int n = j * 16 + row;
-    if (n >= a.ds) continue;
-    const float b = n < a.cout ? a.bias[n] : 0.0f;
+  for (int m = 0; m < kMT; m++) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int px = x0 + quad * 4 + e;
-      if (px >= a.W) continue;
-      float v = 0.0f;
-      if (n < a.cout) {
-        v = acc[j][e] + b;
-        if (a.relu) v = fmaxf(v, 0.0f);
+    for (int j = 0; j < NT; j++) {
+      const int n = (j0 + j) * 16 + row;
+      if (n >= a.ds) continue;
+      const float b = n < a.cout ? a.bias[n] : 0.0f;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int px = x0 + 16 * m + quad * 4 + e;
+        if (px >= a.W) continue;
+        float v = 0.0f;
+        if (n < a.cout) {
+          v = acc[m][j][e] + b;
+          if (a.relu) v = fmaxf(v, 0.0f);
+        }
+        a.dst[((size_t)y * a.W + px) * a.ds + n] = to_store<kBf16>(v);
       }
-      a.dst[((size_t)y * a.W + px) * a.ds + n] = to_store<kBf16>(v);
     }
   }
 }
@@ -290,6 +310,7 @@ struct DenoiserObj {
     DevBuf<uint16_t> wt;
     DevBuf<float> bias;
     int kchunks = 0, kc1 = 0, nt = 0;
+    int tileMt = 0, tileNt = 0;  // pixel / output tiles per wave; 0 = the default of conv_tiling
   } L[16];
   // activations (sized for H x W)
   int H = 0, W = 0;
@@ -345,6 +366,20 @@ static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTens
   L.bias.upload(b.data.data(), b.data.size(), 0);
 }
 
+// Tiling of one convolution: each wave computes mt x 16 pixels of a row for
+// ntw of the layer's 16-channel output tiles (a weight fragment serves mt
+// MFMAs, an input fragment ntw; both cost registers, occupancy and waves).
+// Defaults per layer, network order: the fastest shapes of a per-layer sweep
+// at 1920 x 1088, FP16 (2.95 ms per pass against 3.76 ms with one 16-pixel
+// tile and all outputs per wave); mtx_denoiser_set_tiling overrides them
+// (the denoiser autotuner re-measures them for another resolution).
+constexpr int kConvTiling[16][2] = {{2, 4}, {4, 4}, {2, 7}, {2, 4}, {1, 3}, {2, 3}, {2, 2}, {1, 7},
+                                    {1, 4}, {2, 7}, {2, 7}, {2, 7}, {2, 4}, {4, 4}, {2, 4}, {4, 7}};
+static void conv_tiling(const DenoiserObj::Layer &L, int layer, int &mt, int &ntw) {
+  mt = L.tileMt ? L.tileMt : kConvTiling[layer][0];
+  ntw = std::min(L.tileNt ? L.tileNt : kConvTiling[layer][1], L.nt);
+}
+
 static void ensure_size(DenoiserObj &D, int H, int W) {
   if (D.H == H && D.W == W) return;
   D.in.alloc((size_t)H * W * 16);
@@ -358,21 +393,31 @@ static void ensure_size(DenoiserObj &D, int H, int W) {
   D.W = W;
 }
 
-template <bool kBf16, int NT>
-static void launch_conv_nt(const ConvArgs &a, hipStream_t s) {
-  const int waves = ((a.W + 15) >> 4) * a.H;
-  hipLaunchKernelGGL((k_conv3x3<kBf16, NT>), dim3((waves + 3) / 4), dim3(256), 0, s, a);
+template <bool kBf16, int NT, int kMT>
+static void launch_conv_t(const ConvArgs &a, hipStream_t s) {
+  const long waves = (long)((a.W + 16 * kMT - 1) / (16 * kMT)) * a.H * a.ngroups;
+  hipLaunchKernelGGL((k_conv3x3<kBf16, NT, kMT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
 }
+template <bool kBf16, int NT>
+static void launch_conv_nt(const ConvArgs &a, int mt, hipStream_t s) {
+  switch (mt) {
+    case 1: launch_conv_t<kBf16, NT, 1>(a, s); break;
+    case 2: launch_conv_t<kBf16, NT, 2>(a, s); break;
+    case 4: launch_conv_t<kBf16, NT, 4>(a, s); break;
+    default: launch_conv_t<kBf16, NT, 8>(a, s); break;
+  }
+}
+// nt: output tiles per wave (1..7), mt: pixel tiles per wave (1, 2, 4, 8).
 template <bool kBf16>
-static void launch_conv(const ConvArgs &a, int nt, hipStream_t s) {
+static void launch_conv(const ConvArgs &a, int nt, int mt, hipStream_t s) {
   switch (nt) {
-    case 1: launch_conv_nt<kBf16, 1>(a, s); break;
-    case 2: launch_conv_nt<kBf16, 2>(a, s); break;
-    case 3: launch_conv_nt<kBf16, 3>(a, s); break;
-    case 4: launch_conv_nt<kBf16, 4>(a, s); break;
-    case 5: launch_conv_nt<kBf16, 5>(a, s); break;
-    case 6: launch_conv_nt<kBf16, 6>(a, s); break;
-    default: launch_conv_nt<kBf16, 7>(a, s); break;
+    case 1: launch_conv_nt<kBf16, 1>(a, mt, s); break;
+    case 2: launch_conv_nt<kBf16, 2>(a, mt, s); break;
+    case 3: launch_conv_nt<kBf16, 3>(a, mt, s); break;
+    case 4: launch_conv_nt<kBf16, 4>(a, mt, s); break;
+    case 5: launch_conv_nt<kBf16, 5>(a, mt, s); break;
+    case 6: launch_conv_nt<kBf16, 6>(a, mt, s); break;
+    default: launch_conv_nt<kBf16, 7>(a, mt, s); break;
   }
 }
 
@@ -407,7 +452,11 @@ static void run_unet(DenoiserObj &D, hipStream_t s) {
     a.ds = pad16(Ly.cout);
     a.dst = D.act[i].ptr;
     a.relu = Ly.relu;
-    launch_conv<kBf16>(a, D.L[i].nt, s);
+    a.ntot = D.L[i].nt;
+    int mt, ntw;
+    conv_tiling(D.L[i], i, mt, ntw);
+    a.ngroups = (a.ntot + ntw - 1) / ntw;
+    launch_conv<kBf16>(a, ntw, mt, s);
     prev = D.act[i].ptr;
     prevC = a.ds;
     if (i >= 1 && i <= 4) {  // enc_conv1..4 are pooled; the pooled tensor feeds the next level
@@ -495,6 +544,41 @@ int32_t mtx_denoiser_forward(mtx_handle handle, uint32_t width, uint32_t height,
   } catch (const std::exception &e) {
     set_error(err, cap, e.what());
     return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_set_tiling(mtx_handle handle, uint32_t layer, uint32_t pixel_tiles, uint32_t channel_tiles) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(layer < 16, MTX_INVALID_ARGUMENT, "layer must be in 0..15");
+    const bool autoShape = pixel_tiles == 0 && channel_tiles == 0;
+    MTX_REQUIRE(autoShape || ((pixel_tiles == 1 || pixel_tiles == 2 || pixel_tiles == 4 || pixel_tiles == 8) &&
+                              channel_tiles >= 1 && channel_tiles <= 7),
+                MTX_INVALID_ARGUMENT, "pixel_tiles must be 1, 2, 4 or 8 and channel_tiles 1..7 (or both 0)");
+    D->L[layer].tileMt = (int)pixel_tiles;
+    D->L[layer].tileNt = (int)channel_tiles;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_get_tiling(mtx_handle handle, uint32_t layer, uint32_t width, uint32_t height,
+                                uint32_t *pixel_tiles, uint32_t *channel_tiles) {
+  try {
+    auto D = get_denoiser(handle);
+    MTX_REQUIRE(layer < 16 && pixel_tiles && channel_tiles, MTX_INVALID_ARGUMENT, "bad arguments");
+    (void)width;
+    (void)height;
+    int mt, ntw;
+    conv_tiling(D->L[layer], (int)layer, mt, ntw);
+    *pixel_tiles = (uint32_t)mt;
+    *channel_tiles = (uint32_t)ntw;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
   }
 }
 

@@ -62,6 +62,17 @@ class Denoiser:
             return out, {name: b for (name, _, _), b in zip(LAYERS, bufs)}
         return out
 
+    def set_tiling(self, layer, pixel_tiles=0, channel_tiles=0):
+        """Launch shape of one layer's convolution (0, 0 = automatic)."""
+        abi.check(abi.lib().mtx_denoiser_set_tiling(self.handle.value, layer, pixel_tiles, channel_tiles))
+
+    def tiling(self, layer, width, height):
+        """(pixel_tiles, channel_tiles) the layer uses at width x height."""
+        mt, nt = C.c_uint32(0), C.c_uint32(0)
+        abi.check(abi.lib().mtx_denoiser_get_tiling(self.handle.value, layer, width, height, C.byref(mt),
+                                                    C.byref(nt)))
+        return mt.value, nt.value
+
     def benchmark(self, width, height, iterations=10):
         ms = C.c_double(0)
         abi.check(abi.lib().mtx_denoiser_benchmark(self.handle.value, width, height, iterations, C.byref(ms)))

@@ -79,3 +79,25 @@ def test_run_semantics(net):
     nflat[..., 2] = 1
     o2 = d.run(flat, nflat, flat)
     assert abs(float(o2[8:-8, 8:-8, :3].mean()) - 0.5) < 0.1
+
+
+def test_tiling_shapes_give_identical_outputs(net):
+    """Every launch shape accumulates each output in the same (tap, channel
+    slice) order, so all shapes give the same bits; the tuner keeps them."""
+    denoise, _ = net
+    from mutracer_amd import autotune
+    d = denoise.Denoiser("fp16")
+    x = _input(64, 96, seed=5)
+    ref, ref_layers = d.forward(x, layers=True)
+    for mt, nt in ((1, 1), (8, 7), (2, 3), (4, 2)):
+        for layer in range(16):
+            d.set_tiling(layer, mt, nt)
+            assert d.tiling(layer, 96, 64)[0] == mt
+        out, layers = d.forward(x, layers=True)
+        assert np.array_equal(out, ref), (mt, nt)
+        assert all(np.array_equal(layers[k], ref_layers[k]) for k in ref_layers)
+    with pytest.raises(Exception):
+        d.set_tiling(0, 3, 1)
+    best, before, after = autotune.tune_denoiser(d, 96, 64, iterations=1)
+    assert len(best) == 16 and before > 0 and after > 0
+    assert np.array_equal(d.forward(x), ref)
