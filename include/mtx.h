@@ -172,7 +172,8 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 #define MTX_MAT_COAT_ROUGHNESS 20
 #define MTX_MAT_COAT_IOR 21
 #define MTX_MAT_COAT_DARKENING 22
-#define MTX_MAT_EMISSION 23           /* 3 floats */
+#define MTX_MAT_EMISSION 23           /* 3 floats: materialEmission() (color * luminance, coat-tinted);
+                                         instances with emitting materials are triangle lights */
 #define MTX_MAT_DOUBLE_SIDED 26
 #define MTX_MAT_SPECULAR_ANISOTROPY 27
 #define MTX_MATERIAL_FLOATS 32

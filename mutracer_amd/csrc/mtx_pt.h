@@ -38,6 +38,8 @@ struct GpuScene {
   const float2 *envCdf;        // (envW + 2) x (envH + 1)
   uint32_t envW, envH;
   const float *lut;            // 32 x 2177 energy tables
+  const float4 *emissive;      // 3 per emissive triangle: (p0, pmf), (p1, area), (p2, material), world space
+  uint32_t emissiveCount, pad0;
 };
 
 struct RenderParams {

@@ -23,13 +23,14 @@ struct RenderSceneObj {
   DevBuf<ShadeSlot> shade;
   DevBuf<float> materials, lights, lightPmf, lut;
   DevBuf<LightChoice> choices;
-  DevBuf<float4> env;
+  DevBuf<float4> env, emissive;
   DevBuf<float2> envCdf;
   std::vector<mtx_instance_desc> inst;
   GpuScene view{};
   uint64_t triangles = 0;
   size_t bytes() const {
-    size_t b = shade.bytes() + materials.bytes() + lights.bytes() + lut.bytes() + env.bytes() + envCdf.bytes();
+    size_t b = shade.bytes() + materials.bytes() + lights.bytes() + lut.bytes() + env.bytes() + envCdf.bytes() +
+               emissive.bytes();
     for (auto &x : normals) b += x.bytes();
     for (auto &x : indices) b += x.bytes();
     for (auto &x : blases) b += x->resident_bytes();
@@ -74,10 +75,61 @@ std::shared_ptr<TargetObj> get_target(uint64_t h) {
 // ---------------------------------------------------------------- light tables
 struct LightTables {
   std::vector<LightChoice> choices;
-  std::vector<float> lightPmf;
+  std::vector<float> lightPmf, emissivePmf;
   float envPmf = 0.0f;
   double total = 0.0;
 };
+
+// collectEmissiveTriangles (lightSampling.ts:237-320): every triangle of an
+// instance whose material emits (luminance of the packed emission above
+// 1e-8), in instance then triangle order, moved to world space with the
+// instance transform (f64, f32 storage as vec3.transformMat4 does), with its
+// area (half the f64 length of the f32 edge cross product; degenerate
+// triangles are skipped) and alias weight area * luminance
+// (prepareSceneLightSampling, sceneAssembly.ts:46).
+struct EmissiveList {
+  std::vector<float4> tris;  // 3 per triangle: (p0, pmf), (p1, area), (p2, material)
+  std::vector<double> weight;
+};
+static EmissiveList collect_emissive(const mtx_scene_desc &D) {
+  EmissiveList E;
+  for (uint32_t i = 0; i < D.instance_count; i++) {
+    const mtx_instance_desc &d = D.instances[i];
+    const uint32_t mat = d.material_word >> 8;
+    const float *em = D.materials + (size_t)mat * MTX_MATERIAL_FLOATS + MTX_MAT_EMISSION;
+    const double lum = std::max((double)em[0] * 0.2126 + (double)em[1] * 0.7152 + (double)em[2] * 0.0722, 0.0);
+    if (!(lum > 1e-8)) continue;
+    const mtx_mesh_desc &m = D.meshes[d.blas_index];
+    const double *M = d.transform;
+    for (uint32_t t = 0; t < m.triangle_count; t++) {
+      float w[3][3];
+      for (int c = 0; c < 3; c++) {
+        const float *v = m.positions + (size_t)m.indices[(size_t)t * 3 + c] * 3;
+        const double x = v[0], y = v[1], z = v[2];
+        double h = M[3] * x + M[7] * y + M[11] * z + M[15];
+        if (h == 0.0) h = 1.0;
+        w[c][0] = (float)((M[0] * x + M[4] * y + M[8] * z + M[12]) / h);
+        w[c][1] = (float)((M[1] * x + M[5] * y + M[9] * z + M[13]) / h);
+        w[c][2] = (float)((M[2] * x + M[6] * y + M[10] * z + M[14]) / h);
+      }
+      float e0[3], e1[3];
+      for (int k = 0; k < 3; k++) {
+        e0[k] = (float)((double)w[1][k] - (double)w[0][k]);
+        e1[k] = (float)((double)w[2][k] - (double)w[0][k]);
+      }
+      const float cx = (float)((double)e0[1] * e1[2] - (double)e0[2] * e1[1]);
+      const float cy = (float)((double)e0[2] * e1[0] - (double)e0[0] * e1[2]);
+      const float cz = (float)((double)e0[0] * e1[1] - (double)e0[1] * e1[0]);
+      const double area = 0.5 * std::hypot((double)cx, (double)cy, (double)cz);
+      if (!(area > 1e-10) || !std::isfinite(area)) continue;
+      E.tris.push_back(make_float4(w[0][0], w[0][1], w[0][2], 0.0f));
+      E.tris.push_back(make_float4(w[1][0], w[1][1], w[1][2], (float)area));
+      E.tris.push_back(make_float4(w[2][0], w[2][1], w[2][2], (float)mat));
+      E.weight.push_back(area * lum);
+    }
+  }
+  return E;
+}
 
 // estimateAnalyticLightPower (lightSampling.ts:72-107) from the packed light:
 // the emission holds color * intensity, or the radiance of disk, cylinder and
@@ -104,7 +156,8 @@ static double analytic_power(const float *L) {
   }
 }
 
-static LightTables light_tables(const float *lights, uint32_t nLights, const float *env, uint32_t W, uint32_t H) {
+static LightTables light_tables(const float *lights, uint32_t nLights, const float *env, uint32_t W, uint32_t H,
+                                const std::vector<double> &emissiveWeight = {}) {
   struct In { int32_t type, index; double w; };
   std::vector<In> in;
   auto push = [&](int32_t t, int32_t i, double w) {
@@ -125,8 +178,10 @@ static LightTables light_tables(const float *lights, uint32_t nLights, const flo
   }
   push(0, 0, envPower);
   for (uint32_t i = 0; i < nLights; i++) push(1, (int32_t)i, analytic_power(lights + (size_t)i * MTX_LIGHT_FLOATS));
+  for (size_t i = 0; i < emissiveWeight.size(); i++) push(2, (int32_t)i, emissiveWeight[i]);
   LightTables T;
   T.lightPmf.assign(nLights, 0.0f);
+  T.emissivePmf.assign(emissiveWeight.size(), 0.0f);
   const size_t n = in.size();
   double total = 0.0;
   for (auto &e : in) total += e.w;
@@ -156,7 +211,8 @@ static LightTables light_tables(const float *lights, uint32_t nLights, const flo
     T.choices[i].type = in[i].type;
     T.choices[i].index = in[i].index;
     if (in[i].type == 0) T.envPmf = pmf[i];
-    else T.lightPmf[in[i].index] = pmf[i];
+    else if (in[i].type == 1) T.lightPmf[in[i].index] = pmf[i];
+    else T.emissivePmf[in[i].index] = pmf[i];
   }
   return T;
 }
@@ -262,7 +318,11 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   R->materials.upload(D.materials, (size_t)D.material_count * MTX_MATERIAL_FLOATS, 0);
   R->lights.upload_min1(D.lights, (size_t)D.light_count * MTX_LIGHT_FLOATS, 0);
   const bool hasEnv = D.env_rgba && D.env_width && D.env_height;
-  LightTables T = light_tables(D.lights, D.light_count, hasEnv ? D.env_rgba : nullptr, D.env_width, D.env_height);
+  EmissiveList E = collect_emissive(D);
+  LightTables T = light_tables(D.lights, D.light_count, hasEnv ? D.env_rgba : nullptr, D.env_width, D.env_height,
+                               E.weight);
+  for (size_t i = 0; i < E.weight.size(); i++) E.tris[3 * i].w = T.emissivePmf[i];
+  R->emissive.upload_min1(E.tris.data(), E.tris.size(), 0);
   R->lightPmf.upload_min1(T.lightPmf.data(), T.lightPmf.size(), 0);
   R->choices.upload_min1(T.choices.data(), T.choices.size(), 0);
   if (hasEnv) {
@@ -284,6 +344,8 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   v.lightWeightTotal = (float)T.total;
   v.env = hasEnv ? R->env.ptr : nullptr;
   v.envCdf = hasEnv ? R->envCdf.ptr : nullptr;
+  v.emissive = E.weight.empty() ? nullptr : R->emissive.ptr;
+  v.emissiveCount = (uint32_t)E.weight.size();
   v.envW = hasEnv ? D.env_width : 0;
   v.envH = hasEnv ? D.env_height : 0;
   v.lut = R->lut.ptr;

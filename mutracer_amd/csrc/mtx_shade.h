@@ -215,7 +215,7 @@ __device__ __forceinline__ f3 to_local(const Basis &B, f3 d) { return f3{dot(d, 
 
 // ---------------------------------------------------------------- materials
 struct Mat {
-  f3 baseColor, specularColor, transmissionColor, coatColor;
+  f3 baseColor, specularColor, transmissionColor, coatColor, emission;
   float baseWeight, diffuseRoughness, baseMetalness, specularWeight, specularRoughness, specularIor;
   float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening;
   float relativeIor, alphaX, alphaY;
@@ -240,6 +240,7 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.inside = false;
   ma.specularIor = m[11];
   ma.transmissionWeight = m[12];
+  ma.emission = f3{m[23], m[24], m[25]};
   ma.baseWeight = clampf(m[0], 0.0f, 1.0f);
   ma.diffuseRoughness = clampf(m[4], 0.0f, 1.0f);
   ma.specularWeight = mxf(m[6], 0.0f);
@@ -654,7 +655,7 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
 
 // ---------------------------------------------------------------- lights
 struct LSample {
-  f3 emission, direction;
+  f3 emission, direction, normal;
   float dist, pdf, selPmf;
   uint32_t sourceType;
   bool isDelta;
@@ -853,6 +854,27 @@ __device__ void sample_analytic(const float *__restrict__ L, f3 p, float sx, flo
     default: break;
   }
 }
+// sampleEmissiveTriangleLight (sampleLight.wgsl.ts:316-363): uniform point on
+// the world-space triangle, one-sided, pdf = selection pmf * d^2 / (area cos).
+__device__ void sample_emissive(const GpuScene &S, uint32_t index, f3 p, float rx, float ry, LSample &r) {
+  const float4 a = S.emissive[3 * index], b = S.emissive[3 * index + 1], c = S.emissive[3 * index + 2];
+  const f3 p0 = f3of(a), p1 = f3of(b), p2 = f3of(c);
+  const float sqrtU = sqrtf(rx);
+  const f3 bc{1.0f - sqrtU, sqrtU * (1.0f - ry), sqrtU * ry};
+  const f3 lp = bc.x * p0 + bc.y * p1 + bc.z * p2;
+  const f3 normal = normalize(cross(p1 - p0, p2 - p0));
+  f3 dir = lp - p;
+  r.dist = len(dir);
+  if (r.dist <= kEps) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
+  r.direction = dir / r.dist;
+  r.normal = normal;
+  const float cosLight = dot(normal, -r.direction);
+  if (cosLight <= 0.0f) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
+  const float *m = S.materials + (size_t)(uint32_t)c.w * 32;
+  r.emission = f3{m[23], m[24], m[25]};
+  r.selPmf = a.w;
+  r.pdf = a.w * (r.dist * r.dist) / mxf(b.w * cosLight, 1e-8f);
+}
 __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float rz, uint32_t cand, uint32_t count,
                              LSample &ls) {
   uint32_t cnt = max(count, 1u);
@@ -878,6 +900,9 @@ __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float 
     sample_analytic(S.lights + (size_t)ch.index * 16, p, rx, ry, ls);
     ls.selPmf = S.lightPmf[ch.index];
     ls.pdf *= ls.selPmf;
+  } else if (ch.type == 2) {
+    ls.sourceType = 1;
+    sample_emissive(S, (uint32_t)ch.index, p, rx, ry, ls);
   }
 }
 // intersectSphere (intersect.wgsl.ts:924-945).
@@ -1257,6 +1282,25 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
     ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
     f3 ff = ndv >= 0.0f ? N : -N;
     pr.flags &= ~kIsDelta;
+    // Surface emission (sampleMaterial.wgsl.ts:412-422), weighted by
+    // emissiveHitMisWeight (:279-303): one-sided unless double-sided, weight 1
+    // for camera rays, else MIS against the emissive-triangle light density
+    // (selection pmf / area = luminance / total light weight).
+    {
+      const float cosLight = dot(si.faceNormal, -rd);
+      float w = 1.0f;
+      if (cosLight <= 0.0f && S.materials[(size_t)si.material * 32 + 26] < 0.5f) {
+        w = 0.0f;
+      } else if (!((pr.flags & kFromEye) | (S.lightWeightTotal <= 0.0f))) {
+        const float lum = dot(ma.emission, f3{0.2126f, 0.7152f, 0.0722f});
+        if (!((lum <= 0.0f) | (cosLight <= 0.0f))) {
+          const float hitDistance = len(si.position - pr.ro);
+          const float lightPdf = lum / S.lightWeightTotal * hitDistance * hitDistance / mxf(cosLight, 1e-8f);
+          w = bsdf_mis(P.candidates, pr.scatPdf, lightPdf);
+        }
+      }
+      pr.li = pr.li + w * ma.emission * pr.beta;
+    }
     Prep PR;
     prepare(PR, ma, -rd, ff, L);
     const uint32_t cc = max(P.candidates, 1u);
@@ -1347,6 +1391,14 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       cand.ro = origin;
       cand.rd = ldir;
       cand.maxDistance = ls.dist - kEps;
+      if (ls.sourceType == 1u) {
+        // Emissive triangles: the visibility ray ends EPS off the sampled
+        // point along the light normal (sampleMaterial.wgsl.ts:98-110).
+        const f3 lightEndpoint = si.position + ls.direction * ls.dist;
+        const f3 toOffsetLight = (lightEndpoint + kEps * ls.normal) - origin;
+        cand.maxDistance = len(toOffsetLight);
+        cand.rd = normalize(toOffsetLight);
+      }
       cand.contribution = mis * un;
       cand.selPmf = ls.selPmf;
       cand.valid = true;

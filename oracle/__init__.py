@@ -419,7 +419,35 @@ def _declare_render2(L):
     L.orc_set_energy_lut.argtypes = [vp]
     L.orc_render.restype = C.c_uint64
     L.orc_render.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, u32,
-                             u32, u32, vp]
+                             u32, u32, vp, vp, vp, u32]
+    L.orc_emissive_triangles.restype = u32
+    L.orc_emissive_triangles.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32]
+
+
+def emissive_triangles(scene):
+    """World-space emissive triangles of a mutracer_amd.scene.Scene: (n, 12)
+    f32 records (p0, p1, p2, area, material, 0) and their f64 alias weights."""
+    L = lib()
+    _declare_render2(L)
+    meshes = scene.meshes
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(m["positions"], np.float32).reshape(-1, 3)
+                                               for m in meshes]) if meshes else np.zeros((0, 3), np.float32))
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(m["indices"], np.uint32).reshape(-1)
+                                               for m in meshes]) if meshes else np.zeros(0, np.uint32))
+    voff = np.cumsum([0] + [np.asarray(m["positions"]).reshape(-1, 3).shape[0] for m in meshes]).astype(np.uint32)
+    ioff = np.cumsum([0] + [np.asarray(m["indices"]).size for m in meshes]).astype(np.uint32)
+    tcount = np.array([np.asarray(m["indices"]).size // 3 for m in meshes], np.uint32)
+    inst_mesh = np.ascontiguousarray(scene.mesh_index, np.uint32)
+    tr = np.ascontiguousarray(np.asarray(scene.transforms, np.float64).reshape(-1, 16))
+    inst_mat = np.ascontiguousarray(scene.instance_material, np.uint32)
+    mats = scene.packed_materials()
+    args = [_p(pos), _p(voff), _p(idx), _p(ioff), _p(tcount), _p(inst_mesh), _p(tr), _p(inst_mat),
+            inst_mesh.size, _p(mats)]
+    n = L.orc_emissive_triangles(*args, None, None, 0)
+    out = np.zeros((max(n, 1), 12), np.float32)
+    w = np.zeros(max(n, 1), np.float64)
+    L.orc_emissive_triangles(*args, _p(out), _p(w), n)
+    return out[:n], w[:n]
 
 
 class OracleRenderer:
@@ -444,6 +472,7 @@ class OracleRenderer:
         self.settings = np.array([width, height, bounces, candidates, pattern, seed & 0xFFFFFFFF, log, sample_offset],
                                  np.uint32)
         self.clamping = clamping
+        self.emissive, self.emissive_weight = emissive_triangles(scene)
         self.width, self.height = width, height
         self.rays = 0
 
@@ -456,5 +485,6 @@ class OracleRenderer:
         self.rays += lib().orc_render(self.trace.h, _p(self.normals), _p(self.materials), self.materials.shape[0],
                                       _p(self.lights), self.lights.shape[0], None if env is None else _p(env),
                                       ew, eh, _p(self.camera), _p(self.settings), self.clamping, x0, y0, w, h,
-                                      sample_start, count, _p(out))
+                                      sample_start, count, _p(out), _p(self.emissive), _p(self.emissive_weight),
+                                      self.emissive.shape[0])
         return out

@@ -187,11 +187,24 @@ void orc_set_energy_lut(const float *lut) { set_energy_lut(lut); }
 
 // settings: width, height, bounces, candidates, pattern, seed, logSPP, sampleOffset.
 // out: w*h*13 doubles per pixel (color rgba, albedo rgba, normal rgba, lum^2).
+uint32_t orc_emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,
+                                const uint32_t *indexOffsets, const uint32_t *triangleCounts,
+                                const uint32_t *instanceMesh, const double *transforms,
+                                const uint32_t *instanceMaterial, uint32_t instanceCount, const float *materials,
+                                float *out, double *weight, uint32_t capacity) {
+  return emissive_triangles(positions, vertexOffsets, indices, indexOffsets, triangleCounts, instanceMesh,
+                            transforms, instanceMaterial, instanceCount, materials, out, weight, capacity);
+}
+
 uint64_t orc_render(void *trace, const float *normals, const float *materials, uint32_t materialCount,
                     const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
                     const float *camera, const uint32_t *settings, float clamping, uint32_t x0, uint32_t y0,
-                    uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out) {
+                    uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out,
+                    const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
   RenderScene s{};
+  s.emissive = emissive;
+  s.emissiveWeight = emissiveWeight;
+  s.emissiveCount = emissiveCount;
   s.trace = (const TraceScene *)trace;
   s.normals = normals;
   s.materials = materials; s.materialCount = materialCount;

@@ -645,8 +645,10 @@ void build_tables(const RenderScene &s, HostTables &T) {
     }
     push(1, (int32_t)i, w);
   }
+  for (uint32_t i = 0; i < s.emissiveCount; i++) push(2, (int32_t)i, s.emissiveWeight[i]);  // sceneAssembly.ts:47
   size_t n = weights.size();
   T.lightPmf.assign(s.lightCount, 0.0f);
+  T.emissivePmf.assign(s.emissiveCount, 0.0f);
   double total = 0.0;
   for (double w : weights) total += w;
   T.totalWeight = n && total > 0 ? total : 0.0;
@@ -677,7 +679,8 @@ void build_tables(const RenderScene &s, HostTables &T) {
     T.type[i] = types[i];
     T.index[i] = idxs[i];
     if (types[i] == 0) T.envPmf = pmf[i];
-    else T.lightPmf[idxs[i]] = pmf[i];
+    else if (types[i] == 1) T.lightPmf[idxs[i]] = pmf[i];
+    else T.emissivePmf[idxs[i]] = pmf[i];
   }
   // Small/large lists pop from the back in the reference (small[--smallCount]).
 }
@@ -935,6 +938,81 @@ static void light_sample_analytic(const float *L, V3 p, float sx, float sy, Ligh
   }
 }
 
+// collectEmissiveTriangles (lightSampling.ts:237-320): vec3.transformMat4 in
+// f64 stored to f32, f32 edges and cross product, area = |cross| / 2 in f64
+// (Math.hypot), degenerate triangles dropped; weight = area * luminance of
+// the material's emission (prepareSceneLightSampling, sceneAssembly.ts:47).
+uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,
+                            const uint32_t *indexOffsets, const uint32_t *triangleCounts, const uint32_t *instanceMesh,
+                            const double *transforms, const uint32_t *instanceMaterial, uint32_t instanceCount,
+                            const float *materials, float *out, double *weight, uint32_t capacity) {
+  uint32_t n = 0;
+  for (uint32_t i = 0; i < instanceCount; i++) {
+    const float *em = materials + (size_t)instanceMaterial[i] * 32 + M_EMISSION;
+    double lum = (double)em[0] * 0.2126 + (double)em[1] * 0.7152 + (double)em[2] * 0.0722;
+    if (lum < 0) lum = 0;
+    if (!(lum > 1e-8)) continue;
+    const uint32_t mesh = instanceMesh[i];
+    const double *m = transforms + (size_t)i * 16;
+    for (uint32_t t = 0; t < triangleCounts[mesh]; t++) {
+      float world[3][3];
+      for (int corner = 0; corner < 3; corner++) {
+        uint32_t v = vertexOffsets[mesh] + indices[indexOffsets[mesh] + t * 3 + corner];
+        double x = positions[v * 3], y = positions[v * 3 + 1], z = positions[v * 3 + 2];
+        double w = m[3] * x + m[7] * y + m[11] * z + m[15];
+        if (w == 0.0) w = 1.0;  // `w || 1.0`
+        world[corner][0] = (float)((m[0] * x + m[4] * y + m[8] * z + m[12]) / w);
+        world[corner][1] = (float)((m[1] * x + m[5] * y + m[9] * z + m[13]) / w);
+        world[corner][2] = (float)((m[2] * x + m[6] * y + m[10] * z + m[14]) / w);
+      }
+      float a[3], b[3], cr[3];
+      for (int k = 0; k < 3; k++) {
+        a[k] = (float)((double)world[1][k] - (double)world[0][k]);
+        b[k] = (float)((double)world[2][k] - (double)world[0][k]);
+      }
+      cr[0] = (float)((double)a[1] * b[2] - (double)a[2] * b[1]);
+      cr[1] = (float)((double)a[2] * b[0] - (double)a[0] * b[2]);
+      cr[2] = (float)((double)a[0] * b[1] - (double)a[1] * b[0]);
+      double area = 0.5 * std::hypot((double)cr[0], (double)cr[1], (double)cr[2]);
+      if (!(area > 1e-10) || !std::isfinite(area)) continue;
+      if (n < capacity) {
+        float *o = out + (size_t)n * 12;
+        for (int corner = 0; corner < 3; corner++)
+          for (int k = 0; k < 3; k++) o[corner * 3 + k] = world[corner][k];
+        o[9] = (float)area;
+        o[10] = (float)instanceMaterial[i];
+        o[11] = 0.0f;
+        weight[n] = area * lum;
+      }
+      n++;
+    }
+  }
+  return n;
+}
+
+// sampleEmissiveTriangleLight (sampleLight.wgsl.ts:316-363).
+static void sample_emissive_triangle(const Ctx &c, uint32_t index, V3 p, float rx, float ry, LightSample &r) {
+  const float *t = c.s.emissive + (size_t)index * 12;
+  V3 p0(t[0], t[1], t[2]), p1(t[3], t[4], t[5]), p2(t[6], t[7], t[8]);
+  float area = t[9];
+  uint32_t material = (uint32_t)t[10];
+  float sqrtU = std::sqrt(rx);
+  V3 b(1.0f - sqrtU, sqrtU * (1.0f - ry), sqrtU * ry);
+  V3 lightPosition = b.x * p0 + b.y * p1 + b.z * p2;
+  V3 normal = normalize(cross(p1 - p0, p2 - p0));
+  r.direction = lightPosition - p;
+  r.dist = length(r.direction);
+  if (r.dist <= EPS) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
+  r.direction = r.direction / r.dist;
+  r.normal = normal;
+  float cosLight = dot(normal, -r.direction);
+  if (cosLight <= 0.0f) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
+  const float *m = c.s.materials + (size_t)material * 32;
+  r.emission = V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
+  r.selectionPmf = c.T.emissivePmf[index];
+  r.pdf = r.selectionPmf * (r.dist * r.dist) / mxf(area * cosLight, 1e-8f);
+}
+
 // sampleLightChoiceTable (sampleLight.wgsl.ts:365-420) with the candidate's
 // stratified selection number (:298-308).
 static void sample_light_candidate(const Ctx &c, V3 p, float rx, float ry, float rz, uint32_t cand,
@@ -961,6 +1039,9 @@ static void sample_light_candidate(const Ctx &c, V3 p, float rx, float ry, float
     light_sample_analytic(c.s.lights + (size_t)c.T.index[sel] * 16, p, rx, ry, ls);
     ls.selectionPmf = c.T.lightPmf[c.T.index[sel]];
     ls.pdf *= ls.selectionPmf;
+  } else if (c.T.type[sel] == 2) {
+    ls.sourceType = 1;
+    sample_emissive_triangle(c, (uint32_t)c.T.index[sel], p, rx, ry, ls);
   }
 }
 
@@ -1227,6 +1308,14 @@ static bool finish_direct_light(const Ctx &c, const Surface &si, const PathState
   float amount = nl < cutoff ? clampf(2.0f - (gl + nl) / cutoff, 0.0f, 1.0f) : clampf(1.0f - gl / cutoff, 0.0f, 1.0f);
   origin = origin + si.shadowOffset * amount;
   float visDist = ls.dist - EPS;
+  V3 visDir = lightDir;
+  if (ls.sourceType == 1u) {  // sampleMaterial.wgsl.ts:98-110: end EPS off the emitter along its normal
+    V3 lightEndpoint = si.position + ls.direction * ls.dist;
+    V3 offsetLightEndpoint = lightEndpoint + EPS * ls.normal;
+    V3 toOffsetLight = offsetLightEndpoint - origin;
+    visDist = length(toOffsetLight);
+    visDir = normalize(toOffsetLight);
+  }
   if ((cosThetaL < 0.0f) & (ls.sourceType == 0u)) {
     float u, v;
     environment_uv(lightDir, u, v);
@@ -1236,7 +1325,7 @@ static bool finish_direct_light(const Ctx &c, const Surface &si, const PathState
   if (mis > 0.0f) {
     V3 un = Li * f / lightPdf * path.beta;
     w.risTarget = mxf(dot(vmax(un * selPmf, V3(0.0f)), K_LUM), 0.0f);
-    w.ro = origin; w.rd = lightDir; w.maxDistance = visDist; w.triangleIndex = si.triangleIndex;
+    w.ro = origin; w.rd = visDir; w.maxDistance = visDist; w.triangleIndex = si.triangleIndex;
     w.contribution = mis * un;
     w.lightSelectionPmf = selPmf;
     w.valid = true;
@@ -1333,7 +1422,23 @@ static void sample_material(Ctx &c, Sampler &smp, Surface &si, Mat &ma, PathStat
   ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
   V3 ff = ndv >= 0.0f ? N : -N;
   path.scatteringIsDelta = false;
-  // Surface emission: the configurations have no emissive materials.
+  // Surface emission (sampleMaterial.wgsl.ts:412-422) with emissiveHitMisWeight (:279-303).
+  {
+    float cosLight = dot(si.faceNormal, -path.rd);
+    float weight = 1.0f;
+    const float *m = c.s.materials + (size_t)si.materialIndex * 32;
+    if (cosLight <= 0.0f && m[M_DOUBLE_SIDED] < 0.5f) {
+      weight = 0.0f;
+    } else if (!(path.fromEye | ((float)c.T.totalWeight <= 0.0f))) {  // rtUniform.lightWeightTotal is f32
+      float samplingLuminance = dot(ma.emission, K_LUM);
+      if (!((samplingLuminance <= 0.0f) | (cosLight <= 0.0f))) {
+        float hitDistance = length(si.position - path.ro);
+        float lightPdf = samplingLuminance / (float)c.T.totalWeight * hitDistance * hitDistance / mxf(cosLight, 1e-8f);
+        weight = bsdf_mis(c.s.candidates, path.scatteringPdf, lightPdf);
+      }
+    }
+    path.li = path.li + weight * ma.emission * path.beta;
+  }
   Prepared P = prepare(ma, -path.rd, ff);
   bool discrete = false;
   Scatter sc = sample_scattering(c, smp, si, ma, P, path, ff, discrete);

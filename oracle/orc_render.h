@@ -38,6 +38,9 @@ struct RenderScene {
   const float *env;           // RGBA f32 env map (nullptr: none)
   uint32_t envW, envH;
   const float *camera;        // viewProjInv[16], viewInv[16], aperture, focus
+  const float *emissive;      // 12 per emissive triangle: p0, p1, p2 (world), area, material, unused
+  const double *emissiveWeight;  // alias weight (area * emitted luminance) per emissive triangle
+  uint32_t emissiveCount;
   uint32_t width, height;
   uint32_t bounces, candidates, pattern, seed, logSPP, sampleOffset;
   float clamping;
@@ -55,12 +58,20 @@ struct HostTables {
   std::vector<float> prob;
   std::vector<int32_t> alias, type, index;
   std::vector<float> lightPmf;  // per analytic light
+  std::vector<float> emissivePmf;  // per emissive triangle
   float envPmf = 0.0f;
   double totalWeight = 0.0;
   std::vector<float> cdf;       // (W+2) x (H+1) x 2
   uint32_t cdfW = 0, cdfH = 0;
 };
 void build_tables(const RenderScene &s, HostTables &T);
+// collectEmissiveTriangles (lightSampling.ts:237-320) over instances in order:
+// writes up to `capacity` records of 12 floats and their alias weights,
+// returns the number of emissive triangles.
+uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,
+                            const uint32_t *indexOffsets, const uint32_t *triangleCounts, const uint32_t *instanceMesh,
+                            const double *transforms, const uint32_t *instanceMaterial, uint32_t instanceCount,
+                            const float *materials, float *out, double *weight, uint32_t capacity);
 // Primary ray of (pixel, sample index): origin xyz, direction xyz.
 void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py, uint32_t index,
                    uint32_t pattern, uint32_t seed, uint32_t logSPP, float out[6]);

@@ -155,6 +155,22 @@ def test_all_light_types(env, orc, kernel):
     _check(sums, counts, ref, 6)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_emissive_instances(env, orc, kernel):
+    """Emissive meshes are triangle lights (NEE over their world-space
+    triangles, MIS-weighted emission on BSDF hits) next to the environment
+    and an analytic light."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=6, freq=4, env_size=(64, 32))
+    sc.materials[2].emission = (4.0, 3.0, 2.0)
+    sc.materials[4].emission = (0.5, 2.0, 6.0)
+    sc.materials[4].double_sided = True
+    sc.lights = [scene.SphereLight(position=(1.0, 4.0, 2.0), radius=0.3, intensity=8.0)]
+    sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, bounces=6, max_samples=8, kernel=kernel,
+                                 direct_light_candidates=4)
+    _check(sums, counts, ref, 6)
+
+
 def _resolve_expected(sums):
     """The output pass of WGPURayTraceOutput.ts:32-92 on the sums (bottom row
     first), evaluated per pixel in Python floats."""

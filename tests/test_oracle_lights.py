@@ -169,3 +169,66 @@ def test_mixed_light_table():
                        S.DirectionalLight(direction=(0, -1, 0), intensity=0.5).pack()])
     t = oracle.light_tables(lights)
     np.testing.assert_allclose(t["light_pmf"], np.array([1.0, 1.0, 2.0]) / 4.0, rtol=1e-6)
+
+
+# ---------------------------------------------------------------- emissive triangles
+def _emissive_quad_scene(le, half, h, face_down=True, double_sided=False, candidates_cam=(0.0, 0.6, 3.0)):
+    """Diffuse plane under an emissive square (two triangles) of radiance le at
+    height h; the square faces down (its geometric normal -y) unless
+    face_down is False."""
+    s = S.Scene()
+    s.materials = [S.Material(base_color=(A, A, A), specular_weight=0.0),
+                   S.Material(base_color=(0.0, 0.0, 0.0), specular_weight=0.0, emission=(le, le, le),
+                              double_sided=double_sided)]
+    p, i, n = S.quad((-50, 0, 50), (50, 0, 50), (50, 0, -50), (-50, 0, -50))
+    s.add_instance(s.add_mesh(p, i, n), np.eye(4).reshape(16), 0)
+    pts = np.array([[-half, h, -half], [half, h, -half], [half, h, half], [-half, h, half]], np.float32)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32) if face_down else np.array([0, 2, 1, 0, 3, 2], np.uint32)
+    nrm = np.tile(np.array([[0.0, -1.0 if face_down else 1.0, 0.0]], np.float32), (4, 1))
+    s.add_instance(s.add_mesh(pts, idx, nrm), np.eye(4).reshape(16), 1)
+    s.camera = S.Camera(eye=candidates_cam, target=(0.0, 0.0, 0.0), fov=0.05)
+    return s
+
+
+def test_emissive_triangle_list_and_weights():
+    import oracle as O
+    s = _emissive_quad_scene(2.0, 0.5, 1.0)
+    tris, w = O.emissive_triangles(s)
+    assert tris.shape == (2, 12)
+    np.testing.assert_allclose(tris[:, 9], 0.5, rtol=1e-6)        # two halves of a unit square
+    np.testing.assert_allclose(w, 0.5 * 2.0, rtol=1e-6)           # area * luminance
+    assert np.all(tris[:, 10] == 1)
+    n = np.cross(tris[0, 3:6] - tris[0, 0:3], tris[0, 6:9] - tris[0, 0:3])
+    assert n[1] < 0  # faces the plane
+
+
+@pytest.mark.parametrize("candidates", [1, 4])
+def test_emissive_quad_matches_rect_light(orc, candidates):
+    """An emissive square of radiance L lights the plane like a rect light of
+    the same radiance: light samples of its triangles plus MIS-weighted BSDF
+    hits give the view-factor irradiance."""
+    import test_oracle_render as R
+    le, half, h = 3.0, 0.4, 1.0
+    got = _radiance(orc, _emissive_quad_scene(le, half, h), candidates=candidates)
+    want = A / math.pi * R._parallel_rect_irradiance(le, half, half, h)
+    assert abs(got / want - 1) < 0.01
+
+
+def test_emissive_seen_directly(orc):
+    """Camera rays take the emission with weight 1."""
+    le = 2.5
+    s = _emissive_quad_scene(le, 0.5, 1.0, candidates_cam=(0.0, 0.2, 0.0))
+    s.camera = S.Camera(eye=(0.0, 0.2, 0.0), target=(0.0, 1.0, 0.0), up=(0.0, 0.0, 1.0), fov=0.05)
+    out = orc.OracleRenderer(s, 4, 4, bounces=1, candidates=1, max_samples=4).render(0, 4)
+    np.testing.assert_allclose(out[..., 0:3].sum(axis=(0, 1)) / out[..., 11].sum(), le, rtol=1e-6)
+
+
+def test_emissive_back_face_only_when_double_sided(orc):
+    """A square facing away from the plane lights it only when double-sided;
+    then only BSDF hits reach its back (weight 1: no light sample covers it)."""
+    le, half, h = 3.0, 0.4, 1.0
+    assert _radiance(orc, _emissive_quad_scene(le, half, h, face_down=False), spp=512) == 0.0
+    import test_oracle_render as R
+    got = _radiance(orc, _emissive_quad_scene(le, half, h, face_down=False, double_sided=True), spp=16384)
+    want = A / math.pi * R._parallel_rect_irradiance(le, half, half, h)
+    assert abs(got / want - 1) < 0.02
