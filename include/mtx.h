@@ -252,7 +252,8 @@ typedef struct mtx_render_settings {
   uint32_t width, height;
   uint32_t bounces;                 /* RayTraceUniform.bounces */
   uint32_t direct_light_candidates; /* NEE RIS candidates (render.ts:21) */
-  uint32_t sampling_pattern;        /* 0 sobol, 1 zsobol (sampling.ts:3-8) */
+  uint32_t sampling_pattern;        /* 0 sobol, 1 zsobol, 2 random: pcg4d per path with a Halton camera
+                                       jitter per sample index (sampling.ts:3-8, rand.wgsl.ts:8-23) */
   uint32_t sequence_seed;
   uint32_t max_samples;             /* ZSobol sample budget */
   uint32_t sample_sequence_offset;  /* Render.ts:151 */

@@ -93,6 +93,7 @@ struct WfBuffers {
   float4 *s0, *s1, *s2;
   int32_t *s3;
   uint16_t *item;   // job slot of each path (rounds with several jobs)
+  uint4 *rng;       // pcg4d state of each path (random sampling pattern)
   uint32_t *counts;
   uint32_t nPaths;  // paths in flight this generation
   struct Owner {
@@ -101,6 +102,7 @@ struct WfBuffers {
     DevBuf<uint32_t> qA, qB, counts;
     DevBuf<int32_t> s3;
     DevBuf<uint16_t> item;
+    DevBuf<uint4> rng;
     uint32_t capacity = 0;
     void ensure(uint32_t paths);
     size_t bytes() const;

@@ -50,7 +50,8 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
     for (uint32_t k = 0; k < P.samples; k++) {
       if (acc.targets && (double)(count0 + k) >= (double)acc.targets[pix]) break;
       done++;
-      Smp smp = path_sampler(P, px, py, count0 + k + P.sampleOffset);
+      uint4 rngState;  // pcg4d state of the random pattern
+      Smp smp = path_sampler(P, px, py, count0 + k + P.sampleOffset, &rngState);
       PathRegs pr;
       init_path(P, smp, pr);
       for (int event = 0; event <= eventLimit; event++) {

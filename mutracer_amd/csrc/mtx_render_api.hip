@@ -381,7 +381,8 @@ static RenderParams params_from(const mtx_render_settings &s, const mtx_camera_d
               "mtx_render_settings.struct_size too small");
   MTX_REQUIRE(s.width > 0 && s.height > 0 && (uint64_t)s.width * s.height <= (1ull << 28), MTX_INVALID_ARGUMENT,
               "invalid target size");
-  MTX_REQUIRE(s.sampling_pattern <= 1, MTX_INVALID_ARGUMENT, "sampling_pattern must be 0 (sobol) or 1 (zsobol)");
+  MTX_REQUIRE(s.sampling_pattern <= 2, MTX_INVALID_ARGUMENT,
+              "sampling_pattern must be 0 (sobol), 1 (zsobol) or 2 (random)");
   MTX_REQUIRE(s.bounces <= 50, MTX_INVALID_ARGUMENT, "bounces above 50");
   MTX_REQUIRE(s.kernel <= MTX_KERNEL_MEGAKERNEL, MTX_INVALID_ARGUMENT, "unknown kernel mode");
   MTX_REQUIRE(std::isfinite(s.clamping) && s.clamping >= 0.0f, MTX_INVALID_ARGUMENT, "clamping must be finite");

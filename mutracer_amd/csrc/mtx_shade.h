@@ -97,8 +97,56 @@ __device__ uint32_t zsobol_index(uint32_t morton, uint32_t dim, uint32_t logSPP)
   return result;
 }
 
+// pcg4d (rand.wgsl.ts:13-19; Jarzynski and Olano, "Hash Functions for GPU
+// Rendering", JCGT 9(3), 2020): an LCG step on each lane of a 4 x u32 state,
+// two rounds of cross-lane multiply-adds around a 16-bit xorshift. Every step
+// is invertible, so the map is a bijection of the 128-bit state.
+__device__ __forceinline__ void pcg4d(uint4 &v) {
+  v.x = v.x * 1664525u + 1013904223u; v.y = v.y * 1664525u + 1013904223u;
+  v.z = v.z * 1664525u + 1013904223u; v.w = v.w * 1664525u + 1013904223u;
+  v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+  v.x ^= v.x >> 16; v.y ^= v.y >> 16; v.z ^= v.z >> 16; v.w ^= v.w >> 16;
+  v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+}
+
+// Radical inverse of `index` in `base` in FP64 with halton.ts:6-15's
+// operation order; the random pattern's camera jitter (WGPURenderingPipeline
+// .ts:534-541) is (halton(k, 2) - 0.5) / width, (halton(k, 3) - 0.5) / height
+// for pixel sample k, rounded to f32.
+__device__ __forceinline__ double halton(uint32_t index, uint32_t base) {
+  double result = 0.0, f = 1.0 / (double)base;
+  for (uint32_t i = index; i > 0u; i /= base) {
+    result = result + f * (double)(i % base);
+    f = f / (double)base;
+  }
+  return result;
+}
+
+constexpr uint32_t kPatternRandom = 2u;
+
+// Sample streams of one path. Patterns 0 and 1 address Owen-scrambled
+// Sobol / ZSobol points by dimension; pattern 2 (random) draws from the
+// path's pcg4d state in call order, so `rng` (the path's 16-byte state in
+// device memory or in the kernel's registers) advances on every draw.
 struct Smp {
   uint32_t px, py, width, index, seed, logSPP, pattern, bounce;
+  uint4 *rng;  // pattern 2 only
+  // rng_initialize (rand.wgsl.ts:8-11) as called at rayTrace.compute.ts:1193.
+  __device__ void seed_rng() const { *rng = make_uint4(px, py, index ^ shash(seed), px + py); }
+  // rand (rand.wgsl.ts:21-23): f32(s0.x) / f32(0xffffffff), whose divisor
+  // rounds to 2^32, so the result lies in [0, 1].
+  __device__ float rnd() const {
+    uint4 v = *rng;
+    pcg4d(v);
+    *rng = v;
+    return (float)v.x * (1.0f / 4294967296.0f);
+  }
+  // hashedSample (sobol.wgsl.ts:158-168) where its value cannot matter: the
+  // alpha-coverage test of completeVertex (rayTrace.compute.ts:533) against
+  // an opaque material. Only the random pattern's stream moves.
+  __device__ void hashed_draw() const {
+    if (pattern == kPatternRandom) (void)rnd();
+  }
   __device__ uint32_t bdim(uint32_t slot) const { return 3u + bounce * 64u + slot; }
   __device__ uint32_t ldim(uint32_t cand, uint32_t slot) const {
     if (cand == 0u) return slot == 2u ? bdim(8u) : bdim(slot);
@@ -128,17 +176,26 @@ struct Smp {
     }
   }
   __device__ float s1(uint32_t dim) const {
+    if (pattern == kPatternRandom) return kEpsLo + kEpsScale * rnd();
     float x, y;
     raw(dim, x, y, false);
     return kEpsLo + kEpsScale * x;
   }
   __device__ void s2(uint32_t dim, float &a, float &b) const {
+    if (pattern == kPatternRandom) {  // two sequential draws, x first (sobol.wgsl.ts:150-156)
+      a = s1(dim);
+      b = s1(dim);
+      return;
+    }
     float x, y;
     raw(dim, x, y, true);
     a = kEpsLo + kEpsScale * x;
     b = kEpsLo + kEpsScale * y;
   }
+  // lightCandidateRandom (rayTrace.compute.ts:251-261): the random pattern
+  // discards one draw to keep its stream, the value is always the hash.
   __device__ float candidate_random(uint32_t salt) const {
+    if (pattern == kPatternRandom) (void)rnd();
     uint32_t pixelID = px + py * width;
     uint32_t key = sseed(sseed(pixelID, index), seed);
     uint32_t dimension = sseed(bounce, salt) ^ 0xa511e9b3u;
@@ -1125,8 +1182,11 @@ struct ShadowReq {
   bool valid;
 };
 
-__device__ __forceinline__ Smp path_sampler(const RenderParams &P, uint32_t px, uint32_t py, uint32_t index) {
-  return Smp{px, py, P.width, index, P.seed, P.logSPP, P.pattern, 0u};
+// `rng` holds the path's pcg4d state when P.pattern is the random pattern;
+// it may be null otherwise.
+__device__ __forceinline__ Smp path_sampler(const RenderParams &P, uint32_t px, uint32_t py, uint32_t index,
+                                            uint4 *rng) {
+  return Smp{px, py, P.width, index, P.seed, P.logSPP, P.pattern, 0u, rng};
 }
 
 // sampleCircle (rayTrace.compute.ts:1148-1154): concentric square-to-disk
@@ -1140,16 +1200,27 @@ __device__ __forceinline__ void sample_circle(float ux, float uy, float &cx, flo
   cy = r * mtxfm::sinf(theta);
 }
 
-// main (rayTrace.compute.ts:1188-1205): Sobol pixel jitter, initCameraRay
-// (:591-610, browser projection-inverse form) and, with a positive aperture,
-// the thin lens: focus plane perpendicular to the view direction at the
-// focus distance, lens point = aperture * sampleCircle(CAMERA_LENS sample).
+// main (rayTrace.compute.ts:1188-1205): pixel jitter (Sobol CAMERA_PIXEL
+// sample, or the Halton offset of the random pattern), rng_initialize,
+// initCameraRay (:591-610, browser projection-inverse form) and, with a
+// positive aperture, the thin lens: focus plane perpendicular to the view
+// direction at the focus distance, lens point = aperture *
+// sampleCircle(CAMERA_LENS sample).
 __device__ void init_path(const RenderParams &P, const Smp &smp, PathRegs &pr) {
   const float Wf = (float)P.width, Hf = (float)P.height;
   float uvx = ((float)smp.px + 0.5f) / Wf, uvy = ((float)smp.py + 0.5f) / Hf;
   float jx, jy;
-  smp.s2(0u, jx, jy);
-  float cx = uvx + (jx - 0.5f) / Wf, cy = uvy + (jy - 0.5f) / Hf;
+  if (smp.pattern == kPatternRandom) {
+    jx = (float)((halton(smp.index, 2u) - 0.5) / (double)P.width);
+    jy = (float)((halton(smp.index, 3u) - 0.5) / (double)P.height);
+    smp.seed_rng();
+  } else {
+    float sx, sy;
+    smp.s2(0u, sx, sy);
+    jx = (sx - 0.5f) / Wf;
+    jy = (sy - 0.5f) / Hf;
+  }
+  float cx = uvx + jx, cy = uvy + jy;
   float nx = cx * 2.0f - 1.0f, ny = cy * 2.0f - 1.0f;
   float o4[4], c4[4];
 #pragma unroll
@@ -1225,6 +1296,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       if (ma.transmissionWeight < 0.001f) pr.flags |= kNotTransparent;
     }
   } else {
+    smp.hashed_draw();
     for (uint32_t i = 0; i < S.lightCount; i++) {
       LSample ls;
       float d = intersect_light(S.lights + (size_t)i * 16, pr.ro, pr.rd, ls);

@@ -63,13 +63,14 @@ __device__ __forceinline__ void path_pixel(const JobSlot &J, uint32_t p, uint32_
   s = rel - local * J.params.samples;
 }
 
-// Sampler of path p: pixel and sample index within its job.
-__device__ __forceinline__ Smp path_smp(const JobSlot &J, uint32_t p) {
+// Sampler of path p: pixel and sample index within its job; the random
+// pattern's pcg4d state lives in W.rng[p] between kernels.
+__device__ __forceinline__ Smp path_smp(const JobSlot &J, const WfBuffers &W, uint32_t p) {
   const RenderParams &P = J.params;
   uint32_t local, s;
   path_pixel(J, p, local, s);
   const uint32_t pix = P.pixelBase + local;
-  return path_sampler(P, pix % P.width, pix / P.width, J.acc.count[pix] + P.sampleOffset + s);
+  return path_sampler(P, pix % P.width, pix / P.width, J.acc.count[pix] + P.sampleOffset + s, W.rng + p);
 }
 
 __device__ __forceinline__ void store_path(const WfBuffers &W, uint32_t p, const PathRegs &pr) {
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(256) k_wf_generate(const WfLaunch L, WfBuffers
       const JobSlot &J = kMulti ? L.jobs[j] : L.one;
       if (select) alive = below_target(J, p);
       if (alive) {
-        Smp smp = path_smp(J, p);
+        Smp smp = path_smp(J, W, p);
         PathRegs pr;
         init_path(J.params, smp, pr);
         store_path(W, p, pr);
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaun
     if (active) {
       p = q[i];
       const JobSlot &J = job_of<kMulti>(L, W, p);
-      Smp smp = path_smp(J, p);
+      Smp smp = path_smp(J, W, p);
       const GpuScene &S = J.scene;
       const RenderParams &P = J.params;
       const float4 o = W.o[p], d = W.d[p], l = W.li[p], b = W.beta[p], al = W.alb[p], nr = W.nrm[p];
@@ -320,7 +321,7 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
     const RenderParams &P = J.params;
     const AccelView av = *S.accel;
     const Lut lut{S.lut};
-    Smp smp = path_smp(J, p);
+    Smp smp = path_smp(J, W, p);
     PathRegs pr;
     const float4 o = W.o[p], d = W.d[p], l = W.li[p], b = W.beta[p], al = W.alb[p], nr = W.nrm[p];
     pr.ro = f3of(o); pr.scatPdf = o.w;
@@ -503,19 +504,19 @@ void WfBuffers::Owner::ensure(uint32_t paths) {
   if (paths <= capacity) return;
   o.alloc(paths); d.alloc(paths); li.alloc(paths); beta.alloc(paths); alb.alloc(paths); nrm.alloc(paths);
   hitA.alloc(paths); hitB.alloc(paths); qA.alloc(paths); qB.alloc(paths);
-  s0.alloc(paths); s1.alloc(paths); s2.alloc(paths); s3.alloc(paths); item.alloc(paths);
+  s0.alloc(paths); s1.alloc(paths); s2.alloc(paths); s3.alloc(paths); item.alloc(paths); rng.alloc(paths);
   if (!counts.ptr) counts.alloc(kWfCounters);
   capacity = paths;
 }
 size_t WfBuffers::Owner::bytes() const {
   return o.bytes() * 6 + hitA.bytes() + hitB.bytes() + qA.bytes() * 2 + s0.bytes() * 3 + s3.bytes() + item.bytes() +
-         counts.bytes();
+         rng.bytes() + counts.bytes();
 }
 WfBuffers WfBuffers::Owner::view(uint32_t nPaths) const {
   WfBuffers W;
   W.o = o.ptr; W.d = d.ptr; W.li = li.ptr; W.beta = beta.ptr; W.alb = alb.ptr; W.nrm = nrm.ptr;
   W.hitA = hitA.ptr; W.hitB = hitB.ptr; W.qA = qA.ptr; W.qB = qB.ptr;
-  W.s0 = s0.ptr; W.s1 = s1.ptr; W.s2 = s2.ptr; W.s3 = s3.ptr; W.item = item.ptr;
+  W.s0 = s0.ptr; W.s1 = s1.ptr; W.s2 = s2.ptr; W.s3 = s3.ptr; W.item = item.ptr; W.rng = rng.ptr;
   W.counts = counts.ptr;
   W.nPaths = nPaths;
   return W;
@@ -704,8 +705,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       static_assert(sizeof(GpuScene) == 112 && sizeof(RenderParams) % 4 == 0, "key layouts");
       key = fnv(key, &L.one.scene, sizeof L.one.scene);
       key = fnv(key, &L.one.params, sizeof L.one.params);
-      const void *ptrs[7] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts,
-                             L.one.acc.targets};
+      const void *ptrs[8] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts,
+                             L.one.acc.targets, W.rng};
       key = fnv(key, ptrs, sizeof ptrs);
       key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
       key = fnv(key, &W.nPaths, sizeof W.nPaths);

@@ -116,6 +116,9 @@ def _declare(L):
     L.orc_sobol_d1.restype = u32
     L.orc_sobol_d1.argtypes = [u32]
     L.orc_sobol2d_batch.argtypes = [vp, u32, u32, vp]
+    L.orc_pcg4d.argtypes = [vp]
+    L.orc_halton.restype = C.c_double
+    L.orc_halton.argtypes = [u32, u32]
     L.orc_structured_sample.argtypes = [u32] * 8 + [vp]
     L.orc_bvh_build.restype = vp
     L.orc_bvh_build.argtypes = [vp, u32, u32]
@@ -181,6 +184,23 @@ def sobol2d_bits(indices, seed):
 
 def sobol2d(indices, seed):
     return sobol2d_bits(indices, seed).astype(np.float64) / 16777216.0
+
+
+def pcg4d(state):
+    """One pcg4d step (rand.wgsl.ts:13-19) of a 4-word state; returns the new state."""
+    v = np.array(state, dtype=np.uint32).reshape(4).copy()
+    lib().orc_pcg4d(_p(v))
+    return v
+
+
+def halton(index, base):
+    """Radical inverse of index in base, FP64 (halton.ts:6-15)."""
+    return lib().orc_halton(index & 0xFFFFFFFF, base)
+
+
+# Draw tags of OracleRenderer.draws: sample dimensions, or these flags | slot.
+DRAW_HASHED = 0x80000000
+DRAW_CANDIDATE = 0xC0000000
 
 
 def structured_sample(pattern, px, py, width, index, dim, log_spp, seed):
@@ -422,6 +442,9 @@ def _declare_render2(L):
                              u32, u32, vp, vp, vp, u32]
     L.orc_emissive_triangles.restype = u32
     L.orc_emissive_triangles.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32]
+    L.orc_path_draws.restype = u32
+    L.orc_path_draws.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, vp, u32,
+                                 vp, vp, u32]
 
 
 def emissive_triangles(scene):
@@ -467,7 +490,9 @@ class OracleRenderer:
         self.lights = scene.packed_lights()
         self.env = None if scene.env is None else np.ascontiguousarray(scene.env, np.float32)
         self.camera = scene.camera.pack(width, height)
-        from mutracer_amd.sampling import zsobol_log2spp
+        from mutracer_amd.sampling import sampling_pattern_code, zsobol_log2spp
+        if isinstance(pattern, str):
+            pattern = sampling_pattern_code(pattern)
         log = zsobol_log2spp((max_samples or 1) - 1 if pattern == 1 else (max_samples or 1))
         self.settings = np.array([width, height, bounces, candidates, pattern, seed & 0xFFFFFFFF, log, sample_offset],
                                  np.uint32)
@@ -475,6 +500,20 @@ class OracleRenderer:
         self.emissive, self.emissive_weight = emissive_triangles(scene)
         self.width, self.height = width, height
         self.rays = 0
+
+    def draws(self, px, py, sample, cap=4096):
+        """Tags of the random-pattern draws one path of (px, py, sample) makes,
+        in order: the dimension of each sample1d/sample2d draw, DRAW_HASHED |
+        slot, DRAW_CANDIDATE | salt."""
+        out = np.zeros(cap, np.uint32)
+        env = self.env
+        ew, eh = (0, 0) if env is None else (env.shape[1], env.shape[0])
+        n = lib().orc_path_draws(self.trace.h, _p(self.normals), _p(self.materials), self.materials.shape[0],
+                                 _p(self.lights), self.lights.shape[0], None if env is None else _p(env), ew, eh,
+                                 _p(self.camera), _p(self.settings), self.clamping, px, py, sample, _p(out), cap,
+                                 _p(self.emissive), _p(self.emissive_weight), self.emissive.shape[0])
+        assert n <= cap, "draw log overflow"
+        return out[:n].copy()
 
     def render(self, sample_start, count, x0=0, y0=0, w=None, h=None):
         w = self.width if w is None else w

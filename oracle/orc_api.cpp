@@ -23,6 +23,8 @@ uint32_t orc_owen(uint32_t x, uint32_t s) { return owen(x, s); }
 uint32_t orc_morton2(uint32_t x, uint32_t y) { return morton2(x, y); }
 uint32_t orc_zsobol_index(uint32_t m, uint32_t dim, uint32_t logSPP) { return zsobol_index(m, dim, logSPP); }
 uint32_t orc_sobol_d1(uint32_t i) { return sobol_d1(i); }
+void orc_pcg4d(uint32_t *v) { pcg4d(v); }
+double orc_halton(uint32_t index, uint32_t base) { return halton(index, base); }
 
 void orc_sobol2d_batch(const uint32_t *index, uint32_t n, uint32_t seed, uint32_t *out) {
   for (uint32_t i = 0; i < n; i++) sobol2d_bits(index[i], seed, out + 2 * i);
@@ -196,11 +198,10 @@ uint32_t orc_emissive_triangles(const float *positions, const uint32_t *vertexOf
                             transforms, instanceMaterial, instanceCount, materials, out, weight, capacity);
 }
 
-uint64_t orc_render(void *trace, const float *normals, const float *materials, uint32_t materialCount,
-                    const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
-                    const float *camera, const uint32_t *settings, float clamping, uint32_t x0, uint32_t y0,
-                    uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out,
-                    const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
+static RenderScene render_scene(void *trace, const float *normals, const float *materials, uint32_t materialCount,
+                                const float *lights, uint32_t lightCount, const float *env, uint32_t envW,
+                                uint32_t envH, const float *camera, const uint32_t *settings, float clamping,
+                                const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
   RenderScene s{};
   s.emissive = emissive;
   s.emissiveWeight = emissiveWeight;
@@ -214,8 +215,30 @@ uint64_t orc_render(void *trace, const float *normals, const float *materials, u
   s.width = settings[0]; s.height = settings[1]; s.bounces = settings[2]; s.candidates = settings[3];
   s.pattern = settings[4]; s.seed = settings[5]; s.logSPP = settings[6]; s.sampleOffset = settings[7];
   s.clamping = clamping;
+  return s;
+}
+
+uint64_t orc_render(void *trace, const float *normals, const float *materials, uint32_t materialCount,
+                    const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
+                    const float *camera, const uint32_t *settings, float clamping, uint32_t x0, uint32_t y0,
+                    uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out,
+                    const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
+  RenderScene s = render_scene(trace, normals, materials, materialCount, lights, lightCount, env, envW, envH, camera,
+                               settings, clamping, emissive, emissiveWeight, emissiveCount);
   static_assert(sizeof(Accum) == 13 * sizeof(double), "Accum layout");
   return render(s, x0, y0, w, h, sampleStart, count, reinterpret_cast<Accum *>(out));
+}
+
+// Draw tags of one path of the random pattern (same scene arguments as
+// orc_render); returns the number of draws.
+uint32_t orc_path_draws(void *trace, const float *normals, const float *materials, uint32_t materialCount,
+                        const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
+                        const float *camera, const uint32_t *settings, float clamping, uint32_t px, uint32_t py,
+                        uint32_t sampleIndex, uint32_t *log, uint32_t cap, const float *emissive,
+                        const double *emissiveWeight, uint32_t emissiveCount) {
+  RenderScene s = render_scene(trace, normals, materials, materialCount, lights, lightCount, env, envW, envH, camera,
+                               settings, clamping, emissive, emissiveWeight, emissiveCount);
+  return path_draws(s, px, py, sampleIndex, log, cap);
 }
 
 }  // extern "C"

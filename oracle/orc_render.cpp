@@ -1498,15 +1498,27 @@ static void thin_lens(const float *cam, Sampler &smp, V3 &o, V3 &d) {
   d = normalize(focusPoint - origin);
 }
 
-// main (rayTrace.compute.ts:1188-1205): pixel-centre uv, Sobol jitter,
-// camera ray, thin lens.
+// main (rayTrace.compute.ts:1161-1205): pixel-centre uv, jitter, the
+// random pattern's rng_initialize, camera ray, thin lens. The jitter is the
+// CAMERA_PIXEL sample for Sobol/ZSobol; for the random pattern it is the
+// rtUniform.jitter the pipeline writes before each sample
+// (WGPURenderingPipeline.ts:534-541): Halton bases 2 and 3 of the sample
+// index, minus 0.5, over the image size, in FP64, stored as f32.
 static void primary_ray(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py,
                         Sampler &smp, V3 &o, V3 &d) {
   float W = (float)width, H = (float)height;
   float uvx = ((float)px + 0.5f) / W, uvy = ((float)py + 0.5f) / H;
-  float j[2];
-  smp.sample2d(CAMERA_PIXEL, j);
-  float jx = (j[0] - 0.5f) / W, jy = (j[1] - 0.5f) / H;
+  float jx, jy;
+  if (smp.pattern == 2u) {
+    jx = (float)((halton(smp.pixelSampleIndex, 2) - 0.5) / width);
+    jy = (float)((halton(smp.pixelSampleIndex, 3) - 0.5) / height);
+  } else {
+    float j[2];
+    smp.sample2d(CAMERA_PIXEL, j);
+    jx = (j[0] - 0.5f) / W;
+    jy = (j[1] - 0.5f) / H;
+  }
+  smp.rng_initialize();
   float cx = uvx + jx, cy = uvy + jy;
   camera_ray(camera, cx, cy, o, d);
   if (camera[32] > 0.0f) thin_lens(camera, smp, o, d);
@@ -1530,9 +1542,12 @@ void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_
   out[0] = o.x; out[1] = o.y; out[2] = o.z; out[3] = d.x; out[4] = d.y; out[5] = d.z;
 }
 
-static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleIndex, Accum &acc) {
+static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleIndex, Accum &acc,
+                            uint32_t *drawLog = nullptr, uint32_t drawCap = 0, uint32_t *drawCount = nullptr) {
   const RenderScene &s = c.s;
   Sampler smp = make_sampler(px, py, s.width, sampleIndex + s.sampleOffset, s.pattern, s.seed, s.logSPP);
+  smp.log = drawLog;
+  smp.logCap = drawCap;
   PathState path{};
   primary_ray(s.camera, s.width, s.height, px, py, smp, path.ro, path.rd);
   Aux aux{V3(0.0f), V3(0.0f), false};
@@ -1562,7 +1577,9 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
       }
       isNotTransparent = ma.transmissionWeight < 0.001f && ma.alpha > 0.99f;
     } else {
-      // completeVertex (rayTrace.compute.ts:514-589), opaque alpha.
+      // completeVertex (rayTrace.compute.ts:514-589); materials are opaque,
+      // so the alpha-coverage draw is always below alpha.
+      bool isAlphaOccluded = smp.hashed_sample(3u) < ma.alpha;
       for (uint32_t i = 0; i < s.lightCount; i++) {
         LightSample ls{};
         const float *L = s.lights + (size_t)i * 16;
@@ -1570,7 +1587,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
         ls.pdf *= c.T.lightPmf[i];
         V3 dd = si.position - path.ro;
         bool occluded = si.hit && d > length(dd);
-        if (d != INF && !occluded) {
+        if (d != INF && !(occluded && isAlphaOccluded)) {
           float mis = bsdf_mis(s.candidates, path.scatteringPdf, ls.pdf);
           path.li = path.li + mis * path.beta * ls.emission;
           path.alpha = 1.0f;
@@ -1646,6 +1663,19 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
   acc.normal[0] += aux.normal.x; acc.normal[1] += aux.normal.y; acc.normal[2] += aux.normal.z; acc.normal[3] += 1.0;
   double lum = (double)dot(li, K_LUM);
   acc.lum2 += lum * lum;
+  if (drawCount) *drawCount = smp.logLen;
+}
+
+uint32_t path_draws(const RenderScene &s, uint32_t px, uint32_t py, uint32_t sampleIndex, uint32_t *log,
+                    uint32_t cap) {
+  HostTables T;
+  build_tables(s, T);
+  build_env_cdf(s, T);
+  Ctx c{s, T, 0};
+  Accum acc{};
+  uint32_t n = 0;
+  integrate_pixel(c, px, py, sampleIndex, acc, log, cap, &n);
+  return n;
 }
 
 uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sampleStart,

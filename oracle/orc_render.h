@@ -83,5 +83,9 @@ void set_energy_lut(const float *lut);  // 32 x 2177 floats
 // Returns the number of rays traced (camera + continuation + shadow).
 uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                 uint32_t sampleStart, uint32_t count, Accum *out);
+// Integrates one path and records the tag of each random-pattern draw in
+// log (up to cap); returns the number of draws.
+uint32_t path_draws(const RenderScene &s, uint32_t px, uint32_t py, uint32_t sampleIndex, uint32_t *log,
+                    uint32_t cap);
 
 }  // namespace orc

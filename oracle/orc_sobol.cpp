@@ -125,13 +125,63 @@ void Sampler::structured_raw(uint32_t dim, float out[2]) const {
   out[1] = unit24(bits[1]);
 }
 
+// pcg4d: Jarzynski and Olano, "Hash Functions for GPU Rendering", JCGT 9(3)
+// 2020, as rand.wgsl.ts:13-19 writes it (wrapping u32 arithmetic).
+void pcg4d(uint32_t v[4]) {
+  for (int i = 0; i < 4; i++) v[i] = v[i] * 1664525u + 1013904223u;
+  v[0] += v[1] * v[3];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+  v[3] += v[1] * v[2];
+  for (int i = 0; i < 4; i++) v[i] ^= v[i] >> 16;
+  v[0] += v[1] * v[3];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+  v[3] += v[1] * v[2];
+}
+
+double halton(uint32_t index, uint32_t base) {
+  double result = 0.0;
+  double f = 1.0 / base;
+  uint32_t i = index;
+  while (i > 0) {
+    result = result + f * (double)(i % base);
+    i = i / base;
+    f = f / base;
+  }
+  return result;
+}
+
+// rng_initialize(coord, i32(pixelSampleIndex ^ sampleHash(sequenceSeed))).
+void Sampler::rng_initialize() {
+  s0[0] = px;
+  s0[1] = py;
+  s0[2] = pixelSampleIndex ^ sample_hash(sequenceSeed);
+  s0[3] = px + py;
+}
+
+float Sampler::rand(uint32_t tag) const {
+  pcg4d(s0);
+  if (log && logLen < logCap) log[logLen] = tag;
+  if (log) logLen++;
+  return (float)s0[0] / (float)0xffffffffu;
+}
+
 float Sampler::sample1d(uint32_t dim) const {
+  if (pattern == 2u) return remap(rand(dim));  // sobol.wgsl.ts:146
   float v[2];
   structured_raw(dim, v);
   return remap(v[0]);
 }
 
 void Sampler::sample2d(uint32_t dim, float out[2]) const {
+  if (pattern == 2u) {  // sobol.wgsl.ts:151-154: x then y
+    float x = sample1d(dim);
+    float y = sample1d(dim);
+    out[0] = x;
+    out[1] = y;
+    return;
+  }
   float v[2];
   structured_raw(dim, v);
   out[0] = remap(v[0]);
@@ -141,6 +191,7 @@ void Sampler::sample2d(uint32_t dim, float out[2]) const {
 float Sampler::hashed_sample(uint32_t slot) {
   uint32_t dim = sample_seed(sample_seed(samplingBounce, slot), hashedCounter) | 0x80000000u;
   hashedCounter += 1u;
+  if (pattern == 2u) return remap(rand(DRAW_HASHED | slot));  // sobol.wgsl.ts:161
   uint32_t pixelID = px + py * width;
   uint32_t seed = sample_seed(sample_seed(pixelID, dim), sequenceSeed);
   uint32_t bits[2];
@@ -149,6 +200,7 @@ float Sampler::hashed_sample(uint32_t slot) {
 }
 
 float Sampler::light_candidate_random(uint32_t salt) const {
+  if (pattern == 2u) (void)rand(DRAW_CANDIDATE | salt);  // rayTrace.compute.ts:256
   uint32_t pixelID = px + py * width;
   uint32_t key = sample_seed(sample_seed(pixelID, pixelSampleIndex), sequenceSeed);
   uint32_t dimension = sample_seed(samplingBounce, salt) ^ 0xa511e9b3u;

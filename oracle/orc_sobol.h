@@ -28,16 +28,34 @@ uint32_t morton2(uint32_t x, uint32_t y);         // sobol.wgsl.ts:53-59
 uint32_t zsobol_index(uint32_t morton, uint32_t dim, uint32_t logSPP); // :67-84
 uint32_t zsobol_log2spp(double maxSamples);       // sampling.ts:11-15
 
-// Per-invocation sampler state (sobol.wgsl.ts:19-22 private vars + uniforms).
+// pcg4d (rand.wgsl.ts:13-19) on a 4-word state, in place.
+void pcg4d(uint32_t v[4]);
+// halton (halton.ts:6-15): radical inverse of index in base, FP64.
+double halton(uint32_t index, uint32_t base);
+
+// Tags of the draws the random pattern records when Sampler::log is set:
+// sample1d/sample2d record their dimension (one tag per draw), hashed draws
+// DRAW_HASHED | slot, the discarded draw of lightCandidateRandom
+// DRAW_CANDIDATE | salt.
+enum : uint32_t { DRAW_HASHED = 0x80000000u, DRAW_CANDIDATE = 0xC0000000u };
+
+// Per-invocation sampler state (sobol.wgsl.ts:19-22 private vars + uniforms,
+// rand.wgsl.ts:6 s0).
 struct Sampler {
   uint32_t px, py, width;
   uint32_t pixelSampleIndex;
   uint32_t sequenceSeed;
   uint32_t zSobolLogSPP;
-  uint32_t pattern;        // 0 sobol, 1 zsobol
+  uint32_t pattern;        // 0 sobol, 1 zsobol, 2 random
   uint32_t bevelSamples;   // BEVEL_SAMPLES override constant (0 in the configs)
   uint32_t samplingBounce;
   uint32_t hashedCounter;
+  mutable uint32_t s0[4];  // pcg4d state of the random pattern
+  uint32_t *log;           // optional draw log (random pattern)
+  mutable uint32_t logLen, logCap;
+
+  void rng_initialize();                                     // rand.wgsl.ts:8-11 at rayTrace.compute.ts:1193
+  float rand(uint32_t tag) const;                            // rand.wgsl.ts:21-23
 
   uint32_t bounce_dim(uint32_t slot) const;                  // :88-90
   uint32_t light_dim(uint32_t candidate, uint32_t slot) const; // :91-97

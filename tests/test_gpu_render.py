@@ -41,7 +41,7 @@ def _both(orc, render, sc, w, h, spp, batches=None, **kw):
     tgt.synchronize()
     sums, counts = tgt.read_accumulators()
     okw = dict(bounces=kw.get("bounces", 8), candidates=kw.get("direct_light_candidates", 1),
-               pattern=1 if kw.get("pattern") == "zsobol" else 0, seed=kw.get("sequence_seed", 0),
+               pattern=kw.get("pattern", "sobol"), seed=kw.get("sequence_seed", 0),
                max_samples=kw.get("max_samples", spp), clamping=kw.get("clamping", 0.0),
                sample_offset=kw.get("sample_sequence_offset", 0))
     ref = orc.OracleRenderer(sc, w, h, **okw).render(0, spp)
@@ -128,6 +128,40 @@ def test_thin_lens(env, orc, kernel):
     sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, bounces=6, max_samples=8, kernel=kernel,
                                  direct_light_candidates=2)
     _check(sums, counts, ref, 6)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_random_pattern(env, orc, kernel):
+    """Random pattern: Halton jitter per sample index, one pcg4d stream per
+    path drawn in the reference's call order (lens, alpha coverage, NEE
+    candidates with the discarded replacement draw, BSDF, roulette), over
+    split launches and a sample offset."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=6, freq=6, env_size=(64, 32))
+    sc.camera.aperture = 0.3
+    sc.camera.focus_distance = 12.0
+    sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, batches=[2, 1, 3], bounces=8, max_samples=6,
+                                 kernel=kernel, pattern="random", sequence_seed=3, direct_light_candidates=3,
+                                 sample_sequence_offset=5)
+    _check(sums, counts, ref, 6)
+
+
+@pytest.mark.parametrize("tail_paths", [0, 1500])
+def test_random_pattern_tail_kernel(env, orc, tail_paths):
+    """The pcg4d state of each path passes between the wavefront kernels and
+    the tail kernel through device memory."""
+    torch, render, scene = env
+    sc = scene.cornell_box()
+    rs = render.RenderScene(sc)
+    t = render.RenderTarget(rs, 40, 30, bounces=8, direct_light_candidates=2, max_samples=4, pattern="random",
+                            sequence_seed=21)
+    t.set_tuning(tail_paths=tail_paths)
+    t.render(4)
+    sums, counts = t.read_accumulators()
+    ref = orc.OracleRenderer(sc, 40, 30, bounces=8, candidates=2, max_samples=4, pattern="random",
+                             seed=21).render(0, 4)
+    assert np.all(counts == 4)
+    assert np.array_equal(sums, ref)
 
 
 @pytest.mark.parametrize("kernel", KERNELS)

@@ -29,6 +29,7 @@ JOBS = [  # (scene, width, height, samples, kwargs)
     (2, 33, 17, 12, dict(bounces=6, direct_light_candidates=2, clamping=3.0)),
     (0, 24, 24, 9, dict(bounces=5, direct_light_candidates=4, pattern="zsobol", max_samples=16, sequence_seed=9)),
     (1, 50, 20, 3, dict(bounces=3, direct_light_candidates=1, sample_sequence_offset=11)),
+    (2, 30, 20, 4, dict(bounces=6, direct_light_candidates=3, pattern="random", sequence_seed=5)),
 ]
 
 
