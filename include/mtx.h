@@ -155,7 +155,7 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 
 
 /* ------------------------------------------------------------- rendering */
-/* Packed OpenPBR material: 32 floats at these offsets (reference parameter
+/* Packed OpenPBR material: MTX_MATERIAL_FLOATS floats at these offsets (reference parameter
  * names and defaults: src/scene/openPbrSchema.ts:32-69). */
 #define MTX_MAT_BASE_WEIGHT 0
 #define MTX_MAT_BASE_COLOR 1          /* 3 floats */
@@ -176,7 +176,10 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
                                          instances with emitting materials are triangle lights */
 #define MTX_MAT_DOUBLE_SIDED 26
 #define MTX_MAT_SPECULAR_ANISOTROPY 27
-#define MTX_MATERIAL_FLOATS 32
+#define MTX_MAT_FUZZ_WEIGHT 28
+#define MTX_MAT_FUZZ_COLOR 29         /* 3 floats */
+#define MTX_MAT_FUZZ_ROUGHNESS 32
+#define MTX_MATERIAL_FLOATS 48        /* floats 33..47 are reserved (zero) */
 
 /* Packed rect light (RectLight.ts:5-22): 16 floats. */
 /* Packed analytic light, the fields of the reference's Light record
@@ -471,7 +474,7 @@ int32_t mtx_denoiser_destroy(mtx_handle denoiser);
 /* ------------------------------------------------------------ diagnostics */
 /* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on
  * the device: kind 0 sin(x), 1 cos(x), 2 pow(x, y), 3 atan2(x, y) with x the
- * ordinate, 4 acos(x). Device pointers, n elements, asynchronous on `stream`.
+ * ordinate, 4 acos(x), 5 exp(x). Device pointers, n elements, asynchronous on `stream`.
  * No reference counterpart: it lets hosts check that device and host builds
  * of these functions agree bit for bit. */
 int32_t mtx_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n, void *stream);

@@ -30,7 +30,7 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define MTX_FM_HD __host__ __device__ inline
-/* The five public functions stay out of line: inlined at every call site of
+/* The six public functions stay out of line: inlined at every call site of
  * the shading code their f64 temporaries multiply the kernels' register
  * pressure (and spills); as calls they cost a few instructions each. */
 #define MTX_FM_CALL __host__ __device__ inline __attribute__((noinline))
@@ -167,6 +167,13 @@ MTX_FM_HD double log_d(double x) {
   double lm = (2.0 * s) + (2.0 * s) * (z * p);
   double de = (double)e;
   return (de * kLn2Hi + lm) + de * kLn2Lo;
+}
+
+MTX_FM_CALL float expf(float xf) {
+  double x = xf;
+  if (is_nan(x)) return ffrom(0x7fc00000u);
+  if (is_inf(x)) return x > 0.0 ? ffrom(0x7f800000u) : 0.0f;
+  return (float)exp_d(x);
 }
 
 MTX_FM_CALL float powf(float xf, float yf) {

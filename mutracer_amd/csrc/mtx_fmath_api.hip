@@ -14,7 +14,8 @@ __global__ void k_math_eval(uint32_t kind, const float *__restrict__ x, const fl
       case 1: r = mtxfm::cosf(a); break;
       case 2: r = mtxfm::powf(a, b); break;
       case 3: r = mtxfm::atan2f(a, b); break;
-      default: r = mtxfm::acosf(a); break;
+      case 4: r = mtxfm::acosf(a); break;
+      default: r = mtxfm::expf(a); break;
     }
     out[i] = r;
   }
@@ -25,7 +26,7 @@ __global__ void k_math_eval(uint32_t kind, const float *__restrict__ x, const fl
 extern "C" int32_t mtx_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n,
                                  void *stream) {
   try {
-    MTX_REQUIRE(kind <= 4, MTX_INVALID_ARGUMENT, "kind must be 0..4");
+    MTX_REQUIRE(kind <= 5, MTX_INVALID_ARGUMENT, "kind must be 0..5");
     MTX_REQUIRE(n == 0 || (x && y && out), MTX_INVALID_ARGUMENT, "NULL buffer");
     if (!n) return MTX_OK;
     uint64_t blocks = (n + 255) / 256;

@@ -113,7 +113,7 @@ struct WfBuffers {
 // One job's share of a wavefront generation: its scene, parameters (samples =
 // samples per pixel in this generation), accumulators, and where its paths
 // and pixels sit in the generation's concatenated path / pixel ranges. Paths
-// of a job are ordered sample-major: path = pathBase + s * pixelCount + local.
+// of a job are ordered pixel-major: path = pathBase + local * samples + s.
 struct JobSlot {
   GpuScene scene;
   RenderParams params;

@@ -272,9 +272,9 @@ __device__ __forceinline__ f3 to_local(const Basis &B, f3 d) { return f3{dot(d, 
 
 // ---------------------------------------------------------------- materials
 struct Mat {
-  f3 baseColor, specularColor, transmissionColor, coatColor, emission;
+  f3 baseColor, specularColor, transmissionColor, coatColor, emission, fuzzColor;
   float baseWeight, diffuseRoughness, baseMetalness, specularWeight, specularRoughness, specularIor;
-  float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening;
+  float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening, fuzzWeight, fuzzRoughness;
   float relativeIor, alphaX, alphaY;
   bool inside;
 };
@@ -308,6 +308,9 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.coatRoughness = clampf(m[20], kRoughnessMin, 1.0f);
   ma.coatIor = mxf(m[21], 1.0f);
   ma.coatDarkening = clampf(m[22], 0.0f, 1.0f);
+  ma.fuzzWeight = clampf(m[MTX_MAT_FUZZ_WEIGHT], 0.0f, 1.0f);
+  ma.fuzzColor = vmaxs(f3{m[MTX_MAT_FUZZ_COLOR], m[MTX_MAT_FUZZ_COLOR + 1], m[MTX_MAT_FUZZ_COLOR + 2]}, 0.0f);
+  ma.fuzzRoughness = clampf(m[MTX_MAT_FUZZ_ROUGHNESS], 0.0f, 1.0f);
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
     ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
@@ -460,10 +463,33 @@ __device__ f3 coat_passage(const Mat &m, float c) {
 
 enum { LF = 0, LC = 1, LM = 2, LS = 3, LT = 4, LD = 5, LSS = 6, LN = 7 };
 
+// sqrt(2 * PI) with PI = 3.14159265359 as WGSL's abstract-float constant
+// folding computes it (in f64), rounded to f32.
+constexpr float kSqrtTwoPi = KF(2.506628274631083);
+
+// openpbr_fuzz_directional_albedo (openPbrBsdf.wgsl.ts:595-607): the fitted
+// albedo of the fuzz lobe, a Gaussian in the view cosine whose scale, mean
+// and offset are rational functions of the roughness. pow(z, 2) is written
+// z * z, its correctly rounded value.
+__device__ float fuzz_albedo(float viewCosine, float roughness) {
+  const float x = clampf(viewCosine, 0.0f, 1.0f);
+  const float y = clampf(roughness, 0.01f, 1.0f);
+  const float scale = y * (0.0206607f + 1.58491f * y) / (0.0379424f + y * (1.32227f + y));
+  const float mean = y * (-0.193854f + y * (-1.14885f + y * (1.7932f - 0.95943f * y * y))) / (0.046391f + y);
+  const float offset = y * (0.000654023f + (-0.0207818f + 0.119681f * y) * y) / (1.26264f + y * (-1.92021f + y));
+  const float z = (x - mean) / scale;
+  const float gaussian = mtxfm::expf(-0.5f * (z * z)) / (scale * kSqrtTwoPi);
+  return mxf(gaussian + offset, 0.0f);
+}
+
 struct Prep {
   Mat m;
   Basis basis;
   f3 v, coatBaseScale, metalMsScale, msRefl, msTrans;
+  // Fuzz layer, fixed by the view direction: the factor on every lobe
+  // under it and its own response weight * color * albedo / pi.
+  f3 fuzzResp;
+  float fuzzScale;
   float specEta, refrEta, transReflIor, specAlpha, dsComp;
   float metalMsFactor, metalMsProb, msViewLoss, msRatio, msProb;
   float w[LN], p[LN];
@@ -497,7 +523,7 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
   P.basis = make_basis(normal);
   P.v = to_local(P.basis, view);
   float diel = 1.0f - m.baseMetalness, opaqueDiel = diel * (1.0f - m.transmissionWeight);
-  P.w[LF] = 0.0f;
+  P.w[LF] = m.fuzzWeight;
   P.w[LC] = m.inside ? 0.0f : m.coatWeight;
   P.w[LM] = m.baseMetalness;
   P.w[LS] = diel;
@@ -527,6 +553,14 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
   f3 tint = m.transmissionColor;
   P.dsComp = opaque_ec(L, P.specEta, P.specAlpha, vc) / mxf(opaque_avg_ec(L, P.specEta, P.specAlpha), 1e-12f);
   P.p[LF] = 0.0f;
+  P.fuzzScale = 1.0f;
+  P.fuzzResp = splat(0.0f);
+  if (P.w[LF] > 0.0f) {
+    const float fa = fuzz_albedo(vc, m.fuzzRoughness);
+    P.p[LF] = P.w[LF] * lum_pbr(m.fuzzColor) * mxf(fa, 0.02f);
+    P.fuzzScale = mxf(0.0f, 1.0f - P.w[LF] * fa);
+    P.fuzzResp = P.w[LF] * (m.fuzzColor * fa * kInvPi);
+  }
   P.p[LC] = 0.0f;
   P.coatBaseScale = splat(1.0f);
   if ((m.coatWeight > 0.0f) && !m.inside) {
@@ -649,6 +683,12 @@ __device__ f3 eval_bsdf(const Prep &P, f3 lightDir, float &pdf, const Lut &L) {
     rough_alpha(m.coatRoughness, 0.0f, cax, cay);
     resp = resp + P.w[LC] * eval_refl(cax, cay, v, l, h, coatF, lp);
     pdf += P.p[LC] * lp;
+  }
+  if (reflection & (P.w[LF] > 0.0f)) {  // fuzz on top (openPbrBsdf.wgsl.ts:1400-1418)
+    const bool lit = (v.z > 0.0f) & (l.z > 0.0f);
+    resp = resp * P.fuzzScale;
+    resp = resp + (lit ? P.fuzzResp : splat(0.0f));
+    pdf += P.p[LF] * (lit ? l.z * kInvPi : 0.0f);
   }
   return resp * fabsf(l.z);
 }
@@ -927,7 +967,7 @@ __device__ void sample_emissive(const GpuScene &S, uint32_t index, f3 p, float r
   r.normal = normal;
   const float cosLight = dot(normal, -r.direction);
   if (cosLight <= 0.0f) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
-  const float *m = S.materials + (size_t)(uint32_t)c.w * 32;
+  const float *m = S.materials + (size_t)(uint32_t)c.w * MTX_MATERIAL_FLOATS;
   r.emission = f3{m[23], m[24], m[25]};
   r.selPmf = a.w;
   r.pdf = a.w * (r.dist * r.dist) / mxf(b.w * cosLight, 1e-8f);
@@ -1278,7 +1318,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
   Mat ma;
   if (h.hit) {
     complete_surface(S, h, si);
-    ma = load_material(S.materials + (size_t)si.material * 32, si.color);
+    ma = load_material(S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS, si.color);
   }
   pr.curTri = si.tri;
   pr.curSlot = si.slot;
@@ -1361,7 +1401,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
     {
       const float cosLight = dot(si.faceNormal, -rd);
       float w = 1.0f;
-      if (cosLight <= 0.0f && S.materials[(size_t)si.material * 32 + 26] < 0.5f) {
+      if (cosLight <= 0.0f && S.materials[(size_t)si.material * MTX_MATERIAL_FLOATS + MTX_MAT_DOUBLE_SIDED] < 0.5f) {
         w = 0.0f;
       } else if (!((pr.flags & kFromEye) | (S.lightWeightTotal <= 0.0f))) {
         const float lum = dot(ma.emission, f3{0.2126f, 0.7152f, 0.0722f});

@@ -15,11 +15,13 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# Packed material layout (32 floats), see include/mtx.h mtx_material_desc.
+# Packed material layout (MTX_MATERIAL_FLOATS = 48 floats), see include/mtx.h.
+MATERIAL_FLOATS = 48
 MAT_FIELDS = dict(base_weight=0, base_color=1, base_diffuse_roughness=4, base_metalness=5, specular_weight=6,
                   specular_color=7, specular_roughness=10, specular_ior=11, transmission_weight=12,
                   transmission_color=13, coat_weight=16, coat_color=17, coat_roughness=20, coat_ior=21,
-                  coat_darkening=22, emission=23, double_sided=26, specular_anisotropy=27)
+                  coat_darkening=22, emission=23, double_sided=26, specular_anisotropy=27, fuzz_weight=28,
+                  fuzz_color=29, fuzz_roughness=32)
 
 
 @dataclass
@@ -41,11 +43,14 @@ class Material:
     coat_roughness: float = 0.0
     coat_ior: float = 1.6
     coat_darkening: float = 1.0
+    fuzz_weight: float = 0.0
+    fuzz_color: tuple = (1.0, 1.0, 1.0)
+    fuzz_roughness: float = 0.5
     emission: tuple = (0.0, 0.0, 0.0)
     double_sided: bool = False
 
     def pack(self):
-        p = np.zeros(32, np.float32)
+        p = np.zeros(MATERIAL_FLOATS, np.float32)
         for k, off in MAT_FIELDS.items():
             v = getattr(self, k)
             if isinstance(v, tuple):

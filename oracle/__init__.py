@@ -34,8 +34,8 @@ def lib():
 
 
 def math_eval(kind, x, y=None):
-    """include/mtx_fmath.h on the host: kind in {"sin","cos","pow","atan2","acos"}."""
-    k = ["sin", "cos", "pow", "atan2", "acos"].index(kind)
+    """include/mtx_fmath.h on the host: kind in {"sin","cos","pow","atan2","acos","exp"}."""
+    k = ["sin", "cos", "pow", "atan2", "acos", "exp"].index(kind)
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
     out = np.empty_like(x)
@@ -117,6 +117,8 @@ def _declare(L):
     L.orc_sobol_d1.argtypes = [u32]
     L.orc_sobol2d_batch.argtypes = [vp, u32, u32, vp]
     L.orc_pcg4d.argtypes = [vp]
+    L.orc_fuzz_albedo.restype = C.c_float
+    L.orc_fuzz_albedo.argtypes = [C.c_float, C.c_float]
     L.orc_halton.restype = C.c_double
     L.orc_halton.argtypes = [u32, u32]
     L.orc_structured_sample.argtypes = [u32] * 8 + [vp]
@@ -191,6 +193,11 @@ def pcg4d(state):
     v = np.array(state, dtype=np.uint32).reshape(4).copy()
     lib().orc_pcg4d(_p(v))
     return v
+
+
+def fuzz_albedo(view_cosine, roughness):
+    """Fitted directional albedo of the OpenPBR fuzz lobe (f32)."""
+    return lib().orc_fuzz_albedo(view_cosine, roughness)
 
 
 def halton(index, base):

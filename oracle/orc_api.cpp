@@ -24,6 +24,7 @@ uint32_t orc_morton2(uint32_t x, uint32_t y) { return morton2(x, y); }
 uint32_t orc_zsobol_index(uint32_t m, uint32_t dim, uint32_t logSPP) { return zsobol_index(m, dim, logSPP); }
 uint32_t orc_sobol_d1(uint32_t i) { return sobol_d1(i); }
 void orc_pcg4d(uint32_t *v) { pcg4d(v); }
+float orc_fuzz_albedo(float viewCosine, float roughness) { return fuzz_albedo(viewCosine, roughness); }
 double orc_halton(uint32_t index, uint32_t base) { return halton(index, base); }
 
 void orc_sobol2d_batch(const uint32_t *index, uint32_t n, uint32_t seed, uint32_t *out) {
@@ -261,7 +262,8 @@ extern "C" void orc_math_eval(uint32_t kind, const float *x, const float *y, flo
       case 1: out[i] = mtxfm::cosf(x[i]); break;
       case 2: out[i] = mtxfm::powf(x[i], y[i]); break;
       case 3: out[i] = mtxfm::atan2f(x[i], y[i]); break;
-      default: out[i] = mtxfm::acosf(x[i]); break;
+      case 4: out[i] = mtxfm::acosf(x[i]); break;
+      default: out[i] = mtxfm::expf(x[i]); break;
     }
   }
 }

@@ -117,6 +117,9 @@ struct Mat {
   float coatWeight;
   V3 coatColor;
   float coatRoughness, coatIor, coatDarkening;
+  float fuzzWeight;
+  V3 fuzzColor;
+  float fuzzRoughness;
   V3 emission;
   float alpha;
   bool inside;
@@ -154,6 +157,9 @@ static Mat get_material(const float *m, V3 instanceColor) {
   ma.coatRoughness = clampf(m[M_COAT_ROUGHNESS], ROUGHNESS_MIN, 1.0f);
   ma.coatIor = mxf(m[M_COAT_IOR], 1.0f);
   ma.coatDarkening = clampf(m[M_COAT_DARKENING], 0.0f, 1.0f);
+  ma.fuzzWeight = clampf(m[M_FUZZ_WEIGHT], 0.0f, 1.0f);
+  ma.fuzzColor = vmaxs(V3(m[M_FUZZ_COLOR], m[M_FUZZ_COLOR + 1], m[M_FUZZ_COLOR + 2]), 0.0f);
+  ma.fuzzRoughness = clampf(m[M_FUZZ_ROUGHNESS], 0.0f, 1.0f);
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness;
     float c2 = ma.coatRoughness * ma.coatRoughness;
@@ -353,6 +359,30 @@ static V3 coat_base_scale(const Prepared &P, float lightCos) {
 
 // openpbr_prepare (openPbrBsdf.wgsl.ts:853-1078) without fuzz, subsurface,
 // thin walls, thin film and anisotropy frames.
+// openpbr_fuzz_directional_albedo (openPbrBsdf.wgsl.ts:595-607).
+static float fuzz_directional_albedo(float viewCosine, float roughness) {
+  static const float sqrtTwoPi = (float)std::sqrt(2.0 * 3.14159265359);
+  float x = clampf(viewCosine, 0.0f, 1.0f);
+  float y = clampf(roughness, 0.01f, 1.0f);
+  float scale = y * (0.0206607f + 1.58491f * y) / (0.0379424f + y * (1.32227f + y));
+  float mean = y * (-0.193854f + y * (-1.14885f + y * (1.7932f - 0.95943f * y * y))) / (0.046391f + y);
+  float offset = y * (0.000654023f + (-0.0207818f + 0.119681f * y) * y) / (1.26264f + y * (-1.92021f + y));
+  float z = (x - mean) / scale;
+  float gaussian = mtxfm::expf(-0.5f * (z * z)) / (scale * sqrtTwoPi);  // pow(z, 2) correctly rounded
+  return mxf(gaussian + offset, 0.0f);
+}
+
+float fuzz_albedo(float viewCosine, float roughness) { return fuzz_directional_albedo(viewCosine, roughness); }
+
+// openpbr_eval_fuzz (openPbrBsdf.wgsl.ts:609-625).
+static V3 eval_fuzz(const Mat &m, V3 v, V3 l, float &pdf) {
+  pdf = 0.0f;
+  if ((v.z <= 0.0f) | (l.z <= 0.0f)) return V3(0.0f);
+  float albedo = fuzz_directional_albedo(v.z, m.fuzzRoughness);
+  pdf = l.z * INVPI;
+  return m.fuzzColor * albedo * INVPI;
+}
+
 static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   Prepared P{};
   P.material = m;
@@ -362,7 +392,7 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   P.diffuseColor = m.baseColor;
   float dielBase = 1.0f - m.baseMetalness;
   float opaqueDiel = dielBase * (1.0f - m.transmissionWeight);
-  P.w[L_FUZZ] = 0.0f;
+  P.w[L_FUZZ] = m.fuzzWeight;
   P.w[L_COAT] = m.inside ? 0.0f : m.coatWeight;
   P.w[L_METAL] = m.baseMetalness;
   P.w[L_SPEC] = dielBase;
@@ -398,7 +428,7 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   float viewEC = opaque_energy_complement(P.specularEta, P.specularAlpha, vc);
   float avgEC = opaque_average_energy_complement(P.specularEta, P.specularAlpha);
   P.diffuseSpecularEnergyCompensation = viewEC / mxf(avgEC, 1e-12f);
-  P.p[L_FUZZ] = 0.0f;
+  P.p[L_FUZZ] = P.w[L_FUZZ] * lum_pbr(m.fuzzColor) * mxf(fuzz_directional_albedo(vc, m.fuzzRoughness), 0.02f);
   P.p[L_COAT] = 0.0f;
   P.coatBaseScale = V3(1.0f);
   if ((m.coatWeight > 0.0f) && !m.inside) {
@@ -528,6 +558,12 @@ static V3 eval_prepared(const Prepared &P, V3 lightDir, float &pdf) {
     roughness_to_alpha(m.coatRoughness, 0.0f, cax, cay);
     response = response + P.w[L_COAT] * eval_refl(cax, cay, v, l, h, coatF, lobePdf);
     pdf += P.p[L_COAT] * lobePdf;
+  }
+  if (reflection & (P.w[L_FUZZ] > 0.0f)) {
+    float fuzzAlbedo = fuzz_directional_albedo(fabsf(v.z), m.fuzzRoughness);
+    response = response * vmaxs(V3(1.0f - P.w[L_FUZZ] * fuzzAlbedo), 0.0f);
+    response = response + P.w[L_FUZZ] * eval_fuzz(m, v, l, lobePdf);
+    pdf += P.p[L_FUZZ] * lobePdf;
   }
   return response * fabsf(l.z);
 }
@@ -948,7 +984,7 @@ uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffset
                             const float *materials, float *out, double *weight, uint32_t capacity) {
   uint32_t n = 0;
   for (uint32_t i = 0; i < instanceCount; i++) {
-    const float *em = materials + (size_t)instanceMaterial[i] * 32 + M_EMISSION;
+    const float *em = materials + (size_t)instanceMaterial[i] * M_FLOATS + M_EMISSION;
     double lum = (double)em[0] * 0.2126 + (double)em[1] * 0.7152 + (double)em[2] * 0.0722;
     if (lum < 0) lum = 0;
     if (!(lum > 1e-8)) continue;
@@ -1007,7 +1043,7 @@ static void sample_emissive_triangle(const Ctx &c, uint32_t index, V3 p, float r
   r.normal = normal;
   float cosLight = dot(normal, -r.direction);
   if (cosLight <= 0.0f) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
-  const float *m = c.s.materials + (size_t)material * 32;
+  const float *m = c.s.materials + (size_t)material * M_FLOATS;
   r.emission = V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
   r.selectionPmf = c.T.emissivePmf[index];
   r.pdf = r.selectionPmf * (r.dist * r.dist) / mxf(area * cosLight, 1e-8f);
@@ -1426,7 +1462,7 @@ static void sample_material(Ctx &c, Sampler &smp, Surface &si, Mat &ma, PathStat
   {
     float cosLight = dot(si.faceNormal, -path.rd);
     float weight = 1.0f;
-    const float *m = c.s.materials + (size_t)si.materialIndex * 32;
+    const float *m = c.s.materials + (size_t)si.materialIndex * M_FLOATS;
     if (cosLight <= 0.0f && m[M_DOUBLE_SIDED] < 0.5f) {
       weight = 0.0f;
     } else if (!(path.fromEye | ((float)c.T.totalWeight <= 0.0f))) {  // rtUniform.lightWeightTotal is f32
@@ -1566,7 +1602,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
     intersect_scene(c, path, si, depth, event != 0);
     if (si.hit) {
       int32_t mi = si.materialIndex;
-      ma = get_material(s.materials + (size_t)mi * 32, instance_color(*s.trace, si.instanceIndex));
+      ma = get_material(s.materials + (size_t)mi * M_FLOATS, instance_color(*s.trace, si.instanceIndex));
     }
     if (event == 0) {
       if (si.hit) {

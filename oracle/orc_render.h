@@ -14,13 +14,14 @@
 
 namespace orc {
 
-// Packed OpenPBR material parameters (32 floats), shared input format.
+// Packed OpenPBR material parameters (M_FLOATS floats), shared input format.
 enum MatField {
   M_BASE_WEIGHT = 0, M_BASE_COLOR = 1, M_DIFFUSE_ROUGHNESS = 4, M_METALNESS = 5,
   M_SPEC_WEIGHT = 6, M_SPEC_COLOR = 7, M_SPEC_ROUGHNESS = 10, M_SPEC_IOR = 11,
   M_TRANS_WEIGHT = 12, M_TRANS_COLOR = 13, M_COAT_WEIGHT = 16, M_COAT_COLOR = 17,
   M_COAT_ROUGHNESS = 20, M_COAT_IOR = 21, M_COAT_DARKENING = 22, M_EMISSION = 23,
-  M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27
+  M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27, M_FUZZ_WEIGHT = 28, M_FUZZ_COLOR = 29, M_FUZZ_ROUGHNESS = 32,
+  M_FLOATS = 48
 };
 // Packed light (16 floats): type, position, u, v, emission, spread or radius,
 // area (the reference's Light record, rayTraceCore.wgsl.ts:228-248). Types are
@@ -78,6 +79,8 @@ void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_
 void build_env_cdf(const RenderScene &s, HostTables &T);
 
 void set_energy_lut(const float *lut);  // 32 x 2177 floats
+// Fitted directional albedo of the fuzz lobe (openPbrBsdf.wgsl.ts:595-607).
+float fuzz_albedo(float viewCosine, float roughness);
 // Renders samples [sampleStart, sampleStart+count) for the pixel rectangle
 // and adds them to out (one Accum per pixel of the rectangle, row-major).
 // Returns the number of rays traced (camera + continuation + shadow).

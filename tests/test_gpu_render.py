@@ -131,6 +131,21 @@ def test_thin_lens(env, orc, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+def test_fuzz_materials(env, orc, kernel):
+    """Fuzz layered over diffuse, metal, coated and glass bases."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=8, freq=6, env_size=(64, 32))
+    for k, m in enumerate(sc.materials):
+        if k % 2 == 0:
+            m.fuzz_weight = 0.3 + 0.1 * (k % 5)
+            m.fuzz_color = (0.9, 0.5 + 0.05 * k, 0.3)
+            m.fuzz_roughness = 0.2 + 0.15 * (k % 4)
+    sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, bounces=8, max_samples=6, kernel=kernel,
+                                 direct_light_candidates=2)
+    _check(sums, counts, ref, 6)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_random_pattern(env, orc, kernel):
     """Random pattern: Halton jitter per sample index, one pcg4d stream per
     path drawn in the reference's call order (lens, alpha coverage, NEE

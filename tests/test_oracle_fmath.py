@@ -53,6 +53,19 @@ def test_atan2_acos(orc, inputs):
     assert ulps(got, want).max() <= 1
 
 
+def test_exp(orc, inputs):
+    rng, _ = inputs
+    x = np.concatenate([rng.uniform(-10, 10, 300000), rng.uniform(-103, 88, 100000),
+                        rng.uniform(-1e-4, 1e-4, 20000)]).astype(np.float32)
+    got = orc.math_eval("exp", x)
+    want = np.exp(x.astype(np.float64)).astype(np.float32)
+    u = ulps(got, want)
+    assert u.max() <= 1 and (u == 0).mean() > 0.9999
+    sp = orc.math_eval("exp", np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 89.0, -104.0], np.float32))
+    assert sp[0] == 1 and sp[1] == 1 and sp[2] == np.inf and sp[3] == 0 and np.isnan(sp[4])
+    assert sp[5] == np.inf and sp[6] == 0
+
+
 def test_special_cases(orc):
     inf, nan = np.float32(np.inf), np.float32(np.nan)
     z, nz = np.float32(0.0), np.float32(-0.0)
