@@ -179,7 +179,10 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 #define MTX_MAT_FUZZ_WEIGHT 28
 #define MTX_MAT_FUZZ_COLOR 29         /* 3 floats */
 #define MTX_MAT_FUZZ_ROUGHNESS 32
-#define MTX_MATERIAL_FLOATS 48        /* floats 33..47 are reserved (zero) */
+#define MTX_MAT_SUBSURFACE_WEIGHT 33
+#define MTX_MAT_SUBSURFACE_COLOR 34   /* 3 floats */
+#define MTX_MAT_THIN_WALLED 37        /* > 0.5: thin-walled surface */
+#define MTX_MATERIAL_FLOATS 48        /* floats 38..47 are reserved (zero) */
 
 /* Packed rect light (RectLight.ts:5-22): 16 floats. */
 /* Packed analytic light, the fields of the reference's Light record

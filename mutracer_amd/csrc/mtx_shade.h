@@ -272,11 +272,12 @@ __device__ __forceinline__ f3 to_local(const Basis &B, f3 d) { return f3{dot(d, 
 
 // ---------------------------------------------------------------- materials
 struct Mat {
-  f3 baseColor, specularColor, transmissionColor, coatColor, emission, fuzzColor;
+  f3 baseColor, specularColor, transmissionColor, coatColor, emission, fuzzColor, subsurfaceColor;
   float baseWeight, diffuseRoughness, baseMetalness, specularWeight, specularRoughness, specularIor;
   float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening, fuzzWeight, fuzzRoughness;
+  float subsurfaceWeight;
   float relativeIor, alphaX, alphaY;
-  bool inside;
+  bool inside, thinWalled;
 };
 
 __device__ __forceinline__ void rough_alpha(float r, float aniso, float &ax, float &ay) {
@@ -311,6 +312,10 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.fuzzWeight = clampf(m[MTX_MAT_FUZZ_WEIGHT], 0.0f, 1.0f);
   ma.fuzzColor = vmaxs(f3{m[MTX_MAT_FUZZ_COLOR], m[MTX_MAT_FUZZ_COLOR + 1], m[MTX_MAT_FUZZ_COLOR + 2]}, 0.0f);
   ma.fuzzRoughness = clampf(m[MTX_MAT_FUZZ_ROUGHNESS], 0.0f, 1.0f);
+  ma.subsurfaceWeight = clampf(m[MTX_MAT_SUBSURFACE_WEIGHT], 0.0f, 1.0f);
+  ma.subsurfaceColor = vmaxs(f3{m[MTX_MAT_SUBSURFACE_COLOR], m[MTX_MAT_SUBSURFACE_COLOR + 1],
+                                m[MTX_MAT_SUBSURFACE_COLOR + 2]}, 0.0f);
+  ma.thinWalled = m[MTX_MAT_THIN_WALLED] > 0.5f;
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
     ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
@@ -498,8 +503,8 @@ struct Prep {
 __device__ f3 coat_darkening(const Prep &P) {
   const Mat &m = P.m;
   float diel = 1.0f - m.baseMetalness;
-  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - 0.0f);
-  float sss = diel * (1.0f - m.transmissionWeight) * 0.0f;
+  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - m.subsurfaceWeight);
+  float sss = diel * (1.0f - m.transmissionWeight) * m.subsurfaceWeight;
   float trans = diel * m.transmissionWeight;
   float oSpec = avg_fresnel(P.specEta), tSpec = avg_fresnel(1.0f / P.transReflIor);
   float specularity = clampf(opaque * oSpec + (sss + trans) * tSpec + m.baseMetalness, 0.0f, 1.0f);
@@ -508,7 +513,7 @@ __device__ f3 coat_darkening(const Prep &P) {
   float kr = 1.0f - (1.0f - ks) / (m.coatIor * m.coatIor);
   float k = mixf(ks, kr, effR);
   f3 eb = m.baseMetalness * m.specularWeight * metal_avg_fresnel(m.baseWeight * m.baseColor, m.specularColor) +
-          opaque * mix3(m.baseWeight * m.baseColor, splat(1.0f), oSpec) + sss * splat(0.8f) + splat(trans);
+          opaque * mix3(m.baseWeight * m.baseColor, splat(1.0f), oSpec) + sss * m.subsurfaceColor + splat(trans);
   f3 delta = splat(1.0f - k) / vmax(splat(1.0f) - eb * k, splat(1e-6f));
   return mix3(splat(1.0f), delta, m.coatWeight * m.coatDarkening);
 }
@@ -516,6 +521,32 @@ __device__ __forceinline__ f3 coat_base_scale(const Prep &P, float lc) {
   if (P.m.coatWeight <= 0.0f) return splat(1.0f);
   if (P.m.inside) return coat_passage(P.m, -lc);
   return P.coatBaseScale * coat_passage(P.m, lc);
+}
+
+// Thin-walled dielectric (openPbrBsdf.wgsl.ts:697-731): a slab's two
+// interfaces reflect 2F / (1 + F); its transmission tint is raised to the
+// power 1 / cos of the refracted angle.
+__device__ __forceinline__ float thin_wall_fresnel(float f) { return 2.0f * f / (1.0f + f); }
+__device__ __forceinline__ float thin_wall_reflection(float f, float transmissionWeight) {
+  return mixf(f, 2.0f * f / (1.0f + f), clampf(transmissionWeight, 0.0f, 1.0f));
+}
+__device__ f3 final_transmission_tint(const Mat &m, float incidentCos, float eta) {
+  const f3 tint = m.transmissionColor;
+  if (!m.thinWalled) return tint;
+  const float sinI2 = mxf(0.0f, 1.0f - incidentCos * incidentCos);
+  const float sinT2 = sinI2 / mxf(eta * eta, 1e-9f);
+  if (sinT2 >= 1.0f) return splat(0.0f);
+  const float cosT = sqrtf(1.0f - sinT2);
+  return vpow(vmaxs(tint, 0.0f), splat(1.0f / mxf(cosT, 1e-5f)));
+}
+// openpbr_eval_subsurface_boundary (openPbrBsdf.wgsl.ts:733-750): the
+// diffuse transmission into a non-thin subsurface medium.
+__device__ f3 eval_subsurface_boundary(const Mat &m, f3 v, f3 l, float &pdf) {
+  pdf = 0.0f;
+  if ((v.z <= 0.0f) | (l.z >= 0.0f)) return splat(0.0f);
+  const float entry = fresnel_dielectric(v.z, m.relativeIor);
+  pdf = -l.z * kInvPi;
+  return splat(1.0f - entry) * kInvPi;
 }
 
 __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L) {
@@ -528,8 +559,8 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
   P.w[LM] = m.baseMetalness;
   P.w[LS] = diel;
   P.w[LT] = diel * m.transmissionWeight;
-  P.w[LD] = opaqueDiel * (1.0f - 0.0f) * m.baseWeight;
-  P.w[LSS] = opaqueDiel * 0.0f;
+  P.w[LD] = opaqueDiel * (1.0f - m.subsurfaceWeight) * m.baseWeight;
+  P.w[LSS] = opaqueDiel * m.subsurfaceWeight;
   P.specEta = spec_weighted_ior(1.0f / m.relativeIor, m.specularWeight);
   P.m.relativeIor = 1.0f / P.specEta;
   P.refrEta = P.specEta;
@@ -547,10 +578,12 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
   }
   P.specAlpha = m.specularRoughness * m.specularRoughness;
   float vc = fabsf(P.v.z);
-  f3 dielF = m.specularColor * pbr_fresnel(vc, 1.0f / P.specEta);
+  const float sif = pbr_fresnel(vc, 1.0f / P.specEta);
+  f3 dielF = m.specularColor * (m.thinWalled ? mixf(sif, thin_wall_fresnel(sif), m.transmissionWeight) : sif);
   f3 metalF = m.specularWeight * metal_fresnel(vc, m.baseWeight * m.baseColor, m.specularColor);
   float transF = pbr_fresnel(vc, P.transReflIor);
-  f3 tint = m.transmissionColor;
+  if (m.thinWalled) transF = thin_wall_fresnel(transF);
+  f3 tint = final_transmission_tint(m, vc, P.refrEta);
   P.dsComp = opaque_ec(L, P.specEta, P.specAlpha, vc) / mxf(opaque_avg_ec(L, P.specEta, P.specAlpha), 1e-12f);
   P.p[LF] = 0.0f;
   P.fuzzScale = 1.0f;
@@ -590,7 +623,8 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
   P.msViewLoss = 0.0f;
   P.msRatio = 0.0f;
   P.msProb = 0.0f;
-  if ((m.transmissionWeight > 0.0f) & (P.specAlpha >= 0.0016f) & (P.w[LT] > 0.0f) & (fabsf(P.refrEta - 1.0f) > 1e-6f)) {
+  if (!m.thinWalled & (m.transmissionWeight > 0.0f) & (P.specAlpha >= 0.0016f) & (P.w[LT] > 0.0f) &
+      (fabsf(P.refrEta - 1.0f) > 1e-6f)) {
     float nonOpaque = P.w[LT] + P.w[LSS];
     P.msRefl = (m.inside ? splat(1.0f) : m.specularColor) * nonOpaque;
     P.msTrans = tint * P.w[LT] + splat(P.w[LSS]);
@@ -602,7 +636,7 @@ __device__ void prepare(Prep &P, const Mat &m, f3 view, f3 normal, const Lut &L)
     P.msProb = est / mxf(P.p[LT], 1e-12f);
   }
   P.p[LD] = P.w[LD] * lum_pbr(m.baseColor) * P.dsComp;
-  P.p[LSS] = 0.0f;
+  P.p[LSS] = P.w[LSS] * (1.0f - transF) * lum_pbr(m.subsurfaceColor) * P.dsComp;
   if ((m.coatWeight > 0.0f) && !m.inside) {
     float bs = maxc(P.coatBaseScale);
     P.p[LM] *= bs; P.p[LS] *= bs; P.p[LT] *= bs; P.p[LD] *= bs; P.p[LSS] *= bs;
@@ -625,7 +659,7 @@ __device__ __forceinline__ float substrate_fresnel(const Prep &P, float c, bool 
   float tF = pbr_fresnel(c, P.transReflIor);
   if (!reflection) return tF;
   float oF = pbr_fresnel(c, 1.0f / P.specEta);
-  float nonOpaque = P.m.transmissionWeight + (1.0f - P.m.transmissionWeight) * 0.0f;
+  float nonOpaque = P.m.transmissionWeight + (1.0f - P.m.transmissionWeight) * P.m.subsurfaceWeight;
   return mixf(oF, tF, nonOpaque);
 }
 
@@ -639,16 +673,31 @@ __device__ f3 eval_bsdf(const Prep &P, f3 lightDir, float &pdf, const Lut &L) {
   pdf = 0.0f;
   float lp = 0.0f;
   float vh = fabsf(dot(v, h));
+  // Thin-wall transmission mirrors the light to the view's side: its half
+  // vector is that of a reflection.
+  f3 thinH = h;
+  float dielVh = vh;
+  if (!reflection & m.thinWalled) {
+    thinH = normalize(f3{l.x, l.y, -l.z} + v);
+    if (thinH.z < 0.0f) thinH = -thinH;
+    dielVh = fabsf(dot(v, thinH));
+  }
   bool evalDiel = (reflection & (P.w[LS] > 0.0f)) | (!reflection & (P.w[LT] > 0.0f));
   bool evalMetal = reflection & (P.w[LM] > 0.0f);
   f3 fr0 = splat(0.0f), fr1 = splat(0.0f);
-  if (evalDiel) fr0 = splat(substrate_fresnel(P, vh, reflection));
+  if (evalDiel) fr0 = splat(substrate_fresnel(P, dielVh, reflection));
   if (evalMetal) fr1 = metal_fresnel(vh, m.baseWeight * m.baseColor, m.specularColor);
   float comp = P.dsComp * opaque_ec(L, P.specEta, P.specAlpha, fabsf(l.z));
   if (reflection) {
     if (P.w[LD] > 0.0f) { resp = resp + P.w[LD] * eval_diffuse(m, m.baseColor, v, l, comp, lp); pdf += P.p[LD] * lp; }
+    if ((P.w[LSS] > 0.0f) & m.thinWalled) {
+      resp = resp + 0.5f * P.w[LSS] * eval_diffuse(m, m.subsurfaceColor, v, l, comp, lp);
+      pdf += P.p[LSS] * 0.5f * lp;
+    }
     if (P.w[LS] > 0.0f) {
-      resp = resp + P.w[LS] * eval_refl(m.alphaX, m.alphaY, v, l, h, m.specularColor * fr0, lp);
+      f3 dF = fr0;
+      if (m.thinWalled) dF = splat(thin_wall_reflection(fr0.x, m.transmissionWeight));
+      resp = resp + P.w[LS] * eval_refl(m.alphaX, m.alphaY, v, l, h, m.specularColor * dF, lp);
       pdf += P.p[LS] * lp;
     }
     if (P.w[LM] > 0.0f) {
@@ -660,10 +709,28 @@ __device__ f3 eval_bsdf(const Prep &P, f3 lightDir, float &pdf, const Lut &L) {
       }
       pdf += P.p[LM] * lp;
     }
-  } else if (P.w[LT] > 0.0f) {
-    resp = resp + P.w[LT] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l, h, fr0, lp);
-    if (P.msProb > 0.0f) lp *= 1.0f - P.msProb;
-    pdf += P.p[LT] * lp;
+  } else {
+    if (P.w[LT] > 0.0f) {
+      if (m.thinWalled) {
+        const f3 window = splat(thin_wall_reflection(fr0.x, 1.0f));
+        const f3 tint = final_transmission_tint(m, fabsf(v.z), P.refrEta);
+        resp = resp + P.w[LT] * eval_refl(m.alphaX, m.alphaY, v, f3{l.x, l.y, -l.z}, thinH,
+                                          tint * (splat(1.0f) - window), lp);
+      } else {
+        resp = resp + P.w[LT] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l, h, fr0, lp);
+      }
+      if (P.msProb > 0.0f) lp *= 1.0f - P.msProb;
+      pdf += P.p[LT] * lp;
+    }
+    if (P.w[LSS] > 0.0f) {
+      if (m.thinWalled) {
+        resp = resp + 0.5f * P.w[LSS] * eval_diffuse(m, m.subsurfaceColor, v, f3{l.x, l.y, -l.z}, comp, lp);
+        lp *= 0.5f;
+      } else {
+        resp = resp + P.w[LSS] * eval_subsurface_boundary(m, v, l, lp);
+      }
+      pdf += P.p[LSS] * lp;
+    }
   }
   if (P.msProb > 0.0f) {
     float c = fabsf(l.z);
@@ -736,14 +803,26 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
     } else {
       f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
       if (h.z < 0.0f) h = -h;
-      l = refract3(-v, h, m.relativeIor);
-      if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
-      l = normalize(l);
+      if (m.thinWalled) {
+        const f3 r = normalize(reflect3(-v, h));
+        l = f3{r.x, r.y, -r.z};
+      } else {
+        l = refract3(-v, h, m.relativeIor);
+        if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
+        l = normalize(l);
+      }
     }
   } else {
     transmission = true;
-    l = cosine_hemisphere(rx, ry);
-    l.z = -l.z;
+    if (m.thinWalled) {  // half diffuse reflection, half diffuse transmission
+      const bool transmit = rx >= 0.5f;
+      const float sx = transmit ? (rx - 0.5f) * 2.0f : rx * 2.0f;
+      l = cosine_hemisphere(sx, ry);
+      if (transmit) l.z = -l.z; else transmission = false;
+    } else {
+      l = cosine_hemisphere(rx, ry);
+      l.z = -l.z;
+    }
   }
   dir = to_world(P.basis, l);
   if ((l.z == 0.0f) | ((l.z < 0.0f) != transmission)) return false;
@@ -1333,7 +1412,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       pr.auxNormal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
       if (ma.transmissionWeight > 0.5f) pr.flags |= kAlbedoSampled; else pr.flags &= ~kAlbedoSampled;
       si.albedoSampled = ma.transmissionWeight > 0.5f;
-      if (ma.transmissionWeight < 0.001f) pr.flags |= kNotTransparent;
+      if ((ma.transmissionWeight < 0.001f) & (ma.subsurfaceWeight < 0.001f)) pr.flags |= kNotTransparent;
     }
   } else {
     smp.hashed_draw();
@@ -1383,14 +1462,19 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
     }
     return false;
   }
-  // SampleMaterial (surface vertex, no media, no emission).
+  // SampleMaterial (sampleMaterial.wgsl.ts:316-508; surface vertex, no media).
   {
     const f3 rd = pr.rd;
     float ndv1 = -dot(si.faceNormal, rd), ndv2 = -dot(si.normal, rd);
+    if (((ma.transmissionWeight > 0.0f) | (ma.subsurfaceWeight > 0.0f)) & ma.thinWalled) {
+      // a thin wall is always entered from the viewer's side (:328-338)
+      if (ndv1 < 0.0f) { si.faceNormal = -si.faceNormal; ndv1 = -ndv1; }
+      if (ndv2 < 0.0f) { si.normal = -si.normal; ndv2 = -ndv2; }
+    }
     bool useFaceN = (ndv1 >= 0.0f) & (ndv2 < 0.0f);
     float ndv = useFaceN ? ndv1 : ndv2;
     f3 N = useFaceN ? si.faceNormal : si.normal;
-    ma.inside = ndv <= 0.0f;
+    ma.inside = (ndv <= 0.0f) & !ma.thinWalled;
     ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
     f3 ff = ndv >= 0.0f ? N : -N;
     pr.flags &= ~kIsDelta;
@@ -1447,7 +1531,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
         float d0, d1;
         smp.s2(smp.bdim(3u), d0, d1);
         float lobe = smp.s1(smp.bdim(2u));
-        disc = (fabsf(PR.m.relativeIor - 1.0f) <= 1e-6f) & (PR.p[LT] >= 1.0f - 1e-6f);
+        disc = !PR.m.thinWalled & (fabsf(PR.m.relativeIor - 1.0f) <= 1e-6f) & (PR.p[LT] >= 1.0f - 1e-6f);
         if (disc) {
           ldir = to_world(PR.basis, -PR.v);
           resp = PR.m.transmissionColor * coat_base_scale(PR, -PR.v.z);
@@ -1466,7 +1550,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
         sample_light(S, si.position, a, b, c, ev, cc, ls);
         ldir = ls.direction;
         if ((ls.pdf <= 0.0f) | (ls.selPmf <= 0.0f) | (maxc(ls.emission) <= 0.0f)) continue;
-        if ((dot(ff, ldir) <= 0.0f) & (ma.transmissionWeight <= 0.0f)) continue;
+        if ((dot(ff, ldir) <= 0.0f) & (ma.transmissionWeight <= 0.0f) & (ma.subsurfaceWeight <= 0.0f)) continue;
       }
       if (!disc) resp = eval_bsdf(PR, ldir, lpdf, L);
       if (cont) {

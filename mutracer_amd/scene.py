@@ -21,7 +21,7 @@ MAT_FIELDS = dict(base_weight=0, base_color=1, base_diffuse_roughness=4, base_me
                   specular_color=7, specular_roughness=10, specular_ior=11, transmission_weight=12,
                   transmission_color=13, coat_weight=16, coat_color=17, coat_roughness=20, coat_ior=21,
                   coat_darkening=22, emission=23, double_sided=26, specular_anisotropy=27, fuzz_weight=28,
-                  fuzz_color=29, fuzz_roughness=32)
+                  fuzz_color=29, fuzz_roughness=32, subsurface_weight=33, subsurface_color=34, thin_walled=37)
 
 
 @dataclass
@@ -46,6 +46,9 @@ class Material:
     fuzz_weight: float = 0.0
     fuzz_color: tuple = (1.0, 1.0, 1.0)
     fuzz_roughness: float = 0.5
+    subsurface_weight: float = 0.0
+    subsurface_color: tuple = (0.8, 0.8, 0.8)
+    thin_walled: bool = False
     emission: tuple = (0.0, 0.0, 0.0)
     double_sided: bool = False
 

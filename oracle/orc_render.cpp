@@ -120,6 +120,9 @@ struct Mat {
   float fuzzWeight;
   V3 fuzzColor;
   float fuzzRoughness;
+  float subsurfaceWeight;
+  V3 subsurfaceColor;
+  bool thinWalled;
   V3 emission;
   float alpha;
   bool inside;
@@ -160,6 +163,9 @@ static Mat get_material(const float *m, V3 instanceColor) {
   ma.fuzzWeight = clampf(m[M_FUZZ_WEIGHT], 0.0f, 1.0f);
   ma.fuzzColor = vmaxs(V3(m[M_FUZZ_COLOR], m[M_FUZZ_COLOR + 1], m[M_FUZZ_COLOR + 2]), 0.0f);
   ma.fuzzRoughness = clampf(m[M_FUZZ_ROUGHNESS], 0.0f, 1.0f);
+  ma.subsurfaceWeight = clampf(m[M_SSS_WEIGHT], 0.0f, 1.0f);
+  ma.subsurfaceColor = vmaxs(V3(m[M_SSS_COLOR], m[M_SSS_COLOR + 1], m[M_SSS_COLOR + 2]), 0.0f);
+  ma.thinWalled = m[M_THIN_WALLED] > 0.5f;
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness;
     float c2 = ma.coatRoughness * ma.coatRoughness;
@@ -336,8 +342,8 @@ static float coat_energy(const Mat &m, float c) {
 static V3 coat_darkening(const Prepared &P) {
   const Mat &m = P.material;
   float diel = 1.0f - m.baseMetalness;
-  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - 0.0f);
-  float sss = diel * (1.0f - m.transmissionWeight) * 0.0f;
+  float opaque = diel * (1.0f - m.transmissionWeight) * (1.0f - m.subsurfaceWeight);
+  float sss = diel * (1.0f - m.transmissionWeight) * m.subsurfaceWeight;
   float trans = diel * m.transmissionWeight;
   float oSpec = average_fresnel(P.specularEta);
   float tSpec = average_fresnel(1.0f / P.transmissionReflectionIor);
@@ -347,7 +353,7 @@ static V3 coat_darkening(const Prepared &P) {
   float kr = 1.0f - (1.0f - ks) / (m.coatIor * m.coatIor);
   float k = mixf(ks, kr, effR);
   V3 eb = m.baseMetalness * m.specularWeight * metal_average_fresnel(m.baseWeight * m.baseColor, m.specularColor) +
-          opaque * mix3(m.baseWeight * m.baseColor, V3(1.0f), oSpec) + sss * V3(0.8f) + V3(trans);
+          opaque * mix3(m.baseWeight * m.baseColor, V3(1.0f), oSpec) + sss * m.subsurfaceColor + V3(trans);
   V3 delta = V3(1.0f - k) / vmax(V3(1.0f) - eb * k, V3(1e-6f));
   return mix3(V3(1.0f), delta, m.coatWeight * m.coatDarkening);
 }
@@ -383,6 +389,34 @@ static V3 eval_fuzz(const Mat &m, V3 v, V3 l, float &pdf) {
   return m.fuzzColor * albedo * INVPI;
 }
 
+// openpbr_thin_wall_fresnel, openpbr_thin_wall_reflection_coefficient,
+// openpbr_final_transmission_tint (openPbrBsdf.wgsl.ts:697-731).
+static float thin_wall_fresnel(float f) { return 2.0f * f / (1.0f + f); }
+static V3 thin_wall_reflection_coefficient(V3 f, float transmissionWeight) {
+  V3 window = 2.0f * f / (V3(1.0f) + f);
+  return mix3(f, window, clampf(transmissionWeight, 0.0f, 1.0f));
+}
+static V3 final_transmission_tint(const Mat &m, float incidentCosine, float etaTOverEtaI) {
+  V3 tint = m.transmissionColor;
+  if (!m.thinWalled) return tint;
+  float incidentSineSquared = mxf(0.0f, 1.0f - incidentCosine * incidentCosine);
+  float transmittedSineSquared = incidentSineSquared / mxf(etaTOverEtaI * etaTOverEtaI, 1e-9f);
+  if (transmittedSineSquared >= 1.0f) return V3(0.0f);
+  float transmittedCosine = std::sqrt(1.0f - transmittedSineSquared);
+  float e = 1.0f / mxf(transmittedCosine, 1e-5f);
+  V3 b = vmaxs(tint, 0.0f);
+  return V3(mtxfm::powf(b.x, e), mtxfm::powf(b.y, e), mtxfm::powf(b.z, e));
+}
+// openpbr_eval_subsurface_boundary (openPbrBsdf.wgsl.ts:733-750).
+static V3 eval_subsurface_boundary(const Mat &m, V3 v, V3 l, float &pdf) {
+  pdf = 0.0f;
+  if ((v.z <= 0.0f) | (l.z >= 0.0f)) return V3(0.0f);
+  float transmittedCosine = -l.z;
+  float entryFresnel = dielectric_fresnel_s(v.z, m.relativeIor);
+  pdf = transmittedCosine * INVPI;
+  return V3(1.0f - entryFresnel) * INVPI;
+}
+
 static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   Prepared P{};
   P.material = m;
@@ -397,8 +431,8 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   P.w[L_METAL] = m.baseMetalness;
   P.w[L_SPEC] = dielBase;
   P.w[L_TRANS] = dielBase * m.transmissionWeight;
-  P.w[L_DIFF] = opaqueDiel * (1.0f - 0.0f) * m.baseWeight;
-  P.w[L_SSS] = opaqueDiel * 0.0f;
+  P.w[L_DIFF] = opaqueDiel * (1.0f - m.subsurfaceWeight) * m.baseWeight;
+  P.w[L_SSS] = opaqueDiel * m.subsurfaceWeight;
   P.specularEta = apply_specular_weight_to_ior(1.0f / m.relativeIor, m.specularWeight);
   P.material.relativeIor = 1.0f / P.specularEta;
   P.refractionEta = P.specularEta;
@@ -421,10 +455,11 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   P.specularAlpha = m.specularRoughness * m.specularRoughness;
   float vc = fabsf(P.viewLocal.z);
   float sif = pbr_dielectric_fresnel(vc, 1.0f / P.specularEta);
-  V3 dielF = m.specularColor * sif;
+  V3 dielF = m.specularColor * (m.thinWalled ? mixf(sif, thin_wall_fresnel(sif), m.transmissionWeight) : sif);
   V3 metalF = m.specularWeight * metal_fresnel(vc, m.baseWeight * m.baseColor, m.specularColor);
-  float transF = pbr_dielectric_fresnel(vc, P.transmissionReflectionIor);
-  V3 transTint = m.transmissionColor;
+  float transF = m.thinWalled ? thin_wall_fresnel(pbr_dielectric_fresnel(vc, P.transmissionReflectionIor))
+                              : pbr_dielectric_fresnel(vc, P.transmissionReflectionIor);
+  V3 transTint = final_transmission_tint(m, vc, P.refractionEta);
   float viewEC = opaque_energy_complement(P.specularEta, P.specularAlpha, vc);
   float avgEC = opaque_average_energy_complement(P.specularEta, P.specularAlpha);
   P.diffuseSpecularEnergyCompensation = viewEC / mxf(avgEC, 1e-12f);
@@ -456,7 +491,7 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
   P.dielMsViewLoss = 0.0f;
   P.dielMsRatio = 0.0f;
   P.dielMsProbability = 0.0f;
-  if ((m.transmissionWeight > 0.0f) & (P.specularAlpha >= 0.0016f) & (P.w[L_TRANS] > 0.0f) &
+  if (!m.thinWalled & (m.transmissionWeight > 0.0f) & (P.specularAlpha >= 0.0016f) & (P.w[L_TRANS] > 0.0f) &
       (fabsf(P.refractionEta - 1.0f) > 1e-6f)) {
     float nonOpaque = P.w[L_TRANS] + P.w[L_SSS];
     P.dielMsReflScale = (m.inside ? V3(1.0f) : m.specularColor) * nonOpaque;
@@ -469,7 +504,7 @@ static Prepared prepare(const Mat &m, V3 view, V3 normal) {
     P.dielMsProbability = est / mxf(P.p[L_TRANS], 1e-12f);
   }
   P.p[L_DIFF] = P.w[L_DIFF] * lum_pbr(P.diffuseColor) * P.diffuseSpecularEnergyCompensation;
-  P.p[L_SSS] = 0.0f;
+  P.p[L_SSS] = P.w[L_SSS] * (1.0f - transF) * lum_pbr(m.subsurfaceColor) * P.diffuseSpecularEnergyCompensation;
   if ((m.coatWeight > 0.0f) && !m.inside) {
     float bs = maxc(P.coatBaseScale);
     P.p[L_METAL] *= bs; P.p[L_SPEC] *= bs; P.p[L_TRANS] *= bs; P.p[L_DIFF] *= bs; P.p[L_SSS] *= bs;
@@ -490,7 +525,7 @@ static float substrate_fresnel(const Prepared &P, float c, bool reflection) {
   float tF = pbr_dielectric_fresnel(c, P.transmissionReflectionIor);
   if (!reflection) return tF;
   float oF = pbr_dielectric_fresnel(c, 1.0f / P.specularEta);
-  float nonOpaque = P.material.transmissionWeight + (1.0f - P.material.transmissionWeight) * 0.0f;
+  float nonOpaque = P.material.transmissionWeight + (1.0f - P.material.transmissionWeight) * P.material.subsurfaceWeight;
   return mixf(oF, tF, nonOpaque);
 }
 
@@ -505,10 +540,17 @@ static V3 eval_prepared(const Prepared &P, V3 lightDir, float &pdf) {
   pdf = 0.0f;
   float lobePdf = 0.0f;
   float viewHalf = fabsf(dot(v, h));
+  V3 thinWallHalf = h;
+  float dielectricViewHalf = viewHalf;
+  if (!reflection & m.thinWalled) {
+    thinWallHalf = normalize(V3(l.x, l.y, -l.z) + v);
+    if (thinWallHalf.z < 0.0f) thinWallHalf = -thinWallHalf;
+    dielectricViewHalf = fabsf(dot(v, thinWallHalf));
+  }
   bool evalDiel = (reflection & (P.w[L_SPEC] > 0.0f)) | (!reflection & (P.w[L_TRANS] > 0.0f));
   bool evalMetal = reflection & (P.w[L_METAL] > 0.0f);
   V3 fr0(0.0f), fr1(0.0f);
-  if (evalDiel) fr0 = V3(substrate_fresnel(P, viewHalf, reflection));
+  if (evalDiel) fr0 = V3(substrate_fresnel(P, dielectricViewHalf, reflection));
   if (evalMetal) fr1 = metal_fresnel(viewHalf, m.baseWeight * m.baseColor, m.specularColor);
   float comp = P.diffuseSpecularEnergyCompensation * opaque_energy_complement(P.specularEta, P.specularAlpha, fabsf(l.z));
   if (reflection) {
@@ -516,8 +558,14 @@ static V3 eval_prepared(const Prepared &P, V3 lightDir, float &pdf) {
       response = response + P.w[L_DIFF] * eval_diffuse(m, P.diffuseColor, v, l, comp, lobePdf);
       pdf += P.p[L_DIFF] * lobePdf;
     }
+    if ((P.w[L_SSS] > 0.0f) & m.thinWalled) {
+      response = response + 0.5f * P.w[L_SSS] * eval_diffuse(m, m.subsurfaceColor, v, l, comp, lobePdf);
+      pdf += P.p[L_SSS] * 0.5f * lobePdf;
+    }
     if (P.w[L_SPEC] > 0.0f) {
-      V3 F = m.specularColor * fr0;
+      V3 dielectricFresnel = fr0;
+      if (m.thinWalled) dielectricFresnel = thin_wall_reflection_coefficient(dielectricFresnel, m.transmissionWeight);
+      V3 F = m.specularColor * dielectricFresnel;
       response = response + P.w[L_SPEC] * eval_refl(m.alphaX, m.alphaY, v, l, h, F, lobePdf);
       pdf += P.p[L_SPEC] * lobePdf;
     }
@@ -533,10 +581,28 @@ static V3 eval_prepared(const Prepared &P, V3 lightDir, float &pdf) {
     }
   } else {
     if (P.w[L_TRANS] > 0.0f) {
-      response = response + P.w[L_TRANS] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l, h,
-                                                     fr0, lobePdf);
+      if (m.thinWalled) {
+        V3 mirroredLight(l.x, l.y, -l.z);
+        V3 windowFresnel = thin_wall_reflection_coefficient(fr0, 1.0f);
+        V3 tint = final_transmission_tint(m, fabsf(v.z), P.refractionEta);
+        response = response + P.w[L_TRANS] * eval_refl(m.alphaX, m.alphaY, v, mirroredLight, thinWallHalf,
+                                                       tint * (V3(1.0f) - windowFresnel), lobePdf);
+      } else {
+        response = response + P.w[L_TRANS] * eval_refr(m.alphaX, m.alphaY, m.relativeIor, m.transmissionColor, v, l,
+                                                       h, fr0, lobePdf);
+      }
       if (P.dielMsProbability > 0.0f) lobePdf *= 1.0f - P.dielMsProbability;
       pdf += P.p[L_TRANS] * lobePdf;
+    }
+    if (P.w[L_SSS] > 0.0f) {
+      if (m.thinWalled) {
+        V3 flippedLight(l.x, l.y, -l.z);
+        response = response + 0.5f * P.w[L_SSS] * eval_diffuse(m, m.subsurfaceColor, v, flippedLight, comp, lobePdf);
+        lobePdf *= 0.5f;
+      } else {
+        response = response + P.w[L_SSS] * eval_subsurface_boundary(m, v, l, lobePdf);
+      }
+      pdf += P.p[L_SSS] * lobePdf;
     }
   }
   if (P.dielMsProbability > 0.0f) {
@@ -614,14 +680,28 @@ static bool sample_direction(const Prepared &P, float rx, float ry, float lobe, 
     } else {
       V3 h = ggx_visible_normal(v, m.alphaX, m.alphaY, rx, ry);
       if (h.z < 0.0f) h = -h;
-      l = refract(-v, h, m.relativeIor);
-      if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
-      l = normalize(l);
+      if (m.thinWalled) {
+        V3 reflected = normalize(reflect(-v, h));
+        l = V3(reflected.x, reflected.y, -reflected.z);
+      } else {
+        l = refract(-v, h, m.relativeIor);
+        if (dot(l, l) <= 1e-8f) { transmission = false; return false; }
+        l = normalize(l);
+      }
     }
   } else {
     transmission = true;
-    l = cosine_hemisphere(rx, ry);
-    l.z = -l.z;
+    if (m.thinWalled) {
+      bool transmit = rx >= 0.5f;
+      if (transmit) rx = (rx - 0.5f) * 2.0f;
+      else rx *= 2.0f;
+      l = cosine_hemisphere(rx, ry);
+      if (transmit) l.z = -l.z;
+      else transmission = false;
+    } else {
+      l = cosine_hemisphere(rx, ry);
+      l.z = -l.z;
+    }
   }
   lightDir = to_world(P.basis, l);
   if ((l.z == 0.0f) | ((l.z < 0.0f) != transmission)) return false;
@@ -1404,7 +1484,8 @@ static Scatter sample_scattering(Ctx &c, Sampler &smp, const Surface &si, const 
       float d2[2];
       smp.sample2d(smp.bounce_dim(BSDF_DIRECTION), d2);
       float lobe = smp.sample1d(smp.bounce_dim(BSDF_LOBE));
-      discrete = (fabsf(P.material.relativeIor - 1.0f) <= 1e-6f) & (P.p[L_TRANS] >= 1.0f - 1e-6f);
+      discrete = !P.material.thinWalled & (fabsf(P.material.relativeIor - 1.0f) <= 1e-6f) &
+                 (P.p[L_TRANS] >= 1.0f - 1e-6f);
       if (discrete) {
         lightDir = to_world(P.basis, -P.viewLocal);
         resp = P.material.transmissionColor * coat_base_scale(P, -P.viewLocal.z);
@@ -1424,7 +1505,7 @@ static Scatter sample_scattering(Ctx &c, Sampler &smp, const Surface &si, const 
       lightDir = ls.direction;
       if ((ls.pdf <= 0.0f) | (ls.selectionPmf <= 0.0f) | (maxc(ls.emission) <= 0.0f)) continue;
       float cosThetaL = dot(ff, lightDir);
-      if ((cosThetaL <= 0.0f) & (ma.transmissionWeight <= 0.0f)) continue;
+      if ((cosThetaL <= 0.0f) & (ma.transmissionWeight <= 0.0f) & (ma.subsurfaceWeight <= 0.0f)) continue;
     }
     if (!discrete) resp = eval_prepared(P, lightDir, scatPdf);
     if (cont) {
@@ -1451,10 +1532,21 @@ static Scatter sample_scattering(Ctx &c, Sampler &smp, const Surface &si, const 
 static void sample_material(Ctx &c, Sampler &smp, Surface &si, Mat &ma, PathState &path) {
   float ndv1 = -dot(si.faceNormal, path.rd);
   float ndv2 = -dot(si.normal, path.rd);
+  if (((ma.transmissionWeight > 0.0f) | (ma.subsurfaceWeight > 0.0f)) & ma.thinWalled) {
+    // Always treat thin material as entering medium.
+    if (ndv1 < 0.0f) {
+      si.faceNormal = -si.faceNormal;
+      ndv1 = -ndv1;
+    }
+    if (ndv2 < 0.0f) {
+      si.normal = -si.normal;
+      ndv2 = -ndv2;
+    }
+  }
   bool useFaceN = (ndv1 >= 0.0f) & (ndv2 < 0.0f);
   float ndv = useFaceN ? ndv1 : ndv2;
   V3 N = useFaceN ? si.faceNormal : si.normal;
-  ma.inside = ndv <= 0.0f;
+  ma.inside = (ndv <= 0.0f) & !ma.thinWalled;
   ma.relativeIor = ndv > 0.0f ? 1.0f / ma.specularIor : ma.specularIor;
   V3 ff = ndv >= 0.0f ? N : -N;
   path.scatteringIsDelta = false;
@@ -1611,7 +1703,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
         aux.normal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
         si.albedoSampled = ma.transmissionWeight > 0.5f;
       }
-      isNotTransparent = ma.transmissionWeight < 0.001f && ma.alpha > 0.99f;
+      isNotTransparent = ma.transmissionWeight < 0.001f && ma.subsurfaceWeight < 0.001f && ma.alpha > 0.99f;
     } else {
       // completeVertex (rayTrace.compute.ts:514-589); materials are opaque,
       // so the alpha-coverage draw is always below alpha.

@@ -21,7 +21,7 @@ enum MatField {
   M_TRANS_WEIGHT = 12, M_TRANS_COLOR = 13, M_COAT_WEIGHT = 16, M_COAT_COLOR = 17,
   M_COAT_ROUGHNESS = 20, M_COAT_IOR = 21, M_COAT_DARKENING = 22, M_EMISSION = 23,
   M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27, M_FUZZ_WEIGHT = 28, M_FUZZ_COLOR = 29, M_FUZZ_ROUGHNESS = 32,
-  M_FLOATS = 48
+  M_SSS_WEIGHT = 33, M_SSS_COLOR = 34, M_THIN_WALLED = 37, M_FLOATS = 48
 };
 // Packed light (16 floats): type, position, u, v, emission, spread or radius,
 // area (the reference's Light record, rayTraceCore.wgsl.ts:228-248). Types are

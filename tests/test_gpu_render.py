@@ -146,6 +146,28 @@ def test_fuzz_materials(env, orc, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+def test_thin_walled_and_subsurface(env, orc, kernel):
+    """Thin-walled glass (smooth and rough, tinted), subsurface boundaries
+    of solids and thin subsurface sheets, some under a coat."""
+    torch, render, scene = env
+    sc = scene.instanced_field(n_instances=10, freq=6, env_size=(64, 32))
+    for k, m in enumerate(sc.materials):
+        if k % 3 == 0:
+            m.thin_walled = True
+            m.transmission_weight = 1.0 if k % 2 == 0 else 0.0
+            m.subsurface_weight = 0.0 if k % 2 == 0 else 0.7
+            m.transmission_color = (0.9, 0.7, 0.5)
+            m.specular_roughness = 0.05 * (k % 4)
+        elif k % 3 == 1:
+            m.subsurface_weight = 0.6
+            m.subsurface_color = (0.8, 0.4, 0.3)
+            m.coat_weight = 0.5 if k % 2 else 0.0
+    sums, counts, ref, _ = _both(orc, render, sc, 48, 32, 6, bounces=10, max_samples=6, kernel=kernel,
+                                 direct_light_candidates=3)
+    _check(sums, counts, ref, 6)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_random_pattern(env, orc, kernel):
     """Random pattern: Halton jitter per sample index, one pcg4d stream per
     path drawn in the reference's call order (lens, alpha coverage, NEE
