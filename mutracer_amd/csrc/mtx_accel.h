@@ -59,19 +59,32 @@ struct AccelView {
 MTX_HD float intersect_box(f3 o, f3 bmin, f3 bmax, f3 invD, float tMax) {
   f3 tBot = (bmin - o) * invD;
   f3 tTop = (bmax - o) * invD;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // The hardware min/max (IEEE minNum/maxNum, min3/max3) return the non-NaN
+  // operand as mnf/mxf do and differ from them only in the sign of a zero
+  // result; the returned distance is only ever compared (> 0, with tMax and
+  // with the sibling's), where -0 == +0, so every decision is the same.
+  const float t0 = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tBot.x, tTop.x), __builtin_fminf(tBot.y, tTop.y)),
+                                   __builtin_fminf(tBot.z, tTop.z));
+  const float t1 = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tBot.x, tTop.x), __builtin_fmaxf(tBot.y, tTop.y)),
+                                   __builtin_fmaxf(tBot.z, tTop.z));
+#else
   f3 tNear = vmin(tBot, tTop);
   f3 tFar = vmax(tBot, tTop);
   float t0 = mxf(tNear.x, mxf(tNear.y, tNear.z));
   float t1 = mnf(tFar.x, mnf(tFar.y, tFar.z));
+#endif
   return (t0 > t1 || t0 > tMax) ? -1.0f : (t0 > 0.0f ? t0 : t1);
 }
 
 MTX_HD int max_dimension(f3 v) { return v.x > v.y ? (v.x > v.z ? 0 : 2) : (v.y > v.z ? 1 : 2); }
 
-MTX_HD f3 get_shear(f3 d, int maxDim) {
-  if (maxDim == 0) return f3{-d.y, -d.z, 1.0f} / d.x;
-  if (maxDim == 1) return f3{-d.z, -d.x, 1.0f} / d.y;
-  return f3{-d.x, -d.y, 1.0f} / d.z;
+// The third component, 1 / d[maxDim], is the same IEEE division as the
+// ray's invD component, which callers pass in.
+MTX_HD f3 get_shear(f3 d, f3 invD, int maxDim) {
+  if (maxDim == 0) return f3{-d.y / d.x, -d.z / d.x, invD.x};
+  if (maxDim == 1) return f3{-d.z / d.y, -d.x / d.y, invD.y};
+  return f3{-d.x / d.z, -d.y / d.z, invD.z};
 }
 
 MTX_HD f3 permute_axes(f3 p, int maxDim) {
@@ -224,7 +237,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
       d = xform(w2l, rd, 0.0f);
       invD = splat(1.0f) / d;
       maxDim = max_dimension(vabs(d));
-      shear = get_shear(d, maxDim);
+      shear = get_shear(d, invD, maxDim);
       matDouble = (fbits(c3.w) & 1u) != 0u;
       bnodes = s.nodes;
       btris = s.tris;

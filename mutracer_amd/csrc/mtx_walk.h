@@ -73,7 +73,7 @@ struct Walker {
       d = xform(w2l, rd, 0.0f);
       invD = splat(1.0f) / d;
       maxDim = max_dimension(vabs(d));
-      shear = get_shear(d, maxDim);
+      shear = get_shear(d, invD, maxDim);
       matDouble = (fbits(c3.w) & 1u) != 0u;
       bnodes = s.nodes;
       btris = s.tris;
