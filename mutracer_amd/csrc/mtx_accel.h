@@ -45,6 +45,21 @@ static_assert(sizeof(TNode) == 64, "TNode");
 static_assert(sizeof(TriRec) == 48, "TriRec");
 static_assert(sizeof(InstSlot) == 128, "InstSlot");
 
+// Traversal reads nodes, triangles and instance slots through global
+// address-space pointers: loads through generic (flat) pointers also count
+// against the LDS counter, so every wait for a node would wait for the LDS
+// stack's operations too.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MTX_AS_GLOBAL __attribute__((address_space(1)))
+#else
+#define MTX_AS_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ const MTX_AS_GLOBAL T *as_global(const T *p) {
+  return (const MTX_AS_GLOBAL T *)p;
+}
+
+
 struct AccelView {
   const TNode *tlasNodes;
   const InstSlot *slots;
@@ -170,8 +185,10 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
   int leafFirst = 0, leafRemaining = 0;
   bool matDouble = false;
   int32_t slotIdx = -1;
-  const TNode *bnodes = nullptr;
-  const TriRec *btris = nullptr;
+  const MTX_AS_GLOBAL TNode *bnodes = nullptr;
+  const MTX_AS_GLOBAL TriRec *btris = nullptr;
+  const MTX_AS_GLOBAL TNode *tnodes = as_global(acc.tlasNodes);
+  const MTX_AS_GLOBAL InstSlot *slots = as_global(acc.slots);
   if (kCount) cnt->boxes++;
   if (intersect_box(o, f3{acc.rootMin[0], acc.rootMin[1], acc.rootMin[2]},
                     f3{acc.rootMax[0], acc.rootMax[1], acc.rootMax[2]}, invD, tMax) > 0.0f) {
@@ -225,7 +242,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
     if (leafRemaining > 0) {
       leafRemaining--;
       slotIdx = leafFirst + leafRemaining;
-      const InstSlot &s = acc.slots[slotIdx];
+      const MTX_AS_GLOBAL InstSlot &s = slots[slotIdx];
       if (acc.perInstanceBounds) {
         if (kCount) cnt->boxes++;
         if (intersect_box(ro, f3of(s.bmin), f3of(s.bmax), worldInvD, tMax) <= 0.0f) continue;
@@ -239,8 +256,8 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
       maxDim = max_dimension(vabs(d));
       shear = get_shear(d, invD, maxDim);
       matDouble = (fbits(c3.w) & 1u) != 0u;
-      bnodes = s.nodes;
-      btris = s.tris;
+      bnodes = as_global(s.nodes);
+      btris = as_global(s.tris);
       blasBase = sp;
       st.put(sp++, fint(c0.w));
       continue;
@@ -251,7 +268,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
     d = rd;
     invD = worldInvD;
     if (ref >= 0) {
-      const TNode n = acc.tlasNodes[ref];
+      const TNode n = tnodes[ref];
       float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
       float d1 = intersect_box(o, f3of(n.c), f3of(n.d), invD, tMax);
       if (kCount) cnt->boxes += 2;
@@ -265,7 +282,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
       }
     } else {
       leafFirst = ref & 0x7fffffff;
-      leafRemaining = fint(acc.slots[leafFirst].c1.w);
+      leafRemaining = fint(slots[leafFirst].c1.w);
     }
   }
   return h.hit;
