@@ -194,11 +194,49 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
                     f3{acc.rootMax[0], acc.rootMax[1], acc.rootMax[2]}, invD, tMax) > 0.0f) {
     st.put(sp++, acc.tlasRootRef);
   }
-  while (true) {
-    if (blasBase >= 0 && sp > blasBase) {
-      int32_t ref = st.get(--sp);
+  // Closest hit: one node step serves both levels (the node array and the
+  // ray are picked per lane), so a wave whose lanes sit in the TLAS and in
+  // BLASes runs the box-pair code once per iteration. Any hit keeps a
+  // separate step per level, which measures faster for shadow rays. Either
+  // way each lane pops, tests and pushes in the same order.
+  if constexpr (!kAnyHit) {
+    while (true) {
+      const bool inBlas = (blasBase >= 0) & (sp > blasBase);
+      if (!inBlas) {
+        if (blasBase >= 0) {  // BLAS finished: back to the world ray
+          blasBase = -1;
+          o = ro;
+          d = rd;
+          invD = worldInvD;
+        }
+        if (leafRemaining > 0) {
+          leafRemaining--;
+          slotIdx = leafFirst + leafRemaining;
+          const MTX_AS_GLOBAL InstSlot &s = slots[slotIdx];
+          if (acc.perInstanceBounds) {
+            if (kCount) cnt->boxes++;
+            if (intersect_box(ro, f3of(s.bmin), f3of(s.bmax), worldInvD, tMax) <= 0.0f) continue;
+          }
+          if (kCount) cnt->instances++;
+          float4 c0 = s.c0, c1 = s.c1, c2 = s.c2, c3 = s.c3;
+          Affine w2l{f3of(c0), f3of(c1), f3of(c2), f3of(c3)};
+          o = xform(w2l, ro, 1.0f);
+          d = xform(w2l, rd, 0.0f);
+          invD = splat(1.0f) / d;
+          maxDim = max_dimension(vabs(d));
+          shear = get_shear(d, invD, maxDim);
+          matDouble = (fbits(c3.w) & 1u) != 0u;
+          bnodes = as_global(s.nodes);
+          btris = as_global(s.tris);
+          blasBase = sp;
+          st.put(sp++, fint(c0.w));
+          continue;
+        }
+        if (sp == 0) break;
+      }
+      const int32_t ref = st.get(--sp);
       if (ref >= 0) {
-        const TNode n = bnodes[ref];
+        const TNode n = (inBlas ? bnodes : tnodes)[ref];
         float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
         float d1 = intersect_box(o, f3of(n.c), f3of(n.d), invD, tMax);
         if (kCount) cnt->boxes += 2;
@@ -210,7 +248,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
           if (d1 > 0.0f) st.put(sp++, right);
           if (d0 > 0.0f) st.put(sp++, left);
         }
-      } else {
+      } else if (inBlas) {
         int32_t first = ref & 0x7fffffff;
         int32_t count = fint(btris[first].v1.w);
         for (int32_t k = 0; k < count; k++) {
@@ -235,54 +273,102 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
             if (kAnyHit) return true;
           }
         }
-      }
-      continue;
-    }
-    if (blasBase >= 0) blasBase = -1;
-    if (leafRemaining > 0) {
-      leafRemaining--;
-      slotIdx = leafFirst + leafRemaining;
-      const MTX_AS_GLOBAL InstSlot &s = slots[slotIdx];
-      if (acc.perInstanceBounds) {
-        if (kCount) cnt->boxes++;
-        if (intersect_box(ro, f3of(s.bmin), f3of(s.bmax), worldInvD, tMax) <= 0.0f) continue;
-      }
-      if (kCount) cnt->instances++;
-      float4 c0 = s.c0, c1 = s.c1, c2 = s.c2, c3 = s.c3;
-      Affine w2l{f3of(c0), f3of(c1), f3of(c2), f3of(c3)};
-      o = xform(w2l, ro, 1.0f);
-      d = xform(w2l, rd, 0.0f);
-      invD = splat(1.0f) / d;
-      maxDim = max_dimension(vabs(d));
-      shear = get_shear(d, invD, maxDim);
-      matDouble = (fbits(c3.w) & 1u) != 0u;
-      bnodes = as_global(s.nodes);
-      btris = as_global(s.tris);
-      blasBase = sp;
-      st.put(sp++, fint(c0.w));
-      continue;
-    }
-    if (sp == 0) break;
-    int32_t ref = st.get(--sp);
-    o = ro;
-    d = rd;
-    invD = worldInvD;
-    if (ref >= 0) {
-      const TNode n = tnodes[ref];
-      float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
-      float d1 = intersect_box(o, f3of(n.c), f3of(n.d), invD, tMax);
-      if (kCount) cnt->boxes += 2;
-      int32_t left = fint(n.a.w), right = fint(n.b.w);
-      if (d0 > d1) {
-        if (d0 > 0.0f) st.put(sp++, left);
-        if (d1 > 0.0f) st.put(sp++, right);
       } else {
-        if (d1 > 0.0f) st.put(sp++, right);
-        if (d0 > 0.0f) st.put(sp++, left);
+        leafFirst = ref & 0x7fffffff;
+        leafRemaining = fint(slots[leafFirst].c1.w);
       }
-    } else {
-      leafFirst = ref & 0x7fffffff;
-      leafRemaining = fint(slots[leafFirst].c1.w);
+    }
+  } else {
+    while (true) {
+      if (blasBase >= 0 && sp > blasBase) {
+        int32_t ref = st.get(--sp);
+        if (ref >= 0) {
+          const TNode n = bnodes[ref];
+          float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
+          float d1 = intersect_box(o, f3of(n.c), f3of(n.d), invD, tMax);
+          if (kCount) cnt->boxes += 2;
+          int32_t left = fint(n.a.w), right = fint(n.b.w);
+          if (d0 > d1) {
+            if (d0 > 0.0f) st.put(sp++, left);
+            if (d1 > 0.0f) st.put(sp++, right);
+          } else {
+            if (d1 > 0.0f) st.put(sp++, right);
+            if (d0 > 0.0f) st.put(sp++, left);
+          }
+        } else {
+          int32_t first = ref & 0x7fffffff;
+          int32_t count = fint(btris[first].v1.w);
+          for (int32_t k = 0; k < count; k++) {
+            int32_t tri = first + k;
+            if (bounces > 0 && tri == curTri && slotIdx == curSlot) continue;
+            const TriRec r = btris[tri];
+            f3 p0 = f3of(r.v0), p1 = f3of(r.v1), p2 = f3of(r.v2);
+            if (kCount) cnt->tris++;
+            f3 bary;
+            float t = intersect_triangle(o, p0, p1, p2, maxDim, shear, tMax, bary);
+            if (t > 0.0f) {
+              f3 nrm = normalize(cross(p1 - p0, p2 - p0));
+              if (!doubleSided && !matDouble && dot(nrm, d) > 0.0f) continue;
+              tMax = t;
+              h.hit = true;
+              h.t = t;
+              h.bary = bary;
+              h.normal = nrm;
+              h.position = o + tMax * d;
+              h.slot = slotIdx;
+              h.tri = tri;
+              if (kAnyHit) return true;
+            }
+          }
+        }
+        continue;
+      }
+      if (blasBase >= 0) blasBase = -1;
+      if (leafRemaining > 0) {
+        leafRemaining--;
+        slotIdx = leafFirst + leafRemaining;
+        const MTX_AS_GLOBAL InstSlot &s = slots[slotIdx];
+        if (acc.perInstanceBounds) {
+          if (kCount) cnt->boxes++;
+          if (intersect_box(ro, f3of(s.bmin), f3of(s.bmax), worldInvD, tMax) <= 0.0f) continue;
+        }
+        if (kCount) cnt->instances++;
+        float4 c0 = s.c0, c1 = s.c1, c2 = s.c2, c3 = s.c3;
+        Affine w2l{f3of(c0), f3of(c1), f3of(c2), f3of(c3)};
+        o = xform(w2l, ro, 1.0f);
+        d = xform(w2l, rd, 0.0f);
+        invD = splat(1.0f) / d;
+        maxDim = max_dimension(vabs(d));
+        shear = get_shear(d, invD, maxDim);
+        matDouble = (fbits(c3.w) & 1u) != 0u;
+        bnodes = as_global(s.nodes);
+        btris = as_global(s.tris);
+        blasBase = sp;
+        st.put(sp++, fint(c0.w));
+        continue;
+      }
+      if (sp == 0) break;
+      int32_t ref = st.get(--sp);
+      o = ro;
+      d = rd;
+      invD = worldInvD;
+      if (ref >= 0) {
+        const TNode n = tnodes[ref];
+        float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
+        float d1 = intersect_box(o, f3of(n.c), f3of(n.d), invD, tMax);
+        if (kCount) cnt->boxes += 2;
+        int32_t left = fint(n.a.w), right = fint(n.b.w);
+        if (d0 > d1) {
+          if (d0 > 0.0f) st.put(sp++, left);
+          if (d1 > 0.0f) st.put(sp++, right);
+        } else {
+          if (d1 > 0.0f) st.put(sp++, right);
+          if (d0 > 0.0f) st.put(sp++, left);
+        }
+      } else {
+        leafFirst = ref & 0x7fffffff;
+        leafRemaining = fint(slots[leafFirst].c1.w);
+      }
     }
   }
   return h.hit;
