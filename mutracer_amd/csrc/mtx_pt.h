@@ -40,7 +40,13 @@ struct GpuScene {
   const float *lut;            // 32 x 2177 energy tables
   const float4 *emissive;      // 3 per emissive triangle: (p0, pmf), (p1, area), (p2, material), world space
   uint32_t emissiveCount, pad0;
+  const uint16_t *envGuide;    // search brackets of envCdf (kEnvGuideRows + 1, then envH x (kEnvGuideCols + 1)), or null
 };
+// Bins of the environment CDF guide tables: bin k of the marginal (of row
+// y's conditional) CDF holds the index its search returns for the sample
+// k / bins, so a sample in bin k searches only between bins k and k + 1.
+constexpr uint32_t kEnvGuideRows = 1024;
+constexpr uint32_t kEnvGuideCols = 128;
 
 struct RenderParams {
   float projInv[16], viewInv[16];

@@ -25,12 +25,13 @@ struct RenderSceneObj {
   DevBuf<LightChoice> choices;
   DevBuf<float4> env, emissive;
   DevBuf<float2> envCdf;
+  DevBuf<uint16_t> envGuide;
   std::vector<mtx_instance_desc> inst;
   GpuScene view{};
   uint64_t triangles = 0;
   size_t bytes() const {
     size_t b = shade.bytes() + materials.bytes() + lights.bytes() + lut.bytes() + env.bytes() + envCdf.bytes() +
-               emissive.bytes();
+               envGuide.bytes() + emissive.bytes();
     for (auto &x : normals) b += x.bytes();
     for (auto &x : indices) b += x.bytes();
     for (auto &x : blases) b += x->resident_bytes();
@@ -258,6 +259,36 @@ static std::vector<float2> env_cdf(const float *env, uint32_t W, uint32_t H) {
   return cdf;
 }
 
+// Guide tables of the environment CDF (see kEnvGuideRows): for each bin
+// edge r = k / bins, the result of sample_env's search (first entry with
+// CDF > r, or the end of the range) over the marginal CDF (column 0) and
+// over each row's conditional CDF. Empty when an index does not fit 16 bits.
+static std::vector<uint16_t> env_guides(const std::vector<float2> &cdf, uint32_t W, uint32_t H) {
+  std::vector<uint16_t> g;
+  if (W + 1 > 65535u || H + 1 > 65535u) return g;
+  const uint32_t CW = W + 2;
+  auto search = [](auto at, uint32_t n, float r) {  // first i in [0, n) with at(i) > r, else n
+    uint32_t i = 0;
+    while (i < n && !(at(i) > r)) i++;
+    return (uint16_t)i;
+  };
+  g.reserve((kEnvGuideRows + 1) + (size_t)H * (kEnvGuideCols + 1));
+  auto marginal = [&](uint32_t i) { return cdf[(size_t)i * CW].x; };
+  for (uint32_t k = 0; k <= kEnvGuideRows; k++)
+    g.push_back(search(marginal, H + 1, (float)k / (float)kEnvGuideRows));
+  for (uint32_t y = 0; y < H; y++) {
+    auto row = [&](uint32_t i) { return cdf[(size_t)y * CW + 1 + i].x; };
+    // one pass per row: bin edges are increasing
+    uint32_t i = 0;
+    for (uint32_t k = 0; k <= kEnvGuideCols; k++) {
+      const float r = (float)k / (float)kEnvGuideCols;
+      while (i < W + 1 && !(row(i) > r)) i++;
+      g.push_back((uint16_t)i);
+    }
+  }
+  return g;
+}
+
 static void fill_shade(RenderSceneObj &R) {
   const Accel &A = *R.accel;
   std::vector<ShadeSlot> sh(std::max<uint32_t>(1, A.instanceCount));
@@ -329,6 +360,8 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
     R->env.upload(reinterpret_cast<const float4 *>(D.env_rgba), (size_t)D.env_width * D.env_height, 0);
     std::vector<float2> cdf = env_cdf(D.env_rgba, D.env_width, D.env_height);
     R->envCdf.upload(cdf.data(), cdf.size(), 0);
+    std::vector<uint16_t> guide = env_guides(cdf, D.env_width, D.env_height);
+    if (!guide.empty()) R->envGuide.upload(guide.data(), guide.size(), 0);
   }
   R->lut.upload(reinterpret_cast<const float *>(kEnergyLutBits), 32 * 2177, 0);
   GpuScene &v = R->view;
@@ -344,6 +377,7 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   v.lightWeightTotal = (float)T.total;
   v.env = hasEnv ? R->env.ptr : nullptr;
   v.envCdf = hasEnv ? R->envCdf.ptr : nullptr;
+  v.envGuide = hasEnv ? R->envGuide.ptr : nullptr;
   v.emissive = E.weight.empty() ? nullptr : R->emissive.ptr;
   v.emissiveCount = (uint32_t)E.weight.size();
   v.envW = hasEnv ? D.env_width : 0;

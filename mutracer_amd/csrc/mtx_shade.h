@@ -870,9 +870,22 @@ __device__ float env_pdf(const GpuScene &S, float u, float v) {
   int y = min(max((int)(v * (float)S.envH), 0), (int)S.envH - 1);
   return env_solid_pdf(cdf_at(S, 1 + x, y).y, cdf_at(S, 0, y).y, v);
 }
+// Search bracket [lo, hi] of the first CDF entry above r: the guide table's
+// entries for the bins around r (the search result is monotone in r, so it
+// lies between the results for the bin's two ends), or the whole range.
+__device__ __forceinline__ void env_bracket(const uint16_t *guide, uint32_t bins, float r, int full, int &lo, int &hi) {
+  lo = 0;
+  hi = full;
+  if (guide && r >= 0.0f && r < 1.0f) {
+    const uint32_t k = min((uint32_t)(r * (float)bins), bins - 1u);
+    lo = guide[k];
+    hi = guide[k + 1];
+  }
+}
 __device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float &v, float &pdf) {
   int H = (int)S.envH, W = (int)S.envW;
-  int left = 0, right = H + 1;
+  int left, right;
+  env_bracket(S.envGuide, kEnvGuideRows, r0, H + 1, left, right);
   while (left < right) {
     int mid = (left + right) >> 1;
     if (cdf_at(S, 0, mid).x <= r0) left = mid + 1; else right = mid;
@@ -880,7 +893,9 @@ __device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float 
   int vo = left - 1;
   float2 a0 = cdf_at(S, 0, vo), a1 = cdf_at(S, 0, vo + 1);
   v = ((float)vo + (r0 - a0.x) / (a1.x - a0.x)) / (float)H;
-  left = 0; right = W + 1;
+  const bool rowGuide = (vo >= 0) & (vo < H);
+  env_bracket(rowGuide ? S.envGuide + (kEnvGuideRows + 1) + (size_t)vo * (kEnvGuideCols + 1) : nullptr,
+              kEnvGuideCols, r1, W + 1, left, right);
   while (left < right) {
     int mid = (left + right) >> 1;
     if (cdf_at(S, 1 + mid, vo).x <= r1) left = mid + 1; else right = mid;
