@@ -477,7 +477,7 @@ int32_t mtx_denoiser_destroy(mtx_handle denoiser);
 /* ------------------------------------------------------------ diagnostics */
 /* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on
  * the device: kind 0 sin(x), 1 cos(x), 2 pow(x, y), 3 atan2(x, y) with x the
- * ordinate, 4 acos(x), 5 exp(x). Device pointers, n elements, asynchronous on `stream`.
+ * ordinate, 4 acos(x), 5 exp(x), 6 and 7 the sine and cosine parts of sincos(x). Device pointers, n elements, asynchronous on `stream`.
  * No reference counterpart: it lets hosts check that device and host builds
  * of these functions agree bit for bit. */
 int32_t mtx_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n, void *stream);

@@ -30,7 +30,7 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define MTX_FM_HD __host__ __device__ inline
-/* The six public functions stay out of line: inlined at every call site of
+/* The public functions stay out of line: inlined at every call site of
  * the shading code their f64 temporaries multiply the kernels' register
  * pressure (and spills); as calls they cost a few instructions each. */
 #define MTX_FM_CALL __host__ __device__ inline __attribute__((noinline))
@@ -115,6 +115,24 @@ MTX_FM_CALL float cosf(float xf) {
   double r = reduce_pio2(x, q);
   double v = (q & 1) ? sin_poly(r) : cos_poly(r);
   return (float)(((q + 1) & 2) ? -v : v);
+}
+
+/* sin and cos of one argument from one reduction: the same two values as
+ * sinf(x) and cosf(x), for half the work of two calls. */
+struct SinCos {
+  float s, c;
+};
+MTX_FM_CALL SinCos sincosf(float xf) {
+  double x = xf;
+  if (is_nan(x) || is_inf(x)) return SinCos{ffrom(0x7fc00000u), ffrom(0x7fc00000u)};
+  int q;
+  double r = reduce_pio2(x, q);
+  double s = sin_poly(r), c = cos_poly(r);
+  double sv = (q & 1) ? c : s, cv = (q & 1) ? s : c;
+  SinCos out;
+  out.s = x == 0.0 ? xf : (float)((q & 2) ? -sv : sv);
+  out.c = (float)(((q + 1) & 2) ? -cv : cv);
+  return out;
 }
 
 /* e^t for finite t. */

@@ -15,7 +15,9 @@ __global__ void k_math_eval(uint32_t kind, const float *__restrict__ x, const fl
       case 2: r = mtxfm::powf(a, b); break;
       case 3: r = mtxfm::atan2f(a, b); break;
       case 4: r = mtxfm::acosf(a); break;
-      default: r = mtxfm::expf(a); break;
+      case 5: r = mtxfm::expf(a); break;
+      case 6: r = mtxfm::sincosf(a).s; break;
+      default: r = mtxfm::sincosf(a).c; break;
     }
     out[i] = r;
   }
@@ -26,7 +28,7 @@ __global__ void k_math_eval(uint32_t kind, const float *__restrict__ x, const fl
 extern "C" int32_t mtx_math_eval(uint32_t kind, const float *x, const float *y, float *out, uint64_t n,
                                  void *stream) {
   try {
-    MTX_REQUIRE(kind <= 5, MTX_INVALID_ARGUMENT, "kind must be 0..5");
+    MTX_REQUIRE(kind <= 7, MTX_INVALID_ARGUMENT, "kind must be 0..7");
     MTX_REQUIRE(n == 0 || (x && y && out), MTX_INVALID_ARGUMENT, "NULL buffer");
     if (!n) return MTX_OK;
     uint64_t blocks = (n + 255) / 256;

@@ -212,7 +212,8 @@ __device__ f3 ggx_vndf(f3 view, float ax, float ay, float sx, float sy) {
   float nz = (1.0f - sy) * span, omz = sy * span, opz = nz + (1.0f - vz);
   float radial = sqrtf(mxf(0.0f, opz * omz));
   float az = kTwoPi * sx;
-  f3 sn{sv.x + radial * mtxfm::cosf(az), sv.y + radial * mtxfm::sinf(az), nz};
+  const mtxfm::SinCos a = mtxfm::sincosf(az);
+  f3 sn{sv.x + radial * a.c, sv.y + radial * a.s, nz};
   return normalize(st * sn);
 }
 __device__ __forceinline__ float ggx_ndf(f3 h, float ax, float ay) {
@@ -243,8 +244,9 @@ __device__ f3 cosine_hemisphere(float sx, float sy) {
     float r, a;
     if (fabsf(x) > fabsf(y)) { r = x; a = kQuarterPi * (y / x); }
     else { r = y; a = kHalfPi - kQuarterPi * (x / y); }
-    dx = r * mtxfm::cosf(a);
-    dy = r * mtxfm::sinf(a);
+    const mtxfm::SinCos sc = mtxfm::sincosf(a);
+    dx = r * sc.c;
+    dy = r * sc.s;
   }
   return f3{dx, dy, sqrtf(mxf(0.0f, 1.0f - (dx * dx + dy * dy)))};
 }
@@ -784,7 +786,8 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
     float metalDraw = (lobe - cdf[LC]) / mxf(P.p[LM], 1e-12f);
     if (isMetal & (metalDraw < P.metalMsProb)) {
       float z = rx, radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
+      const mtxfm::SinCos a = mtxfm::sincosf(az);
+      l = f3{radial * a.c, radial * a.s, z};
     } else {
       f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
       if (h.z < 0.0f) h = -h;
@@ -798,7 +801,8 @@ __device__ bool sample_bsdf(const Prep &P, float rx, float ry, float lobe, f3 &d
       bool reflectMs = rx < ratio;
       float z = reflectMs ? mnf(rx / ratio, 0.99999994f) : -mnf((rx - ratio) / (1.0f - ratio), 0.99999994f);
       float radial = sqrtf(mxf(0.0f, 1.0f - z * z)), az = kTwoPi * ry;
-      l = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
+      const mtxfm::SinCos a = mtxfm::sincosf(az);
+      l = f3{radial * a.c, radial * a.s, z};
       transmission = !reflectMs;
     } else {
       f3 h = ggx_vndf(v, m.alphaX, m.alphaY, rx, ry);
@@ -904,7 +908,8 @@ __device__ f3 sample_env(const GpuScene &S, float r0, float r1, float &u, float 
   float2 b0 = cdf_at(S, 1 + uo, vo), b1 = cdf_at(S, 1 + uo + 1, vo);
   u = ((float)uo + (r1 - b0.x) / (b1.x - b0.x)) / (float)W;
   float phi = u * kTwoPi, theta = v * kPi;
-  float ct = mtxfm::cosf(theta), st = mtxfm::sinf(theta), cp = mtxfm::cosf(phi), sp = mtxfm::sinf(phi);
+  const mtxfm::SinCos t = mtxfm::sincosf(theta), f = mtxfm::sincosf(phi);
+  const float ct = t.c, st = t.s, cp = f.c, sp = f.s;
   pdf = env_solid_pdf(b0.y, a0.y, v);
   return f3{-st * cp, ct, -st * sp};
 }
@@ -962,7 +967,8 @@ __device__ void sample_sphere(const float *__restrict__ L, f3 p, float rx, float
     const float z = 1.0f - 2.0f * rx;
     const float radial = sqrtf(mxf(0.0f, 1.0f - z * z));
     const float az = kTwoPi * ry;
-    sd = f3{radial * mtxfm::cosf(az), radial * mtxfm::sinf(az), z};
+    const mtxfm::SinCos a = mtxfm::sincosf(az);
+    sd = f3{radial * a.c, radial * a.s, z};
     r.pdf = kInv4Pi;
   } else {
     const f3 axis = co / cd;
@@ -974,7 +980,8 @@ __device__ void sample_sphere(const float *__restrict__ L, f3 p, float rx, float
     const float st = sqrtf(mxf(0.0f, omct * (2.0f - omct)));
     const float az = kTwoPi * ry;
     const Basis B = make_basis(axis);
-    sd = axis * ct + B.t * (st * mtxfm::cosf(az)) + B.b * (st * mtxfm::sinf(az));
+    const mtxfm::SinCos a = mtxfm::sincosf(az);
+    sd = axis * ct + B.t * (st * a.c) + B.b * (st * a.s);
     r.pdf = 1.0f / mxf(kTwoPi * omc, 1e-8f);
   }
   const float cp = dot(sd, co), c2 = dot(co, co);
@@ -1011,14 +1018,16 @@ __device__ void sample_area_point(const float *__restrict__ L, f3 p, f3 surf, f3
 __device__ void sample_disk(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
   const f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
   const float rr = sqrtf(sx), phi = kTwoPi * sy;
-  const f3 surf = pos + U * (rr * mtxfm::cosf(phi)) + V * (rr * mtxfm::sinf(phi));
+  const mtxfm::SinCos a = mtxfm::sincosf(phi);
+  const f3 surf = pos + U * (rr * a.c) + V * (rr * a.s);
   sample_area_point(L, p, surf, normalize(cross(U, V)), r);
 }
 __device__ void sample_cylinder(const float *__restrict__ L, f3 p, float sx, float sy, LSample &r) {
   const f3 U{L[4], L[5], L[6]}, V{L[7], L[8], L[9]}, pos{L[1], L[2], L[3]};
   const f3 axis = normalize(U), r0 = normalize(V), r1 = normalize(cross(axis, r0));
   const float phi = kTwoPi * sy;
-  const f3 normal = r0 * mtxfm::cosf(phi) + r1 * mtxfm::sinf(phi);
+  const mtxfm::SinCos a = mtxfm::sincosf(phi);
+  const f3 normal = r0 * a.c + r1 * a.s;
   const f3 surf = pos + U * (sx - 0.5f) + normal * L[13];
   sample_area_point(L, p, surf, normal, r);
 }
@@ -1330,8 +1339,9 @@ __device__ __forceinline__ void sample_circle(float ux, float uy, float &cx, flo
   const bool greater = fabsf(px) > fabsf(py);
   const float r = greater ? px : py;
   const float theta = greater ? kQuarterPi * py / px : kPi * (0.5f - 0.25f * px / py);
-  cx = r * mtxfm::cosf(theta);
-  cy = r * mtxfm::sinf(theta);
+  const mtxfm::SinCos a = mtxfm::sincosf(theta);
+  cx = r * a.c;
+  cy = r * a.s;
 }
 
 // main (rayTrace.compute.ts:1188-1205): pixel jitter (Sobol CAMERA_PIXEL

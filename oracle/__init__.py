@@ -34,8 +34,8 @@ def lib():
 
 
 def math_eval(kind, x, y=None):
-    """include/mtx_fmath.h on the host: kind in {"sin","cos","pow","atan2","acos","exp"}."""
-    k = ["sin", "cos", "pow", "atan2", "acos", "exp"].index(kind)
+    """include/mtx_fmath.h on the host: kind in {"sin","cos","pow","atan2","acos","exp","sincos.s","sincos.c"}."""
+    k = ["sin", "cos", "pow", "atan2", "acos", "exp", "sincos.s", "sincos.c"].index(kind)
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
     out = np.empty_like(x)

@@ -263,7 +263,9 @@ extern "C" void orc_math_eval(uint32_t kind, const float *x, const float *y, flo
       case 2: out[i] = mtxfm::powf(x[i], y[i]); break;
       case 3: out[i] = mtxfm::atan2f(x[i], y[i]); break;
       case 4: out[i] = mtxfm::acosf(x[i]); break;
-      default: out[i] = mtxfm::expf(x[i]); break;
+      case 5: out[i] = mtxfm::expf(x[i]); break;
+      case 6: out[i] = mtxfm::sincosf(x[i]).s; break;
+      default: out[i] = mtxfm::sincosf(x[i]).c; break;
     }
   }
 }

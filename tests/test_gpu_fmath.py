@@ -16,7 +16,8 @@ def test_device_matches_host(orc):
     xs = {
         "sin": (rng.uniform(-50, 50, n), None), "cos": (rng.uniform(-50, 50, n), None),
         "pow": (rng.uniform(0, 4, n), rng.uniform(-8, 8, n)), "atan2": (rng.uniform(-3, 3, n), rng.uniform(-3, 3, n)),
-        "acos": (rng.uniform(-1, 1, n), None), "exp": (rng.uniform(-100, 88, n), None)}
+        "acos": (rng.uniform(-1, 1, n), None), "exp": (rng.uniform(-100, 88, n), None),
+        "sincos.s": (rng.uniform(-50, 50, n), None), "sincos.c": (rng.uniform(-50, 50, n), None)}
     for kind, (x, y) in xs.items():
         x = x.astype(np.float32)
         y = np.zeros_like(x) if y is None else y.astype(np.float32)
@@ -24,7 +25,7 @@ def test_device_matches_host(orc):
         host = orc.math_eval(kind, x, y)
         dx, dy = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
         out = torch.empty_like(dx)
-        k = ["sin", "cos", "pow", "atan2", "acos", "exp"].index(kind)
+        k = ["sin", "cos", "pow", "atan2", "acos", "exp", "sincos.s", "sincos.c"].index(kind)
         abi.check(abi.lib().mtx_math_eval(k, C.c_void_p(dx.data_ptr()), C.c_void_p(dy.data_ptr()),
                                           C.c_void_p(out.data_ptr()), n, None))
         torch.cuda.synchronize()

@@ -66,6 +66,13 @@ def test_exp(orc, inputs):
     assert sp[5] == np.inf and sp[6] == 0
 
 
+def test_sincos_equals_sin_and_cos(orc, inputs):
+    _, x = inputs
+    x = np.concatenate([x, np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e6, -3e5], np.float32)])
+    for part, ref in (("sincos.s", "sin"), ("sincos.c", "cos")):
+        assert np.array_equal(orc.math_eval(part, x).view(np.uint32), orc.math_eval(ref, x).view(np.uint32)), part
+
+
 def test_special_cases(orc):
     inf, nan = np.float32(np.inf), np.float32(np.nan)
     z, nz = np.float32(0.0), np.float32(-0.0)
