@@ -347,6 +347,7 @@ typedef struct mtx_wavefront_tuning {
   uint32_t graph_captures;         /* get only: graphs captured */
   uint32_t tail_paths;             /* once an event's queue holds at most this many paths, one kernel
                                       finishes them all (0 = never) */
+  uint32_t queue_sort;             /* 1: group each event's continuation queue by ray direction octant */
 } mtx_wavefront_tuning;
 int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
 int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);

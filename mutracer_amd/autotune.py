@@ -28,9 +28,10 @@ SPACE = {
     "shade_oversubscription": [1, 2],
     "max_paths_in_flight": [1 << 23, 1 << 24, 1 << 25],
     "tail_paths": [0, 16384, 65536, 262144],
+    "queue_sort": [0, 1],
 }
-ORDER = ["max_paths_in_flight", "tail_paths", "trace_kernel", "trace_oversubscription", "shade_waves_per_simd",
-         "shade_oversubscription", "lds_stack"]
+ORDER = ["max_paths_in_flight", "tail_paths", "queue_sort", "trace_kernel", "trace_oversubscription",
+         "shade_waves_per_simd", "shade_oversubscription", "lds_stack"]
 
 
 def _key(target):

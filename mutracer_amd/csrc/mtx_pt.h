@@ -91,7 +91,14 @@ constexpr int kWfFetchBase = kWfShadowBase + kWfMaxEvents + 1;
 constexpr int kWfXcdBase = kWfFetchBase + 2 * (kWfMaxEvents + 1);
 constexpr int kWfXcds = 8;  // XCDs of an MI355X (counter segments per queue)
 constexpr int kWfTailFlag = kWfXcdBase + 2 * (kWfMaxEvents + 1) * kWfXcds;
-constexpr int kWfCounters = kWfTailFlag + 1;
+// Queue sorting: the shade kernel appends each continuing path to the bin of
+// its ray direction's octant, counts[kWfSortBase + kWfSortBins * e + o]
+// entries for queue e, and a gather kernel concatenates the bins into the
+// queue, so a wave's rays share a direction octant (near/far child order and
+// nodes in common) while keeping their pixel order within the octant.
+constexpr int kWfSortBins = 8;
+constexpr int kWfSortBase = kWfTailFlag + 1;
+constexpr int kWfCounters = kWfSortBase + kWfSortBins * (kWfMaxEvents + 2);
 struct WfBuffers {
   float4 *o, *d, *li, *beta, *alb, *nrm, *hitA;
   int2 *hitB;
@@ -100,6 +107,7 @@ struct WfBuffers {
   int32_t *s3;
   uint16_t *item;   // job slot of each path (rounds with several jobs)
   uint4 *rng;       // pcg4d state of each path (random sampling pattern)
+  uint32_t *qSort;  // kWfSortBins bins of nPaths entries (queue sorting on), else null
   uint32_t *counts;
   uint32_t nPaths;  // paths in flight this generation
   struct Owner {
@@ -109,10 +117,11 @@ struct WfBuffers {
     DevBuf<int32_t> s3;
     DevBuf<uint16_t> item;
     DevBuf<uint4> rng;
-    uint32_t capacity = 0;
-    void ensure(uint32_t paths);
+    DevBuf<uint32_t> qSort;
+    uint32_t capacity = 0, sortCapacity = 0;
+    void ensure(uint32_t paths, bool sort);
     size_t bytes() const;
-    WfBuffers view(uint32_t nPaths) const;
+    WfBuffers view(uint32_t nPaths, bool sort) const;
   };
 };
 
@@ -149,6 +158,7 @@ struct WfConfig {
   int shadeWaves = 2;            // occupancy target of the shade kernel (1..4 waves per SIMD)
   bool graphs = true;            // capture each generation's launch sequence into a hipGraph and replay it
   uint32_t tailPaths = 65536;    // the tail kernel finishes the paths once a queue is this short (0 = never)
+  bool queueSort = false;        // continuation queues grouped by ray direction octant (measured slower at 1080p)
 };
 
 // Traversal work totals of an instrumented launch (acc.work): one group of

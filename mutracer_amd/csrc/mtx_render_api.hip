@@ -772,6 +772,7 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   T->wfConfig.shadeWaves = (int)t->shade_waves_per_simd;
   T->wfConfig.graphs = t->use_graphs != 0;
   T->wfConfig.tailPaths = t->tail_paths;
+  T->wfConfig.queueSort = t->queue_sort != 0;
   T->graph.reset();
   return MTX_OK;
   API_CATCH(nullptr, 0)
@@ -792,6 +793,7 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->graph_replays = (uint32_t)T->graph.replays;
   t->graph_captures = (uint32_t)T->graph.captures;
   t->tail_paths = T->wfConfig.tailPaths;
+  t->queue_sort = T->wfConfig.queueSort ? 1u : 0u;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

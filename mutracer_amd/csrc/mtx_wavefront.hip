@@ -238,8 +238,20 @@ __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaun
       alive = shade_event(S, P, lut, pr, h, event, smp, sr);
       store_path(W, p, pr);
     }
-    const uint32_t slot = wave_append(nNext, active && alive);
-    if (active && alive) qNext[slot] = p;
+    if (W.qSort) {
+      const uint32_t oct = active ? (pr.rd.x < 0.0f ? 1u : 0u) | (pr.rd.y < 0.0f ? 2u : 0u) |
+                                        (pr.rd.z < 0.0f ? 4u : 0u)
+                                  : 0u;
+      uint32_t *bins = W.counts + kWfSortBase + kWfSortBins * (event + 1);
+      for (uint32_t o = 0; o < (uint32_t)kWfSortBins; o++) {
+        const bool mine = active && alive && oct == o;
+        const uint32_t slot = wave_append(bins + o, mine);
+        if (mine) W.qSort[(size_t)o * W.nPaths + slot] = p;
+      }
+    } else {
+      const uint32_t slot = wave_append(nNext, active && alive);
+      if (active && alive) qNext[slot] = p;
+    }
     const uint32_t sslot = wave_append(nShadow, active && sr.valid);
     if (active && sr.valid) {
       W.s0[sslot] = make_float4(sr.ro.x, sr.ro.y, sr.ro.z, sr.tMax);
@@ -247,6 +259,25 @@ __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaun
       W.s2[sslot] = make_float4(sr.payload.x, sr.payload.y, sr.payload.z, intf(pr.curTri));
       W.s3[sslot] = pr.curSlot;
     }
+  }
+}
+
+// Concatenates the octant bins of queue `event` into the queue in bin order
+// and sets the queue's length.
+__global__ void __launch_bounds__(256) k_wf_gather(WfBuffers W, int event) {
+  if (W.counts[kWfTailFlag]) return;
+  const uint32_t *bins = W.counts + kWfSortBase + kWfSortBins * event;
+  uint32_t start[kWfSortBins + 1];
+  start[0] = 0;
+#pragma unroll
+  for (int o = 0; o < kWfSortBins; o++) start[o + 1] = start[o] + bins[o];
+  const uint32_t total = start[kWfSortBins];
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[event] = total;
+  uint32_t *q = queue_of(W, event);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    uint32_t o = 0;
+    while (i >= start[o + 1]) o++;
+    q[i] = W.qSort[(size_t)o * W.nPaths + (i - start[o])];
   }
 }
 
@@ -500,7 +531,11 @@ static int cu_count() {
   return cus;
 }
 
-void WfBuffers::Owner::ensure(uint32_t paths) {
+void WfBuffers::Owner::ensure(uint32_t paths, bool sort) {
+  if (sort && sortCapacity < paths) {
+    qSort.alloc((size_t)paths * kWfSortBins);
+    sortCapacity = paths;
+  }
   if (paths <= capacity) return;
   o.alloc(paths); d.alloc(paths); li.alloc(paths); beta.alloc(paths); alb.alloc(paths); nrm.alloc(paths);
   hitA.alloc(paths); hitB.alloc(paths); qA.alloc(paths); qB.alloc(paths);
@@ -510,13 +545,14 @@ void WfBuffers::Owner::ensure(uint32_t paths) {
 }
 size_t WfBuffers::Owner::bytes() const {
   return o.bytes() * 6 + hitA.bytes() + hitB.bytes() + qA.bytes() * 2 + s0.bytes() * 3 + s3.bytes() + item.bytes() +
-         rng.bytes() + counts.bytes();
+         rng.bytes() + qSort.bytes() + counts.bytes();
 }
-WfBuffers WfBuffers::Owner::view(uint32_t nPaths) const {
+WfBuffers WfBuffers::Owner::view(uint32_t nPaths, bool sort) const {
   WfBuffers W;
   W.o = o.ptr; W.d = d.ptr; W.li = li.ptr; W.beta = beta.ptr; W.alb = alb.ptr; W.nrm = nrm.ptr;
   W.hitA = hitA.ptr; W.hitB = hitB.ptr; W.qA = qA.ptr; W.qB = qB.ptr;
   W.s0 = s0.ptr; W.s1 = s1.ptr; W.s2 = s2.ptr; W.s3 = s3.ptr; W.item = item.ptr; W.rng = rng.ptr;
+  W.qSort = sort ? qSort.ptr : nullptr;
   W.counts = counts.ptr;
   W.nPaths = nPaths;
   return W;
@@ -634,6 +670,7 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
       case 4: hipLaunchKernelGGL((k_wf_shade<kMulti, 4>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
       default: hipLaunchKernelGGL((k_wf_shade<kMulti, 1>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
     }
+    if (W.qSort && e < eventLimit) hipLaunchKernelGGL(k_wf_gather, dim3(genGrid), dim3(256), 0, s, W, e + 1);
     te();
     tb(3);
     if (cfg.walker)
@@ -678,8 +715,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
   while (done < p0.samples) {
     const uint32_t gen = std::min(genMax, p0.samples - done);
     const uint32_t nPaths = gen * p0.pixelCount;
-    buf.ensure(nPaths);
-    const WfBuffers W = buf.view(nPaths);
+    buf.ensure(nPaths, cfg.queueSort);
+    const WfBuffers W = buf.view(nPaths, cfg.queueSort);
     WfLaunch L;
     std::memset((void *)&L, 0, sizeof L);  // padding too: the bytes key the graph cache
     L.one.scene = scene;
@@ -710,8 +747,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       key = fnv(key, ptrs, sizeof ptrs);
       key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
       key = fnv(key, &W.nPaths, sizeof W.nPaths);
-      const int64_t cf[8] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
-                             cfg.shadeWaves, cfg.graphs, cfg.tailPaths};
+      const int64_t cf[9] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
+                             cfg.shadeWaves, cfg.graphs, cfg.tailPaths, cfg.queueSort};
       key = fnv(key, cf, sizeof cf);
       key = fnv(key, &spill, sizeof spill);
       key = fnv(key, &eventLimit, sizeof eventLimit);
@@ -744,8 +781,8 @@ void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t n
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
   int32_t *spill = spill_for(s, std::max(std::max(g.trace, g.walk), g.tail), kWfTraceBlock);
-  buf.ensure(nPaths);
-  const WfBuffers W = buf.view(nPaths);
+  buf.ensure(nPaths, cfg.queueSort);
+  const WfBuffers W = buf.view(nPaths, cfg.queueSort);
   WfLaunch L{};
   L.jobs = devSlots;
   L.nJobs = nJobs;

@@ -172,7 +172,8 @@ class RenderTarget:
         return dict(max_paths_in_flight=t.max_paths_in_flight, trace_kernel=t.trace_kernel, lds_stack=t.lds_stack,
                     trace_oversubscription=t.trace_oversubscription, shade_oversubscription=t.shade_oversubscription,
                     shade_waves_per_simd=t.shade_waves_per_simd, use_graphs=t.use_graphs,
-                    graph_replays=t.graph_replays, graph_captures=t.graph_captures, tail_paths=t.tail_paths)
+                    graph_replays=t.graph_replays, graph_captures=t.graph_captures, tail_paths=t.tail_paths,
+                    queue_sort=t.queue_sort)
 
     def set_tuning(self, **kw):
         """Changes the wavefront launch shape (keys as returned by tuning())."""
@@ -188,6 +189,7 @@ class RenderTarget:
             "max_paths_in_flight", "trace_kernel", "lds_stack", "trace_oversubscription", "shade_oversubscription",
             "shade_waves_per_simd", "use_graphs")])
         t.tail_paths = int(cur["tail_paths"])
+        t.queue_sort = int(cur["queue_sort"])
         abi.check(abi.lib().mtx_target_set_tuning(self.handle.value, C.byref(t)))
 
     def synchronize(self):
