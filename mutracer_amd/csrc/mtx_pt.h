@@ -39,12 +39,14 @@ struct GpuScene {
   uint32_t envW, envH;
   const float *lut;            // 32 x 2177 energy tables
   const float4 *emissive;      // 3 per emissive triangle: (p0, pmf), (p1, area), (p2, material), world space
-  uint32_t emissiveCount, pad0;
+  uint32_t emissiveCount;
+  uint32_t features;           // kSceneFull when a material has fuzz, subsurface or a thin wall
   const uint16_t *envGuide;    // search brackets of envCdf (kEnvGuideRows + 1, then envH x (kEnvGuideCols + 1)), or null
 };
 // Bins of the environment CDF guide tables: bin k of the marginal (of row
 // y's conditional) CDF holds the index its search returns for the sample
 // k / bins, so a sample in bin k searches only between bins k and k + 1.
+constexpr uint32_t kSceneFull = 1u;
 constexpr uint32_t kEnvGuideRows = 1024;
 constexpr uint32_t kEnvGuideCols = 128;
 
@@ -201,9 +203,14 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const
                              WfGraph *graph = nullptr);
 // One generation of several jobs (slots on the device, totals given): paths
 // = sum of pathCount, pixels = sum of pixelCount, events = the largest job's
-// event limit.
+// event limit; `full`: some job's scene has kSceneFull or its sampling
+// pattern is the random one.
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, LaunchTimer *timer = nullptr);
+                             hipStream_t s, bool full, LaunchTimer *timer = nullptr);
+// Whether one job needs the full shading kernels.
+inline bool needs_full_shading(const GpuScene &scene, const RenderParams &p) {
+  return (scene.features & kSceneFull) != 0u || p.pattern == 2u;
+}
 
 }  // namespace mtx

@@ -380,6 +380,12 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   v.envGuide = hasEnv ? R->envGuide.ptr : nullptr;
   v.emissive = E.weight.empty() ? nullptr : R->emissive.ptr;
   v.emissiveCount = (uint32_t)E.weight.size();
+  v.features = 0;
+  for (uint32_t m = 0; m < D.material_count; m++) {
+    const float *mat = D.materials + (size_t)m * MTX_MATERIAL_FLOATS;
+    if (mat[MTX_MAT_FUZZ_WEIGHT] > 0.0f || mat[MTX_MAT_SUBSURFACE_WEIGHT] > 0.0f || mat[MTX_MAT_THIN_WALLED] > 0.5f)
+      v.features |= kSceneFull;
+  }
   v.envW = hasEnv ? D.env_width : 0;
   v.envH = hasEnv ? D.env_height : 0;
   v.lut = R->lut.ptr;
@@ -962,8 +968,11 @@ static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
   idle = slots.empty();
   uint32_t finished = 0;
   if (!idle) {
+    bool full = false;
+    for (const JobSlot &js : slots) full = full || needs_full_shading(js.scene, js.params);
     S.slots.upload(slots.data(), slots.size(), s);
-    launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg, s);
+    launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg, s,
+                            full);
     MTX_HIP(hipStreamSynchronize(s));
     const double t = now_ms();
     for (auto &pj : parts) {

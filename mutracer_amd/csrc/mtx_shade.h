@@ -291,6 +291,9 @@ __device__ __forceinline__ void rough_alpha(float r, float aniso, float &ax, flo
   ay = mxf(kAlphaMin, c * at);
 }
 
+// kFull = false: the scene has no fuzz, subsurface or thin-walled material
+// (its kernels are compiled without those lobes).
+template <bool kFull>
 __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   Mat ma;
   ma.baseColor = f3{m[1], m[2], m[3]};
@@ -311,13 +314,13 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.coatRoughness = clampf(m[20], kRoughnessMin, 1.0f);
   ma.coatIor = mxf(m[21], 1.0f);
   ma.coatDarkening = clampf(m[22], 0.0f, 1.0f);
-  ma.fuzzWeight = clampf(m[MTX_MAT_FUZZ_WEIGHT], 0.0f, 1.0f);
+  ma.fuzzWeight = kFull ? clampf(m[MTX_MAT_FUZZ_WEIGHT], 0.0f, 1.0f) : 0.0f;
   ma.fuzzColor = vmaxs(f3{m[MTX_MAT_FUZZ_COLOR], m[MTX_MAT_FUZZ_COLOR + 1], m[MTX_MAT_FUZZ_COLOR + 2]}, 0.0f);
   ma.fuzzRoughness = clampf(m[MTX_MAT_FUZZ_ROUGHNESS], 0.0f, 1.0f);
-  ma.subsurfaceWeight = clampf(m[MTX_MAT_SUBSURFACE_WEIGHT], 0.0f, 1.0f);
+  ma.subsurfaceWeight = kFull ? clampf(m[MTX_MAT_SUBSURFACE_WEIGHT], 0.0f, 1.0f) : 0.0f;
   ma.subsurfaceColor = vmaxs(f3{m[MTX_MAT_SUBSURFACE_COLOR], m[MTX_MAT_SUBSURFACE_COLOR + 1],
                                 m[MTX_MAT_SUBSURFACE_COLOR + 2]}, 0.0f);
-  ma.thinWalled = m[MTX_MAT_THIN_WALLED] > 0.5f;
+  ma.thinWalled = kFull && m[MTX_MAT_THIN_WALLED] > 0.5f;
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
     ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
@@ -1411,8 +1414,13 @@ __device__ __forceinline__ int trace_depth(int event) { return event > 0 ? event
 // ray is added to pr.li; the vertex's NEE shadow ray, if any, is returned in
 // `sr` (its payload is added to pr.li by the caller when unoccluded, before
 // the next event). Returns true when the path continues with pr.ro / pr.rd.
+// kFull = false compiles the event for scenes without fuzz, subsurface and
+// thin-walled materials and without the random pattern; the launcher picks
+// it only then, so it computes the same values as the full form.
+template <bool kFull = true>
 __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut &L, PathRegs &pr, const TraceHit &h,
                             int event, Smp &smp, ShadowReq &sr) {
+  if (!kFull && smp.pattern == kPatternRandom) __builtin_unreachable();
   sr.valid = false;
   Surf si;
   si.hit = h.hit;
@@ -1422,7 +1430,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
   Mat ma;
   if (h.hit) {
     complete_surface(S, h, si);
-    ma = load_material(S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS, si.color);
+    ma = load_material<kFull>(S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS, si.color);
   }
   pr.curTri = si.tri;
   pr.curSlot = si.slot;

@@ -185,7 +185,10 @@ k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill
 
 // kWaves: occupancy the register allocator must reach (waves per SIMD); the
 // allocator spills to reach it, so the best value is measured, not assumed.
-template <bool kMulti, int kWaves>
+// kFull: compiled with the fuzz, subsurface and thin-wall lobes and the
+// random pattern (the launcher passes false when no job of the generation
+// uses them).
+template <bool kMulti, int kWaves, bool kFull>
 __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaunch L, WfBuffers W, int event) {
   const uint32_t n = W.counts[event];
   if (n == 0u || W.counts[kWfTailFlag]) return;
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(kWfShadeBlock, kWaves) k_wf_shade(const WfLaun
         h.position = oo + h.t * od;
       }
       const Lut lut{S.lut};
-      alive = shade_event(S, P, lut, pr, h, event, smp, sr);
+      alive = shade_event<kFull>(S, P, lut, pr, h, event, smp, sr);
       store_path(W, p, pr);
     }
     if (W.qSort) {
@@ -331,7 +334,7 @@ k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spil
 // decision; the flag it raises makes the remaining wavefront launches of the
 // generation return at once. Per-path arithmetic is that of the wavefront
 // kernels, so results do not depend on where the tail starts.
-template <bool kMulti, bool kCount>
+template <bool kMulti, bool kCount, bool kFull>
 __global__ void __launch_bounds__(kWfShadeBlock)
 k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kMinLdsStack * kWfShadeBlock];
@@ -369,7 +372,7 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
                               h, &tc);
       if (kCount) nHits += h.hit ? 1u : 0u;
       ShadowReq sr;
-      const bool alive = shade_event(S, P, lut, pr, h, e, smp, sr);
+      const bool alive = shade_event<kFull>(S, P, lut, pr, h, e, smp, sr);
       if (sr.valid) {
         TraceHit sh;
         rays++;
@@ -614,8 +617,8 @@ static Grids grids_for(const WfConfig &cfg) {
   }
   const int sw = std::min(4, std::max(1, cfg.shadeWaves));
   if (!shadeBpc[sw]) {
-    const void *k = sw == 1 ? (const void *)k_wf_shade<false, 1> : sw == 2 ? (const void *)k_wf_shade<false, 2>
-                  : sw == 3 ? (const void *)k_wf_shade<false, 3> : (const void *)k_wf_shade<false, 4>;
+    const void *k = sw == 1 ? (const void *)k_wf_shade<false, 1, true> : sw == 2 ? (const void *)k_wf_shade<false, 2, true>
+                  : sw == 3 ? (const void *)k_wf_shade<false, 3, true> : (const void *)k_wf_shade<false, 4, true>;
     shadeBpc[sw] = blocks_per_cu(k, kWfShadeBlock, 0);
   }
   if (!walkBpc[li]) {
@@ -624,7 +627,7 @@ static Grids grids_for(const WfConfig &cfg) {
                             : (const void *)k_wf_walk<false, false, false, 32>;
     walkBpc[li] = blocks_per_cu(k, kWfTraceBlock, 0);
   }
-  if (!tailBpc) tailBpc = blocks_per_cu((const void *)k_wf_tail<false, false>, kWfShadeBlock, 0);
+  if (!tailBpc) tailBpc = blocks_per_cu((const void *)k_wf_tail<false, false, true>, kWfShadeBlock, 0);
   Grids g;
   g.cus = (uint32_t)cus;
   g.tail = (uint32_t)(cus * tailBpc);
@@ -634,10 +637,33 @@ static Grids grids_for(const WfConfig &cfg) {
   return g;
 }
 
-// Launches one generation: generate, the event loop, accumulate.
+template <bool kMulti, bool kFull>
+static void launch_shade(const WfConfig &cfg, const Grids &g, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
+                         int e) {
+  switch (cfg.shadeWaves) {
+    case 2: hipLaunchKernelGGL((k_wf_shade<kMulti, 2, kFull>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+    case 3: hipLaunchKernelGGL((k_wf_shade<kMulti, 3, kFull>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+    case 4: hipLaunchKernelGGL((k_wf_shade<kMulti, 4, kFull>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+    default: hipLaunchKernelGGL((k_wf_shade<kMulti, 1, kFull>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
+  }
+}
+template <bool kMulti, bool kFull>
+static void launch_tail(const WfConfig &cfg, const Grids &g, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
+                        int e, int32_t *spill) {
+  if (L.work)
+    hipLaunchKernelGGL((k_wf_tail<kMulti, true, kFull>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+                       cfg.tailPaths, spill);
+  else
+    hipLaunchKernelGGL((k_wf_tail<kMulti, false, kFull>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+                       cfg.tailPaths, spill);
+}
+
+// Launches one generation: generate, the event loop, accumulate. `full`:
+// some job uses fuzz, subsurface or thin-walled materials or the random
+// pattern (else the shading kernels compiled without them run).
 template <bool kMulti>
 static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit, uint32_t pixels, const WfConfig &cfg,
-                           const Grids &g, int32_t *spill, hipStream_t s, LaunchTimer *timer) {
+                           const Grids &g, int32_t *spill, hipStream_t s, LaunchTimer *timer, bool full) {
   auto tb = [&](int c) { if (timer) timer->begin(c); };
   auto te = [&]() { if (timer) timer->end(); };
   const uint32_t genGrid = std::min<uint32_t>((W.nPaths + 255) / 256, g.cus * 16u);
@@ -649,12 +675,8 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
   for (int e = 0; e <= eventLimit; e++) {
     if (cfg.tailPaths) {
       tb(5);
-      if (L.work)
-        hipLaunchKernelGGL((k_wf_tail<kMulti, true>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e, cfg.tailPaths,
-                           spill);
-      else
-        hipLaunchKernelGGL((k_wf_tail<kMulti, false>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
-                           cfg.tailPaths, spill);
+      if (full) launch_tail<kMulti, true>(cfg, g, s, L, W, e, spill);
+      else launch_tail<kMulti, false>(cfg, g, s, L, W, e, spill);
       te();
     }
     tb(1);
@@ -664,12 +686,8 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
       launch_trace<kMulti, false>(cfg, g.trace, s, L, W, e, spill);
     te();
     tb(2);
-    switch (cfg.shadeWaves) {
-      case 2: hipLaunchKernelGGL((k_wf_shade<kMulti, 2>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
-      case 3: hipLaunchKernelGGL((k_wf_shade<kMulti, 3>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
-      case 4: hipLaunchKernelGGL((k_wf_shade<kMulti, 4>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
-      default: hipLaunchKernelGGL((k_wf_shade<kMulti, 1>), dim3(g.shade), dim3(kWfShadeBlock), 0, s, L, W, e); break;
-    }
+    if (full) launch_shade<kMulti, true>(cfg, g, s, L, W, e);
+    else launch_shade<kMulti, false>(cfg, g, s, L, W, e);
     if (W.qSort && e < eventLimit) hipLaunchKernelGGL(k_wf_gather, dim3(genGrid), dim3(256), 0, s, W, e + 1);
     te();
     tb(3);
@@ -706,6 +724,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
                              WfBuffers::Owner &buf, const WfConfig &cfg, hipStream_t s, LaunchTimer *timer,
                              WfGraph *graph) {
   if (!p0.pixelCount || !p0.samples) return;
+  const bool full = needs_full_shading(scene, p0);
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
   const Grids g = grids_for(cfg);
@@ -757,7 +776,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
         graph->reset();
         hipGraph_t gr = nullptr;
         MTX_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
-        run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, gs, nullptr);
+        run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, gs, nullptr, full);
         MTX_HIP(hipStreamEndCapture(gs, &gr));
         hipError_t e = hipGraphInstantiate(&graph->exec, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -768,7 +787,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       MTX_HIP(hipGraphLaunch(graph->exec, gs));
       graph->replays++;
     } else {
-      run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, s, timer);
+      run_generation<false>(L, W, eventLimit, p0.pixelCount, cfg, g, spill, s, timer, full);
     }
     done += gen;
   }
@@ -776,7 +795,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
 
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, LaunchTimer *timer) {
+                             hipStream_t s, bool full, LaunchTimer *timer) {
   if (!nJobs || !nPaths) return;
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
@@ -789,7 +808,7 @@ void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t n
   L.totalPixels = totalPixels;
   L.rays = rays;
   L.work = nullptr;
-  run_generation<true>(L, W, std::min(eventLimit, 50), totalPixels, cfg, g, spill, s, timer);
+  run_generation<true>(L, W, std::min(eventLimit, 50), totalPixels, cfg, g, spill, s, timer, full);
 }
 
 }  // namespace mtx
