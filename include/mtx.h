@@ -348,6 +348,8 @@ typedef struct mtx_wavefront_tuning {
   uint32_t tail_paths;             /* once an event's queue holds at most this many paths, one kernel
                                       finishes them all (0 = never) */
   uint32_t queue_sort;             /* 1: group each event's continuation queue by ray direction octant */
+  uint32_t lds_stack_closest;      /* LDS stack entries of the closest-hit trace kernel: 0 (= lds_stack),
+                                      16, 24 or 32; lds_stack then sets the any-hit kernel's */
 } mtx_wavefront_tuning;
 int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *tuning);
 int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *tuning);

@@ -34,7 +34,7 @@ class WavefrontTuning(C.Structure):
                 ("lds_stack", C.c_uint32), ("trace_oversubscription", C.c_uint32),
                 ("shade_oversubscription", C.c_uint32), ("shade_waves_per_simd", C.c_uint32),
                 ("use_graphs", C.c_uint32), ("graph_replays", C.c_uint32), ("graph_captures", C.c_uint32),
-                ("tail_paths", C.c_uint32), ("queue_sort", C.c_uint32)]
+                ("tail_paths", C.c_uint32), ("queue_sort", C.c_uint32), ("lds_stack_closest", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

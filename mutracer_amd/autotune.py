@@ -29,9 +29,10 @@ SPACE = {
     "max_paths_in_flight": [1 << 23, 1 << 24, 1 << 25],
     "tail_paths": [0, 16384, 65536, 262144],
     "queue_sort": [0, 1],
+    "lds_stack_closest": [16, 24, 32],
 }
 ORDER = ["max_paths_in_flight", "tail_paths", "queue_sort", "trace_kernel", "trace_oversubscription",
-         "shade_waves_per_simd", "shade_oversubscription", "lds_stack"]
+         "shade_waves_per_simd", "shade_oversubscription", "lds_stack", "lds_stack_closest"]
 
 
 def _key(target):

@@ -156,7 +156,8 @@ struct WfConfig {
   int traceOversub = 1;          // trace/shadow grid = CUs * resident blocks * this
   int shadeOversub = 1;
   bool walker = false;           // persistent lane-refill traversal (else one ray per thread)
-  int ldsStack = 16;             // LDS stack entries per lane of the traversal kernels (16, 24 or 32)
+  int ldsStack = 16;             // LDS stack entries per lane of the any-hit (and walker) kernels (16, 24 or 32)
+  int ldsStackClosest = 0;       // the same for the closest-hit trace kernel (0 = ldsStack)
   int shadeWaves = 2;            // occupancy target of the shade kernel (1..4 waves per SIMD)
   bool graphs = true;            // capture each generation's launch sequence into a hipGraph and replay it
   uint32_t tailPaths = 65536;    // the tail kernel finishes the paths once a queue is this short (0 = never)

@@ -762,6 +762,9 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   MTX_REQUIRE(t->trace_kernel <= 1, MTX_INVALID_ARGUMENT, "trace_kernel must be 0 or 1");
   MTX_REQUIRE(t->lds_stack == 16 || t->lds_stack == 24 || t->lds_stack == 32, MTX_INVALID_ARGUMENT,
               "lds_stack must be 16, 24 or 32");
+  MTX_REQUIRE(t->lds_stack_closest == 0 || t->lds_stack_closest == 16 || t->lds_stack_closest == 24 ||
+                  t->lds_stack_closest == 32,
+              MTX_INVALID_ARGUMENT, "lds_stack_closest must be 0, 16, 24 or 32");
   MTX_REQUIRE(t->trace_oversubscription >= 1 && t->trace_oversubscription <= 16 && t->shade_oversubscription >= 1 &&
                   t->shade_oversubscription <= 16,
               MTX_INVALID_ARGUMENT, "oversubscription must be in 1..16");
@@ -779,6 +782,7 @@ int32_t mtx_target_set_tuning(mtx_handle target, const mtx_wavefront_tuning *t) 
   T->wfConfig.graphs = t->use_graphs != 0;
   T->wfConfig.tailPaths = t->tail_paths;
   T->wfConfig.queueSort = t->queue_sort != 0;
+  T->wfConfig.ldsStackClosest = (int)t->lds_stack_closest;
   T->graph.reset();
   return MTX_OK;
   API_CATCH(nullptr, 0)
@@ -800,6 +804,7 @@ int32_t mtx_target_get_tuning(mtx_handle target, mtx_wavefront_tuning *t) {
   t->graph_captures = (uint32_t)T->graph.captures;
   t->tail_paths = T->wfConfig.tailPaths;
   t->queue_sort = T->wfConfig.queueSort ? 1u : 0u;
+  t->lds_stack_closest = (uint32_t)T->wfConfig.ldsStackClosest;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

@@ -596,25 +596,32 @@ static void launch_trace_lds(uint32_t grid, hipStream_t s, const WfLaunch &L, co
 template <bool kMulti, bool kShadow>
 static void launch_trace(const WfConfig &cfg, uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
                          int e, int32_t *spill) {
-  if (cfg.ldsStack <= 16) launch_trace_lds<kMulti, kShadow, 16>(grid, s, L, W, e, spill);
-  else if (cfg.ldsStack <= 24) launch_trace_lds<kMulti, kShadow, 24>(grid, s, L, W, e, spill);
+  const int depth = (!kShadow && cfg.ldsStackClosest) ? cfg.ldsStackClosest : cfg.ldsStack;
+  if (depth <= 16) launch_trace_lds<kMulti, kShadow, 16>(grid, s, L, W, e, spill);
+  else if (depth <= 24) launch_trace_lds<kMulti, kShadow, 24>(grid, s, L, W, e, spill);
   else launch_trace_lds<kMulti, kShadow, 32>(grid, s, L, W, e, spill);
 }
 
 struct Grids {
-  uint32_t trace, walk, shade, tail, cus;
+  uint32_t trace, shadow, walk, shade, tail, cus;
 };
 static Grids grids_for(const WfConfig &cfg) {
-  static int traceBpc[3] = {0, 0, 0}, shadeBpc[5] = {0, 0, 0, 0, 0}, walkBpc[3] = {0, 0, 0}, tailBpc = 0;
+  static int closestBpc[3] = {0, 0, 0}, shadowBpc[3] = {0, 0, 0}, shadeBpc[5] = {0, 0, 0, 0, 0},
+             walkBpc[3] = {0, 0, 0}, tailBpc = 0;
   const int cus = cu_count();
-  const int li = cfg.ldsStack <= 16 ? 0 : (cfg.ldsStack <= 24 ? 1 : 2);
-  if (!traceBpc[li]) {
-    const void *t = li == 0 ? (const void *)k_wf_trace<false, false, 16>
-                  : li == 1 ? (const void *)k_wf_trace<false, false, 24> : (const void *)k_wf_trace<false, false, 32>;
+  auto level = [](int d) { return d <= 16 ? 0 : (d <= 24 ? 1 : 2); };
+  const int li = level(cfg.ldsStack), lc = level(cfg.ldsStackClosest ? cfg.ldsStackClosest : cfg.ldsStack);
+  if (!closestBpc[lc]) {
+    const void *t = lc == 0 ? (const void *)k_wf_trace<false, false, 16>
+                  : lc == 1 ? (const void *)k_wf_trace<false, false, 24> : (const void *)k_wf_trace<false, false, 32>;
+    closestBpc[lc] = blocks_per_cu(t, kWfTraceBlock, 0);
+  }
+  if (!shadowBpc[li]) {
     const void *a = li == 0 ? (const void *)k_wf_shadow<false, false, 16>
                   : li == 1 ? (const void *)k_wf_shadow<false, false, 24> : (const void *)k_wf_shadow<false, false, 32>;
-    traceBpc[li] = std::max(blocks_per_cu(t, kWfTraceBlock, 0), blocks_per_cu(a, kWfTraceBlock, 0));
+    shadowBpc[li] = blocks_per_cu(a, kWfTraceBlock, 0);
   }
+
   const int sw = std::min(4, std::max(1, cfg.shadeWaves));
   if (!shadeBpc[sw]) {
     const void *k = sw == 1 ? (const void *)k_wf_shade<false, 1, true> : sw == 2 ? (const void *)k_wf_shade<false, 2, true>
@@ -631,7 +638,8 @@ static Grids grids_for(const WfConfig &cfg) {
   Grids g;
   g.cus = (uint32_t)cus;
   g.tail = (uint32_t)(cus * tailBpc);
-  g.trace = (uint32_t)(cus * traceBpc[li] * std::max(1, cfg.traceOversub));
+  g.trace = (uint32_t)(cus * closestBpc[lc] * std::max(1, cfg.traceOversub));
+  g.shadow = (uint32_t)(cus * shadowBpc[li] * std::max(1, cfg.traceOversub));
   g.walk = (uint32_t)(cus * walkBpc[li] * std::max(1, cfg.traceOversub));
   g.shade = (uint32_t)(cus * shadeBpc[sw] * std::max(1, cfg.shadeOversub));
   return g;
@@ -694,7 +702,7 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
     if (cfg.walker)
       launch_walk<kMulti, true>(cfg, g.walk, s, L, W, e, spill);
     else
-      launch_trace<kMulti, true>(cfg, g.trace, s, L, W, e, spill);
+      launch_trace<kMulti, true>(cfg, g.shadow, s, L, W, e, spill);
     te();
   }
   const uint32_t accGrid = std::min<uint32_t>((pixels + 255) / 256, g.cus * 16u);
@@ -728,7 +736,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
   const uint32_t maxPaths = std::max<uint32_t>(cfg.maxPaths, p0.pixelCount);
   const uint32_t genMax = std::max<uint32_t>(1u, maxPaths / p0.pixelCount);
   const Grids g = grids_for(cfg);
-  int32_t *spill = spill_for(s, std::max(std::max(g.trace, g.walk), g.tail), kWfTraceBlock);
+  int32_t *spill = spill_for(s, std::max(std::max(std::max(g.trace, g.shadow), g.walk), g.tail), kWfTraceBlock);
   const int eventLimit = std::min((int)p0.bounces * 3, 50);
   uint32_t done = 0;
   while (done < p0.samples) {
@@ -766,8 +774,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
       key = fnv(key, ptrs, sizeof ptrs);
       key = fnv(key, &L.one.acc.pixels, sizeof L.one.acc.pixels);
       key = fnv(key, &W.nPaths, sizeof W.nPaths);
-      const int64_t cf[9] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
-                             cfg.shadeWaves, cfg.graphs, cfg.tailPaths, cfg.queueSort};
+      const int64_t cf[10] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
+                              cfg.shadeWaves, cfg.graphs, cfg.tailPaths, cfg.queueSort, cfg.ldsStackClosest};
       key = fnv(key, cf, sizeof cf);
       key = fnv(key, &spill, sizeof spill);
       key = fnv(key, &eventLimit, sizeof eventLimit);
@@ -799,7 +807,7 @@ void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t n
   if (!nJobs || !nPaths) return;
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
-  int32_t *spill = spill_for(s, std::max(std::max(g.trace, g.walk), g.tail), kWfTraceBlock);
+  int32_t *spill = spill_for(s, std::max(std::max(std::max(g.trace, g.shadow), g.walk), g.tail), kWfTraceBlock);
   buf.ensure(nPaths, cfg.queueSort);
   const WfBuffers W = buf.view(nPaths, cfg.queueSort);
   WfLaunch L{};

@@ -173,7 +173,7 @@ class RenderTarget:
                     trace_oversubscription=t.trace_oversubscription, shade_oversubscription=t.shade_oversubscription,
                     shade_waves_per_simd=t.shade_waves_per_simd, use_graphs=t.use_graphs,
                     graph_replays=t.graph_replays, graph_captures=t.graph_captures, tail_paths=t.tail_paths,
-                    queue_sort=t.queue_sort)
+                    queue_sort=t.queue_sort, lds_stack_closest=t.lds_stack_closest)
 
     def set_tuning(self, **kw):
         """Changes the wavefront launch shape (keys as returned by tuning())."""
@@ -190,6 +190,7 @@ class RenderTarget:
             "shade_waves_per_simd", "use_graphs")])
         t.tail_paths = int(cur["tail_paths"])
         t.queue_sort = int(cur["queue_sort"])
+        t.lds_stack_closest = int(cur["lds_stack_closest"])
         abi.check(abi.lib().mtx_target_set_tuning(self.handle.value, C.byref(t)))
 
     def synchronize(self):

@@ -333,6 +333,8 @@ def test_wavefront_equals_megakernel_large(env, orc):
     dict(tail_paths=1500),
     dict(tail_paths=1 << 30),
     dict(trace_kernel=1, tail_paths=3000, max_paths_in_flight=9000),
+    dict(lds_stack_closest=16),
+    dict(lds_stack_closest=32, lds_stack=24),
     dict(queue_sort=1),
     dict(queue_sort=1, tail_paths=0, max_paths_in_flight=9000),
 ])
