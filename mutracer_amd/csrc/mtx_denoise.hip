@@ -28,6 +28,8 @@ namespace mtx {
 
 typedef _Float16 half4v __attribute__((ext_vector_type(4)));
 typedef short short4v __attribute__((ext_vector_type(4)));
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------- conversions
@@ -60,7 +62,9 @@ struct ConvArgs {
 // kMT: 16-pixel tiles per wave along the row. Each K step loads the NT weight
 // fragments once and reuses them for all kMT tiles (kMT x NT MFMAs per
 // 1 + ... loads), which is what keeps the matrix cores fed: with kMT = 1 the
-// wave issues one weight load per MFMA.
+// wave issues one weight load per MFMA. K steps take two 16-channel chunks
+// at a time on gfx950's 16x16x32 MFMA (16-byte loads, half the MFMA count);
+// an odd last chunk uses the 16x16x16 form.
 template <bool kBf16, int NT, int kMT>
 __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
   const int lane = threadIdx.x & 63;
@@ -77,10 +81,47 @@ __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[m][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
   const int W2 = a.W >> 1;
+  // Input channels of chunk kc at pixel (yy, xx): src1 (upsampled when up1)
+  // for the first kc1 chunks, then src2.
+  auto in_ptr = [&](int kc, int yy, int xx) -> const uint16_t * {
+    if (kc < a.kc1)
+      return (a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1 : a.src1 + ((size_t)yy * a.W + xx) * a.cs1) +
+             kc * 16;
+    return a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16;
+  };
+  const int kpairs = a.kchunks >> 1;
   for (int t = 0; t < 9; t++) {
     const int yy = y + t / 3 - 1, dx = t % 3 - 1;
     const bool rowIn = yy >= 0 && yy < a.H;
-    for (int kc = 0; kc < a.kchunks; kc++) {
+    // Pairs of 16-channel chunks on the 16x16x32 MFMA: lane quad q holds the
+    // 8 channels 8 (q & 1) .. of chunk kc + (q >> 1), for the pixel (A) and
+    // the output channel (B) alike, so the hardware pairs matching channels.
+    for (int kp = 0; kp < kpairs; kp++) {
+      const int kcl = 2 * kp + (quad >> 1), off = (quad & 1) * 8;
+      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kcl) * (a.ntot * 16) + j0 * 16 + row) * 16 + off;
+      uint4 bv[NT];
+#pragma unroll
+      for (int j = 0; j < NT; j++) bv[j] = j0 + j < a.ntot ? *reinterpret_cast<const uint4 *>(wb + j * 256)
+                                                            : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+      for (int m = 0; m < kMT; m++) {
+        const int xx = x0 + 16 * m + row + dx;
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (rowIn && xx >= 0 && xx < a.W) av = *reinterpret_cast<const uint4 *>(in_ptr(kcl, yy, xx) + off);
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+          if (kBf16) {
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, av),
+                                                               __builtin_bit_cast(bf16x8v, bv[j]), acc[m][j], 0, 0, 0);
+          } else {
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8v, av),
+                                                              __builtin_bit_cast(half8v, bv[j]), acc[m][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // An odd last chunk on the 16x16x16 MFMA (quad q: channels 4q ..).
+    for (int kc = 2 * kpairs; kc < a.kchunks; kc++) {
       const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (a.ntot * 16) + j0 * 16 + row) * 16 + quad * 4;
       uint2 bv[NT];
 #pragma unroll
@@ -90,17 +131,7 @@ __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
       for (int m = 0; m < kMT; m++) {
         const int xx = x0 + 16 * m + row + dx;
         uint2 av = make_uint2(0u, 0u);
-        if (rowIn && xx >= 0 && xx < a.W) {
-          const uint16_t *p;
-          if (kc < a.kc1) {
-            p = a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1
-                      : a.src1 + ((size_t)yy * a.W + xx) * a.cs1;
-            p += kc * 16 + quad * 4;
-          } else {
-            p = a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16 + quad * 4;
-          }
-          av = *reinterpret_cast<const uint2 *>(p);
-        }
+        if (rowIn && xx >= 0 && xx < a.W) av = *reinterpret_cast<const uint2 *>(in_ptr(kc, yy, xx) + quad * 4);
 #pragma unroll
         for (int j = 0; j < NT; j++) {
           if (kBf16) {
