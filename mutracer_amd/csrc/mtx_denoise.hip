@@ -401,11 +401,12 @@ static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTens
 // ntw of the layer's 16-channel output tiles (a weight fragment serves mt
 // MFMAs, an input fragment ntw; both cost registers, occupancy and waves).
 // Defaults per layer, network order: the fastest shapes of a per-layer sweep
-// at 1920 x 1088, FP16 (2.95 ms per pass against 3.76 ms with one 16-pixel
-// tile and all outputs per wave); mtx_denoiser_set_tiling overrides them
-// (the denoiser autotuner re-measures them for another resolution).
-constexpr int kConvTiling[16][2] = {{2, 4}, {4, 4}, {2, 7}, {2, 4}, {1, 3}, {2, 3}, {2, 2}, {1, 7},
-                                    {1, 4}, {2, 7}, {2, 7}, {2, 7}, {2, 4}, {4, 4}, {2, 4}, {4, 7}};
+// of the 16x16x32 kernel at 1920 x 1088, FP16 (1.95 ms per pass against 2.05
+// ms with the shapes tuned for the 16x16x16 kernel); mtx_denoiser_set_tiling
+// overrides them (the denoiser autotuner re-measures them for another
+// resolution).
+constexpr int kConvTiling[16][2] = {{2, 3}, {2, 2}, {1, 4}, {2, 7}, {1, 4}, {1, 1}, {1, 1}, {1, 4},
+                                    {1, 4}, {2, 7}, {2, 7}, {2, 4}, {2, 4}, {2, 7}, {4, 2}, {2, 1}};
 static void conv_tiling(const DenoiserObj::Layer &L, int layer, int &mt, int &ntw) {
   mt = L.tileMt ? L.tileMt : kConvTiling[layer][0];
   ntw = std::min(L.tileNt ? L.tileNt : kConvTiling[layer][1], L.nt);
