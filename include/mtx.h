@@ -468,9 +468,12 @@ int32_t mtx_denoiser_benchmark(mtx_handle denoiser, uint32_t width, uint32_t hei
                                double *ms_per_forward);
 /* Launch shape of layer `layer`'s convolution (0..15 in network order): each
  * wave computes pixel_tiles x 16 pixels of a row (1, 2, 4 or 8) for
- * channel_tiles of the layer's 16-channel output tiles (1..7); 0, 0 restores
- * the automatic choice. No reference counterpart: the denoiser autotuner's
- * knob. */
+ * channel_tiles of the layer's 16-channel output tiles (1..7); pixel_tiles
+ * or'ed with MTX_DENOISER_STAGED selects the kernel whose blocks of four
+ * waves (four rows) stage their input rows and weights in LDS; 0, 0 restores
+ * the automatic choice. Every shape gives the same bits. No reference
+ * counterpart: the denoiser autotuner's knob. */
+#define MTX_DENOISER_STAGED 16u
 int32_t mtx_denoiser_set_tiling(mtx_handle denoiser, uint32_t layer, uint32_t pixel_tiles, uint32_t channel_tiles);
 /* The launch shape `layer` uses at width x height. */
 int32_t mtx_denoiser_get_tiling(mtx_handle denoiser, uint32_t layer, uint32_t width, uint32_t height,

@@ -29,7 +29,7 @@ SPACE = {
     "max_paths_in_flight": [1 << 23, 1 << 24, 1 << 25],
     "tail_paths": [0, 16384, 65536, 262144],
     "queue_sort": [0, 1],
-    "lds_stack_closest": [16, 24, 32],
+    "lds_stack_closest": [0, 16, 24, 32],  # 0: the any-hit depth (lds_stack)
 }
 ORDER = ["max_paths_in_flight", "tail_paths", "queue_sort", "trace_kernel", "trace_oversubscription",
          "shade_waves_per_simd", "shade_oversubscription", "lds_stack", "lds_stack_closest"]
@@ -100,14 +100,15 @@ def tune(target, samples=2, cache_path=None, log=None):
 
 
 # ---------------------------------------------------------------- denoiser
-DENOISER_SHAPES = [(mt, nt) for mt in (1, 2, 4, 8) for nt in (1, 2, 3, 4, 7)]
+DENOISER_STAGED = 16  # MTX_DENOISER_STAGED: the LDS-staged convolution kernel
+DENOISER_SHAPES = [(flag | mt, nt) for flag in (0, DENOISER_STAGED) for mt in (1, 2, 4, 8) for nt in (1, 2, 3, 4, 7)]
 
 
 def tune_denoiser(denoiser, width, height, iterations=3, log=None):
     """Picks each U-Net layer's convolution launch shape (pixel tiles x
-    output-channel tiles per wave) by measuring whole forward passes with
-    one layer changed at a time (HIP events, mtx_denoiser_benchmark), and
-    applies the fastest. Returns {layer: (pixel_tiles, channel_tiles)} and the
+    output-channel tiles per wave, direct or LDS-staged kernel) by
+    measuring whole forward passes with one layer changed at a time (HIP
+    events, mtx_denoiser_benchmark), and applies the fastest. Returns {layer: (pixel_tiles, channel_tiles)} and the
     pass time before and after."""
     before = min(denoiser.benchmark(width, height, iterations) for _ in range(2))
     best = {}

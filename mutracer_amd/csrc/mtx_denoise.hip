@@ -26,8 +26,6 @@
 
 namespace mtx {
 
-typedef _Float16 half4v __attribute__((ext_vector_type(4)));
-typedef short short4v __attribute__((ext_vector_type(4)));
 typedef _Float16 half8v __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
@@ -59,93 +57,32 @@ struct ConvArgs {
   int ngroups;  // waves sharing one pixel segment, each on NT of the ntot tiles
 };
 
-// kMT: 16-pixel tiles per wave along the row. Each K step loads the NT weight
-// fragments once and reuses them for all kMT tiles (kMT x NT MFMAs per
-// 1 + ... loads), which is what keeps the matrix cores fed: with kMT = 1 the
-// wave issues one weight load per MFMA. K steps take two 16-channel chunks
-// at a time on gfx950's 16x16x32 MFMA (16-byte loads, half the MFMA count);
-// an odd last chunk uses the 16x16x16 form.
+// One K step of the MFMA for either precision: 32 channels, 8 per lane.
+template <bool kBf16>
+__device__ __forceinline__ float4v mfma_k32(uint4 a, uint4 b, float4v c) {
+  if (kBf16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b), c,
+                                                   0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8v, a), __builtin_bit_cast(half8v, b), c, 0,
+                                                0, 0);
+}
+
+// Input channels of chunk kc at pixel (yy, xx): src1 (upsampled when up1)
+// for the first kc1 chunks, then src2.
+__device__ __forceinline__ const uint16_t *conv_in(const ConvArgs &a, int kc, int yy, int xx) {
+  if (kc < a.kc1)
+    return (a.up1 ? a.src1 + ((size_t)(yy >> 1) * (a.W >> 1) + (xx >> 1)) * a.cs1
+                  : a.src1 + ((size_t)yy * a.W + xx) * a.cs1) +
+           kc * 16;
+  return a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16;
+}
+
+// Bias, ReLU and store of a wave's kMT x NT tiles of row y from pixel x0.
+// D layout: lane holds output channel (lane & 15) of pixels 4 * (lane >> 4) + e.
 template <bool kBf16, int NT, int kMT>
-__global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int segs = (a.W + 16 * kMT - 1) / (16 * kMT);  // the last segment of a row may be partial
-  if (gw >= segs * a.H * a.ngroups) return;             // wave-uniform
-  const int g = gw % a.ngroups, wave = gw / a.ngroups;  // the groups of a segment are neighbours
-  const int y = wave / segs, x0 = (wave - y * segs) * (16 * kMT);
-  const int j0 = g * NT;                                // first output tile of this wave
+__device__ __forceinline__ void conv_store(const ConvArgs &a, const float4v (&acc)[kMT][NT], int y, int x0, int j0,
+                                           int lane) {
   const int row = lane & 15, quad = lane >> 4;
-  float4v acc[kMT][NT];
-#pragma unroll
-  for (int m = 0; m < kMT; m++)
-#pragma unroll
-    for (int j = 0; j < NT; j++) acc[m][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
-  const int W2 = a.W >> 1;
-  // Input channels of chunk kc at pixel (yy, xx): src1 (upsampled when up1)
-  // for the first kc1 chunks, then src2.
-  auto in_ptr = [&](int kc, int yy, int xx) -> const uint16_t * {
-    if (kc < a.kc1)
-      return (a.up1 ? a.src1 + ((size_t)(yy >> 1) * W2 + (xx >> 1)) * a.cs1 : a.src1 + ((size_t)yy * a.W + xx) * a.cs1) +
-             kc * 16;
-    return a.src2 + ((size_t)yy * a.W + xx) * a.cs2 + (kc - a.kc1) * 16;
-  };
-  const int kpairs = a.kchunks >> 1;
-  for (int t = 0; t < 9; t++) {
-    const int yy = y + t / 3 - 1, dx = t % 3 - 1;
-    const bool rowIn = yy >= 0 && yy < a.H;
-    // Pairs of 16-channel chunks on the 16x16x32 MFMA: lane quad q holds the
-    // 8 channels 8 (q & 1) .. of chunk kc + (q >> 1), for the pixel (A) and
-    // the output channel (B) alike, so the hardware pairs matching channels.
-    for (int kp = 0; kp < kpairs; kp++) {
-      const int kcl = 2 * kp + (quad >> 1), off = (quad & 1) * 8;
-      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kcl) * (a.ntot * 16) + j0 * 16 + row) * 16 + off;
-      uint4 bv[NT];
-#pragma unroll
-      for (int j = 0; j < NT; j++) bv[j] = j0 + j < a.ntot ? *reinterpret_cast<const uint4 *>(wb + j * 256)
-                                                            : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-      for (int m = 0; m < kMT; m++) {
-        const int xx = x0 + 16 * m + row + dx;
-        uint4 av = make_uint4(0u, 0u, 0u, 0u);
-        if (rowIn && xx >= 0 && xx < a.W) av = *reinterpret_cast<const uint4 *>(in_ptr(kcl, yy, xx) + off);
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-          if (kBf16) {
-            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, av),
-                                                               __builtin_bit_cast(bf16x8v, bv[j]), acc[m][j], 0, 0, 0);
-          } else {
-            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8v, av),
-                                                              __builtin_bit_cast(half8v, bv[j]), acc[m][j], 0, 0, 0);
-          }
-        }
-      }
-    }
-    // An odd last chunk on the 16x16x16 MFMA (quad q: channels 4q ..).
-    for (int kc = 2 * kpairs; kc < a.kchunks; kc++) {
-      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (a.ntot * 16) + j0 * 16 + row) * 16 + quad * 4;
-      uint2 bv[NT];
-#pragma unroll
-      for (int j = 0; j < NT; j++) bv[j] = j0 + j < a.ntot ? *reinterpret_cast<const uint2 *>(wb + j * 256)
-                                                            : make_uint2(0u, 0u);
-#pragma unroll
-      for (int m = 0; m < kMT; m++) {
-        const int xx = x0 + 16 * m + row + dx;
-        uint2 av = make_uint2(0u, 0u);
-        if (rowIn && xx >= 0 && xx < a.W) av = *reinterpret_cast<const uint2 *>(in_ptr(kc, yy, xx) + quad * 4);
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-          if (kBf16) {
-            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, av),
-                                                                 __builtin_bit_cast(short4v, bv[j]), acc[m][j], 0, 0, 0);
-          } else {
-            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4v, av),
-                                                              __builtin_bit_cast(half4v, bv[j]), acc[m][j], 0, 0, 0);
-          }
-        }
-      }
-    }
-  }
-  // D layout: lane holds output channel (lane & 15) of pixels 4 * (lane >> 4) + e.
 #pragma unroll
   for (int m = 0; m < kMT; m++) {
 #pragma unroll
@@ -166,6 +103,172 @@ __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
       }
     }
   }
+}
+
+// Both convolution kernels accumulate every output in the same order, K
+// step outer (pairs of 16-channel chunks; the second chunk of an odd count's
+// last step is zero), tap inner, with one 16x16x32 MFMA per (step, tap,
+// tile), so every launch shape of either kernel gives the same bits.
+//
+// Direct form. kMT: 16-pixel tiles per wave along the row. Each K step loads
+// the NT weight fragments once and reuses them for all kMT tiles (kMT x NT
+// MFMAs per kMT + NT loads), which is what keeps the matrix cores fed: with
+// kMT = 1 the wave issues one weight load per MFMA. K steps take two
+// 16-channel chunks at a time on gfx950's 16x16x32 MFMA (16-byte loads).
+template <bool kBf16, int NT, int kMT>
+__global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int segs = (a.W + 16 * kMT - 1) / (16 * kMT);  // the last segment of a row may be partial
+  if (gw >= segs * a.H * a.ngroups) return;             // wave-uniform
+  const int g = gw % a.ngroups, wave = gw / a.ngroups;  // the groups of a segment are neighbours
+  const int y = wave / segs, x0 = (wave - y * segs) * (16 * kMT);
+  const int j0 = g * NT;                                // first output tile of this wave
+  const int row = lane & 15, quad = lane >> 4;
+  float4v acc[kMT][NT];
+#pragma unroll
+  for (int m = 0; m < kMT; m++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[m][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+  // K steps of two 16-channel chunks on the 16x16x32 MFMA: lane quad q holds
+  // the 8 channels 8 (q & 1) .. of chunk 2 ks + (q >> 1), for the pixel (A)
+  // and the output channel (B) alike, so the hardware pairs matching
+  // channels; past the last chunk (an odd chunk count) both are zero.
+  const int ksteps = (a.kchunks + 1) >> 1;
+  for (int ks = 0; ks < ksteps; ks++) {
+    const int kc = 2 * ks + (quad >> 1), off = (quad & 1) * 8;
+    const bool inK = kc < a.kchunks;
+#pragma unroll 1
+    for (int t = 0; t < 9; t++) {
+      const int yy = y + t / 3 - 1, dx = t % 3 - 1;
+      const bool rowIn = inK && yy >= 0 && yy < a.H;
+      const uint16_t *wb = a.wt + ((size_t)(t * a.kchunks + kc) * (a.ntot * 16) + j0 * 16 + row) * 16 + off;
+      uint4 bv[NT];
+#pragma unroll
+      for (int j = 0; j < NT; j++) bv[j] = inK && j0 + j < a.ntot ? *reinterpret_cast<const uint4 *>(wb + j * 256)
+                                                                   : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+      for (int m = 0; m < kMT; m++) {
+        const int xx = x0 + 16 * m + row + dx;
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (rowIn && xx >= 0 && xx < a.W) av = *reinterpret_cast<const uint4 *>(conv_in(a, kc, yy, xx) + off);
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[m][j] = mfma_k32<kBf16>(av, bv[j], acc[m][j]);
+      }
+    }
+  }
+  conv_store<kBf16, NT, kMT>(a, acc, y, x0, j0, lane);
+}
+
+// Staged form: a block of four waves computes four consecutive rows (wave w:
+// row y0 + w) of one 16 kMT-pixel segment for NT output tiles. Per K step the
+// block copies the six input rows its taps read (rows y0 - 1 .. y0 + 4,
+// pixels x0 - 1 .. x0 + 16 kMT) and the nine taps' weight fragments of its NT
+// tiles into LDS once, and the waves read every operand from LDS: an input
+// row serves the taps of three waves, a weight fragment all 4 kMT pixel tiles
+// of the block (global and L2 traffic per MFMA falls by about that much).
+// The copy is software-pipelined: each thread loads its pieces of step
+// ks + 1 into registers while the block computes step ks.
+// LDS layout: input [6 rows][4 pieces of 8 channels][16 kMT + 2 pixels][16 B],
+// so the 16 pixel lanes of an operand read 256 contiguous bytes; weights
+// [9 taps][NT][64 lanes][16 B], each lane's register image.
+template <bool kBf16, int NT, int kMT>
+__global__ void __launch_bounds__(256) k_conv3x3_staged(const ConvArgs a) {
+  constexpr int PW = 16 * kMT + 2;
+  constexpr int kAPieces = 6 * 4 * PW;
+  constexpr int kBPieces = 9 * NT * 64;
+  constexpr int kAPer = (kAPieces + 255) / 256, kBPer = (kBPieces + 255) / 256;  // pieces per thread
+  __shared__ uint4 sA[kAPieces];
+  __shared__ uint4 sB[kBPieces];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int segs = (a.W + 16 * kMT - 1) / (16 * kMT);
+  const int g = blockIdx.x % a.ngroups, blk = blockIdx.x / a.ngroups;  // the groups of a segment are neighbours
+  const int y0 = (blk / segs) * 4, x0 = (blk % segs) * (16 * kMT);
+  const int j0 = g * NT;
+  const int row = lane & 15, quad = lane >> 4;
+
+  // This thread's pieces: input piece i = tid + 256 k is piece q = tid & 3
+  // (channels 8 (q & 1) .. of chunk 2 ks + (q >> 1)) of halo pixel i >> 2;
+  // weight piece i is lane tid & 63's fragment of tap / tile i >> 6. Pixel
+  // indices (full and, for an upsampled src1, half resolution) and weight
+  // offsets are fixed over the K steps; -1 marks a zero piece.
+  const int qa = tid & 3, qb = (tid & 63) >> 4;
+  int pix[kAPer], pixHalf[kAPer], wOff[kBPer];
+#pragma unroll
+  for (int k = 0; k < kAPer; k++) {
+    const int i = tid + 256 * k, r = i >> 2, hy = r / PW, px = r - hy * PW;
+    const int yy = y0 - 1 + hy, xx = x0 - 1 + px;
+    const bool in = i < kAPieces && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    pix[k] = in ? yy * a.W + xx : -1;
+    pixHalf[k] = (yy >> 1) * (a.W >> 1) + (xx >> 1);
+  }
+#pragma unroll
+  for (int k = 0; k < kBPer; k++) {
+    const int tj = (tid >> 6) + 4 * k, t = tj / NT, j = tj - t * NT;
+    wOff[k] = tj < 9 * NT && j0 + j < a.ntot
+                  ? ((t * a.kchunks * a.ntot + j0 + j) * 16 + (tid & 15)) * 16 + (qb & 1) * 8
+                  : -1;
+  }
+  uint4 ra[kAPer], rb[kBPer];
+  auto fetch = [&](int ks) {
+    const int kc = 2 * ks + (qa >> 1);
+    const bool inK = kc < a.kchunks, first = kc < a.kc1;
+#pragma unroll
+    for (int k = 0; k < kAPer; k++) {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (inK && pix[k] >= 0) {
+        const uint16_t *p = first ? a.src1 + (size_t)(a.up1 ? pixHalf[k] : pix[k]) * a.cs1 + kc * 16
+                                  : a.src2 + (size_t)pix[k] * a.cs2 + (kc - a.kc1) * 16;
+        v = *reinterpret_cast<const uint4 *>(p + (qa & 1) * 8);
+      }
+      ra[k] = v;
+    }
+    const int kcb = 2 * ks + (qb >> 1);
+    const bool inKb = kcb < a.kchunks;
+#pragma unroll
+    for (int k = 0; k < kBPer; k++) {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (inKb && wOff[k] >= 0) v = *reinterpret_cast<const uint4 *>(a.wt + wOff[k] + (size_t)kcb * (a.ntot * 256));
+      rb[k] = v;
+    }
+  };
+
+  float4v acc[kMT][NT];
+#pragma unroll
+  for (int m = 0; m < kMT; m++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[m][j] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+  const int ksteps = (a.kchunks + 1) >> 1;
+  fetch(0);
+  for (int ks = 0; ks < ksteps; ks++) {
+    if (ks) __syncthreads();  // every wave is done with the previous step's operands
+#pragma unroll
+    for (int k = 0; k < kAPer; k++) {
+      const int i = tid + 256 * k, r = i >> 2, hy = r / PW, px = r - hy * PW;
+      if (k < kAPer - 1 || i < kAPieces) sA[(hy * 4 + qa) * PW + px] = ra[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kBPer; k++) {
+      const int i = tid + 256 * k;
+      if (k < kBPer - 1 || i < kBPieces) sB[i] = rb[k];
+    }
+    __syncthreads();
+    if (ks + 1 < ksteps) fetch(ks + 1);
+#pragma unroll 1
+    for (int t = 0; t < 9; t++) {
+      const uint4 *ap = sA + ((w + t / 3) * 4 + quad) * PW + row + t % 3;
+      uint4 bv[NT];
+#pragma unroll
+      for (int j = 0; j < NT; j++) bv[j] = sB[(t * NT + j) * 64 + lane];
+#pragma unroll
+      for (int m = 0; m < kMT; m++) {
+        const uint4 av = ap[16 * m];
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[m][j] = mfma_k32<kBf16>(av, bv[j], acc[m][j]);
+      }
+    }
+  }
+  if (y0 + w < a.H) conv_store<kBf16, NT, kMT>(a, acc, y0 + w, x0, j0, lane);
 }
 
 // 2x2 max pool over all (padded) channels; 8 channels (16 bytes) per thread.
@@ -399,14 +502,17 @@ static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTens
 
 // Tiling of one convolution: each wave computes mt x 16 pixels of a row for
 // ntw of the layer's 16-channel output tiles (a weight fragment serves mt
-// MFMAs, an input fragment ntw; both cost registers, occupancy and waves).
-// Defaults per layer, network order: the fastest shapes of a per-layer sweep
-// of the 16x16x32 kernel at 1920 x 1088, FP16 (1.95 ms per pass against 2.05
-// ms with the shapes tuned for the 16x16x16 kernel); mtx_denoiser_set_tiling
-// overrides them (the denoiser autotuner re-measures them for another
-// resolution).
-constexpr int kConvTiling[16][2] = {{2, 3}, {2, 2}, {1, 4}, {2, 7}, {1, 4}, {1, 1}, {1, 1}, {1, 4},
-                                    {1, 4}, {2, 7}, {2, 7}, {2, 4}, {2, 4}, {2, 7}, {4, 2}, {2, 1}};
+// MFMAs, an input fragment ntw; both cost registers, occupancy and waves),
+// in the direct kernel or, with kTileStaged, in the LDS-staged one (blocks of
+// four rows). Defaults per layer, network order: the fastest shapes of a
+// per-layer sweep of both kernels at 1920 x 1088, FP16 (1.33 ms per pass;
+// 2.02 ms with the direct kernel alone); mtx_denoiser_set_tiling overrides
+// them (the denoiser autotuner re-measures them for another resolution).
+constexpr int kTileStaged = MTX_DENOISER_STAGED;
+constexpr int kSt = kTileStaged;
+constexpr int kConvTiling[16][2] = {{kSt | 2, 2}, {kSt | 2, 4}, {kSt | 2, 4}, {kSt | 2, 2}, {kSt | 1, 3}, {kSt | 1, 1},
+                                    {kSt | 1, 1}, {kSt | 1, 2}, {kSt | 1, 1}, {kSt | 2, 2}, {kSt | 2, 2}, {kSt | 2, 2},
+                                    {kSt | 2, 2}, {kSt | 2, 2}, {kSt | 2, 4}, {kSt | 2, 7}};
 static void conv_tiling(const DenoiserObj::Layer &L, int layer, int &mt, int &ntw) {
   mt = L.tileMt ? L.tileMt : kConvTiling[layer][0];
   ntw = std::min(L.tileNt ? L.tileNt : kConvTiling[layer][1], L.nt);
@@ -426,30 +532,39 @@ static void ensure_size(DenoiserObj &D, int H, int W) {
 }
 
 template <bool kBf16, int NT, int kMT>
-static void launch_conv_t(const ConvArgs &a, hipStream_t s) {
-  const long waves = (long)((a.W + 16 * kMT - 1) / (16 * kMT)) * a.H * a.ngroups;
+static void launch_conv_t(const ConvArgs &a, bool staged, hipStream_t s) {
+  const long segs = (a.W + 16 * kMT - 1) / (16 * kMT);
+  if (staged) {
+    const long blocks = segs * ((a.H + 3) / 4) * a.ngroups;
+    hipLaunchKernelGGL((k_conv3x3_staged<kBf16, NT, kMT>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return;
+  }
+  const long waves = segs * a.H * a.ngroups;
   hipLaunchKernelGGL((k_conv3x3<kBf16, NT, kMT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a);
 }
 template <bool kBf16, int NT>
-static void launch_conv_nt(const ConvArgs &a, int mt, hipStream_t s) {
+static void launch_conv_nt(const ConvArgs &a, int mt, bool staged, hipStream_t s) {
   switch (mt) {
-    case 1: launch_conv_t<kBf16, NT, 1>(a, s); break;
-    case 2: launch_conv_t<kBf16, NT, 2>(a, s); break;
-    case 4: launch_conv_t<kBf16, NT, 4>(a, s); break;
-    default: launch_conv_t<kBf16, NT, 8>(a, s); break;
+    case 1: launch_conv_t<kBf16, NT, 1>(a, staged, s); break;
+    case 2: launch_conv_t<kBf16, NT, 2>(a, staged, s); break;
+    case 4: launch_conv_t<kBf16, NT, 4>(a, staged, s); break;
+    default: launch_conv_t<kBf16, NT, 8>(a, staged, s); break;
   }
 }
-// nt: output tiles per wave (1..7), mt: pixel tiles per wave (1, 2, 4, 8).
+// nt: output tiles per wave (1..7), mt: pixel tiles per wave (1, 2, 4, 8),
+// or'ed with kTileStaged for the LDS-staged kernel.
 template <bool kBf16>
 static void launch_conv(const ConvArgs &a, int nt, int mt, hipStream_t s) {
+  const bool staged = (mt & kTileStaged) != 0;
+  mt &= ~kTileStaged;
   switch (nt) {
-    case 1: launch_conv_nt<kBf16, 1>(a, mt, s); break;
-    case 2: launch_conv_nt<kBf16, 2>(a, mt, s); break;
-    case 3: launch_conv_nt<kBf16, 3>(a, mt, s); break;
-    case 4: launch_conv_nt<kBf16, 4>(a, mt, s); break;
-    case 5: launch_conv_nt<kBf16, 5>(a, mt, s); break;
-    case 6: launch_conv_nt<kBf16, 6>(a, mt, s); break;
-    default: launch_conv_nt<kBf16, 7>(a, mt, s); break;
+    case 1: launch_conv_nt<kBf16, 1>(a, mt, staged, s); break;
+    case 2: launch_conv_nt<kBf16, 2>(a, mt, staged, s); break;
+    case 3: launch_conv_nt<kBf16, 3>(a, mt, staged, s); break;
+    case 4: launch_conv_nt<kBf16, 4>(a, mt, staged, s); break;
+    case 5: launch_conv_nt<kBf16, 5>(a, mt, staged, s); break;
+    case 6: launch_conv_nt<kBf16, 6>(a, mt, staged, s); break;
+    default: launch_conv_nt<kBf16, 7>(a, mt, staged, s); break;
   }
 }
 
@@ -586,9 +701,10 @@ int32_t mtx_denoiser_set_tiling(mtx_handle handle, uint32_t layer, uint32_t pixe
     if (!lk.owns_lock()) return MTX_BUSY;
     MTX_REQUIRE(layer < 16, MTX_INVALID_ARGUMENT, "layer must be in 0..15");
     const bool autoShape = pixel_tiles == 0 && channel_tiles == 0;
-    MTX_REQUIRE(autoShape || ((pixel_tiles == 1 || pixel_tiles == 2 || pixel_tiles == 4 || pixel_tiles == 8) &&
-                              channel_tiles >= 1 && channel_tiles <= 7),
-                MTX_INVALID_ARGUMENT, "pixel_tiles must be 1, 2, 4 or 8 and channel_tiles 1..7 (or both 0)");
+    const uint32_t mt = pixel_tiles & ~(uint32_t)MTX_DENOISER_STAGED;
+    MTX_REQUIRE(autoShape || ((mt == 1 || mt == 2 || mt == 4 || mt == 8) && channel_tiles >= 1 && channel_tiles <= 7),
+                MTX_INVALID_ARGUMENT,
+                "pixel_tiles must be 1, 2, 4 or 8 (optionally | MTX_DENOISER_STAGED) and channel_tiles 1..7 (or both 0)");
     D->L[layer].tileMt = (int)pixel_tiles;
     D->L[layer].tileNt = (int)channel_tiles;
     return MTX_OK;

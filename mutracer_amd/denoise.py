@@ -62,8 +62,11 @@ class Denoiser:
             return out, {name: b for (name, _, _), b in zip(LAYERS, bufs)}
         return out
 
+    STAGED = 16  # MTX_DENOISER_STAGED
+
     def set_tiling(self, layer, pixel_tiles=0, channel_tiles=0):
-        """Launch shape of one layer's convolution (0, 0 = automatic)."""
+        """Launch shape of one layer's convolution (0, 0 = automatic);
+        pixel_tiles | STAGED selects the LDS-staged kernel."""
         abi.check(abi.lib().mtx_denoiser_set_tiling(self.handle.value, layer, pixel_tiles, channel_tiles))
 
     def tiling(self, layer, width, height):
@@ -82,3 +85,24 @@ class Denoiser:
         h = getattr(self, "handle", None)
         if h is not None and h.value and abi._lib is not None:
             abi.lib().mtx_denoiser_destroy(h.value)
+
+
+def _main(argv=None):
+    """Times network passes at a given size (HIP events), after 100 untimed
+    passes: python3 -m mutracer_amd.denoise [--width W] [--height H]
+    [--precision fp16|bf16] [--iterations N]."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="python3 -m mutracer_amd.denoise")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1088)
+    ap.add_argument("--precision", default="fp16", choices=("fp16", "bf16"))
+    ap.add_argument("--iterations", type=int, default=20)
+    args = ap.parse_args(argv)
+    d = Denoiser(args.precision)
+    d.benchmark(args.width, args.height, 100)
+    ms = d.benchmark(args.width, args.height, args.iterations)
+    print(f"denoiser {args.precision} {args.width}x{args.height}: {ms:.3f} ms per pass", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -81,23 +81,39 @@ def test_run_semantics(net):
     assert abs(float(o2[8:-8, 8:-8, :3].mean()) - 0.5) < 0.1
 
 
-def test_tiling_shapes_give_identical_outputs(net):
-    """Every launch shape accumulates each output in the same (tap, channel
-    slice) order, so all shapes give the same bits; the tuner keeps them."""
+STAGED = 16  # MTX_DENOISER_STAGED
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("h,w", [(64, 96), (48, 80)])
+def test_tiling_shapes_give_identical_outputs(net, precision, h, w):
+    """Every launch shape of the direct and the LDS-staged kernel accumulates
+    each output in the same (channel step, tap) order, so all shapes give the
+    same bits, also where a block's four rows or a wave's pixel tiles run
+    past the image (48 x 80: 3 x 5 pixels at the deepest level)."""
+    denoise, _ = net
+    d = denoise.Denoiser(precision)
+    x = _input(h, w, seed=5)
+    ref, ref_layers = d.forward(x, layers=True)
+    for mt, nt in ((1, 1), (8, 7), (2, 3), (4, 2), (STAGED | 1, 1), (STAGED | 8, 7), (STAGED | 2, 3),
+                   (STAGED | 4, 4), (STAGED | 4, 2)):
+        for layer in range(16):
+            d.set_tiling(layer, mt, nt)
+            assert d.tiling(layer, w, h)[0] == mt
+        out, layers = d.forward(x, layers=True)
+        assert np.array_equal(out, ref), (mt, nt)
+        assert all(np.array_equal(layers[k], ref_layers[k]) for k in ref_layers), (mt, nt)
+    for bad in ((3, 1), (STAGED | 3, 1), (STAGED, 1), (2, 8)):
+        with pytest.raises(Exception):
+            d.set_tiling(0, *bad)
+
+
+def test_denoiser_autotune(net):
     denoise, _ = net
     from mutracer_amd import autotune
     d = denoise.Denoiser("fp16")
     x = _input(64, 96, seed=5)
-    ref, ref_layers = d.forward(x, layers=True)
-    for mt, nt in ((1, 1), (8, 7), (2, 3), (4, 2)):
-        for layer in range(16):
-            d.set_tiling(layer, mt, nt)
-            assert d.tiling(layer, 96, 64)[0] == mt
-        out, layers = d.forward(x, layers=True)
-        assert np.array_equal(out, ref), (mt, nt)
-        assert all(np.array_equal(layers[k], ref_layers[k]) for k in ref_layers)
-    with pytest.raises(Exception):
-        d.set_tiling(0, 3, 1)
+    ref = d.forward(x)
     best, before, after = autotune.tune_denoiser(d, 96, 64, iterations=1)
     assert len(best) == 16 and before > 0 and after > 0
     assert np.array_equal(d.forward(x), ref)
