@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-command-line-argument",
+    "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-command-line-argument", "-Wno-pass-failed",
     f"-I{os.path.join(HERE, '..', 'include')}",
 ]
 

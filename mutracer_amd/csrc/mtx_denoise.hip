@@ -172,8 +172,12 @@ __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
 // LDS layout: input [6 rows][4 pieces of 8 channels][16 kMT + 2 pixels][16 B],
 // so the 16 pixel lanes of an operand read 256 contiguous bytes; weights
 // [9 taps][NT][64 lanes][16 B], each lane's register image.
+// The occupancy hint (two waves per SIMD) keeps the accumulators in the
+// architectural registers: without it the compiler split them across the
+// accumulation registers and tiles of 4 x 4 fragments ran at one wave per
+// SIMD (272 registers; 220 with the hint).
 template <bool kBf16, int NT, int kMT>
-__global__ void __launch_bounds__(256) k_conv3x3_staged(const ConvArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_conv3x3_staged(const ConvArgs a) {
   constexpr int PW = 16 * kMT + 2;
   constexpr int kAPieces = 6 * 4 * PW;
   constexpr int kBPieces = 9 * NT * 64;
@@ -505,14 +509,14 @@ static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTens
 // MFMAs, an input fragment ntw; both cost registers, occupancy and waves),
 // in the direct kernel or, with kTileStaged, in the LDS-staged one (blocks of
 // four rows). Defaults per layer, network order: the fastest shapes of a
-// per-layer sweep of both kernels at 1920 x 1088, FP16 (1.33 ms per pass;
+// per-layer sweep of both kernels at 1920 x 1088, FP16 (1.24 ms per pass;
 // 2.02 ms with the direct kernel alone); mtx_denoiser_set_tiling overrides
 // them (the denoiser autotuner re-measures them for another resolution).
 constexpr int kTileStaged = MTX_DENOISER_STAGED;
 constexpr int kSt = kTileStaged;
-constexpr int kConvTiling[16][2] = {{kSt | 2, 2}, {kSt | 2, 4}, {kSt | 2, 4}, {kSt | 2, 2}, {kSt | 1, 3}, {kSt | 1, 1},
-                                    {kSt | 1, 1}, {kSt | 1, 2}, {kSt | 1, 1}, {kSt | 2, 2}, {kSt | 2, 2}, {kSt | 2, 2},
-                                    {kSt | 2, 2}, {kSt | 2, 2}, {kSt | 2, 4}, {kSt | 2, 7}};
+constexpr int kConvTiling[16][2] = {{kSt | 2, 2}, {kSt | 2, 7}, {kSt | 2, 4}, {kSt | 2, 2}, {kSt | 1, 3}, {kSt | 1, 2},
+                                    {kSt | 1, 2}, {kSt | 2, 4}, {kSt | 1, 2}, {kSt | 2, 3}, {kSt | 2, 2}, {kSt | 2, 7},
+                                    {kSt | 2, 2}, {kSt | 2, 7}, {kSt | 2, 7}, {kSt | 4, 4}};
 static void conv_tiling(const DenoiserObj::Layer &L, int layer, int &mt, int &ntw) {
   mt = L.tileMt ? L.tileMt : kConvTiling[layer][0];
   ntw = std::min(L.tileNt ? L.tileNt : kConvTiling[layer][1], L.nt);
