@@ -51,6 +51,7 @@ struct ConvArgs {
   const uint16_t *wt;    // [9][kchunks][nt * 16][16]
   const float *bias;
   uint16_t *dst;         // [H][W][ds]
+  uint16_t *pool;        // staged kernel: also dst's 2x2 max pool, [H / 2][W / 2][ds] (H, W even), or null
   int c1, cs1, up1, c2, cs2;
   int H, W, kchunks, kc1, cout, ds, relu;
   int ntot;     // output tiles of 16 channels (weights row stride / 16)
@@ -311,6 +312,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       if (idx < 32 * kMT * NT && (c8 >> 1) < tiles && x0 + px < a.W)
         *reinterpret_cast<uint4 *>(a.dst + ((size_t)y * a.W + x0 + px) * a.ds + j0 * 16 + 8 * c8) =
             *reinterpret_cast<const uint4 *>(so + px * kOutRow + 8 * c8);
+    }
+  }
+  // The 2x2 max pool of the block's four rows (waves 2r and 2r + 1 give
+  // pooled row y0 / 2 + r), from the same tiles, in k_pool2's order.
+  if (a.pool) {
+    const int Ho = a.H >> 1, Wo = a.W >> 1;
+    const int tiles = a.ntot - j0 < NT ? a.ntot - j0 : NT;
+    const uint16_t *tile = reinterpret_cast<const uint16_t *>(smem);
+    constexpr int kWave = 16 * kMT * kOutRow;
+    for (int idx = tid; idx < 2 * 8 * kMT * 2 * NT; idx += 256) {
+      const int c8 = idx % (2 * NT), r = idx / (2 * NT), pp = r % (8 * kMT), pr = r / (8 * kMT);
+      const int py = (y0 >> 1) + pr, px = (x0 >> 1) + pp;
+      if ((c8 >> 1) >= tiles || py >= Ho || px >= Wo) continue;
+      const uint16_t *s00 = tile + 2 * pr * kWave + 2 * pp * kOutRow + 8 * c8;
+      const uint16_t *s01 = s00 + kOutRow, *s10 = s00 + kWave, *s11 = s10 + kOutRow;
+      uint16_t out[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const float v = fmaxf(fmaxf(from_store<kBf16>(s00[k]), from_store<kBf16>(s01[k])),
+                              fmaxf(from_store<kBf16>(s10[k]), from_store<kBf16>(s11[k])));
+        out[k] = to_store<kBf16>(v);
+      }
+      *reinterpret_cast<uint4 *>(a.pool + ((size_t)py * Wo + px) * a.ds + j0 * 16 + 8 * c8) =
+          *reinterpret_cast<const uint4 *>(out);
     }
   }
 }
@@ -647,14 +672,20 @@ static void run_unet(DenoiserObj &D, hipStream_t s) {
     int mt, ntw;
     conv_tiling(D.L[i], i, mt, ntw);
     a.ngroups = (a.ntot + ntw - 1) / ntw;
+    // enc_conv1..4 are pooled, the pooled tensor feeds the next level; the
+    // staged kernel pools in its epilogue (levels 0..3 have even sizes).
+    const bool pooled = i >= 1 && i <= 4;
+    a.pool = pooled && (mt & kTileStaged) ? D.pool[i - 1].ptr : nullptr;
     launch_conv<kBf16>(a, ntw, mt, s);
     prev = D.act[i].ptr;
     prevC = a.ds;
-    if (i >= 1 && i <= 4) {  // enc_conv1..4 are pooled; the pooled tensor feeds the next level
-      const int ph = H >> lv, pw = W >> lv;
-      const size_t n = (size_t)(ph >> 1) * (pw >> 1) * (a.ds >> 3);
-      hipLaunchKernelGGL(k_pool2<kBf16>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0, s,
-                         D.act[i].ptr, D.pool[i - 1].ptr, ph, pw, a.ds);
+    if (pooled) {
+      if (!a.pool) {
+        const int ph = H >> lv, pw = W >> lv;
+        const size_t n = (size_t)(ph >> 1) * (pw >> 1) * (a.ds >> 3);
+        hipLaunchKernelGGL(k_pool2<kBf16>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0,
+                           s, D.act[i].ptr, D.pool[i - 1].ptr, ph, pw, a.ds);
+      }
       prev = D.pool[i - 1].ptr;
     }
   }
