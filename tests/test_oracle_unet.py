@@ -21,8 +21,21 @@ def test_weights_digest_and_table():
     assert w["dec_conv0.bias"][0].shape == (3,)
 
 
-def test_cost_matches_survey_figure():
-    assert abs(unet.flops(1088, 1920) / 1e9 - 523.94) < 0.01
+def test_cost_per_1080p_pass():
+    """2 x multiply-adds of the 16 convolutions at 1920 x 1088, summed here
+    from the weight tensors' shapes and each layer's resolution level (the
+    encoder halves it after enc_conv1..4, the decoder doubles it back):
+    523.94 GFLOP, and the oracle's own count agrees."""
+    w = unet.read_tza(open(WEIGHTS, "rb").read())
+    level = {"enc_conv0": 0, "enc_conv1": 0, "enc_conv2": 1, "enc_conv3": 2, "enc_conv4": 3, "enc_conv5a": 4,
+             "enc_conv5b": 4, "dec_conv4a": 3, "dec_conv4b": 3, "dec_conv3a": 2, "dec_conv3b": 2, "dec_conv2a": 1,
+             "dec_conv2b": 1, "dec_conv1a": 0, "dec_conv1b": 0, "dec_conv0": 0}
+    total = 0
+    for name, lv in level.items():
+        cout, cin, kh, kw = w[name + ".weight"][0].shape
+        total += 2 * (1088 >> lv) * (1920 >> lv) * kh * kw * cin * cout
+    assert abs(total / 1e9 - 523.94) < 0.01
+    assert abs(unet.flops(1088, 1920) - total) < 1.0
 
 
 def test_conv_equals_direct_sum():
