@@ -1700,14 +1700,21 @@ __device__ __forceinline__ SampleRec finish_path(const RenderParams &P, const Pa
 }
 // Adds one sample to the 13 running sums (color rgba, albedo rgb + hit,
 // normal rgb + 1, luminance squared).
+// The 13 values one sample adds.
+__device__ __forceinline__ void sample_values(const SampleRec &r, double v[13]) {
+  v[0] = (double)r.li.x; v[1] = (double)r.li.y; v[2] = (double)r.li.z; v[3] = (double)r.alpha;
+  v[4] = (double)r.albedo.x; v[5] = (double)r.albedo.y; v[6] = (double)r.albedo.z;
+  v[7] = r.blank ? 0.0 : 1.0;
+  v[8] = (double)r.normal.x; v[9] = (double)r.normal.y; v[10] = (double)r.normal.z;
+  v[11] = 1.0;
+  const double lum = (double)dot(r.li, f3{0.2126f, 0.7152f, 0.0722f});
+  v[12] = lum * lum;
+}
 __device__ __forceinline__ void add_sample(double s[13], const SampleRec &r) {
-  s[0] += (double)r.li.x; s[1] += (double)r.li.y; s[2] += (double)r.li.z; s[3] += (double)r.alpha;
-  s[4] += (double)r.albedo.x; s[5] += (double)r.albedo.y; s[6] += (double)r.albedo.z;
-  s[7] += r.blank ? 0.0 : 1.0;
-  s[8] += (double)r.normal.x; s[9] += (double)r.normal.y; s[10] += (double)r.normal.z;
-  s[11] += 1.0;
-  double lum = (double)dot(r.li, f3{0.2126f, 0.7152f, 0.0722f});
-  s[12] += lum * lum;
+  double v[13];
+  sample_values(r, v);
+#pragma unroll
+  for (int c = 0; c < 13; c++) s[c] += v[c];
 }
 
 }  // namespace mtx

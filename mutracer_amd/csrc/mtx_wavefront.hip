@@ -471,6 +471,73 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
   }
 }
 
+// Samples of the pixel a lane handles that the accumulation takes: all of
+// them, or (adaptive sampling) those still below the pixel's target.
+__device__ __forceinline__ uint32_t samples_taken(const Accumulators &acc, uint32_t pix, uint32_t count0,
+                                                  uint32_t samples) {
+  if (!acc.targets) return samples;
+  const double target = (double)acc.targets[pix];
+  uint32_t done = 0;
+  while (done < samples && (double)(count0 + done) < target) done++;
+  return done;
+}
+
+// One job with kS samples per pixel, kS dividing 64: a wave takes 64 / kS
+// pixels with one lane per path (a pixel's paths are adjacent, so the path
+// loads are coalesced), puts each sample's 13 values in LDS, and then lane
+// (pixel, channel) adds the pixel's samples in sample order to its FP64 sum,
+// the same additions in the same order as k_wf_accumulate.
+template <int kS>
+__global__ void __launch_bounds__(256) k_wf_accumulate_lanes(const WfLaunch L, WfBuffers W) {
+  constexpr int kPx = 64 / kS;  // pixels per wave
+  __shared__ double vals[4][64][13];
+  __shared__ uint32_t count0s[4][kPx], dones[4][kPx];  // per pixel of a wave: count before, samples taken
+  const JobSlot &J = L.one;
+  const RenderParams &P = J.params;
+  const Accumulators &acc = J.acc;
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint32_t total = P.pixelCount;
+  for (uint32_t first = blockIdx.x * 4u * kPx; first < total; first += gridDim.x * 4u * kPx) {  // block-uniform
+    const uint32_t base = first + wv * kPx;
+    {
+      const uint32_t local = base + lane / kS, s = lane % kS;
+      if (local < total) {
+        const uint32_t pix = P.pixelBase + local;
+        const uint32_t count0 = acc.count[pix], done = samples_taken(acc, pix, count0, kS);
+        if (s == 0u) {
+          count0s[wv][lane / kS] = count0;
+          dones[wv][lane / kS] = done;
+        }
+        if (s < done) {
+          const uint32_t p = J.pathBase + local * kS + s;
+          const float4 l = W.li[p], d = W.d[p], al = W.alb[p], nr = W.nrm[p];
+          PathRegs pr;
+          pr.li = f3of(l);
+          pr.flags = (uint32_t)fint(l.w);
+          pr.alpha = d.w;
+          pr.auxAlbedo = f3of(al);
+          pr.auxNormal = f3of(nr);
+          double v[13];
+          sample_values(finish_path(P, pr), v);
+#pragma unroll
+          for (int c = 0; c < 13; c++) vals[wv][lane][c] = v[c];
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < (uint32_t)kPx * 13u; k += 64u) {
+      const uint32_t slot = k / 13u, c = k - slot * 13u, local = base + slot;
+      if (local >= total) continue;
+      const uint32_t pix = P.pixelBase + local, done = dones[wv][slot];
+      double sum = acc.planes[(size_t)c * acc.pixels + pix];
+      for (uint32_t s = 0; s < done; s++) sum += vals[wv][slot * kS + s][c];
+      acc.planes[(size_t)c * acc.pixels + pix] = sum;
+      if (c == 0u) acc.count[pix] = count0s[wv][slot] + done;
+    }
+    __syncthreads();  // the next pixels reuse vals
+  }
+}
+
 template <bool kMulti>
 __global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffers W) {
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -494,12 +561,10 @@ __global__ void __launch_bounds__(256) k_wf_accumulate(const WfLaunch L, WfBuffe
 #pragma unroll
     for (int c = 0; c < 13; c++) sums[c] = acc.planes[(size_t)c * acc.pixels + pix];
     const uint32_t count0 = acc.count[pix];
-    uint32_t done = P.samples;
-    if (acc.targets) {
-      const double target = (double)acc.targets[pix];
-      done = 0;
-      while (done < P.samples && (double)(count0 + done) < target) done++;
-    }
+    const uint32_t done = samples_taken(acc, pix, count0, P.samples);
+    // Unrolled so that the loads of several samples are in flight at once;
+    // the samples are still added in order.
+#pragma unroll 4
     for (uint32_t s = 0; s < done; s++) {
       const uint32_t p = J.pathBase + local * P.samples + s;
       const float4 l = W.li[p], d = W.d[p], al = W.alb[p], nr = W.nrm[p];
@@ -707,7 +772,21 @@ static void run_generation(const WfLaunch &L, const WfBuffers &W, int eventLimit
   }
   const uint32_t accGrid = std::min<uint32_t>((pixels + 255) / 256, g.cus * 16u);
   tb(0);
-  hipLaunchKernelGGL(k_wf_accumulate<kMulti>, dim3(accGrid), dim3(256), 0, s, L, W);
+  const uint32_t spp = L.one.params.samples;
+  if (!kMulti && spp != 0u && spp <= 64u && 64u % spp == 0u) {
+    const uint32_t blocks = std::min<uint32_t>((pixels + 4u * (64u / spp) - 1u) / (4u * (64u / spp)), g.cus * 16u);
+    switch (spp) {
+      case 1: hipLaunchKernelGGL(k_wf_accumulate_lanes<1>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      case 2: hipLaunchKernelGGL(k_wf_accumulate_lanes<2>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      case 4: hipLaunchKernelGGL(k_wf_accumulate_lanes<4>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      case 8: hipLaunchKernelGGL(k_wf_accumulate_lanes<8>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      case 16: hipLaunchKernelGGL(k_wf_accumulate_lanes<16>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      case 32: hipLaunchKernelGGL(k_wf_accumulate_lanes<32>, dim3(blocks), dim3(256), 0, s, L, W); break;
+      default: hipLaunchKernelGGL(k_wf_accumulate_lanes<64>, dim3(blocks), dim3(256), 0, s, L, W); break;
+    }
+  } else {
+    hipLaunchKernelGGL(k_wf_accumulate<kMulti>, dim3(accGrid), dim3(256), 0, s, L, W);
+  }
   te();
   MTX_HIP(hipGetLastError());
 }
