@@ -13,8 +13,9 @@ constexpr int kXcds = 8;
 // Occupancy the register allocator must reach for the traversal kernels
 // (waves per SIMD). Measured on the 1M-triangle 1080p workload: closest-hit
 // is fastest at 6 (80 VGPRs, a few spilled; 5 unconstrained, 8 spills too
-// much), any-hit at 8 (64 VGPRs). The 16-entry LDS stack keeps LDS from
-// capping these.
+// much), any-hit at 8 (64 VGPRs); re-measured with the unified closest-hit
+// node step, 5 and 7 are slightly slower. The 16-entry LDS stack keeps LDS
+// from capping these.
 constexpr int kTraceWaves = 6;
 constexpr int kShadowWaves = 8;
 
