@@ -8,16 +8,20 @@
 // to a multiple of 16 (padding channels hold zeros), so a 16-channel slice of
 // one pixel is one 32-byte run.
 //
-// Convolution = implicit GEMM on v_mfma_f32_16x16x16 (f16 / bf16): one wave
-// computes kMT tiles of 16 consecutive output pixels of a row (the M = 16
-// rows of each MFMA tile) for all output channels (NT tiles of N = 16),
-// stepping K over 9 taps x 16-channel input slices. The A operand (16 pixels x 16 channels) is one
-// 8-byte load per lane straight from the NHWC tensor; the B operand (16
-// channels x 16 outputs) comes from weights pre-packed per (tap, slice,
-// output, channel). The decoder's upsample + concat is folded into the A load:
-// slices below the first source's channel count read the half-resolution
-// tensor at (y/2, x/2), the rest read the skip tensor. Bias, ReLU and the
-// conversion to the storage type are fused into the epilogue.
+// Convolution = implicit GEMM on gfx950's v_mfma_f32_16x16x32 (f16 / bf16):
+// the M = 16 rows of an MFMA tile are 16 consecutive output pixels of a row,
+// N = 16 output channels, and K steps over 9 taps x pairs of 16-channel input
+// slices (an odd slice count ends with a step whose second slice is zero on
+// both operands). Each lane loads 8 consecutive channels of its pixel (A) and
+// of its output channel (B): one 16-byte load per operand. The B operand comes
+// from weights pre-packed per (tap, slice pair, output, channel). Two kernels
+// share that order: k_conv3x3 (operands straight from L1/L2) and
+// k_conv3x3_staged (a block of four waves computes four rows; the input halo
+// and the weights of each K step are staged in LDS). The decoder's upsample +
+// concat is folded into the A load: slices below the first source's channel
+// count read the half-resolution tensor at (y/2, x/2), the rest read the skip
+// tensor. Bias, ReLU, the optional 2x2 max pool and the conversion to the
+// storage type are fused into the epilogue.
 #include <chrono>
 #include <mutex>
 

@@ -144,6 +144,8 @@ def _declare(L):
     L.orc_trace_brute_batch.argtypes = [vp, vp, u32, u32, vp]
     if hasattr(L, "orc_render"):
         _declare_render(L)
+    if hasattr(L, "orc_bsdf_eval"):
+        _declare_bsdf(L)
 
 
 def _declare_render(L):
@@ -452,6 +454,77 @@ def _declare_render2(L):
     L.orc_path_draws.restype = u32
     L.orc_path_draws.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, vp, u32,
                                  vp, vp, u32]
+
+
+def _declare_bsdf(L):
+    vp, u32, f32 = C.c_void_p, C.c_uint32, C.c_float
+    L.orc_bsdf_eval.argtypes = [vp, vp, u32, vp, vp, vp]
+    L.orc_bsdf_sample.argtypes = [vp, vp, u32, vp, vp, vp]
+    L.orc_bsdf_lobes.argtypes = [vp, vp, vp, vp]
+    L.orc_metal_fresnel.argtypes = [f32, vp, vp, vp, vp]
+    L.orc_coat_energy.argtypes = [vp, f32]
+    L.orc_coat_energy.restype = f32
+
+
+def _material_and_view(material, view):
+    m = np.ascontiguousarray(material.pack() if hasattr(material, "pack") else material, np.float32)
+    v = np.ascontiguousarray(view, np.float32)
+    return m, v
+
+
+def bsdf_eval(material, view, dirs):
+    """BSDF * |cos| (n, 3) and pdf (n,) of unit world directions `dirs` (n, 3)
+    for `material` on the outside of a surface with normal +z, seen along the
+    unit vector `view` (toward the viewer)."""
+    L = lib()
+    _energy_lut()
+    m, v = _material_and_view(material, view)
+    d = np.ascontiguousarray(dirs, np.float32)
+    f = np.zeros((d.shape[0], 3), np.float32)
+    pdf = np.zeros(d.shape[0], np.float32)
+    L.orc_bsdf_eval(_p(m), _p(v), d.shape[0], _p(d), _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample(material, view, u):
+    """Directions (n, 3) and acceptance flags (n,) of the lobe sampler for
+    numbers u (n, 3) = (rx, ry, lobe) in [0, 1)."""
+    L = lib()
+    _energy_lut()
+    m, v = _material_and_view(material, view)
+    uu = np.ascontiguousarray(u, np.float32)
+    d = np.zeros((uu.shape[0], 3), np.float32)
+    ok = np.zeros(uu.shape[0], np.uint8)
+    L.orc_bsdf_sample(_p(m), _p(v), uu.shape[0], _p(uu), _p(d), _p(ok))
+    return d, ok.astype(bool)
+
+
+def bsdf_lobes(material, view):
+    """Lobe probabilities (fuzz, coat, metal, specular, transmission, diffuse,
+    subsurface) and the coat's view-side base scale (RGB)."""
+    L = lib()
+    _energy_lut()
+    m, v = _material_and_view(material, view)
+    p = np.zeros(7, np.float32)
+    cbs = np.zeros(3, np.float32)
+    L.orc_bsdf_lobes(_p(m), _p(v), _p(p), _p(cbs))
+    return p, cbs
+
+
+def metal_fresnel(cosine, f0, tint):
+    """The oracle's F82-tint metal Fresnel at `cosine` and its hemispherical average."""
+    f0 = np.ascontiguousarray(f0, np.float32)
+    tint = np.ascontiguousarray(tint, np.float32)
+    out = np.zeros(3, np.float32)
+    avg = np.zeros(3, np.float32)
+    lib().orc_metal_fresnel(float(cosine), _p(f0), _p(tint), _p(out), _p(avg))
+    return out, avg
+
+
+def coat_energy(material, cosine):
+    _energy_lut()
+    m = np.ascontiguousarray(material.pack() if hasattr(material, "pack") else material, np.float32)
+    return float(lib().orc_coat_energy(_p(m), float(cosine)))
 
 
 def emissive_triangles(scene):

@@ -188,6 +188,20 @@ void orc_trace_brute_batch(void *sp, const float *rays, uint32_t n, uint32_t dou
 // ---------------------------------------------------------------- integrator
 void orc_set_energy_lut(const float *lut) { set_energy_lut(lut); }
 
+void orc_bsdf_eval(const float *m, const float *view, uint32_t n, const float *dirs, float *f, float *pdf) {
+  probe_bsdf_eval(m, view, n, dirs, f, pdf);
+}
+void orc_bsdf_sample(const float *m, const float *view, uint32_t n, const float *u, float *dirs, uint8_t *ok) {
+  probe_bsdf_sample(m, view, n, u, dirs, ok);
+}
+void orc_bsdf_lobes(const float *m, const float *view, float *probabilities, float *coatBaseScale) {
+  probe_lobes(m, view, probabilities, coatBaseScale);
+}
+void orc_metal_fresnel(float c, const float *f0, const float *tint, float *out, float *average) {
+  probe_metal_fresnel(c, f0, tint, out, average);
+}
+float orc_coat_energy(const float *m, float c) { return probe_coat_energy(m, c); }
+
 // settings: width, height, bounces, candidates, pattern, seed, logSPP, sampleOffset.
 // out: w*h*13 doubles per pixel (color rgba, albedo rgba, normal rgba, lum^2).
 uint32_t orc_emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,

@@ -1822,4 +1822,57 @@ uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint
   return rays;
 }
 
+// ------------------------------------------------------------ BSDF probes
+// The material as sample_material prepares it on the outside of a surface
+// whose shading and face normals are +z, seen along `view` (unit vector from
+// the surface toward the viewer, view.z > 0). Used by the oracle's own tests
+// to check the lobes against closed forms and quadrature.
+static Prepared probe_prepare(const float *m, const float *view) {
+  Mat ma = get_material(m, V3(1.0f));
+  ma.inside = false;
+  ma.relativeIor = 1.0f / ma.specularIor;
+  return prepare(ma, V3(view[0], view[1], view[2]), V3(0.0f, 0.0f, 1.0f));
+}
+
+void probe_bsdf_eval(const float *m, const float *view, uint32_t n, const float *dirs, float *f, float *pdf) {
+  Prepared P = probe_prepare(m, view);
+  for (uint32_t i = 0; i < n; i++) {
+    float p = 0.0f;
+    V3 r = eval_prepared(P, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), p);
+    f[3 * i] = r.x;
+    f[3 * i + 1] = r.y;
+    f[3 * i + 2] = r.z;
+    pdf[i] = p;
+  }
+}
+
+void probe_bsdf_sample(const float *m, const float *view, uint32_t n, const float *u, float *dirs, uint8_t *ok) {
+  Prepared P = probe_prepare(m, view);
+  for (uint32_t i = 0; i < n; i++) {
+    V3 d;
+    bool transmission = false;
+    ok[i] = sample_direction(P, u[3 * i], u[3 * i + 1], u[3 * i + 2], d, transmission) ? 1 : 0;
+    dirs[3 * i] = d.x;
+    dirs[3 * i + 1] = d.y;
+    dirs[3 * i + 2] = d.z;
+  }
+}
+
+void probe_lobes(const float *m, const float *view, float *probabilities, float *coatBaseScale) {
+  Prepared P = probe_prepare(m, view);
+  for (int l = 0; l < L_COUNT; l++) probabilities[l] = P.p[l];
+  coatBaseScale[0] = P.coatBaseScale.x;
+  coatBaseScale[1] = P.coatBaseScale.y;
+  coatBaseScale[2] = P.coatBaseScale.z;
+}
+
+void probe_metal_fresnel(float c, const float *f0, const float *tint, float *out, float *average) {
+  V3 a(f0[0], f0[1], f0[2]), t(tint[0], tint[1], tint[2]);
+  V3 r = metal_fresnel(c, a, t), avg = metal_average_fresnel(a, t);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  average[0] = avg.x; average[1] = avg.y; average[2] = avg.z;
+}
+
+float probe_coat_energy(const float *m, float c) { return coat_energy(get_material(m, V3(1.0f)), c); }
+
 }  // namespace orc

@@ -91,4 +91,19 @@ uint64_t render(const RenderScene &s, uint32_t x0, uint32_t y0, uint32_t w, uint
 uint32_t path_draws(const RenderScene &s, uint32_t px, uint32_t py, uint32_t sampleIndex, uint32_t *log,
                     uint32_t cap);
 
+// BSDF probes: material m (M_FLOATS floats) on the outside of a surface with
+// normal +z, seen along the unit vector `view` (view.z > 0).
+// probe_bsdf_eval: f = BSDF * |cos| (RGB) and pdf of n world directions.
+void probe_bsdf_eval(const float *m, const float *view, uint32_t n, const float *dirs, float *f, float *pdf);
+// probe_bsdf_sample: directions for n triples (rx, ry, lobe) of [0,1) numbers;
+// ok[i] = 0 where the sampler rejects the draw.
+void probe_bsdf_sample(const float *m, const float *view, uint32_t n, const float *u, float *dirs, uint8_t *ok);
+// Lobe selection probabilities (fuzz, coat, metal, specular, transmission,
+// diffuse, subsurface) and the coat's view-side base scale.
+void probe_lobes(const float *m, const float *view, float *probabilities, float *coatBaseScale);
+// F82-tint metal Fresnel at cosine c, and its hemispherical average.
+void probe_metal_fresnel(float c, const float *f0, const float *tint, float *out, float *average);
+// Directional energy the coat reflects at cosine c.
+float probe_coat_energy(const float *m, float c);
+
 }  // namespace orc
