@@ -416,9 +416,9 @@ void target_reset(TargetObj &T) {
 }
 
 // Validated render parameters of a settings + camera pair (whole image).
-static RenderParams params_from(const mtx_render_settings &s, const mtx_camera_desc &cam) {
-  MTX_REQUIRE(s.struct_size >= sizeof(mtx_render_settings), MTX_INVALID_ARGUMENT,
-              "mtx_render_settings.struct_size too small");
+static RenderParams params_from(const mtx_target_settings &s, const mtx_camera_desc &cam) {
+  MTX_REQUIRE(s.struct_size >= sizeof(mtx_target_settings), MTX_INVALID_ARGUMENT,
+              "mtx_target_settings.struct_size too small");
   MTX_REQUIRE(s.width > 0 && s.height > 0 && (uint64_t)s.width * s.height <= (1ull << 28), MTX_INVALID_ARGUMENT,
               "invalid target size");
   MTX_REQUIRE(s.sampling_pattern <= 2, MTX_INVALID_ARGUMENT,
@@ -492,7 +492,7 @@ static void interleave_planes(const std::vector<double> &planes, size_t N, doubl
     for (int k = 0; k < 13; k++) sums[i * 13 + k] = planes[(size_t)k * N + i];
 }
 
-std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, const mtx_render_settings &s,
+std::shared_ptr<TargetObj> create_target(std::shared_ptr<RenderSceneObj> scene, const mtx_target_settings &s,
                                          const mtx_camera_desc &cam) {
   auto T = std::make_shared<TargetObj>();
   T->scene = scene;
@@ -601,6 +601,24 @@ int32_t mtx_scene_set_transforms(mtx_handle scene, const double *transforms, uin
   API_CATCH(err, cap)
 }
 
+int32_t mtx_scene_update_mesh(mtx_handle scene, uint32_t mesh, const float *positions, const float *normals,
+                              uint32_t vertex_count, uint32_t *rebuild_advised, char *err, size_t cap) {
+  API_TRY
+  auto R = get_scene(scene);
+  MTX_REQUIRE(positions && normals, MTX_INVALID_ARGUMENT, "positions/normals are NULL");
+  MTX_REQUIRE(mesh < R->blases.size(), MTX_INVALID_ARGUMENT, "mesh index out of range");
+  Blas &B = *R->blases[mesh];
+  MTX_REQUIRE(vertex_count == B.vertexInput, MTX_INVALID_ARGUMENT, "vertex count differs from the commit's");
+  DevBuf<float> pos;
+  pos.upload(positions, (size_t)vertex_count * 3, 0);
+  R->normals[mesh].upload(normals, (size_t)vertex_count * 3, 0);
+  const bool advise = refit_blas(B, pos.ptr, R->indices[mesh].ptr, 0);
+  refit_accel_blas(*R->accel, mesh, 0);
+  if (rebuild_advised) *rebuild_advised = advise ? 1u : 0u;
+  return MTX_OK;
+  API_CATCH(err, cap)
+}
+
 int32_t mtx_scene_info(mtx_handle scene, uint64_t info[4]) {
   API_TRY
   auto R = get_scene(scene);
@@ -618,7 +636,7 @@ int32_t mtx_scene_destroy(mtx_handle scene) {
   return gScenes.erase(scene) ? MTX_OK : MTX_INVALID_HANDLE;
 }
 
-int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings, const mtx_camera_desc *camera,
+int32_t mtx_target_create(mtx_handle scene, const mtx_target_settings *settings, const mtx_camera_desc *camera,
                           mtx_handle *out, char *err, size_t cap) {
   API_TRY
   MTX_REQUIRE(settings && camera && out, MTX_INVALID_ARGUMENT, "settings/camera/out is NULL");
@@ -991,10 +1009,14 @@ static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
     }
     S.rounds++;
   }
-  // Cancelled running jobs end between rounds.
+  // Cancelled jobs end between rounds: running ones, and admitted ones that
+  // never got a slot (max_jobs or the round's path budget ran out).
   for (uint64_t h : S.running) {
     auto J = S.jobs.at(h);
-    if (J->cancel && J->state == MTX_JOB_RUNNING) { J->state = MTX_JOB_CANCELLED; J->completeMs = now_ms(); }
+    if (J->cancel && (J->state == MTX_JOB_RUNNING || J->state == MTX_JOB_QUEUED)) {
+      J->state = MTX_JOB_CANCELLED;
+      J->completeMs = now_ms();
+    }
   }
   std::vector<uint64_t> keep;
   for (uint64_t h : S.running) {
@@ -1028,7 +1050,7 @@ int32_t mtx_server_create(const mtx_server_config *cfg, mtx_handle *out, char *e
   API_CATCH(err, cap)
 }
 
-int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_render_settings *settings,
+int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_target_settings *settings,
                           const mtx_camera_desc *camera, uint32_t samples, mtx_handle *job, char *err, size_t cap) {
   API_TRY
   MTX_REQUIRE(settings && camera && job, MTX_INVALID_ARGUMENT, "settings/camera/job is NULL");
@@ -1062,13 +1084,13 @@ int32_t mtx_server_step(mtx_handle server, void *stream, uint32_t *completed, ui
   API_CATCH(nullptr, 0)
 }
 
-int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_job_info *info) {
+int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_served_job_info *info) {
   API_TRY
   MTX_REQUIRE(info, MTX_INVALID_ARGUMENT, "info is NULL");
   auto S = get_server(server);
   std::lock_guard<std::mutex> g(S->m);
   auto J = get_job(*S, job);
-  info->struct_size = sizeof(mtx_job_info);
+  info->struct_size = sizeof(mtx_served_job_info);
   info->state = J->state;
   info->requested_samples = J->requested;
   info->submitted_samples = J->submitted;
@@ -1128,10 +1150,14 @@ int32_t mtx_server_job_cancel(mtx_handle server, mtx_handle job) {
   std::lock_guard<std::mutex> g(S->m);
   auto J = get_job(*S, job);
   if (J->state == MTX_JOB_QUEUED || J->state == MTX_JOB_RUNNING) J->cancel = true;
-  if (J->state == MTX_JOB_QUEUED && !J->admitted) {
+  // Queued work stops at once (mutracer-context.h:155-157), admitted or not:
+  // no sample of it is on the device between rounds. Running jobs end when
+  // the next round returns.
+  if (J->state == MTX_JOB_QUEUED) {
     J->state = MTX_JOB_CANCELLED;
     J->completeMs = now_ms();
     S->queued.erase(std::remove(S->queued.begin(), S->queued.end(), job), S->queued.end());
+    S->running.erase(std::remove(S->running.begin(), S->running.end(), job), S->running.end());
   }
   return MTX_OK;
   API_CATCH(nullptr, 0)

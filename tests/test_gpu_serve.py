@@ -115,6 +115,28 @@ def test_cancel(env):
     assert idle
 
 
+def test_cancel_admitted_without_slot(env):
+    """A job admitted by the budget but given no slot (max_jobs_per_round=1)
+    that is cancelled ends at once: it leaves the running set, release works
+    and its accumulator bytes return to the budget."""
+    render, serve, scenes, rs = env
+    one_job = 16 * 16 * (13 * 8 + 4)
+    server = serve.Server(max_jobs_per_round=1, max_samples_per_job_round=1)
+    a = server.submit(rs[0], 16, 16, 3)
+    b = server.submit(rs[0], 16, 16, 3)
+    server.step()
+    assert server.info(a)["state"] == "running" and server.info(b)["state"] == "queued"
+    assert server.stats()["admitted_bytes"] == 2 * one_job
+    server.cancel(b)
+    assert server.info(b)["state"] == "cancelled" and server.info(b)["completed"] == 0
+    assert server.stats()["running"] == 1
+    server.release(b)
+    assert server.stats()["admitted_bytes"] == one_job
+    while not server.step()[1]:
+        pass
+    assert server.info(a)["state"] == "complete" and server.info(a)["completed"] == 3
+
+
 def test_stream_latencies(env):
     render, serve, scenes, rs = env
     server = serve.Server(max_samples_per_job_round=8)
