@@ -79,6 +79,18 @@ int32_t mtx_blas_info(mtx_handle blas, uint32_t info[8]);
 int32_t mtx_blas_export(mtx_handle blas, const uint32_t *indices, uint32_t *words,
                         size_t capacity_words, size_t *written_words, uint32_t *source_of_record);
 
+/* refitBLAS (src/renderer/bvhAccel.ts:800-870): new vertex positions of the
+ * mesh the BLAS was built from (same vertex count, same index buffer, host or
+ * device per MTX_INPUT_DEVICE); node boxes are recomputed bottom-up on the GPU
+ * (a leaf spans the finite vertices of its triangles, an interior node its
+ * children), the triangle records re-gathered, topology and leaf order kept.
+ * *rebuild_advised (optional) = 1 when the refitted node surface area exceeds
+ * twice the build's (DYNAMIC_BLAS_REBUILD_RATIO, bvhAccel.ts:743): a dynamic
+ * mesh should then be rebuilt. Synchronous. */
+int32_t mtx_blas_refit(mtx_handle blas, const float *positions, uint32_t vertex_count, const uint32_t *indices,
+                       uint32_t triangle_count, uint32_t flags, uint32_t *rebuild_advised, char *error,
+                       size_t error_capacity);
+
 int32_t mtx_blas_destroy(mtx_handle blas);
 
 /* ----------------------------------------------------- scene acceleration */
@@ -249,11 +261,20 @@ int32_t mtx_scene_create(const mtx_scene_desc *desc, mtx_handle *out_scene, char
 int32_t mtx_scene_set_transforms(mtx_handle scene, const double *transforms, uint32_t instance_count,
                                  const uint32_t *dirty, uint32_t dirty_count, char *error, size_t error_capacity);
 
+/* Vertex deformation of mesh `mesh` of a committed scene (refitBLAS plus
+ * refitTLAS, as WGPURayTracePass.ts:2735 takes the refit path when the
+ * topology is unchanged): new object-space positions and normals of the
+ * commit's vertex count; the mesh's BLAS is refitted, every instance of it
+ * takes the new root box and the TLAS is refitted. Emissive triangles keep
+ * the world-space list of the commit. *rebuild_advised as mtx_blas_refit. */
+int32_t mtx_scene_update_mesh(mtx_handle scene, uint32_t mesh, const float *positions, const float *normals,
+                              uint32_t vertex_count, uint32_t *rebuild_advised, char *error, size_t error_capacity);
+
 /* info[0] triangles, [1] instances, [2] BLAS count, [3] resident MiB. */
 int32_t mtx_scene_info(mtx_handle scene, uint64_t info[4]);
 int32_t mtx_scene_destroy(mtx_handle scene);
 
-typedef struct mtx_render_settings {
+typedef struct mtx_target_settings {
   uint32_t struct_size;
   uint32_t width, height;
   uint32_t bounces;                 /* RayTraceUniform.bounces */
@@ -266,7 +287,7 @@ typedef struct mtx_render_settings {
   float clamping;                   /* 0 = off */
   uint32_t kernel;                  /* MTX_KERNEL_WAVEFRONT (0) or MTX_KERNEL_MEGAKERNEL (1) */
   uint32_t max_paths_in_flight;     /* wavefront generation size; 0 = 8M paths */
-} mtx_render_settings;
+} mtx_target_settings;
 
 #define MTX_KERNEL_WAVEFRONT 0u
 #define MTX_KERNEL_MEGAKERNEL 1u
@@ -284,7 +305,7 @@ typedef struct mtx_camera_desc {
  * normal, luminance^2) and the per-pixel sample counter that gives each
  * pixel's next Sobol index (rayTrace.compute.ts:1188). Replaces
  * RenderTarget.trace (RenderingPipeline.ts:218) / WGPURayTracePass.createPass. */
-int32_t mtx_target_create(mtx_handle scene, const mtx_render_settings *settings, const mtx_camera_desc *camera,
+int32_t mtx_target_create(mtx_handle scene, const mtx_target_settings *settings, const mtx_camera_desc *camera,
                           mtx_handle *out_target, char *error, size_t error_capacity);
 
 /* Adds `samples` path samples to every pixel on `stream` (NULL = default);
@@ -386,6 +407,123 @@ int32_t mtx_target_set_sample_offset(mtx_handle target, uint32_t sample_sequence
 int32_t mtx_target_reset(mtx_handle target);
 int32_t mtx_target_destroy(mtx_handle target);
 
+/* ------------------------------------------------------------ context API */
+/* Persistent contexts over the raw scene descriptor (versions 5..8,
+ * native/docs/scene-format.md), with the semantics of mutracer-context.h:12-187:
+ * commit copies and validates a descriptor (INVALID_ARGUMENT names the
+ * problem; a feature this build does not render returns UNAVAILABLE), a failed
+ * commit keeps the previous scene, every successful commit restarts
+ * accumulation; blocking renders and one asynchronous job per context
+ * (submit / poll / info / cancel); read_pixels returns the accumulated sums
+ * divided by the completed sample count, RGBA f32, bottom row first. A GPU
+ * failure during render, poll, read or reset marks the context failed and
+ * returns MTX_CONTEXT_FAILED; later calls return MTX_INVALID_STATE until a
+ * successful commit. Calls may come from any thread; an operation that
+ * overlaps another on the same context returns MTX_BUSY. Every context owns
+ * one HIP stream (default priority) for its GPU work. */
+
+/* mt_render_settings (mutracer-context.h:110-127), 64 bytes, same field
+ * order and meaning. bevel_samples, caustic_flags, the photon fields and
+ * transmissive_output_alpha_min below 1 select estimators this build does
+ * not have: nonzero caustic flags or photon counts and an alpha minimum other
+ * than 1 return MTX_UNAVAILABLE. */
+typedef struct mtx_render_settings {
+  uint32_t struct_size, sampling_pattern, seed, max_samples;
+  uint32_t bounces, direct_light_candidates, bevel_samples, sample_sequence_offset;
+  double clamping, transmissive_output_alpha_min;
+  uint32_t caustic_flags;
+  uint32_t light_photons, photon_map_photons;
+  float photon_map_radius;
+} mtx_render_settings;
+#if defined(__cplusplus) && __cplusplus >= 201103L
+static_assert(sizeof(mtx_render_settings) == 64, "mtx_render_settings layout");
+#elif defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L
+_Static_assert(sizeof(mtx_render_settings) == 64, "mtx_render_settings layout");
+#endif
+
+/* mt_render_settings_default (mutracer-context.h:128-129): Sobol, seed 12345,
+ * max_samples 64, bounces 6, candidates 2, bevel samples 8, offset 0,
+ * clamping 0, transmissive alpha minimum 1. No device needed. */
+int32_t mtx_render_settings_default(mtx_render_settings *output);
+
+/* mt_context_snapshot (mutracer-context.h:57-62), 24 bytes. */
+typedef struct mtx_context_snapshot {
+  uint32_t struct_size;
+  uint32_t width, height, samples;
+  uint32_t state;       /* MTX_CONTEXT_* */
+  uint32_t last_commit; /* MTX_COMMIT_* */
+} mtx_context_snapshot;
+#define MTX_CONTEXT_EMPTY 0u
+#define MTX_CONTEXT_COMMITTED 1u
+#define MTX_CONTEXT_FAILED_STATE 2u
+#define MTX_CONTEXT_JOB_ACTIVE 3u
+#define MTX_COMMIT_NONE 0u
+#define MTX_COMMIT_FULL 1u
+#define MTX_COMMIT_INCREMENTAL 2u /* camera-only or mesh-transform-only edit (TLAS refit) */
+
+/* mt_job_snapshot (mutracer-context.h:168-172), 24 bytes. */
+typedef struct mtx_job_snapshot {
+  uint32_t struct_size;
+  uint32_t state; /* MTX_JOB_* */
+  uint32_t requested_samples, submitted_samples, completed_samples, base_samples;
+} mtx_job_snapshot;
+
+/* mt_image_kind values accepted by mtx_context_read_pixels. */
+#define MTX_IMAGE_HDR 0u
+#define MTX_IMAGE_NORMAL 1u
+#define MTX_IMAGE_ALBEDO 2u
+
+/* mt_context_create / mt_context_destroy (mutracer-context.h:78,81).
+ * Destroy rejects an active job (MTX_INVALID_STATE) unless the context failed. */
+int32_t mtx_context_create(mtx_handle *output, char *error, size_t error_capacity);
+int32_t mtx_context_destroy(mtx_handle context, char *error, size_t error_capacity);
+/* mt_context_commit / mt_context_commit_with_settings (:92,130). A commit
+ * while a job is active waits for its in-flight batch and expires the job. */
+int32_t mtx_context_commit(mtx_handle context, const uint8_t *input, size_t length, char *error,
+                           size_t error_capacity);
+int32_t mtx_context_commit_with_settings(mtx_handle context, const uint8_t *input, size_t length,
+                                         const mtx_render_settings *settings, char *error, size_t error_capacity);
+/* mt_context_render (:132): adds samples and waits; total 1..16384. */
+int32_t mtx_context_render(mtx_handle context, uint32_t additional_samples, char *error, size_t error_capacity);
+/* mt_context_reset (:133). */
+int32_t mtx_context_reset(mtx_handle context, char *error, size_t error_capacity);
+/* mt_context_wait (:140-141): waits up to timeout_ms for submitted GPU work
+ * (MTX_WAIT_INDEFINITELY: no deadline, 0: only checks); MTX_TIMEOUT on expiry. */
+#define MTX_WAIT_INDEFINITELY UINT32_MAX
+int32_t mtx_context_wait(mtx_handle context, uint32_t timeout_ms, char *error, size_t error_capacity);
+/* mt_context_info (:142). info->struct_size must be sizeof(mtx_context_snapshot). */
+int32_t mtx_context_info(mtx_handle context, mtx_context_snapshot *info, char *error, size_t error_capacity);
+/* mt_context_read_pixels (:143-144): kind MTX_IMAGE_*, `size` in bytes,
+ * *written = width*height*16 (required). pixels == NULL with size 0 queries
+ * the size; a short destination returns MTX_BUFFER_TOO_SMALL. */
+int32_t mtx_context_read_pixels(mtx_handle context, uint32_t kind, void *pixels, size_t size, size_t *written,
+                                char *error, size_t error_capacity);
+/* mt_context_submit / mt_job_poll / mt_job_info / mt_job_cancel (:173-179).
+ * submit records the request; poll (max_new_samples 0..64) credits a finished
+ * batch without waiting, then dispatches the next batch asynchronously on the
+ * context's stream when none is in flight; info reports the last polled
+ * state; cancel stops queued work at once (an in-flight batch finishes:
+ * MTX_JOB_CANCELLING until the next poll). A job handle expires on the next
+ * submit, blocking render, reset, commit or context destruction. */
+int32_t mtx_context_submit(mtx_handle context, uint32_t additional_samples, mtx_handle *job, char *error,
+                           size_t error_capacity);
+int32_t mtx_job_poll(mtx_handle job, uint32_t max_new_samples, char *error, size_t error_capacity);
+int32_t mtx_job_info(mtx_handle job, mtx_job_snapshot *info, char *error, size_t error_capacity);
+int32_t mtx_job_cancel(mtx_handle job, char *error, size_t error_capacity);
+/* The validation and decoding mt_context_commit performs, without a device
+ * or a context (native/host/src/scene.rs validate, scene-format.md rules).
+ * info (optional): [0] width, [1] height, [2] BLAS meshes after splitting
+ * multi-material geometry, [3] instances, [4] materials, [5] analytic lights,
+ * [6] environment width, [7] environment height. */
+int32_t mtx_descriptor_validate(const uint8_t *input, size_t length, uint32_t info[8], char *error,
+                                size_t error_capacity);
+/* Test hook, no reference counterpart beyond its test-only injected flight
+ * failures (context.rs inject_flight_failure): the context's next GPU
+ * operation (render, poll dispatch, read or reset) fails as a GPU error would,
+ * before anything is launched, so failure handling can be tested without
+ * faulting a device. */
+int32_t mtx_context_inject_failure(mtx_handle context);
+
 /* -------------------------------------------------------------- serving */
 /* Batched render server: many render jobs (mixed scenes, resolutions, sample
  * counts) share every wavefront generation. The reference's native job API
@@ -409,17 +547,18 @@ typedef struct mtx_server_config {
 #define MTX_JOB_COMPLETE 2u
 #define MTX_JOB_CANCELLED 3u
 #define MTX_JOB_FAILED 4u
+#define MTX_JOB_CANCELLING 5u /* context jobs: cancelled with a batch still in flight */
 
-typedef struct mtx_job_info {
+typedef struct mtx_served_job_info {
   uint32_t struct_size;
   uint32_t state;
   uint32_t requested_samples, submitted_samples, completed_samples, base_samples;
   double submit_ms, start_ms, complete_ms; /* host steady clock, milliseconds */
-} mtx_job_info;
+} mtx_served_job_info;
 
 int32_t mtx_server_create(const mtx_server_config *config, mtx_handle *out_server, char *error, size_t error_capacity);
 /* Queues a job of `samples` samples per pixel on a committed scene (inputs copied). */
-int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_render_settings *settings,
+int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_target_settings *settings,
                           const mtx_camera_desc *camera, uint32_t samples, mtx_handle *out_job, char *error,
                           size_t error_capacity);
 /* Runs one round on `stream` and waits for it: admits queued jobs in arrival
@@ -428,7 +567,7 @@ int32_t mtx_server_submit(mtx_handle server, mtx_handle scene, const mtx_render_
  * one wavefront generation over all of them. *completed = jobs finished by
  * this round; *idle = 1 when there was nothing to run. */
 int32_t mtx_server_step(mtx_handle server, void *stream, uint32_t *completed, uint32_t *idle);
-int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_job_info *info);
+int32_t mtx_server_job_info(mtx_handle server, mtx_handle job, mtx_served_job_info *info);
 int32_t mtx_server_job_read_pixels(mtx_handle server, mtx_handle job, uint32_t kind, float *pixels,
                                    size_t capacity_floats, size_t *written);
 int32_t mtx_server_job_read_accumulators(mtx_handle server, mtx_handle job, double *sums, size_t capacity_doubles,
@@ -449,8 +588,27 @@ int32_t mtx_server_destroy(mtx_handle server);
 #define MTX_DENOISE_FP16 1u
 #define MTX_DENOISE_BF16 2u
 /* weights: the bytes of a version-2 tensor archive (.tza). */
-int32_t mtx_denoiser_create(const void *weights, size_t size, uint32_t precision, mtx_handle *out_denoiser,
-                            char *error, size_t error_capacity);
+/* mt_denoiser_create (mutracer-denoise.h:36-37), same signature: weights_path
+ * is the .tza network to load (the reference's `library` argument names the
+ * OIDN runtime; here the runtime is this library and the argument selects the
+ * weights); NULL uses the MTX_DENOISER_WEIGHTS environment variable, then the
+ * standard network shipped beside libmtx.so (data/rt_hdr_alb_nrm.tza).
+ * backend: MTX_DENOISE_BACKEND_AUTO or _HIP run the FP16 MFMA network; _CPU,
+ * _METAL and _CUDA return MTX_UNAVAILABLE (explicit backends never fall back). */
+#define MTX_DENOISE_BACKEND_AUTO 0u
+#define MTX_DENOISE_BACKEND_CPU 1u
+#define MTX_DENOISE_BACKEND_METAL 2u
+#define MTX_DENOISE_BACKEND_CUDA 3u
+#define MTX_DENOISE_BACKEND_HIP 4u
+int32_t mtx_denoiser_create(const char *weights_path, uint32_t backend, mtx_handle *output, char *error,
+                            size_t error_capacity);
+/* mt_denoiser_backend (mutracer-denoise.h:38): always MTX_DENOISE_BACKEND_HIP. */
+int32_t mtx_denoiser_backend(mtx_handle denoiser, uint32_t *output, char *error, size_t error_capacity);
+/* The network from TZA bytes at a chosen precision (MTX_DENOISE_FP16 or
+ * MTX_DENOISE_BF16). No reference counterpart: it exposes the precision the
+ * reference leaves to its runtime ('auto'). */
+int32_t mtx_denoiser_create_from_weights(const void *weights, size_t size, uint32_t precision,
+                                         mtx_handle *out_denoiser, char *error, size_t error_capacity);
 /* mt_denoiser_run: color, normal, albedo, output are RGBA f32 (host), count =
  * width*height*4; HDR color through the 'pu' transfer; output alpha = color
  * alpha; output may alias an input; nothing is written on failure. */
@@ -478,7 +636,8 @@ int32_t mtx_denoiser_set_tiling(mtx_handle denoiser, uint32_t layer, uint32_t pi
 /* The launch shape `layer` uses at width x height. */
 int32_t mtx_denoiser_get_tiling(mtx_handle denoiser, uint32_t layer, uint32_t width, uint32_t height,
                                 uint32_t *pixel_tiles, uint32_t *channel_tiles);
-int32_t mtx_denoiser_destroy(mtx_handle denoiser);
+/* mt_denoiser_destroy (mutracer-denoise.h:53); MTX_BUSY while a run is in progress. */
+int32_t mtx_denoiser_destroy(mtx_handle denoiser, char *error, size_t error_capacity);
 
 /* ------------------------------------------------------------ diagnostics */
 /* Evaluates the path tracer's elementary functions (include/mtx_fmath.h) on

@@ -77,6 +77,7 @@ def _declare(L):
         "mtx_blas_build": (i32, [vp, u32, vp, u32, u32, u32, P(u64), C.c_char_p, sz]),
         "mtx_blas_info": (i32, [u64, vp]),
         "mtx_blas_export": (i32, [u64, vp, vp, sz, P(sz), vp]),
+        "mtx_blas_refit": (i32, [u64, vp, u32, vp, u32, u32, P(u32), C.c_char_p, sz]),
         "mtx_blas_destroy": (i32, [u64]),
         "mtx_accel_create": (i32, [vp, u32, vp, u32, u32, u32, P(u64), C.c_char_p, sz]),
         "mtx_accel_refit": (i32, [u64, vp, u32, vp, u32, C.c_char_p, sz]),

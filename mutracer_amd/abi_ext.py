@@ -16,7 +16,7 @@ class SceneDesc(C.Structure):
                 ("blas_bins", C.c_uint32), ("tlas_bins", C.c_uint32), ("flags", C.c_uint32)]
 
 
-class RenderSettings(C.Structure):
+class TargetSettings(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("bounces", C.c_uint32), ("direct_light_candidates", C.c_uint32),
                 ("sampling_pattern", C.c_uint32), ("sequence_seed", C.c_uint32), ("max_samples", C.c_uint32),
@@ -48,6 +48,26 @@ class JobInfo(C.Structure):
                 ("submit_ms", C.c_double), ("start_ms", C.c_double), ("complete_ms", C.c_double)]
 
 
+class RenderSettings(C.Structure):
+    """mtx_render_settings: the 64-byte mt_render_settings record."""
+    _fields_ = [("struct_size", C.c_uint32), ("sampling_pattern", C.c_uint32), ("seed", C.c_uint32),
+                ("max_samples", C.c_uint32), ("bounces", C.c_uint32), ("direct_light_candidates", C.c_uint32),
+                ("bevel_samples", C.c_uint32), ("sample_sequence_offset", C.c_uint32), ("clamping", C.c_double),
+                ("transmissive_output_alpha_min", C.c_double), ("caustic_flags", C.c_uint32),
+                ("light_photons", C.c_uint32), ("photon_map_photons", C.c_uint32),
+                ("photon_map_radius", C.c_float)]
+
+
+class ContextSnapshot(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("samples", C.c_uint32), ("state", C.c_uint32), ("last_commit", C.c_uint32)]
+
+
+class JobSnapshot(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("state", C.c_uint32), ("requested_samples", C.c_uint32),
+                ("submitted_samples", C.c_uint32), ("completed_samples", C.c_uint32), ("base_samples", C.c_uint32)]
+
+
 class AdaptiveOptions(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("max_samples", C.c_uint32), ("min_samples", C.c_uint32),
                 ("tile_size", C.c_uint32), ("absolute_error", C.c_double), ("relative_error", C.c_double),
@@ -61,6 +81,7 @@ def declare(L):
         "mtx_scene_create": (i32, [vp, P(u64), C.c_char_p, sz]),
         "mtx_scene_set_transforms": (i32, [u64, vp, u32, vp, u32, C.c_char_p, sz]),
         "mtx_scene_info": (i32, [u64, vp]),
+        "mtx_scene_update_mesh": (i32, [u64, u32, vp, vp, u32, P(u32), C.c_char_p, sz]),
         "mtx_scene_destroy": (i32, [u64]),
         "mtx_target_create": (i32, [u64, vp, vp, P(u64), C.c_char_p, sz]),
         "mtx_target_render": (i32, [u64, u32, vp]),
@@ -89,13 +110,31 @@ def declare(L):
         "mtx_server_job_release": (i32, [u64, u64]),
         "mtx_server_stats": (i32, [u64, vp]),
         "mtx_server_destroy": (i32, [u64]),
-        "mtx_denoiser_create": (i32, [vp, sz, u32, P(u64), C.c_char_p, sz]),
+        "mtx_render_settings_default": (i32, [vp]),
+        "mtx_context_create": (i32, [P(u64), C.c_char_p, sz]),
+        "mtx_context_destroy": (i32, [u64, C.c_char_p, sz]),
+        "mtx_context_commit": (i32, [u64, vp, sz, C.c_char_p, sz]),
+        "mtx_context_commit_with_settings": (i32, [u64, vp, sz, vp, C.c_char_p, sz]),
+        "mtx_context_render": (i32, [u64, u32, C.c_char_p, sz]),
+        "mtx_context_reset": (i32, [u64, C.c_char_p, sz]),
+        "mtx_context_wait": (i32, [u64, u32, C.c_char_p, sz]),
+        "mtx_context_info": (i32, [u64, vp, C.c_char_p, sz]),
+        "mtx_context_read_pixels": (i32, [u64, u32, vp, sz, P(sz), C.c_char_p, sz]),
+        "mtx_context_submit": (i32, [u64, u32, P(u64), C.c_char_p, sz]),
+        "mtx_job_poll": (i32, [u64, u32, C.c_char_p, sz]),
+        "mtx_job_info": (i32, [u64, vp, C.c_char_p, sz]),
+        "mtx_job_cancel": (i32, [u64, C.c_char_p, sz]),
+        "mtx_descriptor_validate": (i32, [vp, sz, vp, C.c_char_p, sz]),
+        "mtx_context_inject_failure": (i32, [u64]),
+        "mtx_denoiser_create": (i32, [C.c_char_p, u32, P(u64), C.c_char_p, sz]),
+        "mtx_denoiser_create_from_weights": (i32, [vp, sz, u32, P(u64), C.c_char_p, sz]),
+        "mtx_denoiser_backend": (i32, [u64, P(u32), C.c_char_p, sz]),
         "mtx_denoiser_run": (i32, [u64, u32, u32, vp, vp, vp, vp, sz, C.c_char_p, sz]),
         "mtx_denoiser_forward": (i32, [u64, u32, u32, vp, vp, vp, C.c_char_p, sz]),
         "mtx_denoiser_benchmark": (i32, [u64, u32, u32, u32, P(C.c_double)]),
         "mtx_denoiser_set_tiling": (i32, [u64, u32, u32, u32]),
         "mtx_denoiser_get_tiling": (i32, [u64, u32, u32, u32, P(u32), P(u32)]),
-        "mtx_denoiser_destroy": (i32, [u64]),
+        "mtx_denoiser_destroy": (i32, [u64, C.c_char_p, sz]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

@@ -53,6 +53,16 @@ class Blas:
             self.indices_host = idx
         abi.check(code, err)
 
+    def refit(self, positions):
+        """refitBLAS: new vertex positions (host array, same vertex count) with
+        the build's index buffer. Returns True when a rebuild is advised."""
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        advised = C.c_uint32(0)
+        err = abi.errbuf()
+        abi.check(abi.lib().mtx_blas_refit(self.handle.value, _P(pos), pos.shape[0], _P(self.indices_host),
+                                           self.indices_host.size // 3, 0, C.byref(advised), err, 512), err)
+        return bool(advised.value)
+
     def info(self):
         out = np.zeros(8, np.uint32)
         abi.check(abi.lib().mtx_blas_info(self.handle.value, _P(out)))

@@ -220,6 +220,30 @@ int32_t mtx_blas_info(mtx_handle blas, uint32_t info[8]) {
   MTX_API_END(nullptr, 0)
 }
 
+int32_t mtx_blas_refit(mtx_handle blas, const float *positions, uint32_t vertex_count, const uint32_t *indices,
+                       uint32_t triangle_count, uint32_t flags, uint32_t *rebuild_advised, char *err, size_t cap) {
+  MTX_API_BEGIN
+  auto B = registry().get_blas(blas);
+  MTX_REQUIRE(positions && indices, MTX_INVALID_ARGUMENT, "positions/indices are NULL");
+  MTX_REQUIRE(vertex_count == B->vertexInput && triangle_count == B->triangleInput, MTX_INVALID_ARGUMENT,
+              "refit needs the build's vertex and triangle counts");
+  const bool dev = (flags & MTX_INPUT_DEVICE) != 0;
+  DevBuf<float> hpos;
+  DevBuf<uint32_t> hidx;
+  const float *dpos = positions;
+  const uint32_t *didx = indices;
+  if (!dev) {
+    hpos.upload(positions, (size_t)vertex_count * 3, 0);
+    hidx.upload(indices, (size_t)triangle_count * 3, 0);
+    dpos = hpos.ptr;
+    didx = hidx.ptr;
+  }
+  const bool advise = refit_blas(*B, dpos, didx, 0);
+  if (rebuild_advised) *rebuild_advised = advise ? 1u : 0u;
+  return MTX_OK;
+  MTX_API_END(err, cap)
+}
+
 int32_t mtx_blas_export(mtx_handle blas, const uint32_t *indices, uint32_t *words, size_t capacity,
                         size_t *written, uint32_t *source_of_record) {
   MTX_API_BEGIN

@@ -726,6 +726,52 @@ __global__ void k_emit_tris(const uint32_t *__restrict__ prim, const uint32_t *_
   srcOut[q] = t;
 }
 
+// --------------------------------------------------------- BLAS refit kernels
+// refitBLAS (bvhAccel.ts:800-870): node boxes recomputed bottom-up from new
+// vertex positions, topology kept. A leaf's box spans the finite vertices of
+// its triangles, an interior node's the union of its children's; min/max are
+// exact, so the boxes do not depend on the visiting order.
+__global__ void k_refit_level(BNode *__restrict__ nodes, const uint32_t *__restrict__ level, uint32_t cnt,
+                              const uint32_t *__restrict__ prim, const uint32_t *__restrict__ source,
+                              const float *__restrict__ pos, const uint32_t *__restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint32_t v = level[i];
+  const BNode nd = nodes[v];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (nd.primCount) {
+    for (uint32_t q = nd.leftFirst; q < nd.leftFirst + nd.primCount; q++) {
+      const uint32_t t = source[prim[q]];
+      for (int c = 0; c < 3; c++) {
+        const float *p = pos + (size_t)idx[(size_t)t * 3 + c] * 3;
+        const float x = p[0], y = p[1], z = p[2];
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) continue;
+        lo[0] = jminf(lo[0], x); lo[1] = jminf(lo[1], y); lo[2] = jminf(lo[2], z);
+        hi[0] = jmaxf(hi[0], x); hi[1] = jmaxf(hi[1], y); hi[2] = jmaxf(hi[2], z);
+      }
+    }
+  } else {
+    const BNode L = nodes[nd.leftFirst], R = nodes[nd.leftFirst + 1];
+    for (int k = 0; k < 3; k++) { lo[k] = jminf(L.lo[k], R.lo[k]); hi[k] = jmaxf(L.hi[k], R.hi[k]); }
+  }
+  for (int k = 0; k < 3; k++) { nodes[v].lo[k] = lo[k]; nodes[v].hi[k] = hi[k]; }
+}
+
+// Sum over nodes of 2 (xz + xy + zy): the surface-area measure refitBLAS
+// compares with the build's to advise a rebuild.
+__global__ void k_node_area(const BNode *__restrict__ nodes, uint32_t n, double *__restrict__ area) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0;
+  if (i < n) {
+    const BNode nd = nodes[i];
+    const double x = (double)nd.hi[0] - nd.lo[0], y = (double)nd.hi[1] - nd.lo[1], z = (double)nd.hi[2] - nd.lo[2];
+    a = 2.0 * (x * z + x * y + z * y);
+    if (!(a == a) || isinf(a)) a = 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o);
+  if ((threadIdx.x & 63u) == 0u && a != 0.0) atomicAdd(area, a);
+}
+
 // --------------------------------------------------------- TLAS refit kernels
 __global__ void k_instance_update(const double *__restrict__ transforms, const float *__restrict__ blasBoxes,
                                   const uint32_t *__restrict__ dirty, uint32_t nDirty,
@@ -971,6 +1017,7 @@ std::vector<std::shared_ptr<Blas>> build_blases(const MeshInput *meshes, size_t 
     w.B = std::make_shared<Blas>();
     Blas &B = *w.B;
     B.triangleInput = in.tcount;
+    B.vertexInput = in.vcount;
     w.dpos = in.pos;
     w.didx = in.idx;
     if (!in.deviceInput) {
@@ -1081,6 +1128,101 @@ size_t export_blas(const Blas &B, const uint32_t *indices, uint32_t *words, size
     }
   }
   return vec4 * 4;
+}
+
+// Sum of node surface areas (k_node_area) of a topology.
+static double node_area(const Topology &T, hipStream_t s) {
+  DevBuf<double> acc(1);
+  const double zero = 0.0;
+  acc.upload(&zero, 1, s);
+  hipLaunchKernelGGL(k_node_area, dim3(blocks(T.nodeCount)), dim3(256), 0, s, T.nodes.ptr, T.nodeCount, acc.ptr);
+  MTX_HIP(hipGetLastError());
+  return acc.to_host(1, s)[0];
+}
+
+// Node indices grouped by depth, the depth-first TNode index of every node
+// and the build's surface area: computed once, on the first refit.
+static void ensure_refit_data(Blas &B, hipStream_t s) {
+  Topology &T = B.topo;
+  const uint32_t nn = T.nodeCount;
+  if (B.refit.ready || !nn) return;
+  constexpr uint32_t kLevels = 256;
+  DevBuf<uint32_t> hist(kLevels), cursor(kLevels), size(nn);
+  std::vector<uint32_t> zeros(kLevels, 0);
+  hist.upload(zeros.data(), kLevels, s);
+  hipLaunchKernelGGL(k_depth_hist, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, hist.ptr);
+  std::vector<uint32_t> h = hist.to_host(kLevels, s);
+  std::vector<uint32_t> start(kLevels + 1, 0);
+  for (uint32_t d = 0; d < kLevels; d++) start[d + 1] = start[d] + h[d];
+  B.refit.byDepth.alloc(nn);
+  B.refit.dfs.alloc(nn);
+  cursor.upload(start.data(), kLevels, s);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, cursor.ptr,
+                     B.refit.byDepth.ptr);
+  for (uint32_t d = T.maxDepth; d >= 1; d--)
+    if (h[d])
+      hipLaunchKernelGGL(k_subtree, dim3(blocks(h[d])), dim3(256), 0, s, T.nodes.ptr, B.refit.byDepth.ptr + start[d],
+                         h[d], size.ptr);
+  const uint32_t zero = 0;
+  B.refit.dfs.upload(&zero, 1, s);
+  for (uint32_t d = 1; d <= T.maxDepth; d++)
+    if (h[d])
+      hipLaunchKernelGGL(k_dfs, dim3(blocks(h[d])), dim3(256), 0, s, T.nodes.ptr, B.refit.byDepth.ptr + start[d],
+                         h[d], size.ptr, B.refit.dfs.ptr);
+  MTX_HIP(hipGetLastError());
+  B.refit.levelStart.assign(start.begin(), start.end());
+  B.refit.levelCount.assign(h.begin(), h.end());
+  B.refit.buildArea = node_area(T, s);
+  B.refit.ready = true;
+}
+
+bool refit_blas(Blas &B, const float *pos, const uint32_t *idx, hipStream_t s) {
+  Topology &T = B.topo;
+  if (!T.nodeCount) return false;
+  ensure_refit_data(B, s);
+  const auto &start = B.refit.levelStart;
+  const auto &cnt = B.refit.levelCount;
+  for (uint32_t d = T.maxDepth; d >= 1; d--)
+    if (cnt[d])
+      hipLaunchKernelGGL(k_refit_level, dim3(blocks(cnt[d])), dim3(256), 0, s, T.nodes.ptr,
+                         B.refit.byDepth.ptr + start[d], cnt[d], T.prim.ptr, T.source.ptr, pos, idx);
+  if (B.layout.interiorCount)
+    hipLaunchKernelGGL(k_emit_nodes, dim3(blocks(T.nodeCount)), dim3(256), 0, s, T.nodes.ptr, T.nodeCount,
+                       B.refit.dfs.ptr, B.layout.tnodes.ptr);
+  DevBuf<uint32_t> leafCount(T.primCount);
+  hipLaunchKernelGGL(k_leaf_counts, dim3(blocks(T.nodeCount)), dim3(256), 0, s, T.nodes.ptr, T.nodeCount,
+                     leafCount.ptr);
+  hipLaunchKernelGGL(k_emit_tris, dim3(blocks(T.primCount)), dim3(256), 0, s, T.prim.ptr, T.source.ptr,
+                     leafCount.ptr, T.primCount, pos, idx, B.tris.ptr, B.triSource.ptr);
+  MTX_HIP(hipGetLastError());
+  BNode root;
+  T.nodes.download(&root, 1, s);
+  const double area = node_area(T, s);  // synchronizes
+  for (int k = 0; k < 3; k++) { B.layout.rootLo[k] = root.lo[k]; B.layout.rootHi[k] = root.hi[k]; }
+  return area > B.refit.buildArea * kDynamicBlasRebuildRatio;
+}
+
+// New BLAS root boxes for the instances of `blas`, then the TLAS refit over them.
+void refit_accel_blas(Accel &A, uint32_t blas, hipStream_t s) {
+  const Blas &B = *A.blases.at(blas);
+  std::vector<uint32_t> list;
+  for (uint32_t i = 0; i < A.instanceCount; i++)
+    if (A.blasIndex[i] == blas) list.push_back(i);
+  if (list.empty()) return;
+  std::vector<float> boxes = A.blasBoxes.to_host((size_t)A.instanceCount * 6, s);
+  for (uint32_t i : list)
+    for (int k = 0; k < 3; k++) { boxes[(size_t)i * 6 + k] = B.layout.rootLo[k]; boxes[(size_t)i * 6 + 3 + k] = B.layout.rootHi[k]; }
+  A.blasBoxes.upload(boxes.data(), boxes.size(), s);
+  DevBuf<uint32_t> dl;
+  dl.upload(list.data(), list.size(), s);
+  hipLaunchKernelGGL(k_instance_update, dim3(blocks(list.size(), 64)), dim3(64), 0, s, A.transforms.ptr,
+                     A.blasBoxes.ptr, dl.ptr, (uint32_t)list.size(), A.slotOfSource.ptr, A.slots.ptr);
+  DevBuf<uint32_t> dStart;
+  dStart.upload(A.levelStart.data(), A.levelStart.size(), s);
+  hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(256), 0, s, A.layout.tnodes.ptr, A.slots.ptr, A.levelNodes.ptr,
+                     dStart.ptr, (uint32_t)(A.levelStart.size() - 1), A.view.ptr);
+  MTX_HIP(hipGetLastError());
+  MTX_HIP(hipStreamSynchronize(s));
 }
 
 static void fill_slot(InstSlot &sl, const double *m, const float box[6]) {

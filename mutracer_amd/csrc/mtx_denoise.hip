@@ -22,7 +22,11 @@
 // count read the half-resolution tensor at (y/2, x/2), the rest read the skip
 // tensor. Bias, ReLU, the optional 2x2 max pool and the conversion to the
 // storage type are fused into the epilogue.
+#include <dlfcn.h>
+
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
 #include "../../include/mtx_fmath.h"
@@ -713,8 +717,8 @@ using namespace mtx;
 
 extern "C" {
 
-int32_t mtx_denoiser_create(const void *weights, size_t size, uint32_t precision, mtx_handle *out, char *err,
-                            size_t cap) {
+int32_t mtx_denoiser_create_from_weights(const void *weights, size_t size, uint32_t precision, mtx_handle *out,
+                                         char *err, size_t cap) {
   try {
     MTX_REQUIRE(weights && size && out, MTX_INVALID_ARGUMENT, "weights/out is NULL");
     MTX_REQUIRE(precision == MTX_DENOISE_FP16 || precision == MTX_DENOISE_BF16, MTX_INVALID_ARGUMENT,
@@ -894,9 +898,75 @@ int32_t mtx_denoiser_run(mtx_handle handle, uint32_t width, uint32_t height, con
   }
 }
 
-int32_t mtx_denoiser_destroy(mtx_handle handle) {
-  std::lock_guard<std::mutex> g(gDenoiseMutex);
-  return gDenoisers.erase(handle) ? MTX_OK : MTX_INVALID_HANDLE;
+// mt_denoiser_create (mutracer-denoise.h:36-37): `weights_path` names a .tza
+// file; NULL uses MTX_DENOISER_WEIGHTS, then the standard network shipped
+// beside the library (data/rt_hdr_alb_nrm.tza).
+int32_t mtx_denoiser_create(const char *weights_path, uint32_t backend, mtx_handle *out, char *err, size_t cap) {
+  try {
+    MTX_REQUIRE(out, MTX_INVALID_ARGUMENT, "Null denoiser output");
+    MTX_REQUIRE(backend <= MTX_DENOISE_BACKEND_HIP, MTX_INVALID_ARGUMENT, "unknown denoise backend");
+    MTX_REQUIRE(backend == MTX_DENOISE_BACKEND_AUTO || backend == MTX_DENOISE_BACKEND_HIP, MTX_UNAVAILABLE,
+                "only the HIP backend is available");
+    std::string path;
+    if (weights_path) {
+      path = weights_path;
+    } else if (const char *env = std::getenv("MTX_DENOISER_WEIGHTS")) {
+      path = env;
+    } else {
+      Dl_info info{};
+      MTX_REQUIRE(dladdr((void *)&mtx_denoiser_create, &info) && info.dli_fname, MTX_UNAVAILABLE,
+                  "cannot locate the library directory");
+      path = info.dli_fname;
+      path = path.substr(0, path.find_last_of('/') + 1) + "data/rt_hdr_alb_nrm.tza";
+    }
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    MTX_REQUIRE(f, MTX_UNAVAILABLE, "cannot open denoiser weights " + path);
+    std::vector<uint8_t> bytes;
+    uint8_t buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) bytes.insert(bytes.end(), buf, buf + n);
+    std::fclose(f);
+    MTX_REQUIRE(!bytes.empty(), MTX_INVALID_ARGUMENT, "empty denoiser weights " + path);
+    return mtx_denoiser_create_from_weights(bytes.data(), bytes.size(), MTX_DENOISE_FP16, out, err, cap);
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(err, cap, e.what());
+    return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_backend(mtx_handle handle, uint32_t *output, char *err, size_t cap) {
+  try {
+    get_denoiser(handle);
+    MTX_REQUIRE(output, MTX_INVALID_ARGUMENT, "Null backend output");
+    *output = MTX_DENOISE_BACKEND_HIP;
+    return MTX_OK;
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_destroy(mtx_handle handle, char *err, size_t cap) {
+  std::shared_ptr<DenoiserObj> D;
+  {
+    std::lock_guard<std::mutex> g(gDenoiseMutex);
+    auto it = gDenoisers.find(handle);
+    if (it == gDenoisers.end()) {
+      set_error(err, cap, "unknown denoiser handle");
+      return MTX_INVALID_HANDLE;
+    }
+    D = it->second;
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) {
+      set_error(err, cap, "the denoiser is running");
+      return MTX_BUSY;
+    }
+    gDenoisers.erase(it);
+  }
+  return MTX_OK;
 }
 
 }  // extern "C"

@@ -103,13 +103,28 @@ struct Layout {
   float rootLo[3] = {0, 0, 0}, rootHi[3] = {0, 0, 0};
 };
 
+// refitBLAS's ratio of refitted to built surface area above which a rebuild
+// is advised (DYNAMIC_BLAS_REBUILD_RATIO, bvhAccel.ts:743).
+constexpr double kDynamicBlasRebuildRatio = 2.0;
+
+// Refit bookkeeping of a BLAS, made on its first refit: nodes by depth, the
+// depth-first TNode index of every node, the build's node surface area.
+struct RefitData {
+  bool ready = false;
+  DevBuf<uint32_t> byDepth, dfs;
+  std::vector<uint32_t> levelStart, levelCount;
+  double buildArea = 0.0;
+};
+
 struct Blas {
   Topology topo;
   Layout layout;
   DevBuf<TriRec> tris;
   DevBuf<uint32_t> triSource;  // leaf slot -> source triangle
+  RefitData refit;
   uint32_t maxLeaf = 0;
   uint32_t triangleInput = 0;
+  uint32_t vertexInput = 0;
   size_t resident_bytes() const {
     return layout.tnodes.bytes() + tris.bytes() + triSource.bytes();
   }
@@ -150,6 +165,12 @@ size_t export_blas(const Blas &B, const uint32_t *indices, uint32_t *words, size
 std::shared_ptr<Accel> build_accel(const std::vector<std::shared_ptr<Blas>> &blases,
                                    const mtx_instance_desc *inst, uint32_t count, uint32_t bins,
                                    uint32_t flags, hipStream_t s);
+// refitBLAS: new vertex positions (device pointers, the build's vertex count
+// and index buffer); returns true when the refitted surface area exceeds
+// kDynamicBlasRebuildRatio times the build's (a rebuild is advised).
+bool refit_blas(Blas &B, const float *pos, const uint32_t *idx, hipStream_t s);
+// After refit_blas: the instances of BLAS `blas` take its new root box and the TLAS is refitted.
+void refit_accel_blas(Accel &A, uint32_t blas, hipStream_t s);
 void refit_accel(Accel &A, const mtx_instance_desc *inst, uint32_t count, const uint32_t *dirty,
                  uint32_t dirtyCount, hipStream_t s);
 void export_tlas(const Accel &A, std::vector<uint32_t> &tlasWords, std::vector<uint32_t> &instanceWords,

@@ -31,7 +31,7 @@ class Denoiser:
         buf = (C.c_ubyte * len(data)).from_buffer_copy(data)
         self.handle = C.c_uint64(0)
         err = abi.errbuf()
-        abi.check(abi.lib().mtx_denoiser_create(C.cast(buf, C.c_void_p), len(data), PRECISIONS[precision],
+        abi.check(abi.lib().mtx_denoiser_create_from_weights(C.cast(buf, C.c_void_p), len(data), PRECISIONS[precision],
                                                 C.byref(self.handle), err, 512), err)
         self.precision = precision
 
@@ -84,7 +84,7 @@ class Denoiser:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value and abi._lib is not None:
-            abi.lib().mtx_denoiser_destroy(h.value)
+            abi.lib().mtx_denoiser_destroy(h.value, None, 0)
 
 
 def _main(argv=None):

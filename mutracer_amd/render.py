@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi_ext import AdaptiveOptions, CameraDesc, MeshDesc, RenderSettings, SceneDesc, WavefrontTuning
+from .abi_ext import AdaptiveOptions, CameraDesc, MeshDesc, TargetSettings, SceneDesc, WavefrontTuning
 from .sampling import sampling_pattern_code
 
 _P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
@@ -82,6 +82,18 @@ class RenderScene:
                                                      err, 512), err)
         self.scene.transforms = [row.copy() for row in t]
 
+    def update_mesh(self, mesh, positions, normals):
+        """Vertex deformation of one mesh (BLAS refit + TLAS refit); returns
+        True when a rebuild is advised."""
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        advised = C.c_uint32(0)
+        err = abi.errbuf()
+        abi.check(abi.lib().mtx_scene_update_mesh(self.handle.value, mesh, _P(pos), _P(nrm), pos.shape[0],
+                                                  C.byref(advised), err, 512), err)
+        self.scene.meshes[mesh] = dict(self.scene.meshes[mesh], positions=pos, normals=nrm)
+        return bool(advised.value)
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value and abi._lib is not None:
@@ -107,8 +119,8 @@ class RenderTarget:
                  kernel="wavefront", max_paths_in_flight=0):
         self.scene = render_scene
         self.width, self.height = width, height
-        s = RenderSettings()
-        s.struct_size = C.sizeof(RenderSettings)
+        s = TargetSettings()
+        s.struct_size = C.sizeof(TargetSettings)
         s.width, s.height, s.bounces = width, height, bounces
         s.direct_light_candidates = direct_light_candidates
         s.sampling_pattern = sampling_pattern_code(pattern) if isinstance(pattern, str) else int(pattern)

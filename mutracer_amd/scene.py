@@ -232,16 +232,25 @@ class Camera:
     focus_distance: float = 1.0
 
     def matrices(self, width, height):
-        """(projection inverse, view inverse) as column-major f32, computed in f64."""
+        """(projection inverse, view inverse) as column-major f32, computed in f64.
+
+        fov, near and far enter as the f32 values a scene descriptor carries,
+        and the perspective matrix (mat4.perspective: f/aspect, f,
+        A = (far+near)/(near-far), B = 2 far near/(near-far), -1) is inverted
+        in closed form, the same arithmetic as the library's descriptor decoder,
+        so a committed descriptor and this camera give the same bits."""
         aspect = width / height
-        f = 1.0 / math.tan(math.radians(self.fov) / 2)
-        nf = 1.0 / (self.near - self.far)
-        P = np.zeros((4, 4))
-        P[0, 0] = f / aspect
-        P[1, 1] = f
-        P[2, 2] = (self.far + self.near) * nf
-        P[2, 3] = 2 * self.far * self.near * nf
-        P[3, 2] = -1.0
+        fov, near, far = (float(np.float32(v)) for v in (self.fov, self.near, self.far))
+        f = 1.0 / math.tan(fov * math.pi / 180.0 / 2.0)
+        nf = 1.0 / (near - far)
+        A = (far + near) * nf
+        B = 2.0 * far * near * nf
+        proj_inv = np.zeros((4, 4))
+        proj_inv[0, 0] = aspect / f
+        proj_inv[1, 1] = 1.0 / f
+        proj_inv[3, 2] = 1.0 / B
+        proj_inv[2, 3] = -1.0
+        proj_inv[3, 3] = A / B
         eye, target, up = (np.asarray(a, np.float64) for a in (self.eye, self.target, self.up))
         z = eye - target
         z /= np.linalg.norm(z)
@@ -250,7 +259,6 @@ class Camera:
         y = np.cross(z, x)
         world = np.eye(4)
         world[:3, 0], world[:3, 1], world[:3, 2], world[:3, 3] = x, y, z, eye
-        proj_inv = np.linalg.inv(P)
         return (proj_inv.T.reshape(16).astype(np.float32), world.T.reshape(16).astype(np.float32))
 
     def pack(self, width, height):

@@ -16,7 +16,7 @@ import time
 import numpy as np
 
 from . import abi
-from .abi_ext import JobInfo, RenderSettings, ServerConfig
+from .abi_ext import JobInfo, TargetSettings, ServerConfig
 from .render import KERNELS, PIXELS, camera_desc
 from .sampling import sampling_pattern_code
 
@@ -36,8 +36,8 @@ class Server:
 
     def submit(self, render_scene, width, height, samples, bounces=8, direct_light_candidates=1, pattern="sobol",
                sequence_seed=0, max_samples=None, sample_sequence_offset=0, clamping=0.0, camera=None):
-        s = RenderSettings()
-        s.struct_size = C.sizeof(RenderSettings)
+        s = TargetSettings()
+        s.struct_size = C.sizeof(TargetSettings)
         s.width, s.height, s.bounces = width, height, bounces
         s.direct_light_candidates = direct_light_candidates
         s.sampling_pattern = sampling_pattern_code(pattern) if isinstance(pattern, str) else int(pattern)

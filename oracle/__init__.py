@@ -263,6 +263,19 @@ def build_blas(positions, indices, bins=8):
     return out
 
 
+def refit_blas(words, positions):
+    """refitBLAS (src/renderer/bvhAccel.ts:800-870) on BVH2 words whose leaf
+    entries hold the triangle indices: new node boxes for `positions` (same
+    vertex count), topology kept. Returns (words, refitted surface area)."""
+    w = np.array(words, np.uint32).reshape(-1)
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    L = lib()
+    L.orc_blas_refit_words.restype = C.c_double
+    L.orc_blas_refit_words.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_uint32]
+    area = L.orc_blas_refit_words(_p(w), w.size // 4, _p(p), p.shape[0])
+    return w, float(area)
+
+
 def _tlas_args(inst):
     bi = np.ascontiguousarray(inst["blas_index"], np.uint32)
     bounds = np.ascontiguousarray(inst["bounds"], np.float32).reshape(-1, 6)

@@ -78,6 +78,10 @@ void *orc_blas_build(const float *positions, uint32_t vertexCount, const uint32_
   return h;
 }
 
+double orc_blas_refit_words(uint32_t *words, size_t vec4Count, const float *positions, uint32_t vertexCount) {
+  return orc::refit_blas_words(words, vec4Count, positions, vertexCount);
+}
+
 void *orc_tlas_build(const uint32_t *blasIndex, const float *bounds, const double *transforms,
                      const uint32_t *materials, const float *colors, uint32_t count,
                      const uint32_t *blasOffsets, const uint32_t *vertexOffsets, uint32_t bins,

@@ -417,3 +417,69 @@ void refit_tlas(FlatBvh &F, const TlasInstance *inst, uint32_t count, const uint
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// refitBLAS (bvhAccel.ts:800-870, without displacement): the BVH2 words'
+// node boxes recomputed bottom-up from new positions, topology kept. Leaf
+// entries are [i0, i1, i2, h]; a leaf's box spans the distinct finite
+// vertices of its triangles (getBlasRefitData, bvhAccel.ts:748-797, collects
+// them once per leaf), an interior node's the union of its two children
+// (left child after the node's two rows, right child at its row index).
+// Returns the refitted node surface area, sum of 2 (xz + xy + zy) in the
+// reference's postorder.
+double refit_blas_words(uint32_t *words, size_t vec4Count, const float *positions, uint32_t vertexCount) {
+  if (!vec4Count) return 0.0;
+  float *f = reinterpret_cast<float *>(words);
+  const int32_t *ints = reinterpret_cast<const int32_t *>(words);
+  std::vector<uint32_t> preorder;
+  std::vector<uint32_t> stack = {0};
+  while (!stack.empty()) {
+    const uint32_t row = stack.back();
+    stack.pop_back();
+    preorder.push_back(row);
+    const int32_t child = ints[row * 4 + 3];
+    if (child >= 0) {
+      stack.push_back((uint32_t)child);
+      stack.push_back(row + 2);
+    }
+  }
+  double area = 0.0;
+  for (size_t k = preorder.size(); k-- > 0;) {
+    const uint32_t row = preorder[k];
+    const size_t base = (size_t)row * 4;
+    const int32_t child = ints[base + 3];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (child >= 0) {
+      const size_t left = base + 8, right = (size_t)child * 4;
+      for (int a = 0; a < 3; a++) {
+        mn[a] = jmin(f[left + a], f[right + a]);
+        mx[a] = jmax(f[left + 4 + a], f[right + 4 + a]);
+      }
+    } else {
+      std::vector<uint32_t> seen;
+      for (int32_t i = 0; i < -child; i++)
+        for (int j = 0; j < 3; j++) {
+          const uint32_t v = words[base + 8 + (size_t)i * 4 + j];
+          if (std::find(seen.begin(), seen.end(), v) == seen.end()) seen.push_back(v);
+        }
+      for (uint32_t v : seen) {
+        if (v >= vertexCount) continue;
+        const double x = positions[(size_t)v * 3], y = positions[(size_t)v * 3 + 1], z = positions[(size_t)v * 3 + 2];
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) continue;
+        mn[0] = jmin(mn[0], x); mn[1] = jmin(mn[1], y); mn[2] = jmin(mn[2], z);
+        mx[0] = jmax(mx[0], x); mx[1] = jmax(mx[1], y); mx[2] = jmax(mx[2], z);
+      }
+    }
+    const double x = (double)(float)mx[0] - (double)(float)mn[0], y = (double)(float)mx[1] - (double)(float)mn[1],
+                 z = (double)(float)mx[2] - (double)(float)mn[2];
+    area += 2 * (x * z + x * y + z * y);
+    for (int a = 0; a < 3; a++) {
+      f[base + a] = (float)mn[a];
+      f[base + 4 + a] = (float)mx[a];
+    }
+  }
+  return area;
+}
+
+}  // namespace orc

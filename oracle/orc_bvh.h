@@ -2,6 +2,7 @@
 // (test infrastructure only).
 #pragma once
 #include <cstdint>
+#include <cstddef>
 #include <vector>
 
 namespace orc {
@@ -34,6 +35,9 @@ struct FlatBvh {
 FlatBvh build_blas(const float *positions, uint32_t vertexCount,
                    const uint32_t *indices, uint32_t triCount,
                    uint32_t bins, std::vector<uint32_t> *sourceTriangleOfSlot);
+
+// refitBLAS on BVH2 words in place; returns the refitted node surface area.
+double refit_blas_words(uint32_t *words, std::size_t vec4Count, const float *positions, uint32_t vertexCount);
 
 // One TLAS input (bvhAccel.ts:61-68 TLASInstance).
 struct TlasInstance {

@@ -117,3 +117,40 @@ def test_denoiser_autotune(net):
     best, before, after = autotune.tune_denoiser(d, 96, 64, iterations=1)
     assert len(best) == 16 and before > 0 and after > 0
     assert np.array_equal(d.forward(x), ref)
+
+
+def test_create_backend_destroy_contract(net):
+    """mt_denoiser_create/backend/destroy semantics (mutracer-denoise.h:9-53):
+    NULL weights load the standard network shipped beside the library; AUTO
+    and HIP run it; explicit other backends are unavailable (never fall back);
+    destroyed handles are invalid; the result equals the weights-bytes form."""
+    import ctypes as C
+    from mutracer_amd import abi
+    denoise, _ = net
+    L = abi.lib()
+    err = abi.errbuf()
+    h = C.c_uint64(0)
+    abi.check(L.mtx_denoiser_create(None, 0, C.byref(h), err, 512), err)
+    b = C.c_uint32(99)
+    abi.check(L.mtx_denoiser_backend(h.value, C.byref(b), err, 512), err)
+    assert b.value == 4
+    for backend, code in ((1, abi.MTX_UNAVAILABLE), (2, abi.MTX_UNAVAILABLE), (3, abi.MTX_UNAVAILABLE),
+                          (5, abi.MTX_INVALID_ARGUMENT)):
+        other = C.c_uint64(0)
+        assert L.mtx_denoiser_create(None, backend, C.byref(other), err, 512) == code
+        assert other.value == 0
+    assert L.mtx_denoiser_create(b"/nonexistent.tza", 0, C.byref(C.c_uint64(0)), err, 512) == abi.MTX_UNAVAILABLE
+    rng = np.random.default_rng(3)
+    H, W = 24, 40
+    color = rng.uniform(0, 2, (H, W, 4)).astype(np.float32)
+    albedo = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    normal = rng.normal(size=(H, W, 4)).astype(np.float32)
+    out = np.zeros_like(color)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    abi.check(L.mtx_denoiser_run(h.value, W, H, P(color), P(normal), P(albedo), P(out), out.size, err, 512), err)
+    ref = denoise.Denoiser("fp16").run(color, normal, albedo)
+    assert np.array_equal(out, ref)
+    abi.check(L.mtx_denoiser_destroy(h.value, err, 512), err)
+    assert L.mtx_denoiser_destroy(h.value, err, 512) == abi.MTX_INVALID_HANDLE
+    assert L.mtx_denoiser_run(h.value, W, H, P(color), P(normal), P(albedo), P(out), out.size, err,
+                              512) == abi.MTX_INVALID_HANDLE

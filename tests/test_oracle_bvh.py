@@ -276,3 +276,61 @@ def test_world_bounds_translation_scale(orc):
     m[12:15] = [1, 2, 3]
     out = orc.instance_world_bounds(m, np.array([0, 0, 0], np.float32), np.array([1, 1, 1], np.float32))
     assert np.array_equal(out, np.array([1, 2, 3, 3, 3, 4], np.float32))
+
+
+def _subtree_bounds(words, positions):
+    """Brute force: each node's box = the bounds of every finite vertex of
+    every triangle below it (independent of the refit's bottom-up order)."""
+    w = np.asarray(words, np.uint32).reshape(-1, 4)
+    code = w[:, 3].view(np.int32)
+    out = {}
+
+    def verts(row):
+        c = code[row]
+        if c < 0:
+            return set(int(v) for v in w[row + 2:row + 2 - c, 0:3].reshape(-1))
+        return verts(row + 2) | verts(int(c))
+
+    stack = [0]
+    while stack:
+        row = stack.pop()
+        vs = [v for v in verts(row) if np.all(np.isfinite(positions[v]))]
+        p = positions[vs].astype(np.float64) if vs else np.zeros((0, 3))
+        out[row] = (p.min(0) if vs else np.full(3, np.inf), p.max(0) if vs else np.full(3, -np.inf))
+        if code[row] >= 0:
+            stack += [row + 2, int(code[row])]
+    return out
+
+
+def test_refit_identity_and_subtree_bounds(orc):
+    """refitBLAS (bvhAccel.ts:800-870): with unchanged positions the refit
+    reproduces the build's words and surface area; after a deformation every
+    node box equals the brute-force bounds of the finite vertices below it
+    (an interior box taken from one child only, or a leaf box that keeps a
+    NaN vertex, fails this)."""
+    from mutracer_amd import workloads as W
+    pos, idx = W.random_triangles(400, seed=7)
+    ref = orc.build_blas(pos, idx, 8)
+    w, area = orc.refit_blas(ref["words"], pos)
+    assert np.array_equal(w, ref["words"])
+    assert area == pytest.approx(ref["build_surface_area"], rel=1e-12)
+    rng = np.random.default_rng(2)
+    moved = (pos * np.array([1.5, 0.7, 1.0], np.float32) + rng.normal(0, 0.02, pos.shape)).astype(np.float32)
+    moved[5] = np.nan
+    w2, area2 = orc.refit_blas(ref["words"], moved)
+    ww = w2.reshape(-1, 4)
+    f = ww.view(np.float32)
+    bounds = _subtree_bounds(w2, moved)
+    for row, (lo, hi) in bounds.items():
+        assert np.array_equal(f[row, 0:3], lo.astype(np.float32)), row
+        assert np.array_equal(f[row + 1, 0:3], hi.astype(np.float32)), row
+    # topology and leaf entries are untouched
+    code = ww[:, 3].view(np.int32)
+    orig = np.asarray(ref["words"], np.uint32).reshape(-1, 4)
+    assert np.array_equal(ww[:, 3], orig[:, 3])
+    leaf_rows = np.zeros(len(ww), bool)
+    for row in bounds:
+        if code[row] < 0:
+            leaf_rows[row + 2:row + 2 - code[row]] = True
+    assert np.array_equal(ww[leaf_rows], orig[leaf_rows])
+    assert area2 > 0
