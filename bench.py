@@ -8,15 +8,19 @@ depth 12, 4 NEE candidates, Owen-scrambled Sobol seed 0. The scene (BVHs,
 materials, environment, FP64 accumulators) is resident in HBM before timing;
 each step clears the accumulators and renders the 16 spp with the wavefront
 kernels (generate, then per event closest-hit trace, shade, shadow trace, then
-accumulate) or, with --kernel megakernel, in one megakernel launch.
+accumulate), replayed from the step's captured hipGraph (the product path),
+or, with --kernel megakernel, in one megakernel launch.
 
-Prints one JSON line: Msamples/s (value), Mrays/s, the roofline of the
-dominant kernel (the closest-hit trace; its algorithmic bytes from the
-instrumented launch's work totals, its GPU time from HIP events around each of
-its launches on the launch stream) and the CPU
-baseline: the oracle (C++ restatement of the same algorithm, OpenMP) timed on
-this host on 32x32-pixel tiles of the same image, whose results are also
-compared with the GPU's accumulators for those pixels.
+Prints one JSON line: Msamples/s (value, wall clock over the timed steps),
+Mrays/s, the roofline of the dominant kernel (the closest-hit trace; its
+algorithmic bytes from the instrumented launch's work totals, its GPU time
+from HIP events around each of its launches on the launch stream, in a
+separate instrumented pass), the CPU baseline (the oracle, a C++ restatement
+of the same algorithm with OpenMP, timed on this host on 32x32-pixel tiles of
+the same image, whose results are also compared with the GPU's accumulators
+for those pixels), the 8M-triangle build / 16M-ray traversal microbench with
+the build's roofline, the U-Net denoiser at 1080p, and the serving stream
+(batched vs one job at a time).
 """
 from __future__ import annotations
 
@@ -52,13 +56,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", "wf_trace_c2_1080p_traffic.txt")
+def profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/, if any."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_step:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -99,16 +103,21 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
 
 
-SERVE_RES_FULL = [(512, 512), (1024, 768), (1920, 1080), (3840, 2160)]
-SERVE_RES = [(w // 4, h // 4) for w, h in SERVE_RES_FULL]  # default: 1/4 per axis keeps the run short
+SERVE_RES = [(512, 512), (1024, 768), (1920, 1080), (3840, 2160)]
 
 
-def serving_benchmark(n_jobs=64, rate=40.0, full=False):
-    """Scaled serving stream: 8 resident instanced-field scenes with 10k-4M
-    triangles (log-uniform, seed 5), Poisson arrivals at `rate` jobs/s (seed 6),
-    resolutions SERVE_RES, 4-64 spp log-uniform, depth 8, 4 NEE candidates.
-    Runs the same stream through the batched server and through the plain
-    one-job-at-a-time form; returns aggregate throughput and latencies."""
+def serving_benchmark(n_jobs=40, rate=40.0, per_sample=False):
+    """The serving stream: 8 resident instanced-field scenes with 10k-4M
+    triangles (log-uniform, PCG64 seed 5), each with a 1024x512 sky+sun
+    environment; Poisson arrivals at `rate` jobs/s (seed 6) truncated to the
+    first `n_jobs` jobs; resolutions uniform over SERVE_RES; 4-256 spp
+    log-uniform; depth 8, 4 NEE candidates; admission by a 200 GB HBM budget.
+    The same stream runs through the batched server (continuous batching: every
+    running job gets a slice of each round's 2^24-path generation) and through
+    one job at a time with all of its samples in one generation
+    (plain_whole_job, the plain GPU path); `speedup` is served / plain_whole_job.
+    With per_sample, also the reference's dispatch model (one job, one sample
+    per launch sequence) as `plain_per_sample`."""
     import math
     from mutracer_amd import render, scene, serve
     rng = np.random.Generator(np.random.PCG64(5))
@@ -116,40 +125,58 @@ def serving_benchmark(n_jobs=64, rate=40.0, full=False):
     scenes = []
     for k, t in enumerate(tris):
         f = max(2, int(round(math.sqrt(t / 1280))))
-        scenes.append(render.RenderScene(scene.instanced_field(n_instances=64, freq=f, env_size=(256, 128), seed=k)))
+        scenes.append(render.RenderScene(scene.instanced_field(n_instances=64, freq=f, env_size=(1024, 512), seed=k)))
     rng = np.random.Generator(np.random.PCG64(6))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
     jobs, samples = [], 0
-    res = SERVE_RES_FULL if full else SERVE_RES
-    spp_max = 256 if full else 64
     for i in range(n_jobs):
-        w, h = res[rng.integers(0, len(res))]
-        spp = int(round(math.exp(rng.uniform(math.log(4), math.log(spp_max)))))
+        w, h = SERVE_RES[rng.integers(0, len(SERVE_RES))]
+        spp = int(round(math.exp(rng.uniform(math.log(4), math.log(256)))))
         jobs.append(dict(render_scene=scenes[rng.integers(0, 8)], width=w, height=h, samples=spp, bounces=8,
                          direct_light_candidates=4))
         samples += w * h * spp
     out = {}
-    # plain: one job at a time, one sample of it per launch sequence (the
-    # reference's execution model: mt_job_poll dispatches one job's samples,
-    # one sample per dispatch); plain_whole_job: one job at a time with all of
-    # its samples in one wavefront generation (a stronger baseline).
-    for mode, kw in (("served", dict(max_jobs_per_round=0, max_samples_per_job_round=16)),
-                     ("plain", dict(max_jobs_per_round=1, max_samples_per_job_round=1)),
-                     ("plain_whole_job", dict(max_jobs_per_round=1, max_samples_per_job_round=1 << 16))):
+    modes = [("served", dict(max_jobs_per_round=0, max_samples_per_job_round=16)),
+             ("plain_whole_job", dict(max_jobs_per_round=1, max_samples_per_job_round=1 << 16))]
+    if per_sample:
+        modes.append(("plain_per_sample", dict(max_jobs_per_round=1, max_samples_per_job_round=1)))
+    for mode, kw in modes:
         server = serve.Server(**kw)
         server.submit(**dict(jobs[0], samples=1))  # warm the kernels and buffers
         while not server.step()[1]:
             pass
         lat, makespan, _ = serve.run_stream(server, jobs, arrivals)
+        st = server.stats()
         out[mode] = dict(msamples_per_s=round(samples / makespan / 1e6, 3), makespan_s=round(makespan, 3),
                          p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
-                         rounds=server.stats()["rounds"])
+                         rounds=st["rounds"])
         log(f"serving {mode}: {out[mode]}")
-    out["speedup"] = round(out["served"]["msamples_per_s"] / out["plain"]["msamples_per_s"], 3)
-    out["speedup_vs_whole_job"] = round(out["served"]["msamples_per_s"] / out["plain_whole_job"]["msamples_per_s"], 3)
-    out["stream"] = (f"{n_jobs} jobs, Poisson {rate}/s (seed 6), 8 scenes {int(tris.min())}-{int(tris.max())} tris, "
-                     f"res {res}, 4-{spp_max} spp, depth 8, 4 candidates, {samples} samples")
+    out["speedup"] = round(out["served"]["msamples_per_s"] / out["plain_whole_job"]["msamples_per_s"], 3)
+    if per_sample:
+        out["speedup_vs_per_sample"] = round(out["served"]["msamples_per_s"] /
+                                             out["plain_per_sample"]["msamples_per_s"], 3)
+    out["stream"] = (f"first {n_jobs} jobs of the Poisson {rate}/s stream (seed 6), 8 scenes "
+                     f"{int(tris.min())}-{int(tris.max())} tris with 1024x512 env, res {SERVE_RES}, 4-256 spp, "
+                     f"depth 8, 4 candidates, {samples} samples")
     return out
+
+
+def split_work(words):
+    """Sum over split (interior) nodes of their primitive counts, from BVH2
+    words (node rows [min.xyz, right or -count], [max.xyz, 0]; a leaf's
+    primitive rows follow it; the left child follows its parent's two rows):
+    every primitive is counted once per interior node on its root path."""
+    w = np.asarray(words, np.uint32).reshape(-1, 4)
+    code = w[:, 3].view(np.int32)
+    frontier, depth, total = np.array([0], np.int64), 0, 0
+    while frontier.size:
+        c = code[frontier]
+        leaf = c < 0
+        total += int((-c[leaf]).astype(np.int64).sum()) * depth
+        inner = frontier[~leaf]
+        frontier = np.concatenate([inner + 2, code[inner].astype(np.int64)])
+        depth += 1
+    return total
 
 
 def microbench(verify_build=False, subset=1 << 20):
@@ -174,9 +201,21 @@ def microbench(verify_build=False, subset=1 << 20):
     build_ms = (time.perf_counter() - t) * 1e3
     acc = accel.Accel([blas], [0], [W.identity_transform()], material=[0], double_sided=[1])
     torch.cuda.synchronize()
+    words, src = blas.export()
+    info = blas.info()
+    split = split_work(words)
+    # B_build: per split node 40 B per primitive for binning (AABB 24 +
+    # centroid 12 + index 4) and 8 B per primitive for the partition, 64 B per
+    # node written, 16 B per input triangle.
+    nodes = info["node_count"]
+    b_build = 48 * split + 64 * nodes + 16 * 8_000_000
     out = {"triangles": 8_000_000, "bins": 32, "build_ms": round(build_ms, 2),
-           "build_mtris_per_s": round(8.0 / (build_ms / 1e3), 2)}
-    log(f"GPU binned-SAH build of 8M triangles: {build_ms:.1f} ms")
+           "build_mtris_per_s": round(8.0 / (build_ms / 1e3), 2),
+           "build_roofline": {"bound": "hbm", "algorithmic_bytes": int(b_build), "split_node_primitives": split,
+                              "nodes": nodes, "achieved": round(b_build / (build_ms / 1e3) / 1e9, 2),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(b_build / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}}
+    log(f"GPU binned-SAH build of 8M triangles: {build_ms:.1f} ms, B_build {b_build / 1e9:.2f} GB")
 
     def trace(rays, label):
         r = torch.from_numpy(rays).cuda()
@@ -207,7 +246,6 @@ def microbench(verify_build=False, subset=1 << 20):
     out["mrays_per_s"] = round((prim.shape[0] + bounce.shape[0]) / total_ms / 1e3, 1)
 
     # Oracle check on the GPU-built BVH (reference-order traversal).
-    words, src = blas.export()
     tw, iw = acc.export_tlas()
     scene = oracle.TraceScene(tw, words, iw, pos, np.ones(1, np.uint8))
     rng = np.random.default_rng(11)
@@ -321,8 +359,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-denoiser", action="store_true")
     ap.add_argument("--denoise-full-precision", action="store_true", help="FP64 oracle on the full 1080p frame")
-    ap.add_argument("--serving-full", action="store_true", help="full-size resolutions and 4-256 spp")
-    ap.add_argument("--serving-jobs", type=int, default=64)
+    ap.add_argument("--serving-jobs", type=int, default=40, help="jobs of the serving stream (its first N arrivals)")
+    ap.add_argument("--serving-per-sample", action="store_true",
+                    help="also run the stream one sample per launch sequence (the reference's dispatch model)")
+    ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--microbench-subset", type=int, default=1 << 18, help="rays per set checked against the oracle")
     ap.add_argument("--kernel", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--max-paths", type=int, default=0, help="wavefront paths in flight (0 = library default)")
     ap.add_argument("--tuning", default="", help="wavefront tuning as key=value,... (see RenderTarget.tuning)")
@@ -338,7 +379,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.microbench:
-        print(json.dumps({"microbench": microbench(args.verify_build)}), flush=True)
+        print(json.dumps({"microbench": microbench(args.verify_build, args.microbench_subset)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the path shards by "
@@ -390,27 +431,35 @@ def main():
     torch.cuda.synchronize()
     log("warmup done")
 
-    # Timed region: every launch bracketed by HIP events on the launch stream
-    # (per kernel class), the steps themselves by the wall clock.
-    classes = []
+    # Timed region: the product path (each step's launch sequence replayed
+    # from its hipGraph), bracketed by synchronizations and the wall clock.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         tgt.restart_samples()
-        classes.append(tgt.render_timed(SPP, stream))
+        tgt.render(SPP, stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    class_ms = {k: float(np.mean([c[k] for c in classes])) for k in classes[0]}
-    kernel_ms = class_ms["trace"] if args.kernel == "wavefront" else class_ms["megakernel"]
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, per kernel class {class_ms}")
     st = tgt.stats()
     assert st["samples"] == SPP
     rays_per_step = st["rays"]
     value = samples / (ms_per_step / 1e3) / 1e6
-    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     sums, counts = tgt.read_accumulators()
     assert np.all(counts == SPP)
+    tuning = tgt.tuning() if args.kernel == "wavefront" else None
+    log(f"timed {args.steps} steps (graph path): {ms_per_step:.2f} ms/step, tuning {tuning}")
+    # Instrumented pass (not the headline): HIP events around every launch on
+    # the launch stream give each kernel class's GPU time.
+    classes = []
+    for k in range(max(2, min(args.steps, 5))):
+        tgt.restart_samples()
+        classes.append(tgt.render_timed(SPP, stream))
+    class_ms = {k: float(np.mean([c[k] for c in classes])) for k in classes[0]}
+    kernel_ms = class_ms["trace"] if args.kernel == "wavefront" else class_ms["megakernel"]
+    eventful_ms = float(np.mean([sum(c.values()) for c in classes]))
+    log(f"instrumented steps: per kernel class {class_ms} (sum {eventful_ms:.2f} ms)")
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
 
     cpu, parity = None, None
     if not args.no_cpu_baseline and args.cpu_seconds > 0:
@@ -422,12 +471,18 @@ def main():
         log("denoiser: U-Net at 1080p, FP16 and BF16")
         denoiser = denoiser_benchmark(args.denoise_full_precision)
 
+    micro = None
+    if not args.no_microbench:
+        log("traversal/build microbench: 8M triangles, 16M rays")
+        micro = microbench(subset=args.microbench_subset)
+
     serving = None
     if not args.no_serving:
         log("serving stream: batched vs one job at a time")
-        serving = serving_benchmark(n_jobs=args.serving_jobs, full=args.serving_full)
+        serving = serving_benchmark(n_jobs=args.serving_jobs, per_sample=args.serving_per_sample)
 
-    traffic = traffic_from_profile()
+    traffic = profile_value("wf_trace_c2_1080p_traffic.txt", "traffic_bytes_per_step")
+    valu = profile_value("wf_c2_1080p_valu_issue.txt", "k_wf_trace_valu_issue_frac")
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -437,16 +492,25 @@ def main():
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "depth": DEPTH,
                    "direct_light_candidates": CANDIDATES, "sampling": "sobol seed 0",
                    "triangles": info["triangles"], "accumulation": "fp64", "kernel": args.kernel,
-                   "tuning": tgt.tuning() if args.kernel == "wavefront" else None},
+                   "tuning": tuning, "timed_path": "hipGraph replay"},
         "mrays_per_s": round(rays_per_step / (ms_per_step / 1e3) / 1e6, 3),
         "rays_per_step": rays_per_step,
         "scene_commit_ms": round(build_ms, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": KERNEL_NAMES[args.kernel], "kernel_ms": round(kernel_ms, 3),
-                     "algorithmic_bytes_per_step": int(alg_bytes), "work_per_step": work,
-                     "kernel_class_ms_per_step": {k: round(v, 3) for k, v in class_ms.items()}},
+                     "algorithmic_bytes_per_step": int(alg_bytes),
+                     # The roof that binds this kernel is vector instruction issue, not HBM: the
+                     # algorithmic bytes mostly hit in L2; measured traffic and issue rate follow.
+                     "binding_roof": "valu_issue",
+                     "valu_issue_frac": valu,
+                     "hbm_traffic_frac": (round(traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                                          if traffic else None),
+                     "work_per_step": work,
+                     "kernel_class_ms_per_step": {k: round(v, 3) for k, v in class_ms.items()},
+                     "instrumented_step_ms": round(eventful_ms, 3)},
         "cpu_baseline": cpu,
+        "microbench": micro,
         "serving": serving,
         "denoiser": denoiser,
         "parity_relmse_cpu_tiles": parity,
