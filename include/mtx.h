@@ -407,6 +407,62 @@ int32_t mtx_target_set_sample_offset(mtx_handle target, uint32_t sample_sequence
 int32_t mtx_target_reset(mtx_handle target);
 int32_t mtx_target_destroy(mtx_handle target);
 
+/* ------------------------------------------------------- display and JNLM */
+/* Display composition of a render target's resolved HDR image (compose.frag
+ * .wgsl.ts:100-142 without bloom, flat backgrounds and shadow catchers):
+ * exposure, gradeColor (contrast, saturation, temperature, tint), vignette,
+ * tone mapping (TONE_MAPPING_MODES: AgX 0, ACES 1, Cineon 2, Reinhard 3,
+ * linear 4), then sRGB (or gamma 2.2) encoding. Defaults of
+ * src/settings/display.ts:12-18. */
+#define MTX_TONE_AGX 0u
+#define MTX_TONE_ACES 1u
+#define MTX_TONE_CINEON 2u
+#define MTX_TONE_REINHARD 3u
+#define MTX_TONE_LINEAR 4u
+typedef struct mtx_display_settings {
+  uint32_t struct_size;
+  uint32_t tone_mapping;  /* MTX_TONE_* (default AgX) */
+  float exposure;         /* default 1 */
+  float contrast;         /* default 1 */
+  float saturation;       /* default 1 */
+  float temperature;      /* default 0 */
+  float tint;             /* default 0 */
+  float vignette;         /* default 0 */
+  uint32_t gamma22;       /* 0: sRGB encoding (default), 1: gamma 2.2 */
+} mtx_display_settings;
+int32_t mtx_display_settings_default(mtx_display_settings *output);
+
+/* The displayed image, RGBA8 (RGB encoded, A = coverage), top row first:
+ * width*height*4 bytes; rgba8 == NULL queries the size through *written. */
+int32_t mtx_target_compose(mtx_handle target, const mtx_display_settings *display, uint8_t *rgba8, size_t capacity,
+                           size_t *written);
+
+/* Joint non-local-means denoiser (src/renderer/jnlm.ts, wgsl/jnlm.compute.ts):
+ * the per-pixel luminance mean and variance of the mean from the FP64
+ * moments, a 3x3 variance smoothing pass, then `iterations` filter passes of
+ * the composed display-linear image with doubling dilation, guided by the
+ * albedo (relative contrast, coverage) and encoded normals, with the bounded
+ * albedo regression; the result is sRGB-encoded (jnlmEncodeWGSL). Defaults
+ * of jnlm.ts:23-46 (strength 1, search radius 5, patch radius 1, albedo
+ * bandwidth 0.4, normal bandwidth 0.05, 2 iterations). */
+typedef struct mtx_jnlm_options {
+  uint32_t struct_size;
+  float strength;            /* > 0 */
+  uint32_t search_radius;    /* 1..16 */
+  uint32_t patch_radius;     /* 0..3 */
+  float albedo_bandwidth;    /* 0.0001..10 */
+  float normal_bandwidth;    /* 0.0001..10 */
+  uint32_t iterations;       /* 1..4 */
+} mtx_jnlm_options;
+int32_t mtx_jnlm_options_default(mtx_jnlm_options *output);
+/* Filters the target's composed image. linear_rgba (optional, width*height*4
+ * f32, top row first) receives the filtered display-linear RGBA; rgba8
+ * (optional, width*height*4 bytes) the encoded result; `pixels` is the pixel
+ * capacity of each given buffer. */
+int32_t mtx_target_denoise_jnlm(mtx_handle target, const mtx_display_settings *display,
+                                const mtx_jnlm_options *options, float *linear_rgba, uint8_t *rgba8,
+                                size_t pixels, char *error, size_t error_capacity);
+
 /* ------------------------------------------------------------ context API */
 /* Persistent contexts over the raw scene descriptor (versions 5..8,
  * native/docs/scene-format.md), with the semantics of mutracer-context.h:12-187:

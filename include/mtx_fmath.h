@@ -19,6 +19,7 @@
  *   atan     table of atan(j/8), j = 0..8, and the addition formula
  *            atan(a) = atan(c) + atan((a - c) / (1 + a c)), |reduced| <= 1/16.
  *   pow      exp(y log x) for x > 0 (C99 special cases otherwise).
+ *   log2     log x times the f64 rounding of 1 / ln 2.
  *   acos     atan2(sqrt((1 - x)(1 + x)), x).
  * The split constants below are the leading 25 significant bits of the
  * constant, then the next 25, then the f64 rounding of the remainder.
@@ -218,6 +219,16 @@ MTX_FM_CALL float powf(float xf, float yf) {
     return (big == (y > 0.0)) ? ffrom(0x7f800000u) : 0.0f;
   }
   return (float)(sign * exp_d(y * log_d(x)));
+}
+
+/* log2 = log(x) / ln 2 with the C99 special cases (log2(0) = -inf,
+ * log2(x < 0) = NaN, log2(+inf) = +inf). */
+MTX_FM_CALL float log2f(float xf) {
+  double x = xf;
+  if (is_nan(x) || x < 0.0) return ffrom(0x7fc00000u);
+  if (x == 0.0) return ffrom(0xff800000u);
+  if (is_inf(x)) return ffrom(0x7f800000u);
+  return (float)(log_d(x) * kInvLn2);
 }
 
 /* atan for a >= 0 (finite or +inf). */

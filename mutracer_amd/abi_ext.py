@@ -111,6 +111,10 @@ def declare(L):
         "mtx_server_stats": (i32, [u64, vp]),
         "mtx_server_destroy": (i32, [u64]),
         "mtx_render_settings_default": (i32, [vp]),
+        "mtx_display_settings_default": (i32, [vp]),
+        "mtx_jnlm_options_default": (i32, [vp]),
+        "mtx_target_compose": (i32, [u64, vp, vp, sz, P(sz)]),
+        "mtx_target_denoise_jnlm": (i32, [u64, vp, vp, vp, vp, sz, C.c_char_p, sz]),
         "mtx_context_create": (i32, [P(u64), C.c_char_p, sz]),
         "mtx_context_destroy": (i32, [u64, C.c_char_p, sz]),
         "mtx_context_commit": (i32, [u64, vp, sz, C.c_char_p, sz]),
@@ -141,3 +145,15 @@ def declare(L):
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+
+
+class DisplaySettings(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("tone_mapping", C.c_uint32), ("exposure", C.c_float),
+                ("contrast", C.c_float), ("saturation", C.c_float), ("temperature", C.c_float),
+                ("tint", C.c_float), ("vignette", C.c_float), ("gamma22", C.c_uint32)]
+
+
+class JnlmOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("strength", C.c_float), ("search_radius", C.c_uint32),
+                ("patch_radius", C.c_uint32), ("albedo_bandwidth", C.c_float), ("normal_bandwidth", C.c_float),
+                ("iterations", C.c_uint32)]

@@ -2,6 +2,7 @@
 // accumulation), plus the host-side light tables: the Walker alias table over
 // {environment, rect lights} (lightSampling.ts:72-107,172-212,415-431) and the
 // environment CDF with its 3x3 max-dilated luminance (envMapDistribution.ts:57-134).
+#include "mtx_display.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -51,6 +52,7 @@ struct TargetObj {
   WfConfig wfConfig;
   WfGraph graph;
   DevBuf<float> targets;  // per-pixel sample limits (adaptive sampling)
+  JnlmBuffers jnlm;       // work images of the JNLM denoiser
   bool useTargets = false;
   hipStream_t stream = nullptr;
   std::mutex busy;
@@ -886,6 +888,128 @@ int32_t mtx_target_plan_adaptive(mtx_handle target, const mtx_adaptive_options *
   T->useTargets = true;
   return MTX_OK;
   API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_display_settings_default(mtx_display_settings *s) {
+  if (!s) return MTX_INVALID_ARGUMENT;
+  std::memset(s, 0, sizeof *s);
+  s->struct_size = sizeof *s;
+  s->tone_mapping = MTX_TONE_AGX;
+  s->exposure = 1.0f;
+  s->contrast = 1.0f;
+  s->saturation = 1.0f;
+  return MTX_OK;
+}
+
+int32_t mtx_jnlm_options_default(mtx_jnlm_options *o) {
+  if (!o) return MTX_INVALID_ARGUMENT;
+  std::memset(o, 0, sizeof *o);
+  o->struct_size = sizeof *o;
+  o->strength = 1.0f;
+  o->search_radius = 5;
+  o->patch_radius = 1;
+  o->albedo_bandwidth = 0.4f;
+  o->normal_bandwidth = 0.05f;
+  o->iterations = 2;
+  return MTX_OK;
+}
+
+}  // extern "C"
+
+namespace mtx {
+
+static DisplayParams display_params(const mtx_display_settings *d) {
+  mtx_display_settings def;
+  mtx_display_settings_default(&def);
+  if (!d) d = &def;
+  MTX_REQUIRE(d->struct_size == sizeof(mtx_display_settings), MTX_INVALID_ARGUMENT,
+              "mtx_display_settings.struct_size must be sizeof(mtx_display_settings)");
+  MTX_REQUIRE(d->tone_mapping <= MTX_TONE_LINEAR, MTX_INVALID_ARGUMENT, "unknown tone mapping");
+  for (float v : {d->exposure, d->contrast, d->saturation, d->temperature, d->tint, d->vignette})
+    MTX_REQUIRE(std::isfinite(v), MTX_INVALID_ARGUMENT, "display parameters must be finite");
+  return DisplayParams{d->tone_mapping, d->exposure, d->contrast, d->saturation, d->temperature, d->tint, d->vignette,
+                       d->gamma22 ? 1u : 0u};
+}
+
+// A WGSL constant written as (value).toFixed(6), parsed as f32.
+static float fixed6(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.6f", v);
+  return std::strtof(buf, nullptr);
+}
+
+static JnlmParams jnlm_params(const mtx_jnlm_options *o, uint32_t W, uint32_t H) {
+  mtx_jnlm_options def;
+  mtx_jnlm_options_default(&def);
+  if (!o) o = &def;
+  MTX_REQUIRE(o->struct_size == sizeof(mtx_jnlm_options), MTX_INVALID_ARGUMENT,
+              "mtx_jnlm_options.struct_size must be sizeof(mtx_jnlm_options)");
+  MTX_REQUIRE(std::isfinite(o->strength) && o->strength > 0.0f, MTX_INVALID_ARGUMENT, "strength must be finite and positive");
+  MTX_REQUIRE(o->search_radius >= 1 && o->search_radius <= 16, MTX_INVALID_ARGUMENT, "searchRadius must be 1..16");
+  MTX_REQUIRE(o->patch_radius <= 3, MTX_INVALID_ARGUMENT, "patchRadius must be 0..3");
+  MTX_REQUIRE(o->iterations >= 1 && o->iterations <= 4, MTX_INVALID_ARGUMENT, "iterations must be 1..4");
+  for (float b : {o->albedo_bandwidth, o->normal_bandwidth})
+    MTX_REQUIRE(std::isfinite(b) && b >= 0.0001f && b <= 10.0f, MTX_INVALID_ARGUMENT, "bandwidths must be 0.0001..10");
+  JnlmParams J{};
+  J.width = W;
+  J.height = H;
+  J.colorSensitivity = (float)(0.6 * std::max((double)o->strength, 1e-3));
+  J.searchRadius = o->search_radius;
+  J.patchRadius = o->patch_radius;
+  J.iterations = o->iterations;
+  const double ab = o->albedo_bandwidth, nb = o->normal_bandwidth;
+  J.invAlbedoBandwidth2 = fixed6(1.0 / (ab * ab));
+  J.invCoverageBandwidth2 = fixed6(1.0 / (0.1 * 0.1));  // JNLM_COVERAGE_BANDWIDTH
+  J.invNormalBandwidth2 = fixed6(1.0 / (nb * nb));
+  return J;
+}
+
+}  // namespace mtx
+
+extern "C" {
+
+int32_t mtx_target_compose(mtx_handle target, const mtx_display_settings *display, uint8_t *rgba8, size_t capacity,
+                           size_t *written) {
+  API_TRY
+  auto T = get_target(target);
+  const uint32_t W = T->params.width, H = T->params.height;
+  const size_t bytes = (size_t)W * H * 4;
+  if (written) *written = bytes;
+  if (!rgba8) return MTX_OK;
+  if (capacity < bytes) return MTX_BUFFER_TOO_SMALL;
+  const DisplayParams P = display_params(display);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  DevBuf<uint8_t> out(bytes);
+  display_compose(T->planes.ptr, W, H, P, out.ptr, nullptr, nullptr, nullptr, T->stream);
+  out.download(rgba8, bytes, T->stream);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_target_denoise_jnlm(mtx_handle target, const mtx_display_settings *display, const mtx_jnlm_options *options,
+                                float *linear_rgba, uint8_t *rgba8, size_t pixels, char *err, size_t cap) {
+  API_TRY
+  auto T = get_target(target);
+  const uint32_t W = T->params.width, H = T->params.height;
+  const size_t N = (size_t)W * H;
+  MTX_REQUIRE(linear_rgba || rgba8, MTX_INVALID_ARGUMENT, "no output buffer");
+  if (pixels < N) return MTX_BUFFER_TOO_SMALL;
+  const DisplayParams P = display_params(display);
+  const JnlmParams J = jnlm_params(options, W, H);
+  std::unique_lock<std::mutex> lk(T->busy, std::try_to_lock);
+  if (!lk.owns_lock()) return MTX_BUSY;
+  DevBuf<float4> lin;
+  DevBuf<uint8_t> enc;
+  if (linear_rgba) lin.alloc(N);
+  if (rgba8) enc.alloc(N * 4);
+  jnlm_denoise(T->planes.ptr, W, H, P, J, T->jnlm, lin.ptr, enc.ptr, T->stream);
+  if (linear_rgba) lin.download(reinterpret_cast<float4 *>(linear_rgba), N, T->stream);
+  if (rgba8) enc.download(rgba8, N * 4, T->stream);
+  MTX_HIP(hipStreamSynchronize(T->stream));
+  return MTX_OK;
+  API_CATCH(err, cap)
 }
 
 int32_t mtx_target_destroy(mtx_handle target) {

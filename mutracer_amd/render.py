@@ -17,7 +17,8 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi_ext import AdaptiveOptions, CameraDesc, MeshDesc, TargetSettings, SceneDesc, WavefrontTuning
+from .abi_ext import (AdaptiveOptions, CameraDesc, DisplaySettings, JnlmOptions, MeshDesc, SceneDesc,
+                      TargetSettings, WavefrontTuning)
 from .sampling import sampling_pattern_code
 
 _P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
@@ -98,6 +99,20 @@ class RenderScene:
         h = getattr(self, "handle", None)
         if h is not None and h.value and abi._lib is not None:
             abi.lib().mtx_scene_destroy(h.value)
+
+
+TONE_MAPPINGS = dict(agx=0, aces=1, cineon=2, reinhard=3, linear=4)
+
+
+def display_settings(tone_mapping="agx", gamma22=False, **grading):
+    """mtx_display_settings from the library defaults (src/settings/display.ts:12-18)."""
+    d = DisplaySettings()
+    abi.check(abi.lib().mtx_display_settings_default(C.byref(d)))
+    d.tone_mapping = TONE_MAPPINGS[tone_mapping]
+    d.gamma22 = int(bool(gamma22))
+    for k, v in grading.items():
+        setattr(d, k, v)
+    return d
 
 
 def camera_desc(camera, width, height):
@@ -253,6 +268,31 @@ class RenderTarget:
         abi.check(abi.lib().mtx_target_read_pixels(self.handle.value, PIXELS[kind], _P(out), out.size,
                                                    C.byref(written)))
         return out
+
+    def compose(self, tone_mapping="agx", gamma22=False, **grading):
+        """Displayed RGBA8 (H, W, 4), top row first (exposure, contrast,
+        saturation, temperature, tint, vignette as keywords)."""
+        d = display_settings(tone_mapping, gamma22, **grading)
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        written = C.c_size_t(0)
+        abi.check(abi.lib().mtx_target_compose(self.handle.value, C.byref(d), _P(out), out.size, C.byref(written)))
+        return out
+
+    def denoise_jnlm(self, tone_mapping="agx", gamma22=False, grading=None, **options):
+        """JNLM of the composed image: (display-linear RGBA f32, RGBA8), both
+        (H, W, 4) top row first. Options: strength, search_radius,
+        patch_radius, albedo_bandwidth, normal_bandwidth, iterations."""
+        d = display_settings(tone_mapping, gamma22, **(grading or {}))
+        o = JnlmOptions()
+        abi.check(abi.lib().mtx_jnlm_options_default(C.byref(o)))
+        for k, v in options.items():
+            setattr(o, k, v)
+        lin = np.zeros((self.height, self.width, 4), np.float32)
+        enc = np.zeros((self.height, self.width, 4), np.uint8)
+        err = abi.errbuf()
+        abi.check(abi.lib().mtx_target_denoise_jnlm(self.handle.value, C.byref(d), C.byref(o), _P(lin), _P(enc),
+                                                    self.width * self.height, err, 512), err)
+        return lin, enc
 
     def read_accumulators(self):
         """(sums (H, W, 13) f64 bottom row first, counts (H, W) u32)."""

@@ -276,6 +276,68 @@ def refit_blas(words, positions):
     return w, float(area)
 
 
+def _display6(display):
+    d = dict(exposure=1.0, contrast=1.0, saturation=1.0, temperature=0.0, tint=0.0, vignette=0.0)
+    d.update({k: v for k, v in (display or {}).items() if k in d})
+    return np.array([d[k] for k in ("exposure", "contrast", "saturation", "temperature", "tint", "vignette")],
+                    np.float32)
+
+
+TONE = dict(agx=0, aces=1, cineon=2, reinhard=3, linear=4)
+
+
+def compose(sums, display=None):
+    """Displayed RGBA8 (H, W, 4), top row first, from FP64 accumulators
+    (H, W, 13), bottom row first (orc_display.cpp)."""
+    display = display or {}
+    h, w, _ = sums.shape
+    planes = np.ascontiguousarray(np.moveaxis(sums, 2, 0).reshape(13, -1), np.float64)
+    out = np.zeros((h, w, 4), np.uint8)
+    d6 = _display6(display)
+    L = lib()
+    L.orc_compose.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.orc_compose(_p(planes), w, h, _p(d6), TONE[display.get("tone_mapping", "agx")], int(display.get("gamma22", 0)),
+                  _p(out))
+    return out
+
+
+def jnlm(sums, display=None, strength=1.0, search_radius=5, patch_radius=1, albedo_bandwidth=0.4,
+         normal_bandwidth=0.05, iterations=2):
+    """JNLM of the composed image (orc_display.cpp): (filtered display-linear
+    RGBA f32, RGBA8), both (H, W, 4) top row first."""
+    display = display or {}
+    h, w, _ = sums.shape
+    planes = np.ascontiguousarray(np.moveaxis(sums, 2, 0).reshape(13, -1), np.float64)
+    lin = np.zeros((h, w, 4), np.float32)
+    enc = np.zeros((h, w, 4), np.uint8)
+    opts = np.array([strength, albedo_bandwidth, normal_bandwidth], np.float32)
+    shape = np.array([search_radius, patch_radius, iterations], np.uint32)
+    L = lib()
+    L.orc_jnlm.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                           C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_jnlm(_p(planes), w, h, _p(_display6(display)), TONE[display.get("tone_mapping", "agx")],
+               int(display.get("gamma22", 0)), _p(opts), _p(shape), _p(lin), _p(enc))
+    return lin, enc
+
+
+def tonemap(rgb, mode):
+    x = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros_like(x)
+    L = lib()
+    L.orc_tonemap.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.orc_tonemap(_p(x), x.shape[0], TONE[mode], _p(out))
+    return out
+
+
+def srgb_encode(rgb):
+    x = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros_like(x)
+    L = lib()
+    L.orc_srgb_encode.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    L.orc_srgb_encode(_p(x), x.shape[0], _p(out))
+    return out
+
+
 def _tlas_args(inst):
     bi = np.ascontiguousarray(inst["blas_index"], np.uint32)
     bounds = np.ascontiguousarray(inst["bounds"], np.float32).reshape(-1, 6)
