@@ -155,6 +155,19 @@ typedef struct mtx_hit {
 int32_t mtx_trace_closest(mtx_handle accel, const mtx_ray *rays, uint32_t count, mtx_hit *hits,
                           uint32_t flags, void *stream);
 
+/* mtx_trace_closest without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * traversal stacks' spill slices and the work counters live in a static
+ * device array of 72 MiB + 256 B (1024 persistent blocks x 128 lanes x 144
+ * stack entries) that every device holds from library load until the process
+ * ends; its device address is looked up once per device on the first call.
+ * Same results as mtx_trace_closest, bit for bit. At most 2^30 rays per call
+ * (MTX_INVALID_ARGUMENT beyond). Every call on one device shares that array:
+ * calls that overlap on one GPU (different streams, other threads) race. */
+int32_t mtx_trace_closest_noalloc(mtx_handle accel, const mtx_ray *rays, uint32_t count, mtx_hit *hits,
+                                  uint32_t flags, void *stream);
+
 /* intersectSceneShadow (intersect.wgsl.ts:829-879): occluded[i] = 1 when an
  * opaque hit lies within [0, min(max(tmax,0), 9999)]. Device pointers. */
 int32_t mtx_trace_any(mtx_handle accel, const mtx_ray *rays, uint32_t count, uint32_t *occluded,

@@ -84,6 +84,7 @@ def _declare(L):
         "mtx_accel_export_tlas": (i32, [u64, vp, sz, P(sz), vp, sz, P(sz)]),
         "mtx_accel_destroy": (i32, [u64]),
         "mtx_trace_closest": (i32, [u64, vp, u32, vp, u32, vp]),
+        "mtx_trace_closest_noalloc": (i32, [u64, vp, u32, vp, u32, vp]),
         "mtx_trace_any": (i32, [u64, vp, u32, vp, vp]),
         "mtx_trace_count": (i32, [u64, vp, u32, vp, u32, vp]),
     }

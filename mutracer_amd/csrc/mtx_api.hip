@@ -151,6 +151,59 @@ static void launch_trace(int mode, const Accel &A, const mtx_ray *rays, uint32_t
   MTX_HIP(hipGetLastError());
 }
 
+// ----------------------------------------------- allocation-free ray query
+// mtx_trace_closest_noalloc runs k_trace<0> on a fixed persistent grid of
+// kNoAllocBlocks blocks whose traversal-stack spill slices and chunk counters
+// live in `scratch`, a file-scope device array: each device holds its own copy
+// from module load until the process ends, and nothing is allocated per call.
+constexpr uint32_t kNoAllocBlocks = 1024;
+constexpr size_t kNoAllocSpillInts = (size_t)kNoAllocBlocks * kTraceBlock * (kMaxStack - kMinLdsStack);
+constexpr size_t kNoAllocScratchBytes = kNoAllocSpillInts * sizeof(int32_t) + 256;  // + kXcds chunk counters
+constexpr uint32_t kNoAllocMaxRays = 1u << 30;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device, looked up once with that device
+// current (the host-side symbol is only a stand-in, not a device address).
+static std::mutex gScratchMutex;
+static std::map<int, uint8_t *> gScratchPtr;
+
+static uint8_t *scratch_for_current_device() {
+  int dev = 0;
+  MTX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(gScratchMutex);
+  auto it = gScratchPtr.find(dev);
+  if (it != gScratchPtr.end()) return it->second;
+  void *p = nullptr;
+  MTX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  MTX_REQUIRE(p != nullptr, MTX_CONTEXT_FAILED, "scratch symbol has no device address");
+  gScratchPtr[dev] = (uint8_t *)p;
+  return (uint8_t *)p;
+}
+
+static void launch_trace_noalloc(const Accel &A, const mtx_ray *rays, uint32_t n, mtx_hit *hits, uint32_t flags,
+                                 hipStream_t s) {
+  MTX_REQUIRE(A.max_stack() <= (uint32_t)kMaxStack, MTX_INVALID_ARGUMENT, "scene deeper than the traversal stack");
+  MTX_REQUIRE(n <= kNoAllocMaxRays, MTX_INVALID_ARGUMENT, "more than 2^30 rays in one allocation-free query");
+  uint8_t *base = scratch_for_current_device();
+  if (!n) return;
+  int32_t *spill = reinterpret_cast<int32_t *>(base);
+  uint32_t *ctr = reinterpret_cast<uint32_t *>(base + kNoAllocSpillInts * sizeof(int32_t));
+  static uint32_t bpc = 0;
+  if (!bpc) {
+    int b = 0;
+    MTX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_trace<0>, kTraceBlock, 0));
+    bpc = (uint32_t)std::max(1, b);
+  }
+  uint32_t grid = std::min<uint32_t>(trace_grid() / 8u * bpc, kNoAllocBlocks);
+  const uint32_t rayBlocks = (n + kTraceBlock - 1) / kTraceBlock;
+  grid = std::min<uint32_t>(grid, ((rayBlocks + 7u) / 8u) * 8u);
+  grid = std::max<uint32_t>(8u, grid & ~7u);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, ctr);
+  hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, A.view.ptr, rays, n, hits, nullptr, flags, spill,
+                     ctr);
+  MTX_HIP(hipGetLastError());
+}
+
 }  // namespace mtx
 
 using namespace mtx;
@@ -327,6 +380,16 @@ int32_t mtx_trace_closest(mtx_handle accel, const mtx_ray *rays, uint32_t count,
   auto A = registry().get_accel(accel);
   MTX_REQUIRE(count == 0 || (rays && hits), MTX_INVALID_ARGUMENT, "rays/hits are NULL");
   launch_trace(0, *A, rays, count, hits, nullptr, flags, (hipStream_t)stream);
+  return MTX_OK;
+  MTX_API_END(nullptr, 0)
+}
+
+int32_t mtx_trace_closest_noalloc(mtx_handle accel, const mtx_ray *rays, uint32_t count, mtx_hit *hits,
+                                  uint32_t flags, void *stream) {
+  MTX_API_BEGIN
+  auto A = registry().get_accel(accel);
+  MTX_REQUIRE(count == 0 || (rays && hits), MTX_INVALID_ARGUMENT, "rays/hits are NULL");
+  launch_trace_noalloc(*A, rays, count, hits, flags, (hipStream_t)stream);
   return MTX_OK;
   MTX_API_END(nullptr, 0)
 }
