@@ -207,7 +207,61 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 #define MTX_MAT_SUBSURFACE_WEIGHT 33
 #define MTX_MAT_SUBSURFACE_COLOR 34   /* 3 floats */
 #define MTX_MAT_THIN_WALLED 37        /* > 0.5: thin-walled surface */
-#define MTX_MATERIAL_FLOATS 48        /* floats 38..47 are reserved (zero) */
+#define MTX_MAT_OPACITY 38            /* alpha (materialParams.ts:214) */
+#define MTX_MAT_ALPHA_MODE 39         /* resolved mode: 0 opaque, 1 mask, 2 blend (materialConstants.ts:12-16,
+                                         resolveMaterialAlphaMode, MaterialState.ts:251-264); a scene whose
+                                         material is not opaque is rejected with MTX_UNAVAILABLE */
+#define MTX_MAT_ALPHA_CUTOFF 40
+#define MTX_MAT_MAP_ALPHA_USAGE 41    /* base map alpha: 0 none, 2 alpha, 3 transmission = 1 - a
+                                         (mapAlphaUsageConstants, materialConstants.ts:2) */
+#define MTX_MAT_NORMAL_SCALE 42       /* 2 floats: normal map xy scale */
+#define MTX_MAT_UV_SCALE 44           /* 2 floats: UV0/UV1 multiplier of surface hits (uvProjectionScale,
+                                         materialParams.ts:202; default 1, 1) */
+#define MTX_MAT_MAPS 46               /* MTX_MAP_COUNT floats: scene texture index + 1 per slot, 0 = none */
+#define MTX_MATERIAL_FLOATS 64        /* floats 60..63 are reserved (zero) */
+
+/* Texture slots of a material, in the order their samples apply
+ * (materialBuffer.wgsl.ts:78-160, slot table materialTextureSlots.ts:8-140).
+ * Color slots (base, emissive, fuzz color, specular color) decode RGBA8
+ * texels from sRGB; RGBA32F texels and the other slots are linear. */
+#define MTX_MAP_BASE 0                /* baseColor *= rgb; a per MTX_MAT_MAP_ALPHA_USAGE */
+#define MTX_MAP_ALPHA 1               /* alpha *= g */
+#define MTX_MAP_TRANSMISSION 2        /* transmission weight *= r */
+#define MTX_MAP_METALLIC_ROUGHNESS 3  /* metalness *= b, specular roughness *= g */
+#define MTX_MAP_ROUGHNESS 4           /* specular roughness *= g */
+#define MTX_MAP_METALNESS 5           /* metalness *= b */
+#define MTX_MAP_COAT_WEIGHT 6         /* coat weight *= r (emission tint follows it) */
+#define MTX_MAP_COAT_ROUGHNESS 7      /* coat roughness *= g */
+#define MTX_MAP_FUZZ_COLOR 8          /* fuzz color *= rgb */
+#define MTX_MAP_FUZZ_ROUGHNESS 9      /* fuzz roughness *= a */
+#define MTX_MAP_SPECULAR_WEIGHT 10    /* specular weight *= a */
+#define MTX_MAP_SPECULAR_COLOR 11     /* specular color *= rgb */
+#define MTX_MAP_EMISSIVE 12           /* emission *= rgb */
+#define MTX_MAP_NORMAL 13             /* tangent-space normal (cotangent frame of UV0) */
+#define MTX_MAP_COUNT 14
+
+/* Texture storage and wrap modes (mutracer-scene.h:26-27). */
+#define MTX_TEXTURE_RGBA8_SRGB 0u     /* one word per pixel, R in the low byte */
+#define MTX_TEXTURE_RGBA32F 1u        /* four floats per pixel */
+#define MTX_WRAP_REPEAT 0u
+#define MTX_WRAP_CLAMP_TO_EDGE 1u
+#define MTX_WRAP_MIRRORED_REPEAT 2u
+
+/* A material texture (mt_texture_v4, mutracer-scene.h:87-95, without UDIM):
+ * row-major pixels, row 0 first (flip_y reverses the rows at upload). A
+ * sample at uv reads the wrapped coordinate (uv * uv_scale + uv_offset) with
+ * bilinear filtering between texel centers (sampleMap,
+ * materialBuffer.wgsl.ts:314-336). */
+typedef struct mtx_texture_desc {
+  uint32_t struct_size;
+  uint32_t width, height;       /* 1..16384 */
+  uint32_t storage;             /* MTX_TEXTURE_* */
+  uint32_t flip_y;              /* 0 or 1 */
+  uint32_t uv_channel;          /* 0: UV0, 1: UV1 */
+  uint32_t wrap_s, wrap_t;      /* MTX_WRAP_* */
+  float uv_scale[2], uv_offset[2];
+  const void *pixels;
+} mtx_texture_desc;
 
 /* Packed rect light (RectLight.ts:5-22): 16 floats. */
 /* Packed analytic light, the fields of the reference's Light record
@@ -245,7 +299,10 @@ typedef struct mtx_mesh_desc {
   const float *positions;   /* vertex_count * 3 */
   const float *normals;     /* vertex_count * 3 */
   const uint32_t *indices;  /* triangle_count * 3 */
+  const float *uvs;         /* vertex_count * 2, or NULL (UV0 = 0) */
+  const float *uvs1;        /* vertex_count * 2, or NULL (UV1 = 0; emissive triangles then use UV0) */
 } mtx_mesh_desc;
+/* A struct_size that ends before `uvs` is accepted: the mesh has no UVs. */
 
 typedef struct mtx_scene_desc {
   uint32_t struct_size;
@@ -262,7 +319,10 @@ typedef struct mtx_scene_desc {
   uint32_t blas_bins;       /* 8 = reference default */
   uint32_t tlas_bins;       /* 64 = web TLAS */
   uint32_t flags;           /* MTX_ACCEL_PER_INSTANCE_BOUNDS */
+  uint32_t texture_count;   /* material textures (MTX_MAT_MAPS index them) */
+  const mtx_texture_desc *textures;
 } mtx_scene_desc;
+/* A struct_size that ends before `texture_count` is accepted: no textures. */
 
 /* Scene commit (the GPU side of Engine.update / WGPURayTracePass.setMeshes,
  * WGPURayTracePass.ts:2246, setLights :3180): builds every BLAS on the GPU,

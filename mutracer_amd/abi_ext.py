@@ -5,7 +5,14 @@ import ctypes as C
 class MeshDesc(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
                 ("reserved", C.c_uint32), ("positions", C.c_void_p), ("normals", C.c_void_p),
-                ("indices", C.c_void_p)]
+                ("indices", C.c_void_p), ("uvs", C.c_void_p), ("uvs1", C.c_void_p)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("storage", C.c_uint32), ("flip_y", C.c_uint32), ("uv_channel", C.c_uint32),
+                ("wrap_s", C.c_uint32), ("wrap_t", C.c_uint32), ("uv_scale", C.c_float * 2),
+                ("uv_offset", C.c_float * 2), ("pixels", C.c_void_p)]
 
 
 class SceneDesc(C.Structure):
@@ -13,7 +20,8 @@ class SceneDesc(C.Structure):
                 ("instance_count", C.c_uint32), ("instances", C.c_void_p), ("material_count", C.c_uint32),
                 ("materials", C.c_void_p), ("light_count", C.c_uint32), ("lights", C.c_void_p),
                 ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_rgba", C.c_void_p),
-                ("blas_bins", C.c_uint32), ("tlas_bins", C.c_uint32), ("flags", C.c_uint32)]
+                ("blas_bins", C.c_uint32), ("tlas_bins", C.c_uint32), ("flags", C.c_uint32),
+                ("texture_count", C.c_uint32), ("textures", C.c_void_p)]
 
 
 class TargetSettings(C.Structure):

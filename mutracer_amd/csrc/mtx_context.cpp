@@ -136,8 +136,10 @@ struct MeshRef {  // one instance: which mesh record and batch entry it came fro
 
 struct Decoded {
   uint32_t version = 0, width = 0, height = 0;
-  std::vector<std::vector<float>> positions, normals;  // per BLAS mesh
+  std::vector<std::vector<float>> positions, normals, uvs, uvs1;  // per BLAS mesh (uvs: empty when absent)
   std::vector<std::vector<uint32_t>> indices;
+  std::vector<mtx_texture_desc> textures;        // material textures (pixels set by scene_desc)
+  std::vector<std::vector<uint32_t>> texPixels;  // their pixel words (RGBA8 words or RGBA32F float bits)
   std::vector<mtx_instance_desc> instances;
   std::vector<MeshRef> instanceSource;
   std::vector<float> materials, lights;
@@ -147,18 +149,23 @@ struct Decoded {
   size_t cameraOffset = 0;
 };
 
-// Material record + extensions -> the 48-float packed OpenPBR material
+// Material record + extensions -> the packed OpenPBR material
 // (decodeMaterial, native/src/scene-records.ts:40-56, and
 // applyNativeMaterialProperties, native/src/material-properties.ts:5-88;
-// defaults of src/scene/openPbrSchema.ts:32-69).
+// defaults of src/scene/openPbrSchema.ts:32-69). Texture slots hold the
+// descriptor's texture index + 1 (0 = none); the caller remaps them to the
+// scene's material texture table.
 void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textureCount, float *m) {
   W.finite(o + MAT::BaseColor, 14, "material");
   const uint32_t thin = W.u(o + MAT::ThinWalled);
   need(thin <= 1, "material thin_walled must be 0 or 1");
+  uint32_t maps[MTX_MAP_COUNT];
+  std::fill(maps, maps + MTX_MAP_COUNT, kNone);
+  const int recordSlot[4] = {MTX_MAP_BASE, MTX_MAP_NORMAL, MTX_MAP_METALLIC_ROUGHNESS, MTX_MAP_EMISSIVE};
   for (uint32_t t = MAT::Map; t <= MAT::EmissiveMap; t++) {
     const uint32_t tex = W.u(o + t);
     need(tex == kNone || tex < textureCount, "material texture index out of range");
-    if (tex != kNone) unsupported("material textures");
+    maps[recordSlot[t - MAT::Map]] = tex;
   }
   if (W.u(o + MAT::Graph)) {
     need(version >= 8, "material graph_offset must be 0 before version 8");
@@ -170,7 +177,9 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
           specularColor[3] = {1, 1, 1}, roughness, ior, anisotropy = 0, transmission, transmissionColor[3] = {1, 1, 1},
           subsurfaceWeight = 0, subsurfaceColor[3] = {0.8f, 0.8f, 0.8f}, fuzzWeight = 0, fuzzColor[3] = {1, 1, 1},
           fuzzRoughness = 0.5f, coatWeight, coatColor[3] = {1, 1, 1}, coatRoughness, coatIor = 1.6f,
-          coatDarkening = 1, emissionColor[3], emissionLuminance, opacity;
+          coatDarkening = 1, emissionColor[3], emissionLuminance, opacity, alphaCutoff = 0,
+          normalScale[2] = {1, 1}, uvScale[2] = {1, 1};
+    uint32_t alphaMode = 0, mapAlphaUsage = 0;  // native enums: auto/opaque/mask/blend, opaque/alpha/transmission
     bool thinWalled, doubleSided = false;
   } p;
   for (int k = 0; k < 3; k++) {
@@ -222,11 +231,20 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
       color(c);
       if (c[0] != def[0] || c[1] != def[1] || c[2] != def[2]) unsupported(what);
     };
-    auto no_texture = [&](const char *what) {
+    auto texture = [&]() {
       const uint32_t t = W.u(e + 1);
       W.zero(e + 2, 2, "material property");
       need(t == kNone || t < textureCount, "material texture index out of range");
-      if (t != kNone) unsupported(what);
+      return t;
+    };
+    auto no_texture = [&](const char *what) {
+      if (texture() != kNone) unsupported(what);
+    };
+    auto vec2 = [&](float *v) {
+      W.finite(e + 1, 2, "material property");
+      W.zero(e + 3, 1, "material property");
+      v[0] = W.f(e + 1);
+      v[1] = W.f(e + 2);
     };
     static const float kZero3[3] = {0, 0, 0};
     switch (id) {
@@ -270,19 +288,19 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
       case 41: {
         need(W.u(e + 1) <= 3, "alpha mode must be 0..3");
         W.zero(e + 2, 2, "material property");
-        if (W.u(e + 1) >= 2) unsupported("alpha mask/blend modes");
+        p.alphaMode = W.u(e + 1);
       } break;
-      case 42: { float v; scalar(v); } break;  // alpha cutoff: only used by mask mode
-      case 45: case 56: case 75: {  // vec2 scales of maps this build does not read
-        W.finite(e + 1, 2, "material property");
-        W.zero(e + 3, 1, "material property");
-      } break;
+      case 42: scalar(p.alphaCutoff); break;
+      case 45: vec2(p.normalScale); break;
+      case 56: vec2(p.uvScale); break;
+      case 75: { float v[2]; vec2(v); } break;  // coat normal scale: the coat normal map is not built
       case 51: { bool v; flag(v); if (v) unsupported("shadow catchers"); } break;
       case 52: { bool v; flag(v); if (v) unsupported("unlit materials"); } break;
       case 53: only_default(0, "bevel"); break;
       case 54: {
         need(W.u(e + 1) <= 2, "map alpha usage must be 0..2");
         W.zero(e + 2, 2, "material property");
+        p.mapAlphaUsage = W.u(e + 1);
       } break;
       case 55: case 57: {
         need(W.u(e + 1) <= 3, "UV projection must be 0..3");
@@ -294,19 +312,39 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
         need(W.u(e + 1) <= 3, "enum material property out of range");
         W.zero(e + 2, 2, "material property");
       } break;
-      case 39: case 40: case 43: case 44: case 46: case 47: case 48: case 49: case 50: case 59: case 64: case 65:
-      case 66: case 67: case 68: case 69: case 70: case 74:
-        no_texture("material textures");
-        break;
+      case 39: maps[MTX_MAP_BASE] = texture(); break;
+      case 40: maps[MTX_MAP_ALPHA] = texture(); break;
+      case 43: maps[MTX_MAP_EMISSIVE] = texture(); break;
+      case 44: maps[MTX_MAP_NORMAL] = texture(); break;
+      case 46: maps[MTX_MAP_METALLIC_ROUGHNESS] = texture(); break;
+      case 47: maps[MTX_MAP_ROUGHNESS] = texture(); break;
+      case 48: maps[MTX_MAP_METALNESS] = texture(); break;
+      case 49: maps[MTX_MAP_TRANSMISSION] = texture(); break;
+      case 64: maps[MTX_MAP_COAT_WEIGHT] = texture(); break;
+      case 65: maps[MTX_MAP_COAT_ROUGHNESS] = texture(); break;
+      case 66: maps[MTX_MAP_FUZZ_COLOR] = texture(); break;
+      case 67: maps[MTX_MAP_FUZZ_ROUGHNESS] = texture(); break;
+      case 68: maps[MTX_MAP_SPECULAR_WEIGHT] = texture(); break;
+      case 69: maps[MTX_MAP_SPECULAR_COLOR] = texture(); break;
+      case 50: no_texture("thin-film maps"); break;
+      case 59: no_texture("displacement maps"); break;
+      case 70: no_texture("anisotropy maps"); break;
+      case 74: no_texture("coat normal maps"); break;
       default: break;
     }
   }
-  if (p.opacity != 1.0f) unsupported("opacity below 1 (alpha)");
+  // resolveMaterialAlphaMode (src/scene/MaterialState.ts:251-264): coverage
+  // (mask or blend) is not built.
+  bool opaque = p.alphaMode == 1;
+  if (p.alphaMode == 0)
+    opaque = !(p.alphaCutoff > 0.0f) && !(p.opacity < 1.0f) && p.mapAlphaUsage != 1 && maps[MTX_MAP_ALPHA] == kNone;
+  if (!opaque) unsupported("alpha coverage (mask and blend materials)");
   if (p.anisotropy != 0.0f) unsupported("specular anisotropy");
-  // materialEmission (src/renderer/materialEmission.ts:10-22), in f64 as JS.
+  // materialEmission (src/renderer/materialEmission.ts:10-22), in f64 as JS;
+  // a coat weight map leaves the coat tint to the sample.
   double em[3];
   for (int k = 0; k < 3; k++) em[k] = (double)p.emissionColor[k] * (double)p.emissionLuminance;
-  if (p.coatWeight > 0.0f) {
+  if (p.coatWeight > 0.0f && maps[MTX_MAP_COAT_WEIGHT] == kNone) {
     const double w = std::min((double)p.coatWeight, 1.0);
     for (int k = 0; k < 3; k++) em[k] = em[k] * (1.0 + w * (std::sqrt(std::max((double)p.coatColor[k], 0.0)) - 1.0));
   }
@@ -336,6 +374,15 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
   m[MTX_MAT_SUBSURFACE_WEIGHT] = p.subsurfaceWeight;
   put3(MTX_MAT_SUBSURFACE_COLOR, p.subsurfaceColor);
   m[MTX_MAT_THIN_WALLED] = p.thinWalled ? 1.0f : 0.0f;
+  m[MTX_MAT_OPACITY] = p.opacity;
+  m[MTX_MAT_ALPHA_MODE] = 0.0f;  // resolved: opaque
+  m[MTX_MAT_ALPHA_CUTOFF] = std::max(p.alphaCutoff, 0.0f);
+  m[MTX_MAT_MAP_ALPHA_USAGE] = p.mapAlphaUsage == 0 ? 0.0f : p.mapAlphaUsage == 1 ? 2.0f : 3.0f;
+  m[MTX_MAT_NORMAL_SCALE] = p.normalScale[0];
+  m[MTX_MAT_NORMAL_SCALE + 1] = p.normalScale[1];
+  m[MTX_MAT_UV_SCALE] = p.uvScale[0];
+  m[MTX_MAT_UV_SCALE + 1] = p.uvScale[1];
+  for (int k = 0; k < MTX_MAP_COUNT; k++) m[MTX_MAT_MAPS + k] = maps[k] == kNone ? 0.0f : (float)(maps[k] + 1u);
 }
 
 // getRigidLightFrame (src/renderer/lightsData.ts:112-124) in f64.
@@ -596,7 +643,8 @@ Decoded decode(const uint8_t *bytes, size_t length) {
   if (quality & 4u) unsupported("a hidden background");
   if (W.u(H::GroundedBackground)) unsupported("grounded background projection");
 
-  // Textures: only the equirectangular environment is consumed.
+  // Textures: the equirectangular environment, and the material textures
+  // (copied below, once the materials name them).
   const size_t texOff = W.u(H::TextureOffset);
   for (uint32_t i = 0; i < nTex; i++) {
     const size_t o = texOff + (size_t)i * texWords;
@@ -656,12 +704,43 @@ Decoded decode(const uint8_t *bytes, size_t length) {
   const size_t matOff = W.u(H::MaterialOffset);
   for (uint32_t i = 0; i < nMat; i++)
     decode_material(W, matOff + (size_t)i * kMaterialWords, D.version, nTex, &D.materials[(size_t)i * kMatFloats]);
+  // Material textures: the descriptor textures that some slot references,
+  // in first-reference order, copied with their sampling parameters.
+  std::map<uint32_t, uint32_t> texOf;
+  for (size_t i = 0; i < (size_t)nMat * MTX_MAP_COUNT; i++) {
+    float &slot = D.materials[(i / MTX_MAP_COUNT) * kMatFloats + MTX_MAT_MAPS + i % MTX_MAP_COUNT];
+    if (slot == 0.0f) continue;
+    const uint32_t t = (uint32_t)slot - 1u;
+    auto it = texOf.find(t);
+    if (it == texOf.end()) {
+      const size_t o = texOff + (size_t)t * texWords;
+      if (texWords == kTextureWords && W.u(o + T::Udim)) unsupported("UDIM material textures");
+      mtx_texture_desc d{};
+      d.struct_size = sizeof d;
+      d.width = W.u(o + T::Width);
+      d.height = W.u(o + T::Height);
+      d.storage = W.u(o + T::Storage);
+      d.flip_y = W.u(o + T::FlipY);
+      d.uv_channel = W.u(o + T::UvChannel);
+      d.wrap_s = W.u(o + T::WrapS);
+      d.wrap_t = W.u(o + T::WrapT);
+      for (int k = 0; k < 2; k++) {
+        d.uv_scale[k] = W.f(o + T::UvScale + k);
+        d.uv_offset[k] = W.f(o + T::UvOffset + k);
+      }
+      const size_t px = W.u(o + T::Pixels), n = (size_t)d.width * d.height * (d.storage ? 4 : 1);
+      D.texPixels.emplace_back(copy.begin() + px, copy.begin() + px + n);
+      D.textures.push_back(d);
+      it = texOf.emplace(t, (uint32_t)D.textures.size() - 1u).first;
+    }
+    slot = (float)(it->second + 1u);
+  }
 
   // Geometry: validate, generate missing normals, split multi-material
   // geometry into one BLAS per (geometry, mesh-local material slot).
   struct Geo {
     uint32_t vc, ic;
-    size_t pos, idx;
+    size_t pos, idx, uv = 0, uv1 = 0;
     std::vector<float> normals;
     std::vector<uint32_t> triSlot;  // per triangle: mesh-local material slot (empty: one slot)
     uint32_t slots = 1;
@@ -692,8 +771,16 @@ Decoded decode(const uint8_t *bytes, size_t length) {
     } else {
       Gg.normals = vertex_normals(pos, copy.data() + Gg.idx, Gg.ic, Gg.vc);
     }
-    if (const uint32_t uo = W.u(o + G::Uv)) { W.span(uo, (size_t)Gg.vc * 2, "geometry uv"); W.finite(uo, (size_t)Gg.vc * 2, "geometry uv"); }
-    if (const uint32_t uo = W.u(o + G::Uv1)) { W.span(uo, (size_t)Gg.vc * 2, "geometry uv1"); W.finite(uo, (size_t)Gg.vc * 2, "geometry uv1"); }
+    if (const uint32_t uo = W.u(o + G::Uv)) {
+      W.span(uo, (size_t)Gg.vc * 2, "geometry uv");
+      W.finite(uo, (size_t)Gg.vc * 2, "geometry uv");
+      Gg.uv = uo;
+    }
+    if (const uint32_t uo = W.u(o + G::Uv1)) {
+      W.span(uo, (size_t)Gg.vc * 2, "geometry uv1");
+      W.finite(uo, (size_t)Gg.vc * 2, "geometry uv1");
+      Gg.uv1 = uo;
+    }
     if (const uint32_t to = W.u(o + G::Tangent)) { W.span(to, (size_t)Gg.vc * 4, "geometry tangents"); W.finite(to, (size_t)Gg.vc * 4, "geometry tangents"); }
     if (W.u(o + G::Color)) unsupported("vertex colors");
     if (const uint32_t mo = W.u(o + G::MaterialIndex)) {
@@ -720,8 +807,10 @@ Decoded decode(const uint8_t *bytes, size_t length) {
     const Geo &Gg = geos[g];
     std::vector<uint32_t> idx;
     std::vector<uint32_t> remap(Gg.vc, kNone);
-    std::vector<float> pos, nrm;
+    std::vector<float> pos, nrm, uv, uv1;
     const float *P = reinterpret_cast<const float *>(copy.data() + Gg.pos);
+    const float *U = Gg.uv ? reinterpret_cast<const float *>(copy.data() + Gg.uv) : nullptr;
+    const float *U1 = Gg.uv1 ? reinterpret_cast<const float *>(copy.data() + Gg.uv1) : nullptr;
     const bool whole = Gg.triSlot.empty();
     for (uint32_t t = 0; t < Gg.ic / 3; t++) {
       if (!whole && Gg.triSlot[t] != slot) continue;
@@ -732,6 +821,8 @@ Decoded decode(const uint8_t *bytes, size_t length) {
           remap[v] = (uint32_t)(pos.size() / 3);
           pos.insert(pos.end(), P + (size_t)v * 3, P + (size_t)v * 3 + 3);
           nrm.insert(nrm.end(), Gg.normals.begin() + (size_t)v * 3, Gg.normals.begin() + (size_t)v * 3 + 3);
+          if (U) uv.insert(uv.end(), U + (size_t)v * 2, U + (size_t)v * 2 + 2);
+          if (U1) uv1.insert(uv1.end(), U1 + (size_t)v * 2, U1 + (size_t)v * 2 + 2);
         }
         idx.push_back(remap[v]);
       }
@@ -739,10 +830,14 @@ Decoded decode(const uint8_t *bytes, size_t length) {
     if (whole) {
       pos.assign(P, P + (size_t)Gg.vc * 3);
       nrm = Gg.normals;
+      if (U) uv.assign(U, U + (size_t)Gg.vc * 2);
+      if (U1) uv1.assign(U1, U1 + (size_t)Gg.vc * 2);
     }
     const uint32_t b = (uint32_t)D.positions.size();
     D.positions.push_back(std::move(pos));
     D.normals.push_back(std::move(nrm));
+    D.uvs.push_back(std::move(uv));
+    D.uvs1.push_back(std::move(uv1));
     D.indices.push_back(std::move(idx));
     blasOf[key] = b;
     return b;
@@ -953,7 +1048,8 @@ void check_settings(const mtx_render_settings &s) {
   if (s.transmissive_output_alpha_min != 1.0) unsupported("transmissive output alpha below 1");
 }
 
-mtx_scene_desc scene_desc(const Decoded &D, std::vector<mtx_mesh_desc> &meshes) {
+mtx_scene_desc scene_desc(const Decoded &D, std::vector<mtx_mesh_desc> &meshes,
+                          std::vector<mtx_texture_desc> &textures) {
   meshes.resize(D.positions.size());
   for (size_t i = 0; i < meshes.size(); i++) {
     mtx_mesh_desc &m = meshes[i];
@@ -964,7 +1060,11 @@ mtx_scene_desc scene_desc(const Decoded &D, std::vector<mtx_mesh_desc> &meshes) 
     m.positions = D.positions[i].data();
     m.normals = D.normals[i].data();
     m.indices = D.indices[i].data();
+    m.uvs = D.uvs[i].empty() ? nullptr : D.uvs[i].data();
+    m.uvs1 = D.uvs1[i].empty() ? nullptr : D.uvs1[i].data();
   }
+  textures = D.textures;
+  for (size_t i = 0; i < textures.size(); i++) textures[i].pixels = D.texPixels[i].data();
   mtx_scene_desc s{};
   s.struct_size = sizeof s;
   s.mesh_count = (uint32_t)meshes.size();
@@ -981,6 +1081,8 @@ mtx_scene_desc scene_desc(const Decoded &D, std::vector<mtx_mesh_desc> &meshes) 
   s.blas_bins = 8;   // native TLAS/BLAS defaults (bvhAccel.ts:385-454)
   s.tlas_bins = 64;
   s.flags = MTX_ACCEL_PER_INSTANCE_BOUNDS;
+  s.texture_count = (uint32_t)textures.size();
+  s.textures = textures.empty() ? nullptr : textures.data();
   return s;
 }
 
@@ -1058,7 +1160,8 @@ void commit(Context &C, const uint8_t *bytes, size_t length, const mtx_render_se
     C.lastCommit = MTX_COMMIT_INCREMENTAL;
   } else {
     std::vector<mtx_mesh_desc> meshes;
-    mtx_scene_desc sd = scene_desc(D, meshes);
+    std::vector<mtx_texture_desc> textures;
+    mtx_scene_desc sd = scene_desc(D, meshes, textures);
     mtx_handle scene = 0, target = 0;
     call([&](char *e, size_t n) { return mtx_scene_create(&sd, &scene, e, n); });
     mtx_target_settings ts = target_settings(D, s);

@@ -12,8 +12,18 @@ struct ShadeSlot {
   const uint32_t *indices;        // source mesh triangle indices
   uint32_t material;
   float color[3];
+  const float *uvs, *uvs1;        // source mesh UV0 / UV1 (xy), or null
 };
-static_assert(sizeof(ShadeSlot) == 96, "ShadeSlot");
+static_assert(sizeof(ShadeSlot) == 112, "ShadeSlot");
+
+// A material texture: the sample transform, size, flags (bit 0 RGBA32F,
+// bits 1-2 wrap s, bits 3-4 wrap t, bit 5 UV1) and its first texel in the
+// pool of its storage (rows already in sampling order).
+struct GpuTexture {
+  float sx, sy, ox, oy;
+  uint32_t width, height, flags, base;
+};
+static_assert(sizeof(GpuTexture) == 32, "GpuTexture");
 
 // Alias table entry (lightSampling.ts:172-212): probability, alias, type, index.
 struct LightChoice {
@@ -40,13 +50,18 @@ struct GpuScene {
   const float *lut;            // 32 x 2177 energy tables
   const float4 *emissive;      // 3 per emissive triangle: (p0, pmf), (p1, area), (p2, material), world space
   uint32_t emissiveCount;
-  uint32_t features;           // kSceneFull when a material has fuzz, subsurface or a thin wall
+  uint32_t features;           // kSceneFull when a material has fuzz, subsurface, a thin wall or a texture
   const uint16_t *envGuide;    // search brackets of envCdf (kEnvGuideRows + 1, then envH x (kEnvGuideCols + 1)), or null
+  const GpuTexture *textures;  // material textures, or null (no material has a map)
+  const uint32_t *texels8;     // RGBA8 texel pool
+  const float4 *texels32;      // RGBA32F texel pool
+  const float4 *emissiveUv;    // 3 per emissive triangle: (uv0, uv1), (uv2, uv1_0), (uv1_1, uv1_2), with textures
 };
 // Bins of the environment CDF guide tables: bin k of the marginal (of row
 // y's conditional) CDF holds the index its search returns for the sample
 // k / bins, so a sample in bin k searches only between bins k and k + 1.
 constexpr uint32_t kSceneFull = 1u;
+constexpr uint32_t kSceneTextured = 2u;  // set with kSceneFull: the full kernels sample the maps
 constexpr uint32_t kEnvGuideRows = 1024;
 constexpr uint32_t kEnvGuideCols = 128;
 

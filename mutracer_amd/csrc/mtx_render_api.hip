@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <deque>
 #include <mutex>
 
@@ -19,21 +20,26 @@ extern const uint32_t kEnergyLutBits[32 * 2177];
 struct RenderSceneObj {
   std::vector<std::shared_ptr<Blas>> blases;
   std::shared_ptr<Accel> accel;
-  std::vector<DevBuf<float>> normals;
+  std::vector<DevBuf<float>> normals, uvs, uvs1;
   std::vector<DevBuf<uint32_t>> indices;
   DevBuf<ShadeSlot> shade;
   DevBuf<float> materials, lights, lightPmf, lut;
   DevBuf<LightChoice> choices;
-  DevBuf<float4> env, emissive;
+  DevBuf<float4> env, emissive, emissiveUv, texels32;
   DevBuf<float2> envCdf;
   DevBuf<uint16_t> envGuide;
+  DevBuf<GpuTexture> textures;
+  DevBuf<uint32_t> texels8;
   std::vector<mtx_instance_desc> inst;
   GpuScene view{};
   uint64_t triangles = 0;
   size_t bytes() const {
     size_t b = shade.bytes() + materials.bytes() + lights.bytes() + lut.bytes() + env.bytes() + envCdf.bytes() +
-               envGuide.bytes() + emissive.bytes();
+               envGuide.bytes() + emissive.bytes() + emissiveUv.bytes() + textures.bytes() + texels8.bytes() +
+               texels32.bytes();
     for (auto &x : normals) b += x.bytes();
+    for (auto &x : uvs) b += x.bytes();
+    for (auto &x : uvs1) b += x.bytes();
     for (auto &x : indices) b += x.bytes();
     for (auto &x : blases) b += x->resident_bytes();
     return b;
@@ -90,11 +96,22 @@ struct LightTables {
 // area (half the f64 length of the f32 edge cross product; degenerate
 // triangles are skipped) and alias weight area * luminance
 // (prepareSceneLightSampling, sceneAssembly.ts:46).
+// With textures each triangle also records its vertex UVs (3 float4:
+// (uv0, uv1), (uv2, uv1_0), (uv1_1, uv1_2)); UV1 falls back to UV0, and a
+// mesh without UVs records zeros (lightSampling.ts:262-299).
 struct EmissiveList {
   std::vector<float4> tris;  // 3 per triangle: (p0, pmf), (p1, area), (p2, material)
+  std::vector<float4> uvs;
   std::vector<double> weight;
 };
-static EmissiveList collect_emissive(const mtx_scene_desc &D) {
+// UV0 / UV1 arrays of a mesh descriptor (absent in descriptors that end before them).
+static const float *mesh_uvs(const mtx_mesh_desc &m) {
+  return m.struct_size >= offsetof(mtx_mesh_desc, uvs) + sizeof(m.uvs) ? m.uvs : nullptr;
+}
+static const float *mesh_uvs1(const mtx_mesh_desc &m) {
+  return m.struct_size >= offsetof(mtx_mesh_desc, uvs1) + sizeof(m.uvs1) ? m.uvs1 : nullptr;
+}
+static EmissiveList collect_emissive(const mtx_scene_desc &D, bool withUvs) {
   EmissiveList E;
   for (uint32_t i = 0; i < D.instance_count; i++) {
     const mtx_instance_desc &d = D.instances[i];
@@ -129,6 +146,17 @@ static EmissiveList collect_emissive(const mtx_scene_desc &D) {
       E.tris.push_back(make_float4(w[1][0], w[1][1], w[1][2], (float)area));
       E.tris.push_back(make_float4(w[2][0], w[2][1], w[2][2], (float)mat));
       E.weight.push_back(area * lum);
+      if (withUvs) {
+        const float *u0 = mesh_uvs(m), *u1 = mesh_uvs1(m);
+        if (u1 == nullptr) u1 = u0;
+        float q[12] = {};  // uv0, uv1, uv2, then UV1 of the three corners
+        for (int c = 0; c < 3; c++) {
+          const size_t v = m.indices[(size_t)t * 3 + c];
+          if (u0 != nullptr) { q[2 * c] = u0[v * 2]; q[2 * c + 1] = u0[v * 2 + 1]; }
+          if (u1 != nullptr) { q[6 + 2 * c] = u1[v * 2]; q[6 + 2 * c + 1] = u1[v * 2 + 1]; }
+        }
+        for (int k = 0; k < 3; k++) E.uvs.push_back(make_float4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]));
+      }
     }
   }
   return E;
@@ -305,6 +333,8 @@ static void fill_shade(RenderSceneObj &R) {
     s.l2w3 = make_float4((float)m[12], (float)m[13], (float)m[14], 0.0f);
     s.normals = R.normals[d.blas_index].ptr;
     s.indices = R.indices[d.blas_index].ptr;
+    s.uvs = R.uvs[d.blas_index].count ? R.uvs[d.blas_index].ptr : nullptr;
+    s.uvs1 = R.uvs1[d.blas_index].count ? R.uvs1[d.blas_index].ptr : nullptr;
     s.material = d.material_word >> 8;
     for (int k = 0; k < 3; k++) s.color[k] = d.color[k];
     sh[q] = s;
@@ -312,8 +342,85 @@ static void fill_shade(RenderSceneObj &R) {
   R.shade.upload_min1(sh.data(), A.instanceCount, 0);
 }
 
+// Material textures (the descriptor's texture table, mutracer-scene.h:87-95):
+// validated, rows put in sampling order (flip_y reverses them, as
+// textureAtlasPixels.ts:35-37 does) and packed into one pool per storage.
+// Returns the texture count.
+static uint32_t upload_textures(RenderSceneObj &R, const mtx_scene_desc &D) {
+  const uint32_t n =
+      D.struct_size >= offsetof(mtx_scene_desc, textures) + sizeof(D.textures) ? D.texture_count : 0u;
+  if (n == 0) return 0;
+  MTX_REQUIRE(D.textures, MTX_INVALID_ARGUMENT, "textures is NULL");
+  std::vector<GpuTexture> T(n);
+  size_t n8 = 0, n32 = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const mtx_texture_desc &t = D.textures[i];
+    MTX_REQUIRE(t.struct_size >= sizeof(mtx_texture_desc), MTX_INVALID_ARGUMENT,
+                "mtx_texture_desc.struct_size too small");
+    MTX_REQUIRE(t.width >= 1 && t.height >= 1 && t.width <= 16384 && t.height <= 16384, MTX_INVALID_ARGUMENT,
+                "texture size must be 1..16384 per edge");
+    MTX_REQUIRE(t.storage <= MTX_TEXTURE_RGBA32F, MTX_INVALID_ARGUMENT, "unknown texture storage");
+    MTX_REQUIRE(t.flip_y <= 1 && t.uv_channel <= 1 && t.wrap_s <= 2 && t.wrap_t <= 2, MTX_INVALID_ARGUMENT,
+                "texture flag out of range");
+    MTX_REQUIRE(std::isfinite(t.uv_scale[0]) && std::isfinite(t.uv_scale[1]) && std::isfinite(t.uv_offset[0]) &&
+                    std::isfinite(t.uv_offset[1]),
+                MTX_INVALID_ARGUMENT, "texture transform must be finite");
+    MTX_REQUIRE(t.pixels, MTX_INVALID_ARGUMENT, "texture pixels are NULL");
+    const size_t px = (size_t)t.width * t.height;
+    size_t &pool = t.storage == MTX_TEXTURE_RGBA32F ? n32 : n8;
+    MTX_REQUIRE(pool + px <= 0xffffffffull, MTX_INVALID_ARGUMENT, "texture pool exceeds 2^32 texels per storage");
+    T[i] = GpuTexture{t.uv_scale[0], t.uv_scale[1], t.uv_offset[0], t.uv_offset[1], t.width, t.height,
+                      t.storage | (t.wrap_s << 1) | (t.wrap_t << 3) | (t.uv_channel << 5), (uint32_t)pool};
+    pool += px;
+  }
+  std::vector<uint32_t> p8(n8);
+  std::vector<float4> p32(n32);
+  for (uint32_t i = 0; i < n; i++) {
+    const mtx_texture_desc &t = D.textures[i];
+    for (uint32_t y = 0; y < t.height; y++) {
+      const size_t src = (size_t)(t.flip_y ? t.height - 1 - y : y) * t.width;
+      const size_t dst = (size_t)T[i].base + (size_t)y * t.width;
+      if (t.storage == MTX_TEXTURE_RGBA32F) {
+        const float *f = static_cast<const float *>(t.pixels) + src * 4;
+        for (size_t k = 0; k < (size_t)t.width * 4; k++)
+          MTX_REQUIRE(std::isfinite(f[k]), MTX_INVALID_ARGUMENT, "texture pixels must be finite");
+        std::memcpy(&p32[dst], f, (size_t)t.width * sizeof(float4));
+      } else {
+        std::memcpy(&p8[dst], static_cast<const uint32_t *>(t.pixels) + src, (size_t)t.width * sizeof(uint32_t));
+      }
+    }
+  }
+  R.textures.upload(T.data(), T.size(), 0);
+  R.texels8.upload_min1(p8.data(), p8.size(), 0);
+  R.texels32.upload_min1(p32.data(), p32.size(), 0);
+  return n;
+}
+
+// Texture-related material fields: resolved alpha mode opaque (coverage is
+// not built), map alpha usage, slot indices within the texture table, finite
+// UV and normal scales. Returns whether some material samples a texture.
+static bool check_material_maps(const float *mat, uint32_t textureCount) {
+  MTX_REQUIRE(mat[MTX_MAT_ALPHA_MODE] == 0.0f, MTX_UNAVAILABLE,
+              "alpha coverage (mask and blend materials) is not supported by this renderer");
+  const float usage = mat[MTX_MAT_MAP_ALPHA_USAGE];
+  MTX_REQUIRE(usage == 0.0f || usage == 2.0f || usage == 3.0f, MTX_INVALID_ARGUMENT,
+              "map alpha usage must be 0, 2 or 3");
+  for (int k = 0; k < 2; k++)
+    MTX_REQUIRE(std::isfinite(mat[MTX_MAT_UV_SCALE + k]) && std::isfinite(mat[MTX_MAT_NORMAL_SCALE + k]),
+                MTX_INVALID_ARGUMENT, "material UV and normal scales must be finite");
+  bool textured = false;
+  for (int k = 0; k < MTX_MAP_COUNT; k++) {
+    const float v = mat[MTX_MAT_MAPS + k];
+    MTX_REQUIRE(v >= 0.0f && v == std::floor(v) && v <= (float)textureCount, MTX_INVALID_ARGUMENT,
+                "material texture slot out of range");
+    textured |= v > 0.0f;
+  }
+  return textured;
+}
+
 static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
-  MTX_REQUIRE(D.struct_size >= sizeof(mtx_scene_desc), MTX_INVALID_ARGUMENT, "mtx_scene_desc.struct_size too small");
+  MTX_REQUIRE(D.struct_size >= offsetof(mtx_scene_desc, texture_count), MTX_INVALID_ARGUMENT,
+              "mtx_scene_desc.struct_size too small");
   MTX_REQUIRE(D.mesh_count == 0 || D.meshes, MTX_INVALID_ARGUMENT, "meshes is NULL");
   MTX_REQUIRE(D.instance_count == 0 || D.instances, MTX_INVALID_ARGUMENT, "instances is NULL");
   MTX_REQUIRE(D.material_count > 0 && D.materials, MTX_INVALID_ARGUMENT, "at least one material is required");
@@ -323,8 +430,13 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   std::vector<MeshInput> meshes;
   for (uint32_t i = 0; i < D.mesh_count; i++) {
     const mtx_mesh_desc &m = D.meshes[i];
-    MTX_REQUIRE(m.struct_size >= sizeof(mtx_mesh_desc), MTX_INVALID_ARGUMENT, "mtx_mesh_desc.struct_size too small");
+    MTX_REQUIRE(m.struct_size >= offsetof(mtx_mesh_desc, uvs), MTX_INVALID_ARGUMENT,
+                "mtx_mesh_desc.struct_size too small");
     MTX_REQUIRE(m.positions && m.normals && m.indices, MTX_INVALID_ARGUMENT, "mesh arrays are NULL");
+    for (const float *u : {mesh_uvs(m), mesh_uvs1(m)})
+      if (u != nullptr)
+        for (size_t k = 0; k < (size_t)m.vertex_count * 2; k++)
+          MTX_REQUIRE(std::isfinite(u[k]), MTX_INVALID_ARGUMENT, "mesh UVs must be finite");
     for (size_t k = 0; k < (size_t)m.triangle_count * 3; k++)
       MTX_REQUIRE(m.indices[k] < m.vertex_count, MTX_INVALID_ARGUMENT, "mesh index out of range");
     meshes.push_back(MeshInput{m.positions, m.vertex_count, m.indices, m.triangle_count, false});
@@ -336,7 +448,15 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
     R->normals.back().upload_min1(m.normals, (size_t)m.vertex_count * 3, 0);
     R->indices.emplace_back();
     R->indices.back().upload_min1(m.indices, (size_t)m.triangle_count * 3, 0);
+    R->uvs.emplace_back();
+    if (mesh_uvs(m)) R->uvs.back().upload_min1(mesh_uvs(m), (size_t)m.vertex_count * 2, 0);
+    R->uvs1.emplace_back();
+    if (mesh_uvs1(m)) R->uvs1.back().upload_min1(mesh_uvs1(m), (size_t)m.vertex_count * 2, 0);
   }
+  const uint32_t textureCount = upload_textures(*R, D);
+  bool textured = false;
+  for (uint32_t m = 0; m < D.material_count; m++)
+    textured |= check_material_maps(D.materials + (size_t)m * MTX_MATERIAL_FLOATS, textureCount);
   R->inst.assign(D.instances, D.instances + D.instance_count);
   for (auto &d : R->inst) {
     MTX_REQUIRE(d.blas_index < D.mesh_count, MTX_INVALID_ARGUMENT, "instance mesh index out of range");
@@ -351,11 +471,12 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
   R->materials.upload(D.materials, (size_t)D.material_count * MTX_MATERIAL_FLOATS, 0);
   R->lights.upload_min1(D.lights, (size_t)D.light_count * MTX_LIGHT_FLOATS, 0);
   const bool hasEnv = D.env_rgba && D.env_width && D.env_height;
-  EmissiveList E = collect_emissive(D);
+  EmissiveList E = collect_emissive(D, textured);
   LightTables T = light_tables(D.lights, D.light_count, hasEnv ? D.env_rgba : nullptr, D.env_width, D.env_height,
                                E.weight);
   for (size_t i = 0; i < E.weight.size(); i++) E.tris[3 * i].w = T.emissivePmf[i];
   R->emissive.upload_min1(E.tris.data(), E.tris.size(), 0);
+  if (textured) R->emissiveUv.upload_min1(E.uvs.data(), E.uvs.size(), 0);
   R->lightPmf.upload_min1(T.lightPmf.data(), T.lightPmf.size(), 0);
   R->choices.upload_min1(T.choices.data(), T.choices.size(), 0);
   if (hasEnv) {
@@ -387,6 +508,13 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
     const float *mat = D.materials + (size_t)m * MTX_MATERIAL_FLOATS;
     if (mat[MTX_MAT_FUZZ_WEIGHT] > 0.0f || mat[MTX_MAT_SUBSURFACE_WEIGHT] > 0.0f || mat[MTX_MAT_THIN_WALLED] > 0.5f)
       v.features |= kSceneFull;
+  }
+  if (textured) {
+    v.features |= kSceneFull | kSceneTextured;
+    v.textures = R->textures.ptr;
+    v.texels8 = R->texels8.ptr;
+    v.texels32 = R->texels32.ptr;
+    v.emissiveUv = E.weight.empty() ? nullptr : R->emissiveUv.ptr;
   }
   v.envW = hasEnv ? D.env_width : 0;
   v.envH = hasEnv ? D.env_height : 0;

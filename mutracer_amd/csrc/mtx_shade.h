@@ -272,6 +272,81 @@ __device__ __forceinline__ Basis make_basis(f3 n) {
 __device__ __forceinline__ f3 to_world(const Basis &B, f3 d) { return d.x * B.t + d.y * B.b + d.z * B.n; }
 __device__ __forceinline__ f3 to_local(const Basis &B, f3 d) { return f3{dot(d, B.t), dot(d, B.b), dot(d, B.n)}; }
 
+// ---------------------------------------------------------------- material textures
+// wrapMapCoordinate (materialBuffer.wgsl.ts:285-289): repeat, clamp, mirror.
+__device__ __forceinline__ float wrap_coord(float v, uint32_t mode) {
+  if (mode == MTX_WRAP_REPEAT) return fractf(v);
+  if (mode == MTX_WRAP_CLAMP_TO_EDGE) return clampf(v, 0.0f, 1.0f);
+  return 1.0f - fabsf(1.0f - (v - 2.0f * floorf(v * 0.5f)));
+}
+__device__ __forceinline__ float4 map_texel(const GpuScene &S, const GpuTexture &t, int x, int y) {
+  const size_t i = (size_t)t.base + (size_t)(uint32_t)y * t.width + (uint32_t)x;
+  if (t.flags & 1u) return S.texels32[i];
+  const uint32_t w = S.texels8[i];
+  return make_float4((float)(w & 255u) / 255.0f, (float)((w >> 8) & 255u) / 255.0f,
+                     (float)((w >> 16) & 255u) / 255.0f, (float)(w >> 24) / 255.0f);
+}
+__device__ __forceinline__ int map_tap(int c, int size, uint32_t mode) {
+  return mode == MTX_WRAP_REPEAT ? ((c % size) + size) % size : min(max(c, 0), size - 1);
+}
+__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
+  return make_float4(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t));
+}
+// sampleMap + loadMapTexel (materialBuffer.wgsl.ts:314-348) for an ordinary
+// image: the coordinate uv * scale + offset is wrapped per axis, and the
+// texels around (wrapped * size - 0.5) are blended bilinearly in f32, taps
+// past the edge repeating or clamping by the axis' mode. (The reference
+// leaves interior footprints to the hardware sampler, whose weights carry
+// fewer fraction bits; its explicit four-tap form is the one used here for
+// every footprint.) A NaN coordinate (an infinite uv) samples at 0.
+__device__ float4 sample_map(const GpuScene &S, uint32_t index, float2 uv) {
+  const GpuTexture t = S.textures[index];
+  const uint32_t ms = (t.flags >> 1) & 3u, mt = (t.flags >> 3) & 3u;
+  float wu = wrap_coord(uv.x * t.sx + t.ox, ms), wv = wrap_coord(uv.y * t.sy + t.oy, mt);
+  wu = wu == wu ? wu : 0.0f;  // the wrapped value lies in [0, 1] unless it is NaN
+  wv = wv == wv ? wv : 0.0f;
+  const int W = (int)t.width, H = (int)t.height;
+  const float tx = wu * (float)W - 0.5f, ty = wv * (float)H - 0.5f;
+  const float fx = floorf(tx), fy = floorf(ty);
+  const int bx = (int)fx, by = (int)fy;
+  const float ax = tx - fx, ay = ty - fy;
+  const int x0 = map_tap(bx, W, ms), x1 = map_tap(bx + 1, W, ms);
+  const int y0 = map_tap(by, H, mt), y1 = map_tap(by + 1, H, mt);
+  return mix4(mix4(map_texel(S, t, x0, y0), map_texel(S, t, x1, y0), ax),
+              mix4(map_texel(S, t, x0, y1), map_texel(S, t, x1, y1), ax), ay);
+}
+// sRGBToLinear (materialBuffer.wgsl.ts:258-260); decodeMapColor (:262-265)
+// leaves RGBA32F (linear) textures as they are.
+__device__ __forceinline__ float srgb_to_linear(float c) {
+  return c < 0.04045f ? c * 0.0773993808f : mtxfm::powf(c * 0.9478672986f + 0.0521327014f, 2.4f);
+}
+__device__ __forceinline__ f3 map_color(const GpuScene &S, uint32_t index, float4 c) {
+  if (S.textures[index].flags & 1u) return f3{c.x, c.y, c.z};
+  return f3{srgb_to_linear(c.x), srgb_to_linear(c.y), srgb_to_linear(c.z)};
+}
+// Texture of slot k of material m (index + 1 stored; 0 = none), sampled at
+// its UV set (getMapUvSet, materialBuffer.wgsl.ts:303-305).
+__device__ __forceinline__ uint32_t map_slot(const float *__restrict__ m, int k) {
+  return (uint32_t)m[MTX_MAT_MAPS + k];
+}
+__device__ __forceinline__ float4 map_at(const GpuScene &S, uint32_t slot, float2 uv, float2 uv1) {
+  return sample_map(S, slot - 1u, (S.textures[slot - 1u].flags & 32u) ? uv1 : uv);
+}
+// getMaterialEmission (materialBuffer.wgsl.ts:351-372): the packed emission
+// times the emissive map; a coat weight map moves the coat tint, which the
+// packed emission then leaves out, to the sample.
+__device__ f3 material_emission(const GpuScene &S, const float *__restrict__ m, float2 uv, float2 uv1) {
+  f3 e{m[MTX_MAT_EMISSION], m[MTX_MAT_EMISSION + 1], m[MTX_MAT_EMISSION + 2]};
+  if (const uint32_t t = map_slot(m, MTX_MAP_EMISSIVE)) e = e * map_color(S, t - 1u, map_at(S, t, uv, uv1));
+  if (const uint32_t t = map_slot(m, MTX_MAP_COAT_WEIGHT)) {
+    const float w = clampf(m[MTX_MAT_COAT_WEIGHT], 0.0f, 1.0f) * map_at(S, t, uv, uv1).x;
+    const f3 c{sqrtf(mxf(m[MTX_MAT_COAT_COLOR], 0.0f)), sqrtf(mxf(m[MTX_MAT_COAT_COLOR + 1], 0.0f)),
+               sqrtf(mxf(m[MTX_MAT_COAT_COLOR + 2], 0.0f))};
+    e = e * f3{mixf(1.0f, c.x, w), mixf(1.0f, c.y, w), mixf(1.0f, c.z, w)};
+  }
+  return e;
+}
+
 // ---------------------------------------------------------------- materials
 struct Mat {
   f3 baseColor, specularColor, transmissionColor, coatColor, emission, fuzzColor, subsurfaceColor;
@@ -291,10 +366,43 @@ __device__ __forceinline__ void rough_alpha(float r, float aniso, float &ax, flo
   ay = mxf(kAlphaMin, c * at);
 }
 
-// kFull = false: the scene has no fuzz, subsurface or thin-walled material
-// (its kernels are compiled without those lobes).
+// Texture samples of getMaterial (materialBuffer.wgsl.ts:78-160), in its
+// order, on the clamped record values and before the coat broadening: the
+// base map (color; map alpha as transmission), transmission, metallic-
+// roughness, roughness, metalness, coat weight and roughness, fuzz color and
+// roughness, specular weight and color. Materials are opaque here, so the
+// alpha samples (alpha map, map alpha as coverage) have no effect.
+__device__ void apply_maps(const GpuScene &S, const float *__restrict__ m, float2 uv, float2 uv1, Mat &ma) {
+  ma.emission = material_emission(S, m, uv, uv1);
+  if (const uint32_t t = map_slot(m, MTX_MAP_BASE)) {
+    const float4 c = map_at(S, t, uv, uv1);
+    ma.baseColor = ma.baseColor * map_color(S, t - 1u, c);
+    if (m[MTX_MAT_MAP_ALPHA_USAGE] == 3.0f) ma.transmissionWeight = 1.0f - c.w;
+  }
+  if (const uint32_t t = map_slot(m, MTX_MAP_TRANSMISSION)) ma.transmissionWeight *= map_at(S, t, uv, uv1).x;
+  if (const uint32_t t = map_slot(m, MTX_MAP_METALLIC_ROUGHNESS)) {
+    const float4 c = map_at(S, t, uv, uv1);
+    ma.baseMetalness *= c.z;
+    ma.specularRoughness *= c.y;
+  }
+  if (const uint32_t t = map_slot(m, MTX_MAP_ROUGHNESS)) ma.specularRoughness *= map_at(S, t, uv, uv1).y;
+  if (const uint32_t t = map_slot(m, MTX_MAP_METALNESS)) ma.baseMetalness *= map_at(S, t, uv, uv1).z;
+  if (const uint32_t t = map_slot(m, MTX_MAP_COAT_WEIGHT)) ma.coatWeight *= map_at(S, t, uv, uv1).x;
+  if (const uint32_t t = map_slot(m, MTX_MAP_COAT_ROUGHNESS)) ma.coatRoughness *= map_at(S, t, uv, uv1).y;
+  if (const uint32_t t = map_slot(m, MTX_MAP_FUZZ_COLOR))
+    ma.fuzzColor = ma.fuzzColor * map_color(S, t - 1u, map_at(S, t, uv, uv1));
+  if (const uint32_t t = map_slot(m, MTX_MAP_FUZZ_ROUGHNESS)) ma.fuzzRoughness *= map_at(S, t, uv, uv1).w;
+  if (const uint32_t t = map_slot(m, MTX_MAP_SPECULAR_WEIGHT)) ma.specularWeight *= map_at(S, t, uv, uv1).w;
+  if (const uint32_t t = map_slot(m, MTX_MAP_SPECULAR_COLOR))
+    ma.specularColor = ma.specularColor * map_color(S, t - 1u, map_at(S, t, uv, uv1));
+}
+
+// kFull = false: the scene has no fuzz, subsurface, thin-walled or textured
+// material (its kernels are compiled without those lobes and the maps).
+// S: the scene whose maps apply at (uv, uv1), or null.
 template <bool kFull>
-__device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
+__device__ Mat load_material(const float *__restrict__ m, const float col[3], const GpuScene *S = nullptr,
+                             float2 uv = float2{0.0f, 0.0f}, float2 uv1 = float2{0.0f, 0.0f}) {
   Mat ma;
   ma.baseColor = f3{m[1], m[2], m[3]};
   ma.specularRoughness = m[10];
@@ -321,6 +429,7 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3]) {
   ma.subsurfaceColor = vmaxs(f3{m[MTX_MAT_SUBSURFACE_COLOR], m[MTX_MAT_SUBSURFACE_COLOR + 1],
                                 m[MTX_MAT_SUBSURFACE_COLOR + 2]}, 0.0f);
   ma.thinWalled = kFull && m[MTX_MAT_THIN_WALLED] > 0.5f;
+  if (kFull && S != nullptr && S->textures != nullptr) apply_maps(*S, m, uv, uv1, ma);
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
     ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
@@ -1058,7 +1167,10 @@ __device__ void sample_analytic(const float *__restrict__ L, f3 p, float sx, flo
   }
 }
 // sampleEmissiveTriangleLight (sampleLight.wgsl.ts:316-363): uniform point on
-// the world-space triangle, one-sided, pdf = selection pmf * d^2 / (area cos).
+// the world-space triangle, one-sided, pdf = selection pmf * d^2 / (area cos);
+// textured scenes evaluate getMaterialEmission at the point's interpolated
+// (unscaled) UVs.
+template <bool kFull>
 __device__ void sample_emissive(const GpuScene &S, uint32_t index, f3 p, float rx, float ry, LSample &r) {
   const float4 a = S.emissive[3 * index], b = S.emissive[3 * index + 1], c = S.emissive[3 * index + 2];
   const f3 p0 = f3of(a), p1 = f3of(b), p2 = f3of(c);
@@ -1074,10 +1186,18 @@ __device__ void sample_emissive(const GpuScene &S, uint32_t index, f3 p, float r
   const float cosLight = dot(normal, -r.direction);
   if (cosLight <= 0.0f) { r.emission = splat(0.0f); r.pdf = 0.0f; return; }
   const float *m = S.materials + (size_t)(uint32_t)c.w * MTX_MATERIAL_FLOATS;
-  r.emission = f3{m[23], m[24], m[25]};
+  if (kFull && S.emissiveUv != nullptr) {
+    const float4 u0 = S.emissiveUv[3 * index], u1 = S.emissiveUv[3 * index + 1], u2 = S.emissiveUv[3 * index + 2];
+    const float2 uv{bc.x * u0.x + bc.y * u0.z + bc.z * u1.x, bc.x * u0.y + bc.y * u0.w + bc.z * u1.y};
+    const float2 uv1{bc.x * u1.z + bc.y * u2.x + bc.z * u2.z, bc.x * u1.w + bc.y * u2.y + bc.z * u2.w};
+    r.emission = material_emission(S, m, uv, uv1);
+  } else {
+    r.emission = f3{m[23], m[24], m[25]};
+  }
   r.selPmf = a.w;
   r.pdf = a.w * (r.dist * r.dist) / mxf(b.w * cosLight, 1e-8f);
 }
+template <bool kFull>
 __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float rz, uint32_t cand, uint32_t count,
                              LSample &ls) {
   uint32_t cnt = max(count, 1u);
@@ -1105,7 +1225,7 @@ __device__ void sample_light(const GpuScene &S, f3 p, float rx, float ry, float 
     ls.pdf *= ls.selPmf;
   } else if (ch.type == 2) {
     ls.sourceType = 1;
-    sample_emissive(S, (uint32_t)ch.index, p, rx, ry, ls);
+    sample_emissive<kFull>(S, (uint32_t)ch.index, p, rx, ry, ls);
   }
 }
 // intersectSphere (intersect.wgsl.ts:924-945).
@@ -1243,9 +1363,38 @@ struct Surf {
   int32_t tri, slot;
   uint32_t material;
   float color[3];
+  float2 uv, uv1;  // texture coordinates of textured scenes (times the material's UV scale)
   bool hit, albedoSampled;
 };
 
+// getMatNormal (materialBuffer.wgsl.ts:377-415): the normal map's tangent-
+// space normal in the cotangent frame of UV0 (dp1 = p0 - p2, dp2 = p1 - p2,
+// duv the raw vertex UV differences), object space.
+__device__ f3 mapped_normal(const GpuScene &S, uint32_t slot, const float *__restrict__ m, float2 uv, f3 normal,
+                            f3 dp1, f3 dp2, float2 duv1, float2 duv2) {
+  const f3 dp2perp = cross(dp2, normal), dp1perp = cross(normal, dp1);
+  f3 dpdu = dp2perp * duv1.x + dp1perp * duv2.x;
+  f3 dpdv = dp2perp * duv1.y + dp1perp * duv2.y;
+  const float g2 = mxf(dot(dpdu, dpdu), dot(dpdv, dpdv));
+  if (g2 > 1e-12f) {
+    const float inv = 1.0f / sqrtf(g2);
+    dpdu = dpdu * inv;
+    dpdv = dpdv * inv;
+  } else {
+    const f3 ref = fabsf(normal.z) < 0.999f ? f3{0.0f, 0.0f, 1.0f} : f3{0.0f, 1.0f, 0.0f};
+    dpdu = normalize(cross(ref, normal));
+    dpdv = cross(normal, dpdu);
+  }
+  const float4 c = sample_map(S, slot - 1u, uv);
+  f3 n{2.0f * c.x - 1.0f, 2.0f * c.y - 1.0f, 2.0f * c.z - 1.0f};
+  n = normalize(f3{n.x * m[MTX_MAT_NORMAL_SCALE], n.y * m[MTX_MAT_NORMAL_SCALE + 1], n.z});
+  return normalize(dpdu * n.x + dpdv * n.y + normal * n.z);
+}
+
+// completeSurfaceInteraction + fetchAttributes (intersect.wgsl.ts:1233-1419,
+// 1539-1601). kFull: textured scenes also interpolate the UV sets, scale
+// them by the material's UV scale and apply its normal map.
+template <bool kFull = true>
 __device__ void complete_surface(const GpuScene &S, const TraceHit &h, Surf &si) {
   const ShadeSlot &sh = S.shade[h.slot];
   const InstSlot &is = S.accel->slots[h.slot];
@@ -1286,8 +1435,31 @@ __device__ void complete_surface(const GpuScene &S, const TraceHit &h, Surf &si)
   si.slot = h.slot;
   si.material = sh.material;
   si.color[0] = sh.color[0]; si.color[1] = sh.color[1]; si.color[2] = sh.color[2];
+  si.uv = si.uv1 = float2{0.0f, 0.0f};
+  f3 shading = normal;
+  if (kFull && S.textures != nullptr) {
+    const float *__restrict__ m = S.materials + (size_t)sh.material * MTX_MATERIAL_FLOATS;
+    float2 t0{0.0f, 0.0f}, t1{0.0f, 0.0f}, t2{0.0f, 0.0f};
+    if (sh.uvs != nullptr) {
+      t0 = float2{sh.uvs[i0 * 2], sh.uvs[i0 * 2 + 1]};
+      t1 = float2{sh.uvs[i1 * 2], sh.uvs[i1 * 2 + 1]};
+      t2 = float2{sh.uvs[i2 * 2], sh.uvs[i2 * 2 + 1]};
+      si.uv = float2{b.x * t0.x + b.y * t1.x + b.z * t2.x, b.x * t0.y + b.y * t1.y + b.z * t2.y};
+    }
+    if (sh.uvs1 != nullptr) {
+      const float2 s0{sh.uvs1[i0 * 2], sh.uvs1[i0 * 2 + 1]}, s1{sh.uvs1[i1 * 2], sh.uvs1[i1 * 2 + 1]};
+      const float2 s2{sh.uvs1[i2 * 2], sh.uvs1[i2 * 2 + 1]};
+      si.uv1 = float2{b.x * s0.x + b.y * s1.x + b.z * s2.x, b.x * s0.y + b.y * s1.y + b.z * s2.y};
+    }
+    const float su = m[MTX_MAT_UV_SCALE], sv = m[MTX_MAT_UV_SCALE + 1];
+    si.uv = float2{si.uv.x * su, si.uv.y * sv};
+    si.uv1 = float2{si.uv1.x * su, si.uv1.y * sv};
+    if (const uint32_t t = map_slot(m, MTX_MAP_NORMAL))
+      shading = mapped_normal(S, t, m, si.uv, normal, p0 - p2, p1 - p2, float2{t0.x - t2.x, t0.y - t2.y},
+                              float2{t1.x - t2.x, t1.y - t2.y});
+  }
   si.faceNormal = normalize(xform_normal_transposed(w2l, h.normal));
-  si.normal = normalize(xform_normal_transposed(w2l, normal));
+  si.normal = normalize(xform_normal_transposed(w2l, shading));
   si.shadowOffset = xform(l2w, smooth * hh, 0.0f);
   si.position = xform(l2w, position, 1.0f);
 }
@@ -1429,8 +1601,8 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
   si.albedoSampled = (pr.flags & kAlbedoSampled) != 0u;
   Mat ma;
   if (h.hit) {
-    complete_surface(S, h, si);
-    ma = load_material<kFull>(S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS, si.color);
+    complete_surface<kFull>(S, h, si);
+    ma = load_material<kFull>(S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS, si.color, &S, si.uv, si.uv1);
   }
   pr.curTri = si.tri;
   pr.curSlot = si.slot;
@@ -1521,7 +1693,8 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       if (cosLight <= 0.0f && S.materials[(size_t)si.material * MTX_MATERIAL_FLOATS + MTX_MAT_DOUBLE_SIDED] < 0.5f) {
         w = 0.0f;
       } else if (!((pr.flags & kFromEye) | (S.lightWeightTotal <= 0.0f))) {
-        const float lum = dot(ma.emission, f3{0.2126f, 0.7152f, 0.0722f});
+        const float *em = S.materials + (size_t)si.material * MTX_MATERIAL_FLOATS + MTX_MAT_EMISSION;
+        const float lum = dot(f3{em[0], em[1], em[2]}, f3{0.2126f, 0.7152f, 0.0722f});
         if (!((lum <= 0.0f) | (cosLight <= 0.0f))) {
           const float hitDistance = len(si.position - pr.ro);
           const float lightPdf = lum / S.lightWeightTotal * hitDistance * hitDistance / mxf(cosLight, 1e-8f);
@@ -1580,7 +1753,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
         float a, b;
         smp.s2(smp.ldim(ev, 1u), a, b);
         float c = smp.s1(smp.ldim(ev, 0u));
-        sample_light(S, si.position, a, b, c, ev, cc, ls);
+        sample_light<kFull>(S, si.position, a, b, c, ev, cc, ls);
         ldir = ls.direction;
         if ((ls.pdf <= 0.0f) | (ls.selPmf <= 0.0f) | (maxc(ls.emission) <= 0.0f)) continue;
         if ((dot(ff, ldir) <= 0.0f) & (ma.transmissionWeight <= 0.0f) & (ma.subsurfaceWeight <= 0.0f)) continue;

@@ -845,7 +845,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
         gs = graph->own;  // blocking stream: ordered with the legacy null stream
       }
       uint64_t key = 1469598103934665603ull;
-      static_assert(sizeof(GpuScene) == 120 && sizeof(RenderParams) % 4 == 0, "key layouts");
+      static_assert(sizeof(GpuScene) == 152 && sizeof(RenderParams) % 4 == 0, "key layouts");
       key = fnv(key, &L.one.scene, sizeof L.one.scene);
       key = fnv(key, &L.one.params, sizeof L.one.params);
       const void *ptrs[8] = {L.one.acc.planes, L.one.acc.count, L.one.acc.rays, L.one.acc.work, W.o, W.counts,

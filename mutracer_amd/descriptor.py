@@ -6,9 +6,10 @@ records of native/include/mutracer-scene.h, word offsets of
 native/include/mutracer-layout.h). `encode` writes a `scene.Scene` as one
 geometry per mesh, one mesh record per instance, one material record per
 material (OpenPBR parameters outside the basic record go in extension entries),
-one light record per analytic light, a perspective camera and, when the scene
-has one, the environment as an RGBA32F texture used for both lighting and the
-visible background.
+one light record per analytic light, a perspective camera, the material
+textures (their table indices are the materials' slot values) and, when the
+scene has one, the environment as an RGBA32F texture after them, used for
+both lighting and the visible background.
 
 Every value is stored as f32: instance transforms, light frames and the camera
 are rounded once here. `f32_scene` returns the Scene those rounded values
@@ -44,6 +45,13 @@ EXT = dict(base_weight=1, base_diffuse_roughness=3, specular_weight=5, specular_
            transmission_color=11, subsurface_weight=17, subsurface_color=18, fuzz_weight=22, fuzz_color=23,
            fuzz_roughness=24, coat_color=26, coat_ior=29, coat_darkening=30, double_sided=38)
 _DEFAULT = S.Material()
+# Texture slots carried as extension entries (the basic record holds map,
+# normal, metallic-roughness and emissive), and the native enums.
+EXT_MAPS = dict(alpha_map=40, roughness_map=47, metalness_map=48, transmission_map=49, coat_weight_map=64,
+                coat_roughness_map=65, fuzz_color_map=66, fuzz_roughness_map=67, specular_weight_map=68,
+                specular_color_map=69)
+NATIVE_ALPHA_MODE = dict(auto=0, opaque=1, mask=2, blend=3)
+NATIVE_MAP_ALPHA_USAGE = dict(opaque=0, alpha=1, transmission=2)
 
 LIGHT_TYPE = {S.LIGHT_RECT: 0, S.LIGHT_SPHERE: 1, S.LIGHT_DIRECTIONAL: 2, S.LIGHT_SPOT: 3, S.LIGHT_DISK: 4,
               S.LIGHT_CYLINDER: 5}
@@ -133,16 +141,40 @@ def _material_record(m, w):
     f[3], f[4], f[5], f[6] = m.specular_roughness, m.base_metalness, m.transmission_weight, m.specular_ior
     em = np.asarray(m.emission, np.float64)
     if np.any(em != 0):
-        if m.coat_weight > 0:  # materialEmission multiplies the coat tint back in
+        if m.coat_weight > 0 and m.coat_weight_map is None:  # materialEmission multiplies the coat tint back in
             tint = 1.0 + min(m.coat_weight, 1.0) * (np.sqrt(np.maximum(np.asarray(m.coat_color, np.float64), 0)) - 1)
             em = em / tint
         f[7:10], f[10] = em, 1.0
     else:
         f[7:10], f[10] = (1.0, 1.0, 1.0), 0.0
-    f[11], f[12], f[13] = m.coat_weight, m.coat_roughness, 1.0
+    f[11], f[12], f[13] = m.coat_weight, m.coat_roughness, m.opacity
     rec[14] = int(bool(m.thin_walled))
-    rec[15:19] = NO_TEXTURE
+    for k, name in enumerate(("map", "normal_map", "metallic_roughness_map", "emissive_map")):
+        t = getattr(m, name)
+        rec[15 + k] = NO_TEXTURE if t is None else int(t)
     ext = []
+
+    def add(pid, u=None, f=None):
+        e = np.zeros(4, np.uint32)
+        e[0] = pid
+        if u is not None:
+            e[1] = u
+        if f is not None:
+            e.view(np.float32)[1:1 + len(f)] = f
+        ext.append(e)
+    for name, pid in EXT_MAPS.items():
+        if getattr(m, name) is not None:
+            add(pid, u=int(getattr(m, name)))
+    if m.alpha_mode != "auto":
+        add(41, u=NATIVE_ALPHA_MODE[m.alpha_mode])
+    if m.alpha_cutoff != 0:
+        add(42, f=[m.alpha_cutoff])
+    if m.map_alpha_usage != "opaque":
+        add(54, u=NATIVE_MAP_ALPHA_USAGE[m.map_alpha_usage])
+    if tuple(m.normal_scale) != (1.0, 1.0):
+        add(45, f=list(m.normal_scale))
+    if tuple(m.uv_scale) != (1.0, 1.0):
+        add(56, f=list(m.uv_scale))
     for name, pid in EXT.items():
         v, d = getattr(m, name), getattr(_DEFAULT, name)
         if v == d:
@@ -174,6 +206,10 @@ def encode(scene, width, height):
         g[2] = w.append(pos)
         g[3] = w.append(idx)
         g[4] = w.append(np.asarray(mesh["normals"], np.float32))
+        if mesh.get("uvs") is not None:
+            g[5] = w.append(np.asarray(mesh["uvs"], np.float32))
+        if mesh.get("uvs1") is not None:
+            g[6] = w.append(np.asarray(mesh["uvs1"], np.float32))
         geos.append(g)
     meshes = []
     for i in range(len(scene.mesh_index)):
@@ -184,6 +220,15 @@ def encode(scene, width, height):
     mats = [_material_record(m, w) for m in scene.materials]
     lights = [_light_record(l) for l in scene.lights]
     textures = []
+    for tex in scene.textures:  # material textures first: the slots index them directly
+        px = tex.storage_pixels()
+        t = np.zeros(TEXTURE_WORDS, np.uint32)
+        t[0], t[1], t[2] = px.shape[1], px.shape[0], tex.storage
+        t[3] = w.append(px.reshape(-1).view(np.uint32))
+        t[4], t[5] = int(tex.flip_y), int(tex.uv_channel)
+        t.view(np.float32)[6:10] = [*tex.uv_scale, *tex.uv_offset]
+        t[10], t[11] = tex.wrap_code(tex.wrap_s), tex.wrap_code(tex.wrap_t)
+        textures.append(t)
     if scene.env is not None:
         env = np.ascontiguousarray(scene.env, np.float32)
         t = np.zeros(TEXTURE_WORDS, np.uint32)
@@ -212,8 +257,8 @@ def encode(scene, width, height):
     hdr[H_LIGHT] = w.append(np.concatenate(lights)) if lights else HEADER_WORDS
     hdr[H_TEXTURE] = w.append(np.concatenate(textures)) if textures else HEADER_WORDS
     hdr[H_CAMERA] = w.append(cam)
-    if textures:
-        hdr[H_ENV_TEXTURE] = hdr[H_BG_TEXTURE] = 1
+    if scene.env is not None:
+        hdr[H_ENV_TEXTURE] = hdr[H_BG_TEXTURE] = len(textures)
     return w.finish().tobytes()
 
 

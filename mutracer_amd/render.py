@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi_ext import (AdaptiveOptions, CameraDesc, DisplaySettings, JnlmOptions, MeshDesc, SceneDesc,
+from .abi_ext import (AdaptiveOptions, CameraDesc, DisplaySettings, JnlmOptions, MeshDesc, SceneDesc, TextureDesc,
                       TargetSettings, WavefrontTuning)
 from .sampling import sampling_pattern_code
 
@@ -40,9 +40,17 @@ class RenderScene:
             pos = np.ascontiguousarray(m["positions"], np.float32)
             nrm = np.ascontiguousarray(m["normals"], np.float32)
             idx = np.ascontiguousarray(m["indices"], np.uint32)
-            self._keep += [pos, nrm, idx]
+            uv = [None if m.get(k) is None else np.ascontiguousarray(m[k], np.float32) for k in ("uvs", "uvs1")]
+            self._keep += [pos, nrm, idx] + uv
             meshes[i] = MeshDesc(C.sizeof(MeshDesc), pos.size // 3, idx.size // 3, 0, _P(pos).value,
-                                 _P(nrm).value, _P(idx).value)
+                                 _P(nrm).value, _P(idx).value, *[None if u is None else _P(u).value for u in uv])
+        textures = (TextureDesc * max(1, len(scene.textures)))()
+        for i, t in enumerate(scene.textures):
+            px = t.storage_pixels()
+            self._keep.append(px)
+            textures[i] = TextureDesc(C.sizeof(TextureDesc), px.shape[1], px.shape[0], t.storage, int(t.flip_y),
+                                      int(t.uv_channel), t.wrap_code(t.wrap_s), t.wrap_code(t.wrap_t),
+                                      (C.c_float * 2)(*t.uv_scale), (C.c_float * 2)(*t.uv_offset), _P(px).value)
         n = len(scene.mesh_index)
         inst = (abi.InstanceDesc * max(1, n))()
         for i in range(n):
@@ -56,13 +64,14 @@ class RenderScene:
         mats = np.ascontiguousarray(scene.packed_materials(), np.float32)
         lights = np.ascontiguousarray(scene.packed_lights(), np.float32)
         env = None if scene.env is None else np.ascontiguousarray(scene.env, np.float32)
-        self._keep += [mats, lights, env, meshes, inst]
+        self._keep += [mats, lights, env, meshes, inst, textures]
         desc = SceneDesc(C.sizeof(SceneDesc), len(scene.meshes), C.cast(meshes, C.c_void_p).value, n,
                          C.cast(inst, C.c_void_p).value, mats.shape[0], _P(mats).value, lights.shape[0],
                          _P(lights).value if lights.size else None,
                          0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
                          None if env is None else _P(env).value, blas_bins, tlas_bins,
-                         abi.MTX_ACCEL_PER_INSTANCE_BOUNDS if per_instance_bounds else 0)
+                         abi.MTX_ACCEL_PER_INSTANCE_BOUNDS if per_instance_bounds else 0,
+                         len(scene.textures), C.cast(textures, C.c_void_p).value if scene.textures else None)
         self.handle = C.c_uint64(0)
         err = abi.errbuf()
         abi.check(L.mtx_scene_create(C.byref(desc), C.byref(self.handle), err, 512), err)

@@ -15,13 +15,22 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# Packed material layout (MTX_MATERIAL_FLOATS = 48 floats), see include/mtx.h.
-MATERIAL_FLOATS = 48
+# Packed material layout (MTX_MATERIAL_FLOATS = 64 floats), see include/mtx.h.
+MATERIAL_FLOATS = 64
 MAT_FIELDS = dict(base_weight=0, base_color=1, base_diffuse_roughness=4, base_metalness=5, specular_weight=6,
                   specular_color=7, specular_roughness=10, specular_ior=11, transmission_weight=12,
                   transmission_color=13, coat_weight=16, coat_color=17, coat_roughness=20, coat_ior=21,
                   coat_darkening=22, emission=23, double_sided=26, specular_anisotropy=27, fuzz_weight=28,
-                  fuzz_color=29, fuzz_roughness=32, subsurface_weight=33, subsurface_color=34, thin_walled=37)
+                  fuzz_color=29, fuzz_roughness=32, subsurface_weight=33, subsurface_color=34, thin_walled=37,
+                  opacity=38, alpha_cutoff=40, normal_scale=42, uv_scale=44)
+MAT_ALPHA_MODE, MAT_MAP_ALPHA_USAGE, MAT_MAPS = 39, 41, 46
+# Texture slots in packing order (MTX_MAP_*; materialTextureSlots.ts:8-140).
+MAP_SLOTS = ("map", "alpha_map", "transmission_map", "metallic_roughness_map", "roughness_map", "metalness_map",
+             "coat_weight_map", "coat_roughness_map", "fuzz_color_map", "fuzz_roughness_map", "specular_weight_map",
+             "specular_color_map", "emissive_map", "normal_map")
+ALPHA_MODES = dict(opaque=0, mask=1, blend=2)               # materialConstants.ts:12-16
+MAP_ALPHA_USAGE = dict(opaque=0, alpha=2, transmission=3)   # materialConstants.ts:2
+WRAP_MODES = {"repeat": 0, "clamp-to-edge": 1, "mirrored-repeat": 2}
 
 
 @dataclass
@@ -49,18 +58,92 @@ class Material:
     subsurface_weight: float = 0.0
     subsurface_color: tuple = (0.8, 0.8, 0.8)
     thin_walled: bool = False
-    emission: tuple = (0.0, 0.0, 0.0)
+    emission: tuple = (0.0, 0.0, 0.0)   # materialEmission(): color * luminance, coat-tinted unless coat_weight_map
     double_sided: bool = False
+    opacity: float = 1.0
+    alpha_mode: str = "auto"             # auto, opaque, mask, blend
+    alpha_cutoff: float = 0.0
+    map_alpha_usage: str = "opaque"      # what the base map's alpha drives: opaque, alpha, transmission
+    normal_scale: tuple = (1.0, 1.0)
+    uv_scale: tuple = (1.0, 1.0)         # uvProjectionScale
+    # Texture slots: scene texture indices or None.
+    map: int | None = None
+    alpha_map: int | None = None
+    transmission_map: int | None = None
+    metallic_roughness_map: int | None = None
+    roughness_map: int | None = None
+    metalness_map: int | None = None
+    coat_weight_map: int | None = None
+    coat_roughness_map: int | None = None
+    fuzz_color_map: int | None = None
+    fuzz_roughness_map: int | None = None
+    specular_weight_map: int | None = None
+    specular_color_map: int | None = None
+    emissive_map: int | None = None
+    normal_map: int | None = None
+
+    def resolved_alpha_mode(self):
+        """resolveMaterialAlphaMode (MaterialState.ts:251-264)."""
+        if self.alpha_mode != "auto":
+            return self.alpha_mode
+        if self.alpha_cutoff > 0:
+            return "mask"
+        if self.opacity < 1 or self.map_alpha_usage == "alpha" or self.alpha_map is not None:
+            return "blend"
+        return "opaque"
 
     def pack(self):
         p = np.zeros(MATERIAL_FLOATS, np.float32)
         for k, off in MAT_FIELDS.items():
             v = getattr(self, k)
             if isinstance(v, tuple):
-                p[off:off + 3] = v
+                p[off:off + len(v)] = v
             else:
                 p[off] = float(v)
+        p[MAT_ALPHA_MODE] = ALPHA_MODES[self.resolved_alpha_mode()]
+        p[MAT_MAP_ALPHA_USAGE] = MAP_ALPHA_USAGE[self.map_alpha_usage]
+        for k, name in enumerate(MAP_SLOTS):
+            t = getattr(self, name)
+            p[MAT_MAPS + k] = 0 if t is None else int(t) + 1
         return p
+
+    def has_maps(self):
+        return any(getattr(self, name) is not None for name in MAP_SLOTS)
+
+
+@dataclass
+class Texture:
+    """A material texture (mt_texture_v4, mutracer-scene.h:87-95, without UDIM).
+
+    pixels: (H, W, 4) uint8 (RGBA8: color slots decode it from sRGB, data
+    slots read it as linear) or float32 (RGBA32F, linear); row 0 is sampled
+    at v = 0 unless flip_y. A sample at uv reads uv * uv_scale + uv_offset,
+    wrapped per axis (repeat, clamp-to-edge, mirrored-repeat), bilinearly.
+    """
+    pixels: np.ndarray
+    flip_y: bool = False
+    uv_channel: int = 0
+    uv_scale: tuple = (1.0, 1.0)
+    uv_offset: tuple = (0.0, 0.0)
+    wrap_s: str = "repeat"
+    wrap_t: str = "repeat"
+
+    @property
+    def storage(self):
+        return 1 if np.asarray(self.pixels).dtype == np.float32 else 0
+
+    def storage_pixels(self):
+        """Row-major pixels as the C ABI takes them: one little-endian word
+        per RGBA8 pixel (R in the low byte) or four floats per pixel."""
+        px = np.asarray(self.pixels)
+        if px.dtype == np.float32:
+            return np.ascontiguousarray(px.reshape(px.shape[0], px.shape[1], 4))
+        px = np.ascontiguousarray(px.astype(np.uint8).reshape(px.shape[0], px.shape[1], 4))
+        return px.view("<u4").reshape(px.shape[0], px.shape[1])
+
+    @staticmethod
+    def wrap_code(mode):
+        return WRAP_MODES[mode] if isinstance(mode, str) else int(mode)
 
 
 @dataclass
@@ -281,14 +364,19 @@ class Scene:
     lights: list = field(default_factory=list)
     env: np.ndarray | None = None                      # (H, W, 4) f32
     camera: Camera | None = None
+    textures: list = field(default_factory=list)      # Texture, indexed by the materials' map slots
 
-    def add_mesh(self, positions, indices, normals=None):
+    def add_mesh(self, positions, indices, normals=None, uvs=None, uvs1=None):
         positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         indices = np.ascontiguousarray(indices, np.uint32).reshape(-1)
         if normals is None:
             normals = vertex_normals(positions, indices)
-        self.meshes.append(dict(positions=positions, indices=indices,
-                                normals=np.ascontiguousarray(normals, np.float32).reshape(-1, 3)))
+        mesh = dict(positions=positions, indices=indices,
+                    normals=np.ascontiguousarray(normals, np.float32).reshape(-1, 3))
+        for key, value in (("uvs", uvs), ("uvs1", uvs1)):
+            if value is not None:
+                mesh[key] = np.ascontiguousarray(value, np.float32).reshape(-1, 2)
+        self.meshes.append(mesh)
         return len(self.meshes) - 1
 
     def add_instance(self, mesh, transform, material):
@@ -392,6 +480,92 @@ def cornell_box(width=256, height=256):
     s.add_instance(s.add_mesh(*short), m_short.T.reshape(16), 0)
     s.add_instance(s.add_mesh(*tall), m_tall.T.reshape(16), 0)
     s.lights = [RectLight(position=(0.0, 1.999, 0.0), u=(0.25, 0.0, 0.0), v=(0.0, 0.0, 0.25), intensity=15.0)]
+    s.camera = Camera(eye=(0.0, 1.0, 3.94), target=(0.0, 1.0, 0.0), fov=39.3)
+    return s
+
+
+QUAD_UVS = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+
+
+def procedural_textures(seed=7):
+    """Small procedural images for textured scenes: an sRGB checker (RGBA8),
+    metallic-roughness data (RGBA32F: g roughness ramp, b metal stripes), a
+    bump normal map (RGBA32F, tangent space), an sRGB emissive pattern
+    (RGBA8) and a coat-weight / transmission mask (RGBA8)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:64, 0:64]
+    checker = np.where(((x // 8) + (y // 8)) % 2 == 0, 1.0, 0.0)[..., None]
+    c0 = np.array([230, 200, 120]) * checker + np.array([40, 70, 160]) * (1 - checker)
+    checker_rgba = np.concatenate([c0, 255 * np.ones((64, 64, 1))], 2).astype(np.uint8)
+    yy, xx = np.mgrid[0:32, 0:32]
+    mr = np.zeros((32, 32, 4), np.float32)
+    mr[..., 1] = 0.15 + 0.8 * xx / 31.0
+    mr[..., 2] = ((yy // 4) % 2).astype(np.float32)
+    mr[..., 3] = 1.0
+    hy, hx = np.mgrid[0:64, 0:64] / 64.0
+    h = np.sin(2 * np.pi * 3 * hx) * np.cos(2 * np.pi * 2 * hy)
+    gx, gy = np.gradient(h)
+    n = np.stack([-2.0 * gx * 8, -2.0 * gy * 8, np.ones_like(h)], 2)
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    normal_map = np.concatenate([n * 0.5 + 0.5, np.ones((64, 64, 1))], 2).astype(np.float32)
+    em = rng.integers(0, 256, (16, 16, 4)).astype(np.uint8)
+    em[..., 3] = 255
+    my, mx = np.mgrid[0:24, 0:24]
+    mask = np.zeros((24, 24, 4), np.uint8)
+    mask[..., 0] = (255 * ((mx + my) % 6 < 3)).astype(np.uint8)
+    mask[..., 1] = (40 + 8 * mx).astype(np.uint8)
+    mask[..., 2] = (255 * (my > 11)).astype(np.uint8)
+    mask[..., 3] = (60 + 8 * my).astype(np.uint8)
+    return [Texture(checker_rgba, uv_scale=(4.0, 4.0), uv_offset=(0.125, 0.0)),
+            Texture(mr, wrap_s="clamp-to-edge", wrap_t="mirrored-repeat"),
+            Texture(normal_map, wrap_s="mirrored-repeat", uv_scale=(1.5, 1.0)),
+            Texture(em, flip_y=True, uv_channel=1),
+            Texture(mask, wrap_s="clamp-to-edge", wrap_t="clamp-to-edge", uv_scale=(1.2, 1.2),
+                    uv_offset=(-0.1, -0.1))]
+
+
+def textured_room(width=256, height=256):
+    """The Cornell geometry with material textures on every slot kind: a
+    checker base map on the floor, a bump normal map on the back wall, a
+    metallic-roughness map on the tall box, a transmission / coat-weight /
+    specular-color / fuzz-color mask on the short box, and a ceiling panel
+    whose emission is textured (an emissive-triangle light with an emissive
+    map and a coat weight map, sampled on UV1), besides the rect light."""
+    s = Scene()
+    s.textures = procedural_textures()
+    floor = Material(base_color=(0.9, 0.9, 0.9), specular_roughness=0.6, map=0)
+    wall = Material(base_color=(0.73, 0.73, 0.73), normal_map=2, normal_scale=(0.8, 1.2), uv_scale=(2.0, 1.0))
+    red = Material(base_color=(0.65, 0.05, 0.05), roughness_map=1)
+    green = Material(base_color=(0.12, 0.45, 0.15), specular_color_map=0, specular_weight_map=4)
+    metal = Material(base_color=(0.9, 0.7, 0.5), base_metalness=1.0, specular_roughness=1.0,
+                     metallic_roughness_map=1, metalness_map=1, coat_weight=0.4, coat_roughness_map=4)
+    short = Material(base_color=(0.8, 0.8, 0.8), transmission_weight=0.8, transmission_map=4, specular_roughness=0.2,
+                     fuzz_weight=0.5, fuzz_color_map=0, fuzz_roughness_map=4, coat_weight=0.5, coat_weight_map=4,
+                     map=4, map_alpha_usage="transmission")
+    panel = Material(base_color=(0.2, 0.2, 0.2), emission=(3.0, 2.5, 2.0), emissive_map=3, coat_weight=0.6,
+                     coat_color=(0.9, 0.6, 0.3), coat_weight_map=4, double_sided=True)
+    s.materials = [floor, wall, red, green, metal, short, panel]
+    walls = [
+        ([(-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1)], 0),
+        ([(-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)], 1),
+        ([(-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)], 1),
+        ([(-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1)], 2),
+        ([(1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)], 3),
+    ]
+    for pts, mat in walls:
+        p, i, n = quad(*pts)
+        s.add_instance(s.add_mesh(p, i, n, uvs=QUAD_UVS), np.eye(4).reshape(16), mat)
+    box_uv = np.tile(QUAD_UVS, (6, 1))
+    m_short = rot_y(-18)
+    m_short[:3, 3] = (0.33, 0.0, 0.3)
+    m_tall = rot_y(20)
+    m_tall[:3, 3] = (-0.33, 0.0, -0.3)
+    s.add_instance(s.add_mesh(*box_mesh(0.6, 0.6, 0.6), uvs=box_uv, uvs1=box_uv[:, ::-1] * 0.5),
+                   m_short.T.reshape(16), 5)
+    s.add_instance(s.add_mesh(*box_mesh(0.6, 1.2, 0.6), uvs=box_uv * 2.0), m_tall.T.reshape(16), 4)
+    p, i, n = quad((-0.6, 1.95, 0.5), (-0.2, 1.95, 0.5), (-0.2, 1.95, 0.1), (-0.6, 1.95, 0.1))
+    s.add_instance(s.add_mesh(p, i, n, uvs=QUAD_UVS, uvs1=QUAD_UVS * 1.5 - 0.25), np.eye(4).reshape(16), 6)
+    s.lights = [RectLight(position=(0.3, 1.999, -0.2), u=(0.25, 0.0, 0.0), v=(0.0, 0.0, 0.25), intensity=12.0)]
     s.camera = Camera(eye=(0.0, 1.0, 3.94), target=(0.0, 1.0, 0.0), fov=39.3)
     return s
 

@@ -523,12 +523,73 @@ def _declare_render2(L):
     L.orc_set_energy_lut.argtypes = [vp]
     L.orc_render.restype = C.c_uint64
     L.orc_render.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, u32,
-                             u32, u32, vp, vp, vp, u32]
+                             u32, u32, vp, vp, vp, u32, vp]
     L.orc_emissive_triangles.restype = u32
-    L.orc_emissive_triangles.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32]
+    L.orc_emissive_triangles.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32, vp]
     L.orc_path_draws.restype = u32
     L.orc_path_draws.argtypes = [vp, vp, vp, u32, vp, u32, vp, u32, u32, vp, vp, C.c_float, u32, u32, u32, vp, u32,
-                                 vp, vp, u32]
+                                 vp, vp, u32, vp]
+    L.orc_texset_create.restype = vp
+    L.orc_texset_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp]
+    L.orc_texset_destroy.argtypes = [vp]
+    L.orc_sample_map.argtypes = [vp, u32, u32, vp, vp, vp]
+
+
+class TextureSet:
+    """The oracle's copy of a scene's material textures and vertex UVs
+    (orc_texset_create): textures are mutracer_amd.scene.Texture objects;
+    uvs / uvs1 are per global vertex (zeros where a mesh has none);
+    emissive_uv holds 12 floats per emissive triangle."""
+
+    def __init__(self, textures, uvs=None, uvs1=None, emissive_uv=None):
+        L = lib()
+        _declare_render2(L)
+        n = len(textures)
+        desc = np.zeros((max(n, 1), 11), np.uint32)
+        pixels = (C.c_void_p * max(n, 1))()
+        keep = []
+        for i, t in enumerate(textures):
+            px = t.storage_pixels()
+            keep.append(px)
+            f = np.array([*t.uv_scale, *t.uv_offset], np.float32).view(np.uint32)
+            desc[i] = [px.shape[1], px.shape[0], t.storage, int(t.flip_y), int(t.uv_channel),
+                       t.wrap_code(t.wrap_s), t.wrap_code(t.wrap_t), *f]
+            pixels[i] = px.ctypes.data
+        uvs = np.ascontiguousarray(np.zeros((0, 2)) if uvs is None else uvs, np.float32)
+        uvs1 = np.ascontiguousarray(np.zeros_like(uvs) if uvs1 is None else uvs1, np.float32)
+        eu = np.ascontiguousarray(np.zeros((0, 12)) if emissive_uv is None else emissive_uv, np.float32)
+        self.h = L.orc_texset_create(n, _p(desc), pixels, uvs.shape[0], _p(uvs), _p(uvs1), eu.shape[0], _p(eu))
+
+    def sample(self, index, uv):
+        """sampleMap of texture `index` at uv (n, 2): filtered RGBA (n, 4)
+        and decoded color (n, 3)."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        rgba = np.zeros((uv.shape[0], 4), np.float32)
+        dec = np.zeros((uv.shape[0], 3), np.float32)
+        lib().orc_sample_map(self.h, index, uv.shape[0], _p(uv), _p(rgba), _p(dec))
+        return rgba, dec
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_texset_destroy(self.h)
+            self.h = None
+
+
+def scene_uvs(scene):
+    """Per-global-vertex UV0 / UV1 of a scene's meshes in mesh order (zeros
+    where a mesh has none), and the emissive variant of UV1 (UV0 where a mesh
+    has no UV1, lightSampling.ts:262-263)."""
+    uv, uv1, uv1e = [], [], []
+    for m in scene.meshes:
+        nv = np.asarray(m["positions"]).reshape(-1, 3).shape[0]
+        z = np.zeros((nv, 2), np.float32)
+        a = z if m.get("uvs") is None else np.asarray(m["uvs"], np.float32).reshape(-1, 2)
+        b = z if m.get("uvs1") is None else np.asarray(m["uvs1"], np.float32).reshape(-1, 2)
+        uv.append(a)
+        uv1.append(b)
+        uv1e.append(a if m.get("uvs1") is None else b)
+    cat = lambda x: np.concatenate(x) if x else np.zeros((0, 2), np.float32)  # noqa: E731
+    return cat(uv), cat(uv1), cat(uv1e)
 
 
 def _declare_bsdf(L):
@@ -602,9 +663,10 @@ def coat_energy(material, cosine):
     return float(lib().orc_coat_energy(_p(m), float(cosine)))
 
 
-def emissive_triangles(scene):
+def emissive_triangles(scene, with_corners=False):
     """World-space emissive triangles of a mutracer_amd.scene.Scene: (n, 12)
-    f32 records (p0, p1, p2, area, material, 0) and their f64 alias weights."""
+    f32 records (p0, p1, p2, area, material, 0) and their f64 alias weights
+    (and the global vertex index of each corner)."""
     L = lib()
     _declare_render2(L)
     meshes = scene.meshes
@@ -621,10 +683,13 @@ def emissive_triangles(scene):
     mats = scene.packed_materials()
     args = [_p(pos), _p(voff), _p(idx), _p(ioff), _p(tcount), _p(inst_mesh), _p(tr), _p(inst_mat),
             inst_mesh.size, _p(mats)]
-    n = L.orc_emissive_triangles(*args, None, None, 0)
+    n = L.orc_emissive_triangles(*args, None, None, 0, None)
     out = np.zeros((max(n, 1), 12), np.float32)
     w = np.zeros(max(n, 1), np.float64)
-    L.orc_emissive_triangles(*args, _p(out), _p(w), n)
+    corners = np.zeros((max(n, 1), 3), np.uint32)
+    L.orc_emissive_triangles(*args, _p(out), _p(w), n, _p(corners))
+    if with_corners:
+        return out[:n], w[:n], corners[:n]
     return out[:n], w[:n]
 
 
@@ -652,7 +717,12 @@ class OracleRenderer:
         self.settings = np.array([width, height, bounces, candidates, pattern, seed & 0xFFFFFFFF, log, sample_offset],
                                  np.uint32)
         self.clamping = clamping
-        self.emissive, self.emissive_weight = emissive_triangles(scene)
+        self.emissive, self.emissive_weight, corners = emissive_triangles(scene, with_corners=True)
+        self.textures = None
+        if any(m.has_maps() for m in scene.materials):
+            uv, uv1, uv1e = scene_uvs(scene)
+            eu = np.concatenate([uv[corners].reshape(-1, 6), uv1e[corners].reshape(-1, 6)], 1)
+            self.textures = TextureSet(scene.textures, uv, uv1, eu)
         self.width, self.height = width, height
         self.rays = 0
 
@@ -666,7 +736,8 @@ class OracleRenderer:
         n = lib().orc_path_draws(self.trace.h, _p(self.normals), _p(self.materials), self.materials.shape[0],
                                  _p(self.lights), self.lights.shape[0], None if env is None else _p(env), ew, eh,
                                  _p(self.camera), _p(self.settings), self.clamping, px, py, sample, _p(out), cap,
-                                 _p(self.emissive), _p(self.emissive_weight), self.emissive.shape[0])
+                                 _p(self.emissive), _p(self.emissive_weight), self.emissive.shape[0],
+                                 None if self.textures is None else self.textures.h)
         assert n <= cap, "draw log overflow"
         return out[:n].copy()
 
@@ -680,5 +751,5 @@ class OracleRenderer:
                                       _p(self.lights), self.lights.shape[0], None if env is None else _p(env),
                                       ew, eh, _p(self.camera), _p(self.settings), self.clamping, x0, y0, w, h,
                                       sample_start, count, _p(out), _p(self.emissive), _p(self.emissive_weight),
-                                      self.emissive.shape[0])
+                                      self.emissive.shape[0], None if self.textures is None else self.textures.h)
         return out

@@ -212,16 +212,61 @@ uint32_t orc_emissive_triangles(const float *positions, const uint32_t *vertexOf
                                 const uint32_t *indexOffsets, const uint32_t *triangleCounts,
                                 const uint32_t *instanceMesh, const double *transforms,
                                 const uint32_t *instanceMaterial, uint32_t instanceCount, const float *materials,
-                                float *out, double *weight, uint32_t capacity) {
+                                float *out, double *weight, uint32_t capacity, uint32_t *corners) {
   return emissive_triangles(positions, vertexOffsets, indices, indexOffsets, triangleCounts, instanceMesh,
-                            transforms, instanceMaterial, instanceCount, materials, out, weight, capacity);
+                            transforms, instanceMaterial, instanceCount, materials, out, weight, capacity, corners);
+}
+
+// Material textures and UVs for orc_render: n textures of 11 words each
+// (width, height, storage (0 RGBA8, 1 RGBA32F), flip_y, uv_channel, wrap_s,
+// wrap_t, then the f32 bits of uv_scale[2] and uv_offset[2]) with their
+// row-major pixels, UV0 / UV1 per global vertex (2 floats each) and 12 floats
+// per emissive triangle (uv0, uv1, uv2, then the corners' UV1).
+void *orc_texset_create(uint32_t n, const uint32_t *desc, const void *const *pixels, uint32_t vertexCount,
+                        const float *uvs, const float *uvs1, uint32_t emissiveCount, const float *emissiveUv) {
+  auto *T = new TextureSet();
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t *d = desc + (size_t)i * 11;
+    Texture t{};
+    t.width = d[0]; t.height = d[1]; t.storage = d[2];
+    const bool flip = d[3] != 0u;
+    t.uvChannel = d[4]; t.wrapS = d[5]; t.wrapT = d[6];
+    std::memcpy(&t.sx, d + 7, 4); std::memcpy(&t.sy, d + 8, 4);
+    std::memcpy(&t.ox, d + 9, 4); std::memcpy(&t.oy, d + 10, 4);
+    const size_t row = (size_t)t.width * (t.storage == 1u ? 4 : 1);
+    for (uint32_t y = 0; y < t.height; y++) {
+      const size_t src = (size_t)(flip ? t.height - 1 - y : y) * row;
+      if (t.storage == 1u) {
+        const float *f = static_cast<const float *>(pixels[i]) + src;
+        t.rgba32.insert(t.rgba32.end(), f, f + row);
+      } else {
+        const uint32_t *w = static_cast<const uint32_t *>(pixels[i]) + src;
+        t.rgba8.insert(t.rgba8.end(), w, w + row);
+      }
+    }
+    T->tex.push_back(std::move(t));
+  }
+  T->uvs.assign(uvs, uvs + (size_t)vertexCount * 2);
+  T->uvs1.assign(uvs1, uvs1 + (size_t)vertexCount * 2);
+  T->emissiveUv.assign(emissiveUv, emissiveUv + (size_t)emissiveCount * 12);
+  return T;
+}
+void orc_texset_destroy(void *T) { delete static_cast<TextureSet *>(T); }
+
+// sampleMap of texture `index` at n UVs (RGBA, 4 floats each), and the
+// decoded color (decodeMapColor, 3 floats each).
+void orc_sample_map(void *T, uint32_t index, uint32_t n, const float *uv, float *rgba, float *decoded) {
+  const TextureSet &S = *static_cast<TextureSet *>(T);
+  for (uint32_t i = 0; i < n; i++) probe_sample_map(S, index, uv[2 * i], uv[2 * i + 1], rgba + 4 * i, decoded + 3 * i);
 }
 
 static RenderScene render_scene(void *trace, const float *normals, const float *materials, uint32_t materialCount,
                                 const float *lights, uint32_t lightCount, const float *env, uint32_t envW,
                                 uint32_t envH, const float *camera, const uint32_t *settings, float clamping,
-                                const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
+                                const float *emissive, const double *emissiveWeight, uint32_t emissiveCount,
+                                void *textures) {
   RenderScene s{};
+  s.textures = static_cast<const TextureSet *>(textures);
   s.emissive = emissive;
   s.emissiveWeight = emissiveWeight;
   s.emissiveCount = emissiveCount;
@@ -241,9 +286,9 @@ uint64_t orc_render(void *trace, const float *normals, const float *materials, u
                     const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
                     const float *camera, const uint32_t *settings, float clamping, uint32_t x0, uint32_t y0,
                     uint32_t w, uint32_t h, uint32_t sampleStart, uint32_t count, double *out,
-                    const float *emissive, const double *emissiveWeight, uint32_t emissiveCount) {
+                    const float *emissive, const double *emissiveWeight, uint32_t emissiveCount, void *textures) {
   RenderScene s = render_scene(trace, normals, materials, materialCount, lights, lightCount, env, envW, envH, camera,
-                               settings, clamping, emissive, emissiveWeight, emissiveCount);
+                               settings, clamping, emissive, emissiveWeight, emissiveCount, textures);
   static_assert(sizeof(Accum) == 13 * sizeof(double), "Accum layout");
   return render(s, x0, y0, w, h, sampleStart, count, reinterpret_cast<Accum *>(out));
 }
@@ -254,9 +299,9 @@ uint32_t orc_path_draws(void *trace, const float *normals, const float *material
                         const float *lights, uint32_t lightCount, const float *env, uint32_t envW, uint32_t envH,
                         const float *camera, const uint32_t *settings, float clamping, uint32_t px, uint32_t py,
                         uint32_t sampleIndex, uint32_t *log, uint32_t cap, const float *emissive,
-                        const double *emissiveWeight, uint32_t emissiveCount) {
+                        const double *emissiveWeight, uint32_t emissiveCount, void *textures) {
   RenderScene s = render_scene(trace, normals, materials, materialCount, lights, lightCount, env, envW, envH, camera,
-                               settings, clamping, emissive, emissiveWeight, emissiveCount);
+                               settings, clamping, emissive, emissiveWeight, emissiveCount, textures);
   return path_draws(s, px, py, sampleIndex, log, cap);
 }
 

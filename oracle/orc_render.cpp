@@ -138,8 +138,105 @@ static void roughness_to_alpha(float r, float aniso, float &ax, float &ay) {
   ay = mxf(K_ALPHA_MIN, c * at);
 }
 
-// getMaterial (materialBuffer.wgsl.ts:170-245) without textures.
-static Mat get_material(const float *m, V3 instanceColor) {
+// ------------------------------------------------------------------ textures
+// UV pair (WGSL vec2f).
+struct UV { float x, y; };
+
+// wrapMapCoordinate (materialBuffer.wgsl.ts:285-289): 0 repeat (fract),
+// 1 clamp to edge, 2 mirrored repeat.
+static float wrap_map_coordinate(float value, uint32_t mode) {
+  if (mode == 0u) return value - std::floor(value);
+  if (mode == 1u) return clampf(value, 0.0f, 1.0f);
+  return 1.0f - fabsf(1.0f - (value - 2.0f * std::floor(value * 0.5f)));
+}
+// One texel (textureLoad of the atlas): RGBA8 words are unorm bytes, R in
+// the low byte, each byte / 255; RGBA32F texels are read as stored.
+static void load_texel(const Texture &t, int x, int y, float out[4]) {
+  const size_t i = (size_t)(uint32_t)y * t.width + (uint32_t)x;
+  if (t.storage == 1u) {
+    for (int k = 0; k < 4; k++) out[k] = t.rgba32[i * 4 + k];
+    return;
+  }
+  const uint32_t w = t.rgba8[i];
+  for (int k = 0; k < 4; k++) out[k] = (float)((w >> (8 * k)) & 255u) / 255.0f;
+}
+// loadMapTexel (materialBuffer.wgsl.ts:338-348): repeat wraps the integer
+// coordinate modulo the size, the other modes clamp it to the image.
+static int map_texel_coordinate(int c, int size, uint32_t mode) {
+  if (mode == 0u) return ((c % size) + size) % size;
+  return c < 0 ? 0 : (c > size - 1 ? size - 1 : c);
+}
+// sampleMap (materialBuffer.wgsl.ts:314-336) for an ordinary image: the
+// coordinate uv * scale + offset wrapped per axis (wrappedMapUv, :291-297),
+// texel = wrapped * size - 0.5, the four texels around it blended with
+// mix(mix(t00, t10, fx), mix(t01, t11, fx), fy). The reference hands interior
+// footprints to the hardware sampler; this is its explicit four-tap form,
+// used for every footprint. A NaN wrapped coordinate reads as 0.
+static void sample_map(const TextureSet &T, uint32_t index, UV uv, float out[4]) {
+  const Texture &t = T.tex[index];
+  float wu = wrap_map_coordinate(uv.x * t.sx + t.ox, t.wrapS);
+  float wv = wrap_map_coordinate(uv.y * t.sy + t.oy, t.wrapT);
+  if (wu != wu) wu = 0.0f;
+  if (wv != wv) wv = 0.0f;
+  const int W = (int)t.width, H = (int)t.height;
+  const float tx = wu * (float)W - 0.5f, ty = wv * (float)H - 0.5f;
+  const float bx = std::floor(tx), by = std::floor(ty);
+  const float fx = tx - bx, fy = ty - by;
+  const int x0 = map_texel_coordinate((int)bx, W, t.wrapS), x1 = map_texel_coordinate((int)bx + 1, W, t.wrapS);
+  const int y0 = map_texel_coordinate((int)by, H, t.wrapT), y1 = map_texel_coordinate((int)by + 1, H, t.wrapT);
+  float a[4], b[4], c[4], d[4];
+  load_texel(t, x0, y0, a);
+  load_texel(t, x1, y0, b);
+  load_texel(t, x0, y1, c);
+  load_texel(t, x1, y1, d);
+  for (int k = 0; k < 4; k++) out[k] = mixf(mixf(a[k], b[k], fx), mixf(c[k], d[k], fx), fy);
+}
+// sRGBToLinear (materialBuffer.wgsl.ts:258-260) per channel.
+static float srgb_to_linear(float c) {
+  return c < 0.04045f ? c * 0.0773993808f : mtxfm::powf(c * 0.9478672986f + 0.0521327014f, 2.4f);
+}
+// decodeMapColor (:262-265): sRGB decode unless the texture stores floats.
+static V3 decode_map_color(const TextureSet &T, uint32_t index, const float c[4]) {
+  if (T.tex[index].storage == 1u) return V3(c[0], c[1], c[2]);
+  return V3(srgb_to_linear(c[0]), srgb_to_linear(c[1]), srgb_to_linear(c[2]));
+}
+// The texture of slot k (index + 1 stored, 0 = none) sampled at its UV set
+// (getMapUvSet, :303-305); returns false when the slot is empty.
+static bool material_map(const TextureSet &T, const float *m, int k, UV uv, UV uv1, float out[4],
+                         uint32_t *index = nullptr) {
+  const uint32_t slot = (uint32_t)m[M_MAPS + k];
+  if (slot == 0u) return false;
+  sample_map(T, slot - 1u, T.tex[slot - 1u].uvChannel ? uv1 : uv, out);
+  if (index) *index = slot - 1u;
+  return true;
+}
+void probe_sample_map(const TextureSet &T, uint32_t index, float u, float v, float rgba[4], float decoded[3]) {
+  sample_map(T, index, UV{u, v}, rgba);
+  V3 d = decode_map_color(T, index, rgba);
+  decoded[0] = d.x; decoded[1] = d.y; decoded[2] = d.z;
+}
+
+// getMaterialEmission (materialBuffer.wgsl.ts:351-372).
+static V3 material_emission(const TextureSet *T, const float *m, UV uv, UV uv1) {
+  V3 e(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
+  if (!T) return e;
+  float c[4];
+  uint32_t index;
+  if (material_map(*T, m, MAP_EMISSIVE, uv, uv1, c, &index)) e = e * decode_map_color(*T, index, c);
+  if (material_map(*T, m, MAP_COAT_WEIGHT, uv, uv1, c)) {
+    float coatWeight = clampf(m[M_COAT_WEIGHT], 0.0f, 1.0f) * c[0];
+    V3 tint(std::sqrt(mxf(m[M_COAT_COLOR], 0.0f)), std::sqrt(mxf(m[M_COAT_COLOR + 1], 0.0f)),
+            std::sqrt(mxf(m[M_COAT_COLOR + 2], 0.0f)));
+    e = e * mix3(V3(1.0f), tint, coatWeight);
+  }
+  return e;
+}
+
+// getMaterial (materialBuffer.wgsl.ts:170-245), with the texture samples of
+// :78-160 when the scene has textures (materials are opaque: the alpha
+// samples change nothing).
+static Mat get_material(const float *m, V3 instanceColor, const TextureSet *T = nullptr, UV uv = UV{0.0f, 0.0f},
+                        UV uv1 = UV{0.0f, 0.0f}) {
   Mat ma{};
   ma.baseColor = V3(m[M_BASE_COLOR], m[M_BASE_COLOR + 1], m[M_BASE_COLOR + 2]);
   ma.specularRoughness = m[M_SPEC_ROUGHNESS];
@@ -166,6 +263,29 @@ static Mat get_material(const float *m, V3 instanceColor) {
   ma.subsurfaceWeight = clampf(m[M_SSS_WEIGHT], 0.0f, 1.0f);
   ma.subsurfaceColor = vmaxs(V3(m[M_SSS_COLOR], m[M_SSS_COLOR + 1], m[M_SSS_COLOR + 2]), 0.0f);
   ma.thinWalled = m[M_THIN_WALLED] > 0.5f;
+  if (T) {
+    ma.emission = material_emission(T, m, uv, uv1);
+    float c[4];
+    uint32_t index;
+    if (material_map(*T, m, MAP_BASE, uv, uv1, c, &index)) {
+      ma.baseColor = ma.baseColor * decode_map_color(*T, index, c);
+      if (m[M_MAP_ALPHA_USAGE] == 3.0f) ma.transmissionWeight = 1.0f - c[3];
+    }
+    if (material_map(*T, m, MAP_TRANSMISSION, uv, uv1, c)) ma.transmissionWeight = ma.transmissionWeight * c[0];
+    if (material_map(*T, m, MAP_METALLIC_ROUGHNESS, uv, uv1, c)) {
+      ma.baseMetalness = ma.baseMetalness * c[2];
+      ma.specularRoughness = ma.specularRoughness * c[1];
+    }
+    if (material_map(*T, m, MAP_ROUGHNESS, uv, uv1, c)) ma.specularRoughness = ma.specularRoughness * c[1];
+    if (material_map(*T, m, MAP_METALNESS, uv, uv1, c)) ma.baseMetalness = ma.baseMetalness * c[2];
+    if (material_map(*T, m, MAP_COAT_WEIGHT, uv, uv1, c)) ma.coatWeight = ma.coatWeight * c[0];
+    if (material_map(*T, m, MAP_COAT_ROUGHNESS, uv, uv1, c)) ma.coatRoughness = ma.coatRoughness * c[1];
+    if (material_map(*T, m, MAP_FUZZ_COLOR, uv, uv1, c, &index)) ma.fuzzColor = ma.fuzzColor * decode_map_color(*T, index, c);
+    if (material_map(*T, m, MAP_FUZZ_ROUGHNESS, uv, uv1, c)) ma.fuzzRoughness = ma.fuzzRoughness * c[3];
+    if (material_map(*T, m, MAP_SPECULAR_WEIGHT, uv, uv1, c)) ma.specularWeight = ma.specularWeight * c[3];
+    if (material_map(*T, m, MAP_SPECULAR_COLOR, uv, uv1, c, &index))
+      ma.specularColor = ma.specularColor * decode_map_color(*T, index, c);
+  }
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness;
     float c2 = ma.coatRoughness * ma.coatRoughness;
@@ -1061,7 +1181,8 @@ static void light_sample_analytic(const float *L, V3 p, float sx, float sy, Ligh
 uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,
                             const uint32_t *indexOffsets, const uint32_t *triangleCounts, const uint32_t *instanceMesh,
                             const double *transforms, const uint32_t *instanceMaterial, uint32_t instanceCount,
-                            const float *materials, float *out, double *weight, uint32_t capacity) {
+                            const float *materials, float *out, double *weight, uint32_t capacity,
+                            uint32_t *corners) {
   uint32_t n = 0;
   for (uint32_t i = 0; i < instanceCount; i++) {
     const float *em = materials + (size_t)instanceMaterial[i] * M_FLOATS + M_EMISSION;
@@ -1099,6 +1220,9 @@ uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffset
         o[10] = (float)instanceMaterial[i];
         o[11] = 0.0f;
         weight[n] = area * lum;
+        if (corners)
+          for (int corner = 0; corner < 3; corner++)
+            corners[(size_t)n * 3 + corner] = vertexOffsets[mesh] + indices[indexOffsets[mesh] + t * 3 + corner];
       }
       n++;
     }
@@ -1124,7 +1248,14 @@ static void sample_emissive_triangle(const Ctx &c, uint32_t index, V3 p, float r
   float cosLight = dot(normal, -r.direction);
   if (cosLight <= 0.0f) { r.emission = V3(0.0f); r.pdf = 0.0f; return; }
   const float *m = c.s.materials + (size_t)material * M_FLOATS;
-  r.emission = V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
+  if (c.s.textures) {
+    const float *u = c.s.textures->emissiveUv.data() + (size_t)index * 12;
+    UV uv{b.x * u[0] + b.y * u[2] + b.z * u[4], b.x * u[1] + b.y * u[3] + b.z * u[5]};
+    UV uv1{b.x * u[6] + b.y * u[8] + b.z * u[10], b.x * u[7] + b.y * u[9] + b.z * u[11]};
+    r.emission = material_emission(c.s.textures, m, uv, uv1);
+  } else {
+    r.emission = V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]);
+  }
   r.selectionPmf = c.T.emissivePmf[index];
   r.pdf = r.selectionPmf * (r.dist * r.dist) / mxf(area * cosLight, 1e-8f);
 }
@@ -1318,7 +1449,33 @@ struct Surface {
   V3 position, normal, faceNormal, shadowOffset;
   int32_t triangleIndex, instanceIndex, materialIndex;
   bool albedoSampled;
+  UV uv, uv1;  // times the material's UV scale (textured scenes)
 };
+
+// getMatNormal (materialBuffer.wgsl.ts:377-415), object space: the normal
+// map's tangent-space normal in the cotangent frame of UV0
+// (thetenthplanet.de/archives/1180), a fixed frame where UV0 has no gradient.
+static V3 mat_normal(const TextureSet &T, uint32_t index, const float *m, UV uv, V3 normal, V3 dp1, V3 dp2, UV duv1,
+                     UV duv2) {
+  V3 dp2perp = cross(dp2, normal), dp1perp = cross(normal, dp1);
+  V3 dpdu = dp2perp * duv1.x + dp1perp * duv2.x;
+  V3 dpdv = dp2perp * duv1.y + dp1perp * duv2.y;
+  float gradientLengthSquared = mxf(dot(dpdu, dpdu), dot(dpdv, dpdv));
+  if (gradientLengthSquared > 1e-12f) {
+    float invmax = 1.0f / std::sqrt(gradientLengthSquared);
+    dpdu = dpdu * invmax;
+    dpdv = dpdv * invmax;
+  } else {
+    V3 referenceAxis = fabsf(normal.z) < 0.999f ? V3(0.0f, 0.0f, 1.0f) : V3(0.0f, 1.0f, 0.0f);
+    dpdu = normalize(cross(referenceAxis, normal));
+    dpdv = cross(normal, dpdu);
+  }
+  float c[4];
+  sample_map(T, index, uv, c);
+  V3 n(2.0f * c[0] - 1.0f, 2.0f * c[1] - 1.0f, 2.0f * c[2] - 1.0f);
+  n = normalize(V3(n.x * m[M_NORMAL_SCALE], n.y * m[M_NORMAL_SCALE + 1], n.z));
+  return normalize(dpdu * n.x + dpdv * n.y + normal * n.z);
+}
 struct PathState {
   V3 ro, rd, li, beta;
   float alpha, scatteringPdf;
@@ -1376,13 +1533,30 @@ static void intersect_scene(Ctx &c, const PathState &path, Surface &si, int boun
     hh = mnf(-mnf(mnf(h0, h1), h2), hh * 0.5f);
   }
   V3 shadowOffset = smooth * hh;
+  si.uv = UV{0.0f, 0.0f};
+  si.uv1 = UV{0.0f, 0.0f};
+  V3 shading = normal;
+  if (const TextureSet *T = c.s.textures) {
+    auto at = [&](const std::vector<float> &a, int32_t i) { return UV{a[(size_t)i * 2], a[(size_t)i * 2 + 1]}; };
+    UV t0 = at(T->uvs, i0), t1 = at(T->uvs, i1), t2 = at(T->uvs, i2);
+    UV s0 = at(T->uvs1, i0), s1 = at(T->uvs1, i1), s2 = at(T->uvs1, i2);
+    UV uv{b.x * t0.x + b.y * t1.x + b.z * t2.x, b.x * t0.y + b.y * t1.y + b.z * t2.y};
+    UV uv1{b.x * s0.x + b.y * s1.x + b.z * s2.x, b.x * s0.y + b.y * s1.y + b.z * s2.y};
+    const float *m = c.s.materials + (size_t)si.materialIndex * M_FLOATS;
+    si.uv = UV{uv.x * m[M_UV_SCALE], uv.y * m[M_UV_SCALE + 1]};
+    si.uv1 = UV{uv1.x * m[M_UV_SCALE], uv1.y * m[M_UV_SCALE + 1]};
+    const uint32_t slot = (uint32_t)m[M_MAPS + MAP_NORMAL];
+    if (slot)
+      shading = mat_normal(*T, slot - 1u, m, si.uv, normal, p0 - p2, p1 - p2, UV{t0.x - t2.x, t0.y - t2.y},
+                           UV{t1.x - t2.x, t1.y - t2.y});
+  }
   float l2w[16], w2l[16];
   for (int k = 0; k < 16; k++) { w2l[k] = u2f(ins[4 + k]); l2w[k] = u2f(ins[20 + k]); }
   auto nt = [&](V3 n) {  // transpose(mat3(w2l)) * n
     return V3(dot(V3(w2l[0], w2l[1], w2l[2]), n), dot(V3(w2l[4], w2l[5], w2l[6]), n), dot(V3(w2l[8], w2l[9], w2l[10]), n));
   };
   si.faceNormal = normalize(nt(h.normal));
-  si.normal = normalize(nt(normal));
+  si.normal = normalize(nt(shading));
   si.shadowOffset = mat_mul_point(l2w, shadowOffset, 0.0f);
   si.position = mat_mul_point(l2w, position, 1.0f);
 }
@@ -1558,7 +1732,7 @@ static void sample_material(Ctx &c, Sampler &smp, Surface &si, Mat &ma, PathStat
     if (cosLight <= 0.0f && m[M_DOUBLE_SIDED] < 0.5f) {
       weight = 0.0f;
     } else if (!(path.fromEye | ((float)c.T.totalWeight <= 0.0f))) {  // rtUniform.lightWeightTotal is f32
-      float samplingLuminance = dot(ma.emission, K_LUM);
+      float samplingLuminance = dot(V3(m[M_EMISSION], m[M_EMISSION + 1], m[M_EMISSION + 2]), K_LUM);
       if (!((samplingLuminance <= 0.0f) | (cosLight <= 0.0f))) {
         float hitDistance = length(si.position - path.ro);
         float lightPdf = samplingLuminance / (float)c.T.totalWeight * hitDistance * hitDistance / mxf(cosLight, 1e-8f);
@@ -1694,7 +1868,8 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
     intersect_scene(c, path, si, depth, event != 0);
     if (si.hit) {
       int32_t mi = si.materialIndex;
-      ma = get_material(s.materials + (size_t)mi * M_FLOATS, instance_color(*s.trace, si.instanceIndex));
+      ma = get_material(s.materials + (size_t)mi * M_FLOATS, instance_color(*s.trace, si.instanceIndex), s.textures,
+                        si.uv, si.uv1);
     }
     if (event == 0) {
       if (si.hit) {

@@ -21,7 +21,28 @@ enum MatField {
   M_TRANS_WEIGHT = 12, M_TRANS_COLOR = 13, M_COAT_WEIGHT = 16, M_COAT_COLOR = 17,
   M_COAT_ROUGHNESS = 20, M_COAT_IOR = 21, M_COAT_DARKENING = 22, M_EMISSION = 23,
   M_DOUBLE_SIDED = 26, M_SPEC_ANISOTROPY = 27, M_FUZZ_WEIGHT = 28, M_FUZZ_COLOR = 29, M_FUZZ_ROUGHNESS = 32,
-  M_SSS_WEIGHT = 33, M_SSS_COLOR = 34, M_THIN_WALLED = 37, M_FLOATS = 48
+  M_SSS_WEIGHT = 33, M_SSS_COLOR = 34, M_THIN_WALLED = 37, M_OPACITY = 38, M_ALPHA_MODE = 39, M_ALPHA_CUTOFF = 40,
+  M_MAP_ALPHA_USAGE = 41, M_NORMAL_SCALE = 42, M_UV_SCALE = 44, M_MAPS = 46, M_FLOATS = 64
+};
+// Texture slots (texture index + 1 at M_MAPS + slot; 0 = none).
+enum MapSlot {
+  MAP_BASE = 0, MAP_ALPHA, MAP_TRANSMISSION, MAP_METALLIC_ROUGHNESS, MAP_ROUGHNESS, MAP_METALNESS, MAP_COAT_WEIGHT,
+  MAP_COAT_ROUGHNESS, MAP_FUZZ_COLOR, MAP_FUZZ_ROUGHNESS, MAP_SPECULAR_WEIGHT, MAP_SPECULAR_COLOR, MAP_EMISSIVE,
+  MAP_NORMAL, MAP_COUNT
+};
+// Material textures and vertex UVs of a scene. Texel rows are in sampling
+// order (flip_y already applied); uvs / uvs1 hold xy per global vertex (zeros
+// where a mesh has none); emissiveUv holds 12 floats per emissive triangle:
+// uv0, uv1, uv2 and the corners' UV1 (UV0 where the mesh has no UV1).
+struct Texture {
+  uint32_t width, height, storage, wrapS, wrapT, uvChannel;
+  float sx, sy, ox, oy;
+  std::vector<uint32_t> rgba8;
+  std::vector<float> rgba32;
+};
+struct TextureSet {
+  std::vector<Texture> tex;
+  std::vector<float> uvs, uvs1, emissiveUv;
 };
 // Packed light (16 floats): type, position, u, v, emission, spread or radius,
 // area (the reference's Light record, rayTraceCore.wgsl.ts:228-248). Types are
@@ -45,6 +66,7 @@ struct RenderScene {
   uint32_t width, height;
   uint32_t bounces, candidates, pattern, seed, logSPP, sampleOffset;
   float clamping;
+  const TextureSet *textures = nullptr;  // null: no material samples a texture
 };
 
 struct Accum {
@@ -67,18 +89,22 @@ struct HostTables {
 };
 void build_tables(const RenderScene &s, HostTables &T);
 // collectEmissiveTriangles (lightSampling.ts:237-320) over instances in order:
-// writes up to `capacity` records of 12 floats and their alias weights,
-// returns the number of emissive triangles.
+// writes up to `capacity` records of 12 floats and their alias weights (and,
+// when `corners` is given, the global vertex index of each corner), returns
+// the number of emissive triangles.
 uint32_t emissive_triangles(const float *positions, const uint32_t *vertexOffsets, const uint32_t *indices,
                             const uint32_t *indexOffsets, const uint32_t *triangleCounts, const uint32_t *instanceMesh,
                             const double *transforms, const uint32_t *instanceMaterial, uint32_t instanceCount,
-                            const float *materials, float *out, double *weight, uint32_t capacity);
+                            const float *materials, float *out, double *weight, uint32_t capacity,
+                            uint32_t *corners = nullptr);
 // Primary ray of (pixel, sample index): origin xyz, direction xyz.
 void camera_sample(const float *camera, uint32_t width, uint32_t height, uint32_t px, uint32_t py, uint32_t index,
                    uint32_t pattern, uint32_t seed, uint32_t logSPP, float out[6]);
 void build_env_cdf(const RenderScene &s, HostTables &T);
 
 void set_energy_lut(const float *lut);  // 32 x 2177 floats
+// sampleMap of texture `index` at (u, v): filtered RGBA and its decoded color.
+void probe_sample_map(const TextureSet &T, uint32_t index, float u, float v, float rgba[4], float decoded[3]);
 // Fitted directional albedo of the fuzz lobe (openPbrBsdf.wgsl.ts:595-607).
 float fuzz_albedo(float viewCosine, float roughness);
 // Renders samples [sampleStart, sampleStart+count) for the pixel rectangle

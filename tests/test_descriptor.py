@@ -200,3 +200,57 @@ def test_version5_light_records(cornell):
     # after it; the encoder writes arrays first, so only table offsets move.
     v5[1] = v5.size
     assert context.validate(v5.tobytes())["lights"] == 1
+
+
+def test_material_textures_and_alpha_resolution():
+    """Material textures travel through the descriptor (basic-record slots,
+    extension slots, UV sets, texture records); alpha resolves as
+    resolveMaterialAlphaMode (MaterialState.ts:251-264) and coverage is
+    refused as UNAVAILABLE; texture features outside the row (UDIM sets, coat
+    normal maps) are UNAVAILABLE too."""
+    room = scene.textured_room()
+    s = context.validate(descriptor.encode(room, 24, 16))
+    assert s["materials"] == len(room.materials) and s["instances"] == len(room.mesh_index)
+
+    def status_of(edit):
+        sc = scene.textured_room()
+        edit(sc)
+        try:
+            context.validate(descriptor.encode(sc, 8, 8))
+            return abi.MTX_OK
+        except abi.MtxError as e:
+            return e.code
+
+    def opacity(sc):
+        sc.materials[0].opacity = 0.5
+    assert status_of(opacity) == abi.MTX_UNAVAILABLE
+
+    def opaque_override(sc):
+        sc.materials[0].opacity = 0.5
+        sc.materials[0].alpha_mode = "opaque"
+    assert status_of(opaque_override) == abi.MTX_OK
+
+    def mask(sc):
+        sc.materials[0].alpha_cutoff = 0.5
+    assert status_of(mask) == abi.MTX_UNAVAILABLE
+
+    def alpha_map(sc):
+        sc.materials[0].alpha_map = 1
+    assert status_of(alpha_map) == abi.MTX_UNAVAILABLE
+
+    def map_alpha(sc):
+        sc.materials[0].map_alpha_usage = "alpha"
+    assert status_of(map_alpha) == abi.MTX_UNAVAILABLE
+    w = _words(descriptor.encode(room, 8, 8))
+    tex = int(w[13])
+    udim = w.copy()
+    udim[tex + 12] = 1  # texture 0 (the floor's base map) as a UDIM set
+    assert _status(udim) == abi.MTX_UNAVAILABLE
+    bad = w.copy()
+    bad[tex + 10] = 3  # wrap mode out of range
+    assert _status(bad) == abi.MTX_INVALID_ARGUMENT
+    assert _status(_extended(scene.cornell_box(), [74, 0xFFFFFFFF, 0, 0])) == abi.MTX_OK
+    room_words = _words(descriptor.encode(room, 8, 8))
+    mat = int(room_words[11])
+    room_words[mat + 16] = 40  # normal map slot naming a texture past the table
+    assert _status(room_words) == abi.MTX_INVALID_ARGUMENT

@@ -183,3 +183,22 @@ def test_c_host_job_flow(mods, orc, tmp_path):
     for kind, base in (("hdr", 0), ("albedo", 4), ("normal", 8)):
         got = np.fromfile(tmp_path / f"{kind}.rgba32f", np.float32).reshape(H, W, 4)
         assert np.array_equal(got, _expected(ref, spp, base)), kind
+
+
+def test_textured_descriptor_matches_target(mods):
+    """The textured room through the descriptor (texture records, UV sets,
+    texture slots in the basic record and in extensions) renders exactly as
+    a render target of the same Scene."""
+    abi, context, descriptor, render, scene = mods
+    sc = scene.textured_room()
+    W, H, spp = 40, 30, 4
+    ctx = context.RenderContext()
+    ctx.commit(descriptor.encode(sc, W, H), _settings(context, spp, bounces=8, candidates=2))
+    ctx.render(spp)
+    t = render.RenderTarget(render.RenderScene(descriptor.f32_scene(sc)), W, H, bounces=8,
+                            direct_light_candidates=2, max_samples=spp)
+    t.render(spp)
+    sums, counts = t.read_accumulators()
+    assert np.all(counts == spp)
+    for kind, base in (("hdr", 0), ("albedo", 4), ("normal", 8)):
+        assert np.array_equal(ctx.read_pixels(kind), _expected(sums, spp, base)), kind
