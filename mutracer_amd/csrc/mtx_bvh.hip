@@ -146,6 +146,82 @@ __device__ bool sah_split(const uint32_t *kmin, const uint32_t *kmax, const uint
   return true;
 }
 
+// The same split chosen by a whole workgroup: thread t evaluates split
+// position t % 64 of axis t / 64 (its left box folded over bins 0..i in
+// ascending order and its right box over bins - 1..i + 1 in descending
+// order, as the sequential sweep accumulates them, so every box, count and
+// f64 cost is the sweep's own), then the lowest cost wins with ties going to
+// the earliest (axis, position), which is the sweep's strict `<` order.
+// Needs blockDim.x >= 3 * 64; all threads must call it. Shared scratch:
+// best (2 words) and the result written by thread 0.
+__device__ bool sah_split_block(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt, int32_t bins,
+                                const NodeFrame &f, const double *minDim, uint32_t nodePrims,
+                                unsigned long long *best, Split &out, int32_t &doSplit) {
+  const int t = threadIdx.x, axis = t >> 6, i = t & 63;
+  if (t == 0) { best[0] = ~0ull; best[1] = ~0ull; }
+  __syncthreads();
+  double cost = kFar;
+  if (axis < 3 && i < bins - 1 && f.extent[axis] > minDim[axis]) {
+    double lMin[3] = {kFar, kFar, kFar}, lMax[3] = {-kFar, -kFar, -kFar};
+    double rMin[3] = {kFar, kFar, kFar}, rMax[3] = {-kFar, -kFar, -kFar};
+    uint32_t leftN = 0, rightN = 0;
+    for (int b = 0; b <= i; b++) {
+      const int l = axis * bins + b;
+      if (cnt[l])
+        for (int k = 0; k < 3; k++) {
+          const double vmn = (double)unkey(kmin[l * 3 + k]), vmx = (double)unkey(kmax[l * 3 + k]);
+          lMin[k] = vmn < lMin[k] ? vmn : lMin[k];
+          lMax[k] = vmx > lMax[k] ? vmx : lMax[k];
+        }
+      leftN += cnt[l];
+    }
+    for (int b = bins - 1; b > i; b--) {
+      const int r = axis * bins + b;
+      if (cnt[r])
+        for (int k = 0; k < 3; k++) {
+          const double vmn = (double)unkey(kmin[r * 3 + k]), vmx = (double)unkey(kmax[r * 3 + k]);
+          rMin[k] = vmn < rMin[k] ? vmn : rMin[k];
+          rMax[k] = vmx > rMax[k] ? vmx : rMax[k];
+        }
+      rightN += cnt[r];
+    }
+    const double lx = lMax[0] - lMin[0], ly = lMax[1] - lMin[1], lz = lMax[2] - lMin[2];
+    const double rx = rMax[0] - rMin[0], ry = rMax[1] - rMin[1], rz = rMax[2] - rMin[2];
+    const double lc = leftN == 0 ? kFar : (lx * ly + ly * lz + lz * lx) * (double)leftN;
+    const double rc = rightN == 0 ? kFar : (rx * ry + ry * rz + rz * rx) * (double)rightN;
+    cost = (lc + rc) + 0.0;  // -0 -> +0, so that the bit order below is the numeric order
+    if (cost < kFar) atomicMin(&best[0], (unsigned long long)__double_as_longlong(cost));
+  }
+  __syncthreads();
+  if (cost < kFar && (unsigned long long)__double_as_longlong(cost) == best[0])
+    atomicMin(&best[1], (unsigned long long)(axis * 64 + i));
+  __syncthreads();
+  if (t == 0) {
+    const bool any = best[1] != ~0ull;
+    const double splitCost = 1.0 + 1.0 * (any ? __longlong_as_double((long long)best[0]) : kFar) / f.area;
+    doSplit = splitCost < (double)nodePrims * 1.0 ? 1 : 0;
+    if (doSplit) {
+      const int bestAxis = (int)(best[1] >> 6), bestPos = (int)(best[1] & 63);
+      out.axis = bestAxis;
+      out.pos = bestPos;
+      const float farF = (float)kFar;
+      for (int sd = 0; sd < 2; sd++)
+        for (int k = 0; k < 3; k++) { out.lo[sd][k] = farF; out.hi[sd][k] = -farF; }
+      for (int b = 0; b < bins; b++) {
+        const int bb = bestAxis * bins + b, sd = b <= bestPos ? 0 : 1;
+        for (int k = 0; k < 3; k++) {
+          const double vmn = cnt[bb] ? (double)unkey(kmin[bb * 3 + k]) : kFar;
+          const double vmx = cnt[bb] ? (double)unkey(kmax[bb * 3 + k]) : -kFar;
+          if (vmn < (double)out.lo[sd][k]) out.lo[sd][k] = (float)vmn;
+          if (vmx > (double)out.hi[sd][k]) out.hi[sd][k] = (float)vmx;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return doSplit != 0;
+}
+
 // ------------------------------------------------------------ input kernels
 __global__ void k_tri_aabbs(const float *__restrict__ pos, const uint32_t *__restrict__ idx,
                             uint32_t triCount, uint32_t vertexCount, float4 *__restrict__ lo,
@@ -198,18 +274,23 @@ __global__ void k_centers(const float4 *__restrict__ plo, const float4 *__restri
 // keep the earliest primitive's value. 64-bit keys: (value key, index).
 __device__ __forceinline__ uint32_t canon_key(float v) { return fkey(v == 0.0f ? 0.0f : v); }
 
-__global__ void k_root_reduce(const float4 *__restrict__ plo, const float4 *__restrict__ phi, uint32_t n,
-                              unsigned long long *__restrict__ keys, uint32_t *__restrict__ prim) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) prim[p] = p;
+// Grid-stride: each thread reduces its primitives, then its wave, then the
+// block (LDS), and the block issues six atomics.
+__global__ void __launch_bounds__(256) k_root_reduce(const float4 *__restrict__ plo, const float4 *__restrict__ phi,
+                                                     uint32_t n, unsigned long long *__restrict__ keys,
+                                                     uint32_t *__restrict__ prim) {
+  __shared__ unsigned long long part[4][6];
   unsigned long long k[6];
   for (int i = 0; i < 6; i++) k[i] = (i < 3) ? ~0ull : 0ull;
-  if (p < n) {
-    float4 a = plo[p], b = phi[p];
-    float mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    prim[p] = p;
+    const float4 a = plo[p], b = phi[p];
+    const float mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
     for (int i = 0; i < 3; i++) {
-      k[i] = ((unsigned long long)canon_key(mn[i]) << 32) | p;
-      k[3 + i] = ((unsigned long long)canon_key(mx[i]) << 32) | (0xffffffffu - p);
+      const unsigned long long lo = ((unsigned long long)canon_key(mn[i]) << 32) | p;
+      const unsigned long long hi = ((unsigned long long)canon_key(mx[i]) << 32) | (0xffffffffu - p);
+      k[i] = lo < k[i] ? lo : k[i];
+      k[3 + i] = hi > k[3 + i] ? hi : k[3 + i];
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -218,7 +299,14 @@ __global__ void k_root_reduce(const float4 *__restrict__ plo, const float4 *__re
       k[i] = i < 3 ? (o < k[i] ? o : k[i]) : (o > k[i] ? o : k[i]);
     }
   }
-  if ((threadIdx.x & 63) == 0) {
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < 6; i++) part[wave][i] = k[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+      for (int i = 0; i < 6; i++)
+        k[i] = i < 3 ? (part[w][i] < k[i] ? part[w][i] : k[i]) : (part[w][i] > k[i] ? part[w][i] : k[i]);
     for (int i = 0; i < 3; i++) atomicMin(&keys[i], k[i]);
     for (int i = 3; i < 6; i++) atomicMax(&keys[i], k[i]);
   }
@@ -473,6 +561,18 @@ __global__ void k_copyback(const Chunk *__restrict__ chunks, const BigState *__r
   for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += blockDim.x) prim[first + e] = tmp[first + e];
 }
 
+// The level's big nodes: their records (for the host's chunk list) and
+// fresh per-node states.
+__global__ void k_level_nodes(const BNode *__restrict__ nodes, const uint32_t *__restrict__ ids, uint32_t nBig,
+                              BNode *__restrict__ out, BigState *__restrict__ big) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nBig) return;
+  out[b] = nodes[ids[b]];
+  BigState st{};
+  st.node = ids[b];
+  big[b] = st;
+}
+
 // Children of split big nodes; route them to the next big level or the
 // small-subtree list.
 __global__ void k_children(const BigState *__restrict__ big, uint32_t nBig, BNode *__restrict__ nodes,
@@ -516,20 +616,158 @@ struct SmallTask {
   uint32_t root, pad;
 };
 
+// Nodes of at most kWaveNode primitives are split by one wave each, the
+// primitives one per lane: no workgroup barriers, and the four waves of a
+// workgroup work on four such subtrees at once.
+constexpr int kWaveNode = 64;
+constexpr int kBuildWaves = kBuildThreads / 64;
+
+// Ordering point for LDS between the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SmallBlockArrays {
+  uint32_t ltmp[kSmall], X[kSmall], Z[kSmall], rk[kSmall];
+  uint32_t kmin[3 * kMaxBins * 3], kmax[3 * kMaxBins * 3], cnt[3 * kMaxBins];
+};
+struct SmallWaveArrays {
+  uint32_t kmin[3 * kMaxBins * 3], kmax[3 * kMaxBins * 3], cnt[3 * kMaxBins];
+  uint32_t X[kWaveNode], Z[kWaveNode], tmp[kWaveNode];
+  uint32_t stack[kWaveNode + 2];  // pending node ids and their records
+  BNode snode[kWaveNode + 2];
+};
+
+// The sweep of sah_split for one wave: lane b holds bin b of an axis; an
+// inclusive prefix scan keeps the leftmost of equal minima / maxima (the
+// left fold's strict comparisons), a suffix scan the rightmost (the right
+// fold runs from the last bin down); lane i then prices split position i
+// with the sweep's f64 expressions, and the wave takes the lowest cost, ties
+// to the earliest (axis, position). Child boxes are folded over each side's
+// bins in ascending order, as the sweep's epilogue does. Returns false for
+// a leaf; every lane returns the same split.
+__device__ bool sah_split_wave(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt, int32_t bins,
+                               const NodeFrame &f, const double *minDim, uint32_t nodePrims, Split &out) {
+  const int lane = threadIdx.x & 63;
+  double bestCost = INFINITY;
+  int bestIdx = 0x7fffffff;
+  for (int axis = 0; axis < 3; axis++) {
+    if (!(f.extent[axis] > minDim[axis])) continue;
+    float lmn[3], lmx[3], rmn[3], rmx[3];
+    uint32_t lc = 0;
+    const bool has = lane < bins && cnt[axis * bins + lane] != 0;
+    for (int k = 0; k < 3; k++) {
+      const int b = (axis * bins + lane) * 3 + k;
+      lmn[k] = has ? unkey(kmin[b]) : INFINITY;
+      lmx[k] = has ? unkey(kmax[b]) : -INFINITY;
+    }
+    if (lane < bins) lc = cnt[axis * bins + lane];
+    uint32_t rc = lc;
+    for (int k = 0; k < 3; k++) { rmn[k] = lmn[k]; rmx[k] = lmx[k]; }
+    for (int off = 1; off < 64; off <<= 1) {
+      const bool up = lane >= off, down = lane + off < 64;
+      const uint32_t oc = __shfl_up(lc, off), od = __shfl_down(rc, off);
+      for (int k = 0; k < 3; k++) {
+        const float a = __shfl_up(lmn[k], off), b = __shfl_up(lmx[k], off);
+        const float c = __shfl_down(rmn[k], off), d = __shfl_down(rmx[k], off);
+        if (up) { lmn[k] = lmn[k] < a ? lmn[k] : a; lmx[k] = lmx[k] > b ? lmx[k] : b; }
+        if (down) { rmn[k] = rmn[k] < c ? rmn[k] : c; rmx[k] = rmx[k] > d ? rmx[k] : d; }
+      }
+      if (up) lc += oc;
+      if (down) rc += od;
+    }
+    // right side of position i = the suffix from bin i + 1
+    const uint32_t rightN = __shfl_down(rc, 1);
+    float sr[6];
+    for (int k = 0; k < 3; k++) { sr[k] = __shfl_down(rmn[k], 1); sr[3 + k] = __shfl_down(rmx[k], 1); }
+    if (lane < bins - 1) {
+      // the folds start from +-kFar: values beyond it leave the start value
+      double L[6], R[6];
+      for (int k = 0; k < 3; k++) {
+        L[k] = (double)lmn[k] < kFar ? (double)lmn[k] : kFar;
+        L[3 + k] = (double)lmx[k] > -kFar ? (double)lmx[k] : -kFar;
+        R[k] = (double)sr[k] < kFar ? (double)sr[k] : kFar;
+        R[3 + k] = (double)sr[3 + k] > -kFar ? (double)sr[3 + k] : -kFar;
+      }
+      const double lx = L[3] - L[0], ly = L[4] - L[1], lz = L[5] - L[2];
+      const double rx = R[3] - R[0], ry = R[4] - R[1], rz = R[5] - R[2];
+      const double lcost = lc == 0 ? kFar : (lx * ly + ly * lz + lz * lx) * (double)lc;
+      const double rcost = rightN == 0 ? kFar : (rx * ry + ry * rz + rz * rx) * (double)rightN;
+      const double cost = (lcost + rcost) + 0.0;
+      if (cost < kFar && cost < bestCost) { bestCost = cost; bestIdx = axis * 64 + lane; }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oc = __shfl_xor(bestCost, off);
+    const int oi = __shfl_xor(bestIdx, off);
+    if (oc < bestCost || (oc == bestCost && oi < bestIdx)) { bestCost = oc; bestIdx = oi; }
+  }
+  const double splitCost = 1.0 + 1.0 * (bestIdx != 0x7fffffff ? bestCost : kFar) / f.area;
+  if (!(splitCost < (double)nodePrims * 1.0)) return false;
+  const int bestAxis = bestIdx >> 6, bestPos = bestIdx & 63;
+  out.axis = bestAxis;
+  out.pos = bestPos;
+  const float farF = (float)kFar;
+  for (int sd = 0; sd < 2; sd++)
+    for (int k = 0; k < 3; k++) { out.lo[sd][k] = farF; out.hi[sd][k] = -farF; }
+  for (int b = 0; b < bins; b++) {
+    const int bb = bestAxis * bins + b, sd = b <= bestPos ? 0 : 1;
+    for (int k = 0; k < 3; k++) {
+      const double vmn = cnt[bb] ? (double)unkey(kmin[bb * 3 + k]) : kFar;
+      const double vmx = cnt[bb] ? (double)unkey(kmax[bb * 3 + k]) : -kFar;
+      if (vmn < (double)out.lo[sd][k]) out.lo[sd][k] = (float)vmn;
+      if (vmx > (double)out.hi[sd][k]) out.hi[sd][k] = (float)vmx;
+    }
+  }
+  return true;
+}
+
+// Writes the two children of `parent` (split at totalL), returns the first
+// child's index and, when `out` is given, the two records; one lane only.
+__device__ uint32_t emit_children(BNode *__restrict__ nodes, uint32_t *__restrict__ counter, uint32_t parentId,
+                                  const BNode &parent, const Split &split, uint32_t totalL, uint32_t totalR,
+                                  BNode *out = nullptr) {
+  const uint32_t c = atomicAdd(counter, 2u);
+  for (int side = 0; side < 2; side++) {
+    BNode ch{};
+    for (int k = 0; k < 3; k++) { ch.lo[k] = split.lo[side][k]; ch.hi[k] = split.hi[side][k]; }
+    ch.leftFirst = side == 0 ? parent.leftFirst : parent.leftFirst + totalL;
+    ch.primCount = side == 0 ? totalL : totalR;
+    ch.depth = parent.depth + 1;
+    ch.parent = (int32_t)parentId;
+    nodes[c + side] = ch;
+    if (out) out[side] = ch;
+  }
+  BNode nd = parent;
+  nd.leftFirst = c;
+  nd.primCount = 0;
+  nodes[parentId] = nd;
+  return c;
+}
+
 __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *__restrict__ tasks, int32_t bins) {
   typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
   __shared__ typename Scan::TempStorage scanTmp;
-  __shared__ uint32_t lp[kSmall], ltmp[kSmall], X[kSmall], Z[kSmall], rk[kSmall];
+  __shared__ uint32_t lp[kSmall];
   __shared__ uint8_t rflag[kSmall];
-  __shared__ uint32_t kmin[3 * kMaxBins * 3], kmax[3 * kMaxBins * 3], cnt[3 * kMaxBins];
-  __shared__ uint32_t stack[kSmall + 1];  // pending subtree roots; at most 1 + number of splits
+  __shared__ union {
+    SmallBlockArrays a;
+    SmallWaveArrays w[kBuildWaves];
+  } sh;
+  __shared__ uint32_t stack[kSmall + 1];   // pending nodes above kWaveNode primitives
+  __shared__ uint32_t queue[kSmall];       // pending nodes of at most kWaveNode primitives
   __shared__ int32_t sp;
+  __shared__ uint32_t qCount, qHead;
   __shared__ int32_t popped;
   __shared__ uint32_t curNode, P;
   __shared__ BNode cur;
   __shared__ Split split;
   __shared__ int32_t doSplit;
   __shared__ uint32_t totalR;
+  __shared__ unsigned long long best[2];
+  static_assert(kBuildThreads >= 3 * 64, "one thread per (axis, split position)");
 
   const SmallTask &tk = tasks[blockIdx.x];
   BNode *__restrict__ nodes = tk.nodes;
@@ -543,8 +781,17 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
   const BNode rn = nodes[root];
   const uint32_t base = rn.leftFirst, total = rn.primCount;
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) lp[e] = prim[base + e];
-  if (threadIdx.x == 0) { stack[0] = root; sp = 1; }
+  if (threadIdx.x == 0) {
+    sp = 0;
+    qCount = 0;
+    qHead = 0;
+    if (total > (uint32_t)kWaveNode) stack[sp++] = root;
+    else queue[qCount++] = root;
+  }
   __syncthreads();
+  uint32_t *const kmin = sh.a.kmin, *const kmax = sh.a.kmax, *const cnt = sh.a.cnt;
+  uint32_t *const ltmp = sh.a.ltmp, *const X = sh.a.X, *const Z = sh.a.Z, *const rk = sh.a.rk;
+  // Phase 1: nodes above kWaveNode primitives, split by the whole workgroup.
   while (true) {
     __syncthreads();  // every lane is done with the previous node's shared state
     if (threadIdx.x == 0) {
@@ -577,13 +824,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) {
-        Split s;
-        doSplit = sah_split(kmin, kmax, cnt, bins, f, minDim, n, s) ? 1 : 0;
-        split = s;
-      }
-      __syncthreads();
-      leaf = !doSplit;
+      leaf = !sah_split_block(kmin, kmax, cnt, bins, f, minDim, n, best, split, doSplit);
     }
     if (!leaf) {
       // Classify and rank (elements held blocked: thread t owns 4t..4t+3).
@@ -623,23 +864,93 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
       __syncthreads();
       for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) lp[first + e] = ltmp[e];
       if (threadIdx.x == 0) {
-        uint32_t c = atomicAdd(counter, 2u);
-        for (int side = 0; side < 2; side++) {
-          BNode ch{};
-          for (int k = 0; k < 3; k++) { ch.lo[k] = split.lo[side][k]; ch.hi[k] = split.hi[side][k]; }
-          ch.leftFirst = side == 0 ? cur.leftFirst : cur.leftFirst + totalL;
-          ch.primCount = side == 0 ? totalL : totalR;
-          ch.depth = cur.depth + 1;
-          ch.parent = (int32_t)curNode;
-          nodes[c + side] = ch;
+        const uint32_t c = emit_children(nodes, counter, curNode, cur, split, totalL, totalR);
+        const uint32_t count[2] = {totalL, totalR};
+        for (int side = 1; side >= 0; side--) {
+          if (count[side] > (uint32_t)kWaveNode) stack[sp++] = c + side;
+          else queue[qCount++] = c + side;
         }
-        BNode nd = cur;
-        nd.leftFirst = c;
-        nd.primCount = 0;
-        nodes[curNode] = nd;
-        stack[sp++] = c + 1;
-        stack[sp++] = c;
       }
+    }
+  }
+  // Phase 2: each wave takes queued nodes and splits them and their
+  // descendants on its own.
+  __syncthreads();
+  __threadfence_block();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  SmallWaveArrays &W = sh.w[wv];
+  while (true) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(&qHead, 1u);
+    item = __shfl(item, 0);
+    if (item >= qCount) break;
+    if (lane == 0) {
+      W.stack[0] = queue[item];
+      W.snode[0] = nodes[queue[item]];
+    }
+    int wsp = 1;
+    wave_sync();
+    while (wsp > 0) {
+      --wsp;
+      const uint32_t nodeId = W.stack[wsp];
+      const BNode nd = W.snode[wsp];
+      const uint32_t n = nd.primCount, first = nd.leftFirst - base;
+      const NodeFrame f = node_frame(nd.lo, nd.hi, bins, minDim);
+      if ((f.area == 0.0) || (n == 1)) continue;
+      for (int i = lane; i < 3 * bins * 3; i += 64) { W.kmin[i] = 0xffffffffu; W.kmax[i] = 0u; }
+      for (int i = lane; i < 3 * bins; i += 64) W.cnt[i] = 0;
+      wave_sync();
+      const bool live = (uint32_t)lane < n;
+      const uint32_t p = live ? lp[first + lane] : 0u;
+      float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (live) {
+        c = cen[p];
+        const float4 a = plo[p], b = phi[p];
+        const float cc[3] = {c.x, c.y, c.z}, mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
+        for (int ax = 0; ax < 3; ax++) {
+          const int bin = ax * bins + bin_of(cc[ax], f.nMin[ax], f.scale[ax], bins);
+          atomicAdd(&W.cnt[bin], 1u);
+          for (int k = 0; k < 3; k++) {
+            atomicMin(&W.kmin[bin * 3 + k], fkey(mn[k]));
+            atomicMax(&W.kmax[bin * 3 + k], fkey(mx[k]));
+          }
+        }
+      }
+      wave_sync();
+      Split sp2;
+      if (!sah_split_wave(W.kmin, W.kmax, W.cnt, bins, f, minDim, n, sp2)) continue;
+      // The swap loop's permutation (see k_small_build's workgroup form) from
+      // wave ballots: X[k] = k-th right element from the front, Z[k] = k-th
+      // left element from the back.
+      bool r = false;
+      if (live) {
+        const float val = sp2.axis == 0 ? c.x : (sp2.axis == 1 ? c.y : c.z);
+        r = bin_of(val, f.nMin[sp2.axis], f.scale[sp2.axis], bins) > sp2.pos;
+      }
+      const unsigned long long rmask = __ballot(r);
+      const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      const uint32_t ex = (uint32_t)__popcll(rmask & below), nR = (uint32_t)__popcll(rmask);
+      const uint32_t nL = n - nR;
+      uint32_t rank = 0;
+      if (live) {
+        if (r) { W.X[ex] = lane; rank = ex; }
+        else { rank = nL - ((uint32_t)lane - ex) - 1; W.Z[rank] = lane; }
+      }
+      wave_sync();
+      const uint32_t Pw = (uint32_t)__popcll(__ballot(live && r && rank < nL && (uint32_t)lane < W.Z[rank]));
+      if (live) W.tmp[partition_dest(lane, r, rank, Pw, n, nR, W.X, W.Z)] = p;
+      wave_sync();
+      if (live) lp[first + lane] = W.tmp[lane];
+      if (lane == 0) {
+        BNode ch[2];
+        const uint32_t c0 = emit_children(nodes, counter, nodeId, nd, sp2, nL, nR, ch);
+        W.stack[wsp] = c0 + 1;
+        W.snode[wsp] = ch[1];
+        W.stack[wsp + 1] = c0;
+        W.snode[wsp + 1] = ch[0];
+      }
+      wsp += 2;
+      wave_sync();
     }
   }
   __syncthreads();
@@ -647,15 +958,41 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
 }
 
 // --------------------------------------------------------------- layout
+// Node counts per depth: a block counts its grid-stride share in LDS and
+// adds each nonzero count to the global histogram once.
+constexpr uint32_t kDepthBins = 256;
 __global__ void k_depth_hist(const BNode *__restrict__ nodes, uint32_t n, uint32_t *__restrict__ hist) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&hist[min(nodes[i].depth, 255u)], 1u);
+  __shared__ uint32_t h[kDepthBins];
+  for (uint32_t d = threadIdx.x; d < kDepthBins; d += blockDim.x) h[d] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&h[min(nodes[i].depth, kDepthBins - 1)], 1u);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kDepthBins; d += blockDim.x)
+    if (h[d]) atomicAdd(&hist[d], h[d]);
 }
 
+// Node ids grouped by depth (any order within a depth): a block counts its
+// share per depth in LDS, reserves one range per depth with a single global
+// atomic, then writes its nodes into the ranges.
 __global__ void k_depth_scatter(const BNode *__restrict__ nodes, uint32_t n, uint32_t *__restrict__ cursor,
                                 uint32_t *__restrict__ byDepth) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) byDepth[atomicAdd(&cursor[min(nodes[i].depth, 255u)], 1u)] = i;
+  __shared__ uint32_t cnt[kDepthBins], base[kDepthBins];
+  for (uint32_t d = threadIdx.x; d < kDepthBins; d += blockDim.x) cnt[d] = 0;
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    atomicAdd(&cnt[min(nodes[i].depth, kDepthBins - 1)], 1u);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kDepthBins; d += blockDim.x) {
+    base[d] = cnt[d] ? atomicAdd(&cursor[d], cnt[d]) : 0u;
+    cnt[d] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t d = min(nodes[i].depth, kDepthBins - 1);
+    byDepth[base[d] + atomicAdd(&cnt[d], 1u)] = i;
+  }
 }
 
 // Interior-node subtree sizes, one depth level at a time (deepest first).
@@ -708,6 +1045,17 @@ __global__ void k_leaf_counts(const BNode *__restrict__ nodes, uint32_t n, uint3
   const BNode nd = nodes[i];
   if (!nd.primCount) return;
   for (uint32_t k = 0; k < nd.primCount; k++) leafCount[nd.leftFirst + k] = nd.primCount;
+}
+
+// Largest leaf primitive count of a tree, max-reduced into out[0], which must
+// hold no more than it: the caller passes the leaf-count array, whose first
+// entry is the count of slot 0's leaf (k_emit_tris has read it by then).
+__global__ void k_max_leaf(const BNode *__restrict__ nodes, uint32_t n, uint32_t *__restrict__ out) {
+  uint32_t v = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v = max(v, nodes[i].primCount);
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
 }
 
 __global__ void k_emit_tris(const uint32_t *__restrict__ prim, const uint32_t *__restrict__ source,
@@ -857,7 +1205,8 @@ static void build_topology(Topology &T, uint32_t bins, hipStream_t s, std::vecto
   DevBuf<unsigned long long> keys(6);
   unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
   keys.upload(init, 6, s);
-  hipLaunchKernelGGL(k_root_reduce, dim3(blocks(n)), dim3(256), 0, s, T.plo.ptr, T.phi.ptr, n, keys.ptr, T.prim.ptr);
+  hipLaunchKernelGGL(k_root_reduce, dim3(std::min<uint32_t>(blocks(n), 2048)), dim3(256), 0, s, T.plo.ptr, T.phi.ptr,
+                     n, keys.ptr, T.prim.ptr);
   hipLaunchKernelGGL(k_root_init, dim3(1), dim3(64), 0, s, keys.ptr, T.plo.ptr, T.phi.ptr, n, T.nodes.ptr, counter.ptr);
   BNode root;
   T.nodes.download(&root, 1, s);
@@ -865,42 +1214,56 @@ static void build_topology(Topology &T, uint32_t bins, hipStream_t s, std::vecto
   const double minDim[3] = {((double)root.hi[0] - (double)root.lo[0]) * 1e-20,
                             ((double)root.hi[1] - (double)root.lo[1]) * 1e-20,
                             ((double)root.hi[2] - (double)root.lo[2]) * 1e-20};
-  std::vector<uint32_t> bigIds, smallIds;
-  if (n > (uint32_t)kSmall) bigIds.push_back(0); else smallIds.push_back(0);
+  std::vector<uint32_t> smallIds;
+  uint32_t nb = n > (uint32_t)kSmall ? 1u : 0u;
+  if (!nb) smallIds.push_back(0);
+  // Level buffers, sized once: a big node holds more than kSmall primitives,
+  // so a level has at most n / kSmall of them and n / kChunk + nb chunks.
+  const uint32_t maxBig = n / (uint32_t)kSmall + 1u, maxChunks = n / (uint32_t)kChunk + maxBig + 1u;
   DevBuf<uint8_t> isR;
-  DevBuf<uint32_t> X, Z, rank, tmp;
-  DevBuf<uint32_t> dNext, dSmall, dCounts(2);
-  if (!bigIds.empty()) {
+  DevBuf<uint32_t> X, Z, rank, tmp, dNext, dSmall, dCounts(2), dFirst, gkmin, gkmax, gcnt, chunkR, chunkRB;
+  DevBuf<BNode> dLevel;
+  DevBuf<BigState> dBig;
+  DevBuf<Chunk> dChunks;
+  if (nb) {
     isR.alloc(n); X.alloc(n); Z.alloc(n); rank.alloc(n); tmp.alloc(n);
-    dNext.alloc(n / kSmall * 2 + 4);
-    dSmall.alloc(n / kSmall * 2 + 4);
+    dNext.alloc(2 * maxBig + 4);
+    dSmall.alloc(2 * maxBig + 4);
+    dLevel.alloc(maxBig);
+    dBig.alloc(maxBig);
+    dFirst.alloc(maxBig + 1);
+    dChunks.alloc(maxChunks);
+    chunkR.alloc(maxChunks);
+    chunkRB.alloc(maxChunks);
+    gkmin.alloc((size_t)maxBig * 3 * bins * 3);
+    gkmax.alloc((size_t)maxBig * 3 * bins * 3);
+    gcnt.alloc((size_t)maxBig * 3 * bins);
+    const uint32_t root = 0;
+    dNext.upload(&root, 1, s);
   }
-  while (!bigIds.empty()) {
-    const uint32_t nb = (uint32_t)bigIds.size();
-    std::vector<BNode> hn(nb);
-    for (uint32_t i = 0; i < nb; i++)
-      MTX_HIP(hipMemcpyAsync(&hn[i], T.nodes.ptr + bigIds[i], sizeof(BNode), hipMemcpyDeviceToHost, s));
+  std::vector<BNode> hn;
+  std::vector<Chunk> chunks;
+  std::vector<uint32_t> firstChunk;
+  while (nb) {
+    MTX_REQUIRE(nb <= maxBig, MTX_ERROR, "BVH builder: big-node level larger than its bound");
+    hipLaunchKernelGGL(k_level_nodes, dim3(blocks(nb, 64)), dim3(64), 0, s, T.nodes.ptr, dNext.ptr, nb, dLevel.ptr,
+                       dBig.ptr);
+    hn.resize(nb);
+    dLevel.download(hn.data(), nb, s);
     MTX_HIP(hipStreamSynchronize(s));
-    std::vector<Chunk> chunks;
-    std::vector<uint32_t> firstChunk(nb + 1);
-    std::vector<BigState> states(nb);
+    chunks.clear();
+    firstChunk.resize(nb + 1);
     for (uint32_t b = 0; b < nb; b++) {
       firstChunk[b] = (uint32_t)chunks.size();
       for (uint32_t e = 0; e < hn[b].primCount; e += kChunk)
         chunks.push_back(Chunk{b, e, std::min<uint32_t>(e + kChunk, hn[b].primCount), 0});
-      states[b] = BigState{};
-      states[b].node = bigIds[b];
     }
     firstChunk[nb] = (uint32_t)chunks.size();
     const uint32_t nc = (uint32_t)chunks.size();
-    DevBuf<Chunk> dChunks;
+    MTX_REQUIRE(nc <= maxChunks, MTX_ERROR, "BVH builder: chunk list larger than its bound");
     dChunks.upload(chunks.data(), nc, s);
-    DevBuf<uint32_t> dFirst;
     dFirst.upload(firstChunk.data(), nb + 1, s);
-    DevBuf<BigState> dBig;
-    dBig.upload(states.data(), nb, s);
     const uint32_t nBins = nb * 3 * bins;
-    DevBuf<uint32_t> gkmin(nBins * 3), gkmax(nBins * 3), gcnt(nBins), chunkR(nc), chunkRB(nc);
     hipLaunchKernelGGL(k_bins_clear, dim3(blocks(nBins * 3)), dim3(256), 0, s, gkmin.ptr, gkmax.ptr, gcnt.ptr, nBins);
     hipLaunchKernelGGL(k_bin_chunks, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
                        T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2],
@@ -919,23 +1282,21 @@ static void build_topology(Topology &T, uint32_t bins, hipStream_t s, std::vecto
                        isR.ptr, X.ptr, Z.ptr, rank.ptr, T.prim.ptr, tmp.ptr);
     hipLaunchKernelGGL(k_copyback, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
                        tmp.ptr, T.prim.ptr);
-    uint32_t zero[2] = {0, 0};
-    dCounts.upload(zero, 2, s);
+    MTX_HIP(hipMemsetAsync(dCounts.ptr, 0, 2 * sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_children, dim3(blocks(nb, 64)), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr, counter.ptr,
                        dNext.ptr, dCounts.ptr, dSmall.ptr, dCounts.ptr + 1);
     MTX_HIP(hipGetLastError());
     uint32_t hc[2];
     dCounts.download(hc, 2, s);
     MTX_HIP(hipStreamSynchronize(s));
-    bigIds.resize(hc[0]);
-    if (hc[0]) dNext.download(bigIds.data(), hc[0], s);
+    nb = hc[0];
     if (hc[1]) {
       size_t old = smallIds.size();
       smallIds.resize(old + hc[1]);
       dSmall.download(smallIds.data() + old, hc[1], s);
     }
-    MTX_HIP(hipStreamSynchronize(s));
   }
+  MTX_HIP(hipStreamSynchronize(s));
   for (uint32_t r : smallIds)
     small.push_back(SmallTask{T.nodes.ptr, T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, counter.ptr,
                               {minDim[0], minDim[1], minDim[2]}, r, 0u});
@@ -968,7 +1329,8 @@ static void layout_topology(Topology &T, Layout &L, hipStream_t s) {
   DevBuf<uint32_t> hist(kLevels), cursor(kLevels), byDepth(nn), size(nn), dfs(nn);
   std::vector<uint32_t> zeros(kLevels, 0);
   hist.upload(zeros.data(), kLevels, s);
-  hipLaunchKernelGGL(k_depth_hist, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, hist.ptr);
+  hipLaunchKernelGGL(k_depth_hist, dim3(std::min<uint32_t>(blocks(nn), 2048)), dim3(256), 0, s, T.nodes.ptr, nn,
+                     hist.ptr);
   std::vector<uint32_t> h = hist.to_host(kLevels, s);
   uint32_t maxDepth = 0;
   for (uint32_t d = 0; d < kLevels; d++) if (h[d]) maxDepth = d;
@@ -976,7 +1338,8 @@ static void layout_topology(Topology &T, Layout &L, hipStream_t s) {
   std::vector<uint32_t> start(kLevels + 1, 0);
   for (uint32_t d = 0; d < kLevels; d++) start[d + 1] = start[d] + h[d];
   cursor.upload(start.data(), kLevels, s);
-  hipLaunchKernelGGL(k_depth_scatter, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, cursor.ptr, byDepth.ptr);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(std::min<uint32_t>(blocks(nn), 2048)), dim3(256), 0, s, T.nodes.ptr, nn,
+                     cursor.ptr, byDepth.ptr);
   for (uint32_t d = maxDepth; d >= 1; d--)
     if (h[d]) hipLaunchKernelGGL(k_subtree, dim3(blocks(h[d])), dim3(256), 0, s, T.nodes.ptr, byDepth.ptr + start[d], h[d], size.ptr);
   uint32_t zero = 0;
@@ -1073,8 +1436,10 @@ std::vector<std::shared_ptr<Blas>> build_blases(const MeshInput *meshes, size_t 
     hipLaunchKernelGGL(k_emit_tris, dim3(blocks(n)), dim3(256), 0, s, T.prim.ptr, T.source.ptr, leafCount.ptr, n,
                        w.dpos, w.didx, B.tris.ptr, B.triSource.ptr);
     MTX_HIP(hipGetLastError());
-    std::vector<BNode> hn = T.nodes.to_host(T.nodeCount, s);
-    for (auto &nd : hn) B.maxLeaf = std::max(B.maxLeaf, nd.primCount);
+    hipLaunchKernelGGL(k_max_leaf, dim3(std::min<uint32_t>(blocks(T.nodeCount), 1024)), dim3(256), 0, s, T.nodes.ptr,
+                       T.nodeCount, leafCount.ptr);
+    MTX_HIP(hipGetLastError());
+    B.maxLeaf = leafCount.to_host(1, s)[0];
     // The build scratch (centers, input boxes) is not needed on the hot path.
     T.cen.release();
     T.plo.release();
@@ -1150,15 +1515,16 @@ static void ensure_refit_data(Blas &B, hipStream_t s) {
   DevBuf<uint32_t> hist(kLevels), cursor(kLevels), size(nn);
   std::vector<uint32_t> zeros(kLevels, 0);
   hist.upload(zeros.data(), kLevels, s);
-  hipLaunchKernelGGL(k_depth_hist, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, hist.ptr);
+  hipLaunchKernelGGL(k_depth_hist, dim3(std::min<uint32_t>(blocks(nn), 2048)), dim3(256), 0, s, T.nodes.ptr, nn,
+                     hist.ptr);
   std::vector<uint32_t> h = hist.to_host(kLevels, s);
   std::vector<uint32_t> start(kLevels + 1, 0);
   for (uint32_t d = 0; d < kLevels; d++) start[d + 1] = start[d] + h[d];
   B.refit.byDepth.alloc(nn);
   B.refit.dfs.alloc(nn);
   cursor.upload(start.data(), kLevels, s);
-  hipLaunchKernelGGL(k_depth_scatter, dim3(blocks(nn)), dim3(256), 0, s, T.nodes.ptr, nn, cursor.ptr,
-                     B.refit.byDepth.ptr);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(std::min<uint32_t>(blocks(nn), 2048)), dim3(256), 0, s, T.nodes.ptr, nn,
+                     cursor.ptr, B.refit.byDepth.ptr);
   for (uint32_t d = T.maxDepth; d >= 1; d--)
     if (h[d])
       hipLaunchKernelGGL(k_subtree, dim3(blocks(h[d])), dim3(256), 0, s, T.nodes.ptr, B.refit.byDepth.ptr + start[d],
