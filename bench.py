@@ -286,10 +286,12 @@ def denoiser_benchmark(full_precision=False, iterations=20):
     color, albedo, normal = (t.read_pixels(k) for k in ("color", "albedo", "normal"))
     norm = 1.0 / _pu_forward(65504.0)
     x = np.zeros((1088, 1920, 9), np.float32)
-    x[:1080, :, 0:3] = _pu_forward(np.maximum(color[..., :3].astype(np.float64), 0)) * norm
+    scale = unet.autoexposure(color)
+    x[:1080, :, 0:3] = _pu_forward(np.maximum(color[..., :3].astype(np.float64) * scale, 0)) * norm
     x[:1080, :, 3:6] = np.clip(albedo[..., :3], 0, 1)
-    # readPixels('normal') is display-encoded (n * 0.5 + 0.5, A = 1 where valid)
-    x[:1080, :, 6:9] = np.clip(np.where(normal[..., 3:4] > 0, normal[..., :3] * 2.0 - 1.0, 0.0), -1, 1)
+    # readPixels('normal') is display-encoded (n * 0.5 + 0.5, A = 1 where valid),
+    # the network's normal input is the same mapping of n (0.5 where n = 0).
+    x[:1080, :, 6:9] = np.where(normal[..., 3:4] > 0, np.clip(normal[..., :3], 0, 1), 0.5)
     gflop = unet.flops(1088, 1920) / 1e9
     weights = unet.read_tza(denoise.WEIGHTS)
     crop = x if full_precision else x[416:672, 832:1088]
@@ -311,6 +313,24 @@ def denoiser_benchmark(full_precision=False, iterations=20):
                      "output_max_abs_error": float(diff.max()), "output_mean_abs_error": float(diff.mean()),
                      "output_p99_abs_error": float(np.percentile(diff, 99))}
         log(f"denoiser {prec}: {ms:.2f} ms, {gflop / ms:.1f} TFLOP/s, output max err {diff.max():.2e}")
+    # The other network shapes at the same size, FP16: the bundled small
+    # model and the large clean-auxiliary model with its two prefilters
+    # (weights of the large models' exact shapes, He-scaled random values:
+    # they are not bundled, and the time does not depend on the values).
+    small = unet.read_tza(os.path.join(denoise.WEIGHTS_DIR, "rt_hdr_alb_nrm_small.tza"))
+    large9 = unet.synthetic_weights(unet.network_shapes(unet.WIDTHS["large"], 9, True))
+    large3 = unet.synthetic_weights(unet.network_shapes(unet.WIDTHS["large"], 3, True))
+    out["models_fp16"] = {}
+    for model, w, src in (("small", small, os.path.join(denoise.WEIGHTS_DIR, "rt_hdr_alb_nrm_small.tza")),
+                          ("cleanAuxLarge", large9, unet.write_tza(large9)),
+                          ("prefilter_large", large3, unet.write_tza(large3))):
+        d = denoise.Denoiser("fp16", src)
+        d.benchmark(1920, 1088, 50)
+        ms = sorted(d.benchmark(1920, 1088, iterations) for _ in range(3))[1]
+        g = unet.flops(1088, 1920, w) / 1e9
+        out["models_fp16"][model] = {"ms_per_pass": round(ms, 3), "gflop_per_pass": round(g, 2),
+                                     "tflops": round(g / ms, 2), "mfma_frac": round(g / ms / 2500.0, 4)}
+        log(f"denoiser {model} fp16: {ms:.2f} ms, {g / ms:.1f} TFLOP/s")
     return out
 
 

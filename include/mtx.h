@@ -382,7 +382,9 @@ int32_t mtx_target_create(mtx_handle scene, const mtx_target_settings *settings,
                           mtx_handle *out_target, char *error, size_t error_capacity);
 
 /* Adds `samples` path samples to every pixel on `stream` (NULL = default);
- * asynchronous. One megakernel launch per call. */
+ * asynchronous. The target's kernel setting chooses the work: the wavefront
+ * form (per bounce: trace, shade, shadow and compaction kernels, replayed
+ * from a HIP graph) or one megakernel launch per call. */
 int32_t mtx_target_render(mtx_handle target, uint32_t samples, void *stream);
 
 /* mtx_target_render plus per-launch totals of the traversal work, in two
@@ -710,10 +712,14 @@ int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]);
 int32_t mtx_server_destroy(mtx_handle server);
 
 /* ------------------------------------------------------------- denoiser */
-/* OIDN RT U-Net ("rt_hdr_alb_nrm", the TZA weights the reference ships) on
- * MFMA with FP32 accumulation. Replaces mt_denoiser_create/run/destroy
- * (mutracer-denoise.h:41-53) and the browser path's tileExecute call
- * (WGPURenderingPipeline.ts:1445-1476). */
+/* OIDN RT U-Nets (the TZA weights the reference ships) on MFMA with FP32
+ * accumulation: the "UNet" shape of the standard and small models
+ * (rt_hdr_alb_nrm, rt_hdr_alb_nrm_small) and the "UNetLarge" shape of the
+ * clean-auxiliary model and its guide prefilters (rt_hdr_calb_cnrm_large,
+ * rt_alb_large, rt_nrm_large), told apart by their layer names, channel
+ * counts read from the weights. Replaces mt_denoiser_create/run/destroy
+ * (mutracer-denoise.h:41-53) and the browser path's tileExecute calls
+ * (WGPURenderingPipeline.ts:1352-1476). */
 #define MTX_DENOISE_FP16 1u
 #define MTX_DENOISE_BF16 2u
 /* weights: the bytes of a version-2 tensor archive (.tza). */
@@ -739,21 +745,24 @@ int32_t mtx_denoiser_backend(mtx_handle denoiser, uint32_t *output, char *error,
 int32_t mtx_denoiser_create_from_weights(const void *weights, size_t size, uint32_t precision,
                                          mtx_handle *out_denoiser, char *error, size_t error_capacity);
 /* mt_denoiser_run: color, normal, albedo, output are RGBA f32 (host), count =
- * width*height*4; HDR color through the 'pu' transfer; output alpha = color
+ * width*height*4; OIDN's RT input processing: HDR color times the input
+ * scale (autoexposure unless set) through the 'pu' transfer, albedo clamped
+ * to [0, 1], normals in [-1, 1] mapped to [0, 1]; output alpha = color
  * alpha; output may alias an input; nothing is written on failure. */
 int32_t mtx_denoiser_run(mtx_handle denoiser, uint32_t width, uint32_t height, const float *color,
                          const float *normal, const float *albedo, float *output, size_t count, char *error,
                          size_t error_capacity);
-/* The network alone: input H x W x 9 f32 (host, H and W multiples of 16) ->
- * output H x W x 3 f32; layer_outputs (16 pointers or NULL) receive each
- * convolution's output (H_l x W_l x C f32, NULL entries skipped). No
+/* The network alone: input H x W x C f32 (host, H and W multiples of 16, C
+ * the network's input channels, 9 or 3) -> output H x W x 3 f32;
+ * layer_outputs (one pointer per convolution, or NULL) receive each
+ * convolution's output (H_l x W_l x C_l f32, NULL entries skipped). No
  * reference counterpart: it exposes the per-layer precision. */
 int32_t mtx_denoiser_forward(mtx_handle denoiser, uint32_t width, uint32_t height, const float *input,
                              float *output, float *const *layer_outputs, char *error, size_t error_capacity);
 /* Average GPU time of `iterations` network passes at width x height (HIP events). */
 int32_t mtx_denoiser_benchmark(mtx_handle denoiser, uint32_t width, uint32_t height, uint32_t iterations,
                                double *ms_per_forward);
-/* Launch shape of layer `layer`'s convolution (0..15 in network order): each
+/* Launch shape of layer `layer`'s convolution (network order): each
  * wave computes pixel_tiles x 16 pixels of a row (1, 2, 4 or 8) for
  * channel_tiles of the layer's 16-channel output tiles (1..7); pixel_tiles
  * or'ed with MTX_DENOISER_STAGED selects the kernel whose blocks of four
@@ -765,6 +774,32 @@ int32_t mtx_denoiser_set_tiling(mtx_handle denoiser, uint32_t layer, uint32_t pi
 /* The launch shape `layer` uses at width x height. */
 int32_t mtx_denoiser_get_tiling(mtx_handle denoiser, uint32_t layer, uint32_t width, uint32_t height,
                                 uint32_t *pixel_tiles, uint32_t *channel_tiles);
+/* Prefilters of the guides (the clean-auxiliary pipeline,
+ * WGPURenderingPipeline.ts:1352-1440 and oidn_webgpu.rs:84-150): two other
+ * denoisers made from 3-channel networks (rt_alb_*, rt_nrm_*) of this one's
+ * precision. mtx_denoiser_run then passes the albedo (sRGB-encoded, clamped
+ * to [0, 1]) and the normal (mapped to [0, 1]) through them
+ * (oidnAuxiliary.wgsl.ts:2-16), decodes the results (sRGB to linear; [0, 1]
+ * back to [-1, 1]) and uses them as this network's guides. Both 0 removes
+ * them. No C-ABI counterpart: the reference's loader takes them as
+ * auxiliaryModelUrls (oidnNetwork.ts:94-141). */
+int32_t mtx_denoiser_set_auxiliary(mtx_handle denoiser, mtx_handle albedo_filter, mtx_handle normal_filter,
+                                   char *error, size_t error_capacity);
+/* Image tiling of mtx_denoiser_run (oidn-webgpu's maxTileSize / tileOverlap,
+ * WGPURenderingPipeline.ts:1330-1342): tiles of max_tile_size pixels (0 = the
+ * whole image in one pass, the default), each run on its region grown by
+ * `overlap` pixels per side (clipped to the image) and kept inside the tile;
+ * multiples of 16. A tile whose region covers the image gives the untiled
+ * result; smaller regions differ near tile edges (the network sees less
+ * context there). */
+int32_t mtx_denoiser_set_tiles(mtx_handle denoiser, uint32_t max_tile_size, uint32_t overlap);
+/* Scale of the HDR color before the transfer (OIDN's inputScale; the output
+ * is divided by it): NaN (the default) computes OIDN's autoexposure of each
+ * image, 0.18 over the geometric mean of its 16 x 16-bin mean luminances. */
+int32_t mtx_denoiser_set_input_scale(mtx_handle denoiser, float scale);
+/* info[0] convolution layers (16 or 19), [1] input channels (9 or 3), [2] 1
+ * for the UNetLarge shape, [3] precision (MTX_DENOISE_FP16 / _BF16). */
+int32_t mtx_denoiser_info(mtx_handle denoiser, uint32_t info[4]);
 /* mt_denoiser_destroy (mutracer-denoise.h:53); MTX_BUSY while a run is in progress. */
 int32_t mtx_denoiser_destroy(mtx_handle denoiser, char *error, size_t error_capacity);
 

@@ -146,6 +146,10 @@ def declare(L):
         "mtx_denoiser_benchmark": (i32, [u64, u32, u32, u32, P(C.c_double)]),
         "mtx_denoiser_set_tiling": (i32, [u64, u32, u32, u32]),
         "mtx_denoiser_get_tiling": (i32, [u64, u32, u32, u32, P(u32), P(u32)]),
+        "mtx_denoiser_set_auxiliary": (i32, [u64, u64, u64, C.c_char_p, sz]),
+        "mtx_denoiser_set_tiles": (i32, [u64, u32, u32]),
+        "mtx_denoiser_set_input_scale": (i32, [u64, C.c_float]),
+        "mtx_denoiser_info": (i32, [u64, P(u32)]),
         "mtx_denoiser_destroy": (i32, [u64, C.c_char_p, sz]),
     }
     for name, (res, args) in sig.items():

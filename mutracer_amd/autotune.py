@@ -112,7 +112,7 @@ def tune_denoiser(denoiser, width, height, iterations=3, log=None):
     pass time before and after."""
     before = min(denoiser.benchmark(width, height, iterations) for _ in range(2))
     best = {}
-    for layer in range(16):
+    for layer in range(len(denoiser.layers)):
         cur = denoiser.tiling(layer, width, height)
         trials = []
         for mt, nt in DENOISER_SHAPES:

@@ -1,12 +1,16 @@
 // OIDN RT U-Net denoiser on MFMA (gfx950), FP16 or BF16 storage with FP32
 // accumulation.
 //
-// Network (OIDN 2.3 "rt_hdr_alb_nrm", weights from the TZA the reference
-// ships, oidn/weights/rt_hdr_alb_nrm.tza; call site WGPURenderingPipeline.ts
-// :1445-1476): 16 3x3 convolutions, 4 max pools, 4 nearest upsamples with skip
-// concatenation (upsampled tensor first). Tensors are NHWC with channels padded
-// to a multiple of 16 (padding channels hold zeros), so a 16-channel slice of
-// one pixel is one 32-byte run.
+// Networks (OIDN 2.3's RT U-Nets, weights from the TZAs the reference ships
+// or fetches, oidn/weights/*.tza; call sites WGPURenderingPipeline.ts
+// :1352-1476): "UNet" (rt_hdr_alb_nrm, rt_hdr_alb_nrm_small: 16 3x3
+// convolutions) or "UNetLarge" (rt_hdr_calb_cnrm_large and the rt_alb_large /
+// rt_nrm_large guide prefilters: 19), each with 4 max pools and 4 nearest
+// upsamples with skip concatenation (upsampled tensor first); the layer list
+// is data read from the weights. Tensors are NHWC with channels padded to a
+// multiple of 16 (padding channels hold zeros), so a 16-channel slice of one
+// pixel is one 32-byte run. mtx_denoiser_run adds OIDN's autoexposure, the
+// input / output transfers, the guide prefilters and image tiles.
 //
 // Convolution = implicit GEMM on gfx950's v_mfma_f32_16x16x32 (f16 / bf16):
 // the M = 16 rows of an MFMA tile are 16 consecutive output pixels of a row,
@@ -24,7 +28,9 @@
 // storage type are fused into the epilogue.
 #include <dlfcn.h>
 
+#include <cfloat>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -392,28 +398,119 @@ MTX_HD double pu_inverse(double x) {
   return mtxfm::exp_d((x - Pu::G) / Pu::E) - Pu::F;
 }
 
-// Input tensor (H x W padded, 16 channels): pu(color * scale) * norm, albedo
-// clamped to [0,1], normal clamped to [-1,1]; zero outside the image.
+// What a network reads and writes: the HDR beauty pass (color through the pu
+// transfer, albedo and normal guides), or one auxiliary guide through the
+// prefilter networks' transfers (oidnAuxiliary.wgsl.ts:2-16: albedo as
+// sRGB-encoded [0, 1], normals mapped to [0, 1]).
+enum { kNetBeauty = 0, kNetAlbedo = 1, kNetNormal = 2 };
+
+__device__ __forceinline__ float nan_to_zero(float v) { return v == v ? v : 0.0f; }
+__device__ __forceinline__ float srgb_encode_clamped(float v) {
+  const float l = fminf(fmaxf(v, 0.0f), 1.0f);
+  return l <= 0.0031308f ? 12.92f * l : 1.055f * mtxfm::powf(l, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ float srgb_decode_clamped(float v) {
+  const float e = fminf(fmaxf(v, 0.0f), 1.0f);
+  return e <= 0.04045f ? e / 12.92f : mtxfm::powf((e + 0.055f) / 1.055f, 2.4f);
+}
+
+// OIDN's autoexposure of HDR color, the input scale its RT filter uses when
+// none is set (mt_denoiser_run sets none): the image is cut into
+// ceil(h / 16) x ceil(w / 16) bins with bounds b * h / nbh (integer division);
+// each bin keeps the mean luminance (0.212671 r + 0.715160 g + 0.072169 b of
+// the color clamped to [0, FLT_MAX], NaN read as 0) of its pixels; the scale
+// is 0.18 / 2^(mean of log2 L over the bins with L > 1e-8), 1 when no bin is.
+__device__ __forceinline__ float ae_luminance(float4 c) {
+  const float r = fminf(fmaxf(c.x, 0.0f), FLT_MAX), g = fminf(fmaxf(c.y, 0.0f), FLT_MAX);
+  const float b = fminf(fmaxf(c.z, 0.0f), FLT_MAX);
+  return 0.212671f * r + 0.715160f * g + 0.072169f * b;
+}
+__global__ void __launch_bounds__(256) k_ae_bins(const float4 *__restrict__ color, int w, int h, int nbw, int nbh,
+                                                 float *__restrict__ bins) {
+  __shared__ float red[256];
+  const int b = blockIdx.x, bh = b / nbw, bw = b - bh * nbw;
+  const int y0 = (int)((int64_t)bh * h / nbh), y1 = (int)((int64_t)(bh + 1) * h / nbh);
+  const int x0 = (int)((int64_t)bw * w / nbw), x1 = (int)((int64_t)(bw + 1) * w / nbw);
+  const int bwid = x1 - x0, n = bwid * (y1 - y0);
+  float L = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int yy = y0 + i / bwid, xx = x0 + i % bwid;
+    L += ae_luminance(color[(size_t)yy * w + xx]);
+  }
+  red[threadIdx.x] = L;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bins[b] = red[0] / (float)n;
+}
+__global__ void __launch_bounds__(1024) k_ae_scale(const float *__restrict__ bins, int n, float *__restrict__ scale) {
+  __shared__ float rs[1024];
+  __shared__ int rc[1024];
+  float s = 0.0f;
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const float L = bins[i];
+    if (L > 1e-8f) {
+      s += log2f(L);
+      c++;
+    }
+  }
+  rs[threadIdx.x] = s;
+  rc[threadIdx.x] = c;
+  __syncthreads();
+  for (int k = 512; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      rs[threadIdx.x] += rs[threadIdx.x + k];
+      rc[threadIdx.x] += rc[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scale[0] = rc[0] > 0 ? 0.18f / exp2f(rs[0] / (float)rc[0]) : 1.0f;
+}
+
+// Input tensor of a network region (H x W, 16 channels) whose pixel (0, 0)
+// is image pixel (rx, ry); zero outside the image. Beauty (OIDN's RT input
+// processing): pu(clamp(color * scale, 0, FLT_MAX)) * norm, albedo clamped
+// to [0, 1], normal clamped to [-1, 1] and mapped to [0, 1] (n * 0.5 + 0.5);
+// albedo prefilter: the sRGB-encoded albedo; normal prefilter: normal * 0.5
+// + 0.5, clamped to [0, 1]. The scale is *scalePtr when scalePtr is set
+// (the autoexposure result), else scaleValue.
 template <bool kBf16>
 __global__ void k_denoise_in(const float4 *__restrict__ color, const float4 *__restrict__ albedo,
-                             const float4 *__restrict__ normal, int w, int h, int H, int W, float scale, float norm,
+                             const float4 *__restrict__ normal, int w, int h, int rx, int ry, int H, int W, int kind,
+                             const float *__restrict__ scalePtr, float scaleValue, float norm,
                              uint16_t *__restrict__ dst) {
   const size_t n = (size_t)H * W;
+  const float scale = scalePtr ? *scalePtr : scaleValue;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % W), y = (int)(i / W);
+    const int x = (int)(i % W) + rx, y = (int)(i / W) + ry;
     float v[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) v[k] = 0.0f;
     if (x < w && y < h) {
       const size_t s = (size_t)y * w + x;
-      const float4 c = color[s], al = albedo[s], nr = normal[s];
-      const float cc[3] = {c.x, c.y, c.z}, aa[3] = {al.x, al.y, al.z}, nn[3] = {nr.x, nr.y, nr.z};
+      if (kind == kNetBeauty) {
+        const float4 c = color[s], al = albedo[s], nr = normal[s];
+        const float cc[3] = {c.x, c.y, c.z}, aa[3] = {al.x, al.y, al.z}, nn[3] = {nr.x, nr.y, nr.z};
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const double cv = fmax((double)cc[k] * scale, 0.0);
-        v[k] = (float)(pu_forward(cv) * norm);
-        v[3 + k] = fminf(fmaxf(aa[k], 0.0f), 1.0f);
-        v[6 + k] = fminf(fmaxf(nn[k], -1.0f), 1.0f);
+        for (int k = 0; k < 3; k++) {
+          const double cv = fmin(fmax((double)(cc[k] * scale), 0.0), (double)FLT_MAX);
+          v[k] = (float)(pu_forward(cv) * norm);
+          v[3 + k] = fminf(fmaxf(aa[k], 0.0f), 1.0f);
+          v[6 + k] = fminf(fmaxf(nan_to_zero(nn[k]), -1.0f), 1.0f) * 0.5f + 0.5f;
+        }
+      } else if (kind == kNetAlbedo) {
+        const float4 al = albedo[s];
+        v[0] = srgb_encode_clamped(al.x);
+        v[1] = srgb_encode_clamped(al.y);
+        v[2] = srgb_encode_clamped(al.z);
+      } else {
+        const float4 nr = normal[s];
+        v[0] = fminf(fmaxf(nan_to_zero(nr.x) * 0.5f + 0.5f, 0.0f), 1.0f);
+        v[1] = fminf(fmaxf(nan_to_zero(nr.y) * 0.5f + 0.5f, 0.0f), 1.0f);
+        v[2] = fminf(fmaxf(nan_to_zero(nr.z) * 0.5f + 0.5f, 0.0f), 1.0f);
       }
     }
     uint16_t o[16];
@@ -425,20 +522,36 @@ __global__ void k_denoise_in(const float4 *__restrict__ color, const float4 *__r
   }
 }
 
+// Image pixels of the tile [tx0, tx1) x [ty0, ty1) from the network output of
+// the region at (rx, ry) (row stride W). Beauty: pu^-1(max(v, 0) / norm) /
+// scale; albedo: the sRGB decode of the clamped value; normal: the clamped
+// value * 2 - 1. Alpha is the source image's (color, albedo or normal).
 template <bool kBf16>
-__global__ void k_denoise_out(const uint16_t *__restrict__ net, const float4 *__restrict__ color, int w, int h, int W,
-                              float scale, float norm, float4 *__restrict__ out) {
-  const size_t n = (size_t)w * h;
+__global__ void k_denoise_out(const uint16_t *__restrict__ net, const float4 *__restrict__ source, int w, int W,
+                              int tx0, int ty0, int tx1, int ty1, int rx, int ry, int kind,
+                              const float *__restrict__ scalePtr, float scaleValue, float norm,
+                              float4 *__restrict__ out) {
+  const int tw = tx1 - tx0;
+  const float scale = scalePtr ? *scalePtr : scaleValue;
+  const size_t n = (size_t)tw * (ty1 - ty0);
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % w), y = (int)(i / w);
-    const uint16_t *p = net + ((size_t)y * W + x) * 16;
+    const int x = tx0 + (int)(i % tw), y = ty0 + (int)(i / tw);
+    const uint16_t *p = net + ((size_t)(y - ry) * W + (x - rx)) * 16;
     float r[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const double v = fmax((double)from_store<kBf16>(p[k]), 0.0) / norm;
-      r[k] = (float)(pu_inverse(v) / scale);
+      const float f = from_store<kBf16>(p[k]);
+      if (kind == kNetBeauty) {
+        const double v = fmax((double)f, 0.0) / norm;
+        r[k] = (float)(pu_inverse(v) / scale);
+      } else if (kind == kNetAlbedo) {
+        r[k] = srgb_decode_clamped(f);
+      } else {
+        r[k] = fminf(fmaxf(f, 0.0f), 1.0f) * 2.0f - 1.0f;
+      }
     }
-    out[i] = make_float4(r[0], r[1], r[2], color[i].w);
+    const size_t o = (size_t)y * w + x;
+    out[o] = make_float4(r[0], r[1], r[2], source[o].w);
   }
 }
 
@@ -502,35 +615,63 @@ static std::unordered_map<std::string, TzaTensor> parse_tza(const uint8_t *d, si
   return out;
 }
 
-struct UNetLayer {
-  const char *name;
-  int level, c1, c2, cout, up1, relu;
+// One 3x3 convolution of a U-Net: output level (0 = full resolution), c1
+// main input channels (upsampled from the half-resolution tensor when up1)
+// and c2 skip channels (the pooled output of the encoder level above, or the
+// network input at level 0), output channels, ReLU, and the pooled buffer its
+// 2x2 max pool feeds (-1: none; the pooled tensor is the next layer's input).
+struct NetLayer {
+  std::string name;
+  int level, c1, c2, cout, up1, relu, pool;
 };
-// Inputs of each convolution: c1 channels from the previous tensor (upsampled
-// for the decoder's first conv of a level) and c2 skip channels.
-static const UNetLayer kLayers[16] = {
-    {"enc_conv0", 0, 9, 0, 32, 0, 1},     {"enc_conv1", 0, 32, 0, 32, 0, 1},   {"enc_conv2", 1, 32, 0, 48, 0, 1},
-    {"enc_conv3", 2, 48, 0, 64, 0, 1},    {"enc_conv4", 3, 64, 0, 80, 0, 1},   {"enc_conv5a", 4, 80, 0, 96, 0, 1},
-    {"enc_conv5b", 4, 96, 0, 96, 0, 1},   {"dec_conv4a", 3, 96, 64, 112, 1, 1}, {"dec_conv4b", 3, 112, 0, 112, 0, 1},
-    {"dec_conv3a", 2, 112, 48, 96, 1, 1}, {"dec_conv3b", 2, 96, 0, 96, 0, 1},  {"dec_conv2a", 1, 96, 32, 64, 1, 1},
-    {"dec_conv2b", 1, 64, 0, 64, 0, 1},   {"dec_conv1a", 0, 64, 9, 64, 1, 1},  {"dec_conv1b", 0, 64, 0, 32, 0, 1},
-    {"dec_conv0", 0, 32, 0, 3, 0, 0}};
+// OIDN's two RT network shapes, told apart by their layer names: "UNet"
+// (standard and small models: enc_conv0 .. dec_conv0) and "UNetLarge" (the
+// large and clean-auxiliary models: enc_conv1a .. dec_conv1c). The last
+// convolution of encoder levels 0..3 is pooled; the first convolution of
+// each decoder level upsamples and concatenates that level's skip.
+struct NetPlan {
+  const char *name;
+  int level, pool;
+};
+static const NetPlan kUNet[16] = {
+    {"enc_conv0", 0, -1},  {"enc_conv1", 0, 0},   {"enc_conv2", 1, 1},   {"enc_conv3", 2, 2},
+    {"enc_conv4", 3, 3},   {"enc_conv5a", 4, -1}, {"enc_conv5b", 4, -1}, {"dec_conv4a", 3, -1},
+    {"dec_conv4b", 3, -1}, {"dec_conv3a", 2, -1}, {"dec_conv3b", 2, -1}, {"dec_conv2a", 1, -1},
+    {"dec_conv2b", 1, -1}, {"dec_conv1a", 0, -1}, {"dec_conv1b", 0, -1}, {"dec_conv0", 0, -1}};
+static const NetPlan kUNetLarge[19] = {
+    {"enc_conv1a", 0, -1}, {"enc_conv1b", 0, 0},  {"enc_conv2a", 1, -1}, {"enc_conv2b", 1, 1},
+    {"enc_conv3a", 2, -1}, {"enc_conv3b", 2, 2},  {"enc_conv4a", 3, -1}, {"enc_conv4b", 3, 3},
+    {"enc_conv5a", 4, -1}, {"enc_conv5b", 4, -1}, {"dec_conv4a", 3, -1}, {"dec_conv4b", 3, -1},
+    {"dec_conv3a", 2, -1}, {"dec_conv3b", 2, -1}, {"dec_conv2a", 1, -1}, {"dec_conv2b", 1, -1},
+    {"dec_conv1a", 0, -1}, {"dec_conv1b", 0, -1}, {"dec_conv1c", 0, -1}};
+constexpr int kMaxNetLayers = 19;
 
 static int pad16(int c) { return (c + 15) & ~15; }
 
 struct DenoiserObj {
   bool bf16 = false;
+  bool large = false;
+  int inC = 9;  // network input channels: 9 (color, albedo, normal) or 3 (one auxiliary image)
+  std::vector<NetLayer> net;
+  int poolC[4] = {0, 0, 0, 0};
   struct Layer {
     DevBuf<uint16_t> wt;
     DevBuf<float> bias;
     int kchunks = 0, kc1 = 0, nt = 0;
     int tileMt = 0, tileNt = 0;  // pixel / output tiles per wave; 0 = the default of conv_tiling
-  } L[16];
+  } L[kMaxNetLayers];
   // activations (sized for H x W)
   int H = 0, W = 0;
-  DevBuf<uint16_t> in, act[16], pool[4];
-  DevBuf<float4> io[4];  // color, albedo, normal, output staging
+  DevBuf<uint16_t> in, act[kMaxNetLayers], pool[4];
+  DevBuf<float4> io[6];  // color, albedo, normal, output, prefiltered albedo and normal
   size_t ioPixels = 0;
+  // Image tiling (oidn-webgpu's maxTileSize / tileOverlap): 0 = whole image.
+  uint32_t maxTile = 0, overlap = 0;
+  // HDR input scale (OIDN's inputScale): NaN = autoexposure per run.
+  float inputScale = NAN;
+  DevBuf<float> aeBins, aeScale;
+  // Prefilters of the albedo and normal guides (clean-auxiliary models).
+  std::shared_ptr<DenoiserObj> auxAlbedo, auxNormal;
   hipStream_t stream = nullptr;
   std::mutex busy;
   ~DenoiserObj() {
@@ -548,12 +689,53 @@ static std::shared_ptr<DenoiserObj> get_denoiser(uint64_t h) {
   return it->second;
 }
 
+// The layer list of a weight archive: the plan its names select, channel
+// counts from the weight shapes, checked against each other (every input
+// takes the channels the previous tensor holds, skips those of their level).
+static std::vector<NetLayer> network_from(const std::unordered_map<std::string, TzaTensor> &t, bool &large,
+                                          int &inC, int poolC[4]) {
+  large = t.count("enc_conv1a.weight") != 0;
+  const NetPlan *plan = large ? kUNetLarge : kUNet;
+  const int count = large ? 19 : 16;
+  std::vector<NetLayer> net;
+  int prevC = 0;
+  for (int i = 0; i < count; i++) {
+    const std::string n = plan[i].name;
+    auto w = t.find(n + ".weight"), b = t.find(n + ".bias");
+    MTX_REQUIRE(w != t.end() && b != t.end(), MTX_INVALID_ARGUMENT, "missing tensor " + n);
+    const auto &sh = w->second.shape;
+    MTX_REQUIRE(sh.size() == 4 && sh[2] == 3 && sh[3] == 3 && (int)b->second.data.size() == (int)sh[0] &&
+                    sh[0] >= 1 && sh[0] <= 512 && sh[1] >= 1 && sh[1] <= 1024,
+                MTX_INVALID_ARGUMENT, "unexpected shape of " + n);
+    NetLayer L;
+    L.name = n;
+    L.level = plan[i].level;
+    L.pool = plan[i].pool;
+    L.cout = (int)sh[0];
+    L.relu = i + 1 < count;
+    L.up1 = n.rfind("dec_", 0) == 0 && n.back() == 'a';
+    if (i == 0) {
+      inC = (int)sh[1];
+      MTX_REQUIRE(inC == 9 || inC == 3, MTX_INVALID_ARGUMENT, "the network input must have 9 or 3 channels");
+      prevC = inC;
+    }
+    L.c2 = L.up1 ? (L.level == 0 ? inC : poolC[L.level - 1]) : 0;
+    L.c1 = (int)sh[1] - L.c2;
+    MTX_REQUIRE(L.c1 == prevC, MTX_INVALID_ARGUMENT, "input channels of " + n + " do not match the previous layer");
+    if (L.pool >= 0) poolC[L.pool] = L.cout;
+    prevC = L.cout;
+    net.push_back(L);
+  }
+  MTX_REQUIRE(net.back().cout == 3, MTX_INVALID_ARGUMENT, "the network output must have 3 channels");
+  return net;
+}
+
 static void pack_layer(DenoiserObj &D, int li, const TzaTensor &w, const TzaTensor &b) {
-  const UNetLayer &Ly = kLayers[li];
+  const NetLayer &Ly = D.net[li];
   const int cin = Ly.c1 + Ly.c2;
   MTX_REQUIRE(w.shape.size() == 4 && (int)w.shape[0] == Ly.cout && (int)w.shape[1] == cin && w.shape[2] == 3 &&
                   w.shape[3] == 3 && (int)b.data.size() == Ly.cout,
-              MTX_INVALID_ARGUMENT, std::string("unexpected shape of ") + Ly.name);
+              MTX_INVALID_ARGUMENT, "unexpected shape of " + Ly.name);
   auto &L = D.L[li];
   L.kc1 = pad16(Ly.c1) / 16;
   L.kchunks = L.kc1 + pad16(Ly.c2) / 16;
@@ -593,20 +775,34 @@ constexpr int kSt = kTileStaged;
 constexpr int kConvTiling[16][2] = {{kSt | 4, 2}, {kSt | 4, 4}, {kSt | 2, 3}, {kSt | 4, 2}, {kSt | 1, 7}, {kSt | 1, 2},
                                     {kSt | 2, 2}, {kSt | 4, 2}, {kSt | 2, 4}, {kSt | 2, 3}, {kSt | 2, 2}, {kSt | 2, 4},
                                     {kSt | 2, 7}, {kSt | 4, 4}, {kSt | 4, 3}, {kSt | 4, 4}};
-static void conv_tiling(const DenoiserObj::Layer &L, int layer, int &mt, int &ntw) {
-  mt = L.tileMt ? L.tileMt : kConvTiling[layer][0];
-  ntw = std::min(L.tileNt ? L.tileNt : kConvTiling[layer][1], L.nt);
+// The table holds the bundled standard network's measured shapes; other
+// networks start from staged 2-tile rows over 4 output tiles (the autotuner
+// re-measures any network).
+static bool standard_shape(const DenoiserObj &D) {
+  static const int kStdOut[16] = {32, 32, 48, 64, 80, 96, 96, 112, 112, 96, 96, 64, 64, 64, 32, 3};
+  if (D.large || D.net.size() != 16 || D.inC != 9) return false;
+  for (int i = 0; i < 16; i++)
+    if (D.net[i].cout != kStdOut[i]) return false;
+  return true;
+}
+static void conv_tiling(const DenoiserObj &D, int layer, int &mt, int &ntw) {
+  const DenoiserObj::Layer &L = D.L[layer];
+  const bool tab = standard_shape(D);
+  mt = L.tileMt ? L.tileMt : (tab ? kConvTiling[layer][0] : (kTileStaged | 2));
+  ntw = std::min(L.tileNt ? L.tileNt : (tab ? kConvTiling[layer][1] : 4), L.nt);
 }
 
+// Buffers for networks of up to H x W pixels (kept when already larger).
 static void ensure_size(DenoiserObj &D, int H, int W) {
-  if (D.H == H && D.W == W) return;
+  if (D.H >= H && D.W >= W) return;
+  H = std::max(H, D.H);
+  W = std::max(W, D.W);
   D.in.alloc((size_t)H * W * 16);
-  for (int i = 0; i < 16; i++) {
-    const int lv = kLayers[i].level;
-    D.act[i].alloc((size_t)(H >> lv) * (W >> lv) * pad16(kLayers[i].cout));
+  for (size_t i = 0; i < D.net.size(); i++) {
+    const int lv = D.net[i].level;
+    D.act[i].alloc((size_t)(H >> lv) * (W >> lv) * pad16(D.net[i].cout));
   }
-  const int poolC[4] = {32, 48, 64, 80};
-  for (int i = 0; i < 4; i++) D.pool[i].alloc((size_t)(H >> (i + 1)) * (W >> (i + 1)) * poolC[i]);
+  for (int i = 0; i < 4; i++) D.pool[i].alloc((size_t)(H >> (i + 1)) * (W >> (i + 1)) * pad16(D.poolC[i]));
   D.H = H;
   D.W = W;
 }
@@ -648,15 +844,15 @@ static void launch_conv(const ConvArgs &a, int nt, int mt, hipStream_t s) {
   }
 }
 
-// Network on D.in (H x W x 16) -> D.act[15]; layer i's output in D.act[i].
+// Network on D.in (H x W x 16, within the buffers' size) -> D.act[last];
+// layer i's output in D.act[i].
 template <bool kBf16>
-static void run_unet(DenoiserObj &D, hipStream_t s) {
-  const int H = D.H, W = D.W;
+static void run_unet(DenoiserObj &D, int H, int W, hipStream_t s) {
   // inputs per layer: (src1, up1) and src2
   const uint16_t *prev = D.in.ptr;
   int prevC = 16;
-  for (int i = 0; i < 16; i++) {
-    const UNetLayer &Ly = kLayers[i];
+  for (int i = 0; i < (int)D.net.size(); i++) {
+    const NetLayer &Ly = D.net[i];
     const int lv = Ly.level;
     ConvArgs a{};
     a.H = H >> lv;
@@ -681,12 +877,13 @@ static void run_unet(DenoiserObj &D, hipStream_t s) {
     a.relu = Ly.relu;
     a.ntot = D.L[i].nt;
     int mt, ntw;
-    conv_tiling(D.L[i], i, mt, ntw);
+    conv_tiling(D, i, mt, ntw);
     a.ngroups = (a.ntot + ntw - 1) / ntw;
-    // enc_conv1..4 are pooled, the pooled tensor feeds the next level; the
-    // staged kernel pools in its epilogue (levels 0..3 have even sizes).
-    const bool pooled = i >= 1 && i <= 4;
-    a.pool = pooled && (mt & kTileStaged) ? D.pool[i - 1].ptr : nullptr;
+    // The last convolution of encoder levels 0..3 is pooled, the pooled
+    // tensor feeds the next level; the staged kernel pools in its epilogue
+    // (levels 0..3 have even sizes).
+    const bool pooled = Ly.pool >= 0;
+    a.pool = pooled && (mt & kTileStaged) ? D.pool[Ly.pool].ptr : nullptr;
     launch_conv<kBf16>(a, ntw, mt, s);
     prev = D.act[i].ptr;
     prevC = a.ds;
@@ -695,20 +892,81 @@ static void run_unet(DenoiserObj &D, hipStream_t s) {
         const int ph = H >> lv, pw = W >> lv;
         const size_t n = (size_t)(ph >> 1) * (pw >> 1) * (a.ds >> 3);
         hipLaunchKernelGGL(k_pool2<kBf16>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0,
-                           s, D.act[i].ptr, D.pool[i - 1].ptr, ph, pw, a.ds);
+                           s, D.act[i].ptr, D.pool[Ly.pool].ptr, ph, pw, a.ds);
       }
-      prev = D.pool[i - 1].ptr;
+      prev = D.pool[Ly.pool].ptr;
     }
   }
   MTX_HIP(hipGetLastError());
 }
 
-static void download_layer(DenoiserObj &D, int i, float *out, hipStream_t s) {
-  const int lv = kLayers[i].level, cs = pad16(kLayers[i].cout), co = kLayers[i].cout;
-  const size_t pix = (size_t)(D.H >> lv) * (D.W >> lv);
+static void download_layer(DenoiserObj &D, int i, int H, int W, float *out, hipStream_t s) {
+  const int lv = D.net[i].level, cs = pad16(D.net[i].cout), co = D.net[i].cout;
+  const size_t pix = (size_t)(H >> lv) * (W >> lv);
   std::vector<uint16_t> h = D.act[i].to_host(pix * cs, s);
   for (size_t p = 0; p < pix; p++)
     for (int c = 0; c < co; c++) out[p * co + c] = D.bf16 ? bf16_to_f32(h[p * cs + c]) : f16_to_f32(h[p * cs + c]);
+}
+
+// One image through a network, tile by tile: tiles of max_tile pixels (0 =
+// the whole image), each run on its region grown by `overlap` pixels on
+// every side (clipped to the image, padded to a multiple of 16) and written
+// back inside the tile only (oidn-webgpu's maxTileSize / tileOverlap).
+static void denoise_image(DenoiserObj &D, uint32_t maxTile, uint32_t overlap, const float4 *color,
+                          const float4 *albedo, const float4 *normal, int w, int h, int kind, float4 *out,
+                          hipStream_t s) {
+  const int T = maxTile ? (int)maxTile : std::max(pad16(w), pad16(h)), O = (int)overlap;
+  auto region = [&](int t0, int t1, int size, int &r0, int &r1) {
+    r0 = std::max(0, t0 - O);
+    r1 = std::min(size, t1 + O);
+  };
+  int maxH = 0, maxW = 0;
+  for (int ty = 0; ty < h; ty += T)
+    for (int tx = 0; tx < w; tx += T) {
+      int x0, x1, y0, y1;
+      region(tx, std::min(tx + T, w), w, x0, x1);
+      region(ty, std::min(ty + T, h), h, y0, y1);
+      maxW = std::max(maxW, pad16(x1 - x0));
+      maxH = std::max(maxH, pad16(y1 - y0));
+    }
+  ensure_size(D, maxH, maxW);
+  // The HDR scale of the whole image, before any tile runs.
+  const float *scalePtr = nullptr;
+  const float scale = kind == kNetBeauty && !std::isnan(D.inputScale) ? D.inputScale : 1.0f;
+  if (kind == kNetBeauty && std::isnan(D.inputScale)) {
+    const int nbw = (w + 15) / 16, nbh = (h + 15) / 16;
+    if (D.aeBins.count < (size_t)nbw * nbh) D.aeBins.alloc((size_t)nbw * nbh);
+    if (!D.aeScale.count) D.aeScale.alloc(1);
+    hipLaunchKernelGGL(k_ae_bins, dim3((unsigned)(nbw * nbh)), dim3(256), 0, s, color, w, h, nbw, nbh, D.aeBins.ptr);
+    hipLaunchKernelGGL(k_ae_scale, dim3(1), dim3(1024), 0, s, D.aeBins.ptr, nbw * nbh, D.aeScale.ptr);
+    scalePtr = D.aeScale.ptr;
+  }
+  const float norm = (float)(1.0 / pu_forward(65504.0));
+  const float4 *source = kind == kNetBeauty ? color : (kind == kNetAlbedo ? albedo : normal);
+  const int last = (int)D.net.size() - 1;
+  for (int ty = 0; ty < h; ty += T)
+    for (int tx = 0; tx < w; tx += T) {
+      const int tx1 = std::min(tx + T, w), ty1 = std::min(ty + T, h);
+      int x0, x1, y0, y1;
+      region(tx, tx1, w, x0, x1);
+      region(ty, ty1, h, y0, y1);
+      const int RW = pad16(x1 - x0), RH = pad16(y1 - y0);
+      const unsigned gin = (unsigned)std::min<size_t>(((size_t)RH * RW + 255) / 256, 65536);
+      const unsigned gout = (unsigned)std::min<size_t>(((size_t)(tx1 - tx) * (ty1 - ty) + 255) / 256, 65536);
+      if (D.bf16) {
+        hipLaunchKernelGGL(k_denoise_in<true>, dim3(gin), dim3(256), 0, s, color, albedo, normal, w, h, x0, y0, RH,
+                           RW, kind, scalePtr, scale, norm, D.in.ptr);
+        run_unet<true>(D, RH, RW, s);
+        hipLaunchKernelGGL(k_denoise_out<true>, dim3(gout), dim3(256), 0, s, D.act[last].ptr, source, w, RW, tx, ty,
+                           tx1, ty1, x0, y0, kind, scalePtr, scale, norm, out);
+      } else {
+        hipLaunchKernelGGL(k_denoise_in<false>, dim3(gin), dim3(256), 0, s, color, albedo, normal, w, h, x0, y0, RH,
+                           RW, kind, scalePtr, scale, norm, D.in.ptr);
+        run_unet<false>(D, RH, RW, s);
+        hipLaunchKernelGGL(k_denoise_out<false>, dim3(gout), dim3(256), 0, s, D.act[last].ptr, source, w, RW, tx, ty,
+                           tx1, ty1, x0, y0, kind, scalePtr, scale, norm, out);
+      }
+    }
 }
 
 }  // namespace mtx
@@ -726,11 +984,9 @@ int32_t mtx_denoiser_create_from_weights(const void *weights, size_t size, uint3
     auto D = std::make_shared<DenoiserObj>();
     D->bf16 = precision == MTX_DENOISE_BF16;
     auto t = parse_tza((const uint8_t *)weights, size);
-    for (int i = 0; i < 16; i++) {
-      const std::string n = kLayers[i].name;
-      MTX_REQUIRE(t.count(n + ".weight") && t.count(n + ".bias"), MTX_INVALID_ARGUMENT, "missing tensor " + n);
-      pack_layer(*D, i, t[n + ".weight"], t[n + ".bias"]);
-    }
+    D->net = network_from(t, D->large, D->inC, D->poolC);
+    for (int i = 0; i < (int)D->net.size(); i++)
+      pack_layer(*D, i, t[D->net[i].name + ".weight"], t[D->net[i].name + ".bias"]);
     MTX_HIP(hipStreamCreate(&D->stream));
     MTX_HIP(hipStreamSynchronize(0));
     std::lock_guard<std::mutex> g(gDenoiseMutex);
@@ -756,20 +1012,23 @@ int32_t mtx_denoiser_forward(mtx_handle handle, uint32_t width, uint32_t height,
     MTX_REQUIRE(input && output, MTX_INVALID_ARGUMENT, "input/output is NULL");
     MTX_REQUIRE(width && height && width % 16 == 0 && height % 16 == 0 && (uint64_t)width * height <= (1u << 24),
                 MTX_INVALID_ARGUMENT, "width and height must be nonzero multiples of 16, at most 16M pixels");
-    ensure_size(*D, (int)height, (int)width);
+    const int H = (int)height, W = (int)width, C = D->inC;
+    ensure_size(*D, H, W);
     const size_t pix = (size_t)width * height;
     std::vector<uint16_t> in(pix * 16, 0);
     for (size_t p = 0; p < pix; p++)
-      for (int c = 0; c < 9; c++) in[p * 16 + c] = D->bf16 ? f32_to_bf16(input[p * 9 + c]) : f32_to_f16(input[p * 9 + c]);
+      for (int c = 0; c < C; c++)
+        in[p * 16 + c] = D->bf16 ? f32_to_bf16(input[p * C + c]) : f32_to_f16(input[p * C + c]);
     D->in.upload(in.data(), in.size(), D->stream);
-    if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+    if (D->bf16) run_unet<true>(*D, H, W, D->stream); else run_unet<false>(*D, H, W, D->stream);
     MTX_HIP(hipStreamSynchronize(D->stream));
     std::vector<float> o(pix * 3);
-    download_layer(*D, 15, o.data(), D->stream);
+    const int last = (int)D->net.size() - 1;
+    download_layer(*D, last, H, W, o.data(), D->stream);
     std::memcpy(output, o.data(), o.size() * sizeof(float));
     if (layer_outputs)
-      for (int i = 0; i < 16; i++)
-        if (layer_outputs[i]) download_layer(*D, i, layer_outputs[i], D->stream);
+      for (int i = 0; i <= last; i++)
+        if (layer_outputs[i]) download_layer(*D, i, H, W, layer_outputs[i], D->stream);
     return MTX_OK;
   } catch (const Error &e) {
     set_error(err, cap, e.what());
@@ -785,7 +1044,7 @@ int32_t mtx_denoiser_set_tiling(mtx_handle handle, uint32_t layer, uint32_t pixe
     auto D = get_denoiser(handle);
     std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
     if (!lk.owns_lock()) return MTX_BUSY;
-    MTX_REQUIRE(layer < 16, MTX_INVALID_ARGUMENT, "layer must be in 0..15");
+    MTX_REQUIRE(layer < D->net.size(), MTX_INVALID_ARGUMENT, "layer out of range");
     const bool autoShape = pixel_tiles == 0 && channel_tiles == 0;
     const uint32_t mt = pixel_tiles & ~(uint32_t)MTX_DENOISER_STAGED;
     MTX_REQUIRE(autoShape || ((mt == 1 || mt == 2 || mt == 4 || mt == 8) && channel_tiles >= 1 && channel_tiles <= 7),
@@ -803,11 +1062,11 @@ int32_t mtx_denoiser_get_tiling(mtx_handle handle, uint32_t layer, uint32_t widt
                                 uint32_t *pixel_tiles, uint32_t *channel_tiles) {
   try {
     auto D = get_denoiser(handle);
-    MTX_REQUIRE(layer < 16 && pixel_tiles && channel_tiles, MTX_INVALID_ARGUMENT, "bad arguments");
+    MTX_REQUIRE(layer < D->net.size() && pixel_tiles && channel_tiles, MTX_INVALID_ARGUMENT, "bad arguments");
     (void)width;
     (void)height;
     int mt, ntw;
-    conv_tiling(D->L[layer], (int)layer, mt, ntw);
+    conv_tiling(*D, (int)layer, mt, ntw);
     *pixel_tiles = (uint32_t)mt;
     *channel_tiles = (uint32_t)ntw;
     return MTX_OK;
@@ -825,15 +1084,16 @@ int32_t mtx_denoiser_benchmark(mtx_handle handle, uint32_t width, uint32_t heigh
     MTX_REQUIRE(ms_per_forward && iterations, MTX_INVALID_ARGUMENT, "bad arguments");
     MTX_REQUIRE(width && height && width % 16 == 0 && height % 16 == 0, MTX_INVALID_ARGUMENT,
                 "width and height must be multiples of 16");
-    ensure_size(*D, (int)height, (int)width);
+    const int H = (int)height, W = (int)width;
+    ensure_size(*D, H, W);
     MTX_HIP(hipMemsetAsync(D->in.ptr, 0, D->in.bytes(), D->stream));
-    if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+    if (D->bf16) run_unet<true>(*D, H, W, D->stream); else run_unet<false>(*D, H, W, D->stream);
     hipEvent_t e0, e1;
     MTX_HIP(hipEventCreate(&e0));
     MTX_HIP(hipEventCreate(&e1));
     MTX_HIP(hipEventRecord(e0, D->stream));
     for (uint32_t k = 0; k < iterations; k++) {
-      if (D->bf16) run_unet<true>(*D, D->stream); else run_unet<false>(*D, D->stream);
+      if (D->bf16) run_unet<true>(*D, H, W, D->stream); else run_unet<false>(*D, H, W, D->stream);
     }
     MTX_HIP(hipEventRecord(e1, D->stream));
     MTX_HIP(hipEventSynchronize(e1));
@@ -858,8 +1118,13 @@ int32_t mtx_denoiser_run(mtx_handle handle, uint32_t width, uint32_t height, con
     MTX_REQUIRE(width && height && (uint64_t)width * height <= 16777216ull, MTX_INVALID_ARGUMENT,
                 "dimensions must be nonzero, at most 16777216 pixels");
     MTX_REQUIRE(count == (size_t)width * height * 4, MTX_INVALID_ARGUMENT, "count must be width*height*4");
-    const int H = pad16((int)height), W = pad16((int)width);
-    ensure_size(*D, H, W);
+    MTX_REQUIRE(D->inC == 9, MTX_INVALID_ARGUMENT, "this network filters one auxiliary image, not color + guides");
+    std::unique_lock<std::mutex> la, ln;
+    if (D->auxAlbedo) {
+      la = std::unique_lock<std::mutex>(D->auxAlbedo->busy, std::try_to_lock);
+      ln = std::unique_lock<std::mutex>(D->auxNormal->busy, std::try_to_lock);
+      if (!la.owns_lock() || !ln.owns_lock()) return MTX_BUSY;
+    }
     const size_t n = (size_t)width * height;
     if (D->ioPixels < n) {
       for (auto &b : D->io) b.alloc(n);
@@ -869,22 +1134,19 @@ int32_t mtx_denoiser_run(mtx_handle handle, uint32_t width, uint32_t height, con
     D->io[0].upload(reinterpret_cast<const float4 *>(color), n, s);
     D->io[1].upload(reinterpret_cast<const float4 *>(albedo), n, s);
     D->io[2].upload(reinterpret_cast<const float4 *>(normal), n, s);
-    const float scale = 1.0f;
-    const float norm = (float)(1.0 / pu_forward(65504.0));
-    const unsigned grid = (unsigned)std::min<size_t>(((size_t)H * W + 255) / 256, 65536);
-    if (D->bf16) {
-      hipLaunchKernelGGL(k_denoise_in<true>, dim3(grid), dim3(256), 0, s, D->io[0].ptr, D->io[1].ptr, D->io[2].ptr,
-                         (int)width, (int)height, H, W, scale, norm, D->in.ptr);
-      run_unet<true>(*D, s);
-      hipLaunchKernelGGL(k_denoise_out<true>, dim3(grid), dim3(256), 0, s, D->act[15].ptr, D->io[0].ptr, (int)width,
-                         (int)height, W, scale, norm, D->io[3].ptr);
-    } else {
-      hipLaunchKernelGGL(k_denoise_in<false>, dim3(grid), dim3(256), 0, s, D->io[0].ptr, D->io[1].ptr, D->io[2].ptr,
-                         (int)width, (int)height, H, W, scale, norm, D->in.ptr);
-      run_unet<false>(*D, s);
-      hipLaunchKernelGGL(k_denoise_out<false>, dim3(grid), dim3(256), 0, s, D->act[15].ptr, D->io[0].ptr, (int)width,
-                         (int)height, W, scale, norm, D->io[3].ptr);
+    const float4 *guideA = D->io[1].ptr, *guideN = D->io[2].ptr;
+    if (D->auxAlbedo) {
+      // Clean auxiliary guides (WGPURenderingPipeline.ts:1352-1440,
+      // oidn_webgpu.rs:84-150): each guide through its prefilter network.
+      denoise_image(*D->auxAlbedo, D->maxTile, D->overlap, nullptr, D->io[1].ptr, nullptr, (int)width, (int)height,
+                    kNetAlbedo, D->io[4].ptr, s);
+      denoise_image(*D->auxNormal, D->maxTile, D->overlap, nullptr, nullptr, D->io[2].ptr, (int)width, (int)height,
+                    kNetNormal, D->io[5].ptr, s);
+      guideA = D->io[4].ptr;
+      guideN = D->io[5].ptr;
     }
+    denoise_image(*D, D->maxTile, D->overlap, D->io[0].ptr, guideA, guideN, (int)width, (int)height, kNetBeauty,
+                  D->io[3].ptr, s);
     MTX_HIP(hipGetLastError());
     std::vector<float4> res = D->io[3].to_host(n, s);  // no partial result on failure: copy after success
     std::memcpy(output, res.data(), n * sizeof(float4));
@@ -895,6 +1157,78 @@ int32_t mtx_denoiser_run(mtx_handle handle, uint32_t width, uint32_t height, con
   } catch (const std::exception &e) {
     set_error(err, cap, e.what());
     return MTX_ERROR;
+  }
+}
+
+int32_t mtx_denoiser_set_auxiliary(mtx_handle handle, mtx_handle albedo_filter, mtx_handle normal_filter, char *err,
+                                   size_t cap) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE((albedo_filter == 0) == (normal_filter == 0), MTX_INVALID_ARGUMENT,
+                "give both prefilters or neither");
+    if (!albedo_filter) {
+      D->auxAlbedo.reset();
+      D->auxNormal.reset();
+      return MTX_OK;
+    }
+    MTX_REQUIRE(albedo_filter != handle && normal_filter != handle && albedo_filter != normal_filter,
+                MTX_INVALID_ARGUMENT, "the prefilters must be two other denoisers");
+    auto A = get_denoiser(albedo_filter), N = get_denoiser(normal_filter);
+    MTX_REQUIRE(D->inC == 9 && A->inC == 3 && N->inC == 3, MTX_INVALID_ARGUMENT,
+                "prefilters take one 3-channel image; the beauty network takes color, albedo and normal");
+    MTX_REQUIRE(A->bf16 == D->bf16 && N->bf16 == D->bf16, MTX_INVALID_ARGUMENT,
+                "prefilters must use the beauty network's precision");
+    D->auxAlbedo = A;
+    D->auxNormal = N;
+    return MTX_OK;
+  } catch (const Error &e) {
+    set_error(err, cap, e.what());
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_set_tiles(mtx_handle handle, uint32_t max_tile_size, uint32_t overlap) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(max_tile_size % 16 == 0 && overlap % 16 == 0 && max_tile_size <= 16384 && overlap <= 4096,
+                MTX_INVALID_ARGUMENT, "tile size and overlap must be multiples of 16 (tile 0 = whole image)");
+    D->maxTile = max_tile_size;
+    D->overlap = overlap;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_set_input_scale(mtx_handle handle, float scale) {
+  try {
+    auto D = get_denoiser(handle);
+    std::unique_lock<std::mutex> lk(D->busy, std::try_to_lock);
+    if (!lk.owns_lock()) return MTX_BUSY;
+    MTX_REQUIRE(std::isnan(scale) || (std::isfinite(scale) && scale > 0.0f), MTX_INVALID_ARGUMENT,
+                "the input scale must be positive and finite, or NaN for autoexposure");
+    D->inputScale = scale;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int32_t mtx_denoiser_info(mtx_handle handle, uint32_t info[4]) {
+  try {
+    auto D = get_denoiser(handle);
+    MTX_REQUIRE(info, MTX_INVALID_ARGUMENT, "info is NULL");
+    info[0] = (uint32_t)D->net.size();
+    info[1] = (uint32_t)D->inC;
+    info[2] = D->large ? 1u : 0u;
+    info[3] = D->bf16 ? MTX_DENOISE_BF16 : MTX_DENOISE_FP16;
+    return MTX_OK;
+  } catch (const Error &e) {
+    return e.code;
   }
 }
 
