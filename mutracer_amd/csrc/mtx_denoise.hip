@@ -194,17 +194,23 @@ __global__ void __launch_bounds__(256) k_conv3x3(const ConvArgs a) {
 template <bool kBf16, int NT, int kMT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_conv3x3_staged(const ConvArgs a) {
   constexpr int PW = 16 * kMT + 2;
+  // Stride of the staged input's planes (one per halo row and 8-channel
+  // piece), in 16-byte pieces. A stride of 16 kMT + 4 (the 16 lanes of a
+  // store phase on distinct 4-bank groups under a 64-bank model) measured
+  // more bank-conflict cycles and no faster pass on gfx950.
+  constexpr int PS = PW;
   constexpr int kAPieces = 6 * 4 * PW;
+  constexpr int kASlots = 6 * 4 * PS;
   constexpr int kBPieces = 9 * NT * 64;
   constexpr int kAPer = (kAPieces + 255) / 256, kBPer = (kBPieces + 255) / 256;  // pieces per thread
   // Output tile of a wave in the epilogue: [16 kMT pixels][16 NT + 8 channels]
   // (8 channels of padding put the four lane groups' writes on distinct banks).
   constexpr int kOutRow = 16 * NT + 8;
   constexpr int kOutPieces = (4 * 16 * kMT * kOutRow * 2 + 15) / 16;
-  constexpr int kSmem = kAPieces + kBPieces > kOutPieces ? kAPieces + kBPieces : kOutPieces;
+  constexpr int kSmem = kASlots + kBPieces > kOutPieces ? kASlots + kBPieces : kOutPieces;
   __shared__ uint4 smem[kSmem];
   uint4 *const sA = smem;
-  uint4 *const sB = smem + kAPieces;
+  uint4 *const sB = smem + kASlots;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int segs = (a.W + 16 * kMT - 1) / (16 * kMT);
   const int g = blockIdx.x % a.ngroups, blk = blockIdx.x / a.ngroups;  // the groups of a segment are neighbours
@@ -270,7 +276,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll
     for (int k = 0; k < kAPer; k++) {
       const int i = tid + 256 * k, r = i >> 2, hy = r / PW, px = r - hy * PW;
-      if (k < kAPer - 1 || i < kAPieces) sA[(hy * 4 + qa) * PW + px] = ra[k];
+      if (k < kAPer - 1 || i < kAPieces) sA[(hy * 4 + qa) * PS + px] = ra[k];
     }
 #pragma unroll
     for (int k = 0; k < kBPer; k++) {
@@ -284,7 +290,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     // unrolled costs occupancy).
 #pragma unroll 3
     for (int t = 0; t < 9; t++) {
-      const uint4 *ap = sA + ((w + t / 3) * 4 + quad) * PW + row + t % 3;
+      const uint4 *ap = sA + ((w + t / 3) * 4 + quad) * PS + row + t % 3;
       uint4 bv[NT];
 #pragma unroll
       for (int j = 0; j < NT; j++) bv[j] = sB[(t * NT + j) * 64 + lane];
@@ -342,13 +348,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       const int c8 = idx % (2 * NT), r = idx / (2 * NT), pp = r % (8 * kMT), pr = r / (8 * kMT);
       const int py = (y0 >> 1) + pr, px = (x0 >> 1) + pp;
       if ((c8 >> 1) >= tiles || py >= Ho || px >= Wo) continue;
+      // Four 16-byte reads (8 channels of the four source pixels; rows of
+      // kOutRow elements keep them 16-byte aligned).
       const uint16_t *s00 = tile + 2 * pr * kWave + 2 * pp * kOutRow + 8 * c8;
-      const uint16_t *s01 = s00 + kOutRow, *s10 = s00 + kWave, *s11 = s10 + kOutRow;
+      uint4 q4[4];
+      q4[0] = *reinterpret_cast<const uint4 *>(s00);
+      q4[1] = *reinterpret_cast<const uint4 *>(s00 + kOutRow);
+      q4[2] = *reinterpret_cast<const uint4 *>(s00 + kWave);
+      q4[3] = *reinterpret_cast<const uint4 *>(s00 + kWave + kOutRow);
+      const uint16_t *h = reinterpret_cast<const uint16_t *>(q4);
       uint16_t out[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const float v = fmaxf(fmaxf(from_store<kBf16>(s00[k]), from_store<kBf16>(s01[k])),
-                              fmaxf(from_store<kBf16>(s10[k]), from_store<kBf16>(s11[k])));
+        const float v = fmaxf(fmaxf(from_store<kBf16>(h[k]), from_store<kBf16>(h[8 + k])),
+                              fmaxf(from_store<kBf16>(h[16 + k]), from_store<kBf16>(h[24 + k])));
         out[k] = to_store<kBf16>(v);
       }
       *reinterpret_cast<uint4 *>(a.pool + ((size_t)py * Wo + px) * a.ds + j0 * 16 + 8 * c8) =

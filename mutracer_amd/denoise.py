@@ -196,20 +196,22 @@ class Denoiser:
 
 
 def _main(argv=None):
-    """Times network passes at a given size (HIP events), after 100 untimed
+    """Times network passes at a given size (HIP events), after untimed
     passes: python3 -m mutracer_amd.denoise [--width W] [--height H]
-    [--precision fp16|bf16] [--iterations N]."""
+    [--precision fp16|bf16] [--iterations N] [--warmup N] [--model M]."""
     import argparse
     ap = argparse.ArgumentParser(prog="python3 -m mutracer_amd.denoise")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1088)
     ap.add_argument("--precision", default="fp16", choices=("fp16", "bf16"))
     ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100, help="untimed passes first")
     ap.add_argument("--model", default="standard", choices=sorted(MODEL_FILENAMES))
     ap.add_argument("--weights-dir", default=None)
     args = ap.parse_args(argv)
     d = Denoiser.load(args.model, args.weights_dir, precision=args.precision)
-    d.benchmark(args.width, args.height, 100)
+    if args.warmup:
+        d.benchmark(args.width, args.height, args.warmup)
     ms = d.benchmark(args.width, args.height, args.iterations)
     print(f"denoiser {args.model} {args.precision} {args.width}x{args.height}: {ms:.3f} ms per pass", flush=True)
 

@@ -317,3 +317,32 @@ def test_denoiser_validation(net):
         three.run(color, normal, albedo)  # a prefilter network is no beauty network
     with pytest.raises(ValueError):
         three.forward(np.zeros((16, 16, 9), np.float32))
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_rendered_cornell_run_vs_oracle(net, precision):
+    """mtx_denoiser_run on a rendered Cornell box (80 x 60 at 4 spp: the
+    color, albedo and normal of read_pixels, normals decoded from their
+    display encoding to [-1, 1]) against the oracle's run of the same
+    images; the denoised image is also smoother than the noisy one."""
+    denoise, w = net
+    from mutracer_amd import render, scene
+    t = render.RenderTarget(render.RenderScene(scene.cornell_box()), 80, 60, bounces=8, max_samples=4)
+    t.render(4)
+    t.synchronize()
+    color, albedo, enc = (t.read_pixels(k) for k in ("color", "albedo", "normal"))
+    n = np.where(enc[..., 3:4] > 0, enc[..., :3] * 2.0 - 1.0, 0.0)
+    normal = np.concatenate([n, np.ones_like(enc[..., 3:4])], axis=2).astype(np.float32)
+    d = denoise.Denoiser(precision)
+    out = d.run(color, normal, albedo)
+    q = unet.to_fp16 if precision == "fp16" else unet.to_bf16
+    ref = unet.run(w, color, albedo, normal, quantize=q)
+    err = _rel(out[..., :3], ref[..., :3])
+    print(f"{precision} rendered Cornell run: {err:.2e}")
+    assert err <= RUN_TOL[precision]
+    # Neighbour differences inside the back wall drop (the noise is removed).
+    def rough(img):
+        c = img[10:40, 25:55, :3]
+        return float(np.abs(np.diff(c, axis=1)).mean())
+    print(f"{precision} neighbour difference: noisy {rough(color):.4f}, denoised {rough(out):.4f}")
+    assert rough(out) < 0.5 * rough(color)
