@@ -44,14 +44,21 @@ class Blas:
             self._keep = (pos, idx)
             code = L.mtx_blas_build(_dev_ptr(pos), pos.numel() // 3, _dev_ptr(idx), idx.numel() // 3, bins,
                                     abi.MTX_INPUT_DEVICE, C.byref(self.handle), err, 512)
-            self.indices_host = idx.cpu().numpy().astype(np.uint32).reshape(-1)
+            self._indices_host = None  # copied from the device tensor when refit or export needs it
         else:
             pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
             idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
             code = L.mtx_blas_build(_P(pos), pos.shape[0], _P(idx), idx.size // 3, bins, 0,
                                     C.byref(self.handle), err, 512)
-            self.indices_host = idx
+            self._indices_host = idx
         abi.check(code, err)
+
+    @property
+    def indices_host(self):
+        """The build's index buffer on the host (uint32, flat)."""
+        if self._indices_host is None:
+            self._indices_host = self._keep[1].cpu().numpy().astype(np.uint32).reshape(-1)
+        return self._indices_host
 
     def refit(self, positions):
         """refitBLAS: new vertex positions (host array, same vertex count) with

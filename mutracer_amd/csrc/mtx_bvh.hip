@@ -57,11 +57,14 @@ __device__ __forceinline__ int32_t bin_of(float center, double nodeMin, double s
   return b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
 }
 
-// Math.min / Math.max over non-NaN values (-0 below +0).
-
 struct NodeFrame {
   double nMin[3], extent[3], scale[3], area;
 };
+
+// Component i of a three-element local array through constant indices: a
+// runtime index into a local array places the array (and the struct that
+// holds it) in scratch memory.
+__device__ __forceinline__ double pick3(const double *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
 
 __device__ __forceinline__ NodeFrame node_frame(const float *lo, const float *hi, int32_t bins,
                                                 const double *minDim) {
@@ -75,152 +78,14 @@ __device__ __forceinline__ NodeFrame node_frame(const float *lo, const float *hi
   return f;
 }
 
-// SAH sweep of binnedSahBvh.ts:153-186 over bins stored as keys:
-// kmin/kmax[(axis*bins + i)*3 + k], cnt[axis*bins + i]. Returns false for a leaf.
+// A node's split: axis, last bin of the left side, the two children's boxes
+// (the SAH sweep of binnedSahBvh.ts:153-224, sah_split_wave, over bins
+// stored as keys: kmin/kmax[(axis*bins + i)*3 + k], cnt[axis*bins + i]).
 struct Split {
   int32_t axis, pos;
   float lo[2][3], hi[2][3];
 };
 
-__device__ bool sah_split(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt,
-                          int32_t bins, const NodeFrame &f, const double *minDim, uint32_t nodePrims,
-                          Split &out) {
-  double leftCost[kMaxBins], rightCost[kMaxBins];
-  double splitCost = kFar;
-  int32_t bestAxis = 0, bestPos = 0;
-  for (int axis = 0; axis < 3; axis++) {
-    if (!(f.extent[axis] > minDim[axis])) continue;
-    double lMin[3] = {kFar, kFar, kFar}, lMax[3] = {-kFar, -kFar, -kFar};
-    double rMin[3] = {kFar, kFar, kFar}, rMax[3] = {-kFar, -kFar, -kFar};
-    uint32_t leftN = 0, rightN = 0;
-    for (int i = 0; i < bins - 1; i++) {
-      int l = axis * bins + i, r = axis * bins + bins - 1 - i;
-      if (cnt[l]) {
-        for (int k = 0; k < 3; k++) {
-          double vmn = (double)unkey(kmin[l * 3 + k]), vmx = (double)unkey(kmax[l * 3 + k]);
-          lMin[k] = vmn < lMin[k] ? vmn : lMin[k];
-          lMax[k] = vmx > lMax[k] ? vmx : lMax[k];
-        }
-      }
-      if (cnt[r]) {
-        for (int k = 0; k < 3; k++) {
-          double vmn = (double)unkey(kmin[r * 3 + k]), vmx = (double)unkey(kmax[r * 3 + k]);
-          rMin[k] = vmn < rMin[k] ? vmn : rMin[k];
-          rMax[k] = vmx > rMax[k] ? vmx : rMax[k];
-        }
-      }
-      leftN += cnt[l];
-      rightN += cnt[r];
-      double lx = lMax[0] - lMin[0], ly = lMax[1] - lMin[1], lz = lMax[2] - lMin[2];
-      double rx = rMax[0] - rMin[0], ry = rMax[1] - rMin[1], rz = rMax[2] - rMin[2];
-      leftCost[i] = leftN == 0 ? kFar : (lx * ly + ly * lz + lz * lx) * (double)leftN;
-      rightCost[bins - 2 - i] = rightN == 0 ? kFar : (rx * ry + ry * rz + rz * rx) * (double)rightN;
-    }
-    for (int i = 0; i < bins - 1; i++) {
-      double cost = leftCost[i] + rightCost[i];
-      if (cost < splitCost) {
-        splitCost = cost;
-        bestAxis = axis;
-        bestPos = i;
-      }
-    }
-  }
-  splitCost = 1.0 + 1.0 * splitCost / f.area;
-  if (!(splitCost < (double)nodePrims * 1.0)) return false;
-  out.axis = bestAxis;
-  out.pos = bestPos;
-  // Child boxes: union of the bins on each side (binnedSahBvh.ts:211-224).
-  const float farF = (float)kFar;
-  for (int s = 0; s < 2; s++)
-    for (int k = 0; k < 3; k++) { out.lo[s][k] = farF; out.hi[s][k] = -farF; }
-  for (int i = 0; i < bins; i++) {
-    int b = bestAxis * bins + i;
-    int s = i <= bestPos ? 0 : 1;
-    for (int k = 0; k < 3; k++) {
-      double vmn = cnt[b] ? (double)unkey(kmin[b * 3 + k]) : kFar;
-      double vmx = cnt[b] ? (double)unkey(kmax[b * 3 + k]) : -kFar;
-      if (vmn < (double)out.lo[s][k]) out.lo[s][k] = (float)vmn;
-      if (vmx > (double)out.hi[s][k]) out.hi[s][k] = (float)vmx;
-    }
-  }
-  return true;
-}
-
-// The same split chosen by a whole workgroup: thread t evaluates split
-// position t % 64 of axis t / 64 (its left box folded over bins 0..i in
-// ascending order and its right box over bins - 1..i + 1 in descending
-// order, as the sequential sweep accumulates them, so every box, count and
-// f64 cost is the sweep's own), then the lowest cost wins with ties going to
-// the earliest (axis, position), which is the sweep's strict `<` order.
-// Needs blockDim.x >= 3 * 64; all threads must call it. Shared scratch:
-// best (2 words) and the result written by thread 0.
-__device__ bool sah_split_block(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt, int32_t bins,
-                                const NodeFrame &f, const double *minDim, uint32_t nodePrims,
-                                unsigned long long *best, Split &out, int32_t &doSplit) {
-  const int t = threadIdx.x, axis = t >> 6, i = t & 63;
-  if (t == 0) { best[0] = ~0ull; best[1] = ~0ull; }
-  __syncthreads();
-  double cost = kFar;
-  if (axis < 3 && i < bins - 1 && f.extent[axis] > minDim[axis]) {
-    double lMin[3] = {kFar, kFar, kFar}, lMax[3] = {-kFar, -kFar, -kFar};
-    double rMin[3] = {kFar, kFar, kFar}, rMax[3] = {-kFar, -kFar, -kFar};
-    uint32_t leftN = 0, rightN = 0;
-    for (int b = 0; b <= i; b++) {
-      const int l = axis * bins + b;
-      if (cnt[l])
-        for (int k = 0; k < 3; k++) {
-          const double vmn = (double)unkey(kmin[l * 3 + k]), vmx = (double)unkey(kmax[l * 3 + k]);
-          lMin[k] = vmn < lMin[k] ? vmn : lMin[k];
-          lMax[k] = vmx > lMax[k] ? vmx : lMax[k];
-        }
-      leftN += cnt[l];
-    }
-    for (int b = bins - 1; b > i; b--) {
-      const int r = axis * bins + b;
-      if (cnt[r])
-        for (int k = 0; k < 3; k++) {
-          const double vmn = (double)unkey(kmin[r * 3 + k]), vmx = (double)unkey(kmax[r * 3 + k]);
-          rMin[k] = vmn < rMin[k] ? vmn : rMin[k];
-          rMax[k] = vmx > rMax[k] ? vmx : rMax[k];
-        }
-      rightN += cnt[r];
-    }
-    const double lx = lMax[0] - lMin[0], ly = lMax[1] - lMin[1], lz = lMax[2] - lMin[2];
-    const double rx = rMax[0] - rMin[0], ry = rMax[1] - rMin[1], rz = rMax[2] - rMin[2];
-    const double lc = leftN == 0 ? kFar : (lx * ly + ly * lz + lz * lx) * (double)leftN;
-    const double rc = rightN == 0 ? kFar : (rx * ry + ry * rz + rz * rx) * (double)rightN;
-    cost = (lc + rc) + 0.0;  // -0 -> +0, so that the bit order below is the numeric order
-    if (cost < kFar) atomicMin(&best[0], (unsigned long long)__double_as_longlong(cost));
-  }
-  __syncthreads();
-  if (cost < kFar && (unsigned long long)__double_as_longlong(cost) == best[0])
-    atomicMin(&best[1], (unsigned long long)(axis * 64 + i));
-  __syncthreads();
-  if (t == 0) {
-    const bool any = best[1] != ~0ull;
-    const double splitCost = 1.0 + 1.0 * (any ? __longlong_as_double((long long)best[0]) : kFar) / f.area;
-    doSplit = splitCost < (double)nodePrims * 1.0 ? 1 : 0;
-    if (doSplit) {
-      const int bestAxis = (int)(best[1] >> 6), bestPos = (int)(best[1] & 63);
-      out.axis = bestAxis;
-      out.pos = bestPos;
-      const float farF = (float)kFar;
-      for (int sd = 0; sd < 2; sd++)
-        for (int k = 0; k < 3; k++) { out.lo[sd][k] = farF; out.hi[sd][k] = -farF; }
-      for (int b = 0; b < bins; b++) {
-        const int bb = bestAxis * bins + b, sd = b <= bestPos ? 0 : 1;
-        for (int k = 0; k < 3; k++) {
-          const double vmn = cnt[bb] ? (double)unkey(kmin[bb * 3 + k]) : kFar;
-          const double vmx = cnt[bb] ? (double)unkey(kmax[bb * 3 + k]) : -kFar;
-          if (vmn < (double)out.lo[sd][k]) out.lo[sd][k] = (float)vmn;
-          if (vmx > (double)out.hi[sd][k]) out.hi[sd][k] = (float)vmx;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  return doSplit != 0;
-}
 
 // ------------------------------------------------------------ input kernels
 __global__ void k_tri_aabbs(const float *__restrict__ pos, const uint32_t *__restrict__ idx,
@@ -348,11 +213,15 @@ struct BigState {
   float clo[2][3], chi[2][3];
 };
 
+// The level's bins (nb * 3 * bins of them; nb read on the device).
 __global__ void k_bins_clear(uint32_t *__restrict__ kmin, uint32_t *__restrict__ kmax,
-                             uint32_t *__restrict__ cnt, uint32_t nBins) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nBins) cnt[i] = 0;
-  if (i < nBins * 3) { kmin[i] = 0xffffffffu; kmax[i] = 0u; }
+                             uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nbp, int32_t bins) {
+  const uint32_t nBins = *nbp * 3u * (uint32_t)bins;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nBins * 3; i += gridDim.x * blockDim.x) {
+    if (i < nBins) cnt[i] = 0;
+    kmin[i] = 0xffffffffu;
+    kmax[i] = 0u;
+  }
 }
 
 __global__ void __launch_bounds__(kBuildThreads)
@@ -360,8 +229,10 @@ k_bin_chunks(const Chunk *__restrict__ chunks, const BigState *__restrict__ big,
              const BNode *__restrict__ nodes, const uint32_t *__restrict__ prim,
              const float4 *__restrict__ plo, const float4 *__restrict__ phi,
              const float4 *__restrict__ cen, int32_t bins, double md0, double md1, double md2,
-             uint32_t *__restrict__ gkmin, uint32_t *__restrict__ gkmax, uint32_t *__restrict__ gcnt) {
+             uint32_t *__restrict__ gkmin, uint32_t *__restrict__ gkmax, uint32_t *__restrict__ gcnt,
+             const uint32_t *__restrict__ ncp) {
   __shared__ uint32_t kmin[3 * kMaxBins * 3], kmax[3 * kMaxBins * 3], cnt[3 * kMaxBins];
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BNode nd = nodes[big[ch.big].node];
   const double minDim[3] = {md0, md1, md2};
@@ -395,45 +266,20 @@ k_bin_chunks(const Chunk *__restrict__ chunks, const BigState *__restrict__ big,
   }
 }
 
-__global__ void k_sweep(BigState *__restrict__ big, uint32_t nBig, const BNode *__restrict__ nodes,
-                        const uint32_t *__restrict__ gkmin, const uint32_t *__restrict__ gkmax,
-                        const uint32_t *__restrict__ gcnt, int32_t bins, double md0, double md1,
-                        double md2) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nBig) return;
-  BigState s = big[b];
-  const BNode nd = nodes[s.node];
-  const double minDim[3] = {md0, md1, md2};
-  NodeFrame f = node_frame(nd.lo, nd.hi, bins, minDim);
-  s.split = 0;
-  s.totalL = s.totalR = s.P = 0;
-  if (f.area != 0.0 && nd.primCount != 1) {
-    Split sp;
-    uint32_t base = b * 3 * bins;
-    if (sah_split(gkmin + base * 3, gkmax + base * 3, gcnt + base, bins, f, minDim, nd.primCount, sp)) {
-      s.split = 1;
-      s.axis = sp.axis;
-      s.pos = sp.pos;
-      for (int c = 0; c < 2; c++)
-        for (int k = 0; k < 3; k++) { s.clo[c][k] = sp.lo[c][k]; s.chi[c][k] = sp.hi[c][k]; }
-    }
-  }
-  big[b] = s;
-}
-
 __device__ __forceinline__ bool goes_right(const float4 *cen, uint32_t p, const BNode &nd, const BigState &s,
                                            int32_t bins, const double *minDim) {
   NodeFrame f = node_frame(nd.lo, nd.hi, bins, minDim);
   float4 c = cen[p];
   float v = s.axis == 0 ? c.x : (s.axis == 1 ? c.y : c.z);
-  return bin_of(v, f.nMin[s.axis], f.scale[s.axis], bins) > s.pos;
+  return bin_of(v, pick3(f.nMin, s.axis), pick3(f.scale, s.axis), bins) > s.pos;
 }
 
 __global__ void __launch_bounds__(kBuildThreads)
 k_classify(const Chunk *__restrict__ chunks, const BigState *__restrict__ big,
            const BNode *__restrict__ nodes, const uint32_t *__restrict__ prim,
            const float4 *__restrict__ cen, int32_t bins, double md0, double md1, double md2,
-           uint8_t *__restrict__ isR, uint32_t *__restrict__ chunkR) {
+           uint8_t *__restrict__ isR, uint32_t *__restrict__ chunkR, const uint32_t *__restrict__ ncp) {
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BigState s = big[ch.big];
   __shared__ uint32_t total;
@@ -454,11 +300,11 @@ k_classify(const Chunk *__restrict__ chunks, const BigState *__restrict__ big,
 }
 
 // One thread per big node: exclusive prefix of right counts over its chunks.
-__global__ void k_chunk_prefix(BigState *__restrict__ big, uint32_t nBig, const BNode *__restrict__ nodes,
-                               const uint32_t *__restrict__ firstChunk, const uint32_t *__restrict__ chunkR,
-                               uint32_t *__restrict__ chunkRBefore) {
+__global__ void k_chunk_prefix(BigState *__restrict__ big, const uint32_t *__restrict__ nbp,
+                               const BNode *__restrict__ nodes, const uint32_t *__restrict__ firstChunk,
+                               const uint32_t *__restrict__ chunkR, uint32_t *__restrict__ chunkRBefore) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nBig) return;
+  if (b >= *nbp) return;
   uint32_t c0 = firstChunk[b], c1 = firstChunk[b + 1];
   uint32_t acc = 0;
   for (uint32_t c = c0; c < c1; c++) { chunkRBefore[c] = acc; acc += chunkR[c]; }
@@ -476,9 +322,11 @@ constexpr int kItems = kChunk / kBuildThreads;  // 8 elements per thread
 __global__ void __launch_bounds__(kBuildThreads)
 k_xz(const Chunk *__restrict__ chunks, const BigState *__restrict__ big, const BNode *__restrict__ nodes,
      const uint8_t *__restrict__ isR, const uint32_t *__restrict__ chunkRBefore,
-     uint32_t *__restrict__ X, uint32_t *__restrict__ Z, uint32_t *__restrict__ rank) {
+     uint32_t *__restrict__ X, uint32_t *__restrict__ Z, uint32_t *__restrict__ rank,
+     const uint32_t *__restrict__ ncp) {
   typedef hipcub::BlockScan<uint32_t, kBuildThreads> Scan;
   __shared__ typename Scan::TempStorage tmp;
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BigState s = big[ch.big];
   if (!s.split) return;
@@ -508,7 +356,9 @@ k_xz(const Chunk *__restrict__ chunks, const BigState *__restrict__ big, const B
 
 __global__ void __launch_bounds__(kBuildThreads)
 k_count_p(const Chunk *__restrict__ chunks, BigState *__restrict__ big, const BNode *__restrict__ nodes,
-          const uint8_t *__restrict__ isR, const uint32_t *__restrict__ Z, const uint32_t *__restrict__ rank) {
+          const uint8_t *__restrict__ isR, const uint32_t *__restrict__ Z, const uint32_t *__restrict__ rank,
+          const uint32_t *__restrict__ ncp) {
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BigState s = big[ch.big];
   if (!s.split) return;
@@ -538,7 +388,9 @@ __device__ __forceinline__ uint32_t partition_dest(uint32_t e, bool right, uint3
 __global__ void __launch_bounds__(kBuildThreads)
 k_scatter(const Chunk *__restrict__ chunks, const BigState *__restrict__ big, const BNode *__restrict__ nodes,
           const uint8_t *__restrict__ isR, const uint32_t *__restrict__ X, const uint32_t *__restrict__ Z,
-          const uint32_t *__restrict__ rank, const uint32_t *__restrict__ prim, uint32_t *__restrict__ tmp) {
+          const uint32_t *__restrict__ rank, const uint32_t *__restrict__ prim, uint32_t *__restrict__ tmp,
+          const uint32_t *__restrict__ ncp) {
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BigState s = big[ch.big];
   if (!s.split) return;
@@ -553,7 +405,8 @@ k_scatter(const Chunk *__restrict__ chunks, const BigState *__restrict__ big, co
 
 __global__ void k_copyback(const Chunk *__restrict__ chunks, const BigState *__restrict__ big,
                            const BNode *__restrict__ nodes, const uint32_t *__restrict__ tmp,
-                           uint32_t *__restrict__ prim) {
+                           uint32_t *__restrict__ prim, const uint32_t *__restrict__ ncp) {
+  if (blockIdx.x >= *ncp) return;
   const Chunk ch = chunks[blockIdx.x];
   const BigState s = big[ch.big];
   if (!s.split) return;
@@ -561,26 +414,68 @@ __global__ void k_copyback(const Chunk *__restrict__ chunks, const BigState *__r
   for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += blockDim.x) prim[first + e] = tmp[first + e];
 }
 
-// The level's big nodes: their records (for the host's chunk list) and
-// fresh per-node states.
-__global__ void k_level_nodes(const BNode *__restrict__ nodes, const uint32_t *__restrict__ ids, uint32_t nBig,
-                              BNode *__restrict__ out, BigState *__restrict__ big) {
+// The level's big nodes: their records (for the chunk list) and fresh
+// per-node states.
+__global__ void k_level_nodes(const BNode *__restrict__ nodes, const uint32_t *__restrict__ ids,
+                              const uint32_t *__restrict__ nbp, BNode *__restrict__ out, BigState *__restrict__ big) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nBig) return;
+  if (b >= *nbp) return;
   out[b] = nodes[ids[b]];
   BigState st{};
   st.node = ids[b];
   big[b] = st;
 }
 
+// The level's chunk list on the device: the first chunk of each big node
+// (exclusive scan of ceil(primCount / kChunk) in node order) and the count.
+__global__ void __launch_bounds__(1024) k_chunk_first(const BNode *__restrict__ level,
+                                                      const uint32_t *__restrict__ nbp,
+                                                      uint32_t *__restrict__ first, uint32_t *__restrict__ ncp) {
+  typedef hipcub::BlockScan<uint32_t, 1024> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  __shared__ uint32_t carry;
+  const uint32_t nb = *nbp;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t b = base + threadIdx.x;
+    const uint32_t v = b < nb ? (level[b].primCount + kChunk - 1) / kChunk : 0u;
+    uint32_t ex, agg;
+    Scan(tmp).ExclusiveSum(v, ex, agg);
+    if (b < nb) first[b] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += agg;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    first[nb] = carry;
+    *ncp = carry;
+  }
+}
+// Chunk c: kChunk elements of the node whose range of chunks holds c (every
+// big node has at least one chunk, so the first-chunk list increases).
+__global__ void k_chunk_fill(const BNode *__restrict__ level, const uint32_t *__restrict__ nbp,
+                             const uint32_t *__restrict__ first, const uint32_t *__restrict__ ncp,
+                             Chunk *__restrict__ chunks) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= *ncp) return;
+  uint32_t lo = 0, hi = *nbp;  // first[lo] <= c < first[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= c) lo = mid; else hi = mid;
+  }
+  const uint32_t e = (c - first[lo]) * (uint32_t)kChunk;
+  chunks[c] = Chunk{lo, e, min(e + (uint32_t)kChunk, level[lo].primCount), 0u};
+}
+
 // Children of split big nodes; route them to the next big level or the
 // small-subtree list.
-__global__ void k_children(const BigState *__restrict__ big, uint32_t nBig, BNode *__restrict__ nodes,
-                           uint32_t *__restrict__ counter, uint32_t *__restrict__ nextBig,
+__global__ void k_children(const BigState *__restrict__ big, const uint32_t *__restrict__ nbp,
+                           BNode *__restrict__ nodes, uint32_t *__restrict__ counter, uint32_t *__restrict__ nextBig,
                            uint32_t *__restrict__ nextBigCount, uint32_t *__restrict__ small,
                            uint32_t *__restrict__ smallCount) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nBig) return;
+  if (b >= *nbp) return;
   const BigState s = big[b];
   if (!s.split) return;
   BNode nd = nodes[s.node];
@@ -640,49 +535,102 @@ struct SmallWaveArrays {
   BNode snode[kWaveNode + 2];
 };
 
-// The sweep of sah_split for one wave: lane b holds bin b of an axis; an
-// inclusive prefix scan keeps the leftmost of equal minima / maxima (the
-// left fold's strict comparisons), a suffix scan the rightmost (the right
-// fold runs from the last bin down); lane i then prices split position i
-// with the sweep's f64 expressions, and the wave takes the lowest cost, ties
-// to the earliest (axis, position). Child boxes are folded over each side's
-// bins in ascending order, as the sweep's epilogue does. Returns false for
-// a leaf; every lane returns the same split.
-__device__ bool sah_split_wave(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt, int32_t bins,
-                               const NodeFrame &f, const double *minDim, uint32_t nodePrims, Split &out) {
+// Cross-lane steps through DPP (gfx9 row shifts and row broadcasts, no LDS
+// traffic); a lane without a source keeps `old`.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, v), kCtrl, kRowMask, 0xf,
+                                                               false));
+}
+// Inclusive prefix scans over segments of 32 (or 64) lanes: row_shr 1, 2, 4,
+// 8 within rows of 16, then row 15's total into the next row (rows 1 and 3)
+// and, for 64, lane 31's into rows 2 and 3. The min / max keep the earlier
+// lane's value on ties.
+template <bool kMax>
+__device__ __forceinline__ float scan_minmax(float v, bool seg64) {
+  const float nt = kMax ? -INFINITY : INFINITY;
+  float e;
+  e = dpp_f<0x111, 0xf>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e);
+  e = dpp_f<0x112, 0xf>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e);
+  e = dpp_f<0x114, 0xf>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e);
+  e = dpp_f<0x118, 0xf>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e);
+  e = dpp_f<0x142, 0xa>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e);
+  if (seg64) { e = dpp_f<0x143, 0xc>(nt, v); v = kMax ? (v > e ? v : e) : (v < e ? v : e); }
+  return v;
+}
+__device__ __forceinline__ uint32_t scan_add(uint32_t v, bool seg64) {
+  v += dpp_u<0x111, 0xf>(0u, v);
+  v += dpp_u<0x112, 0xf>(0u, v);
+  v += dpp_u<0x114, 0xf>(0u, v);
+  v += dpp_u<0x118, 0xf>(0u, v);
+  v += dpp_u<0x142, 0xa>(0u, v);
+  if (seg64) v += dpp_u<0x143, 0xc>(0u, v);
+  return v;
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// The sweep of sah_split for one wave: lane b holds bin b of an axis (with
+// at most 32 bins, lanes 32..63 hold the next axis: two axes per pass, the
+// scans segmented at 32 lanes); an inclusive prefix scan keeps the leftmost
+// of equal minima / maxima (the left fold's strict comparisons); the right
+// fold runs from the last bin down and keeps the rightmost, computed as the
+// same prefix scan over the bins loaded in reverse order and moved to the
+// lane of its split position; lane i then prices split position i with the
+// sweep's f64 expressions, and the wave takes the lowest cost, ties to the
+// earliest (axis, position). Child boxes are each side's fold over its bins
+// in ascending order, as the sweep's epilogue does: a prefix scan over the
+// side's bins (leftmost on ties, as the fold's strict comparison keeps the
+// first) read at the side's last bin, the fold's start value kept when no
+// bin beats it. Returns false for a leaf; every lane returns the same split.
+__device__ __forceinline__ bool sah_split_wave(const uint32_t *kmin, const uint32_t *kmax, const uint32_t *cnt,
+                                               int32_t bins, const NodeFrame &f, const double *minDim,
+                                               uint32_t nodePrims, Split &out) {
   const int lane = threadIdx.x & 63;
+  const bool halves = bins <= 32;
+  const int b = halves ? (lane & 31) : lane, passes = halves ? 2 : 3;
   double bestCost = INFINITY;
   int bestIdx = 0x7fffffff;
-  for (int axis = 0; axis < 3; axis++) {
-    if (!(f.extent[axis] > minDim[axis])) continue;
+  for (int pass = 0; pass < passes; pass++) {
+    const int axis = halves ? 2 * pass + (lane >> 5) : pass;
+    const int ax = axis < 3 ? axis : 2;
+    const bool ok = axis < 3 && pick3(f.extent, ax) > pick3(minDim, ax);
+    // Skip a pass whose axes are all degenerate (f is the same on every lane).
+    const int axLo = halves ? 2 * pass : pass;
+    const bool okLo = pick3(f.extent, axLo) > pick3(minDim, axLo);
+    const bool okHi = halves && pass == 0 && f.extent[1] > minDim[1];
+    if (!okLo && !okHi) continue;
+    const bool inb = ok && b < bins;
+    const int fb = ax * bins + (inb ? b : 0), rb = ax * bins + (inb ? bins - 1 - b : 0);
+    const bool hasL = inb && cnt[fb] != 0, hasR = inb && cnt[rb] != 0;
     float lmn[3], lmx[3], rmn[3], rmx[3];
-    uint32_t lc = 0;
-    const bool has = lane < bins && cnt[axis * bins + lane] != 0;
     for (int k = 0; k < 3; k++) {
-      const int b = (axis * bins + lane) * 3 + k;
-      lmn[k] = has ? unkey(kmin[b]) : INFINITY;
-      lmx[k] = has ? unkey(kmax[b]) : -INFINITY;
+      lmn[k] = hasL ? unkey(kmin[fb * 3 + k]) : INFINITY;
+      lmx[k] = hasL ? unkey(kmax[fb * 3 + k]) : -INFINITY;
+      rmn[k] = hasR ? unkey(kmin[rb * 3 + k]) : INFINITY;
+      rmx[k] = hasR ? unkey(kmax[rb * 3 + k]) : -INFINITY;
     }
-    if (lane < bins) lc = cnt[axis * bins + lane];
-    uint32_t rc = lc;
-    for (int k = 0; k < 3; k++) { rmn[k] = lmn[k]; rmx[k] = lmx[k]; }
-    for (int off = 1; off < 64; off <<= 1) {
-      const bool up = lane >= off, down = lane + off < 64;
-      const uint32_t oc = __shfl_up(lc, off), od = __shfl_down(rc, off);
-      for (int k = 0; k < 3; k++) {
-        const float a = __shfl_up(lmn[k], off), b = __shfl_up(lmx[k], off);
-        const float c = __shfl_down(rmn[k], off), d = __shfl_down(rmx[k], off);
-        if (up) { lmn[k] = lmn[k] < a ? lmn[k] : a; lmx[k] = lmx[k] > b ? lmx[k] : b; }
-        if (down) { rmn[k] = rmn[k] < c ? rmn[k] : c; rmx[k] = rmx[k] > d ? rmx[k] : d; }
-      }
-      if (up) lc += oc;
-      if (down) rc += od;
+    const uint32_t lc = scan_add(inb ? cnt[fb] : 0u, !halves);
+    const uint32_t rcr = scan_add(inb ? cnt[rb] : 0u, !halves);
+    for (int k = 0; k < 3; k++) {
+      lmn[k] = scan_minmax<false>(lmn[k], !halves);
+      lmx[k] = scan_minmax<true>(lmx[k], !halves);
+      rmn[k] = scan_minmax<false>(rmn[k], !halves);
+      rmx[k] = scan_minmax<true>(rmx[k], !halves);
     }
-    // right side of position i = the suffix from bin i + 1
-    const uint32_t rightN = __shfl_down(rc, 1);
+    // right side of position i = the suffix from bin i + 1 = the reversed
+    // scan at lane bins - 2 - i of the segment
+    const int src = (halves ? (lane & 32) : 0) + (b < bins - 1 ? bins - 2 - b : 0);
+    const uint32_t rightN = __shfl(rcr, src);
     float sr[6];
-    for (int k = 0; k < 3; k++) { sr[k] = __shfl_down(rmn[k], 1); sr[3 + k] = __shfl_down(rmx[k], 1); }
-    if (lane < bins - 1) {
+    for (int k = 0; k < 3; k++) { sr[k] = __shfl(rmn[k], src); sr[3 + k] = __shfl(rmx[k], src); }
+    if (ok && b < bins - 1) {
       // the folds start from +-kFar: values beyond it leave the start value
       double L[6], R[6];
       for (int k = 0; k < 3; k++) {
@@ -696,7 +644,7 @@ __device__ bool sah_split_wave(const uint32_t *kmin, const uint32_t *kmax, const
       const double lcost = lc == 0 ? kFar : (lx * ly + ly * lz + lz * lx) * (double)lc;
       const double rcost = rightN == 0 ? kFar : (rx * ry + ry * rz + rz * rx) * (double)rightN;
       const double cost = (lcost + rcost) + 0.0;
-      if (cost < kFar && cost < bestCost) { bestCost = cost; bestIdx = axis * 64 + lane; }
+      if (cost < kFar && cost < bestCost) { bestCost = cost; bestIdx = axis * 64 + b; }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -710,18 +658,47 @@ __device__ bool sah_split_wave(const uint32_t *kmin, const uint32_t *kmax, const
   out.axis = bestAxis;
   out.pos = bestPos;
   const float farF = (float)kFar;
-  for (int sd = 0; sd < 2; sd++)
-    for (int k = 0; k < 3; k++) { out.lo[sd][k] = farF; out.hi[sd][k] = -farF; }
-  for (int b = 0; b < bins; b++) {
-    const int bb = bestAxis * bins + b, sd = b <= bestPos ? 0 : 1;
+  const int bb = bestAxis * bins + (lane < bins ? lane : 0);
+  const bool filled = lane < bins && cnt[bb] != 0;
+  for (int sd = 0; sd < 2; sd++) {
+    const bool mine = filled && (sd == 0 ? lane <= bestPos : lane > bestPos);
+    const int at = sd == 0 ? bestPos : bins - 1;
     for (int k = 0; k < 3; k++) {
-      const double vmn = cnt[bb] ? (double)unkey(kmin[bb * 3 + k]) : kFar;
-      const double vmx = cnt[bb] ? (double)unkey(kmax[bb * 3 + k]) : -kFar;
-      if (vmn < (double)out.lo[sd][k]) out.lo[sd][k] = (float)vmn;
-      if (vmx > (double)out.hi[sd][k]) out.hi[sd][k] = (float)vmx;
+      const float vmn = readlane_f(scan_minmax<false>(mine ? unkey(kmin[bb * 3 + k]) : INFINITY, true), at);
+      const float vmx = readlane_f(scan_minmax<true>(mine ? unkey(kmax[bb * 3 + k]) : -INFINITY, true), at);
+      out.lo[sd][k] = (double)vmn < (double)farF ? vmn : farF;
+      out.hi[sd][k] = (double)vmx > (double)-farF ? vmx : -farF;
     }
   }
   return true;
+}
+
+// One wave per big node of the level: its split from the level's bins (the
+// wave sweep, with the sequential fold's choice and tie rules).
+__global__ void __launch_bounds__(256) k_sweep(BigState *__restrict__ big, const uint32_t *__restrict__ nbp,
+                                               const BNode *__restrict__ nodes, const uint32_t *__restrict__ gkmin,
+                                               const uint32_t *__restrict__ gkmax, const uint32_t *__restrict__ gcnt,
+                                               int32_t bins, double md0, double md1, double md2) {
+  const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (b >= *nbp) return;  // the same for every lane of the wave
+  BigState s = big[b];
+  const BNode nd = nodes[s.node];
+  const double minDim[3] = {md0, md1, md2};
+  const NodeFrame f = node_frame(nd.lo, nd.hi, bins, minDim);
+  s.split = 0;
+  s.totalL = s.totalR = s.P = 0;
+  if (f.area != 0.0 && nd.primCount != 1) {
+    Split sp;
+    const uint32_t base = b * 3u * (uint32_t)bins;
+    if (sah_split_wave(gkmin + base * 3, gkmax + base * 3, gcnt + base, bins, f, minDim, nd.primCount, sp)) {
+      s.split = 1;
+      s.axis = sp.axis;
+      s.pos = sp.pos;
+      for (int c = 0; c < 2; c++)
+        for (int k = 0; k < 3; k++) { s.clo[c][k] = sp.lo[c][k]; s.chi[c][k] = sp.hi[c][k]; }
+    }
+  }
+  if ((threadIdx.x & 63) == 0) big[b] = s;
 }
 
 // Writes the two children of `parent` (split at totalL), returns the first
@@ -766,8 +743,6 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
   __shared__ Split split;
   __shared__ int32_t doSplit;
   __shared__ uint32_t totalR;
-  __shared__ unsigned long long best[2];
-  static_assert(kBuildThreads >= 3 * 64, "one thread per (axis, split position)");
 
   const SmallTask &tk = tasks[blockIdx.x];
   BNode *__restrict__ nodes = tk.nodes;
@@ -824,7 +799,18 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
         }
       }
       __syncthreads();
-      leaf = !sah_split_block(kmin, kmax, cnt, bins, f, minDim, n, best, split, doSplit);
+      // The split: the wave sweep (the sequential fold's choice and tie
+      // rules) on wave 0, broadcast through shared memory.
+      if (threadIdx.x < 64) {
+        Split s1;
+        const bool ok = sah_split_wave(kmin, kmax, cnt, bins, f, minDim, n, s1);
+        if (threadIdx.x == 0) {
+          doSplit = ok ? 1 : 0;
+          if (ok) split = s1;
+        }
+      }
+      __syncthreads();
+      leaf = doSplit == 0;
     }
     if (!leaf) {
       // Classify and rank (elements held blocked: thread t owns 4t..4t+3).
@@ -835,7 +821,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
         if (e < n) {
           float4 c = cen[lp[first + e]];
           float val = split.axis == 0 ? c.x : (split.axis == 1 ? c.y : c.z);
-          r = bin_of(val, f.nMin[split.axis], f.scale[split.axis], bins) > split.pos;
+          r = bin_of(val, pick3(f.nMin, split.axis), pick3(f.scale, split.axis), bins) > split.pos;
           rflag[e] = r ? 1 : 0;
         }
         v[i] = r ? 1u : 0u;
@@ -925,7 +911,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_small_build(const SmallTask *
       bool r = false;
       if (live) {
         const float val = sp2.axis == 0 ? c.x : (sp2.axis == 1 ? c.y : c.z);
-        r = bin_of(val, f.nMin[sp2.axis], f.scale[sp2.axis], bins) > sp2.pos;
+        r = bin_of(val, pick3(f.nMin, sp2.axis), pick3(f.scale, sp2.axis), bins) > sp2.pos;
       }
       const unsigned long long rmask = __ballot(r);
       const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1215,88 +1201,79 @@ static void build_topology(Topology &T, uint32_t bins, hipStream_t s, std::vecto
                             ((double)root.hi[1] - (double)root.lo[1]) * 1e-20,
                             ((double)root.hi[2] - (double)root.lo[2]) * 1e-20};
   std::vector<uint32_t> smallIds;
-  uint32_t nb = n > (uint32_t)kSmall ? 1u : 0u;
-  if (!nb) smallIds.push_back(0);
+  const uint32_t nb0 = n > (uint32_t)kSmall ? 1u : 0u;
+  if (!nb0) smallIds.push_back(0);
   // Level buffers, sized once: a big node holds more than kSmall primitives,
-  // so a level has at most n / kSmall of them and n / kChunk + nb chunks.
+  // so a level has at most n / kSmall of them and n / kChunk + nb chunks;
+  // the small-subtree roots are disjoint, at most n.
   const uint32_t maxBig = n / (uint32_t)kSmall + 1u, maxChunks = n / (uint32_t)kChunk + maxBig + 1u;
-  DevBuf<uint8_t> isR;
-  DevBuf<uint32_t> X, Z, rank, tmp, dNext, dSmall, dCounts(2), dFirst, gkmin, gkmax, gcnt, chunkR, chunkRB;
-  DevBuf<BNode> dLevel;
-  DevBuf<BigState> dBig;
-  DevBuf<Chunk> dChunks;
-  if (nb) {
-    isR.alloc(n); X.alloc(n); Z.alloc(n); rank.alloc(n); tmp.alloc(n);
-    dNext.alloc(2 * maxBig + 4);
-    dSmall.alloc(2 * maxBig + 4);
-    dLevel.alloc(maxBig);
-    dBig.alloc(maxBig);
-    dFirst.alloc(maxBig + 1);
-    dChunks.alloc(maxChunks);
-    chunkR.alloc(maxChunks);
-    chunkRB.alloc(maxChunks);
-    gkmin.alloc((size_t)maxBig * 3 * bins * 3);
-    gkmax.alloc((size_t)maxBig * 3 * bins * 3);
-    gcnt.alloc((size_t)maxBig * 3 * bins);
-    const uint32_t root = 0;
-    dNext.upload(&root, 1, s);
-  }
-  std::vector<BNode> hn;
-  std::vector<Chunk> chunks;
-  std::vector<uint32_t> firstChunk;
-  while (nb) {
-    MTX_REQUIRE(nb <= maxBig, MTX_ERROR, "BVH builder: big-node level larger than its bound");
-    hipLaunchKernelGGL(k_level_nodes, dim3(blocks(nb, 64)), dim3(64), 0, s, T.nodes.ptr, dNext.ptr, nb, dLevel.ptr,
-                       dBig.ptr);
-    hn.resize(nb);
-    dLevel.download(hn.data(), nb, s);
-    MTX_HIP(hipStreamSynchronize(s));
-    chunks.clear();
-    firstChunk.resize(nb + 1);
-    for (uint32_t b = 0; b < nb; b++) {
-      firstChunk[b] = (uint32_t)chunks.size();
-      for (uint32_t e = 0; e < hn[b].primCount; e += kChunk)
-        chunks.push_back(Chunk{b, e, std::min<uint32_t>(e + kChunk, hn[b].primCount), 0});
+  if (nb0) {
+    DevBuf<uint8_t> isR(n);
+    DevBuf<uint32_t> X(n), Z(n), rank(n), tmp(n), dNext(2 * maxBig + 4), dSmall(n + 2), dNb(2), dNc(1),
+        dSmallCount(1), dFirst(maxBig + 1), chunkR(maxChunks), chunkRB(maxChunks);
+    DevBuf<uint32_t> gkmin((size_t)maxBig * 3 * bins * 3), gkmax((size_t)maxBig * 3 * bins * 3),
+        gcnt((size_t)maxBig * 3 * bins);
+    DevBuf<BNode> dLevel(maxBig);
+    DevBuf<BigState> dBig(maxBig);
+    DevBuf<Chunk> dChunks(maxChunks);
+    const uint32_t init[4] = {0u, 1u, 0u, 0u};  // dNext[0] = root; level counts {1, 0}; no small roots yet
+    dNext.upload(init, 1, s);
+    dNb.upload(init + 1, 2, s);
+    dSmallCount.upload(init + 2, 1, s);
+    // Levels run back to back with their counts on the device: every launch
+    // is sized for the level's bound and its threads beyond the level's
+    // nodes or chunks return at once; the host reads the next level's size
+    // once per kLevelsPerCheck levels (a finished build runs empty levels).
+    constexpr int kLevelsPerCheck = 4;
+    const uint32_t gNodes = blocks(maxBig, 64), gChunks = maxChunks;
+    for (uint32_t level = 0;;) {
+      for (int k = 0; k < kLevelsPerCheck; k++, level++) {
+        const uint32_t *nb = dNb.ptr + (level & 1u);
+        uint32_t *nbNext = dNb.ptr + ((level + 1u) & 1u);
+        hipLaunchKernelGGL(k_level_nodes, dim3(gNodes), dim3(64), 0, s, T.nodes.ptr, dNext.ptr, nb, dLevel.ptr,
+                           dBig.ptr);
+        hipLaunchKernelGGL(k_chunk_first, dim3(1), dim3(1024), 0, s, dLevel.ptr, nb, dFirst.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_chunk_fill, dim3(blocks(maxChunks)), dim3(256), 0, s, dLevel.ptr, nb, dFirst.ptr,
+                           dNc.ptr, dChunks.ptr);
+        hipLaunchKernelGGL(k_bins_clear, dim3(std::min<uint32_t>(blocks((size_t)maxBig * 9 * bins), 1024)),
+                           dim3(256), 0, s, gkmin.ptr, gkmax.ptr, gcnt.ptr, nb, (int32_t)bins);
+        hipLaunchKernelGGL(k_bin_chunks, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr,
+                           T.nodes.ptr, T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, (int32_t)bins, minDim[0],
+                           minDim[1], minDim[2], gkmin.ptr, gkmax.ptr, gcnt.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_sweep, dim3(blocks((size_t)maxBig * 64)), dim3(256), 0, s, dBig.ptr, nb, T.nodes.ptr,
+                           gkmin.ptr, gkmax.ptr, gcnt.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2]);
+        hipLaunchKernelGGL(k_classify, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr,
+                           T.nodes.ptr, T.prim.ptr, T.cen.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2],
+                           isR.ptr, chunkR.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_chunk_prefix, dim3(gNodes), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr, dFirst.ptr,
+                           chunkR.ptr, chunkRB.ptr);
+        hipLaunchKernelGGL(k_xz, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
+                           isR.ptr, chunkRB.ptr, X.ptr, Z.ptr, rank.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_count_p, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
+                           isR.ptr, Z.ptr, rank.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_scatter, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
+                           isR.ptr, X.ptr, Z.ptr, rank.ptr, T.prim.ptr, tmp.ptr, dNc.ptr);
+        hipLaunchKernelGGL(k_copyback, dim3(gChunks), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr,
+                           T.nodes.ptr, tmp.ptr, T.prim.ptr, dNc.ptr);
+        MTX_HIP(hipMemsetAsync(nbNext, 0, sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_children, dim3(gNodes), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr, counter.ptr,
+                           dNext.ptr, nbNext, dSmall.ptr, dSmallCount.ptr);
+        MTX_HIP(hipGetLastError());
+      }
+      uint32_t next = 0;
+      MTX_HIP(hipMemcpyAsync(&next, dNb.ptr + (level & 1u), sizeof next, hipMemcpyDeviceToHost, s));
+      MTX_HIP(hipStreamSynchronize(s));
+      MTX_REQUIRE(next <= maxBig, MTX_ERROR, "BVH builder: big-node level larger than its bound");
+      if (!next) break;
     }
-    firstChunk[nb] = (uint32_t)chunks.size();
-    const uint32_t nc = (uint32_t)chunks.size();
-    MTX_REQUIRE(nc <= maxChunks, MTX_ERROR, "BVH builder: chunk list larger than its bound");
-    dChunks.upload(chunks.data(), nc, s);
-    dFirst.upload(firstChunk.data(), nb + 1, s);
-    const uint32_t nBins = nb * 3 * bins;
-    hipLaunchKernelGGL(k_bins_clear, dim3(blocks(nBins * 3)), dim3(256), 0, s, gkmin.ptr, gkmax.ptr, gcnt.ptr, nBins);
-    hipLaunchKernelGGL(k_bin_chunks, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
-                       T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2],
-                       gkmin.ptr, gkmax.ptr, gcnt.ptr);
-    hipLaunchKernelGGL(k_sweep, dim3(blocks(nb, 64)), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr, gkmin.ptr,
-                       gkmax.ptr, gcnt.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2]);
-    hipLaunchKernelGGL(k_classify, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
-                       T.prim.ptr, T.cen.ptr, (int32_t)bins, minDim[0], minDim[1], minDim[2], isR.ptr, chunkR.ptr);
-    hipLaunchKernelGGL(k_chunk_prefix, dim3(blocks(nb, 64)), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr,
-                       dFirst.ptr, chunkR.ptr, chunkRB.ptr);
-    hipLaunchKernelGGL(k_xz, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr, isR.ptr,
-                       chunkRB.ptr, X.ptr, Z.ptr, rank.ptr);
-    hipLaunchKernelGGL(k_count_p, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
-                       isR.ptr, Z.ptr, rank.ptr);
-    hipLaunchKernelGGL(k_scatter, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
-                       isR.ptr, X.ptr, Z.ptr, rank.ptr, T.prim.ptr, tmp.ptr);
-    hipLaunchKernelGGL(k_copyback, dim3(nc), dim3(kBuildThreads), 0, s, dChunks.ptr, dBig.ptr, T.nodes.ptr,
-                       tmp.ptr, T.prim.ptr);
-    MTX_HIP(hipMemsetAsync(dCounts.ptr, 0, 2 * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_children, dim3(blocks(nb, 64)), dim3(64), 0, s, dBig.ptr, nb, T.nodes.ptr, counter.ptr,
-                       dNext.ptr, dCounts.ptr, dSmall.ptr, dCounts.ptr + 1);
-    MTX_HIP(hipGetLastError());
-    uint32_t hc[2];
-    dCounts.download(hc, 2, s);
+    uint32_t nSmall = 0;
+    dSmallCount.download(&nSmall, 1, s);
     MTX_HIP(hipStreamSynchronize(s));
-    nb = hc[0];
-    if (hc[1]) {
-      size_t old = smallIds.size();
-      smallIds.resize(old + hc[1]);
-      dSmall.download(smallIds.data() + old, hc[1], s);
-    }
+    MTX_REQUIRE(nSmall <= n, MTX_ERROR, "BVH builder: small-subtree list larger than its bound");
+    smallIds.resize(nSmall);
+    dSmall.download(smallIds.data(), nSmall, s);
+    MTX_HIP(hipStreamSynchronize(s));  // the level buffers are freed on return
   }
-  MTX_HIP(hipStreamSynchronize(s));
   for (uint32_t r : smallIds)
     small.push_back(SmallTask{T.nodes.ptr, T.prim.ptr, T.plo.ptr, T.phi.ptr, T.cen.ptr, counter.ptr,
                               {minDim[0], minDim[1], minDim[2]}, r, 0u});
