@@ -149,7 +149,7 @@ def serving_benchmark(n_jobs=40, rate=40.0, per_sample=False):
         st = server.stats()
         out[mode] = dict(msamples_per_s=round(samples / makespan / 1e6, 3), makespan_s=round(makespan, 3),
                          p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
-                         rounds=st["rounds"])
+                         rounds=st["rounds"], graph_replays=st["graph_replays"], graph_captures=st["graph_captures"])
         log(f"serving {mode}: {out[mode]}")
     out["speedup"] = round(out["served"]["msamples_per_s"] / out["plain_whole_job"]["msamples_per_s"], 3)
     if per_sample:

@@ -709,6 +709,11 @@ int32_t mtx_server_job_cancel(mtx_handle server, mtx_handle job);
 int32_t mtx_server_job_release(mtx_handle server, mtx_handle job);
 /* stats[0] rounds, [1] rays traced, [2] running jobs, [3] queued jobs, [4] admitted bytes. */
 int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]);
+/* stats[0] rounds launched from a captured HIP graph (a round whose job
+ * count, path and pixel totals, event limit and lobe set match the
+ * previous round's replays its capture), [1] captures. No reference
+ * counterpart (measurement). */
+int32_t mtx_server_graph_stats(mtx_handle server, uint64_t stats[2]);
 int32_t mtx_server_destroy(mtx_handle server);
 
 /* ------------------------------------------------------------- denoiser */

@@ -117,6 +117,7 @@ def declare(L):
         "mtx_server_job_cancel": (i32, [u64, u64]),
         "mtx_server_job_release": (i32, [u64, u64]),
         "mtx_server_stats": (i32, [u64, vp]),
+        "mtx_server_graph_stats": (i32, [u64, vp]),
         "mtx_server_destroy": (i32, [u64]),
         "mtx_render_settings_default": (i32, [vp]),
         "mtx_display_settings_default": (i32, [vp]),

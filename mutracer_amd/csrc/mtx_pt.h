@@ -221,9 +221,12 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const
 // = sum of pathCount, pixels = sum of pixelCount, events = the largest job's
 // event limit; `full`: some job's scene has kSceneFull or its sampling
 // pattern is the random one.
+// With `graph` (and cfg.graphs), the launch sequence is captured once and
+// replayed while its shape (job count, totals, buffers, stream) is unchanged:
+// the kernels read the slots from device memory.
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, bool full, LaunchTimer *timer = nullptr);
+                             hipStream_t s, bool full, LaunchTimer *timer = nullptr, WfGraph *graph = nullptr);
 // Whether one job needs the full shading kernels.
 inline bool needs_full_shading(const GpuScene &scene, const RenderParams &p) {
   return (scene.features & kSceneFull) != 0u || p.pattern == 2u;

@@ -1175,6 +1175,7 @@ struct ServerObj {
   std::unordered_map<uint64_t, std::shared_ptr<ServerJob>> jobs;
   WfBuffers::Owner wf;
   WfConfig cfg;
+  WfGraph graph;  // the round's launch sequence, replayed while the round's shape repeats
   DevBuf<JobSlot> slots;
   DevBuf<uint64_t> rays;
 };
@@ -1245,10 +1246,20 @@ static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
   if (!idle) {
     bool full = false;
     for (const JobSlot &js : slots) full = full || needs_full_shading(js.scene, js.params);
-    S.slots.upload(slots.data(), slots.size(), s);
-    launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg, s,
-                            full);
-    MTX_HIP(hipStreamSynchronize(s));
+    // A fixed-size slot table keeps its address, so the captured round
+    // replays whenever the next round has the same shape.
+    if (S.slots.count < (size_t)kMaxJobsPerRound) S.slots.alloc(kMaxJobsPerRound);
+    // The null stream cannot be captured: rounds then run on the server's
+    // own blocking stream (ordered with the null stream's admission work).
+    hipStream_t rs = s;
+    if (!rs && S.cfg.graphs) {
+      if (!S.graph.own) MTX_HIP(hipStreamCreate(&S.graph.own));
+      rs = S.graph.own;
+    }
+    S.slots.upload(slots.data(), slots.size(), rs);
+    launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg,
+                            rs, full, nullptr, &S.graph);
+    MTX_HIP(hipStreamSynchronize(rs));
     const double t = now_ms();
     for (auto &pj : parts) {
       pj.first->submitted += pj.second;
@@ -1438,6 +1449,17 @@ int32_t mtx_server_stats(mtx_handle server, uint64_t stats[5]) {
   stats[2] = S->running.size();
   stats[3] = S->queued.size();
   stats[4] = S->used;
+  return MTX_OK;
+  API_CATCH(nullptr, 0)
+}
+
+int32_t mtx_server_graph_stats(mtx_handle server, uint64_t stats[2]) {
+  API_TRY
+  MTX_REQUIRE(stats, MTX_INVALID_ARGUMENT, "stats is NULL");
+  auto S = get_server(server);
+  std::lock_guard<std::mutex> g(S->m);
+  stats[0] = S->graph.replays;
+  stats[1] = S->graph.captures;
   return MTX_OK;
   API_CATCH(nullptr, 0)
 }

@@ -882,20 +882,55 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
 
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, bool full, LaunchTimer *timer) {
+                             hipStream_t s, bool full, LaunchTimer *timer, WfGraph *graph) {
   if (!nJobs || !nPaths) return;
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
   int32_t *spill = spill_for(s, std::max(std::max(std::max(g.trace, g.shadow), g.walk), g.tail), kWfTraceBlock);
   buf.ensure(nPaths, cfg.queueSort);
   const WfBuffers W = buf.view(nPaths, cfg.queueSort);
-  WfLaunch L{};
+  WfLaunch L;
+  std::memset((void *)&L, 0, sizeof L);
   L.jobs = devSlots;
   L.nJobs = nJobs;
   L.totalPixels = totalPixels;
   L.rays = rays;
   L.work = nullptr;
-  run_generation<true>(L, W, std::min(eventLimit, 50), totalPixels, cfg, g, spill, s, timer, full);
+  const int events = std::min(eventLimit, 50);
+  if (graph && cfg.graphs && !timer) {
+    // The caller's slot upload precedes the launch on the same stream (the
+    // server's own blocking stream is ordered with the null stream).
+    hipStream_t gs = s;
+    if (!gs) {
+      if (!graph->own) MTX_HIP(hipStreamCreate(&graph->own));
+      gs = graph->own;
+    }
+    uint64_t key = 1469598103934665603ull;
+    key = fnv(key, &L, sizeof L);
+    const void *ptrs[4] = {W.o, W.counts, W.rng, spill};
+    key = fnv(key, ptrs, sizeof ptrs);
+    const int64_t cf[13] = {cfg.maxPaths, cfg.traceOversub, cfg.shadeOversub, cfg.walker, cfg.ldsStack,
+                            cfg.shadeWaves, cfg.graphs, cfg.tailPaths, cfg.queueSort, cfg.ldsStackClosest,
+                            (int64_t)nPaths, (int64_t)events, (int64_t)full};
+    key = fnv(key, cf, sizeof cf);
+    key = fnv(key, &gs, sizeof gs);
+    if (!graph->exec || graph->key != key) {
+      graph->reset();
+      hipGraph_t gr = nullptr;
+      MTX_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+      run_generation<true>(L, W, events, totalPixels, cfg, g, spill, gs, nullptr, full);
+      MTX_HIP(hipStreamEndCapture(gs, &gr));
+      hipError_t e = hipGraphInstantiate(&graph->exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      MTX_HIP(e);
+      graph->key = key;
+      graph->captures++;
+    }
+    MTX_HIP(hipGraphLaunch(graph->exec, gs));
+    graph->replays++;
+    return;
+  }
+  run_generation<true>(L, W, events, totalPixels, cfg, g, spill, s, timer, full);
 }
 
 }  // namespace mtx

@@ -94,8 +94,10 @@ class Server:
     def stats(self):
         out = np.zeros(5, np.uint64)
         abi.check(abi.lib().mtx_server_stats(self.handle.value, _P(out)))
+        g = np.zeros(2, np.uint64)
+        abi.check(abi.lib().mtx_server_graph_stats(self.handle.value, _P(g)))
         return dict(rounds=int(out[0]), rays=int(out[1]), running=int(out[2]), queued=int(out[3]),
-                    admitted_bytes=int(out[4]))
+                    admitted_bytes=int(out[4]), graph_replays=int(g[0]), graph_captures=int(g[1]))
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -68,6 +68,25 @@ def test_mixed_equals_alone(env, max_jobs, slice_):
     assert server.stats()["rays"] > 0
 
 
+def test_graph_replayed_rounds_equal_alone(env):
+    """Rounds of the same shape (job count, totals, event limit) replay one
+    captured HIP graph; the kernels read each round's slots from device
+    memory, so replayed rounds still give every job its alone result."""
+    render, serve, scenes, rs = env
+    server = serve.Server(max_samples_per_job_round=2)
+    jobs = [(1, 64, 36, 12, dict(bounces=8, direct_light_candidates=2)),
+            (0, 40, 30, 12, dict(bounces=8, direct_light_candidates=1))]
+    handles, rounds = _serve_all(server, rs, jobs)
+    st = server.stats()
+    assert rounds == 6 and st["graph_captures"] >= 1 and st["graph_replays"] == rounds
+    assert st["graph_replays"] > st["graph_captures"]
+    for h, (si, w, hh, spp, kw) in zip(handles, jobs):
+        sums, counts = server.read_accumulators(h)
+        ref_sums, ref_counts = _alone(render, rs, si, w, hh, spp, kw)
+        assert np.array_equal(counts, ref_counts)
+        assert np.array_equal(sums, ref_sums)
+
+
 def test_job_matches_oracle(env, orc):
     render, serve, scenes, rs = env
     server = serve.Server(max_samples_per_job_round=2)
