@@ -162,16 +162,29 @@ struct LaneStackT {
 };
 using LaneStack = LaneStackT<kLdsStack>;
 
+// Bit 1 of an instance slot's c3.w word: the instance's material is
+// alpha-masked, so an any-hit query accepts its triangles only where the
+// coverage test passes (intersectSceneQuery, intersect.wgsl.ts:643-646).
+constexpr uint32_t kSlotMasked = 2u;
+
+// Coverage test of any-hit queries that have no material data: every
+// triangle occludes.
+struct NoCoverage {
+  __device__ __forceinline__ bool covers(int32_t, int32_t, f3) const { return true; }
+};
+
 // Two-level traversal: TLAS near-first stack, leaf instances in reverse
 // order (optionally rejected by their world box), ray to object space with
 // the instance's w2l, BLAS near-first stack with the root pushed untested,
 // then the triangle loop with the self-hit skip and back-face culling of
-// getBVHTraverseCode. anyHit returns at the first accepted triangle.
-template <bool kAnyHit, bool kCount, class Stack>
+// getBVHTraverseCode. anyHit returns at the first accepted triangle; on a
+// masked instance a triangle is accepted only when cov.covers(slot,
+// triangle, barycentrics) holds, and a rejected one leaves tMax unchanged.
+template <bool kAnyHit, bool kCount, class Stack, class Cov = NoCoverage>
 __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, float tMax,
                                          bool doubleSided, int bounces, int32_t curTri,
                                          int32_t curSlot, Stack &st, TraceHit &h,
-                                         TraceCounters *cnt) {
+                                         TraceCounters *cnt, const Cov &cov = Cov()) {
   h.hit = false;
   h.slot = -1;
   h.tri = -1;
@@ -183,7 +196,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
   int sp = 0;
   int blasBase = -1;
   int leafFirst = 0, leafRemaining = 0;
-  bool matDouble = false;
+  bool matDouble = false, masked = false;
   int32_t slotIdx = -1;
   const MTX_AS_GLOBAL TNode *bnodes = nullptr;
   const MTX_AS_GLOBAL TriRec *btris = nullptr;
@@ -309,6 +322,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
             if (t > 0.0f) {
               f3 nrm = normalize(cross(p1 - p0, p2 - p0));
               if (!doubleSided && !matDouble && dot(nrm, d) > 0.0f) continue;
+              if (masked && !cov.covers(slotIdx, tri, bary)) continue;
               tMax = t;
               h.hit = true;
               h.t = t;
@@ -341,6 +355,7 @@ __device__ __forceinline__ bool traverse(const AccelView &acc, f3 ro, f3 rd, flo
         maxDim = max_dimension(vabs(d));
         shear = get_shear(d, invD, maxDim);
         matDouble = (fbits(c3.w) & 1u) != 0u;
+        masked = (fbits(c3.w) & kSlotMasked) != 0u;
         bnodes = as_global(s.nodes);
         btris = as_global(s.tris);
         blasBase = sp;

@@ -333,12 +333,14 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
       default: break;
     }
   }
-  // resolveMaterialAlphaMode (src/scene/MaterialState.ts:251-264): coverage
-  // (mask or blend) is not built.
-  bool opaque = p.alphaMode == 1;
+  // resolveMaterialAlphaMode (src/scene/MaterialState.ts:251-264), to the
+  // packed modes 0 opaque, 1 mask, 2 blend; blend (stochastic coverage) is
+  // not built.
+  uint32_t resolvedAlpha = p.alphaMode == 1 ? 0u : p.alphaMode == 2 ? 1u : 2u;
   if (p.alphaMode == 0)
-    opaque = !(p.alphaCutoff > 0.0f) && !(p.opacity < 1.0f) && p.mapAlphaUsage != 1 && maps[MTX_MAP_ALPHA] == kNone;
-  if (!opaque) unsupported("alpha coverage (mask and blend materials)");
+    resolvedAlpha = p.alphaCutoff > 0.0f ? 1u
+                    : (p.opacity < 1.0f || p.mapAlphaUsage == 1 || maps[MTX_MAP_ALPHA] != kNone) ? 2u : 0u;
+  if (resolvedAlpha == 2u) unsupported("blend alpha coverage (stochastic transparency)");
   if (p.anisotropy != 0.0f) unsupported("specular anisotropy");
   // materialEmission (src/renderer/materialEmission.ts:10-22), in f64 as JS;
   // a coat weight map leaves the coat tint to the sample.
@@ -375,7 +377,7 @@ void decode_material(const Words &W, size_t o, uint32_t version, uint32_t textur
   put3(MTX_MAT_SUBSURFACE_COLOR, p.subsurfaceColor);
   m[MTX_MAT_THIN_WALLED] = p.thinWalled ? 1.0f : 0.0f;
   m[MTX_MAT_OPACITY] = p.opacity;
-  m[MTX_MAT_ALPHA_MODE] = 0.0f;  // resolved: opaque
+  m[MTX_MAT_ALPHA_MODE] = (float)resolvedAlpha;
   m[MTX_MAT_ALPHA_CUTOFF] = std::max(p.alphaCutoff, 0.0f);
   m[MTX_MAT_MAP_ALPHA_USAGE] = p.mapAlphaUsage == 0 ? 0.0f : p.mapAlphaUsage == 1 ? 2.0f : 3.0f;
   m[MTX_MAT_NORMAL_SCALE] = p.normalScale[0];

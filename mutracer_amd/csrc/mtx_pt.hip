@@ -58,7 +58,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
         TraceHit h;
         rays++;
         if (kCount) nClosest++;
-        traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, event != 0, trace_depth(event), pr.curTri,
+        traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, event != 0, path_depth(pr.flags), pr.curTri,
                                 pr.curSlot, st, h, &tc);
         if (kCount) nHits += h.hit ? 1u : 0u;
         ShadowReq sr;
@@ -67,7 +67,8 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
           TraceHit sh;
           rays++;
           if (kCount) nShadow++;
-          if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts))
+          if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts,
+                                      SceneCoverage{&S}))
             pr.li = pr.li + sr.payload;
         }
         if (!alive) break;

@@ -396,12 +396,15 @@ static uint32_t upload_textures(RenderSceneObj &R, const mtx_scene_desc &D) {
   return n;
 }
 
-// Texture-related material fields: resolved alpha mode opaque (coverage is
-// not built), map alpha usage, slot indices within the texture table, finite
-// UV and normal scales. Returns whether some material samples a texture.
+// Texture-related material fields: resolved alpha mode opaque or mask (blend
+// coverage is not built), map alpha usage, slot indices within the texture
+// table, finite UV and normal scales. Returns whether some material samples a
+// texture.
 static bool check_material_maps(const float *mat, uint32_t textureCount) {
-  MTX_REQUIRE(mat[MTX_MAT_ALPHA_MODE] == 0.0f, MTX_UNAVAILABLE,
-              "alpha coverage (mask and blend materials) is not supported by this renderer");
+  MTX_REQUIRE(mat[MTX_MAT_ALPHA_MODE] != 2.0f, MTX_UNAVAILABLE,
+              "blend alpha coverage (stochastic transparency) is not supported by this renderer");
+  MTX_REQUIRE(mat[MTX_MAT_ALPHA_MODE] == 0.0f || mat[MTX_MAT_ALPHA_MODE] == 1.0f, MTX_INVALID_ARGUMENT,
+              "material alpha mode must be 0 (opaque), 1 (mask) or 2 (blend)");
   const float usage = mat[MTX_MAT_MAP_ALPHA_USAGE];
   MTX_REQUIRE(usage == 0.0f || usage == 2.0f || usage == 3.0f, MTX_INVALID_ARGUMENT,
               "map alpha usage must be 0, 2 or 3");
@@ -463,6 +466,9 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
     uint32_t mat = d.material_word >> 8;
     MTX_REQUIRE(mat < D.material_count, MTX_INVALID_ARGUMENT, "instance material out of range");
     if (D.materials[(size_t)mat * MTX_MATERIAL_FLOATS + MTX_MAT_DOUBLE_SIDED] > 0.5f) d.double_sided = 1;
+    // Shadow queries run the coverage test on instances of masked materials.
+    d.material_word = (d.material_word & ~kSlotMasked) |
+                      (D.materials[(size_t)mat * MTX_MATERIAL_FLOATS + MTX_MAT_ALPHA_MODE] == 1.0f ? kSlotMasked : 0u);
     R->triangles += D.meshes[d.blas_index].triangle_count;
   }
   R->accel = build_accel(R->blases, R->inst.data(), D.instance_count, tbins, D.flags, 0);

@@ -354,6 +354,7 @@ struct Mat {
   float transmissionWeight, coatWeight, coatRoughness, coatIor, coatDarkening, fuzzWeight, fuzzRoughness;
   float subsurfaceWeight;
   float relativeIor, alphaX, alphaY;
+  float alpha;  // coverage after the alpha mode: 1 opaque; mask 0 or 1
   bool inside, thinWalled;
 };
 
@@ -368,17 +369,20 @@ __device__ __forceinline__ void rough_alpha(float r, float aniso, float &ax, flo
 
 // Texture samples of getMaterial (materialBuffer.wgsl.ts:78-160), in its
 // order, on the clamped record values and before the coat broadening: the
-// base map (color; map alpha as transmission), transmission, metallic-
-// roughness, roughness, metalness, coat weight and roughness, fuzz color and
-// roughness, specular weight and color. Materials are opaque here, so the
-// alpha samples (alpha map, map alpha as coverage) have no effect.
+// base map (color; map alpha as transmission or as coverage), the alpha map
+// (coverage *= g), transmission, metallic-roughness, roughness, metalness,
+// coat weight and roughness, fuzz color and roughness, specular weight and
+// color. The coverage samples multiply ma.alpha before its alpha-mode
+// resolution in load_material.
 __device__ void apply_maps(const GpuScene &S, const float *__restrict__ m, float2 uv, float2 uv1, Mat &ma) {
   ma.emission = material_emission(S, m, uv, uv1);
   if (const uint32_t t = map_slot(m, MTX_MAP_BASE)) {
     const float4 c = map_at(S, t, uv, uv1);
     ma.baseColor = ma.baseColor * map_color(S, t - 1u, c);
     if (m[MTX_MAT_MAP_ALPHA_USAGE] == 3.0f) ma.transmissionWeight = 1.0f - c.w;
+    else if (m[MTX_MAT_MAP_ALPHA_USAGE] == 2.0f) ma.alpha *= c.w;
   }
+  if (const uint32_t t = map_slot(m, MTX_MAP_ALPHA)) ma.alpha *= map_at(S, t, uv, uv1).y;
   if (const uint32_t t = map_slot(m, MTX_MAP_TRANSMISSION)) ma.transmissionWeight *= map_at(S, t, uv, uv1).x;
   if (const uint32_t t = map_slot(m, MTX_MAP_METALLIC_ROUGHNESS)) {
     const float4 c = map_at(S, t, uv, uv1);
@@ -429,7 +433,12 @@ __device__ Mat load_material(const float *__restrict__ m, const float col[3], co
   ma.subsurfaceColor = vmaxs(f3{m[MTX_MAT_SUBSURFACE_COLOR], m[MTX_MAT_SUBSURFACE_COLOR + 1],
                                 m[MTX_MAT_SUBSURFACE_COLOR + 2]}, 0.0f);
   ma.thinWalled = kFull && m[MTX_MAT_THIN_WALLED] > 0.5f;
+  ma.alpha = m[MTX_MAT_OPACITY];
   if (kFull && S != nullptr && S->textures != nullptr) apply_maps(*S, m, uv, uv1, ma);
+  // alpha mode (materialBuffer.wgsl.ts:69-77): opaque covers fully; mask
+  // covers where the coverage reaches the cutoff. Blend materials are
+  // refused when the scene is created.
+  ma.alpha = m[MTX_MAT_ALPHA_MODE] == 1.0f ? (ma.alpha >= m[MTX_MAT_ALPHA_CUTOFF] ? 1.0f : 0.0f) : 1.0f;
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness, c2 = ma.coatRoughness * ma.coatRoughness;
     ma.specularRoughness = mixf(ma.specularRoughness, mtxfm::powf(mnf(1.0f, b2 * b2 + 2.0f * c2 * c2), 0.25f), ma.coatWeight);
@@ -1464,6 +1473,42 @@ __device__ void complete_surface(const GpuScene &S, const TraceHit &h, Surf &si)
   si.position = xform(l2w, position, 1.0f);
 }
 
+// intersectionAlphaPasses (intersectionAlpha.wgsl.ts:28-88) for the
+// triangle `tri` of slot `slot` hit at barycentrics b, whose material is
+// alpha-masked: coverage = opacity, times the base map's alpha when map alpha
+// is coverage, times the alpha map's g; both maps are sampled at UV0 * the
+// material's UV scale (the any-hit test reads no UV1 set), and the triangle
+// occludes when the coverage reaches the cutoff.
+__device__ bool alpha_covers(const GpuScene &S, int32_t slot, int32_t tri, f3 b) {
+  const ShadeSlot &sh = S.shade[slot];
+  const float *__restrict__ m = S.materials + (size_t)sh.material * MTX_MATERIAL_FLOATS;
+  float alpha = m[MTX_MAT_OPACITY];
+  if (S.textures != nullptr) {
+    const uint32_t base = m[MTX_MAT_MAP_ALPHA_USAGE] == 2.0f ? map_slot(m, MTX_MAP_BASE) : 0u;
+    const uint32_t amap = map_slot(m, MTX_MAP_ALPHA);
+    if (base | amap) {
+      float2 uv{0.0f, 0.0f};
+      if (sh.uvs != nullptr) {
+        const TriRec r = S.accel->slots[slot].tris[tri];
+        const uint32_t src = (uint32_t)fint(r.v0.w);
+        const uint32_t i0 = sh.indices[src * 3], i1 = sh.indices[src * 3 + 1], i2 = sh.indices[src * 3 + 2];
+        const float2 t0{sh.uvs[i0 * 2], sh.uvs[i0 * 2 + 1]}, t1{sh.uvs[i1 * 2], sh.uvs[i1 * 2 + 1]};
+        const float2 t2{sh.uvs[i2 * 2], sh.uvs[i2 * 2 + 1]};
+        uv = float2{b.x * t0.x + b.y * t1.x + b.z * t2.x, b.x * t0.y + b.y * t1.y + b.z * t2.y};
+      }
+      uv = float2{uv.x * m[MTX_MAT_UV_SCALE], uv.y * m[MTX_MAT_UV_SCALE + 1]};
+      if (base) alpha *= sample_map(S, base - 1u, uv).w;
+      if (amap) alpha *= sample_map(S, amap - 1u, uv).y;
+    }
+  }
+  return alpha >= m[MTX_MAT_ALPHA_CUTOFF];
+}
+// Coverage test of the shadow queries of a render scene.
+struct SceneCoverage {
+  const GpuScene *S;
+  __device__ __forceinline__ bool covers(int32_t slot, int32_t tri, f3 b) const { return alpha_covers(*S, slot, tri, b); }
+};
+
 __device__ __forceinline__ float nee_mis(uint32_t count, float lp, float sp) {
   if (count <= 1u) return power_heuristic(lp, sp);
   float w = (float)count * lp;
@@ -1484,8 +1529,13 @@ struct Shadow {
 // ---------------------------------------------------------------- path
 // State of one path between integrator events (rayTrace.compute.ts Path +
 // RayTraceAuxiliary, rayTraceCore.wgsl.ts:192-210, :323-331).
+// Bits 0-6 of `flags` are the path's booleans (kIgnored: the last vertex was
+// an alpha-masked pass-through, sampleIgnored of rayTrace.compute.ts:730);
+// bits 8-15 hold the path depth, the integrator's `depth`, which counts the
+// completed vertices that were not passed through.
 enum : uint32_t {
-  kFromEye = 1u, kIsDelta = 2u, kNextRefr = 4u, kIsBlank = 8u, kAlbedoSampled = 16u, kNotTransparent = 32u
+  kFromEye = 1u, kIsDelta = 2u, kNextRefr = 4u, kIsBlank = 8u, kAlbedoSampled = 16u, kNotTransparent = 32u,
+  kIgnored = 64u, kDepthShift = 8u
 };
 struct PathRegs {
   f3 ro, rd, li, beta, auxAlbedo, auxNormal;
@@ -1576,9 +1626,9 @@ __device__ void init_path(const RenderParams &P, const Smp &smp, PathRegs &pr) {
   pr.flags = kFromEye;
 }
 
-// The `bounces` argument of the closest-hit query at `event`: the path depth
-// before the event's increment (rayTrace.compute.ts:899-946).
-__device__ __forceinline__ int trace_depth(int event) { return event > 0 ? event - 1 : 0; }
+// The path depth: the `bounces` argument of the next closest-hit query
+// (rayTrace.compute.ts:745, before the event's completeVertex increment).
+__device__ __forceinline__ int path_depth(uint32_t flags) { return (int)(flags >> kDepthShift); }
 
 // One integrator event after its closest-hit query `h` (rayTrace.compute.ts
 // :891-946 = completeVertex :514-589 + shadeVertex :343-512 with
@@ -1609,7 +1659,7 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
   const f3 camPos{P.viewInv[12], P.viewInv[13], P.viewInv[14]};
   const int bounceLimit = (int)P.bounces;
   const int eventLimit = min(bounceLimit * 3, 50);
-  const int depth = event;
+  int depth = path_depth(pr.flags);
   bool abort = false;
   if (event == 0) {
     if (si.hit) {
@@ -1617,23 +1667,28 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       pr.auxNormal = dot(si.normal, si.position - camPos) > 0.0f ? -si.normal : si.normal;
       if (ma.transmissionWeight > 0.5f) pr.flags |= kAlbedoSampled; else pr.flags &= ~kAlbedoSampled;
       si.albedoSampled = ma.transmissionWeight > 0.5f;
-      if ((ma.transmissionWeight < 0.001f) & (ma.subsurfaceWeight < 0.001f)) pr.flags |= kNotTransparent;
+      if ((ma.transmissionWeight < 0.001f) & (ma.subsurfaceWeight < 0.001f) & (ma.alpha > 0.99f))
+        pr.flags |= kNotTransparent;
     }
-  } else {
+  } else if (!(pr.flags & kIgnored)) {
+    // completeVertex: the coverage draw (hashedSample(3u) < alpha) lies in
+    // (0, 1), so against a coverage of 0 or 1 it is `alpha > 0`; a
+    // passed-through surface does not occlude the light or the environment.
     smp.hashed_draw();
+    const bool alphaOccluded = ma.alpha > 0.0f;
     for (uint32_t i = 0; i < S.lightCount; i++) {
       LSample ls;
       float d = intersect_light(S.lights + (size_t)i * 16, pr.ro, pr.rd, ls);
       ls.pdf *= S.lightPmf[i];
       bool occluded = si.hit && d > len(si.position - pr.ro);
-      if (d != kInf && !occluded) {
+      if (d != kInf && !(occluded && alphaOccluded)) {
         float mis = bsdf_mis(P.candidates, pr.scatPdf, ls.pdf);
         pr.li = pr.li + mis * pr.beta * ls.emission;
         pr.alpha = 1.0f;
         abort = true;
       }
     }
-    if (S.env && !si.hit) {
+    if (S.env && !(si.hit && alphaOccluded)) {
       float u, v;
       env_uv(pr.rd, u, v);
       f3 em = texture_linear(S, u, v);
@@ -1647,9 +1702,12 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       pr.alpha = 1.0f;
       abort = true;
     }
+    depth++;
+    pr.flags = (pr.flags & ((1u << kDepthShift) - 1u)) | ((uint32_t)depth << kDepthShift);
   }
   if ((pr.flags & kNotTransparent) && event > bounceLimit) abort = true;
   if (abort || event >= eventLimit) return false;
+  pr.flags &= ~kIgnored;
   smp.bounce = (uint32_t)depth;
   if (!si.hit) {
     if (pr.flags & kFromEye) {
@@ -1666,6 +1724,14 @@ __device__ bool shade_event(const GpuScene &S, const RenderParams &P, const Lut 
       if (ba < 0.001f) pr.flags |= kIsBlank;
     }
     return false;
+  }
+  if (!(ma.alpha > 0.0f)) {
+    // A masked-out surface (shadeVertex :432-458): no material sample, no
+    // guides, no roulette; the ray continues behind it from position +
+    // d * EPS without consuming a depth.
+    pr.flags |= kIgnored;
+    pr.ro = si.position + kEps * pr.rd;
+    return true;
   }
   // SampleMaterial (sampleMaterial.wgsl.ts:316-508; surface vertex, no media).
   {

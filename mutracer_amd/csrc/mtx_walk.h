@@ -39,7 +39,7 @@ struct Walker {
   int sp, blasBase, leafFirst, leafRemaining;
   int32_t slotIdx, curTri, curSlot;
   int bounces;
-  bool inBlas, done, matDouble, doubleSided;
+  bool inBlas, done, matDouble, masked, doubleSided;
   const TNode *bnodes;
   const TriRec *btris;
   // result
@@ -75,6 +75,7 @@ struct Walker {
       maxDim = max_dimension(vabs(d));
       shear = get_shear(d, invD, maxDim);
       matDouble = (fbits(c3.w) & 1u) != 0u;
+      masked = kAnyHit && (fbits(c3.w) & kSlotMasked) != 0u;
       bnodes = s.nodes;
       btris = s.tris;
       inBlas = true;
@@ -93,7 +94,7 @@ struct Walker {
     hit = false; t = kRayMaxDistance; hitSlot = -1; hitTri = -1; bary = splat(0.0f);
     if (kCount) cnt = TraceCounters{0u, 0u, 0u};
     sp = 0; blasBase = 0; leafFirst = 0; leafRemaining = 0; slotIdx = -1;
-    inBlas = false; matDouble = false; bnodes = nullptr; btris = nullptr;
+    inBlas = false; matDouble = false; masked = false; bnodes = nullptr; btris = nullptr;
     maxDim = 2; shear = splat(0.0f);
     winv = splat(1.0f) / rd;
     o = ro; d = rd; invD = winv;
@@ -108,9 +109,11 @@ struct Walker {
     }
   }
 
-  // Processes `cur`; returns true once the query has finished.
-  template <class Stack>
-  __device__ __forceinline__ bool step(const AccelView &acc, Stack &st) {
+  // Processes `cur`; returns true once the query has finished. Any-hit
+  // queries accept a triangle of a masked instance only where cov.covers()
+  // holds (the coverage rule of traverse()).
+  template <class Stack, class Cov = NoCoverage>
+  __device__ __forceinline__ bool step(const AccelView &acc, Stack &st, const Cov &cov = Cov()) {
     if (cur >= 0) {
       const TNode n = (inBlas ? bnodes : acc.tlasNodes)[cur];
       const float d0 = intersect_box(o, f3of(n.a), f3of(n.b), invD, tMax);
@@ -144,6 +147,7 @@ struct Walker {
             const f3 nrm = normalize(cross(p1 - p0, p2 - p0));
             if (dot(nrm, d) > 0.0f) continue;
           }
+          if (kAnyHit && masked && !cov.covers(slotIdx, tri, b)) continue;
           tMax = tt;
           hit = true;
           t = tt;

@@ -78,7 +78,11 @@ __device__ __forceinline__ void store_path(const WfBuffers &W, uint32_t p, const
   W.d[p] = make_float4(pr.rd.x, pr.rd.y, pr.rd.z, pr.alpha);
   W.li[p] = make_float4(pr.li.x, pr.li.y, pr.li.z, intf((int32_t)pr.flags));
   W.beta[p] = make_float4(pr.beta.x, pr.beta.y, pr.beta.z, intf(pr.curTri));
-  W.alb[p] = make_float4(pr.auxAlbedo.x, pr.auxAlbedo.y, pr.auxAlbedo.z, intf(pr.curSlot));
+  // The next closest-hit query skips the last hit triangle from depth 1 on
+  // (traverse's bounces > 0); a path at depth 0 stores slot -1, which no
+  // instance has, so the trace kernels pass bounces = 1 for every path.
+  W.alb[p] = make_float4(pr.auxAlbedo.x, pr.auxAlbedo.y, pr.auxAlbedo.z,
+                         intf(path_depth(pr.flags) > 0 ? pr.curSlot : -1));
   W.nrm[p] = make_float4(pr.auxNormal.x, pr.auxNormal.y, pr.auxNormal.z, 0.0f);
 }
 
@@ -164,11 +168,9 @@ k_wf_trace(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill
     TraceHit h;
     if (kMulti) {
       const AccelView av = *job_of<true>(L, W, p).scene.accel;
-      traverse<false, kCount>(av, f3of(o), f3of(d), kRayMaxDistance, event != 0, trace_depth(event), curTri, curSlot,
-                              st, h, &tc);
+      traverse<false, kCount>(av, f3of(o), f3of(d), kRayMaxDistance, event != 0, 1, curTri, curSlot, st, h, &tc);
     } else {
-      traverse<false, kCount>(one, f3of(o), f3of(d), kRayMaxDistance, event != 0, trace_depth(event), curTri,
-                              curSlot, st, h, &tc);
+      traverse<false, kCount>(one, f3of(o), f3of(d), kRayMaxDistance, event != 0, 1, curTri, curSlot, st, h, &tc);
     }
     W.hitA[p] = make_float4(h.t, h.bary.x, h.bary.y, h.bary.z);
     W.hitB[p] = make_int2(h.hit ? h.slot : -1, h.tri);
@@ -310,10 +312,13 @@ k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spil
     TraceHit h;
     bool occluded;
     if (kMulti) {
-      const AccelView av = *job_of<true>(L, W, p).scene.accel;
-      occluded = traverse<true, kCount>(av, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc);
+      const GpuScene &S = job_of<true>(L, W, p).scene;
+      const AccelView av = *S.accel;
+      occluded = traverse<true, kCount>(av, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc,
+                                        SceneCoverage{&S});
     } else {
-      occluded = traverse<true, kCount>(one, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc);
+      occluded = traverse<true, kCount>(one, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc,
+                                        SceneCoverage{&L.one.scene});
     }
     if (!occluded) add_payload(W, p, c);
   };
@@ -368,8 +373,8 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
       TraceHit h;
       rays++;
       if (kCount) nClosest++;
-      traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, e != 0, trace_depth(e), pr.curTri, pr.curSlot, st,
-                              h, &tc);
+      traverse<false, kCount>(av, pr.ro, pr.rd, kRayMaxDistance, e != 0, path_depth(pr.flags), pr.curTri, pr.curSlot,
+                              st, h, &tc);
       if (kCount) nHits += h.hit ? 1u : 0u;
       ShadowReq sr;
       const bool alive = shade_event<kFull>(S, P, lut, pr, h, e, smp, sr);
@@ -377,7 +382,8 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
         TraceHit sh;
         rays++;
         if (kCount) nShadow++;
-        if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts))
+        if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts,
+                                    SceneCoverage{&S}))
           pr.li = pr.li + sr.payload;
       }
       if (!alive) break;
@@ -410,13 +416,14 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
   const uint32_t *q = queue_of(W, event);
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)L.rays, (unsigned long long)n);
   AccelView av{};
+  const GpuScene *scene = &L.one.scene;  // the scene of the lane's ray (coverage of masked instances)
   if (!kMulti) av = *L.one.scene.accel;
   Walker<kShadow, kCount> w;
   bool have = false, exhausted = false;
   uint32_t entry = 0, path = 0;
   uint64_t totals[4] = {0, 0, 0, 0};
   const bool doubleSided = kShadow ? true : event != 0;
-  const int bounces = kShadow ? 1 : trace_depth(event);
+  const int bounces = 1;  // closest hit: depth-0 paths store slot -1 (store_path)
   while (true) {
     const bool need = !have && !exhausted;
     const uint32_t idx = wave_append(fetch, need);
@@ -426,7 +433,10 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
         if (kShadow) {
           const float4 a = W.s0[idx], b = W.s1[idx], c = W.s2[idx];
           path = (uint32_t)fint(b.w);
-          if (kMulti) av = *job_of<true>(L, W, path).scene.accel;
+          if (kMulti) {
+            scene = &job_of<true>(L, W, path).scene;
+            av = *scene->accel;
+          }
           w.init(av, f3of(a), f3of(b), a.w, doubleSided, bounces, fint(c.w), W.s3[idx], st);
         } else {
           path = q[idx];
@@ -441,7 +451,7 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
       }
     }
     if (!__any(have)) break;
-    if (have && (w.done || w.step(av, st))) {
+    if (have && (w.done || w.step(av, st, SceneCoverage{scene}))) {
       have = false;
       if (kShadow) {
         if (!w.hit) add_payload(W, path, W.s2[entry]);

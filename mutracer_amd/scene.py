@@ -571,6 +571,91 @@ def textured_room(width=256, height=256):
 
 
 # ---------------------------------------------------------------- config 2
+def _gradient_sky(w=16, h=8):
+    """Small RGBA32F environment: a blue-to-white vertical gradient."""
+    t = (np.arange(h, dtype=np.float32) + 0.5) / h
+    env = np.zeros((h, w, 4), np.float32)
+    env[..., 0] = (0.3 + 0.7 * t)[:, None]
+    env[..., 1] = (0.5 + 0.5 * t)[:, None]
+    env[..., 2] = 1.0
+    env[..., 3] = 1.0
+    return env
+
+
+ALPHA_CASTERS = ("absent", "opaque", "mask_covered", "mask_open", "alpha_map_covered", "alpha_map_open",
+                 "base_alpha_covered", "base_alpha_open", "checker")
+
+
+def alpha_caster(kind):
+    """A floor with a box under a downward rect light and a sky environment;
+    a quad at y = 2 (outside the camera's view, with nothing above it but the
+    light) casts the shadow. `kind` picks its material: absent, opaque, mask
+    with opacity at / just below the cutoff, mask with an alpha map whose g
+    is above / below the cutoff, mask with the base map's alpha as coverage
+    above / below it, and a checker alpha map (half the texels covered).
+    Every variant carries the same textures (constant RGBA32F images and an
+    RGBA8 checker)."""
+    s = Scene()
+    s.textures = [Texture(np.tile(np.array([0.1, 0.9, 0.2, 0.3], np.float32), (2, 2, 1))),
+                  Texture(np.tile(np.array([0.9, 0.3, 0.9, 0.9], np.float32), (2, 2, 1))),
+                  Texture(np.tile(np.array([1.0, 1.0, 1.0, 0.9], np.float32), (2, 2, 1))),
+                  Texture(np.tile(np.array([1.0, 1.0, 1.0, 0.3], np.float32), (2, 2, 1))),
+                  Texture(np.where(((np.arange(8)[:, None] + np.arange(8)[None, :]) % 2)[..., None] == 0,
+                                   np.array([255, 255, 255, 255], np.uint8), np.array([255, 0, 255, 255], np.uint8)),
+                          uv_scale=(1.0, 1.0))]
+    s.materials = [Material(base_color=(0.8, 0.8, 0.8)), Material(base_color=(0.7, 0.2, 0.1))]
+    p, i, n = quad((-3, 0, 3), (3, 0, 3), (3, 0, -3), (-3, 0, -3))
+    s.add_instance(s.add_mesh(p, i, n), np.eye(4).reshape(16), 0)
+    s.add_instance(s.add_mesh(*box_mesh(0.6, 0.5, 0.6)), translate((0.2, 0.0, -0.1)), 1)
+    s.lights = [RectLight(position=(0.0, 3.0, 0.0), u=(0.5, 0.0, 0.0), v=(0.0, 0.0, 0.5), intensity=20.0)]
+    s.env = _gradient_sky()
+    s.camera = Camera(eye=(0.0, 1.5, 4.0), target=(0.0, 0.3, 0.0), fov=40.0)
+    if kind == "absent":
+        return s
+    caster = dict(opaque=Material(base_color=(0.5, 0.5, 0.5)),
+                  mask_covered=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", opacity=0.5, alpha_cutoff=0.5),
+                  mask_open=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", opacity=0.4999,
+                                     alpha_cutoff=0.5),
+                  alpha_map_covered=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", alpha_cutoff=0.5,
+                                             alpha_map=0),
+                  alpha_map_open=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", alpha_cutoff=0.5,
+                                          alpha_map=1),
+                  base_alpha_covered=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", alpha_cutoff=0.5,
+                                              map=2, map_alpha_usage="alpha"),
+                  base_alpha_open=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", alpha_cutoff=0.5,
+                                           map=3, map_alpha_usage="alpha"),
+                  checker=Material(base_color=(0.5, 0.5, 0.5), alpha_mode="mask", alpha_cutoff=0.5, alpha_map=4))[kind]
+    s.materials.append(caster)
+    p, i, n = quad((-0.6, 2.0, 0.6), (0.6, 2.0, 0.6), (0.6, 2.0, -0.6), (-0.6, 2.0, -0.6))
+    s.add_instance(s.add_mesh(p, i, n, uvs=QUAD_UVS), np.eye(4).reshape(16), len(s.materials) - 1)
+    return s
+
+
+def masked_panes(panes, checker=False):
+    """A floor under a sky environment (no light, nothing else) seen through
+    `panes` camera-facing quads of a mask material with opacity below its
+    cutoff (fully passed through), or, with checker, of a checker alpha map
+    (half the texels covered) in front of a box on the floor."""
+    s = Scene()
+    s.textures = [Texture(np.where(((np.arange(8)[:, None] + np.arange(8)[None, :]) % 2)[..., None] == 0,
+                                   np.array([255, 255, 255, 255], np.uint8), np.array([255, 0, 255, 255], np.uint8)))]
+    s.materials = [Material(base_color=(0.8, 0.7, 0.6)),
+                   Material(base_color=(0.3, 0.6, 0.9), alpha_mode="mask", opacity=0.2, alpha_cutoff=0.5),
+                   Material(base_color=(0.3, 0.6, 0.9), alpha_mode="mask", alpha_cutoff=0.5, alpha_map=0),
+                   Material(base_color=(0.7, 0.2, 0.1))]
+    p, i, n = quad((-4, 0, 4), (4, 0, 4), (4, 0, -4), (-4, 0, -4))
+    s.add_instance(s.add_mesh(p, i, n), np.eye(4).reshape(16), 0)
+    if checker:
+        s.add_instance(s.add_mesh(*box_mesh(0.8, 0.8, 0.8)), translate((0.0, 0.0, -0.5)), 3)
+    for k in range(panes):
+        z = 2.5 - 0.4 * k
+        p, i, n = quad((-3, -1, z), (3, -1, z), (3, 4, z), (-3, 4, z))
+        s.add_instance(s.add_mesh(p, i, n, uvs=QUAD_UVS), np.eye(4).reshape(16), 2 if checker else 1)
+    s.env = _gradient_sky()
+    s.camera = Camera(eye=(0.0, 1.2, 4.0), target=(0.0, 0.2, 0.0), fov=40.0)
+    return s
+
+
 def icosphere(freq):
     """Geodesic (class I) icosphere: 20 * freq^2 triangles, shared vertices."""
     t = (1 + 5 ** 0.5) / 2

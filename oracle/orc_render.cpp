@@ -233,8 +233,8 @@ static V3 material_emission(const TextureSet *T, const float *m, UV uv, UV uv1) 
 }
 
 // getMaterial (materialBuffer.wgsl.ts:170-245), with the texture samples of
-// :78-160 when the scene has textures (materials are opaque: the alpha
-// samples change nothing).
+// :78-160 when the scene has textures and the alpha-mode resolution of
+// :66-77 (opaque -> 1, mask -> alpha >= cutoff ? 1 : 0).
 static Mat get_material(const float *m, V3 instanceColor, const TextureSet *T = nullptr, UV uv = UV{0.0f, 0.0f},
                         UV uv1 = UV{0.0f, 0.0f}) {
   Mat ma{};
@@ -246,7 +246,7 @@ static Mat get_material(const float *m, V3 instanceColor, const TextureSet *T = 
   ma.inside = false;
   ma.specularIor = m[M_SPEC_IOR];
   ma.transmissionWeight = m[M_TRANS_WEIGHT];
-  ma.alpha = 1.0f;
+  ma.alpha = m[M_OPACITY];
   ma.baseWeight = clampf(m[M_BASE_WEIGHT], 0.0f, 1.0f);
   ma.diffuseRoughness = clampf(m[M_DIFFUSE_ROUGHNESS], 0.0f, 1.0f);
   ma.specularWeight = mxf(m[M_SPEC_WEIGHT], 0.0f);
@@ -270,7 +270,9 @@ static Mat get_material(const float *m, V3 instanceColor, const TextureSet *T = 
     if (material_map(*T, m, MAP_BASE, uv, uv1, c, &index)) {
       ma.baseColor = ma.baseColor * decode_map_color(*T, index, c);
       if (m[M_MAP_ALPHA_USAGE] == 3.0f) ma.transmissionWeight = 1.0f - c[3];
+      else if (m[M_MAP_ALPHA_USAGE] == 2.0f) ma.alpha = ma.alpha * c[3];
     }
+    if (material_map(*T, m, MAP_ALPHA, uv, uv1, c)) ma.alpha = ma.alpha * c[1];
     if (material_map(*T, m, MAP_TRANSMISSION, uv, uv1, c)) ma.transmissionWeight = ma.transmissionWeight * c[0];
     if (material_map(*T, m, MAP_METALLIC_ROUGHNESS, uv, uv1, c)) {
       ma.baseMetalness = ma.baseMetalness * c[2];
@@ -286,6 +288,8 @@ static Mat get_material(const float *m, V3 instanceColor, const TextureSet *T = 
     if (material_map(*T, m, MAP_SPECULAR_COLOR, uv, uv1, c, &index))
       ma.specularColor = ma.specularColor * decode_map_color(*T, index, c);
   }
+  if (m[M_ALPHA_MODE] == 0.0f) ma.alpha = 1.0f;
+  else if (m[M_ALPHA_MODE] == 1.0f) ma.alpha = ma.alpha >= m[M_ALPHA_CUTOFF] ? 1.0f : 0.0f;
   if (ma.coatWeight > 0.0f && ma.coatRoughness > 0.0f) {
     float b2 = ma.specularRoughness * ma.specularRoughness;
     float c2 = ma.coatRoughness * ma.coatRoughness;
@@ -1561,6 +1565,43 @@ static void intersect_scene(Ctx &c, const PathState &path, Surface &si, int boun
   si.position = mat_mul_point(l2w, position, 1.0f);
 }
 
+// intersectionAlphaPasses (intersectionAlpha.wgsl.ts:28-88) for the shadow
+// query: alpha = opacity, times the base map's a when map alpha is coverage
+// (:43-61) and the alpha map's g (:63-78), each sampled with sampleMap at the
+// UV0 barycentric blend times the material's UV scale; then the alpha mode
+// (:80-87) and `alpha >= 1`.
+static bool alpha_covers(const void *ctx, int32_t instance, int32_t record, V3 b) {
+  const Ctx &c = *(const Ctx *)ctx;
+  const TraceScene &t = *c.s.trace;
+  const uint32_t *ins = &t.instances[(size_t)instance * 40];
+  const float *m = c.s.materials + (size_t)(((int32_t)ins[2]) >> 8) * M_FLOATS;
+  float alpha = m[M_OPACITY];
+  if (const TextureSet *T = c.s.textures) {
+    const int32_t vo = (int32_t)ins[1];
+    const size_t r3 = (size_t)record * 4;
+    const int32_t i0 = (int32_t)t.blas[r3] + vo, i1 = (int32_t)t.blas[r3 + 1] + vo, i2 = (int32_t)t.blas[r3 + 2] + vo;
+    auto uv_at = [&]() {
+      auto at = [&](int32_t i) { return UV{T->uvs[(size_t)i * 2], T->uvs[(size_t)i * 2 + 1]}; };
+      const UV t0 = at(i0), t1 = at(i1), t2 = at(i2);
+      UV uv{b.x * t0.x + b.y * t1.x + b.z * t2.x, b.x * t0.y + b.y * t1.y + b.z * t2.y};
+      return UV{uv.x * m[M_UV_SCALE], uv.y * m[M_UV_SCALE + 1]};
+    };
+    float s[4];
+    const uint32_t base = (uint32_t)m[M_MAPS + MAP_BASE], amap = (uint32_t)m[M_MAPS + MAP_ALPHA];
+    if (m[M_MAP_ALPHA_USAGE] == 2.0f && base) {
+      sample_map(*T, base - 1u, uv_at(), s);
+      alpha = alpha * s[3];
+    }
+    if (amap) {
+      sample_map(*T, amap - 1u, uv_at(), s);
+      alpha = alpha * s[1];
+    }
+  }
+  if (m[M_ALPHA_MODE] == 0.0f) alpha = 1.0f;
+  else if (m[M_ALPHA_MODE] == 1.0f) alpha = alpha >= m[M_ALPHA_CUTOFF] ? 1.0f : 0.0f;
+  return alpha >= 1.0f;
+}
+
 static float nee_mis(uint32_t count, float lightPdf, float scatPdf) {
   if (count <= 1) return power_heuristic(lightPdf, scatPdf);
   float w = (float)count * lightPdf;
@@ -1652,7 +1693,8 @@ static Scatter sample_scattering(Ctx &c, Sampler &smp, const Surface &si, const 
       }
       if (valid) {
         c.rays++;
-        if (!trace_any(*c.s.trace, sel.ro, sel.rd, sel.maxDistance, sel.triangleIndex, si.instanceIndex, nullptr))
+        const Coverage cov{alpha_covers, &c};
+        if (!trace_any(*c.s.trace, sel.ro, sel.rd, sel.maxDistance, sel.triangleIndex, si.instanceIndex, nullptr, &cov))
           path.li = path.li + sel.contribution * ucw;
       }
       float d2[2];
@@ -1864,6 +1906,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
   int eventLimit = std::min(bounceLimit * 3, 50);
   bool isNotTransparent = false;
   int depth = 0;
+  bool sampleIgnored = false;
   for (int event = 0; event <= eventLimit; event++) {
     intersect_scene(c, path, si, depth, event != 0);
     if (si.hit) {
@@ -1879,9 +1922,9 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
         si.albedoSampled = ma.transmissionWeight > 0.5f;
       }
       isNotTransparent = ma.transmissionWeight < 0.001f && ma.subsurfaceWeight < 0.001f && ma.alpha > 0.99f;
-    } else {
-      // completeVertex (rayTrace.compute.ts:514-589); materials are opaque,
-      // so the alpha-coverage draw is always below alpha.
+    } else if (!sampleIgnored) {
+      // completeVertex (rayTrace.compute.ts:514-589); it returns at once
+      // after a passed-through vertex (:527-529).
       bool isAlphaOccluded = smp.hashed_sample(3u) < ma.alpha;
       for (uint32_t i = 0; i < s.lightCount; i++) {
         LightSample ls{};
@@ -1897,7 +1940,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
           path.abort = true;
         }
       }
-      if (s.env && !si.hit) {
+      if (s.env && !(si.hit && isAlphaOccluded)) {
         float u, v;
         environment_uv(path.rd, u, v);
         V3 em = texture_linear(s, u, v);
@@ -1917,6 +1960,7 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
     if (isNotTransparent && event > bounceLimit) path.abort = true;
     if (path.abort || event >= eventLimit) break;
     // shadeVertex (rayTrace.compute.ts:343-512)
+    sampleIgnored = false;
     smp.samplingBounce = (uint32_t)std::max(depth, 0);
     if (!si.hit) {
       if (path.fromEye) {
@@ -1934,6 +1978,15 @@ static void integrate_pixel(Ctx &c, uint32_t px, uint32_t py, uint32_t sampleInd
       }
       path.abort = true;
       break;
+    }
+    // Alpha coverage (:432-458): a surface of coverage <= 0 (or one the
+    // coverage draw misses) is passed through: no material sample, guides or
+    // roulette, and the ray moves on from position + d * EPS.
+    const float coverage = clampf(ma.alpha, 0.0f, 1.0f);
+    if (coverage < 1.0f && (coverage <= 0.0f || smp.hashed_sample(3u) > coverage)) {
+      sampleIgnored = true;
+      path.ro = si.position + path.rd * EPS;
+      continue;
     }
     sample_material(c, smp, si, ma, path);
     if (!si.albedoSampled) {

@@ -65,7 +65,7 @@ static inline V3 pos(const TraceScene &s, int32_t i) {
 // anyHit: stop at the first accepted triangle (shadow query).
 static ClosestHit traverse(const TraceScene &s, V3 ro, V3 rd, float tMax, bool doubleSided,
                            int bounces, int32_t curTri, int32_t curInst, bool anyHit,
-                           TraceStats *st) {
+                           TraceStats *st, const Coverage *cov = nullptr) {
   ClosestHit res{};
   res.hit = false; res.instance = -1; res.record = -1; res.t = RAY_MAX_DISTANCE;
   if (s.tlas.empty()) return res;
@@ -155,6 +155,9 @@ static ClosestHit traverse(const TraceScene &s, V3 ro, V3 rd, float tMax, bool d
           if (h.t > 0.0f) {
             V3 n = normalize(cross(p1 - p0, p2 - p0));
             if (!doubleSided && !matDouble && dot(n, d) > 0.0f) continue;
+            // intersect.wgsl.ts:643-646: a candidate the coverage test
+            // rejects leaves tMax at the shadow ray's maximum.
+            if (anyHit && cov && !cov->covers(cov->ctx, insIdx, tri, h.bary)) continue;
             tMax = h.t;
             res.hit = true; res.instance = insIdx; res.record = tri; res.t = tMax;
             res.bary = h.bary; res.normal = n; res.position = o + tMax * d;
@@ -173,9 +176,9 @@ ClosestHit trace_closest(const TraceScene &s, V3 ro, V3 rd, float tMax, bool dou
 }
 
 bool trace_any(const TraceScene &s, V3 ro, V3 rd, float maxDistance, int32_t curTri,
-               int32_t curInst, TraceStats *st) {
+               int32_t curInst, TraceStats *st, const Coverage *cov) {
   float tMax = fminf(fmaxf(maxDistance, 0.0f), RAY_MAX_DISTANCE);
-  return traverse(s, ro, rd, tMax, true, 1, curTri, curInst, true, st).hit;
+  return traverse(s, ro, rd, tMax, true, 1, curTri, curInst, true, st, cov).hit;
 }
 
 ClosestHit trace_brute(const TraceScene &s, V3 ro, V3 rd, float tMax, bool doubleSided) {

@@ -44,9 +44,18 @@ struct ClosestHit {
 ClosestHit trace_closest(const TraceScene &s, V3 ro, V3 rd, float tMax, bool doubleSided,
                          int bounces, int32_t currentTriangle, int32_t currentInstance,
                          TraceStats *stats);
-// intersectSceneShadow (intersect.wgsl.ts:829-879) without alpha: first opaque hit.
+// Coverage test of an any-hit candidate (intersectionAlphaPasses,
+// intersectionAlpha.wgsl.ts:28-88): false lets the shadow ray pass the
+// triangle `record` of `instance` at barycentrics `bary`.
+struct Coverage {
+  bool (*covers)(const void *ctx, int32_t instance, int32_t record, V3 bary);
+  const void *ctx;
+};
+// intersectSceneShadow (intersect.wgsl.ts:829-879): first hit that the
+// coverage test accepts (every hit when cov is null).
 bool trace_any(const TraceScene &s, V3 ro, V3 rd, float maxDistance,
-               int32_t currentTriangle, int32_t currentInstance, TraceStats *stats);
+               int32_t currentTriangle, int32_t currentInstance, TraceStats *stats,
+               const Coverage *cov = nullptr);
 // All-triangles closest hit (same triangle test and tie rule, no BVH).
 ClosestHit trace_brute(const TraceScene &s, V3 ro, V3 rd, float tMax, bool doubleSided);
 

@@ -205,9 +205,10 @@ def test_version5_light_records(cornell):
 def test_material_textures_and_alpha_resolution():
     """Material textures travel through the descriptor (basic-record slots,
     extension slots, UV sets, texture records); alpha resolves as
-    resolveMaterialAlphaMode (MaterialState.ts:251-264) and coverage is
-    refused as UNAVAILABLE; texture features outside the row (UDIM sets, coat
-    normal maps) are UNAVAILABLE too."""
+    resolveMaterialAlphaMode (MaterialState.ts:251-264): mask materials
+    (with or without coverage maps) are accepted, blend coverage is refused
+    as UNAVAILABLE; texture features outside the row (UDIM sets, coat normal
+    maps) are UNAVAILABLE too."""
     room = scene.textured_room()
     s = context.validate(descriptor.encode(room, 24, 16))
     assert s["materials"] == len(room.materials) and s["instances"] == len(room.mesh_index)
@@ -232,7 +233,18 @@ def test_material_textures_and_alpha_resolution():
 
     def mask(sc):
         sc.materials[0].alpha_cutoff = 0.5
-    assert status_of(mask) == abi.MTX_UNAVAILABLE
+    assert status_of(mask) == abi.MTX_OK
+
+    def mask_with_maps(sc):
+        sc.materials[0].alpha_cutoff = 0.5
+        sc.materials[0].alpha_map = 1
+        sc.materials[0].map_alpha_usage = "alpha"
+        sc.materials[0].opacity = 0.75
+    assert status_of(mask_with_maps) == abi.MTX_OK
+
+    def blend(sc):
+        sc.materials[0].alpha_mode = "blend"
+    assert status_of(blend) == abi.MTX_UNAVAILABLE
 
     def alpha_map(sc):
         sc.materials[0].alpha_map = 1

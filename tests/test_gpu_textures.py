@@ -11,8 +11,8 @@ is textured (emissive map plus the coat-weight-map tint) so that the NEE
 samples and the BSDF hits both read it. Both sides run the same restatement
 of the reference's sampleMap / getMaterial / getMatNormal, so the FP64
 accumulators must agree bit for bit, on both kernel forms and every sampling
-pattern. A scene whose material needs alpha coverage is refused with
-MTX_UNAVAILABLE.
+pattern. A scene whose material needs blend coverage (stochastic
+transparency) is refused with MTX_UNAVAILABLE.
 """
 import numpy as np
 import pytest
@@ -65,11 +65,20 @@ def test_textures_change_the_image(env, orc):
     assert np.abs(a[..., 8:11] - b[..., 8:11]).max() > 0.01  # the normal map moves the normal AOV
 
 
-@pytest.mark.parametrize("mode", ["mask", "blend"])
-def test_alpha_coverage_is_refused(env, mode):
+@pytest.mark.parametrize("edit", ["blend", "opacity", "alpha map"])
+def test_blend_coverage_is_refused(env, edit):
+    """Blend coverage (explicit, or resolved from opacity < 1 or an alpha
+    map without a cutoff) is refused; mask materials are rendered (their
+    parity is covered with the alpha-coverage tests)."""
     abi, render, scene = env
     sc = scene.cornell_box()
-    sc.materials[1].alpha_mode = mode
+    if edit == "blend":
+        sc.materials[1].alpha_mode = "blend"
+    elif edit == "opacity":
+        sc.materials[1].opacity = 0.5
+    else:
+        sc.textures = [scene.Texture(np.ones((2, 2, 4), np.float32))]
+        sc.materials[1].alpha_map = 0
     with pytest.raises(abi.MtxError) as e:
         render.RenderScene(sc)
     assert e.value.code == abi.MTX_UNAVAILABLE
