@@ -97,7 +97,9 @@ int32_t mtx_blas_destroy(mtx_handle blas);
 typedef struct mtx_instance_desc {
   uint32_t struct_size;
   uint32_t blas_index;      /* index into the blas array of mtx_accel_create */
-  uint32_t material_word;   /* packed material word (material << 8) */
+  uint32_t material_word;   /* packed material word (material << 8); bit 1 set: the material is
+                               alpha-masked (render scenes set it from MTX_MAT_ALPHA_MODE; the
+                               raw query of mtx_trace_any does not test coverage) */
   uint32_t double_sided;    /* material's doubleSided flag (no back-face culling) */
   double transform[16];     /* local-to-world, column-major (f64, as the reference) */
   float color[3];           /* instance color (instancesBuffer row 9) */
@@ -209,9 +211,12 @@ int32_t mtx_trace_count(mtx_handle accel, const mtx_ray *rays, uint32_t count, u
 #define MTX_MAT_THIN_WALLED 37        /* > 0.5: thin-walled surface */
 #define MTX_MAT_OPACITY 38            /* alpha (materialParams.ts:214) */
 #define MTX_MAT_ALPHA_MODE 39         /* resolved mode: 0 opaque, 1 mask, 2 blend (materialConstants.ts:12-16,
-                                         resolveMaterialAlphaMode, MaterialState.ts:251-264); a scene whose
-                                         material is not opaque is rejected with MTX_UNAVAILABLE */
-#define MTX_MAT_ALPHA_CUTOFF 40
+                                         resolveMaterialAlphaMode, MaterialState.ts:251-264). Mask coverage =
+                                         opacity x base-map a (map alpha usage 2) x alpha-map g, covered where
+                                         >= the cutoff: camera and bounce rays pass uncovered points without a
+                                         depth, shadow rays ignore them. A scene with a blend material is
+                                         rejected with MTX_UNAVAILABLE */
+#define MTX_MAT_ALPHA_CUTOFF 40       /* max(0, alphaCutoff) (materialParams.ts:218) */
 #define MTX_MAT_MAP_ALPHA_USAGE 41    /* base map alpha: 0 none, 2 alpha, 3 transmission = 1 - a
                                          (mapAlphaUsageConstants, materialConstants.ts:2) */
 #define MTX_MAT_NORMAL_SCALE 42       /* 2 floats: normal map xy scale */
