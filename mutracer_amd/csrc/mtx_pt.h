@@ -62,6 +62,7 @@ struct GpuScene {
 // k / bins, so a sample in bin k searches only between bins k and k + 1.
 constexpr uint32_t kSceneFull = 1u;
 constexpr uint32_t kSceneTextured = 2u;  // set with kSceneFull: the full kernels sample the maps
+constexpr uint32_t kSceneMasked = 4u;    // a material is alpha-masked: shadow rays run the coverage test
 constexpr uint32_t kEnvGuideRows = 1024;
 constexpr uint32_t kEnvGuideCols = 128;
 
@@ -163,6 +164,8 @@ struct WfLaunch {
   uint32_t totalPixels;
   uint64_t *rays;  // traced-ray counter
   uint64_t *work;  // optional traversal work totals (8 counters)
+  uint32_t masked; // a job's scene has masked materials: the host launches the coverage-testing
+                   // any-hit kernels (the kernels do not read this field)
 };
 
 // Launch shape of the wavefront kernels (the autotuner's search space).
@@ -226,7 +229,8 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p, const
 // the kernels read the slots from device memory.
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, bool full, LaunchTimer *timer = nullptr, WfGraph *graph = nullptr);
+                             hipStream_t s, bool full, bool masked, LaunchTimer *timer = nullptr,
+                             WfGraph *graph = nullptr);
 // Whether one job needs the full shading kernels.
 inline bool needs_full_shading(const GpuScene &scene, const RenderParams &p) {
   return (scene.features & kSceneFull) != 0u || p.pattern == 2u;

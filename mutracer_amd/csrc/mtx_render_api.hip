@@ -514,6 +514,7 @@ static std::shared_ptr<RenderSceneObj> create_scene(const mtx_scene_desc &D) {
     const float *mat = D.materials + (size_t)m * MTX_MATERIAL_FLOATS;
     if (mat[MTX_MAT_FUZZ_WEIGHT] > 0.0f || mat[MTX_MAT_SUBSURFACE_WEIGHT] > 0.0f || mat[MTX_MAT_THIN_WALLED] > 0.5f)
       v.features |= kSceneFull;
+    if (mat[MTX_MAT_ALPHA_MODE] == 1.0f) v.features |= kSceneMasked;
   }
   if (textured) {
     v.features |= kSceneFull | kSceneTextured;
@@ -1251,7 +1252,11 @@ static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
   uint32_t finished = 0;
   if (!idle) {
     bool full = false;
-    for (const JobSlot &js : slots) full = full || needs_full_shading(js.scene, js.params);
+    bool masked = false;
+    for (const JobSlot &js : slots) {
+      full = full || needs_full_shading(js.scene, js.params);
+      masked = masked || (js.scene.features & kSceneMasked) != 0u;
+    }
     // A fixed-size slot table keeps its address, so the captured round
     // replays whenever the next round has the same shape.
     if (S.slots.count < (size_t)kMaxJobsPerRound) S.slots.alloc(kMaxJobsPerRound);
@@ -1264,7 +1269,7 @@ static uint32_t server_round(ServerObj &S, hipStream_t s, bool &idle) {
     }
     S.slots.upload(slots.data(), slots.size(), rs);
     launch_generation_multi(S.slots.ptr, (uint32_t)slots.size(), paths, pixels, eventLimit, S.rays.ptr, S.wf, S.cfg,
-                            rs, full, nullptr, &S.graph);
+                            rs, full, masked, nullptr, &S.graph);
     MTX_HIP(hipStreamSynchronize(rs));
     const double t = now_ms();
     for (auto &pj : parts) {

@@ -1508,6 +1508,13 @@ struct SceneCoverage {
   const GpuScene *S;
   __device__ __forceinline__ bool covers(int32_t slot, int32_t tri, f3 b) const { return alpha_covers(*S, slot, tri, b); }
 };
+// The coverage test of kernels compiled for scenes with (kAlpha) or without
+// masked materials; without, no instance carries the masked flag.
+template <bool kAlpha>
+__device__ __forceinline__ auto coverage(const GpuScene *S) {
+  if constexpr (kAlpha) return SceneCoverage{S};
+  else return NoCoverage();
+}
 
 __device__ __forceinline__ float nee_mis(uint32_t count, float lp, float sp) {
   if (count <= 1u) return power_heuristic(lp, sp);

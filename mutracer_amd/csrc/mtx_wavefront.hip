@@ -294,7 +294,7 @@ __device__ __forceinline__ void add_payload(const WfBuffers &W, uint32_t p, floa
   W.li[p] = l;
 }
 
-template <bool kMulti, bool kCount, int kLds>
+template <bool kMulti, bool kCount, int kLds, bool kAlpha>
 __global__ void __launch_bounds__(kWfTraceBlock, kShadowWaves)
 k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLds * kWfTraceBlock];
@@ -315,10 +315,10 @@ k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spil
       const GpuScene &S = job_of<true>(L, W, p).scene;
       const AccelView av = *S.accel;
       occluded = traverse<true, kCount>(av, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc,
-                                        SceneCoverage{&S});
+                                        coverage<kAlpha>(&S));
     } else {
       occluded = traverse<true, kCount>(one, f3of(a), f3of(b), a.w, true, 1, fint(c.w), W.s3[i], st, h, &tc,
-                                        SceneCoverage{&L.one.scene});
+                                        coverage<kAlpha>(&L.one.scene));
     }
     if (!occluded) add_payload(W, p, c);
   };
@@ -339,7 +339,7 @@ k_wf_shadow(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spil
 // decision; the flag it raises makes the remaining wavefront launches of the
 // generation return at once. Per-path arithmetic is that of the wavefront
 // kernels, so results do not depend on where the tail starts.
-template <bool kMulti, bool kCount, bool kFull>
+template <bool kMulti, bool kCount, bool kFull, bool kAlpha>
 __global__ void __launch_bounds__(kWfShadeBlock)
 k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kMinLdsStack * kWfShadeBlock];
@@ -383,7 +383,7 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
         rays++;
         if (kCount) nShadow++;
         if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts,
-                                    SceneCoverage{&S}))
+                                    coverage<kAlpha>(&S)))
           pr.li = pr.li + sr.payload;
       }
       if (!alive) break;
@@ -404,7 +404,7 @@ k_wf_tail(const WfLaunch L, WfBuffers W, int event, uint32_t tailPaths, int32_t 
 // Persistent closest-hit / any-hit kernels over Walker: each lane takes a
 // new queue entry (wave-aggregated atomic on counts[kWfFetchBase + 2e (+1)])
 // as soon as its previous ray has finished.
-template <bool kMulti, bool kShadow, bool kCount, int kLds>
+template <bool kMulti, bool kShadow, bool kCount, int kLds, bool kAlpha>
 __global__ void __launch_bounds__(kWfTraceBlock)
 k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLds * kWfTraceBlock];
@@ -451,7 +451,7 @@ k_wf_walk(const WfLaunch L, WfBuffers W, int event, int32_t *__restrict__ spill)
       }
     }
     if (!__any(have)) break;
-    if (have && (w.done || w.step(av, st, SceneCoverage{scene}))) {
+    if (have && (w.done || w.step(av, st, coverage<kAlpha>(scene)))) {
       have = false;
       if (kShadow) {
         if (!w.hit) add_payload(W, path, W.s2[entry]);
@@ -639,10 +639,20 @@ WfBuffers WfBuffers::Owner::view(uint32_t nPaths, bool sort) const {
 template <bool kMulti, bool kShadow, int kLds>
 static void launch_walk_lds(uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W, int e,
                             int32_t *spill) {
-  if (L.work)
-    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
-  else
-    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+  if (kShadow && L.masked) {
+    if (L.work)
+      hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, true, kLds, true>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                         spill);
+    else
+      hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, false, kLds, true>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W,
+                         e, spill);
+  } else if (L.work) {
+    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, true, kLds, false>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                       spill);
+  } else {
+    hipLaunchKernelGGL((k_wf_walk<kMulti, kShadow, false, kLds, false>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W,
+                       e, spill);
+  }
 }
 template <bool kMulti, bool kShadow>
 static void launch_walk(const WfConfig &cfg, uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
@@ -656,10 +666,20 @@ template <bool kMulti, bool kShadow, int kLds>
 static void launch_trace_lds(uint32_t grid, hipStream_t s, const WfLaunch &L, const WfBuffers &W, int e,
                              int32_t *spill) {
   if (kShadow) {
-    if (L.work)
-      hipLaunchKernelGGL((k_wf_shadow<kMulti, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
-    else
-      hipLaunchKernelGGL((k_wf_shadow<kMulti, false, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
+    if (L.masked) {
+      if (L.work)
+        hipLaunchKernelGGL((k_wf_shadow<kMulti, true, kLds, true>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                           spill);
+      else
+        hipLaunchKernelGGL((k_wf_shadow<kMulti, false, kLds, true>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                           spill);
+    } else if (L.work) {
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, true, kLds, false>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                         spill);
+    } else {
+      hipLaunchKernelGGL((k_wf_shadow<kMulti, false, kLds, false>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e,
+                         spill);
+    }
   } else {
     if (L.work)
       hipLaunchKernelGGL((k_wf_trace<kMulti, true, kLds>), dim3(grid), dim3(kWfTraceBlock), 0, s, L, W, e, spill);
@@ -692,8 +712,9 @@ static Grids grids_for(const WfConfig &cfg) {
     closestBpc[lc] = blocks_per_cu(t, kWfTraceBlock, 0);
   }
   if (!shadowBpc[li]) {
-    const void *a = li == 0 ? (const void *)k_wf_shadow<false, false, 16>
-                  : li == 1 ? (const void *)k_wf_shadow<false, false, 24> : (const void *)k_wf_shadow<false, false, 32>;
+    const void *a = li == 0   ? (const void *)k_wf_shadow<false, false, 16, false>
+                    : li == 1 ? (const void *)k_wf_shadow<false, false, 24, false>
+                              : (const void *)k_wf_shadow<false, false, 32, false>;
     shadowBpc[li] = blocks_per_cu(a, kWfTraceBlock, 0);
   }
 
@@ -704,12 +725,12 @@ static Grids grids_for(const WfConfig &cfg) {
     shadeBpc[sw] = blocks_per_cu(k, kWfShadeBlock, 0);
   }
   if (!walkBpc[li]) {
-    const void *k = li == 0 ? (const void *)k_wf_walk<false, false, false, 16>
-                  : li == 1 ? (const void *)k_wf_walk<false, false, false, 24>
-                            : (const void *)k_wf_walk<false, false, false, 32>;
+    const void *k = li == 0 ? (const void *)k_wf_walk<false, false, false, 16, false>
+                  : li == 1 ? (const void *)k_wf_walk<false, false, false, 24, false>
+                            : (const void *)k_wf_walk<false, false, false, 32, false>;
     walkBpc[li] = blocks_per_cu(k, kWfTraceBlock, 0);
   }
-  if (!tailBpc) tailBpc = blocks_per_cu((const void *)k_wf_tail<false, false, true>, kWfShadeBlock, 0);
+  if (!tailBpc) tailBpc = blocks_per_cu((const void *)k_wf_tail<false, false, true, false>, kWfShadeBlock, 0);
   Grids g;
   g.cus = (uint32_t)cus;
   g.tail = (uint32_t)(cus * tailBpc);
@@ -733,12 +754,20 @@ static void launch_shade(const WfConfig &cfg, const Grids &g, hipStream_t s, con
 template <bool kMulti, bool kFull>
 static void launch_tail(const WfConfig &cfg, const Grids &g, hipStream_t s, const WfLaunch &L, const WfBuffers &W,
                         int e, int32_t *spill) {
-  if (L.work)
-    hipLaunchKernelGGL((k_wf_tail<kMulti, true, kFull>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+  if (L.masked) {
+    if (L.work)
+      hipLaunchKernelGGL((k_wf_tail<kMulti, true, kFull, true>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+                         cfg.tailPaths, spill);
+    else
+      hipLaunchKernelGGL((k_wf_tail<kMulti, false, kFull, true>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+                         cfg.tailPaths, spill);
+  } else if (L.work) {
+    hipLaunchKernelGGL((k_wf_tail<kMulti, true, kFull, false>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
                        cfg.tailPaths, spill);
-  else
-    hipLaunchKernelGGL((k_wf_tail<kMulti, false, kFull>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
+  } else {
+    hipLaunchKernelGGL((k_wf_tail<kMulti, false, kFull, false>), dim3(g.tail), dim3(kWfShadeBlock), 0, s, L, W, e,
                        cfg.tailPaths, spill);
+  }
 }
 
 // Launches one generation: generate, the event loop, accumulate. `full`:
@@ -846,6 +875,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
     L.totalPixels = p0.pixelCount;
     L.rays = acc.rays;
     L.work = acc.work;
+    L.masked = (scene.features & kSceneMasked) != 0u ? 1u : 0u;
     if (graph && cfg.graphs && !timer) {
       // The sequence has no host-side dependence on device values, so one
       // capture replays for as long as its arguments are unchanged.
@@ -892,7 +922,7 @@ void launch_render_wavefront(const GpuScene &scene, const RenderParams &p0, cons
 
 void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t nPaths, uint32_t totalPixels,
                              int eventLimit, uint64_t *rays, WfBuffers::Owner &buf, const WfConfig &cfg,
-                             hipStream_t s, bool full, LaunchTimer *timer, WfGraph *graph) {
+                             hipStream_t s, bool full, bool masked, LaunchTimer *timer, WfGraph *graph) {
   if (!nJobs || !nPaths) return;
   MTX_REQUIRE(nJobs <= kMaxJobsPerRound, MTX_INVALID_ARGUMENT, "too many jobs in one round");
   const Grids g = grids_for(cfg);
@@ -906,6 +936,7 @@ void launch_generation_multi(const JobSlot *devSlots, uint32_t nJobs, uint32_t n
   L.totalPixels = totalPixels;
   L.rays = rays;
   L.work = nullptr;
+  L.masked = masked ? 1u : 0u;
   const int events = std::min(eventLimit, 50);
   if (graph && cfg.graphs && !timer) {
     // The caller's slot upload precedes the launch on the same stream (the
