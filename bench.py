@@ -345,6 +345,36 @@ def _pu_forward(y):
                                                  E * np.log(y + F) + G))
 
 
+def alpha_benchmark(opaque_sums, steps=3):
+    """Cost of mask coverage at the headline workload: every material of the
+    instanced field as a mask material whose coverage (opacity 1, cutoff
+    0.5) covers everywhere. The image must equal the opaque render bit for
+    bit; the difference in time is what the coverage-testing shadow, walker
+    and tail kernels and the per-hit coverage resolution cost."""
+    import torch
+    from mutracer_amd import render, scene
+    sc = scene.instanced_field()
+    for m in sc.materials:
+        m.alpha_mode, m.opacity, m.alpha_cutoff = "mask", 1.0, 0.5
+    tgt = render.RenderTarget(render.RenderScene(sc), WIDTH, HEIGHT, bounces=DEPTH,
+                              direct_light_candidates=CANDIDATES, pattern="sobol", sequence_seed=0,
+                              max_samples=SPP)
+    stream = torch.cuda.current_stream()
+    tgt.restart_samples()
+    tgt.render(SPP, stream)
+    torch.cuda.synchronize()
+    sums, counts = tgt.read_accumulators()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tgt.restart_samples()
+        tgt.render(SPP, stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"workload": "instanced field, every material mask (covered everywhere)", "ms_per_step": round(ms, 3),
+            "msamples_per_s": round(WIDTH * HEIGHT * SPP / (ms / 1e3) / 1e6, 3),
+            "equal_to_opaque": bool(np.array_equal(sums, opaque_sums))}
+
+
 def cpu_baseline(sc, gpu_sums, seconds):
     """Oracle on 32x32 tiles spread over the frame until `seconds` elapse."""
     import oracle
@@ -497,6 +527,9 @@ def main():
         micro = microbench(subset=args.microbench_subset)
 
     serving = None
+    log("mask coverage at the headline workload")
+    alpha = alpha_benchmark(sums)
+    log(f"mask coverage: {alpha}")
     if not args.no_serving:
         log("serving stream: batched vs one job at a time")
         serving = serving_benchmark(n_jobs=args.serving_jobs, per_sample=args.serving_per_sample)
@@ -532,6 +565,7 @@ def main():
         "cpu_baseline": cpu,
         "microbench": micro,
         "serving": serving,
+        "mask_coverage": alpha,
         "denoiser": denoiser,
         "parity_relmse_cpu_tiles": parity,
     }
