@@ -19,7 +19,7 @@ namespace mtx {
 // kCount adds per-launch totals of the traversal work to acc.work (first
 // group of mtx_target_render_counted: closest rays, closest box/triangle/
 // instance tests, hits, shadow rays, shadow box/triangle/instance tests).
-template <bool kCount>
+template <bool kCount, bool kAlpha>
 __global__ void __launch_bounds__(kBlock)
 k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__restrict__ spill) {
   __shared__ int32_t lds[kLdsStack * kBlock];
@@ -68,7 +68,7 @@ k_render(const GpuScene S, const RenderParams P, Accumulators acc, int32_t *__re
           rays++;
           if (kCount) nShadow++;
           if (!traverse<true, kCount>(av, sr.ro, sr.rd, sr.tMax, true, 1, pr.curTri, pr.curSlot, st, sh, &ts,
-                                      SceneCoverage{&S}))
+                                      coverage<kAlpha>(&S)))
             pr.li = pr.li + sr.payload;
         }
         if (!alive) break;
@@ -129,10 +129,16 @@ void launch_render(const GpuScene &scene, const RenderParams &p, const Accumulat
   grid = std::min(grid, need);
   int32_t *spill = spill_for(s, grid, kBlock);
   if (timer) timer->begin(4);
-  if (acc.work)
-    hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  // Shadow rays test coverage only in scenes with mask materials.
+  const bool masked = (scene.features & kSceneMasked) != 0u;
+  if (acc.work && masked)
+    hipLaunchKernelGGL((k_render<true, true>), dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  else if (acc.work)
+    hipLaunchKernelGGL((k_render<true, false>), dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+  else if (masked)
+    hipLaunchKernelGGL((k_render<false, true>), dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
   else
-    hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
+    hipLaunchKernelGGL((k_render<false, false>), dim3(grid), dim3(kBlock), 0, s, scene, p, acc, spill);
   if (timer) timer->end();
   MTX_HIP(hipGetLastError());
 }

@@ -185,12 +185,17 @@ def test_c_host_job_flow(mods, orc, tmp_path):
         assert np.array_equal(got, _expected(ref, spp, base)), kind
 
 
-def test_textured_descriptor_matches_target(mods):
+@pytest.mark.parametrize("which", ["textured_room", "mask_checker_caster", "mask_checker_panes"])
+def test_textured_descriptor_matches_target(mods, which):
     """The textured room through the descriptor (texture records, UV sets,
     texture slots in the basic record and in extensions) renders exactly as
-    a render target of the same Scene."""
+    a render target of the same Scene; so do the mask-material scenes (the
+    descriptor's alpha mode, cutoff, opacity and alpha map resolve to the
+    same packed coverage)."""
     abi, context, descriptor, render, scene = mods
-    sc = scene.textured_room()
+    sc = {"textured_room": scene.textured_room,
+          "mask_checker_caster": lambda: scene.alpha_caster("checker"),
+          "mask_checker_panes": lambda: scene.masked_panes(2, checker=True)}[which]()
     W, H, spp = 40, 30, 4
     ctx = context.RenderContext()
     ctx.commit(descriptor.encode(sc, W, H), _settings(context, spp, bounces=8, candidates=2))
